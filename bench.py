@@ -1,0 +1,137 @@
+"""Benchmark: RK4 cell-updates/s of the ShallowWaters time step at 4096x4096 on one MI355X.
+
+Workload: the nonperiodic double gyre at 4096x4096 (Lx = Ly = 3840 km, seamount, double-gyre
+wind Fx0 = 0.12 Pa, free slip, Smagorinsky biharmonic diffusion, quadratic drag, tracer on,
+RK4, Float32, dt = 13 s), started from a seeded smoothed η (σ = 0.1 m) with geostrophic
+u, v.  A step is one full pass of the time loop (4 RK stages, ghost points, dissipation,
+tracer when due) over the whole grid; the state stays resident in HBM.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the middle RK stage), timed
+with HIP events on the library's stream; `cpu_baseline` times the CPU restatement (oracle,
+one thread) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NX = 4096
+
+
+def workload_parameter(swm, T="Float32", nx=NX):
+    return swm.Parameter(T=T, nx=nx, Lx=3840e3, L_ratio=1, bc="nonperiodic", α=0.0, topography="seamount",
+                         wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky", bottom_drag="quadratic",
+                         tracer_advection=True, Ndays=10)
+
+
+def bytes_per_cell_stage(es: int) -> dict:
+    """Algorithmic HBM bytes per cell: each kernel reads every field it needs once and
+    writes each output once (u, v, η per field; H static)."""
+    return {
+        "stage1": 10 * es,        # read u,v,η,H; write u0(3), u1(3)
+        "stage_mid": 16 * es,     # read u1(3),H,u(3),u0(3); write u0(3),u1(3)
+        "stage4": 10 * es,        # read u1(3),H,u0(3); write u0(3)
+        "dissipation": 6 * es,    # read u0,v0,η0,H; write u,v
+    }
+
+
+def cpu_baseline(swm, orc, sample_steps: int, nx: int):
+    P = workload_parameter(swm, nx=nx)
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    cfg = swm.make_config(S)
+    orc.run(S, cfg, prog, 1)                     # warm-up
+    t = time.perf_counter()
+    orc.run(S, cfg, prog, sample_steps)
+    dt = time.perf_counter() - t
+    return {"value": S.grid.nx * S.grid.ny * sample_steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_steps} steps of the same workload at {S.grid.nx}x{S.grid.ny}, oracle/swm_oracle.cpp "
+                      f"(g++ -O2, one thread) on {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--dtype", default="Float32", choices=["Float32", "Float64"])
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 skips)")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the time step is "
+              f"benchmarked on a single MI355X; multi-GPU sharding is a design note, not a run).", file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_oracle, load_package
+    swm = load_package()
+    P = workload_parameter(swm, T=args.dtype, nx=args.nx)
+    S = swm.make_setup(P)
+    G = S.grid
+    prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    es = 4 if args.dtype == "Float32" else 8
+    cells = G.nx * G.ny
+
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        done, nan = sim.step(args.warmup, 1, 0)
+        assert nan == 0
+        i0 = 1 + args.warmup
+        t0 = time.perf_counter()
+        done, nan = sim.step(args.steps, i0, args.warmup * G.dtint)   # returns after a stream sync
+        elapsed = time.perf_counter() - t0
+        assert (done, nan) == (args.steps, 0), (done, nan)
+        # dominant kernel: middle RK stage, HIP events on the library stream
+        kms = {name: sim.time_kernel(k, args.kernel_reps)
+               for name, k in (("stage_mid", 0), ("stage1", 1), ("stage4", 2), ("dissipation", 3))}
+        mass, energy = sim.reduce()
+
+    value = cells * args.steps / elapsed
+    bpc = bytes_per_cell_stage(es)
+    alg_bytes = bpc["stage_mid"] * cells
+    achieved = alg_bytes / (kms["stage_mid"] * 1e-3) / 1e9
+    step_bytes = (bpc["stage1"] + 2 * bpc["stage_mid"] + bpc["stage4"] + bpc["dissipation"]) * cells
+    out = {
+        "metric": "RK4 cell-updates/s at 4096x4096 (nonperiodic double gyre)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if es == 4 else "f64",
+        "data": "synthetic: seeded smoothed η (σ=0.1 m, seed 2024) with geostrophic u,v; analytic sst",
+        "config": {"workload": f"double_gyre_{G.nx}x{G.ny}", "nx": G.nx, "ny": G.ny, "Lx_m": G.Lx, "bc": "nonperiodic",
+                   "topography": "seamount", "wind": "double_gyre Fx0=0.12", "diffusion": "Smagorinsky",
+                   "bottom_drag": "quadratic", "alpha": 0.0, "time_scheme": "RK4", "dt_s": G.dtint,
+                   "tracer_every": G.nadvstep, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "rk_stage_kernel (middle stage)", "bytes_per_cell": bpc["stage_mid"],
+                     "kernel_ms": kms["stage_mid"]},
+        "step_roofline": {"alg_bytes_per_cell_step": step_bytes / cells,
+                          "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
+                          "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms": kms,
+        "diagnostics": {"mass": mass, "energy": energy},
+    }
+    if args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * swm.h -- C ABI of the MI355X (gfx950) time-stepping library for ShallowWaters.jl.
+ *
+ * The library replaces the hot loop of ShallowWaters.jl,
+ *     time_integration(Prog, Diag, S)            src/time_integration.jl:2-250
+ * with hand-written HIP kernels.  Everything before it (Parameter, Grid, Constants,
+ * Forcing, initial_conditions: src/run_model.jl:25-39) stays on the host and hands the
+ * library resolved enums, pre-rounded constants and the static fields.
+ *
+ * Conventions
+ *  - Plain C types only.  Every entry point returns an int status: SWM_OK (0) or a
+ *    negative SWM_E_* code; swm_last_error(ctx) gives the message of the last failure.
+ *  - Host arrays are Julia Array{T,2} layout: column-major, no pitch, halo included,
+ *    sizes exactly as src/preallocate.jl allocates them (listed per entry point).  Host
+ *    buffers are borrowed for the duration of the call only.
+ *  - Floating-point host arrays are stored in the format named by the config: SWM_F64
+ *    (double), SWM_F32 (float), SWM_F16 (IEEE binary16) or SWM_BF16 (bfloat16 bits).
+ *  - A context owns its device buffers and one HIP stream; it is not re-entrant.
+ *    Contexts are independent of each other.
+ *  - No exceptions or callbacks cross the ABI.
+ */
+#ifndef SWM_H
+#define SWM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWM_ABI_VERSION 1
+
+/* status codes */
+#define SWM_OK 0
+#define SWM_E_ARG (-1)       /* invalid argument / shape (Julia: ArgumentError, BoundsError) */
+#define SWM_E_UNSUPPORTED (-2) /* option not implemented by the GPU path */
+#define SWM_E_HIP (-3)       /* HIP runtime failure */
+#define SWM_E_STATE (-4)     /* call out of order (e.g. step before set_static) */
+
+/* number formats: T (right-hand side), Tprog (prognostic), Tcomm (ghost copies)
+ * src/default_parameters.jl:3-6 */
+enum { SWM_F64 = 0, SWM_F32 = 1, SWM_F16 = 2, SWM_BF16 = 3 };
+/* P.bc, src/default_parameters.jl:60 */
+enum { SWM_PERIODIC = 0, SWM_NONPERIODIC = 1 };
+/* P.time_scheme, src/default_parameters.jl:47-48 */
+enum { SWM_RK = 0, SWM_SSPRK2 = 1, SWM_SSPRK3 = 2, SWM_4SSPRK3 = 3 };
+/* P.bottom_drag, src/default_parameters.jl:71 */
+enum { SWM_DRAG_QUADRATIC = 0, SWM_DRAG_LINEAR = 1, SWM_DRAG_NONE = 2 };
+/* P.diffusion, src/default_parameters.jl:76 */
+enum { SWM_DIFF_CONSTANT = 0, SWM_DIFF_SMAGORINSKY = 1 };
+/* P.adv_scheme / P.dynamics, src/default_parameters.jl:66-67 */
+enum { SWM_ADV_ARAKAWAHSU = 0, SWM_ADV_SADOURNY = 1 };
+enum { SWM_NONLINEAR = 0, SWM_LINEAR = 1 };
+/* continuity variant, src/continuity.jl:91-97 */
+enum { SWM_CONT_ITSELF = 0, SWM_CONT_RELAX = 1, SWM_CONT_FORCING = 2 };
+
+/* Resolved run configuration.  Values marked (Tprog) / (T) must already be rounded to
+ * that format (the host computes them in Float64 exactly as src/grid.jl and
+ * src/constants.jl do, then rounds); (F64) values stay Float64. */
+typedef struct swm_config {
+    int32_t abi_version;          /* SWM_ABI_VERSION */
+    int32_t nx, ny;               /* grid.jl:4,22 */
+    int32_t bc;                   /* SWM_PERIODIC (ep=1) or SWM_NONPERIODIC (ep=0) */
+    int32_t T, Tprog, Tcomm;      /* SWM_F64 ... */
+    int32_t time_scheme, RKo, RKs, RKn;
+    int32_t nstep_diff, nstep_advcor;
+    int32_t adv_scheme, dynamics, bottom_drag, diffusion, continuity;
+    int32_t tracer_advection, tracer_consumption;
+    int32_t seasonal_wind_x, seasonal_wind_y;
+    int32_t pad0;
+    int64_t dtint, nadvstep, nadvstep_half, nout;   /* grid.jl:70,77-78,86 */
+    /* (Tprog) constants.jl:69-75,81,86 and grid.jl:74 */
+    double RKaDt[4], RKbDt[3];
+    double Dt_Ds, Dt_D, Dt_D_half;
+    int32_t ssp_n, ssp_s, ssp_kn, ssp_mn;            /* constants.jl:20-27 */
+    double ssp_Dt_Dn, ssp_kna, ssp_knb, ssp_Dt_Dnc;
+    double one_minus_alpha, Dt_diff;
+    /* (T) constants.jl:87-105, grid.jl:80-83 */
+    double g, cD, rD, gamma, cSmag, nuB;
+    double dtadvu, dtadvv, half_dtadvu, half_dtadvv;
+    double jSST, SSTmin;
+    /* (F64) constants.jl:108-110 */
+    double omegaFx, omegaFy, omegaFeta;
+} swm_config;
+
+typedef struct swm_ctx swm_ctx;
+
+/* Library identification and device check.  *ndev receives hipGetDeviceCount. */
+int swm_version(int32_t *abi_version);
+int swm_device_count(int32_t *ndev);
+
+/* Create a context on the current HIP device; validates the config.
+ * Replaces the allocation half of preallocate (src/preallocate.jl:454-471). */
+int swm_create(const swm_config *cfg, swm_ctx **out);
+int swm_destroy(swm_ctx *ctx);
+const char *swm_last_error(const swm_ctx *ctx);
+
+/* Static fields of Forcing{T} and Grid (format T, Julia layout):
+ *   H     (nx+2, ny+2)        forcing.jl:39-47
+ *   Fx    (nux, nuy)          forcing.jl:15-25
+ *   Fy    (nvx, nvy)          forcing.jl:27-37
+ *   f_q   (nx+1, ny+1)        grid.jl:95
+ *   eta_ref, Feta (nx, ny)    forcing.jl:49-50; may be NULL unless continuity needs them */
+int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy,
+                   const void *f_q, const void *eta_ref, const void *Feta);
+
+/* Prognostic variables in format Tprog, halo included (PrognosticVars,
+ * src/initial_conditions.jl:6-11):
+ *   u (nux+4, nuy+4), v (nvx+4, nvy+4), eta (nx+2, ny+2), sst (nx+2, ny).
+ * Upload applies ghost_points! and ghost_points_sst! as the first step of the loop would
+ * (time_integration.jl:48; ghost_points.jl:182-242).  Halo cells that the reference never
+ * writes must be zero (as add_halo leaves them); upload rejects other values. */
+int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta, const void *sst);
+int swm_download_prog(swm_ctx *ctx, void *u, void *v, void *eta, void *sst);
+
+/* Run time steps i = i0 .. i0+nsteps-1 of the loop of time_integration.jl:45-243 with
+ * model time t0 at step i0 (the reference starts at i0=1, t0=0; t advances by dtint per
+ * step).  NaN detection runs every nout steps as feedback.jl:136-148 does; on detection
+ * the loop stops before the copy-back (time_integration.jl:235-237) and *nan_step
+ * receives the step, else 0.  *steps_done receives the number of steps completed. */
+int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0,
+             int64_t *steps_done, int64_t *nan_step);
+
+/* Per-operator entry points (parity tests).
+ * swm_stages: at step i0 with time t0, run the first n_stages stages of the time scheme
+ * (time_integration.jl:53-196) from the current Prog, without ghost points, dissipation,
+ * tracer or copy-back.  swm_download_rk then returns u0,v0,eta0 and u1,v1,eta1
+ * (RungeKuttaVars, preallocate.jl:21-26, format Tprog; NULL skips an array). */
+int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0);
+int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void *v1, void *eta1);
+
+/* FP64 reductions over the current Prog (no counterpart in the reference, which only sums
+ * isnan, feedback.jl:68): mass = sum of eta over the nx*ny T-cells; energy = sum over
+ * T-cells (x,y) of 0.5*g*eta^2 + 0.5*h*u^2 + 0.5*h*v^2 with h = eta + H of that cell, u
+ * at its right face (x = 1..nx) and v at its top face (y = 1..ny-1). */
+int swm_reduce(swm_ctx *ctx, double *mass, double *energy);
+
+/* Measurement hook (no counterpart in the reference): launch kernel `which` `reps` times
+ * back to back on the context's stream between two HIP events and return the average
+ * duration in milliseconds.  which: 0 middle RK stage (stage 2), 1 first stage, 2 last
+ * stage, 3 dissipation, 4 tracer advection.  Operates on the current buffers; the
+ * prognostic state is left unchanged except the RK scratch slots. */
+int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWM_H */

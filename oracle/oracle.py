@@ -1,0 +1,104 @@
+"""ctypes front end of the CPU oracle (``oracle/swm_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the cpu_baseline
+leg of bench.py, as the checker.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswm_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (strict IEEE flags)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library {LIB_PATH} is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        L.swmo_run.argtypes = [vp] * 11 + [i64, i64, i64, i32, ctypes.POINTER(vp),
+                                           ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.swmo_run.restype = ctypes.c_int
+        L.swmo_stencil.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
+        L.swmo_round.argtypes = [ctypes.c_int, vp, vp, i64]
+        L.swmo_arith.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, i64]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _fortran(a):
+    assert a.flags.f_contiguous, "arrays must be in Julia (Fortran) layout"
+    return a
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def run(S, cfg, prog, nsteps, i0=1, t0=0, stop_stage=0, want_rk=False):
+    """Run ``nsteps`` steps of the reference loop on copies of ``prog`` (u, v, η, sst in
+    Tprog host storage).  Returns ``(u, v, η, sst, rk, steps_done, nan_step)`` where ``rk``
+    is ``None`` or the tuple u0 v0 η0 u1 v1 η1 du dv dη of the last step."""
+    F, G = S.forcing, S.grid
+    u, v, e, s = (np.array(x, order="F", copy=True) for x in (prog.u, prog.v, prog.η, prog.sst))
+    rk = None
+    rk_ptrs = None
+    if want_rk:
+        rk = tuple(np.zeros_like(x, order="F") for x in (u, v, e, u, v, e, u, v, e))
+        rk_ptrs = (ctypes.c_void_p * 9)(*[x.ctypes.data for x in rk])
+    sd, ns = ctypes.c_int64(0), ctypes.c_int64(0)
+    statics = [_fortran(F.H), _fortran(F.Fx), _fortran(F.Fy), _fortran(G.f_q),
+               _fortran(F.η_ref), _fortran(F.Fη)]
+    rc = lib().swmo_run(ctypes.byref(cfg), *[_ptr(a) for a in statics], _ptr(u), _ptr(v), _ptr(e), _ptr(s),
+                        nsteps, i0, t0, stop_stage, rk_ptrs, ctypes.byref(sd), ctypes.byref(ns))
+    if rc != 0:
+        raise OracleError("oracle run failed (shape/option error or an index the reference throws on)")
+    return u, v, e, s, rk, sd.value, ns.value
+
+
+_STENCIL = {"dx": (0, -1, 0), "dy": (1, 0, -1), "lap": (2, -2, -2), "Ix": (3, -1, 0), "Iy": (4, 0, -1),
+            "Ixy": (5, -1, -1)}
+
+
+def stencil(name, a):
+    """Apply one reference stencil (∂x, ∂y, ∇², Ix, Iy, Ixy) to a float64 array."""
+    op, dm, dn = _STENCIL[name]
+    a = np.asfortranarray(a, dtype=np.float64)
+    out = np.zeros((a.shape[0] + dm, a.shape[1] + dn), order="F")
+    if lib().swmo_stencil(op, _ptr(a), a.shape[0], a.shape[1], _ptr(out)) != 0:
+        raise OracleError("shape error")
+    return out
+
+
+def round_fmt(fmt_code, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().swmo_round(fmt_code, _ptr(x), _ptr(out), x.size)
+    return out
+
+
+def arith(fmt_code, op, a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty_like(a)
+    lib().swmo_arith(fmt_code, op, _ptr(a), _ptr(b), _ptr(out), a.size)
+    return out

@@ -1,0 +1,1099 @@
+// swm_oracle.cpp -- CPU restatement of ShallowWaters.jl's time-stepping path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle of the GPU library: only
+// tests/, the smoke check and the cpu_baseline leg of bench.py load it, and only to check
+// or time against it.  Nothing in the product path calls it.
+//
+// It restates the reference operator by operator: one loop nest per reference function,
+// the same array shapes (src/preallocate.jl), the same 1-based column-major indexing and the
+// same floating-point evaluation order (Julia evaluates a+b+c left to right, x^2 as x*x,
+// and contracts nothing into FMAs; build with -ffp-contract=off, no fast-math).  Each
+// function cites the reference lines it follows.
+//
+// Number formats: double, float, and binary16 / bfloat16 emulated on float with one rounding
+// after each operation (float carries >= 2p+2 significand bits for both, so + - * / sqrt
+// are correctly rounded; Julia's Float16 arithmetic is likewise Float32-then-round).
+//
+// Parity pins (tests/test_oracle_*.py): the exact-zero known answer of test/runtests.jl:4-9,
+// closed forms of every stencil, exact mass conservation of continuity, x-shift
+// equivariance, and hand-derived single-cell values.  Julia is not installed here, so the
+// reference itself cannot be run; see the tests for what each pin covers.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../include/swm.h"
+
+// ------------------------------------------------------------------ number formats
+
+static inline float f32_round_odd(double x) {
+    float f = (float)x;
+    if (!std::isfinite(f) || (double)f == x) return f;
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if (std::fabs((double)f) > std::fabs(x)) b -= 1u;
+    b |= 1u;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+static inline float round_binary16(float x) {
+    if (!std::isfinite(x)) return x;
+    float ax = std::fabs(x);
+    if (ax >= 65520.0f) return std::copysign(INFINITY, x);
+    float q;
+    if (ax < 6.103515625e-05f) {
+        q = 5.9604644775390625e-08f;        // 2^-24, subnormal quantum
+    } else {
+        int e;
+        std::frexp(ax, &e);
+        q = std::ldexp(1.0f, e - 11);       // 10 fraction bits
+    }
+    float r = std::nearbyint(ax / q) * q;   // ties to even (default rounding mode)
+    return std::copysign(r, x);
+}
+
+static inline float round_bfloat16(float x) {
+    if (std::isnan(x)) return x;
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    b &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+
+template <int KIND>  // 0: binary16, 1: bfloat16
+struct Low {
+    float v;
+    static float rnd(float x) { return KIND == 0 ? round_binary16(x) : round_bfloat16(x); }
+    Low() : v(0.0f) {}
+    explicit Low(double x) : v(rnd(f32_round_odd(x))) {}
+    explicit Low(float x) : v(rnd(x)) {}
+    explicit Low(int x) : v(rnd((float)x)) {}
+    template <int K2>
+    explicit Low(Low<K2> o) : v(rnd(o.v)) {}
+    static Low raw(float x) { Low r; r.v = x; return r; }
+    explicit operator double() const { return (double)v; }
+    explicit operator float() const { return v; }
+    friend Low operator+(Low a, Low b) { return raw(rnd(a.v + b.v)); }
+    friend Low operator-(Low a, Low b) { return raw(rnd(a.v - b.v)); }
+    friend Low operator*(Low a, Low b) { return raw(rnd(a.v * b.v)); }
+    friend Low operator/(Low a, Low b) { return raw(rnd(a.v / b.v)); }
+    friend Low operator-(Low a) { return raw(-a.v); }
+    Low &operator+=(Low b) { v = rnd(v + b.v); return *this; }
+    friend bool operator<(Low a, Low b) { return a.v < b.v; }
+    friend bool operator>(Low a, Low b) { return a.v > b.v; }
+    friend bool operator==(Low a, Low b) { return a.v == b.v; }
+};
+typedef Low<0> f16;
+typedef Low<1> bf16;
+
+template <class T> static inline T num_sqrt(T x) { return std::sqrt(x); }
+template <> inline f16 num_sqrt<f16>(f16 x) { return f16::raw(f16::rnd(std::sqrt(x.v))); }
+template <> inline bf16 num_sqrt<bf16>(bf16 x) { return bf16::raw(bf16::rnd(std::sqrt(x.v))); }
+template <class T> static inline double to_f64(T x) { return (double)x; }
+template <class T> static inline bool is_nan(T x) { return std::isnan((double)x); }
+
+// Julia's T(x): conversion between formats with a single rounding.
+template <class To, class From> static inline To cvt(From x) {
+    if constexpr (std::is_same_v<To, From>) return x;
+    else if constexpr (std::is_same_v<To, double>) return (double)x;
+    else if constexpr (std::is_same_v<To, float>) {
+        if constexpr (std::is_same_v<From, double>) return (float)x;
+        else return (float)x;               // from f16/bf16: exact widening
+    } else {
+        if constexpr (std::is_same_v<From, double>) return To(x);
+        else if constexpr (std::is_same_v<From, float>) return To(x);
+        else return To((float)x);            // f16 <-> bf16
+    }
+}
+
+// host storage of a format (binary16 / bfloat16 are 16-bit patterns)
+static inline float half_bits_to_float(uint16_t h) {
+    uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    float r;
+    if (e == 0) {
+        r = std::ldexp((float)m, -24);
+        return s ? -r : r;
+    }
+    uint32_t b;
+    if (e == 31) b = s | 0x7f800000u | (m << 13);
+    else b = s | ((e + 112) << 23) | (m << 13);
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+static inline uint16_t float_to_half_bits(float x) {  // x already binary16-representable
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    uint16_t s = (uint16_t)((b >> 16) & 0x8000u);
+    float ax = std::fabs(x);
+    if (std::isnan(x)) return s | 0x7e00u;
+    if (std::isinf(x)) return s | 0x7c00u;
+    if (ax < 6.103515625e-05f) return s | (uint16_t)std::lrint(ax * 16777216.0f);
+    uint32_t e = ((b >> 23) & 0xff) - 112, m = (b >> 13) & 0x3ff;
+    return s | (uint16_t)(e << 10) | (uint16_t)m;
+}
+template <class T> static inline T load_elem(const void *p, size_t k) {
+    if constexpr (std::is_same_v<T, double>) return ((const double *)p)[k];
+    else if constexpr (std::is_same_v<T, float>) return ((const float *)p)[k];
+    else if constexpr (std::is_same_v<T, f16>) return T::raw(half_bits_to_float(((const uint16_t *)p)[k]));
+    else {
+        uint32_t b = (uint32_t)((const uint16_t *)p)[k] << 16;
+        float f;
+        std::memcpy(&f, &b, 4);
+        return T::raw(f);
+    }
+}
+template <class T> static inline void store_elem(void *p, size_t k, T x) {
+    if constexpr (std::is_same_v<T, double>) ((double *)p)[k] = x;
+    else if constexpr (std::is_same_v<T, float>) ((float *)p)[k] = x;
+    else if constexpr (std::is_same_v<T, f16>) ((uint16_t *)p)[k] = float_to_half_bits(x.v);
+    else {
+        uint32_t b;
+        std::memcpy(&b, &x.v, 4);
+        ((uint16_t *)p)[k] = (uint16_t)(b >> 16);
+    }
+}
+
+// ------------------------------------------------------------------ Julia-layout arrays
+
+template <class T> struct Arr {
+    int m = 0, n = 0;
+    std::vector<T> a;
+    Arr() {}
+    Arr(int m_, int n_) : m(m_), n(n_), a((size_t)m_ * n_, T(0.0)) {}
+    T &operator()(int i, int j) { return a[(size_t)(i - 1) + (size_t)(j - 1) * m]; }
+    const T &operator()(int i, int j) const { return a[(size_t)(i - 1) + (size_t)(j - 1) * m]; }
+    size_t size() const { return a.size(); }
+    T &end(int di, int j) { return (*this)(m - di, j); }
+};
+
+struct ShapeError {};
+#define CHECK(cond) \
+    do { if (!(cond)) throw ShapeError(); } while (0)
+
+template <class T> static void load_arr(Arr<T> &A, const void *p) {
+    for (size_t k = 0; k < A.size(); ++k) A.a[k] = load_elem<T>(p, k);
+}
+template <class T> static void store_arr(const Arr<T> &A, void *p) {
+    for (size_t k = 0; k < A.size(); ++k) store_elem<T>(p, k, A.a[k]);
+}
+template <class D, class S> static void copyto(Arr<D> &d, const Arr<S> &s) {  // copyto!/convert
+    CHECK(d.m == s.m && d.n == s.n);
+    for (size_t k = 0; k < d.size(); ++k) d.a[k] = cvt<D>(s.a[k]);
+}
+
+// ------------------------------------------------------------------ gradients.jl
+
+// ∂x!  gradients.jl:3-12
+template <class T> static void dx(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m + 1 == u.m && o.n == u.n);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) o(i, j) = u(i + 1, j) - u(i, j);
+}
+// ∂y!  gradients.jl:16-25
+template <class T> static void dy(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m == u.m && o.n + 1 == u.n);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) o(i, j) = u(i, j + 1) - u(i, j);
+}
+// ∇²!  gradients.jl:29-42 (low-precision resilient form)
+template <class T> static void lap(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m + 2 == u.m && o.n + 2 == u.n);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) {
+            T c = u(i + 1, j + 1);
+            o(i, j) = ((u(i, j + 1) - c) + (u(i + 2, j + 1) - c)) + ((u(i + 1, j) - c) + (u(i + 1, j + 2) - c));
+        }
+}
+
+// ------------------------------------------------------------------ interpolations.jl
+
+// Ix!  interpolations.jl:3-14
+template <class T> static void Ix(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m + 1 == u.m && o.n == u.n);
+    const T half = T(0.5);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) o(i, j) = half * (u(i + 1, j) + u(i, j));
+}
+// Iy!  interpolations.jl:18-29
+template <class T> static void Iy(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m == u.m && o.n + 1 == u.n);
+    const T half = T(0.5);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) o(i, j) = half * (u(i, j + 1) + u(i, j));
+}
+// Ixy!  interpolations.jl:33-45
+template <class T> static void Ixy(Arr<T> &o, const Arr<T> &u) {
+    CHECK(o.m + 1 == u.m && o.n + 1 == u.n);
+    const T q = T(0.25);
+    for (int j = 1; j <= o.n; ++j)
+        for (int i = 1; i <= o.m; ++i) o(i, j) = q * (u(i, j) + u(i + 1, j)) + q * (u(i, j + 1) + u(i + 1, j + 1));
+}
+
+// ------------------------------------------------------------------ the model
+
+template <class T, class Tp, class Tc> struct Model {
+    const swm_config &c;
+    int nx, ny, nux, nuy, nvx, nvy, ep;
+    bool periodic;
+    // static fields (Forcing{T}, Grid.f_q)
+    Arr<T> H, Fx, Fy, f_q, eta_ref, Feta;
+    // PrognosticVars{Tprog}
+    Arr<Tp> u, v, eta, sst;
+    // RungeKuttaVars{Tprog}, TendencyVars{Tprog}   preallocate.jl:2-73
+    Arr<Tp> u0, u1, v0, v1, eta0, eta1, du, dv, deta;
+    // PrognosticVarsRHS{T}
+    Arr<T> ur, vr, etar, sstr;
+    // VolumeFluxVars{T}   preallocate.jl:77-116
+    Arr<T> h, h_u, U, h_v, V, dUdx, dVdy;
+    // VorticityVars{T}    preallocate.jl:119-169
+    Arr<T> h_q, q, qhu, qhv, dudx, dudy, dvdx, dvdy;
+    // BernoulliVars{T}    preallocate.jl:172-211
+    Arr<T> u2, v2, KEu, KEv, p, dpdx, dpdy;
+    // BottomdragVars{T}   preallocate.jl:214-250
+    Arr<T> sqrtKE, sqrtKE_u, sqrtKE_v, Bu, Bv;
+    // ArakawaHsuVars{T}   preallocate.jl:253-288
+    Arr<T> qa, qb, qg, qd;
+    // LaplaceVars{T}      preallocate.jl:291-329
+    Arr<T> Lu, Lv, dLudx, dLudy, dLvdx, dLvdy;
+    // SmagorinskyVars{T}  preallocate.jl:332-385
+    Arr<T> DT, DS, nuSmag, DS_q, nuSmag_q, S12, S21, S11, S22, LLu1, LLu2, LLv1, LLv2;
+    // SemiLagrangeVars{T} preallocate.jl:388-436
+    Arr<T> xd, yd, um, vm, u_T, um_T, v_T, vm_T, uinterp, vinterp, ssti;
+
+    explicit Model(const swm_config &cfg) : c(cfg) {
+        nx = c.nx;
+        ny = c.ny;
+        periodic = c.bc == SWM_PERIODIC;
+        ep = periodic ? 1 : 0;
+        nux = periodic ? nx : nx - 1;
+        nuy = ny;
+        nvx = nx;
+        nvy = ny - 1;
+        H = Arr<T>(nx + 2, ny + 2);
+        Fx = Arr<T>(nux, nuy);
+        Fy = Arr<T>(nvx, nvy);
+        f_q = Arr<T>(nx + 1, ny + 1);
+        eta_ref = Arr<T>(nx, ny);
+        Feta = Arr<T>(nx, ny);
+        u = u0 = u1 = du = Arr<Tp>(nux + 4, nuy + 4);
+        v = v0 = v1 = dv = Arr<Tp>(nvx + 4, nvy + 4);
+        eta = eta0 = eta1 = deta = Arr<Tp>(nx + 2, ny + 2);
+        sst = Arr<Tp>(nx + 2, ny);
+        ur = u2 = um = Arr<T>(nux + 4, nuy + 4);
+        vr = v2 = vm = Arr<T>(nvx + 4, nvy + 4);
+        etar = h = p = sqrtKE = DT = DS = nuSmag = Arr<T>(nx + 2, ny + 2);
+        sstr = ssti = Arr<T>(nx + 2, ny);
+        h_u = U = dpdx = sqrtKE_u = Bu = Arr<T>(nx + 1, ny + 2);
+        h_v = V = dpdy = sqrtKE_v = Bv = Arr<T>(nx + 2, ny + 1);
+        dUdx = Arr<T>(nx, ny + 2);
+        dVdy = Arr<T>(nx + 2, ny);
+        h_q = q = nuSmag_q = S12 = S21 = Arr<T>(nx + 1, ny + 1);
+        qhu = Arr<T>(nvx, nvy);
+        qhv = Arr<T>(nux, nuy);
+        dudx = KEu = u_T = um_T = Arr<T>(nux + 3, nuy + 4);
+        dudy = Arr<T>(nux + 4, nuy + 3);
+        dvdx = DS_q = Arr<T>(nvx + 3, nvy + 4);
+        dvdy = KEv = v_T = vm_T = Arr<T>(nvx + 4, nvy + 3);
+        qa = qd = xd = yd = uinterp = vinterp = Arr<T>(nx, ny);
+        qb = qg = Arr<T>(nx + 1, ny);
+        Lu = Arr<T>(nux + 2, nuy + 2);
+        Lv = Arr<T>(nvx + 2, nvy + 2);
+        dLudx = S11 = Arr<T>(nux + 1, nuy + 2);
+        dLudy = Arr<T>(nux + 2, nuy + 1);
+        dLvdx = Arr<T>(nvx + 1, nvy + 2);
+        dLvdy = S22 = Arr<T>(nvx + 2, nvy + 1);
+        LLu1 = Arr<T>(nux, nuy + 2);
+        LLu2 = Arr<T>(nx + 1, ny);
+        LLv1 = Arr<T>(nx, ny + 1);
+        LLv2 = Arr<T>(nvx + 2, nvy);
+    }
+
+    // Base.convert of time_integration.jl:330-347: alias if the formats agree, else copy.
+    const Arr<T> &rhs_view(const Arr<Tp> &x, Arr<T> &buf) {
+        if constexpr (std::is_same_v<T, Tp>) return x;
+        else {
+            copyto(buf, x);
+            return buf;
+        }
+    }
+
+    // ------------------------------------------------------------ ghost_points.jl
+    static Tp comm(Tp x) { return cvt<Tp>(cvt<Tc>(x)); }  // T(Tcomm(x)), ghost_points.jl:65
+
+    void ghost_u_nonperiodic(Arr<Tp> &a) {                // ghost_points.jl:47-57
+        const Tp oma = Tp(c.one_minus_alpha);
+        for (int i = 1; i <= a.m; ++i) {
+            a(i, 1) = oma * a(i, 4);
+            a(i, 2) = oma * a(i, 3);
+            a(i, a.n - 1) = oma * a(i, a.n - 2);
+            a(i, a.n) = oma * a(i, a.n - 3);
+        }
+    }
+    void ghost_u_periodic(Arr<Tp> &a) {                   // ghost_points.jl:60-73
+        for (int j = 1; j <= a.n; ++j) {
+            a(1, j) = comm(a(a.m - 3, j));
+            a(2, j) = comm(a(a.m - 2, j));
+            a(a.m - 1, j) = comm(a(3, j));
+            a(a.m, j) = comm(a(4, j));
+        }
+    }
+    void ghost_v_nonperiodic(Arr<Tp> &a) {                // ghost_points.jl:76-86
+        const Tp oma = Tp(c.one_minus_alpha);
+        for (int j = 1; j <= a.n; ++j) {
+            a(1, j) = oma * a(4, j);
+            a(2, j) = oma * a(3, j);
+            a(a.m - 1, j) = oma * a(a.m - 2, j);
+            a(a.m, j) = oma * a(a.m - 3, j);
+        }
+    }
+    void ghost_v_periodic(Arr<Tp> &a) {                   // ghost_points.jl:89-99
+        for (int j = 1; j <= a.n; ++j) {
+            a(1, j) = comm(a(a.m - 3, j));
+            a(2, j) = comm(a(a.m - 2, j));
+            a(a.m - 1, j) = comm(a(3, j));
+            a(a.m, j) = comm(a(4, j));
+        }
+    }
+    void ghost_eta_nonperiodic(Arr<Tp> &a) {              // ghost_points.jl:102-116
+        for (int j = 1; j <= a.n; ++j) {
+            a(1, j) = a(2, j);
+            a(a.m, j) = a(a.m - 1, j);
+        }
+        for (int i = 1; i <= a.m; ++i) {
+            a(i, 1) = a(i, 2);
+            a(i, a.n) = a(i, a.n - 1);
+        }
+    }
+    void ghost_eta_periodic(Arr<Tp> &a) {                 // ghost_points.jl:119-133
+        for (int j = 1; j <= a.n; ++j) {
+            a(1, j) = comm(a(a.m - 1, j));
+            a(a.m, j) = comm(a(2, j));
+        }
+        for (int i = 1; i <= a.m; ++i) {
+            a(i, 1) = a(i, 2);
+            a(i, a.n) = a(i, a.n - 1);
+        }
+    }
+    template <class S> void ghost_sst_any(Arr<S> &a) {    // ghost_points.jl:136-179, halosstx=1, halossty=0
+        const int hx = 1;
+        for (int j = 1; j <= a.n; ++j)
+            for (int i = 1; i <= hx; ++i) {
+                if (periodic) {
+                    a(i, j) = cvt<S>(cvt<Tc>(a(a.m - 2 * hx + i, j)));
+                    a(a.m - hx + i, j) = cvt<S>(cvt<Tc>(a(hx + i, j)));
+                } else {
+                    a(i, j) = a(hx + 1, j);
+                    a(a.m - i + 1, j) = a(a.m - hx, j);
+                }
+            }
+    }
+    void ghost_points(Arr<Tp> &uu, Arr<Tp> &vv, Arr<Tp> &ee) {  // ghost_points.jl:182-199
+        if (periodic) { ghost_u_periodic(uu); ghost_v_periodic(vv); ghost_eta_periodic(ee); }
+        else { ghost_u_nonperiodic(uu); ghost_v_nonperiodic(vv); ghost_eta_nonperiodic(ee); }
+    }
+    void ghost_points_uv(Arr<Tp> &uu, Arr<Tp> &vv) {           // ghost_points.jl:202-216
+        if (periodic) { ghost_u_periodic(uu); ghost_v_periodic(vv); }
+        else { ghost_u_nonperiodic(uu); ghost_v_nonperiodic(vv); }
+    }
+    void ghost_points_eta(Arr<Tp> &ee) {                       // ghost_points.jl:219-229
+        if (periodic) ghost_eta_periodic(ee);
+        else ghost_eta_nonperiodic(ee);
+    }
+
+    // ------------------------------------------------------------ rhs.jl
+    void thickness(const Arr<T> &e) {                          // rhs.jl:131-139
+        CHECK(h.m == e.m && h.n == e.n && H.m == e.m && H.n == e.n);
+        for (size_t k = 0; k < h.size(); ++k) h.a[k] = e.a[k] + H.a[k];
+    }
+    void speed(const Arr<T> &uu, const Arr<T> &vv) {           // rhs.jl:142-159
+        for (size_t k = 0; k < uu.size(); ++k) u2.a[k] = uu.a[k] * uu.a[k];
+        for (size_t k = 0; k < vv.size(); ++k) v2.a[k] = vv.a[k] * vv.a[k];
+    }
+    void Uflux(const Arr<T> &uu) {                             // rhs.jl:277-291
+        CHECK(U.m + 2 + ep == uu.m && U.n + 2 == uu.n);
+        for (int j = 1; j <= U.n; ++j)
+            for (int i = 1; i <= U.m; ++i) U(i, j) = uu(1 + ep + i, 1 + j) * h_u(i, j);
+    }
+    void Vflux(const Arr<T> &vv) {                             // rhs.jl:294-304
+        CHECK(V.m + 2 == vv.m && V.n + 2 == vv.n);
+        for (int j = 1; j <= V.n; ++j)
+            for (int i = 1; i <= V.m; ++i) V(i, j) = vv(i + 1, j + 1) * h_v(i, j);
+    }
+    void UVfluxes(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e) {  // rhs.jl:307-324
+        thickness(e);
+        Ix(h_u, h);
+        Iy(h_v, h);
+        Uflux(uu);
+        Vflux(vv);
+    }
+    void PV() {                                                // PV_advection.jl:2-18
+        CHECK(q.m + 2 == dvdx.m && q.n + 2 == dvdx.n && q.m + 2 + ep == dudy.m);
+        for (int j = 1; j <= q.n; ++j)
+            for (int i = 1; i <= q.m; ++i)
+                q(i, j) = (f_q(i, j) + dvdx(i + 1, j + 1) - dudy(i + 1 + ep, j + 1)) / h_q(i, j);
+    }
+    void AHa() {                                               // PV_advection.jl:88-99
+        const T k = T(1.0 / 12.0);
+        for (int j = 1; j <= qa.n; ++j)
+            for (int i = 1; i <= qa.m; ++i) qa(i, j) = k * (q(i, j) + q(i, j + 1) + q(i + 1, j + 1));
+    }
+    void AHb() {                                               // PV_advection.jl:103-115
+        const T k = T(1.0 / 12.0);
+        for (int j = 1; j <= qb.n; ++j) {
+            qb(1, j) = k * (q(1, j) + q(1, j + 1) + q(q.m, j + 1));
+            for (int i = 2; i <= qb.m; ++i) qb(i, j) = k * (q(i, j) + q(i, j + 1) + q(i - 1, j + 1));
+        }
+    }
+    void AHg() {                                               // PV_advection.jl:119-131
+        const T k = T(1.0 / 12.0);
+        for (int j = 1; j <= qg.n; ++j) {
+            qg(1, j) = k * (q(1, j) + q(1, j + 1) + q(q.m, j));
+            for (int i = 2; i <= qg.m; ++i) qg(i, j) = k * (q(i, j) + q(i - 1, j) + q(i, j + 1));
+        }
+    }
+    void AHd() {                                               // PV_advection.jl:135-146
+        const T k = T(1.0 / 12.0);
+        for (int j = 1; j <= qd.n; ++j)
+            for (int i = 1; i <= qd.m; ++i) qd(i, j) = k * (q(i, j) + q(i, j + 1) + q(i + 1, j));
+    }
+    void advection_coriolis(const Arr<T> &uu, const Arr<T> &vv) {  // rhs.jl:89-128
+        Ixy(h_q, h);
+        dx(dvdx, vv);
+        dy(dudy, uu);
+        speed(uu, vv);
+        Ix(KEu, u2);
+        Iy(KEv, v2);
+        PV();
+        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) throw ShapeError();
+        AHa();
+        AHb();
+        AHg();
+        AHd();
+    }
+    void PV_ArakawaHsu() {                                     // PV_advection.jl:63-84
+        for (int j = 1; j <= qhv.n; ++j)
+            for (int i = 1; i <= qhv.m; ++i)
+                qhv(i, j) = qa(1 - ep + i, j) * V(2 - ep + i, j + 1) + qb(1 - ep + i, j) * V(1 - ep + i, j + 1) +
+                            qg(1 - ep + i, j) * V(1 - ep + i, j) + qd(1 - ep + i, j) * V(2 - ep + i, j);
+        for (int j = 1; j <= qhu.n; ++j)
+            for (int i = 1; i <= qhu.m; ++i)
+                qhu(i, j) = qa(i, j) * U(i, j + 1) + qb(i + 1, j) * U(i + 1, j + 1) + qg(i + 1, j + 1) * U(i + 1, j + 2) +
+                            qd(i, j + 1) * U(i, j + 2);
+    }
+    void bernoulli(const Arr<T> &e) {                          // rhs.jl:162-180
+        CHECK(p.m + ep == KEu.m && p.n + 2 == KEu.n && p.m + 2 == KEv.m && p.n == KEv.n);
+        const T half = T(0.5), g = T(c.g);
+        for (int j = 1; j <= p.n; ++j)
+            for (int i = 1; i <= p.m; ++i) p(i, j) = half * (KEu(i + ep, j + 1) + KEv(i + 1, j)) + g * e(i, j);
+    }
+    T Ftime(int64_t t, double w) const { return cvt<T>(std::sin(w * (double)t)); }  // forcing.jl:215-217
+    void momentum_u(int64_t t) {                               // rhs.jl:218-245
+        const int m = du.m - 4, n = du.n - 4;
+        CHECK(m == qhv.m && n == qhv.n && m + 2 - ep == dpdx.m && n + 2 == dpdx.n && m == Fx.m && n == Fx.n);
+        const T Fxt = c.seasonal_wind_x ? Ftime(t, c.omegaFx) : T(1.0);
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i)
+                du(i + 2, j + 2) = (cvt<Tp>(qhv(i, j)) - cvt<Tp>(dpdx(i + 1 - ep, j + 1))) + cvt<Tp>(Fxt * Fx(i, j));
+    }
+    void momentum_v(int64_t t) {                               // rhs.jl:248-274
+        const int m = dv.m - 4, n = dv.n - 4;
+        CHECK(m == qhu.m && n == qhu.n && m + 2 == dpdy.m && n + 2 == dpdy.n && m == Fy.m && n == Fy.n);
+        const T Fyt = c.seasonal_wind_y ? Ftime(t, c.omegaFy) : T(1.0);
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i)
+                dv(i + 2, j + 2) = -(cvt<Tp>(qhu(i, j)) + cvt<Tp>(dpdy(i + 1, j + 1))) + cvt<Tp>(Fyt * Fy(i, j));
+    }
+    void rhs(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e, int64_t t) {  // rhs.jl:2-51
+        if (c.dynamics != SWM_NONLINEAR) throw ShapeError();
+        UVfluxes(uu, vv, e);
+        if (c.nstep_advcor == 0) advection_coriolis(uu, vv);
+        PV_ArakawaHsu();
+        bernoulli(e);
+        dx(dpdx, p);
+        dy(dpdy, p);
+        momentum_u(t);
+        momentum_v(t);
+    }
+
+    // ------------------------------------------------------------ continuity.jl
+    void continuity(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e, int64_t t) {  // continuity.jl:72-98
+        if (c.time_scheme != SWM_RK) UVfluxes(uu, vv, e);
+        dx(dUdx, U);
+        dy(dVdy, V);
+        const int m = deta.m - 2, n = deta.n - 2;
+        CHECK(m == dUdx.m && n + 2 == dUdx.n && m + 2 == dVdy.m && n == dVdy.n);
+        if (c.continuity == SWM_CONT_RELAX) {                  // continuity.jl:3-24
+            const T gam = T(c.gamma);
+            for (int j = 1; j <= n; ++j)
+                for (int i = 1; i <= m; ++i)
+                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) +
+                                         cvt<Tp>(gam * (eta_ref(i, j) - e(i + 1, j + 1)));
+        } else if (c.continuity == SWM_CONT_FORCING) {         // continuity.jl:27-49
+            const T Fet = Ftime(t, c.omegaFeta);
+            for (int j = 1; j <= n; ++j)
+                for (int i = 1; i <= m; ++i)
+                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) + cvt<Tp>(Fet * Feta(i, j));
+        } else {                                               // continuity.jl:52-68
+            for (int j = 1; j <= n; ++j)
+                for (int i = 1; i <= m; ++i) deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j)));
+        }
+    }
+
+    // ------------------------------------------------------------ bottom_drag.jl
+    void bottom_drag(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e) {  // bottom_drag.jl:2-17
+        if (c.bottom_drag == SWM_DRAG_QUADRATIC) {             // bottom_drag.jl:20-76
+            const T cD = T(c.cD);
+            thickness(e);
+            Ix(h_u, h);
+            Iy(h_v, h);
+            speed(uu, vv);
+            Ix(KEu, u2);
+            Iy(KEv, v2);
+            CHECK(sqrtKE.m + ep == KEu.m && sqrtKE.n + 2 == KEu.n);
+            for (int j = 1; j <= sqrtKE.n; ++j)
+                for (int i = 1; i <= sqrtKE.m; ++i) sqrtKE(i, j) = num_sqrt(KEu(i + ep, j + 1) + KEv(i + 1, j));
+            Ix(sqrtKE_u, sqrtKE);
+            Iy(sqrtKE_v, sqrtKE);
+            for (int j = 1; j <= Bu.n; ++j)
+                for (int i = 1; i <= Bu.m; ++i) Bu(i, j) = cD * sqrtKE_u(i, j) * uu(i + 1 + ep, j + 1) / h_u(i, j);
+            for (int j = 1; j <= Bv.n; ++j)
+                for (int i = 1; i <= Bv.m; ++i) Bv(i, j) = cD * sqrtKE_v(i, j) * vv(i + 1, j + 1) / h_v(i, j);
+        } else if (c.bottom_drag == SWM_DRAG_LINEAR) {         // bottom_drag.jl:80-107
+            const T rD = T(c.rD);
+            for (int j = 1; j <= Bu.n; ++j)
+                for (int i = 1; i <= Bu.m; ++i) Bu(i, j) = rD * uu(i + 1 + ep, j + 1);
+            for (int j = 1; j <= Bv.n; ++j)
+                for (int i = 1; i <= Bv.m; ++i) Bv(i, j) = rD * vv(i + 1, j + 1);
+        } else {                                               // bottom_drag.jl:110-119
+            for (auto &x : Bu.a) x = T(0.0);
+            for (auto &x : Bv.a) x = T(0.0);
+        }
+    }
+
+    // ------------------------------------------------------------ diffusion.jl
+    void stress_tensor(const Arr<T> &uu, const Arr<T> &vv) {   // diffusion.jl:114-122
+        lap(Lu, uu);
+        lap(Lv, vv);
+        dx(dLudx, Lu);
+        dy(dLudy, Lu);
+        dx(dLvdx, Lv);
+        dy(dLvdy, Lv);
+    }
+    void smagorinsky_coeff() {                                 // diffusion.jl:68-111
+        const T cS = T(c.cSmag);
+        CHECK(DS_q.m + ep == dudy.m && DS_q.n == dudy.n && DS_q.m == dvdx.m && DS_q.n == dvdx.n);
+        for (int j = 1; j <= DS_q.n; ++j)
+            for (int i = 1; i <= DS_q.m; ++i) {
+                T s = dudy(i + ep, j) + dvdx(i, j);
+                DS_q(i, j) = s * s;
+            }
+        Ixy(DS, DS_q);
+        CHECK(DT.m + ep == dudx.m && DT.n + 2 == dudx.n && DT.m + 2 == dvdy.m && DT.n == dvdy.n);
+        for (int j = 1; j <= DT.n; ++j)
+            for (int i = 1; i <= DT.m; ++i) {
+                T s = dudx(i + ep, j + 1) + dvdy(i + 1, j);
+                DT(i, j) = s * s;
+            }
+        for (int j = 1; j <= nuSmag.n; ++j)
+            for (int i = 1; i <= nuSmag.m; ++i) nuSmag(i, j) = cS * num_sqrt(DS(i, j) + DT(i, j));
+        Ixy(nuSmag_q, nuSmag);
+    }
+    void viscous_tensor_smagorinsky() {                        // diffusion.jl:126-171
+        CHECK(S11.m + 2 - ep == nuSmag.m && S11.n == nuSmag.n);
+        for (int j = 1; j <= S11.n; ++j)
+            for (int i = 1; i <= S11.m; ++i) S11(i, j) = nuSmag(i + 1 - ep, j) * dLudx(i, j);
+        for (int j = 1; j <= S12.n; ++j)
+            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nuSmag_q(i, j) * dLudy(i + ep, j);
+        for (int j = 1; j <= S21.n; ++j)
+            for (int i = 1; i <= S21.m; ++i) S21(i, j) = nuSmag_q(i, j) * dLvdx(i, j);
+        for (int j = 1; j <= S22.n; ++j)
+            for (int i = 1; i <= S22.m; ++i) S22(i, j) = nuSmag(i, j + 1) * dLvdy(i, j);
+    }
+    void viscous_tensor_constant() {                           // diffusion.jl:175-218
+        const T nu = T(c.nuB);
+        for (int j = 1; j <= S11.n; ++j)
+            for (int i = 1; i <= S11.m; ++i) S11(i, j) = nu * dLudx(i, j);
+        for (int j = 1; j <= S12.n; ++j)
+            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nu * dLudy(i + ep, j);
+        for (int j = 1; j <= S21.n; ++j)
+            for (int i = 1; i <= S21.m; ++i) S21(i, j) = nu * dLvdx(i, j);
+        for (int j = 1; j <= S22.n; ++j)
+            for (int i = 1; i <= S22.m; ++i) S22(i, j) = nu * dLvdy(i, j);
+    }
+    void diffusion(const Arr<T> &uu, const Arr<T> &vv) {      // diffusion.jl:2-64
+        if (c.diffusion == SWM_DIFF_SMAGORINSKY) {
+            dx(dudx, uu);
+            dy(dvdy, vv);
+            dx(dvdx, vv);
+            dy(dudy, uu);
+            stress_tensor(uu, vv);
+            smagorinsky_coeff();
+            viscous_tensor_smagorinsky();
+        } else {
+            stress_tensor(uu, vv);
+            viscous_tensor_constant();
+        }
+        dx(LLu1, S11);
+        dy(LLu2, S12);
+        dx(LLv1, S21);
+        dy(LLv2, S22);
+    }
+    void add_drag_diff_tendencies(Arr<Tp> &uu, Arr<Tp> &vv) { // diffusion.jl:221-251
+        const Tp dtd = Tp(c.Dt_diff);
+        int m = uu.m - 4, n = uu.n - 4;
+        CHECK(m + 2 - ep == Bu.m && n + 2 == Bu.n && m == LLu1.m && n + 2 == LLu1.n && m + 2 - ep == LLu2.m && n == LLu2.n);
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i)
+                uu(i + 2, j + 2) += dtd * (cvt<Tp>(Bu(i + 1 - ep, j + 1)) + cvt<Tp>(LLu1(i, j + 1)) + cvt<Tp>(LLu2(i + 1 - ep, j)));
+        m = vv.m - 4;
+        n = vv.n - 4;
+        CHECK(m + 2 == Bv.m && n + 2 == Bv.n && m == LLv1.m && n + 2 == LLv1.n && m + 2 == LLv2.m && n == LLv2.n);
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i)
+                vv(i + 2, j + 2) += dtd * (cvt<Tp>(Bv(i + 1, j + 1)) + cvt<Tp>(LLv1(i, j + 1)) + cvt<Tp>(LLv2(i + 1, j)));
+    }
+
+    // ------------------------------------------------------------ tracer_advection.jl
+    // backtraj!  tracer_advection.jl:96-123
+    void backtraj(Arr<T> &rd, T dt, const Arr<T> &uv) {
+        int is, js;
+        if (rd.m == uv.m && rd.n == uv.n) { is = 0; js = 0; }
+        else if (rd.m + 4 == uv.m && rd.n + 2 == uv.n) { is = 2; js = 1; }
+        else if (rd.m + 2 + ep == uv.m && rd.n + 4 == uv.n) { is = 1 + ep; js = 2; }
+        else throw ShapeError();
+        for (int j = 1; j <= rd.n; ++j)
+            for (int i = 1; i <= rd.m; ++i) rd(i, j) = -dt * uv(i + is, j + js);
+    }
+    // clip / wrap  tracer_advection.jl:211-239 ; returns the absolute index, writes the coordinate
+    static int clip(int64_t xyi, int ij, T xy, int mn, int h1, T *x0) {
+        int64_t s = xyi + ij + h1;
+        if (s < 1 + h1) { *x0 = T(0.0); return 1 + h1; }
+        if (s > mn - 1) { *x0 = T(1.0); return mn - 1; }
+        *x0 = xy - cvt<T>((double)xyi);
+        return (int)s;
+    }
+    static int wrap(int64_t xyi, int ij, T xy, int mn, int h1, int h2, T *x0) {
+        int64_t s = xyi + ij + h1;
+        *x0 = xy - cvt<T>((double)xyi);
+        if (s < 1 + h1) return (int)(s + mn - h1 - h2);
+        if (s > mn - 1) return (int)(s - mn + h1 + h2);
+        return (int)s;
+    }
+    static T bilin(T f00, T f10, T f01, T f11, T x, T y) {     // tracer_advection.jl:243-246
+        const T one = T(1.0);
+        return f00 * (one - x) * (one - y) + f10 * x * (one - y) + f01 * (one - x) * y + f11 * x * y;
+    }
+    static int64_t ifloor(T x) {                                // Int(floor(x)); NaN throws InexactError
+        double d = std::floor(to_f64(x));
+        if (!std::isfinite(d)) throw ShapeError();
+        return (int64_t)d;
+    }
+    void interp_uv(Arr<T> &uvi, const Arr<T> &uv, const Arr<T> &xx, const Arr<T> &yy) {  // :130-173
+        const int m = uvi.m, n = uvi.n, muv = uv.m, nuv = uv.n;
+        int xh1, xh2, yh1, yh2;
+        if (m + 2 + ep == muv && n + 4 == nuv) { xh1 = 1 + ep; xh2 = 1; yh1 = 2; yh2 = 2; }
+        else if (m + 4 == muv && n + 2 == nuv) { xh1 = 2; xh2 = 2; yh1 = 1; yh2 = 1; }
+        else throw ShapeError();
+        (void)yh2;
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i) {
+                int64_t xi = ifloor(xx(i, j)), yi = ifloor(yy(i, j));
+                T x0, y0;
+                int k = periodic ? wrap(xi, i, xx(i, j), muv, xh1, xh2, &x0) : clip(xi, i, xx(i, j), muv, xh1, &x0);
+                int l = clip(yi, j, yy(i, j), nuv, yh1, &y0);
+                if (k < 1 || k + 1 > muv || l < 1 || l + 1 > nuv) throw ShapeError();  // Julia BoundsError
+                uvi(i, j) = bilin(uv(k, l), uv(k + 1, l), uv(k, l + 1), uv(k + 1, l + 1), x0, y0);
+            }
+    }
+    void departure(const Arr<T> &uu, const Arr<T> &vv) {       // tracer_advection.jl:52-85
+        Ix(u_T, uu);
+        Ix(um_T, um);
+        Iy(v_T, vv);
+        Iy(vm_T, vm);
+        backtraj(xd, T(c.half_dtadvu), u_T);
+        backtraj(yd, T(c.half_dtadvv), v_T);
+        interp_uv(uinterp, um_T, xd, yd);
+        interp_uv(vinterp, vm_T, xd, yd);
+        backtraj(xd, T(c.dtadvu), uinterp);
+        backtraj(yd, T(c.dtadvv), vinterp);
+    }
+    void adv_sst(const Arr<T> &s) {                            // tracer_advection.jl:179-206
+        const int m = ssti.m, n = ssti.n, hx = 1, hy = 0;
+        for (int j = 1; j <= n - 2 * hy; ++j)
+            for (int i = 1; i <= m - 2 * hx; ++i) {
+                int64_t xi = ifloor(xd(i, j)), yi = ifloor(yd(i, j));
+                T x0, y0;
+                int k = periodic ? wrap(xi, i, xd(i, j), m, hx, hx, &x0) : clip(xi, i, xd(i, j), m, hx, &x0);
+                int l = clip(yi, j, yd(i, j), n, hy, &y0);
+                if (k < 1 || k + 1 > m || l < 1 || l + 1 > n) throw ShapeError();
+                ssti(i + hx, j + hy) = bilin(s(k, l), s(k + 1, l), s(k, l + 1), s(k + 1, l + 1), x0, y0);
+            }
+    }
+    void tracer_consumption() {                                // tracer_advection.jl:262-275
+        const T j0 = T(c.jSST), smin = T(c.SSTmin);
+        for (int j = 1; j <= ssti.n; ++j)
+            for (int i = 2; i <= ssti.m - 1; ++i) ssti(i, j) += j0 * (smin - ssti(i, j));
+    }
+    void tracer(int64_t i, const Arr<T> &uu, const Arr<T> &vv) {  // tracer_advection.jl:1-44
+        if (!c.tracer_advection) return;
+        if ((i + c.nadvstep_half) % c.nadvstep == 0) {
+            copyto(um, uu);
+            copyto(vm, vv);
+        }
+        if (i % c.nadvstep == 0) {
+            const Arr<T> &sr = rhs_view(sst, sstr);
+            departure(uu, vv);
+            adv_sst(sr);
+            if (c.tracer_consumption) tracer_consumption();
+            ghost_sst_any(ssti);
+            copyto(sst, ssti);
+        }
+    }
+
+    // ------------------------------------------------------------ time_integration.jl
+    // axb!, caxb!, cxab!, cxayb!, dxaybzc!  time_integration.jl:253-326 (whole arrays)
+    static void axb(Arr<Tp> &a, Tp x, const Arr<Tp> &b) {
+        for (size_t k = 0; k < a.size(); ++k) a.a[k] += x * b.a[k];
+    }
+    static void caxb(Arr<Tp> &cc, const Arr<Tp> &a, Tp x, const Arr<Tp> &b) {
+        for (size_t k = 0; k < a.size(); ++k) cc.a[k] = a.a[k] + x * b.a[k];
+    }
+    static void cxab(Arr<Tp> &cc, double x, const Arr<Tp> &a, const Arr<Tp> &b) {
+        const Tp xT = cvt<Tp>(x);
+        for (size_t k = 0; k < a.size(); ++k) cc.a[k] = xT * (a.a[k] + b.a[k]);
+    }
+    static void cxayb(Arr<Tp> &cc, double x, const Arr<Tp> &a, double y, const Arr<Tp> &b) {
+        const Tp xT = cvt<Tp>(x), yT = cvt<Tp>(y);
+        for (size_t k = 0; k < a.size(); ++k) cc.a[k] = xT * a.a[k] + yT * b.a[k];
+    }
+    static void dxaybzc(Arr<Tp> &d, double x, const Arr<Tp> &a, double y, const Arr<Tp> &b, double z, const Arr<Tp> &cc) {
+        const Tp xT = cvt<Tp>(x), yT = cvt<Tp>(y), zT = cvt<Tp>(z);
+        for (size_t k = 0; k < a.size(); ++k) d.a[k] = xT * a.a[k] + yT * b.a[k] + zT * cc.a[k];
+    }
+
+    bool has_nan() const {                                    // feedback.jl:61-72
+        for (auto &x : u.a) if (is_nan(x)) return true;
+        for (auto &x : v.a) if (is_nan(x)) return true;
+        for (auto &x : eta.a) if (is_nan(x)) return true;
+        for (auto &x : sst.a) if (is_nan(x)) return true;
+        return false;
+    }
+
+    // Initial thicknesses and PV terms, time_integration.jl:20-36.  thickness! is called
+    // there with the Tprog array η: Julia promotes η[i]+H[i] to the wider format and
+    // rounds the sum to T on the store.
+    void init() {
+        using W = std::conditional_t<std::is_same_v<T, double> || std::is_same_v<Tp, double>, double, float>;
+        for (size_t k = 0; k < h.size(); ++k) h.a[k] = cvt<T>(cvt<W>(eta.a[k]) + cvt<W>(H.a[k]));
+        Ix(h_u, h);
+        Iy(h_v, h);
+        Ixy(h_q, h);
+        const Arr<T> &ur_ = rhs_view(u, ur);
+        const Arr<T> &vr_ = rhs_view(v, vr);
+        advection_coriolis(ur_, vr_);
+        PV_ArakawaHsu();
+        copyto(u0, u);
+        copyto(v0, v);
+        copyto(eta0, eta);
+    }
+
+    // One pass of the loop body of time_integration.jl:45-243.  stop_stage>0 ends the step
+    // after that many stages of the scheme (for stage-level parity).  Returns true when NaNs
+    // were detected (the caller breaks).
+    bool step(int64_t i, int64_t t, int stop_stage) {
+        ghost_points(u, v, eta);                               // :48
+        copyto(u1, u);
+        copyto(v1, v);
+        copyto(eta1, eta);
+        int stages_run = 0;
+        auto stop = [&]() { return stop_stage > 0 && stages_run >= stop_stage; };
+
+        if (c.time_scheme == SWM_RK) {                         // :53-78
+            for (int rki = 1; rki <= c.RKo; ++rki) {
+                if (rki > 1) ghost_points(u1, v1, eta1);
+                const Arr<T> &a = rhs_view(u1, ur), &b = rhs_view(v1, vr), &e = rhs_view(eta1, etar);
+                rhs(a, b, e, t);
+                continuity(a, b, e, t);
+                if (rki < c.RKo) {
+                    caxb(u1, u, Tp(c.RKbDt[rki - 1]), du);
+                    caxb(v1, v, Tp(c.RKbDt[rki - 1]), dv);
+                    caxb(eta1, eta, Tp(c.RKbDt[rki - 1]), deta);
+                }
+                axb(u0, Tp(c.RKaDt[rki - 1]), du);
+                axb(v0, Tp(c.RKaDt[rki - 1]), dv);
+                axb(eta0, Tp(c.RKaDt[rki - 1]), deta);
+                ++stages_run;
+                if (stop()) return false;
+            }
+        } else if (c.time_scheme == SWM_SSPRK2) {              // :80-110
+            const Tp dts = Tp(c.Dt_Ds);
+            for (int rki = 1; rki <= c.RKs; ++rki) {
+                if (rki > 1) ghost_points_eta(eta1);
+                const Arr<T> &e = rhs_view(eta1, etar);
+                rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                axb(u1, dts, du);
+                axb(v1, dts, dv);
+                ghost_points_uv(u1, v1);
+                continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                axb(eta1, dts, deta);
+                ++stages_run;
+                if (stop()) return false;
+            }
+            const double a = 1.0 / c.RKs, b = (double)(c.RKs - 1) / c.RKs;
+            cxayb(u0, a, u, b, u1);
+            cxayb(v0, a, v, b, v1);
+            cxayb(eta0, a, eta, b, eta1);
+        } else if (c.time_scheme == SWM_SSPRK3) {              // :112-156
+            const Tp dtn = Tp(c.ssp_Dt_Dn), dtnc = Tp(c.ssp_Dt_Dnc);
+            for (int rki = 2; rki <= c.ssp_s + 1; ++rki) {
+                if (rki > 2) ghost_points_eta(eta1);
+                const Arr<T> &e = rhs_view(eta1, etar);
+                rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                if (rki == c.ssp_kn) {
+                    dxaybzc(u1, c.ssp_kna, u1, c.ssp_knb, u0, to_f64(dtnc), du);
+                    dxaybzc(v1, c.ssp_kna, v1, c.ssp_knb, v0, to_f64(dtnc), dv);
+                } else {
+                    axb(u1, dtn, du);
+                    axb(v1, dtn, dv);
+                }
+                ghost_points_uv(u1, v1);
+                continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                if (rki == c.ssp_kn) dxaybzc(eta1, c.ssp_kna, eta1, c.ssp_knb, eta0, to_f64(dtnc), deta);
+                else axb(eta1, dtn, deta);
+                if (rki == c.ssp_mn || rki == c.ssp_s + 1) {
+                    copyto(u0, u1);
+                    copyto(v0, v1);
+                    ghost_points_eta(eta1);
+                    copyto(eta0, eta1);
+                }
+                ++stages_run;
+                if (stop()) return false;
+            }
+        } else {                                               // 4SSPRK3 :158-195
+            const Tp dt = Tp(c.Dt_D);
+            for (int rki = 1; rki <= 4; ++rki) {
+                if (rki > 1) ghost_points(u1, v1, eta1);
+                const Arr<T> &e = rhs_view(eta1, etar);
+                rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                caxb(u0, u1, dt, du);
+                caxb(v0, v1, dt, dv);
+                cxab(u1, 0.5, u1, u0);
+                cxab(v1, 0.5, v1, v0);
+                ghost_points_uv(u1, v1);
+                continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                caxb(eta0, eta1, dt, deta);
+                cxab(eta1, 0.5, eta1, eta0);
+                if (rki == 3) {
+                    cxayb(u1, 2.0 / 3.0, u, 1.0 / 3.0, u1);
+                    cxayb(v1, 2.0 / 3.0, v, 1.0 / 3.0, v1);
+                    cxayb(eta1, 2.0 / 3.0, eta, 1.0 / 3.0, eta1);
+                } else if (rki == 4) {
+                    copyto(u0, u1);
+                    copyto(v0, v1);
+                    copyto(eta0, eta1);
+                }
+                ++stages_run;
+                if (stop()) return false;
+            }
+        }
+
+        ghost_points(u0, v0, eta0);                            // :198
+        const Arr<T> &a0 = rhs_view(u0, ur), &b0 = rhs_view(v0, vr), &e0 = rhs_view(eta0, etar);
+        if (c.nstep_advcor > 0 && i % c.nstep_advcor == 0) {   // :208-211
+            UVfluxes(a0, b0, e0);
+            advection_coriolis(a0, b0);
+        }
+        if (i % c.nstep_diff == 0) {                           // :216-221
+            bottom_drag(a0, b0, e0);
+            diffusion(a0, b0);
+            add_drag_diff_tendencies(u0, v0);
+            ghost_points_uv(u0, v0);
+        }
+        // :226-228 tracer with the post-dissipation velocities
+        tracer(i, rhs_view(u0, ur), rhs_view(v0, vr));
+        if (c.nout > 0 && i % c.nout == 0 && has_nan()) return true;  // feedback.jl:136-148
+        copyto(u, u0);                                         // :240-242
+        copyto(v, v0);
+        copyto(eta, eta0);
+        return false;
+    }
+};
+
+// ------------------------------------------------------------------ C entry points
+
+struct OracleStatic {
+    const void *H, *Fx, *Fy, *f_q, *eta_ref, *Feta;
+};
+
+template <class T, class Tp, class Tc>
+static int run_typed(const swm_config *c, const OracleStatic *s, void *u, void *v, void *eta, void *sst,
+                     int64_t nsteps, int64_t i0, int64_t t0, int32_t stop_stage, void **rk_out,
+                     int64_t *steps_done, int64_t *nan_step) {
+    Model<T, Tp, Tc> M(*c);
+    load_arr(M.H, s->H);
+    load_arr(M.Fx, s->Fx);
+    load_arr(M.Fy, s->Fy);
+    load_arr(M.f_q, s->f_q);
+    if (s->eta_ref) load_arr(M.eta_ref, s->eta_ref);
+    if (s->Feta) load_arr(M.Feta, s->Feta);
+    load_arr(M.u, u);
+    load_arr(M.v, v);
+    load_arr(M.eta, eta);
+    load_arr(M.sst, sst);
+    M.init();
+    *steps_done = 0;
+    *nan_step = 0;
+    int64_t t = t0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t i = i0 + k;
+        const bool last = k == nsteps - 1;
+        const int ss = last ? stop_stage : 0;
+        if (M.step(i, t, ss)) {
+            *nan_step = i;
+            break;
+        }
+        *steps_done = k + 1;
+        if (ss > 0) break;
+        t += c->dtint;
+    }
+    store_arr(M.u, u);
+    store_arr(M.v, v);
+    store_arr(M.eta, eta);
+    store_arr(M.sst, sst);
+    if (rk_out) {
+        Arr<Tp> *arrs[9] = {&M.u0, &M.v0, &M.eta0, &M.u1, &M.v1, &M.eta1, &M.du, &M.dv, &M.deta};
+        for (int k = 0; k < 9; ++k)
+            if (rk_out[k]) store_arr(*arrs[k], rk_out[k]);
+    }
+    return 0;
+}
+
+template <class T, class Tp>
+static int dispatch_comm(const swm_config *c, const OracleStatic *s, void *u, void *v, void *eta, void *sst,
+                         int64_t n, int64_t i0, int64_t t0, int32_t ss, void **rk, int64_t *sd, int64_t *ns) {
+    switch (c->Tcomm) {
+    case SWM_F64: return run_typed<T, Tp, double>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_F32: return run_typed<T, Tp, float>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_F16: return run_typed<T, Tp, f16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_BF16: return run_typed<T, Tp, bf16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    }
+    return -1;
+}
+
+template <class T>
+static int dispatch_prog(const swm_config *c, const OracleStatic *s, void *u, void *v, void *eta, void *sst,
+                         int64_t n, int64_t i0, int64_t t0, int32_t ss, void **rk, int64_t *sd, int64_t *ns) {
+    switch (c->Tprog) {
+    case SWM_F64: return dispatch_comm<T, double>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_F32: return dispatch_comm<T, float>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_F16: return dispatch_comm<T, f16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    case SWM_BF16: return dispatch_comm<T, bf16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
+    }
+    return -1;
+}
+
+extern "C" {
+
+// Run nsteps steps of the time loop from Prog (u, v, eta, sst; format Tprog, Julia layout,
+// updated in place).  rk_out (nullable, 9 nullable pointers) receives u0 v0 eta0 u1 v1 eta1
+// du dv deta of the last step.  Returns 0, or -1 on a shape/option error or an index
+// the reference would throw on (NaN in the tracer, departure outside the arrays).
+int swmo_run(const swm_config *c, const void *H, const void *Fx, const void *Fy, const void *f_q,
+             const void *eta_ref, const void *Feta, void *u, void *v, void *eta, void *sst, int64_t nsteps,
+             int64_t i0, int64_t t0, int32_t stop_stage, void **rk_out, int64_t *steps_done, int64_t *nan_step) {
+    OracleStatic s{H, Fx, Fy, f_q, eta_ref, Feta};
+    try {
+        switch (c->T) {
+        case SWM_F64: return dispatch_prog<double>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
+        case SWM_F32: return dispatch_prog<float>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
+        case SWM_F16: return dispatch_prog<f16>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
+        case SWM_BF16: return dispatch_prog<bf16>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
+        }
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return -1;
+}
+
+// tracer! alone (tracer_advection.jl:1-44) at step i for T = Tprog = double: u, v are the
+// post-dissipation velocities, um/vm the mid-point snapshot (updated in place when due),
+// sst the tracer (updated in place).  Closed-form pins of departure points and adv_sst!.
+int swmo_tracer(const swm_config *c, int64_t i, const double *u, const double *v, double *um, double *vm, double *sst) {
+    try {
+        Model<double, double, double> M(*c);
+        std::memcpy(M.um.a.data(), um, sizeof(double) * M.um.size());
+        std::memcpy(M.vm.a.data(), vm, sizeof(double) * M.vm.size());
+        std::memcpy(M.sst.a.data(), sst, sizeof(double) * M.sst.size());
+        Arr<double> uu(M.u.m, M.u.n), vv(M.v.m, M.v.n);
+        std::memcpy(uu.a.data(), u, sizeof(double) * uu.size());
+        std::memcpy(vv.a.data(), v, sizeof(double) * vv.size());
+        M.tracer(i, uu, vv);
+        std::memcpy(um, M.um.a.data(), sizeof(double) * M.um.size());
+        std::memcpy(vm, M.vm.a.data(), sizeof(double) * M.vm.size());
+        std::memcpy(sst, M.sst.a.data(), sizeof(double) * M.sst.size());
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return 0;
+}
+
+// Single-operator entry points on float64 arrays (closed-form pins of the stencils).
+// op: 0 ∂x, 1 ∂y, 2 ∇², 3 Ix, 4 Iy, 5 Ixy.  `in` is (m_in, n_in); `out` gets the
+// reference's output shape.
+int swmo_stencil(int op, const double *in, int m_in, int n_in, double *out) {
+    try {
+        Arr<double> a(m_in, n_in), o;
+        std::memcpy(a.a.data(), in, sizeof(double) * a.size());
+        switch (op) {
+        case 0: o = Arr<double>(m_in - 1, n_in); dx(o, a); break;
+        case 1: o = Arr<double>(m_in, n_in - 1); dy(o, a); break;
+        case 2: o = Arr<double>(m_in - 2, n_in - 2); lap(o, a); break;
+        case 3: o = Arr<double>(m_in - 1, n_in); Ix(o, a); break;
+        case 4: o = Arr<double>(m_in, n_in - 1); Iy(o, a); break;
+        case 5: o = Arr<double>(m_in - 1, n_in - 1); Ixy(o, a); break;
+        default: return -1;
+        }
+        std::memcpy(out, o.a.data(), sizeof(double) * o.size());
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return 0;
+}
+
+// Format conversions of the oracle (Julia's T(x)), for checking the emulated formats.
+// fmt: SWM_F16 or SWM_BF16; in float64, out float64 values of the rounded numbers.
+int swmo_round(int fmt, const double *in, double *out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) {
+        if (fmt == SWM_F16) out[k] = (double)f16(in[k]).v;
+        else if (fmt == SWM_BF16) out[k] = (double)bf16(in[k]).v;
+        else if (fmt == SWM_F32) out[k] = (double)(float)in[k];
+        else out[k] = in[k];
+    }
+    return 0;
+}
+
+// binary op in an emulated format: op 0 +, 1 -, 2 *, 3 /, 4 sqrt(a)
+int swmo_arith(int fmt, int op, const double *a, const double *b, double *out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) {
+        if (fmt == SWM_F16) {
+            f16 x = f16(a[k]), y = f16(b[k]), r;
+            r = op == 0 ? x + y : op == 1 ? x - y : op == 2 ? x * y : op == 3 ? x / y : num_sqrt(x);
+            out[k] = r.v;
+        } else {
+            bf16 x = bf16(a[k]), y = bf16(b[k]), r;
+            r = op == 0 ? x + y : op == 1 ? x - y : op == 2 ? x * y : op == 3 ? x / y : num_sqrt(x);
+            out[k] = r.v;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

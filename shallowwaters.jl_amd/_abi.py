@@ -1,0 +1,69 @@
+"""ctypes binding of ``libswm_hip.so`` (the C ABI of include/swm.h).
+
+There is no fallback: if the HIP library is missing or no GPU is visible, every call that
+needs it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswm_hip.so")
+
+# every symbol include/swm.h declares
+EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_last_error",
+           "swm_set_static", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
+           "swm_download_rk", "swm_reduce", "swm_time_kernel"]
+
+_lib = None
+
+
+class SwmError(RuntimeError):
+    """A non-zero status of the C ABI (message from swm_last_error)."""
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library {LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    pi64 = ctypes.POINTER(i64)
+    L.swm_version.argtypes = [ctypes.POINTER(i32)]
+    L.swm_device_count.argtypes = [ctypes.POINTER(i32)]
+    L.swm_create.argtypes = [vp, ctypes.POINTER(vp)]
+    L.swm_destroy.argtypes = [vp]
+    L.swm_last_error.argtypes = [vp]
+    L.swm_last_error.restype = ctypes.c_char_p
+    L.swm_set_static.argtypes = [vp] * 7
+    L.swm_upload_prog.argtypes = [vp] * 5
+    L.swm_download_prog.argtypes = [vp] * 5
+    L.swm_step.argtypes = [vp, i64, i64, i64, pi64, pi64]
+    L.swm_stages.argtypes = [vp, i32, i64, i64]
+    L.swm_download_rk.argtypes = [vp] * 7
+    L.swm_reduce.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.swm_time_kernel.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]
+    for name in EXPORTS:
+        if name != "swm_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None, what: str = ""):
+    if rc != 0:
+        msg = ""
+        if ctx is not None:
+            m = load().swm_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise SwmError(f"{what} failed with status {rc}: {msg}")
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.flags.f_contiguous, "host arrays must be in Julia (Fortran) layout"
+    return ctypes.c_void_p(a.ctypes.data)

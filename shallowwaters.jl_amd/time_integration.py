@@ -1,0 +1,146 @@
+"""``time_integration`` and ``run_model`` on the MI355X library.
+
+``time_integration(Prog, Diag, S)`` keeps the signature and contract of
+``src/time_integration.jl:2-250``: it integrates ``S.grid.nt`` steps from the halo-padded
+``Prog`` (mutated in place, as the reference mutates its arrays), stops early when NaNs are
+detected (``feedback.jl:136-148``) and returns a new ``PrognosticVars`` without halos
+(``remove_halo``).  The loop itself runs on the GPU through ``include/swm.h``.
+
+``run_model`` mirrors ``src/run_model.jl:12-39``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .initial_conditions import PrognosticVars, initial_conditions, remove_halo
+from .model_setup import ModelSetup, make_config, make_setup
+from .numtypes import canonical_type, storage_dtype
+from .parameter import Parameter
+
+
+@dataclass
+class DiagnosticVars:
+    """``DiagnosticVars{T,Tprog}`` (``src/preallocate.jl:439-451``).  On the GPU the
+    intermediate fields are fused away inside the kernels; what remains host-visible is
+    the Runge-Kutta state, downloaded on request (``Simulation.download_rk``)."""
+    T: str
+    Tprog: str
+    RungeKutta: dict = field(default_factory=dict)
+
+
+def preallocate(T, Tprog, G) -> DiagnosticVars:
+    """``preallocate(T,Tprog,G)``, ``src/preallocate.jl:454-471``."""
+    return DiagnosticVars(T=canonical_type(T), Tprog=canonical_type(Tprog))
+
+
+class Simulation:
+    """One library context: device buffers, static fields and the prognostic state."""
+
+    def __init__(self, S: ModelSetup):
+        self.S = S
+        self.cfg = make_config(S)
+        L = _abi.load()
+        ctx = ctypes.c_void_p()
+        _abi.check(L.swm_create(ctypes.byref(self.cfg), ctypes.byref(ctx)), None, "swm_create")
+        self.ctx = ctx
+        F, G = S.forcing, S.grid
+        self._static = [np.asfortranarray(a) for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
+        _abi.check(L.swm_set_static(ctx, *[_abi.ptr(a) for a in self._static]), ctx, "swm_set_static")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            _abi.load().swm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, Prog: PrognosticVars):
+        arrs = [np.asfortranarray(a) for a in (Prog.u, Prog.v, Prog.η, Prog.sst)]
+        dt = storage_dtype(self.S.parameters.Tprog)
+        for a in arrs:
+            if a.dtype != dt:
+                raise TypeError(f"Prog arrays must be stored as {dt}, got {a.dtype}")
+        _abi.check(_abi.load().swm_upload_prog(self.ctx, *[_abi.ptr(a) for a in arrs]), self.ctx, "swm_upload_prog")
+
+    def download(self, Prog: PrognosticVars | None = None) -> PrognosticVars:
+        if Prog is None:
+            G, dt = self.S.grid, storage_dtype(self.S.parameters.Tprog)
+            z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
+            Prog = PrognosticVars(u=z(G.nux + 4, G.nuy + 4), v=z(G.nvx + 4, G.nvy + 4), η=z(G.nx + 2, G.ny + 2),
+                                  sst=z(G.nx + 2, G.ny), T=self.S.parameters.Tprog)
+        _abi.check(_abi.load().swm_download_prog(self.ctx, *[_abi.ptr(a) for a in (Prog.u, Prog.v, Prog.η, Prog.sst)]),
+                   self.ctx, "swm_download_prog")
+        return Prog
+
+    def step(self, nsteps: int, i0: int = 1, t0: int = 0):
+        """Steps ``i0 .. i0+nsteps-1``; returns ``(steps_done, nan_step)``."""
+        sd, ns = ctypes.c_int64(0), ctypes.c_int64(0)
+        _abi.check(_abi.load().swm_step(self.ctx, nsteps, i0, t0, ctypes.byref(sd), ctypes.byref(ns)),
+                   self.ctx, "swm_step")
+        return sd.value, ns.value
+
+    def stages(self, n_stages: int, i0: int = 1, t0: int = 0):
+        _abi.check(_abi.load().swm_stages(self.ctx, n_stages, i0, t0), self.ctx, "swm_stages")
+
+    def download_rk(self):
+        G, dt = self.S.grid, storage_dtype(self.S.parameters.Tprog)
+        z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
+        out = (z(G.nux + 4, G.nuy + 4), z(G.nvx + 4, G.nvy + 4), z(G.nx + 2, G.ny + 2),
+               z(G.nux + 4, G.nuy + 4), z(G.nvx + 4, G.nvy + 4), z(G.nx + 2, G.ny + 2))
+        _abi.check(_abi.load().swm_download_rk(self.ctx, *[_abi.ptr(a) for a in out]), self.ctx, "swm_download_rk")
+        return out
+
+    def time_kernel(self, which: int, reps: int) -> float:
+        """Average duration (ms) of one kernel launched ``reps`` times on the context's
+        stream, timed with HIP events (0 middle RK stage, 1 first stage, 2 last stage,
+        3 dissipation, 4 tracer advection)."""
+        ms = ctypes.c_double(0)
+        _abi.check(_abi.load().swm_time_kernel(self.ctx, which, reps, ctypes.byref(ms)), self.ctx, "swm_time_kernel")
+        return ms.value
+
+    def reduce(self):
+        m, e = ctypes.c_double(0), ctypes.c_double(0)
+        _abi.check(_abi.load().swm_reduce(self.ctx, ctypes.byref(m), ctypes.byref(e)), self.ctx, "swm_reduce")
+        return m.value, e.value
+
+
+def time_integration(Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup) -> PrognosticVars:
+    """``src/time_integration.jl:2-250`` on the GPU."""
+    G = S.grid
+    with Simulation(S) as sim:
+        sim.upload(Prog)
+        done, nan_step = sim.step(G.nt, 1, 0)
+        sim.download(Prog)
+    if nan_step:
+        print(f" NaNs detected at time step {nan_step}")
+    u, v, η, sst = remove_halo(Prog.u, Prog.v, Prog.η, Prog.sst, S)
+    return PrognosticVars(u=u, v=v, η=η, sst=sst, T=S.parameters.Tprog)
+
+
+def run_model(T=None, P: Parameter | None = None, **kwargs) -> PrognosticVars:
+    """``run_model(T; kwargs...)``, ``run_model(P)`` and ``run_model(T,P)``
+    (``src/run_model.jl:12-39``)."""
+    if isinstance(T, Parameter):
+        P, T = T, T.T
+    if P is None:
+        P = Parameter(T=T if T is not None else "Float32", **kwargs)
+    elif kwargs:
+        raise TypeError("pass either a Parameter or keyword arguments, not both")
+    S = make_setup(P)
+    Prog = initial_conditions(P.Tprog, S)
+    Diag = preallocate(P.T, P.Tprog, S.grid)
+    return time_integration(Prog, Diag, S)

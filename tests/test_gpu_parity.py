@@ -1,0 +1,165 @@
+"""Parity of the HIP time step with the CPU oracle, through the C ABI.
+
+Bar: Float64 and Float32 results are bitwise equal to the oracle (same operations in the
+same order, IEEE division and sqrt, no FMA contraction); the tolerances of the
+specification (relative L2 <= 1e-12 at Float64, <= 1e-5 at Float32, per field after 10
+steps) are asserted as well.
+"""
+import numpy as np
+import pytest
+
+from conftest import first_mismatch, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"Float64": 1e-12, "Float32": 1e-5}
+
+# small grids that cross strip (60 columns) and segment (32 rows) boundaries
+CASES = {
+    "np-f64-smag-quad": dict(T="Float64", nx=70, L_ratio=0.9, bc="nonperiodic", α=0.0, diffusion="Smagorinsky",
+                             bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre"),
+    "np-f32-const-lin": dict(T="Float32", nx=64, L_ratio=1.3, bc="nonperiodic", α=2.0, diffusion="constant",
+                             bottom_drag="linear", topography="ridge", wind_forcing_x="channel"),
+    "np-f32-smag-quad-partial": dict(T="Float32", nx=61, L_ratio=0.8, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                                     bottom_drag="quadratic", topography="seamount", wind_forcing_x="shear"),
+    "per-f64-smag-quad": dict(T="Float64", nx=72, L_ratio=1.1, bc="periodic", α=1.0, diffusion="Smagorinsky",
+                              bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre"),
+    "per-f32-const-none": dict(T="Float32", nx=123, L_ratio=2.0, bc="periodic", diffusion="constant",
+                               bottom_drag="none", topography="flat", wind_forcing_x="channel"),
+    "per-f32-smag-quad-rk3": dict(T="Float32", nx=66, L_ratio=1.0, bc="periodic", diffusion="Smagorinsky",
+                                  bottom_drag="quadratic", topography="ridges", wind_forcing_x="constant", RKo=3,
+                                  cfl=0.6),
+    "np-f64-const-quad-rk2": dict(T="Float64", nx=50, L_ratio=0.5, bc="nonperiodic", α=1.5, diffusion="constant",
+                                  bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre", RKo=2,
+                                  cfl=0.5),
+}
+
+
+def setup_case(swm, name, seed=11, σ=0.8, **over):
+    kw = dict(CASES[name])
+    kw.update(over)
+    kw.setdefault("Uadv", 20.0)          # tracer every few steps
+    P = swm.Parameter(**kw)
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=seed, σ=σ)
+    return P, S, prog
+
+
+def ghosted(swm, S, u, v, e):
+    """Apply the host ghost points (the reference's ghost_points!) to copies."""
+    T = S.parameters.Tprog
+    u, v, e = (swm.to_float64(a, T).copy(order="F") for a in (u, v, e))
+    from shallowwaters_amd.initial_conditions import ghost_points
+    ghost_points(u, v, e, S)
+    return u, v, e
+
+
+def assert_same(name, ref, got, T):
+    ref64 = np.asarray(ref, dtype=np.float64)
+    got64 = np.asarray(got, dtype=np.float64)
+    mm = first_mismatch(ref64, got64)
+    err = rel_l2(ref64, got64)
+    assert err <= TOL[T], f"{name}: rel L2 {err:.3e}; {mm}"
+    assert mm is None, f"{name}: not bitwise equal ({err:.3e}): {mm}"
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_rk_stages_bitwise(swm, orc, gpu, case):
+    P, S, prog = setup_case(swm, case)
+    cfg = swm.make_config(S)
+    T = P.Tprog
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        for k in range(1, P.RKo + 1):
+            sim.stages(k, 1, 0)
+            g_u0, g_v0, g_e0, g_u1, g_v1, g_e1 = sim.download_rk()
+            *_, rk, _, _ = orc.run(S, cfg, prog, 1, stop_stage=k, want_rk=True)
+            r_u0, r_v0, r_e0, r_u1, r_v1, r_e1 = rk[:6]
+            # u1 enters the next stage after ghost_points! (time_integration.jl:57)
+            r_u1, r_v1, r_e1 = ghosted(swm, S, r_u1, r_v1, r_e1)
+            assert_same(f"{case} stage {k} u1", r_u1, g_u1, T)
+            assert_same(f"{case} stage {k} v1", r_v1, g_v1, T)
+            assert_same(f"{case} stage {k} η1", r_e1, g_e1, T)
+            if k < P.RKo:   # u0 halo is not maintained between stages; compare the interior
+                assert_same(f"{case} stage {k} u0", r_u0[2:-2, 2:-2], g_u0[2:-2, 2:-2], T)
+                assert_same(f"{case} stage {k} v0", r_v0[2:-2, 2:-2], g_v0[2:-2, 2:-2], T)
+                assert_same(f"{case} stage {k} η0", r_e0[1:-1, 1:-1], g_e0[1:-1, 1:-1], T)
+            else:           # ghost_points!(u0,v0,η0) after the stages (:198)
+                r_u0, r_v0, r_e0 = ghosted(swm, S, r_u0, r_v0, r_e0)
+                assert_same(f"{case} stage {k} u0", r_u0, g_u0, T)
+                assert_same(f"{case} stage {k} v0", r_v0, g_v0, T)
+                assert_same(f"{case} stage {k} η0", r_e0, g_e0, T)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_ten_steps_bitwise(swm, orc, gpu, case):
+    P, S, prog = setup_case(swm, case)
+    assert S.grid.nadvstep <= 5, S.grid.nadvstep
+    nsteps = 10
+    r = orc.run(S, swm.make_config(S), prog, nsteps)
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        done, nan_step = sim.step(nsteps, 1, 0)
+        g = sim.download()
+    assert (done, nan_step) == (nsteps, 0)
+    for name, a, b in zip(("u", "v", "η", "sst"), r[:4], (g.u, g.v, g.η, g.sst)):
+        assert_same(f"{case} {name}", swm.to_float64(a, P.Tprog), swm.to_float64(b, P.Tprog), P.Tprog)
+
+
+def test_nstep_diff_and_chunked_calls(swm, orc, gpu):
+    """Dissipation every 3rd step, steps issued in two calls with i0/t0 carried over."""
+    P, S, prog = setup_case(swm, "per-f64-smag-quad", nstep_diff=3)
+    r = orc.run(S, swm.make_config(S), prog, 11)
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        assert sim.step(4, 1, 0) == (4, 0)
+        assert sim.step(7, 5, 4 * S.grid.dtint) == (7, 0)
+        g = sim.download()
+    for name, a, b in zip(("u", "v", "η", "sst"), r[:4], (g.u, g.v, g.η, g.sst)):
+        assert_same(f"chunked {name}", a, b, P.Tprog)
+
+
+def test_exact_zero_known_answer(swm, gpu):
+    """test/runtests.jl:4-9: no forcing from rest stays exactly zero (1 model day)."""
+    P = swm.Parameter(T="Float32", Ndays=1, Fx0=0)
+    prog = swm.run_model(P)
+    assert np.all(prog.u == 0) and np.all(prog.v == 0) and np.all(prog.η == 0)
+
+
+def test_nan_detection_stops_before_copy_back(swm, orc, gpu):
+    """feedback.jl:136-148 + time_integration.jl:235-237: NaNs seen at i % nout == 0 stop
+    the loop and Prog keeps the state of the previous step."""
+    P, S, prog = setup_case(swm, "np-f64-smag-quad", tracer_advection=False, output_dt=0.25)
+    nout = S.grid.nout
+    assert 1 <= nout <= 4, nout
+    u = prog.u.copy(order="F")
+    u[10, 12] = np.nan
+    prog.u = u
+    r = orc.run(S, swm.make_config(S), prog, 10)
+    assert r[6] == nout
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        done, nan_step = sim.step(10, 1, 0)
+        g = sim.download()
+    assert (done, nan_step) == (nout - 1, nout)
+    for name, a, b in zip(("u", "v", "η", "sst"), r[:4], (g.u, g.v, g.η, g.sst)):
+        assert np.array_equal(a, b, equal_nan=True), name
+
+
+def test_upload_rejects_nonzero_unwritten_halo(swm, gpu):
+    P, S, prog = setup_case(swm, "np-f32-const-lin")
+    u = prog.u.copy(order="F")
+    u[0, 10] = 1.0          # normal-velocity halo of a nonperiodic u
+    prog.u = u
+    with swm.Simulation(S) as sim:
+        with pytest.raises(Exception, match="halo"):
+            sim.upload(prog)
+
+
+def test_unsupported_options_fail_loudly(swm, gpu):
+    for kw in (dict(time_scheme="SSPRK2"), dict(nstep_advcor=1), dict(dynamics="linear"),
+               dict(adv_scheme="Sadourny"), dict(surface_relax=True)):
+        P = swm.Parameter(nx=40, **kw)
+        S = swm.make_setup(P)
+        with pytest.raises(Exception):
+            swm.Simulation(S)
