@@ -75,6 +75,18 @@ def run(S, cfg, prog, nsteps, i0=1, t0=0, stop_stage=0, want_rk=False):
     return u, v, e, s, rk, sd.value, ns.value
 
 
+def tracer(cfg, i, u, v, um, vm, sst):
+    """``tracer!`` alone at step ``i`` (Float64): returns updated copies of um, vm, sst."""
+    um, vm, sst = (np.array(x, dtype=np.float64, order="F", copy=True) for x in (um, vm, sst))
+    u = np.asfortranarray(u, dtype=np.float64)
+    v = np.asfortranarray(v, dtype=np.float64)
+    L = lib()
+    L.swmo_tracer.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    if L.swmo_tracer(ctypes.byref(cfg), i, _ptr(u), _ptr(v), _ptr(um), _ptr(vm), _ptr(sst)) != 0:
+        raise OracleError("tracer failed (NaN or departure outside the arrays)")
+    return um, vm, sst
+
+
 _STENCIL = {"dx": (0, -1, 0), "dy": (1, 0, -1), "lap": (2, -2, -2), "Ix": (3, -1, 0), "Iy": (4, 0, -1),
             "Ixy": (5, -1, -1)}
 

@@ -465,6 +465,10 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     ctx->rows = c.ny + 2 * OY + 4;
     ctx->elems = (size_t)ctx->P * ctx->rows;
     ctx->L = c.ny >= 2048 ? 64 : 32;
+    if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {   // tuning knob: rows each wavefront marches
+        const int l = atoi(s);
+        if (l >= 4 && l <= 4096) ctx->L = l;
+    }
     if (hipGetDevice(&ctx->dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         fprintf(stderr, "swm_create: no HIP device/stream\n");
         delete ctx;

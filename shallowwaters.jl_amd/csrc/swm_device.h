@@ -183,11 +183,31 @@ __global__ void __launch_bounds__(256) rk_stage_kernel(StageArgs<R> A) {
     R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
     R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
 
+    // Loads run one row ahead of their use: the stencil inputs of row r+1 and the
+    // pointwise operands (Prog, u0) of the rows emitted next iteration are in flight
+    // while row r is computed.
     int r = t.ya - 1;
+    const int rlast = t.yb + 1;
     const R *pu = A.in_u + x + r * P, *pv = A.in_v + x + r * P, *pe = A.in_e + x + r * P;
     const R *pH = A.st.H + x + r * P;
-    for (; r <= t.yb + 1; ++r, pu += P, pv += P, pe += P, pH += P) {
-        const R u_0 = *pu, v_0 = *pv, e_0 = *pe, H_0 = *pH;
+    R nu = *pu, nv = *pv, ne = *pe, nH = *pH;
+    R npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
+    for (; r <= rlast; ++r) {
+        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
+        const R pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+        if (r < rlast) {
+            pu += P; pv += P; pe += P; pH += P;
+            nu = *pu; nv = *pv; ne = *pe; nH = *pH;
+        }
+        if (!FIRST && r >= t.ya && r < t.yb) {               // row r is emitted (u, η) next iteration
+            if (!LAST) { npu = A.pr_u[x + r * P]; npe = A.pr_e[x + r * P]; }
+            nau = A.acc_u[x + r * P];
+            nae = A.acc_e[x + r * P];
+        }
+        if (!FIRST && r - 1 >= t.ya && r - 1 < t.yb) {       // row r-1 of v is emitted next iteration
+            if (!LAST) npv = A.pr_v[x + (r - 1) * P];
+            nav = A.acc_v[x + (r - 1) * P];
+        }
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
         const R hx_0 = sh_xp(h_0);
@@ -208,15 +228,16 @@ __global__ void __launch_bounds__(256) rk_stage_kernel(StageArgs<R> A) {
         // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
         const R qxp_1 = sh_xp(q_1);
         R qxm_1 = sh_xm(q_1);
-        R qxm_2w = qxm_2;
         R qend_1 = 0;
         if (PER) {
             qend_1 = A.qend[r - 1];
-            if (qfix) { qxm_1 = qend_1; qxm_2w = qend_2; }
+            if (qfix) qxm_1 = qend_1;
         }
         const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
         const R qb_1 = k12 * (q_2 + q_1 + qxm_1);
-        const R qg_1 = k12 * (q_2 + qxm_2w + q_1);
+        // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
+        // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
+        const R qg_1 = (PER && qfix) ? k12 * (q_2 + q_1 + qend_2) : k12 * (q_2 + qxm_2 + q_1);
         const R qd_1 = k12 * (q_2 + q_1 + qxp_2);
 
         // du at row r-1: PV_ArakawaHsu! qhv  PV_advection.jl:73 ; momentum_u!  rhs.jl:242
@@ -226,8 +247,8 @@ __global__ void __launch_bounds__(256) rk_stage_kernel(StageArgs<R> A) {
         if (u_col && yu >= t.ya && yu < t.yb) {
             const R qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
             const R du = (qhv - (p_xp1 - p_1)) + A.Fxt * Fx_row[yu * A.st.Fx_sy];
-            const R base = FIRST ? u_1 : A.pr_u[x + yu * P];
-            const R acc = FIRST ? base : A.acc_u[x + yu * P];
+            const R base = FIRST ? u_1 : pu_1;
+            const R acc = FIRST ? base : au_1;
             if (!LAST) store_u<R, PER>(A.out_u, x, yu, base + A.b * du, nx, ny, P, A.oma);
             const R a0 = acc + A.a * du;
             if (LAST) store_u<R, PER>(A.acc_u, x, yu, a0, nx, ny, P, A.oma);
@@ -237,8 +258,8 @@ __global__ void __launch_bounds__(256) rk_stage_kernel(StageArgs<R> A) {
         const R Uxm_1 = sh_xm(U_1);
         if (e_col && yu >= t.ya && yu < t.yb) {
             const R deta = -((U_1 - Uxm_1) + (V_1 - V_2));
-            const R base = FIRST ? e_1 : A.pr_e[x + yu * P];
-            const R acc = FIRST ? base : A.acc_e[x + yu * P];
+            const R base = FIRST ? e_1 : pe_1;
+            const R acc = FIRST ? base : ae_1;
             if (!LAST) store_e<R, PER>(A.out_e, x, yu, base + A.b * deta, nx, ny, P);
             const R a0 = acc + A.a * deta;
             if (LAST) store_e<R, PER>(A.acc_e, x, yu, a0, nx, ny, P);
@@ -250,8 +271,8 @@ __global__ void __launch_bounds__(256) rk_stage_kernel(StageArgs<R> A) {
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
             const R qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
             const R dv = -(qhu + (p_1 - p_2)) + A.Fyt * Fy_row[yv * A.st.Fy_sy];
-            const R base = FIRST ? v_2 : A.pr_v[x + yv * P];
-            const R acc = FIRST ? base : A.acc_v[x + yv * P];
+            const R base = FIRST ? v_2 : pv_2;
+            const R acc = FIRST ? base : av_2;
             if (!LAST) store_v<R, PER>(A.out_v, x, yv, base + A.b * dv, nx, P, A.oma);
             const R a0 = acc + A.a * dv;
             if (LAST) store_v<R, PER>(A.acc_v, x, yv, a0, nx, P, A.oma);
@@ -307,14 +328,22 @@ __global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<R> A) {
     R S11_2 = 0, S12_3 = 0, S22_2 = 0;
 
     int r = t.ya - 2;
+    const int rlast = t.yb + 1;
     const R *pu = A.u0 + x + r * P, *pv = A.v0 + x + r * P, *pe = A.e0 + x + r * P, *pH = A.H + x + r * P;
-    for (; r <= t.yb + 1; ++r, pu += P, pv += P, pe += P, pH += P) {
-        const R u_0 = *pu, v_0 = *pv;
+    R nu = *pu, nv = *pv, ne = 0, nH = 0;              // row r+1 is loaded while row r is computed
+    if (DRAG == 0) { ne = *pe; nH = *pH; }
+    for (; r <= rlast; ++r) {
+        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
+        if (r < rlast) {
+            pu += P; pv += P;
+            nu = *pu; nv = *pv;
+            if (DRAG == 0) { pe += P; pH += P; ne = *pe; nH = *pH; }
+        }
         const R uxm_0 = sh_xm(u_0);
         R h_0 = 0, sk_0 = 0;
         if (DRAG == 0) {
             // bottom_drag_quadratic!  bottom_drag.jl:20-76 (thickness, speed, KEu, KEv, sqrtKE)
-            h_0 = *pe + *pH;
+            h_0 = e_0 + H_0;
             const R KEu_0 = half * (u_0 * u_0 + uxm_0 * uxm_0);
             const R KEv_0 = half * (v_0 * v_0 + v_1 * v_1);
             sk_0 = sqrt(KEu_0 + KEv_0);

@@ -1,0 +1,58 @@
+"""The C ABI library without a GPU: it loads, exports every entry point include/swm.h
+declares, and the product path never reaches the oracle."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "swm.h")
+PKG = os.path.join(ROOT, "shallowwaters.jl_amd")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*\*?(swm_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_abi():
+    syms = declared_symbols()
+    for s in ("swm_create", "swm_destroy", "swm_set_static", "swm_upload_prog", "swm_download_prog", "swm_step",
+              "swm_stages", "swm_download_rk", "swm_reduce", "swm_last_error", "swm_time_kernel"):
+        assert s in syms
+
+
+def test_library_loads_and_exports_every_declared_symbol(swm):
+    from shallowwaters_amd import _abi
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.fail("libswm_hip.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(_abi.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(_abi.EXPORTS) == declared_symbols()
+    v = ctypes.c_int32(0)
+    assert L.swm_version(ctypes.byref(v)) == 0 and v.value == 1
+
+
+def test_config_struct_matches_header(swm):
+    """The ctypes mirror of swm_config lists the header's fields in order."""
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct swm_config {"):text.index("} swm_config;")]
+    names = []
+    for line in body.splitlines()[1:]:
+        line = line.split("/*")[0].strip()
+        if not line or not (line.startswith("int") or line.startswith("double")):
+            continue
+        decl = line.split(None, 1)[1].rstrip(";")
+        for part in decl.split(","):
+            names.append(part.strip().split("[")[0])
+    assert [f[0] for f in swm.swm_config._fields_] == names
+
+
+def test_product_path_never_uses_the_oracle():
+    for dirpath, _, files in os.walk(PKG):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                text = open(os.path.join(dirpath, f), encoding="utf-8").read()
+                assert "oracle" not in text.lower(), os.path.join(dirpath, f)

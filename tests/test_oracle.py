@@ -1,0 +1,231 @@
+"""Pins of the CPU oracle (the restatement every GPU result is checked against).
+
+Julia is not available, so the reference cannot be run.  Its only value-level known answer
+is test/runtests.jl:4-9 (no forcing from rest stays exactly zero); everything else here is
+fixed by mathematics the reference's algorithm implies: closed forms of each stencil,
+telescoping mass conservation, the Coriolis balance of a uniform flow, exact tracer
+shifts, and the IEEE formats the mixed-precision paths emulate.  Each check is chosen so a
+dropped term, a wrong sign, a shifted index or a transposed operand fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+
+# ------------------------------------------------------------------ stencils (gradients.jl, interpolations.jl)
+
+def test_stencils_closed_forms(orc):
+    m, n = 9, 7
+    i = np.arange(1, m + 1, dtype=np.float64)[:, None]
+    j = np.arange(1, n + 1, dtype=np.float64)[None, :]
+    ramp = np.asfortranarray(3.0 * i - 5.0 * j + 0.0 * i * j)
+    assert np.all(orc.stencil("dx", ramp) == 3.0)
+    assert np.all(orc.stencil("dy", ramp) == -5.0)
+    # Ix/Iy of a ramp: the midpoint value
+    assert np.array_equal(orc.stencil("Ix", ramp), 3.0 * (i[:-1] + 0.5) - 5.0 * j)
+    assert np.array_equal(orc.stencil("Iy", ramp), 3.0 * i - 5.0 * (j[:, :-1] + 0.5))
+    # Ixy of a bilinear function is its value at the cell centre
+    f = np.asfortranarray(i * j)
+    assert np.array_equal(orc.stencil("Ixy", f), (i[:-1] + 0.5) * (j[:, :-1] + 0.5))
+    # ∇² (resilient form) of x² + y² is 4, of x·y is 0; the operator is asymmetric-safe
+    q = np.asfortranarray(i ** 2 + j ** 2)
+    assert np.all(orc.stencil("lap", q) == 4.0)
+    assert np.all(orc.stencil("lap", np.asfortranarray(i * j + 0 * q)) == 0.0)
+    # orientation: ∂x differentiates along the first (column-major fastest) index
+    g = np.asfortranarray(i ** 2 + 0 * j)
+    assert np.array_equal(orc.stencil("dx", g), (2 * i[:-1] + 1) + 0 * j)
+    assert np.all(orc.stencil("dy", g) == 0)
+
+
+# ------------------------------------------------------------------ emulated formats
+
+def test_binary16_matches_ieee(orc):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.standard_normal(4000) * 10.0 ** rng.integers(-9, 5, 4000),
+                        [65504.0, 65519.99, 65520.0, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8, -0.0, 0.0]])
+    assert np.array_equal(orc.round_fmt(2, x), x.astype(np.float16).astype(np.float64))
+    a = rng.standard_normal(4000).astype(np.float16).astype(np.float64) * 3
+    b = rng.standard_normal(4000).astype(np.float16).astype(np.float64) + 4
+    a16, b16 = a.astype(np.float16), b.astype(np.float16)
+    for op, ref in ((0, a16 + b16), (1, a16 - b16), (2, a16 * b16), (3, a16 / b16)):
+        assert np.array_equal(orc.arith(2, op, a, b), ref.astype(np.float64)), op
+    assert np.array_equal(orc.arith(2, 4, b, b), np.sqrt(b16).astype(np.float64))
+
+
+def _bf16_exact(x: float) -> float:
+    """Round-to-nearest-even of a float64 to bfloat16 with exact rational arithmetic."""
+    from fractions import Fraction
+    if x == 0:
+        return x
+    fx = Fraction(x)
+    e = math.floor(math.log2(abs(x)))
+    while Fraction(2) ** e > abs(fx):
+        e -= 1
+    while Fraction(2) ** (e + 1) <= abs(fx):
+        e += 1
+    q = Fraction(2) ** (max(e, -126) - 7)
+    k = fx / q
+    fl = math.floor(k)
+    r = k - fl
+    if r > Fraction(1, 2) or (r == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return float(fl * q)
+
+
+def test_bfloat16_rounding_is_correct(orc):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(3000) * 10.0 ** rng.integers(-30, 30, 3000)
+    # values on and next to the bf16 ties, where double rounding via float32 would fail
+    t = np.array([1.0 + 2.0 ** -8, 1.0 + 2.0 ** -8 + 2.0 ** -30, 1.0 + 3 * 2.0 ** -8, 3.0 + 2.0 ** -7 + 2.0 ** -40])
+    x = np.concatenate([x, t, -t])
+    got = orc.round_fmt(3, x)
+    want = np.array([_bf16_exact(float(v)) for v in x])
+    assert np.array_equal(got, want)
+
+
+def test_host_bfloat16_storage_matches_oracle(swm, orc):
+    from shallowwaters_amd.numtypes import to_storage, to_float64
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(5000) * 10.0 ** rng.integers(-20, 20, 5000)
+    assert np.array_equal(to_float64(to_storage(x, "BFloat16"), "BFloat16"), orc.round_fmt(3, x))
+    assert np.array_equal(to_float64(to_storage(x, "Float16"), "Float16"), orc.round_fmt(2, x))
+
+
+# ------------------------------------------------------------------ whole step
+
+def _run(swm, orc, P, prog=None, nsteps=1, **kw):
+    S = swm.make_setup(P)
+    if prog is None:
+        prog = swm.initial_conditions(P.Tprog, S)
+    return S, orc.run(S, swm.make_config(S), prog, nsteps, **kw)
+
+
+def test_known_answer_no_forcing_stays_zero(swm, orc):
+    """test/runtests.jl:4-9 (run_model(Ndays=1,Fx0=0), Float32, periodic 100x50)."""
+    P = swm.Parameter(Ndays=1, Fx0=0)
+    S, r = _run(swm, orc, P, nsteps=swm.make_setup(P).grid.nt)
+    assert r[5] == S.grid.nt
+    for a in r[:3]:
+        assert np.all(a == 0)
+
+
+@pytest.mark.parametrize("bc,T", [("nonperiodic", "Float64"), ("periodic", "Float64"), ("nonperiodic", "Float32")])
+def test_rest_state_tendencies_are_the_wind(swm, orc, bc, T):
+    """From rest every tendency but the wind vanishes: du = Fx at every interior u point
+    (momentum_u!, rhs.jl:242), dv = Fy, dη = 0 -- pins the forcing's index mapping."""
+    P = swm.Parameter(T=T, nx=23, L_ratio=0.7, bc=bc, wind_forcing_x="double_gyre", topography="seamount",
+                      Fy0=0.3, wind_forcing_y="constant")
+    S, r = _run(swm, orc, P, stop_stage=1, want_rk=True)
+    du, dv, de = r[4][6:9]
+    F = S.forcing
+    assert np.array_equal(du[2:-2, 2:-2], F.Fx) and np.all(du[:2] == 0) and np.all(du[:, -2:] == 0)
+    assert np.array_equal(dv[2:-2, 2:-2], F.Fy) and np.any(F.Fy != 0)
+    assert np.all(de == 0)
+
+
+def test_uniform_zonal_flow_coriolis_balance(swm, orc):
+    """u = U0, v = η = 0, flat bottom, f constant (ϕ = 90°), no wind: the Arakawa-Hsu
+    weights sum to q, so dv = -f·U0·Δ (Coriolis), du = 0 and dη = 0 exactly."""
+    P = swm.Parameter(T="Float64", nx=24, L_ratio=1, bc="periodic", ϕ=90.0, topography="flat",
+                      wind_forcing_x="none", wind_forcing_y="none", Fx0=0.0)
+    S = swm.make_setup(P)
+    G = S.grid
+    assert G.β == 0.0
+    U0 = 0.37
+    u = np.full((G.nux, G.nuy), U0)
+    prog = swm.PrognosticVars(*swm.add_halo(u, np.zeros((G.nvx, G.nvy)), np.zeros((G.nx, G.ny)),
+                                            np.zeros((G.nx, G.ny)), S, T="Float64"), T="Float64")
+    r = orc.run(S, swm.make_config(S), prog, 1, stop_stage=1, want_rk=True)
+    du, dv, de = r[4][6:9]
+    assert np.all(du == 0) and np.all(de == 0)
+    # rows next to the walls of the periodic channel see the never-written u y-halo (0)
+    inner = dv[2:-2, 3:-3]
+    want = -G.f0 * G.Δ * U0
+    assert np.allclose(inner, want, rtol=1e-14, atol=0)
+    assert rel_l2(np.full_like(inner, want), inner) < 1e-15
+
+
+@pytest.mark.parametrize("bc,topo", [("nonperiodic", "seamount"), ("periodic", "flat")])
+def test_mass_is_conserved(swm, orc, bc, topo):
+    """continuity! telescopes: Σ dη over the interior is -(boundary fluxes), zero through
+    the never-written normal-velocity halo (nonperiodic) or cancelling through the periodic
+    copies (flat bottom).  Dissipation does not touch η.  Drift stays at rounding level."""
+    P = swm.Parameter(T="Float64", nx=40, L_ratio=1.25, bc=bc, topography=topo, wind_forcing_x="double_gyre",
+                      diffusion="Smagorinsky", α=0.5, Uadv=10.0)
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=5, σ=0.5)
+    m0 = prog.η[1:-1, 1:-1].sum()
+    r = orc.run(S, swm.make_config(S), prog, 60)
+    m1 = r[2][1:-1, 1:-1].sum()
+    scale = np.abs(r[2][1:-1, 1:-1]).sum()
+    assert abs(m1 - m0) < 1e-12 * scale, (m0, m1, scale)
+    # and the flow evolved
+    assert rel_l2(prog.η, r[2]) > 1e-2
+
+
+# ------------------------------------------------------------------ tracer (tracer_advection.jl)
+
+def _tracer_setup(swm, bc):
+    # c = sqrt(g H) = 64 m/s, Δ = 4096 m -> dt = 64 s; Uadv = 16 -> nadvstep = 4,
+    # dtadvu = 256/4096 = 1/16: a uniform u = 16 m/s moves the tracer exactly one cell
+    P = swm.Parameter(T="Float64", nx=16, Lx=16 * 4096.0, L_ratio=2, g=8.192, H=500.0, bc=bc, Uadv=16.0)
+    S = swm.make_setup(P)
+    G = S.grid
+    assert (G.dtint, G.nadvstep, G.dtadvu, G.half_dtadvu) == (64, 4, 1 / 16, 1 / 32)
+    return P, S, G
+
+
+@pytest.mark.parametrize("bc", ["periodic", "nonperiodic"])
+def test_tracer_uniform_flow_is_an_exact_shift(swm, orc, bc):
+    P, S, G = _tracer_setup(swm, bc)
+    rng = np.random.default_rng(7)
+    sst0 = rng.random((G.nx, G.ny))
+    u = np.full((G.nux, G.nuy), 16.0)
+    U, V, E, SS = swm.add_halo(u, np.zeros((G.nvx, G.nvy)), np.zeros((G.nx, G.ny)), sst0, S, T="Float64")
+    cfg = swm.make_config(S)
+    um, vm, sst = orc.tracer(cfg, 2, U, V, np.zeros_like(U), np.zeros_like(V), SS)   # snapshot step
+    assert np.array_equal(um, U) and np.array_equal(sst, SS)
+    um, vm, sst = orc.tracer(cfg, 4, U, V, um, vm, sst)                                 # advection step
+    if bc == "periodic":
+        assert np.array_equal(sst[1:-1, :], np.roll(sst0, 1, axis=0))
+    else:   # walls: the face velocity is 0, so only columns 3..nx-1 see a uniform flow
+        assert np.array_equal(sst[3:-2, :], sst0[1:-2, :])
+    # zero velocity leaves the tracer bitwise unchanged (floor(-0.0) = 0, x0 = -0.0)
+    Z = np.zeros_like(U)
+    _, _, s2 = orc.tracer(cfg, 4, Z, np.zeros_like(V), Z, np.zeros_like(V), SS)
+    assert np.array_equal(s2, SS)
+
+
+def test_tracer_linear_field_reproduced_by_bilinear(swm, orc):
+    """A tracer linear in x and y is reproduced exactly along a uniform shift of half a cell
+    in both directions (bilin of a linear function is exact)."""
+    P, S, G = _tracer_setup(swm, "periodic")
+    x = np.arange(G.nx, dtype=np.float64)[:, None]
+    y = np.arange(G.ny, dtype=np.float64)[None, :]
+    sst0 = np.asfortranarray(0.25 * x + 0.5 * y)
+    u = np.full((G.nux, G.nuy), 8.0)          # half a cell per advection step
+    v = np.full((G.nvx, G.nvy), 8.0 * G.dtadvu / G.dtadvv)
+    U, V, E, SS = swm.add_halo(u, v, np.zeros((G.nx, G.ny)), sst0, S, T="Float64")
+    cfg = swm.make_config(S)
+    um, vm, _ = orc.tracer(cfg, 2, U, V, np.zeros_like(U), np.zeros_like(V), SS)
+    _, _, sst = orc.tracer(cfg, 4, U, V, um, vm, SS)
+    # interior columns away from the periodic seam; rows away from the clipped walls
+    got = sst[3:-3, 3:-3]
+    want = sst0[2:-2, 3:-3] - 0.25 * 0.5 - 0.5 * 0.5
+    assert np.allclose(got, want, atol=1e-13)
+
+
+# ------------------------------------------------------------------ committed fixtures (regression pin)
+
+def test_golden_fixtures(swm, orc):
+    import os
+    from golden.make_fixtures import CASES, run_case
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name in CASES:
+        ref = np.load(os.path.join(here, f"{name}.npz"))
+        got = run_case(swm, orc, name)
+        for k in ("u", "v", "eta", "sst"):
+            assert np.array_equal(ref[k], got[k]), (name, k)
