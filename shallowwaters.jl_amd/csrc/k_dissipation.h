@@ -1,0 +1,114 @@
+// k_dissipation.h -- bottom drag + biharmonic diffusion, once per step (gfx950).
+#pragma once
+
+#include "swm_common.h"
+
+namespace swm {
+
+template <class R> struct DissArgs {
+    const R *u0, *v0, *e0;            // state after the RK stages (ghosts applied)
+    R *out_u, *out_v;                 // u0 + Δt_diff*(drag + diffusion), with ghosts
+    const R *H;
+    R cD, rD, cS, nuB, dtd, oma;
+    int nx, ny, P, L, nstrips, nseg;
+};
+
+// time_integration.jl:216-221:
+//   bottom_drag!   bottom_drag.jl:2-119 (quadratic, linear or none)
+//   diffusion!     diffusion.jl:2-64 (Smagorinsky or constant biharmonic)
+//   add_drag_diff_tendencies!  diffusion.jl:221-251
+//   ghost_points_uv!           ghost_points.jl:202-216
+// At input row r the wave forms the row r quantities (KE, sqrtKE, DT), the Laplacians and
+// DS of row r-1, ν and the viscous tensor of rows r-1/r-2, and emits u and v at row r-2.
+template <class R, bool PER, int DRAG, bool SMAG>
+__global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<R> A) {
+    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    if (!t.live) return;
+    const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
+    const bool lane_ok = t.lane >= HALO && t.lane < WAVE - HALO;
+    const bool u_col = lane_ok && (PER ? (x >= 0 && x < nx) : (x >= 1 && x <= nx - 1));
+    const bool v_col = lane_ok && x >= 1 && x <= nx;
+    const R half = R(0.5), quarter = R(0.25);
+
+    R u_1 = 0, u_2 = 0, uxm_1 = 0, v_1 = 0, v_2 = 0, h_1 = 0, h_2 = 0, sk_1 = 0, sk_2 = 0;
+    R DT_1 = 0, DSq_2 = 0, DSqxm_2 = 0, nu_2 = 0, nuxp_2 = 0, Lu_2 = 0, Lv_2 = 0, Lvxp_2 = 0;
+    R S11_2 = 0, S12_3 = 0, S22_2 = 0;
+
+    int r = t.ya - 2;
+    const int rlast = t.yb + 1;
+    const R *pu = A.u0 + x + r * P, *pv = A.v0 + x + r * P, *pe = A.e0 + x + r * P, *pH = A.H + x + r * P;
+    R nu = *pu, nv = *pv, ne = 0, nH = 0;              // row r+1 is loaded while row r is computed
+    if (DRAG == 0) { ne = *pe; nH = *pH; }
+    for (; r <= rlast; ++r) {
+        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
+        if (r < rlast) {
+            pu += P; pv += P;
+            nu = *pu; nv = *pv;
+            if (DRAG == 0) { pe += P; pH += P; ne = *pe; nH = *pH; }
+        }
+        const R uxm_0 = sh_xm(u_0);
+        R h_0 = 0, sk_0 = 0;
+        if (DRAG == 0) {
+            // bottom_drag_quadratic!  bottom_drag.jl:20-76 (thickness, speed, KEu, KEv, sqrtKE)
+            h_0 = e_0 + H_0;
+            const R KEu_0 = half * (u_0 * u_0 + uxm_0 * uxm_0);
+            const R KEv_0 = half * (v_0 * v_0 + v_1 * v_1);
+            sk_0 = sqrt(KEu_0 + KEv_0);
+        }
+        // stress_tensor!  diffusion.jl:114-122: ∇²!  gradients.jl:29-42 at row r-1
+        const R uxp_1 = sh_xp(u_1);
+        const R Lu_1 = ((uxm_1 - u_1) + (uxp_1 - u_1)) + ((u_2 - u_1) + (u_0 - u_1));
+        const R vxp_1 = sh_xp(v_1), vxm_1 = sh_xm(v_1);
+        const R Lv_1 = ((vxm_1 - v_1) + (vxp_1 - v_1)) + ((v_2 - v_1) + (v_0 - v_1));
+        R DT_0 = 0, nu_1, nuq_2, DSq_1 = 0, DSqxm_1 = 0;
+        if (SMAG) {
+            // smagorinsky_coeff!  diffusion.jl:68-111 (tension uses +, :95)
+            const R dudx_0 = u_0 - uxm_0, dvdy_0 = v_0 - v_1;
+            const R s0 = dudx_0 + dvdy_0;
+            DT_0 = s0 * s0;
+            const R s1 = (u_0 - u_1) + (vxp_1 - v_1);
+            DSq_1 = s1 * s1;
+            DSqxm_1 = sh_xm(DSq_1);
+            const R DS_1 = quarter * (DSqxm_2 + DSq_2) + quarter * (DSqxm_1 + DSq_1);
+            nu_1 = A.cS * sqrt(DS_1 + DT_1);
+            const R nuxp_1 = sh_xp(nu_1);
+            nuq_2 = quarter * (nu_2 + nuxp_2) + quarter * (nu_1 + nuxp_1);
+            nuxp_2 = nuxp_1;
+        } else {
+            nu_1 = A.nuB;
+            nuq_2 = A.nuB;
+        }
+        // viscous_tensor_*!  diffusion.jl:126-218
+        const R S11_1 = nu_1 * (Lu_1 - sh_xm(Lu_1));
+        const R S12_2 = nuq_2 * (Lu_1 - Lu_2);
+        const R Lvxp_1 = sh_xp(Lv_1);
+        const R S21_2 = nuq_2 * (Lvxp_2 - Lv_2);
+        const R S22_1 = nu_1 * (Lv_1 - Lv_2);
+        // ∂x!(LLu1,S11), ∂y!(LLu2,S12), ∂x!(LLv1,S21), ∂y!(LLv2,S22)  diffusion.jl:60-63 at row r-2
+        const R LLu1 = sh_xp(S11_2) - S11_2;
+        const R LLu2 = S12_2 - S12_3;
+        const R LLv1 = S21_2 - sh_xm(S21_2);
+        const R LLv2 = S22_1 - S22_2;
+        R Bu = 0, Bv = 0;
+        if (DRAG == 0) {
+            const R sku = half * (sh_xp(sk_2) + sk_2), hu = half * (sh_xp(h_2) + h_2);
+            Bu = A.cD * sku * u_2 / hu;
+            const R skv = half * (sk_1 + sk_2), hv = half * (h_1 + h_2);
+            Bv = A.cD * skv * v_2 / hv;
+        } else if (DRAG == 1) {
+            Bu = A.rD * u_2;
+            Bv = A.rD * v_2;
+        }
+        const int y = r - 2;
+        if (u_col && y >= t.ya && y < t.yb)
+            store_u<R, PER>(A.out_u, x, y, u_2 + A.dtd * (Bu + LLu1 + LLu2), nx, ny, P, A.oma);
+        if (v_col && y >= t.ya && y < t.yb && y <= ny - 1)
+            store_v<R, PER>(A.out_v, x, y, v_2 + A.dtd * (Bv + LLv1 + LLv2), nx, P, A.oma);
+        // advance
+        u_2 = u_1; u_1 = u_0; uxm_1 = uxm_0; v_2 = v_1; v_1 = v_0; h_2 = h_1; h_1 = h_0; sk_2 = sk_1; sk_1 = sk_0;
+        DT_1 = DT_0; DSq_2 = DSq_1; DSqxm_2 = DSqxm_1; nu_2 = nu_1; Lu_2 = Lu_1; Lv_2 = Lv_1; Lvxp_2 = Lvxp_1;
+        S11_2 = S11_1; S12_3 = S12_2; S22_2 = S22_1;
+    }
+}
+
+}  // namespace swm

@@ -1,0 +1,264 @@
+// k_stage.h -- right-hand side kernels of one Runge-Kutta stage (gfx950).
+#pragma once
+
+#include "swm_common.h"
+
+namespace swm {
+
+// Stage combinations of time_integration.jl:53-196, applied pointwise to the tendency d:
+//   RK_FIRST   o1 = w + b·d (u1),  o2 = w + a·d (u0)            caxb!/axb!, stage 1 (u0 == u)
+//   RK_MID     o1 = pr + b·d,      o2 = ac + a·d                 caxb!(u1,u,...), axb!(u0,...)
+//   RK_LAST    o2 = ac + a·d  (+ ghosts of u0, :198)
+//   SSP2       o1 = w + c·d                                      axb!(u1,Δt_Δs,du)  :95
+//   SSP2_LAST  o1 = w + c·d,  o2 = x·pr + y·o1                   cxayb!(u0,a,u,b,u1) :108
+//   SSP3       o1 = w + c·d                                      axb!(u1,Δt_Δn,du)  :132
+//   SSP3_KN    o1 = x·w + y·ac + z·d                             dxaybzc!  :129
+//   S4         t = w + c·d,  o1 = h·(w + t)                      caxb!(u0,u1,Δt_Δ,du), cxab!  :172-175
+//   S4_3       ... and o2 = x·pr + y·o1                          cxayb!(u1,2/3,u,1/3,u1)  :187
+// w is the stage input at the point, pr the Prog value, ac the u0 value.
+enum { M_RK_FIRST = 0, M_RK_MID, M_RK_LAST, M_SSP2, M_SSP2_LAST, M_SSP3, M_SSP3_KN, M_S4, M_S4_3 };
+
+template <int M> struct Mode {
+    static constexpr bool rk = M <= M_RK_LAST;   // rhs! and continuity! in one kernel
+    static constexpr bool first = M == M_RK_FIRST;
+    static constexpr bool need_pr = M == M_RK_MID || M == M_SSP2_LAST || M == M_S4_3;
+    static constexpr bool need_ac = M == M_RK_MID || M == M_RK_LAST || M == M_SSP3_KN;
+    static constexpr bool out1 = M != M_RK_LAST;
+    static constexpr bool out2 = M == M_RK_FIRST || M == M_RK_MID || M == M_RK_LAST || M == M_SSP2_LAST || M == M_S4_3;
+    static constexpr bool ghost2 = M != M_RK_FIRST && M != M_RK_MID;    // u0 between RK stages: interior only
+};
+
+template <class R> struct Coef {
+    R a, b, c, x, y, z, h;
+};
+
+template <int M, class R>
+__device__ __forceinline__ void combine(R w, R pr, R ac, R d, const Coef<R> &k, R &o1, R &o2) {
+    if constexpr (M == M_RK_FIRST) { o1 = w + k.b * d; o2 = w + k.a * d; }
+    else if constexpr (M == M_RK_MID) { o1 = pr + k.b * d; o2 = ac + k.a * d; }
+    else if constexpr (M == M_RK_LAST) { o2 = ac + k.a * d; }
+    else if constexpr (M == M_SSP2 || M == M_SSP3) { o1 = w + k.c * d; }
+    else if constexpr (M == M_SSP2_LAST) { o1 = w + k.c * d; o2 = k.x * pr + k.y * o1; }
+    else if constexpr (M == M_SSP3_KN) { o1 = k.x * w + k.y * ac + k.z * d; }
+    else if constexpr (M == M_S4) { const R t = w + k.c * d; o1 = k.h * (w + t); }
+    else { const R t = w + k.c * d; o1 = k.h * (w + t); o2 = k.x * pr + k.y * o1; }
+}
+
+template <class R> struct StageArgs {
+    const R *in_u, *in_v, *in_e;      // stage input (u1 of the reference; Prog at stage 1)
+    const R *pr_u, *pr_v, *pr_e;      // Prog
+    const R *ac_u, *ac_v, *ac_e;      // u0
+    R *o1_u, *o1_v, *o1_e;            // first output (u1 / u1')
+    R *o2_u, *o2_v, *o2_e;            // second output (u0 / u1'')
+    const R *qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap)
+    Static<R> st;
+    Coef<R> k;
+    R Fxt, Fyt, Fet, g, oma, gamma;
+    int cont;
+    int nx, ny, P, L, nstrips, nseg;
+};
+
+// ------------------------------------------------------------------ stage kernel (K1)
+//
+// rhs_nonlinear!  rhs.jl:24-51 (UVfluxes!, advection_coriolis!, PV_ArakawaHsu!, bernoulli!,
+// ∂x/∂y of p, momentum_u!/momentum_v!), and for the classic RK scheme also continuity!
+// (continuity.jl:72-98) on the same stage input, followed by the stage combination and the
+// ghost copies of what the next stage reads.  At input row r the wave forms U(r), then the
+// row-(r-1) quantities h_v, V, h_q, q, KE, p and the Arakawa-Hsu weights, and emits du (and
+// dη) at row r-1 and dv at row r-2.
+template <class R, bool PER, int M>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
+    using MI = Mode<M>;
+    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    if (!t.live) return;
+    const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
+    const bool lane_ok = t.lane >= HALO && t.lane < WAVE - HALO;
+    const bool u_col = lane_ok && (PER ? (x >= 0 && x < nx) : (x >= 1 && x <= nx - 1));
+    const bool v_col = lane_ok && x >= 1 && x <= nx;
+    const bool e_col = v_col;
+    const bool qfix = PER && x == 0;
+
+    const R half = R(0.5), quarter = R(0.25), k12 = R(1.0 / 12.0);
+    const R *fq_row = A.st.fq + x * A.st.fq_sx;
+    const R *Fx_row = A.st.Fx + x * A.st.Fx_sx;
+    const R *Fy_row = A.st.Fy + x * A.st.Fy_sx;
+
+    // row history (suffix _1 = r-1, _2 = r-2)
+    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
+    R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
+    R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
+
+    // Loads run one row ahead of their use: the stencil inputs of row r+1 and the
+    // pointwise operands (Prog, u0) of the rows emitted next iteration are in flight
+    // while row r is computed.
+    int r = t.ya - 1;
+    const int rlast = t.yb + 1;
+    const R *pu = A.in_u + x + r * P, *pv = A.in_v + x + r * P, *pe = A.in_e + x + r * P;
+    const R *pH = A.st.H + x + r * P;
+    R nu = *pu, nv = *pv, ne = *pe, nH = *pH;
+    R npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
+    for (; r <= rlast; ++r) {
+        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
+        const R pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+        if (r < rlast) {
+            pu += P; pv += P; pe += P; pH += P;
+            nu = *pu; nv = *pv; ne = *pe; nH = *pH;
+        }
+        if (r >= t.ya && r < t.yb) {                          // row r of u, η is emitted next iteration
+            if (MI::need_pr) { npu = A.pr_u[x + r * P]; if (MI::rk) npe = A.pr_e[x + r * P]; }
+            if (MI::need_ac) { nau = A.ac_u[x + r * P]; if (MI::rk) nae = A.ac_e[x + r * P]; }
+        }
+        if (r - 1 >= t.ya && r - 1 < t.yb) {                  // row r-1 of v is emitted next iteration
+            if (MI::need_pr) npv = A.pr_v[x + (r - 1) * P];
+            if (MI::need_ac) nav = A.ac_v[x + (r - 1) * P];
+        }
+        // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
+        const R h_0 = e_0 + H_0;
+        const R hx_0 = sh_xp(h_0);
+        const R U_0 = u_0 * (half * (hx_0 + h_0));
+        // row r-1: Iy!(h_v), Vflux!  rhs.jl:294-304
+        const R V_1 = v_1 * (half * (h_0 + h_1));
+        // Ixy!(h_q,h)  interpolations.jl:33-45
+        const R hq_1 = quarter * (h_1 + hx_1) + quarter * (h_0 + hx_0);
+        // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
+        const R dvdx_1 = sh_xp(v_1) - v_1;
+        const R dudy_1 = u_0 - u_1;
+        const R q_1 = (fq_row[(r - 1) * A.st.fq_sy] + dvdx_1 - dudy_1) / hq_1;
+        // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
+        const R uu_1 = u_1 * u_1;
+        const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
+        const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
+        const R p_1 = half * (KEu_1 + KEv_1) + A.g * e_1;
+        // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
+        const R qxp_1 = sh_xp(q_1);
+        R qxm_1 = sh_xm(q_1);
+        R qend_1 = 0;
+        if (PER) {
+            qend_1 = A.qend[r - 1];
+            if (qfix) qxm_1 = qend_1;
+        }
+        const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
+        const R qb_1 = k12 * (q_2 + q_1 + qxm_1);
+        // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
+        // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
+        const R qg_1 = (PER && qfix) ? k12 * (q_2 + q_1 + qend_2) : k12 * (q_2 + qxm_2 + q_1);
+        const R qd_1 = k12 * (q_2 + q_1 + qxp_2);
+
+        // du at row r-1: PV_ArakawaHsu! qhv  PV_advection.jl:73 ; momentum_u!  rhs.jl:242
+        const R Vxp_1 = sh_xp(V_1);
+        const R p_xp1 = sh_xp(p_1);
+        const int yu = r - 1;
+        if (u_col && yu >= t.ya && yu < t.yb) {
+            const R qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
+            const R du = (qhv - (p_xp1 - p_1)) + A.Fxt * Fx_row[yu * A.st.Fx_sy];
+            R o1 = 0, o2 = 0;
+            combine<M>(u_1, MI::first ? u_1 : pu_1, au_1, du, A.k, o1, o2);
+            if (MI::out1) store_u<R, PER>(A.o1_u, x, yu, o1, nx, ny, P, A.oma);
+            if (MI::out2) {
+                if (MI::ghost2) store_u<R, PER>(A.o2_u, x, yu, o2, nx, ny, P, A.oma);
+                else A.o2_u[x + yu * P] = o2;
+            }
+        }
+        // dη at row r-1: ∂x!(dUdx,U), ∂y!(dVdy,V), continuity!  continuity.jl:52-98
+        const R Uxm_1 = sh_xm(U_1);
+        if (MI::rk && e_col && yu >= t.ya && yu < t.yb) {
+            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, A.st, x, yu, P, A.gamma, A.Fet);
+            R o1 = 0, o2 = 0;
+            combine<M>(e_1, MI::first ? e_1 : pe_1, ae_1, deta, A.k, o1, o2);
+            if (MI::out1) store_e<R, PER>(A.o1_e, x, yu, o1, nx, ny, P);
+            if (MI::out2) {
+                if (MI::ghost2) store_e<R, PER>(A.o2_e, x, yu, o2, nx, ny, P);
+                else A.o2_e[x + yu * P] = o2;
+            }
+        }
+        // dv at row r-2: PV_ArakawaHsu! qhu  PV_advection.jl:81 ; momentum_v!  rhs.jl:271
+        const R qdxm_1 = sh_xm(qd_1);
+        const int yv = r - 2;
+        if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
+            const R qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
+            const R dv = -(qhu + (p_1 - p_2)) + A.Fyt * Fy_row[yv * A.st.Fy_sy];
+            R o1 = 0, o2 = 0;
+            combine<M>(v_2, MI::first ? v_2 : pv_2, av_2, dv, A.k, o1, o2);
+            if (MI::out1) store_v<R, PER>(A.o1_v, x, yv, o1, nx, P, A.oma);
+            if (MI::out2) {
+                if (MI::ghost2) store_v<R, PER>(A.o2_v, x, yv, o2, nx, P, A.oma);
+                else A.o2_v[x + yv * P] = o2;
+            }
+        }
+        // advance the row history
+        const R qaxm_1 = sh_xm(qa_1);
+        u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hx_1 = hx_0;
+        U_2 = U_1; U_1 = U_0; Uxm_2 = Uxm_1; V_2 = V_1; Vxp_2 = Vxp_1;
+        q_2 = q_1; qxp_2 = qxp_1; qxm_2 = qxm_1; qaxm_2 = qaxm_1; qb_2 = qb_1; p_2 = p_1;
+        qend_2 = qend_1;
+    }
+}
+
+// ------------------------------------------------------------------ semi-implicit continuity
+//
+// continuity! of the SSPRK schemes (continuity.jl:72-98 with UVfluxes! recomputed from the
+// updated u1, v1 and the stage's η1, :83-85), followed by the η combination of the scheme
+// (time_integration.jl:103,143-146,183-189) and the η ghost copies.  Emits row r-1.
+template <class R, bool PER, int M>
+__global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
+    using MI = Mode<M>;
+    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    if (!t.live) return;
+    const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
+    const bool e_col = t.lane >= HALO && t.lane < WAVE - HALO && x >= 1 && x <= nx;
+    const R half = R(0.5);
+
+    R v_1 = 0, e_1 = 0, h_1 = 0, U_1 = 0, V_2 = 0;
+    int r = t.ya - 1;
+    const int rlast = t.yb;
+    const R *pu = A.in_u + x + r * P, *pv = A.in_v + x + r * P, *pe = A.in_e + x + r * P;
+    const R *pH = A.st.H + x + r * P;
+    R nu = *pu, nv = *pv, ne = *pe, nH = *pH, npe = 0, nae = 0;
+    for (; r <= rlast; ++r) {
+        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH, pe_1 = npe, ae_1 = nae;
+        if (r < rlast) {
+            pu += P; pv += P; pe += P; pH += P;
+            nu = *pu; nv = *pv; ne = *pe; nH = *pH;
+        }
+        if (r >= t.ya && r < t.yb) {
+            if (MI::need_pr) npe = A.pr_e[x + r * P];
+            if (MI::need_ac) nae = A.ac_e[x + r * P];
+        }
+        // UVfluxes!  rhs.jl:307-324 from the updated velocities
+        const R h_0 = e_0 + H_0;
+        const R U_0 = u_0 * (half * (sh_xp(h_0) + h_0));
+        const R V_1 = v_1 * (half * (h_0 + h_1));
+        const R Uxm_1 = sh_xm(U_1);
+        const int y = r - 1;
+        if (e_col && y >= t.ya && y < t.yb) {
+            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, A.st, x, y, P, A.gamma, A.Fet);
+            R o1 = 0, o2 = 0;
+            combine<M>(e_1, pe_1, ae_1, deta, A.k, o1, o2);
+            if (M == M_S4_3) store_e<R, PER>(A.o1_e, x, y, o2, nx, ny, P);    // only u1'' = 2/3 η + 1/3 η1' lives on
+            else {
+                store_e<R, PER>(A.o1_e, x, y, o1, nx, ny, P);
+                if (MI::out2) store_e<R, PER>(A.o2_e, x, y, o2, nx, ny, P);
+            }
+        }
+        v_1 = v_0; e_1 = e_0; h_1 = h_0; U_1 = U_0; V_2 = V_1;
+    }
+}
+
+// q at the corner column x = nx for every corner row y = 0..ny (PV!, PV_advection.jl:15):
+// the periodic AHβ!/AHγ! read q[end,·] for their first column (PV_advection.jl:110,126).
+template <class R>
+__global__ void qend_kernel(const R *u, const R *v, const R *e, const R *H, const R *fq, int fq_sx, int fq_sy,
+                            R *qend, int nx, int ny, int P) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y > ny) return;
+    const R quarter = R(0.25);
+    const int x = nx;
+    const R h00 = e[x + y * P] + H[x + y * P], h10 = e[x + 1 + y * P] + H[x + 1 + y * P];
+    const R h01 = e[x + (y + 1) * P] + H[x + (y + 1) * P], h11 = e[x + 1 + (y + 1) * P] + H[x + 1 + (y + 1) * P];
+    const R hq = quarter * (h00 + h10) + quarter * (h01 + h11);
+    const R dvdx = v[x + 1 + y * P] - v[x + y * P];
+    const R dudy = u[x + (y + 1) * P] - u[x + y * P];
+    qend[y] = (fq[x * fq_sx + y * fq_sy] + dvdx - dudy) / hq;
+}
+
+}  // namespace swm

@@ -2,38 +2,44 @@
 //
 // The driver replays the loop of time_integration.jl:45-243 as kernel launches on one HIP
 // stream.  The reference's arrays u, u0, u1 (and v, η alike) become four device slots per
-// field whose roles rotate instead of being copied (copyto! at :49-51 and :240-242):
-//   role 0  Prog        state at the start of the step
-//   role 1  u0          RK accumulator (axb!), then dissipation input
-//   role 2/3  u1        stage inputs, ping-pong (a stencil cannot update in place)
-// Dissipation writes the new Prog into a free slot, so the previous Prog stays intact until
-// the step is accepted: when NaNs are detected the loop stops before the copy-back exactly
-// as the reference does (time_integration.jl:235-237).
+// field.  Slots take roles instead of being copied (copyto! at :49-51, :151-154, :191-193,
+// :240-242 become a change of role): each stage writes into a slot that holds neither Prog
+// nor a value still to be read.  Dissipation writes the new Prog into a free slot, so the
+// previous Prog stays intact until the step is accepted: when NaNs are detected the loop
+// stops before the copy-back exactly as the reference does (time_integration.jl:235-237).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
 #include "../../include/swm.h"
-#include "swm_device.h"
+#include "k_dissipation.h"
+#include "k_misc.h"
+#include "k_stage.h"
 
 using namespace swm;
 
-#define HIPCHK(expr)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
+#define HIPCHK(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
         if (e_ != hipSuccess) return ctx->fail(SWM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
 namespace {
 
-struct Field {            // one padded device array
+struct Field {             // one padded device array
     void *base = nullptr;  // allocation
     char *p0 = nullptr;    // element (0,0)
+};
+
+struct Slots {             // where u0 / u1 live after a run of stages
+    int uv = -1, e = -1;
 };
 
 }  // namespace
@@ -50,11 +56,10 @@ struct swm_ctx {
     int L = 64;            // rows per wave segment
     Field u[4], v[4], e[4], sst[2], um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0;
-    int ruv[4] = {0, 1, 2, 3};   // slot of role k for u and v
-    int re[4] = {0, 1, 2, 3};    // slot of role k for η
+    int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
     int rsst = 0;
-    int rk_u0 = 1, rk_u1 = 2, rk_e0 = 1, rk_e1 = 2;  // slots of u0/u1 after swm_stages
-    int *dflags = nullptr;       // [0] NaN flag, [1] tracer index error
+    Slots rk0, rk1;        // u0 / u1 after swm_stages
+    int *dflags = nullptr; // [0] NaN flag, [1] tracer index error
     double *dpart = nullptr;
     bool have_static = false, have_prog = false;
     std::vector<void *> allocs;
@@ -73,8 +78,18 @@ struct swm_ctx {
 
 namespace {
 
-int alloc_field(swm_ctx *ctx, Field &f, size_t bytes_elems, bool two_d) {
-    const size_t n = two_d ? ctx->elems : bytes_elems;
+// the lowest slot not in `busy` (four slots always leave one free: at most three are live)
+int pick(std::initializer_list<int> busy) {
+    for (int s = 0; s < 4; ++s) {
+        bool used = false;
+        for (int b : busy) used |= b == s;
+        if (!used) return s;
+    }
+    return -1;
+}
+
+int alloc_field(swm_ctx *ctx, Field &f, size_t elems_1d, bool two_d) {
+    const size_t n = two_d ? ctx->elems : elems_1d;
     void *p = nullptr;
     HIPCHK(hipMalloc(&p, n * ctx->es));
     HIPCHK(hipMemsetAsync(p, 0, n * ctx->es, ctx->stream));
@@ -131,6 +146,7 @@ inline int nblocks_waves(int nwaves) { return (nwaves + WAVES_PER_BLOCK - 1) / W
 template <class R> struct Launch {
     static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
     static int nseg(const swm_ctx *ctx) { return (ctx->c.ny + ctx->L - 1) / ctx->L; }
+    static R *uf(swm_ctx *ctx, Field *f, int s) { return s < 0 ? nullptr : ctx->at<R>(f[s]); }
 
     static Static<R> statics(const swm_ctx *ctx) {
         Static<R> s;
@@ -138,55 +154,158 @@ template <class R> struct Launch {
         s.Fx = ctx->at<R>(ctx->Fx); s.Fx_sx = ctx->Fx_sx; s.Fx_sy = ctx->Fx_sy;
         s.Fy = ctx->at<R>(ctx->Fy); s.Fy_sx = ctx->Fy_sx; s.Fy_sy = ctx->Fy_sy;
         s.fq = ctx->at<R>(ctx->fq); s.fq_sx = ctx->fq_sx; s.fq_sy = ctx->fq_sy;
+        s.eta_ref = ctx->at<R>(ctx->eta_ref);
+        s.Feta = ctx->at<R>(ctx->Feta);
         return s;
     }
 
-    // stage `k` (1-based) of the classic RK scheme (time_integration.jl:55-78)
-    static int rk_stage(swm_ctx *ctx, int k, int in_u, int in_e, int out_u, int out_e, R Fxt, R Fyt) {
+    struct Times { R Fxt, Fyt, Fet; };
+    static Times forcing_times(const swm_ctx *ctx, int64_t t) {     // Ftime, forcing.jl:215-217
         const swm_config &c = ctx->c;
-        const bool per = c.bc == SWM_PERIODIC, first = k == 1, last = k == c.RKo;
+        Times T;
+        T.Fxt = c.seasonal_wind_x ? (R)std::sin(c.omegaFx * (double)t) : R(1);
+        T.Fyt = c.seasonal_wind_y ? (R)std::sin(c.omegaFy * (double)t) : R(1);
+        T.Fet = c.continuity == SWM_CONT_FORCING ? (R)std::sin(c.omegaFeta * (double)t) : R(0);
+        return T;
+    }
+
+    static StageArgs<R> args(swm_ctx *ctx, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+                             const Times &T) {
+        const swm_config &c = ctx->c;
         StageArgs<R> A;
-        A.in_u = ctx->at<R>(ctx->u[in_u]); A.in_v = ctx->at<R>(ctx->v[in_u]); A.in_e = ctx->at<R>(ctx->e[in_e]);
-        A.pr_u = ctx->at<R>(ctx->u[ctx->ruv[0]]); A.pr_v = ctx->at<R>(ctx->v[ctx->ruv[0]]);
-        A.pr_e = ctx->at<R>(ctx->e[ctx->re[0]]);
-        A.acc_u = ctx->at<R>(ctx->u[ctx->ruv[1]]); A.acc_v = ctx->at<R>(ctx->v[ctx->ruv[1]]);
-        A.acc_e = ctx->at<R>(ctx->e[ctx->re[1]]);
-        A.out_u = last ? nullptr : ctx->at<R>(ctx->u[out_u]);
-        A.out_v = last ? nullptr : ctx->at<R>(ctx->v[out_u]);
-        A.out_e = last ? nullptr : ctx->at<R>(ctx->e[out_e]);
+        A.in_u = uf(ctx, ctx->u, in.uv); A.in_v = uf(ctx, ctx->v, in.uv); A.in_e = uf(ctx, ctx->e, in.e);
+        A.pr_u = uf(ctx, ctx->u, pr.uv); A.pr_v = uf(ctx, ctx->v, pr.uv); A.pr_e = uf(ctx, ctx->e, pr.e);
+        A.ac_u = uf(ctx, ctx->u, ac.uv); A.ac_v = uf(ctx, ctx->v, ac.uv); A.ac_e = uf(ctx, ctx->e, ac.e);
+        A.o1_u = uf(ctx, ctx->u, o1.uv); A.o1_v = uf(ctx, ctx->v, o1.uv); A.o1_e = uf(ctx, ctx->e, o1.e);
+        A.o2_u = uf(ctx, ctx->u, o2.uv); A.o2_v = uf(ctx, ctx->v, o2.uv); A.o2_e = uf(ctx, ctx->e, o2.e);
         A.qend = ctx->at<R>(ctx->qend);
         A.st = statics(ctx);
-        A.a = (R)c.RKaDt[k - 1];
-        A.b = last ? R(0) : (R)c.RKbDt[k - 1];
-        A.Fxt = Fxt; A.Fyt = Fyt; A.g = (R)c.g; A.oma = (R)c.one_minus_alpha;
+        A.k = k;
+        A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
+        A.g = (R)c.g; A.oma = (R)c.one_minus_alpha; A.gamma = (R)c.gamma;
+        A.cont = c.continuity;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
         A.nstrips = nstrips(ctx); A.nseg = nseg(ctx);
-        if (per) {
+        return A;
+    }
+
+    // one stage kernel (momentum, and continuity for the RK modes) in mode m
+    static int stage(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+                     const Times &T) {
+        const swm_config &c = ctx->c;
+        const bool per = c.bc == SWM_PERIODIC;
+        StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
+        if (per)
             qend_kernel<R><<<(c.ny + 1 + 255) / 256, 256, 0, ctx->stream>>>(
                 A.in_u, A.in_v, A.in_e, A.st.H, A.st.fq, A.st.fq_sx, A.st.fq_sy, ctx->at<R>(ctx->qend), c.nx, c.ny, ctx->P);
-        }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-#define SWM_STAGE(PER_, F_, L_) hipLaunchKernelGGL((rk_stage_kernel<R, PER_, F_, L_>), grid, block, 0, ctx->stream, A)
-        if (per) {
-            if (first && last) SWM_STAGE(true, true, true);
-            else if (first) SWM_STAGE(true, true, false);
-            else if (last) SWM_STAGE(true, false, true);
-            else SWM_STAGE(true, false, false);
-        } else {
-            if (first && last) SWM_STAGE(false, true, true);
-            else if (first) SWM_STAGE(false, true, false);
-            else if (last) SWM_STAGE(false, false, true);
-            else SWM_STAGE(false, false, false);
+#define SWM_K(M_)                                                                                          \
+    case M_:                                                                                               \
+        if (per) hipLaunchKernelGGL((stage_kernel<R, true, M_>), grid, block, 0, ctx->stream, A);          \
+        else hipLaunchKernelGGL((stage_kernel<R, false, M_>), grid, block, 0, ctx->stream, A);             \
+        break;
+        switch (m) {
+            SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
+            SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
+        default: return ctx->fail(SWM_E_ARG, "stage mode %d", m);
         }
-#undef SWM_STAGE
+#undef SWM_K
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
 
-    static int dissipation(swm_ctx *ctx, int in_uv, int in_e, int out_uv) {
+    // semi-implicit continuity of the SSPRK schemes in mode m
+    static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+                          const Times &T) {
+        const bool per = ctx->c.bc == SWM_PERIODIC;
+        StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
+        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+#define SWM_K(M_)                                                                                          \
+    case M_:                                                                                               \
+        if (per) hipLaunchKernelGGL((continuity_kernel<R, true, M_>), grid, block, 0, ctx->stream, A);     \
+        else hipLaunchKernelGGL((continuity_kernel<R, false, M_>), grid, block, 0, ctx->stream, A);        \
+        break;
+        switch (m) {
+            SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3) SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
+        default: return ctx->fail(SWM_E_ARG, "continuity mode %d", m);
+        }
+#undef SWM_K
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
+    // The first n_stages stages of the configured scheme (time_integration.jl:53-196) from
+    // Prog.  On return *u0 and *u1 name the slots of the reference's u0 and u1.
+    static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1) {
+        const swm_config &c = ctx->c;
+        const Times T = forcing_times(ctx, t);
+        const Slots P{ctx->pu, ctx->pe}, none{};
+        Slots cur = P, z0 = P;
+        Coef<R> k{};
+        int rc = SWM_OK;
+        if (c.time_scheme == SWM_RK) {                                     // :55-78
+            const Slots a{pick({P.uv}), pick({P.e})};
+            for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
+                const int m = s == 1 ? M_RK_FIRST : (s == c.RKo ? M_RK_LAST : M_RK_MID);
+                k.a = (R)c.RKaDt[s - 1];
+                k.b = s < c.RKo ? (R)c.RKbDt[s - 1] : R(0);
+                const Slots n{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
+                rc = stage(ctx, m, cur, P, a, s < c.RKo ? n : none, a, k, T);
+                if (s < c.RKo) cur = n;
+            }
+            z0 = a;
+        } else if (c.time_scheme == SWM_SSPRK2) {                          // :82-110
+            k.c = (R)c.Dt_Ds;
+            k.x = (R)(1.0 / c.RKs);
+            k.y = (R)((double)(c.RKs - 1) / c.RKs);
+            for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
+                const bool last = s == c.RKs;
+                const Slots n{pick({P.uv, cur.uv}), pick({P.e, cur.e})};
+                const Slots z = last ? Slots{pick({P.uv, cur.uv, n.uv}), pick({P.e, cur.e, n.e})} : none;
+                const int m = last ? M_SSP2_LAST : M_SSP2;
+                rc = stage(ctx, m, cur, P, none, n, z, k, T);
+                if (!rc) rc = continuity(ctx, m, Slots{n.uv, cur.e}, P, none, n, z, k, T);
+                cur = n;
+                if (last) z0 = z;
+            }
+        } else if (c.time_scheme == SWM_SSPRK3) {                          // :114-156
+            k.c = (R)c.ssp_Dt_Dn;
+            k.x = (R)c.ssp_kna;
+            k.y = (R)c.ssp_knb;
+            k.z = (R)c.ssp_Dt_Dnc;
+            for (int s = 2; s <= n_stages + 1 && s <= c.ssp_s + 1 && rc == SWM_OK; ++s) {
+                const Slots n{pick({P.uv, cur.uv, z0.uv}), pick({P.e, cur.e, z0.e})};
+                const int m = s == c.ssp_kn ? M_SSP3_KN : M_SSP3;
+                rc = stage(ctx, m, cur, P, z0, n, none, k, T);
+                if (!rc) rc = continuity(ctx, m, Slots{n.uv, cur.e}, P, z0, n, none, k, T);
+                cur = n;
+                if (s == c.ssp_mn || s == c.ssp_s + 1) z0 = n;            // copyto!(u0,u1) :150-155
+            }
+        } else {                                                            // 4SSPRK3 :160-195
+            k.c = (R)c.Dt_D;
+            k.h = R(0.5);
+            k.x = (R)(2.0 / 3.0);
+            k.y = (R)(1.0 / 3.0);
+            for (int s = 1; s <= n_stages && s <= 4 && rc == SWM_OK; ++s) {
+                const int n1 = pick({P.uv, cur.uv});
+                const int n2 = s == 3 ? pick({P.uv, cur.uv, n1}) : -1;
+                const int ne = pick({P.e, cur.e});
+                const int m = s == 3 ? M_S4_3 : M_S4;
+                rc = stage(ctx, m, cur, P, none, Slots{n1, -1}, Slots{n2, -1}, k, T);
+                if (!rc) rc = continuity(ctx, m, Slots{n1, cur.e}, P, none, Slots{-1, ne}, none, k, T);
+                cur = Slots{s == 3 ? n2 : n1, ne};
+                if (s == 4) z0 = cur;                                       // copyto!(u0,u1) :191-193
+            }
+        }
+        *u0 = z0;
+        *u1 = cur;
+        return rc;
+    }
+
+    static int dissipation(swm_ctx *ctx, Slots in, int out_uv) {
         const swm_config &c = ctx->c;
         DissArgs<R> A;
-        A.u0 = ctx->at<R>(ctx->u[in_uv]); A.v0 = ctx->at<R>(ctx->v[in_uv]); A.e0 = ctx->at<R>(ctx->e[in_e]);
+        A.u0 = ctx->at<R>(ctx->u[in.uv]); A.v0 = ctx->at<R>(ctx->v[in.uv]); A.e0 = ctx->at<R>(ctx->e[in.e]);
         A.out_u = ctx->at<R>(ctx->u[out_uv]); A.out_v = ctx->at<R>(ctx->v[out_uv]);
         A.H = ctx->at<R>(ctx->H);
         A.cD = (R)c.cD; A.rD = (R)c.rD; A.cS = (R)c.cSmag; A.nuB = (R)c.nuB; A.dtd = (R)c.Dt_diff;
@@ -196,11 +315,11 @@ template <class R> struct Launch {
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
 #define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<R, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
-#define SWM_DISS_D(PER_, S_)                      \
-    switch (c.bottom_drag) {                      \
-    case SWM_DRAG_QUADRATIC: SWM_DISS(PER_, 0, S_); break; \
-    case SWM_DRAG_LINEAR: SWM_DISS(PER_, 1, S_); break;    \
-    default: SWM_DISS(PER_, 2, S_); break;                 \
+#define SWM_DISS_D(PER_, S_)                                   \
+    switch (c.bottom_drag) {                                   \
+    case SWM_DRAG_QUADRATIC: SWM_DISS(PER_, 0, S_); break;     \
+    case SWM_DRAG_LINEAR: SWM_DISS(PER_, 1, S_); break;        \
+    default: SWM_DISS(PER_, 2, S_); break;                     \
     }
         if (per) {
             if (smag) { SWM_DISS_D(true, true) } else { SWM_DISS_D(true, false) }
@@ -210,23 +329,6 @@ template <class R> struct Launch {
 #undef SWM_DISS_D
 #undef SWM_DISS
         HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    static int tracer(swm_ctx *ctx, int64_t i, int uv_slot) {
-        const swm_config &c = ctx->c;
-        if (!c.tracer_advection) return SWM_OK;
-        const size_t n = ctx->elems;
-        if ((i + c.nadvstep_half) % c.nadvstep == 0) {      // tracer_advection.jl:12-17
-            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->um.base, (const R *)ctx->u[uv_slot].base, n);
-            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->vm.base, (const R *)ctx->v[uv_slot].base, n);
-        }
-        if (i % c.nadvstep == 0) {                          // tracer_advection.jl:19-42
-            const int src = ctx->rsst, dst = 1 - ctx->rsst;
-            int rc = tracer_advect(ctx, uv_slot, src, dst);
-            if (rc) return rc;
-            ctx->rsst = dst;
-        }
         return SWM_OK;
     }
 
@@ -243,35 +345,23 @@ template <class R> struct Launch {
         return SWM_OK;
     }
 
-    static int time_kernel(swm_ctx *ctx, int which, int reps, double *ms) {
+    // tracer!  tracer_advection.jl:1-44 with the post-dissipation velocities (:226-228)
+    static int tracer(swm_ctx *ctx, int64_t i, int uv_slot) {
         const swm_config &c = ctx->c;
-        const int *ru = ctx->ruv, *re = ctx->re;
-        auto launch = [&]() -> int {
-            switch (which) {
-            case 0: return c.RKo >= 3 ? rk_stage(ctx, 2, ru[2], re[2], ru[3], re[3], R(1), R(1))
-                                      : rk_stage(ctx, 1, ru[0], re[0], ru[2], re[2], R(1), R(1));
-            case 1: return rk_stage(ctx, 1, ru[0], re[0], ru[2], re[2], R(1), R(1));
-            case 2: return rk_stage(ctx, c.RKo, ru[2], re[2], ru[3], re[3], R(1), R(1));
-            case 3: return dissipation(ctx, ru[1], re[1], ru[3]);
-            case 4: return tracer_advect(ctx, ru[0], ctx->rsst, 1 - ctx->rsst);
-            }
-            return ctx->fail(SWM_E_ARG, "unknown kernel %d", which);
-        };
-        int rc = launch();                                   // warm-up
-        if (rc) return rc;
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
-        for (int k = 0; k < reps && rc == SWM_OK; ++k) rc = launch();
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float t = 0.0f;
-        HIPCHK(hipEventElapsedTime(&t, e0, e1));
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        *ms = (double)t / reps;
-        return rc;
+        if (!c.tracer_advection) return SWM_OK;
+        const size_t n = ctx->elems;
+        if ((i + c.nadvstep_half) % c.nadvstep == 0) {      // mid-point snapshot :12-17
+            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->um.base, (const R *)ctx->u[uv_slot].base, n);
+            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->vm.base, (const R *)ctx->v[uv_slot].base, n);
+            HIPCHK(hipGetLastError());
+        }
+        if (i % c.nadvstep == 0) {                          // :19-42
+            const int src = ctx->rsst, dst = 1 - ctx->rsst;
+            int rc = tracer_advect(ctx, uv_slot, src, dst);
+            if (rc) return rc;
+            ctx->rsst = dst;
+        }
+        return SWM_OK;
     }
 
     static int ghosts(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot) {
@@ -288,7 +378,7 @@ template <class R> struct Launch {
         return SWM_OK;
     }
 
-    // NaN check of Prog (u, v, η of slot uv/e) and sst, feedback.jl:61-72
+    // NaN check of Prog (u, v of slot uv, η of slot e) and sst, feedback.jl:61-72
     static int nan_check(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot, bool *found) {
         const swm_config &c = ctx->c;
         NanArgs A;
@@ -308,72 +398,69 @@ template <class R> struct Launch {
         return SWM_OK;
     }
 
-    // Stages of one RK step; leaves u0 in role 1 and the last stage input in rk_u1.
-    static int rk_stages(swm_ctx *ctx, int n_stages, int64_t t) {
-        const swm_config &c = ctx->c;
-        const R Fxt = c.seasonal_wind_x ? (R)std::sin(c.omegaFx * (double)t) : R(1);
-        const R Fyt = c.seasonal_wind_y ? (R)std::sin(c.omegaFy * (double)t) : R(1);
-        int cur_u = ctx->ruv[0], cur_e = ctx->re[0];
-        int nxt_u = ctx->ruv[2], nxt_e = ctx->re[2];
-        for (int k = 1; k <= n_stages; ++k) {
-            int rc = rk_stage(ctx, k, cur_u, cur_e, nxt_u, nxt_e, Fxt, Fyt);
-            if (rc) return rc;
-            if (k < c.RKo) {
-                const int pu = cur_u, pe = cur_e;
-                cur_u = nxt_u;
-                cur_e = nxt_e;
-                nxt_u = (k == 1) ? ctx->ruv[3] : pu;
-                nxt_e = (k == 1) ? ctx->re[3] : pe;
-            }
-        }
-        ctx->rk_u0 = ctx->ruv[1];
-        ctx->rk_e0 = ctx->re[1];
-        ctx->rk_u1 = cur_u;
-        ctx->rk_e1 = cur_e;
-        return SWM_OK;
-    }
-
+    // one pass of the loop body, time_integration.jl:45-243
     static int step(swm_ctx *ctx, int64_t i, int64_t t, bool *nan) {
         const swm_config &c = ctx->c;
         *nan = false;
-        int rc = rk_stages(ctx, c.RKo, t);
+        Slots u0, u1;
+        int rc = stages(ctx, 1 << 20, t, &u0, &u1);
         if (rc) return rc;
-        const int p_uv = ctx->ruv[0], a_uv = ctx->ruv[1], p_e = ctx->re[0], a_e = ctx->re[1];
-        // a free u/v slot for the dissipation output: the stage slot not holding u1
-        const int free_uv = (ctx->rk_u1 == ctx->ruv[2]) ? ctx->ruv[3] : ctx->ruv[2];
-        int new_uv = a_uv;
-        if (i % c.nstep_diff == 0) {                         // time_integration.jl:216-221
-            rc = dissipation(ctx, a_uv, a_e, free_uv);
+        int new_uv = u0.uv;
+        if (i % c.nstep_diff == 0) {                         // :216-221
+            const int out = pick({ctx->pu, u0.uv});
+            rc = dissipation(ctx, u0, out);
             if (rc) return rc;
-            new_uv = free_uv;
+            new_uv = out;
         }
-        const int old_sst = ctx->rsst;
         rc = tracer(ctx, i, new_uv);                         // :226-228
         if (rc) return rc;
         if (c.nout > 0 && i % c.nout == 0) {                 // feedback.jl:136-148
-            rc = nan_check(ctx, p_uv, p_e, ctx->rsst, nan);
+            rc = nan_check(ctx, ctx->pu, ctx->pe, ctx->rsst, nan);
             if (rc) return rc;
             if (*nan) return SWM_OK;                         // keep Prog: no copy-back
         }
-        (void)old_sst;
-        // copy-back by role rotation (:240-242)
-        int others_uv[3], k = 0;
-        for (int s = 0; s < 4; ++s)
-            if (ctx->ruv[s] != new_uv) others_uv[k++] = ctx->ruv[s];
-        ctx->ruv[0] = new_uv;
-        ctx->ruv[1] = others_uv[0];
-        ctx->ruv[2] = others_uv[1];
-        ctx->ruv[3] = others_uv[2];
-        int others_e[3];
-        k = 0;
-        for (int s = 0; s < 4; ++s)
-            if (ctx->re[s] != a_e) others_e[k++] = ctx->re[s];
-        ctx->re[0] = a_e;
-        ctx->re[1] = others_e[0];
-        ctx->re[2] = others_e[1];
-        ctx->re[3] = others_e[2];
-        (void)p_uv;
+        ctx->pu = new_uv;                                    // copy-back (:240-242)
+        ctx->pe = u0.e;
         return SWM_OK;
+    }
+
+    static int time_kernel(swm_ctx *ctx, int which, int reps, double *ms) {
+        const swm_config &c = ctx->c;
+        const Slots P{ctx->pu, ctx->pe}, none{};
+        const Slots a{pick({P.uv}), pick({P.e})};
+        const Slots s1{pick({P.uv, a.uv}), pick({P.e, a.e})};
+        const Slots s2{pick({P.uv, a.uv, s1.uv}), pick({P.e, a.e, s1.e})};
+        const Times T{R(1), R(1), R(0)};
+        Coef<R> k{};
+        k.a = (R)c.RKaDt[0];
+        k.b = (R)c.RKbDt[0];
+        k.c = (R)c.Dt_D;
+        auto launch = [&]() -> int {
+            switch (which) {
+            case 0: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_MID : M_SSP2, s1, P, a, s2, a, k, T) ||
+                           (c.time_scheme != SWM_RK && continuity(ctx, M_SSP2, Slots{s2.uv, s1.e}, P, none, s2, none, k, T));
+            case 1: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_FIRST : M_SSP2, P, P, a, s1, a, k, T);
+            case 2: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_LAST : M_SSP2, s1, P, a, none, a, k, T);
+            case 3: return dissipation(ctx, a, s2.uv);
+            case 4: return tracer_advect(ctx, P.uv, ctx->rsst, 1 - ctx->rsst);
+            }
+            return ctx->fail(SWM_E_ARG, "unknown kernel %d", which);
+        };
+        int rc = launch();                                   // warm-up
+        if (rc) return rc;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps && rc == SWM_OK; ++r) rc = launch();
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float tm = 0.0f;
+        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        *ms = (double)tm / reps;
+        return rc;
     }
 };
 
@@ -385,12 +472,14 @@ int check_config(swm_ctx *ctx) {
     if (!(c.T == c.Tprog && c.Tprog == c.Tcomm) || (c.T != SWM_F64 && c.T != SWM_F32))
         return ctx->fail(SWM_E_UNSUPPORTED, "formats T=%d Tprog=%d Tcomm=%d: this build supports T=Tprog=Tcomm in {Float64, Float32}",
                          c.T, c.Tprog, c.Tcomm);
-    if (c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "time_scheme %d: only RK is implemented", c.time_scheme);
-    if (c.RKo < 2 || c.RKo > 4) return ctx->fail(SWM_E_ARG, "RKo %d", c.RKo);
+    if (c.time_scheme < SWM_RK || c.time_scheme > SWM_4SSPRK3) return ctx->fail(SWM_E_ARG, "time_scheme %d", c.time_scheme);
+    if (c.time_scheme == SWM_RK && (c.RKo < 2 || c.RKo > 4)) return ctx->fail(SWM_E_ARG, "RKo %d", c.RKo);
+    if (c.time_scheme == SWM_SSPRK2 && c.RKs < 2) return ctx->fail(SWM_E_ARG, "RKs %d", c.RKs);
+    if (c.time_scheme == SWM_SSPRK3 && (c.ssp_n < 2 || c.ssp_s != c.ssp_n * c.ssp_n)) return ctx->fail(SWM_E_ARG, "RKn %d", c.ssp_n);
     if (c.nstep_advcor != 0) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
     if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) return ctx->fail(SWM_E_UNSUPPORTED, "adv_scheme Sadourny is not implemented");
     if (c.dynamics != SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "linear dynamics is not implemented");
-    if (c.continuity != SWM_CONT_ITSELF) return ctx->fail(SWM_E_UNSUPPORTED, "surface relaxation/forcing is not implemented");
+    if (c.continuity < SWM_CONT_ITSELF || c.continuity > SWM_CONT_FORCING) return ctx->fail(SWM_E_ARG, "continuity %d", c.continuity);
     if (c.nstep_diff < 1 || (c.tracer_advection && c.nadvstep < 1)) return ctx->fail(SWM_E_ARG, "nstep_diff/nadvstep must be >= 1");
     if (c.bottom_drag < 0 || c.bottom_drag > 2 || c.diffusion < 0 || c.diffusion > 1)
         return ctx->fail(SWM_E_ARG, "bottom_drag %d / diffusion %d", c.bottom_drag, c.diffusion);
@@ -487,6 +576,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         (rc = alloc_field(ctx, ctx->um, 0, true)) || (rc = alloc_field(ctx, ctx->vm, 0, true)) ||
         (rc = alloc_field(ctx, ctx->H, 0, true)) || (rc = alloc_field(ctx, ctx->qend, (size_t)c.ny + 2 * OY + 16, false)))
         return bail(rc);
+    if (c.continuity != SWM_CONT_ITSELF &&
+        ((rc = alloc_field(ctx, ctx->eta_ref, 0, true)) || (rc = alloc_field(ctx, ctx->Feta, 0, true))))
+        return bail(rc);
     void *p = nullptr;
     if (hipMalloc(&p, 4 * sizeof(int)) != hipSuccess) return bail(SWM_E_HIP);
     ctx->allocs.push_back(p);
@@ -513,8 +605,6 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
                    const void *Feta) {
     if (!ctx) return SWM_E_ARG;
     if (!H || !Fx || !Fy || !f_q) return ctx->fail(SWM_E_ARG, "H, Fx, Fy and f_q are required");
-    (void)eta_ref;
-    (void)Feta;
     if (ctx->have_static) return ctx->fail(SWM_E_STATE, "static fields already set");
     const swm_config &c = ctx->c;
     int rc;
@@ -522,6 +612,14 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
     if ((rc = put_static(ctx, ctx->Fx, &ctx->Fx_sx, &ctx->Fx_sy, Fx, ctx->nux, ctx->nuy, 1 - ctx->ep, 1))) return rc;
     if ((rc = put_static(ctx, ctx->Fy, &ctx->Fy_sx, &ctx->Fy_sy, Fy, ctx->nvx, ctx->nvy, 1, 1))) return rc;
     if ((rc = put_static(ctx, ctx->fq, &ctx->fq_sx, &ctx->fq_sy, f_q, c.nx + 1, c.ny + 1, 0, 0))) return rc;
+    if (c.continuity == SWM_CONT_RELAX) {
+        if (!eta_ref) return ctx->fail(SWM_E_ARG, "eta_ref is required for surface relaxation");
+        if ((rc = put2d(ctx, ctx->eta_ref, eta_ref, c.nx, c.ny, 1, 1))) return rc;
+    }
+    if (c.continuity == SWM_CONT_FORCING) {
+        if (!Feta) return ctx->fail(SWM_E_ARG, "Feta is required for surface forcing");
+        if ((rc = put2d(ctx, ctx->Feta, Feta, c.nx, c.ny, 1, 1))) return rc;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_static = true;
     return SWM_OK;
@@ -558,7 +656,7 @@ int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta,
     int rc = check_never_written(ctx, u, v);
     if (rc) return rc;
     const swm_config &c = ctx->c;
-    const int pu = ctx->ruv[0], pe = ctx->re[0];
+    const int pu = ctx->pu, pe = ctx->pe;
     if ((rc = put2d(ctx, ctx->u[pu], u, ctx->nux + 4, ctx->nuy + 4, -1 - ctx->ep, -1))) return rc;
     if ((rc = put2d(ctx, ctx->v[pu], v, ctx->nvx + 4, ctx->nvy + 4, -1, -1))) return rc;
     if ((rc = put2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
@@ -575,7 +673,7 @@ int swm_download_prog(swm_ctx *ctx, void *u, void *v, void *eta, void *sst) {
     if (!ctx->have_prog) return ctx->fail(SWM_E_STATE, "no prognostic state uploaded");
     const swm_config &c = ctx->c;
     int rc;
-    const int pu = ctx->ruv[0], pe = ctx->re[0];
+    const int pu = ctx->pu, pe = ctx->pe;
     if (u && (rc = get2d(ctx, ctx->u[pu], u, ctx->nux + 4, ctx->nuy + 4, -1 - ctx->ep, -1))) return rc;
     if (v && (rc = get2d(ctx, ctx->v[pu], v, ctx->nvx + 4, ctx->nvy + 4, -1, -1))) return rc;
     if (eta && (rc = get2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
@@ -595,9 +693,13 @@ int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *step
 int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
     if (!ctx) return SWM_E_ARG;
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
-    if (n_stages < 1 || n_stages > ctx->c.RKo) return ctx->fail(SWM_E_ARG, "n_stages %d out of 1..%d", n_stages, ctx->c.RKo);
+    const swm_config &c = ctx->c;
+    const int total = c.time_scheme == SWM_RK ? c.RKo : c.time_scheme == SWM_SSPRK2 ? c.RKs
+                    : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
+    if (n_stages < 1 || n_stages > total) return ctx->fail(SWM_E_ARG, "n_stages %d out of 1..%d", n_stages, total);
     (void)i0;
-    int rc = ctx->c.T == SWM_F64 ? Launch<double>::rk_stages(ctx, n_stages, t0) : Launch<float>::rk_stages(ctx, n_stages, t0);
+    int rc = c.T == SWM_F64 ? Launch<double>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1)
+                            : Launch<float>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
@@ -605,15 +707,16 @@ int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
 
 int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void *v1, void *eta1) {
     if (!ctx) return SWM_E_ARG;
+    if (ctx->rk0.uv < 0 || ctx->rk1.uv < 0) return ctx->fail(SWM_E_STATE, "call swm_stages first");
     const swm_config &c = ctx->c;
     int rc;
     const int mu = ctx->nux + 4, nu = ctx->nuy + 4, mv = ctx->nvx + 4, nv = ctx->nvy + 4;
-    if (u0 && (rc = get2d(ctx, ctx->u[ctx->rk_u0], u0, mu, nu, -1 - ctx->ep, -1))) return rc;
-    if (v0 && (rc = get2d(ctx, ctx->v[ctx->rk_u0], v0, mv, nv, -1, -1))) return rc;
-    if (eta0 && (rc = get2d(ctx, ctx->e[ctx->rk_e0], eta0, c.nx + 2, c.ny + 2, 0, 0))) return rc;
-    if (u1 && (rc = get2d(ctx, ctx->u[ctx->rk_u1], u1, mu, nu, -1 - ctx->ep, -1))) return rc;
-    if (v1 && (rc = get2d(ctx, ctx->v[ctx->rk_u1], v1, mv, nv, -1, -1))) return rc;
-    if (eta1 && (rc = get2d(ctx, ctx->e[ctx->rk_e1], eta1, c.nx + 2, c.ny + 2, 0, 0))) return rc;
+    if (u0 && (rc = get2d(ctx, ctx->u[ctx->rk0.uv], u0, mu, nu, -1 - ctx->ep, -1))) return rc;
+    if (v0 && (rc = get2d(ctx, ctx->v[ctx->rk0.uv], v0, mv, nv, -1, -1))) return rc;
+    if (eta0 && (rc = get2d(ctx, ctx->e[ctx->rk0.e], eta0, c.nx + 2, c.ny + 2, 0, 0))) return rc;
+    if (u1 && (rc = get2d(ctx, ctx->u[ctx->rk1.uv], u1, mu, nu, -1 - ctx->ep, -1))) return rc;
+    if (v1 && (rc = get2d(ctx, ctx->v[ctx->rk1.uv], v1, mv, nv, -1, -1))) return rc;
+    if (eta1 && (rc = get2d(ctx, ctx->e[ctx->rk1.e], eta1, c.nx + 2, c.ny + 2, 0, 0))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
 }
@@ -630,7 +733,7 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     if (!ctx || !mass || !energy) return SWM_E_ARG;
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     const swm_config &c = ctx->c;
-    const int pu = ctx->ruv[0], pe = ctx->re[0];
+    const int pu = ctx->pu, pe = ctx->pe;
     if (c.T == SWM_F64)
         reduce_kernel<double><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<double>(ctx->u[pu]), ctx->at<double>(ctx->v[pu]),
                                                              ctx->at<double>(ctx->e[pe]), ctx->at<double>(ctx->H), c.nx,

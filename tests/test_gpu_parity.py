@@ -32,7 +32,32 @@ CASES = {
     "np-f64-const-quad-rk2": dict(T="Float64", nx=50, L_ratio=0.5, bc="nonperiodic", α=1.5, diffusion="constant",
                                   bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre", RKo=2,
                                   cfl=0.5),
+    # the strong-stability-preserving schemes with their semi-implicit continuity
+    # (test/runtests.jl:48-82 pairs of RKs/RKn and cfl)
+    "np-f64-ssprk2-s3": dict(T="Float64", nx=66, L_ratio=1.0, bc="nonperiodic", α=0.0, diffusion="Smagorinsky",
+                             topography="seamount", wind_forcing_x="double_gyre", time_scheme="SSPRK2", RKs=3, cfl=1.2),
+    "per-f32-ssprk2-s2": dict(T="Float32", nx=64, L_ratio=2.0, bc="periodic", time_scheme="SSPRK2", RKs=2, cfl=0.7),
+    "np-f32-ssprk2-s4": dict(T="Float32", nx=62, L_ratio=0.9, bc="nonperiodic", α=1.0, diffusion="Smagorinsky",
+                             topography="ridge", time_scheme="SSPRK2", RKs=4, cfl=1.4),
+    "per-f64-ssprk3-n2": dict(T="Float64", nx=70, L_ratio=1.2, bc="periodic", diffusion="Smagorinsky",
+                              topography="seamount", time_scheme="SSPRK3", RKn=2, cfl=1.2),
+    "np-f32-ssprk3-n3": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=2.0, topography="seamount",
+                             wind_forcing_x="double_gyre", time_scheme="SSPRK3", RKn=3, cfl=2.5),
+    "np-f64-4ssprk3": dict(T="Float64", nx=61, L_ratio=1.0, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                           topography="seamount", wind_forcing_x="double_gyre", time_scheme="4SSPRK3", cfl=1.0),
+    "per-f32-4ssprk3": dict(T="Float32", nx=64, L_ratio=2.0, bc="periodic", time_scheme="4SSPRK3", cfl=1.0),
+    # continuity variants and seasonal winds (continuity.jl:3-49, rhs.jl:233-238)
+    "np-f64-relax": dict(T="Float64", nx=64, L_ratio=1.0, bc="nonperiodic", surface_relax=True, t_relax=2.0,
+                         topography="seamount", wind_forcing_x="double_gyre"),
+    "per-f32-forcing-seasonal": dict(T="Float32", nx=64, L_ratio=2.0, bc="periodic", surface_forcing=True,
+                                     seasonal_wind_x=True, ωFx=400.0, A=3e-3),
+    "np-f32-ssprk2-relax": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", surface_relax=True, t_relax=2.0,
+                                time_scheme="SSPRK2", RKs=3, cfl=1.2),
 }
+
+
+def n_stages(P):
+    return {"RK": P.RKo, "SSPRK2": P.RKs, "SSPRK3": P.RKn ** 2, "4SSPRK3": 4}[P.time_scheme]
 
 
 def setup_case(swm, name, seed=11, σ=0.8, **over):
@@ -68,22 +93,24 @@ def test_rk_stages_bitwise(swm, orc, gpu, case):
     P, S, prog = setup_case(swm, case)
     cfg = swm.make_config(S)
     T = P.Tprog
+    ns = n_stages(P)
     with swm.Simulation(S) as sim:
         sim.upload(prog)
-        for k in range(1, P.RKo + 1):
+        for k in range(1, ns + 1):
             sim.stages(k, 1, 0)
             g_u0, g_v0, g_e0, g_u1, g_v1, g_e1 = sim.download_rk()
             *_, rk, _, _ = orc.run(S, cfg, prog, 1, stop_stage=k, want_rk=True)
             r_u0, r_v0, r_e0, r_u1, r_v1, r_e1 = rk[:6]
-            # u1 enters the next stage after ghost_points! (time_integration.jl:57)
+            # u1 enters the next stage after ghost_points! (time_integration.jl:57,84,118,162)
             r_u1, r_v1, r_e1 = ghosted(swm, S, r_u1, r_v1, r_e1)
             assert_same(f"{case} stage {k} u1", r_u1, g_u1, T)
             assert_same(f"{case} stage {k} v1", r_v1, g_v1, T)
             assert_same(f"{case} stage {k} η1", r_e1, g_e1, T)
-            if k < P.RKo:   # u0 halo is not maintained between stages; compare the interior
-                assert_same(f"{case} stage {k} u0", r_u0[2:-2, 2:-2], g_u0[2:-2, 2:-2], T)
-                assert_same(f"{case} stage {k} v0", r_v0[2:-2, 2:-2], g_v0[2:-2, 2:-2], T)
-                assert_same(f"{case} stage {k} η0", r_e0[1:-1, 1:-1], g_e0[1:-1, 1:-1], T)
+            if k < ns:
+                if P.time_scheme == "RK":   # u0 halo is not maintained between stages; compare the interior
+                    assert_same(f"{case} stage {k} u0", r_u0[2:-2, 2:-2], g_u0[2:-2, 2:-2], T)
+                    assert_same(f"{case} stage {k} v0", r_v0[2:-2, 2:-2], g_v0[2:-2, 2:-2], T)
+                    assert_same(f"{case} stage {k} η0", r_e0[1:-1, 1:-1], g_e0[1:-1, 1:-1], T)
             else:           # ghost_points!(u0,v0,η0) after the stages (:198)
                 r_u0, r_v0, r_e0 = ghosted(swm, S, r_u0, r_v0, r_e0)
                 assert_same(f"{case} stage {k} u0", r_u0, g_u0, T)
@@ -94,7 +121,7 @@ def test_rk_stages_bitwise(swm, orc, gpu, case):
 @pytest.mark.parametrize("case", list(CASES))
 def test_ten_steps_bitwise(swm, orc, gpu, case):
     P, S, prog = setup_case(swm, case)
-    assert S.grid.nadvstep <= 5, S.grid.nadvstep
+    assert S.grid.nadvstep <= 8, S.grid.nadvstep     # the tracer fires within the run
     nsteps = 10
     r = orc.run(S, swm.make_config(S), prog, nsteps)
     with swm.Simulation(S) as sim:
@@ -157,8 +184,7 @@ def test_upload_rejects_nonzero_unwritten_halo(swm, gpu):
 
 
 def test_unsupported_options_fail_loudly(swm, gpu):
-    for kw in (dict(time_scheme="SSPRK2"), dict(nstep_advcor=1), dict(dynamics="linear"),
-               dict(adv_scheme="Sadourny"), dict(surface_relax=True)):
+    for kw in (dict(nstep_advcor=1), dict(dynamics="linear"), dict(adv_scheme="Sadourny")):
         P = swm.Parameter(nx=40, **kw)
         S = swm.make_setup(P)
         with pytest.raises(Exception):
