@@ -845,7 +845,7 @@ template <class T, class Tp, class Tc> struct Model {
                 continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
                 axb(eta1, dts, deta);
                 ++stages_run;
-                if (stop()) return false;
+                if (stop() && rki < c.RKs) return false;
             }
             const double a = 1.0 / c.RKs, b = (double)(c.RKs - 1) / c.RKs;
             cxayb(u0, a, u, b, u1);
@@ -905,6 +905,7 @@ template <class T, class Tp, class Tc> struct Model {
             }
         }
 
+        if (stop_stage > 0) return false;                      // all stages run: state before :198
         ghost_points(u0, v0, eta0);                            // :198
         const Arr<T> &a0 = rhs_view(u0, ur), &b0 = rhs_view(v0, vr), &e0 = rhs_view(eta0, etar);
         if (c.nstep_advcor > 0 && i % c.nstep_advcor == 0) {   // :208-211

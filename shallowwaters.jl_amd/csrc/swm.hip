@@ -189,11 +189,29 @@ template <class R> struct Launch {
         return A;
     }
 
+    // every buffer a mode reads or writes must name a slot (no launch through a null base)
+    static bool slots_ok(int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, bool cont) {
+        const bool rk = m <= M_RK_LAST;
+        const bool need_pr = m == M_RK_MID || m == M_SSP2_LAST || m == M_S4_3;
+        const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN;
+        const bool out1 = m != M_RK_LAST;
+        const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont);
+        auto ok = [](int s) { return s >= 0 && s < 4; };
+        const bool uv = !cont, e = rk || cont;            // which of (u,v) and η the kernel touches
+        if (!ok(in.uv) || !ok(in.e)) return false;
+        if (need_pr && ((uv && !ok(pr.uv)) || (e && !ok(pr.e)))) return false;
+        if (need_ac && ((uv && !ok(ac.uv)) || (e && !ok(ac.e)))) return false;
+        if (out1 && ((uv && !ok(o1.uv)) || (e && !ok(o1.e)))) return false;
+        if (out2 && ((uv && !ok(o2.uv)) || (e && !ok(o2.e)))) return false;
+        return true;
+    }
+
     // one stage kernel (momentum, and continuity for the RK modes) in mode m
     static int stage(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
                      const Times &T) {
         const swm_config &c = ctx->c;
         const bool per = c.bc == SWM_PERIODIC;
+        if (!slots_ok(m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
         StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
         if (per)
             qend_kernel<R><<<(c.ny + 1 + 255) / 256, 256, 0, ctx->stream>>>(
@@ -218,6 +236,8 @@ template <class R> struct Launch {
     static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
                           const Times &T) {
         const bool per = ctx->c.bc == SWM_PERIODIC;
+        if (m <= M_RK_LAST || !slots_ok(m, in, pr, ac, o1, o2, true))
+            return ctx->fail(SWM_E_STATE, "continuity mode %d: buffer slot missing", m);
         StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
@@ -236,8 +256,13 @@ template <class R> struct Launch {
 
     // The first n_stages stages of the configured scheme (time_integration.jl:53-196) from
     // Prog.  On return *u0 and *u1 name the slots of the reference's u0 and u1.
+    static int total_stages(const swm_config &c) {
+        return c.time_scheme == SWM_RK ? c.RKo : c.time_scheme == SWM_SSPRK2 ? c.RKs : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
+    }
+
     static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1) {
         const swm_config &c = ctx->c;
+        if (n_stages > total_stages(c)) n_stages = total_stages(c);
         const Times T = forcing_times(ctx, t);
         const Slots P{ctx->pu, ctx->pe}, none{};
         Slots cur = P, z0 = P;
