@@ -58,6 +58,27 @@ def cpu_baseline(swm, orc, sample_steps: int, nx: int):
             "seconds": dt}
 
 
+def copy_probe(mib: int, reps: int = 20):
+    """A plain copy of `mib` MiB (read mib, write mib), timed with events: the calibration
+    kernel for the HBM traffic counters and the practical copy bandwidth of the device."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "ms": ms,
+                      "GBs": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,7 +88,13 @@ def main():
     ap.add_argument("--dtype", default="Float32", choices=["Float32", "Float64"])
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 skips)")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only time a plain device-to-device copy of this many MiB (the reference kernel of "
+                         "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the time step is "
               f"benchmarked on a single MI355X; multi-GPU sharding is a design note, not a run).", file=sys.stderr)

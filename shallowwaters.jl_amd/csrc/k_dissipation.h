@@ -6,9 +6,10 @@
 namespace swm {
 
 template <class R> struct DissArgs {
-    const R *u0, *v0, *e0;            // state after the RK stages (ghosts applied)
-    R *out_u, *out_v;                 // u0 + Δt_diff*(drag + diffusion), with ghosts
-    const R *H;
+    ArenaRef ar;                      // the context's fields; the members below are origins in it
+    uint32_t u0, v0, e0;              // state after the RK stages (ghosts applied)
+    uint32_t out_u, out_v;            // u0 + Δt_diff*(drag + diffusion), with ghosts
+    uint32_t H;
     R cD, rD, cS, nuB, dtd, oma;
     int nx, ny, P, L, nstrips, nseg;
 };
@@ -28,6 +29,10 @@ __global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<R> A) {
     const bool lane_ok = t.lane >= HALO && t.lane < WAVE - HALO;
     const bool u_col = lane_ok && (PER ? (x >= 0 && x < nx) : (x >= 1 && x <= nx - 1));
     const bool v_col = lane_ok && x >= 1 && x <= nx;
+    const GhostLanes gl = ghost_lanes<PER>(x, nx);
+    const Arena<R> ar(A.ar);
+    const Fld<R> fu{&ar, A.u0}, fv{&ar, A.v0}, fe{&ar, A.e0}, fH{&ar, A.H}, ou{&ar, A.out_u}, ov{&ar, A.out_v};
+    const uint32_t vx = colb<R>(x), dso = (uint32_t)P * sizeof(R);
     const R half = R(0.5), quarter = R(0.25);
 
     R u_1 = 0, u_2 = 0, uxm_1 = 0, v_1 = 0, v_2 = 0, h_1 = 0, h_2 = 0, sk_1 = 0, sk_2 = 0;
@@ -36,15 +41,15 @@ __global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<R> A) {
 
     int r = t.ya - 2;
     const int rlast = t.yb + 1;
-    const R *pu = A.u0 + x + r * P, *pv = A.v0 + x + r * P, *pe = A.e0 + x + r * P, *pH = A.H + x + r * P;
-    R nu = *pu, nv = *pv, ne = 0, nH = 0;              // row r+1 is loaded while row r is computed
-    if (DRAG == 0) { ne = *pe; nH = *pH; }
-    for (; r <= rlast; ++r) {
+    uint32_t so = rowb<R>(r, P);
+    R nu = fu.ld(vx, so), nv = fv.ld(vx, so), ne = 0, nH = 0;   // row r+1 is loaded while row r is computed
+    if (DRAG == 0) { ne = fe.ld(vx, so); nH = fH.ld(vx, so); }
+    for (; r <= rlast; ++r, so += dso) {
         const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
         if (r < rlast) {
-            pu += P; pv += P;
-            nu = *pu; nv = *pv;
-            if (DRAG == 0) { pe += P; pH += P; ne = *pe; nH = *pH; }
+            const uint32_t sn = so + dso;
+            nu = fu.ld(vx, sn); nv = fv.ld(vx, sn);
+            if (DRAG == 0) { ne = fe.ld(vx, sn); nH = fH.ld(vx, sn); }
         }
         const R uxm_0 = sh_xm(u_0);
         R h_0 = 0, sk_0 = 0;
@@ -100,10 +105,11 @@ __global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<R> A) {
             Bv = A.rD * v_2;
         }
         const int y = r - 2;
-        if (u_col && y >= t.ya && y < t.yb)
-            store_u<R, PER>(A.out_u, x, y, u_2 + A.dtd * (Bu + LLu1 + LLu2), nx, ny, P, A.oma);
-        if (v_col && y >= t.ya && y < t.yb && y <= ny - 1)
-            store_v<R, PER>(A.out_v, x, y, v_2 + A.dtd * (Bv + LLv1 + LLv2), nx, P, A.oma);
+        const uint32_t so2 = so - 2 * dso;
+        if (y >= t.ya && y < t.yb) {
+            if (u_col) put_u<R, PER>(ou, x, vx, y, so2, u_2 + A.dtd * (Bu + LLu1 + LLu2), gl.u, nx, ny, P, A.oma);
+            if (v_col && y <= ny - 1) put_v<R, PER>(ov, x, vx, y, so2, v_2 + A.dtd * (Bv + LLv1 + LLv2), gl.v, nx, P, A.oma);
+        }
         // advance
         u_2 = u_1; u_1 = u_0; uxm_1 = uxm_0; v_2 = v_1; v_1 = v_0; h_2 = h_1; h_1 = h_0; sk_2 = sk_1; sk_1 = sk_0;
         DT_1 = DT_0; DSq_2 = DSq_1; DSqxm_2 = DSqxm_1; nu_2 = nu_1; Lu_2 = Lu_1; Lv_2 = Lv_1; Lvxp_2 = Lvxp_1;

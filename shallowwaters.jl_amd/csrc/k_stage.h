@@ -45,18 +45,29 @@ __device__ __forceinline__ void combine(R w, R pr, R ac, R d, const Coef<R> &k, 
 }
 
 template <class R> struct StageArgs {
-    const R *in_u, *in_v, *in_e;      // stage input (u1 of the reference; Prog at stage 1)
-    const R *pr_u, *pr_v, *pr_e;      // Prog
-    const R *ac_u, *ac_v, *ac_e;      // u0
-    R *o1_u, *o1_v, *o1_e;            // first output (u1 / u1')
-    R *o2_u, *o2_v, *o2_e;            // second output (u0 / u1'')
-    const R *qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap)
-    Static<R> st;
+    ArenaRef ar;                      // the context's fields; the members below are origins in it
+    uint32_t in_u, in_v, in_e;        // stage input (u1 of the reference; Prog at stage 1)
+    uint32_t pr_u, pr_v, pr_e;        // Prog
+    uint32_t ac_u, ac_v, ac_e;        // u0
+    uint32_t o1_u, o1_v, o1_e;        // first output (u1 / u1')
+    uint32_t o2_u, o2_v, o2_e;        // second output (u0 / u1'')
+    uint32_t qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap), 1-D
+    StaticRefs st;
     Coef<R> k;
     R Fxt, Fyt, Fet, g, oma, gamma;
     int cont;
     int nx, ny, P, L, nstrips, nseg;
 };
+
+// dη at T-point (x, y) from the flux divergence, continuity.jl:3-68
+template <class R>
+__device__ __forceinline__ R continuity_rhs(R dUdx, R dVdy, R e, int cont, const Fld<R> &eref, const Fld<R> &feta,
+                                            uint32_t vo, uint32_t so, R gamma, R Fet) {
+    R d = -(dUdx + dVdy);
+    if (cont == CONT_RELAX) d = d + gamma * (eref.ld(vo, so) - e);
+    else if (cont == CONT_FORCING) d = d + Fet * feta.ld(vo, so);
+    return d;
+}
 
 // ------------------------------------------------------------------ stage kernel (K1)
 //
@@ -77,11 +88,20 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
     const bool v_col = lane_ok && x >= 1 && x <= nx;
     const bool e_col = v_col;
     const bool qfix = PER && x == 0;
+    const GhostLanes gl = ghost_lanes<PER>(x, nx);
+
+    const Arena<R> ar(A.ar);
+    const Fld<R> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e}, fH{&ar, A.st.H};
+    const Fld<R> pr_u{&ar, A.pr_u}, pr_v{&ar, A.pr_v}, pr_e{&ar, A.pr_e};
+    const Fld<R> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
+    const Fld<R> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
+    const Fld<R> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
+    const Fld<R> fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy}, fqend{&ar, A.qend};
+    const Fld<R> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const uint32_t vx = colb<R>(x), dso = (uint32_t)P * sizeof(R);
+    const uint32_t vfq = scolb<R>(x, A.st.fq_2d), vFx = scolb<R>(x, A.st.Fx_2d), vFy = scolb<R>(x, A.st.Fy_2d);
 
     const R half = R(0.5), quarter = R(0.25), k12 = R(1.0 / 12.0);
-    const R *fq_row = A.st.fq + x * A.st.fq_sx;
-    const R *Fx_row = A.st.Fx + x * A.st.Fx_sx;
-    const R *Fy_row = A.st.Fy + x * A.st.Fy_sx;
 
     // row history (suffix _1 = r-1, _2 = r-2)
     R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
@@ -93,24 +113,23 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
     // while row r is computed.
     int r = t.ya - 1;
     const int rlast = t.yb + 1;
-    const R *pu = A.in_u + x + r * P, *pv = A.in_v + x + r * P, *pe = A.in_e + x + r * P;
-    const R *pH = A.st.H + x + r * P;
-    R nu = *pu, nv = *pv, ne = *pe, nH = *pH;
+    uint32_t so = rowb<R>(r, P);                               // byte offset of row r
+    R nu = in_u.ld(vx, so), nv = in_v.ld(vx, so), ne = in_e.ld(vx, so), nH = fH.ld(vx, so);
     R npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
-    for (; r <= rlast; ++r) {
+    for (; r <= rlast; ++r, so += dso) {
         const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
         const R pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         if (r < rlast) {
-            pu += P; pv += P; pe += P; pH += P;
-            nu = *pu; nv = *pv; ne = *pe; nH = *pH;
+            const uint32_t sn = so + dso;
+            nu = in_u.ld(vx, sn); nv = in_v.ld(vx, sn); ne = in_e.ld(vx, sn); nH = fH.ld(vx, sn);
         }
         if (r >= t.ya && r < t.yb) {                          // row r of u, η is emitted next iteration
-            if (MI::need_pr) { npu = A.pr_u[x + r * P]; if (MI::rk) npe = A.pr_e[x + r * P]; }
-            if (MI::need_ac) { nau = A.ac_u[x + r * P]; if (MI::rk) nae = A.ac_e[x + r * P]; }
+            if (MI::need_pr) { npu = pr_u.ld(vx, so); if (MI::rk) npe = pr_e.ld(vx, so); }
+            if (MI::need_ac) { nau = ac_u.ld(vx, so); if (MI::rk) nae = ac_e.ld(vx, so); }
         }
         if (r - 1 >= t.ya && r - 1 < t.yb) {                  // row r-1 of v is emitted next iteration
-            if (MI::need_pr) npv = A.pr_v[x + (r - 1) * P];
-            if (MI::need_ac) nav = A.ac_v[x + (r - 1) * P];
+            if (MI::need_pr) npv = pr_v.ld(vx, so - dso);
+            if (MI::need_ac) nav = ac_v.ld(vx, so - dso);
         }
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
@@ -123,7 +142,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
         const R dvdx_1 = sh_xp(v_1) - v_1;
         const R dudy_1 = u_0 - u_1;
-        const R q_1 = (fq_row[(r - 1) * A.st.fq_sy] + dvdx_1 - dudy_1) / hq_1;
+        const R q_1 = (fq.ld(vfq, srowb<R>(r - 1, P, A.st.fq_2d)) + dvdx_1 - dudy_1) / hq_1;
         // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
         const R uu_1 = u_1 * u_1;
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
@@ -134,7 +153,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         R qxm_1 = sh_xm(q_1);
         R qend_1 = 0;
         if (PER) {
-            qend_1 = A.qend[r - 1];
+            qend_1 = fqend.ld(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(R));
             if (qfix) qxm_1 = qend_1;
         }
         const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
@@ -148,41 +167,44 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         const R Vxp_1 = sh_xp(V_1);
         const R p_xp1 = sh_xp(p_1);
         const int yu = r - 1;
-        if (u_col && yu >= t.ya && yu < t.yb) {
+        const uint32_t so1 = so - dso;
+        const bool row1 = yu >= t.ya && yu < t.yb;
+        if (u_col && row1) {
             const R qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
-            const R du = (qhv - (p_xp1 - p_1)) + A.Fxt * Fx_row[yu * A.st.Fx_sy];
+            const R du = (qhv - (p_xp1 - p_1)) + A.Fxt * fFx.ld(vFx, srowb<R>(yu, P, A.st.Fx_2d));
             R o1 = 0, o2 = 0;
             combine<M>(u_1, MI::first ? u_1 : pu_1, au_1, du, A.k, o1, o2);
-            if (MI::out1) store_u<R, PER>(A.o1_u, x, yu, o1, nx, ny, P, A.oma);
+            if (MI::out1) put_u<R, PER>(o1_u, x, vx, yu, so1, o1, gl.u, nx, ny, P, A.oma);
             if (MI::out2) {
-                if (MI::ghost2) store_u<R, PER>(A.o2_u, x, yu, o2, nx, ny, P, A.oma);
-                else A.o2_u[x + yu * P] = o2;
+                if (MI::ghost2) put_u<R, PER>(o2_u, x, vx, yu, so1, o2, gl.u, nx, ny, P, A.oma);
+                else o2_u.st(o2, vx, so1);
             }
         }
         // dη at row r-1: ∂x!(dUdx,U), ∂y!(dVdy,V), continuity!  continuity.jl:52-98
         const R Uxm_1 = sh_xm(U_1);
-        if (MI::rk && e_col && yu >= t.ya && yu < t.yb) {
-            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, A.st, x, yu, P, A.gamma, A.Fet);
+        if (MI::rk && e_col && row1) {
+            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vx, so1, A.gamma, A.Fet);
             R o1 = 0, o2 = 0;
             combine<M>(e_1, MI::first ? e_1 : pe_1, ae_1, deta, A.k, o1, o2);
-            if (MI::out1) store_e<R, PER>(A.o1_e, x, yu, o1, nx, ny, P);
+            if (MI::out1) put_e<R, PER>(o1_e, x, vx, yu, so1, o1, gl.e, nx, ny, P);
             if (MI::out2) {
-                if (MI::ghost2) store_e<R, PER>(A.o2_e, x, yu, o2, nx, ny, P);
-                else A.o2_e[x + yu * P] = o2;
+                if (MI::ghost2) put_e<R, PER>(o2_e, x, vx, yu, so1, o2, gl.e, nx, ny, P);
+                else o2_e.st(o2, vx, so1);
             }
         }
         // dv at row r-2: PV_ArakawaHsu! qhu  PV_advection.jl:81 ; momentum_v!  rhs.jl:271
         const R qdxm_1 = sh_xm(qd_1);
         const int yv = r - 2;
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
+            const uint32_t so2 = so - 2 * dso;
             const R qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
-            const R dv = -(qhu + (p_1 - p_2)) + A.Fyt * Fy_row[yv * A.st.Fy_sy];
+            const R dv = -(qhu + (p_1 - p_2)) + A.Fyt * fFy.ld(vFy, srowb<R>(yv, P, A.st.Fy_2d));
             R o1 = 0, o2 = 0;
             combine<M>(v_2, MI::first ? v_2 : pv_2, av_2, dv, A.k, o1, o2);
-            if (MI::out1) store_v<R, PER>(A.o1_v, x, yv, o1, nx, P, A.oma);
+            if (MI::out1) put_v<R, PER>(o1_v, x, vx, yv, so2, o1, gl.v, nx, P, A.oma);
             if (MI::out2) {
-                if (MI::ghost2) store_v<R, PER>(A.o2_v, x, yv, o2, nx, P, A.oma);
-                else A.o2_v[x + yv * P] = o2;
+                if (MI::ghost2) put_v<R, PER>(o2_v, x, vx, yv, so2, o2, gl.v, nx, P, A.oma);
+                else o2_v.st(o2, vx, so2);
             }
         }
         // advance the row history
@@ -206,23 +228,28 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
     const bool e_col = t.lane >= HALO && t.lane < WAVE - HALO && x >= 1 && x <= nx;
+    const GhostLanes gl = ghost_lanes<PER>(x, nx);
+    const Arena<R> ar(A.ar);
+    const Fld<R> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e}, fH{&ar, A.st.H};
+    const Fld<R> pr_e{&ar, A.pr_e}, ac_e{&ar, A.ac_e}, o1_e{&ar, A.o1_e}, o2_e{&ar, A.o2_e};
+    const Fld<R> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const uint32_t vx = colb<R>(x), dso = (uint32_t)P * sizeof(R);
     const R half = R(0.5);
 
     R v_1 = 0, e_1 = 0, h_1 = 0, U_1 = 0, V_2 = 0;
     int r = t.ya - 1;
     const int rlast = t.yb;
-    const R *pu = A.in_u + x + r * P, *pv = A.in_v + x + r * P, *pe = A.in_e + x + r * P;
-    const R *pH = A.st.H + x + r * P;
-    R nu = *pu, nv = *pv, ne = *pe, nH = *pH, npe = 0, nae = 0;
-    for (; r <= rlast; ++r) {
+    uint32_t so = rowb<R>(r, P);
+    R nu = in_u.ld(vx, so), nv = in_v.ld(vx, so), ne = in_e.ld(vx, so), nH = fH.ld(vx, so), npe = 0, nae = 0;
+    for (; r <= rlast; ++r, so += dso) {
         const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH, pe_1 = npe, ae_1 = nae;
         if (r < rlast) {
-            pu += P; pv += P; pe += P; pH += P;
-            nu = *pu; nv = *pv; ne = *pe; nH = *pH;
+            const uint32_t sn = so + dso;
+            nu = in_u.ld(vx, sn); nv = in_v.ld(vx, sn); ne = in_e.ld(vx, sn); nH = fH.ld(vx, sn);
         }
         if (r >= t.ya && r < t.yb) {
-            if (MI::need_pr) npe = A.pr_e[x + r * P];
-            if (MI::need_ac) nae = A.ac_e[x + r * P];
+            if (MI::need_pr) npe = pr_e.ld(vx, so);
+            if (MI::need_ac) nae = ac_e.ld(vx, so);
         }
         // UVfluxes!  rhs.jl:307-324 from the updated velocities
         const R h_0 = e_0 + H_0;
@@ -231,13 +258,14 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
         const R Uxm_1 = sh_xm(U_1);
         const int y = r - 1;
         if (e_col && y >= t.ya && y < t.yb) {
-            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, A.st, x, y, P, A.gamma, A.Fet);
+            const uint32_t so1 = so - dso;
+            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vx, so1, A.gamma, A.Fet);
             R o1 = 0, o2 = 0;
             combine<M>(e_1, pe_1, ae_1, deta, A.k, o1, o2);
-            if (M == M_S4_3) store_e<R, PER>(A.o1_e, x, y, o2, nx, ny, P);    // only u1'' = 2/3 η + 1/3 η1' lives on
+            if (M == M_S4_3) put_e<R, PER>(o1_e, x, vx, y, so1, o2, gl.e, nx, ny, P);   // only 2/3 η + 1/3 η1' lives on
             else {
-                store_e<R, PER>(A.o1_e, x, y, o1, nx, ny, P);
-                if (MI::out2) store_e<R, PER>(A.o2_e, x, y, o2, nx, ny, P);
+                put_e<R, PER>(o1_e, x, vx, y, so1, o1, gl.e, nx, ny, P);
+                if (MI::out2) put_e<R, PER>(o2_e, x, vx, y, so1, o2, gl.e, nx, ny, P);
             }
         }
         v_1 = v_0; e_1 = e_0; h_1 = h_0; U_1 = U_0; V_2 = V_1;
@@ -247,18 +275,21 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
 // q at the corner column x = nx for every corner row y = 0..ny (PV!, PV_advection.jl:15):
 // the periodic AHβ!/AHγ! read q[end,·] for their first column (PV_advection.jl:110,126).
 template <class R>
-__global__ void qend_kernel(const R *u, const R *v, const R *e, const R *H, const R *fq, int fq_sx, int fq_sy,
-                            R *qend, int nx, int ny, int P) {
+__global__ void qend_kernel(StageArgs<R> A) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y > ny) return;
+    if (y > A.ny) return;
+    const Arena<R> ar(A.ar);
+    const Fld<R> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e}, H{&ar, A.st.H}, fq{&ar, A.st.fq}, qe{&ar, A.qend};
     const R quarter = R(0.25);
-    const int x = nx;
-    const R h00 = e[x + y * P] + H[x + y * P], h10 = e[x + 1 + y * P] + H[x + 1 + y * P];
-    const R h01 = e[x + (y + 1) * P] + H[x + (y + 1) * P], h11 = e[x + 1 + (y + 1) * P] + H[x + 1 + (y + 1) * P];
+    const int x = A.nx, P = A.P;
+    const uint32_t c0 = colb<R>(x), c1 = colb<R>(x + 1), r0 = rowb<R>(y, P), r1 = rowb<R>(y + 1, P);
+    const R h00 = e.ld(c0, r0) + H.ld(c0, r0), h10 = e.ld(c1, r0) + H.ld(c1, r0);
+    const R h01 = e.ld(c0, r1) + H.ld(c0, r1), h11 = e.ld(c1, r1) + H.ld(c1, r1);
     const R hq = quarter * (h00 + h10) + quarter * (h01 + h11);
-    const R dvdx = v[x + 1 + y * P] - v[x + y * P];
-    const R dudy = u[x + (y + 1) * P] - u[x + y * P];
-    qend[y] = (fq[x * fq_sx + y * fq_sy] + dvdx - dudy) / hq;
+    const R dvdx = v.ld(c1, r0) - v.ld(c0, r0);
+    const R dudy = u.ld(c0, r1) - u.ld(c0, r0);
+    const R f = fq.ld(scolb<R>(x, A.st.fq_2d), srowb<R>(y, P, A.st.fq_2d));
+    qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(R));
 }
 
 }  // namespace swm

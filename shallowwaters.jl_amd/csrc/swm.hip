@@ -33,9 +33,11 @@ using namespace swm;
 
 namespace {
 
-struct Field {             // one padded device array
-    void *base = nullptr;  // allocation
+struct Field {             // one padded device array, carved from the context's arena
+    void *base = nullptr;  // start of the field
     char *p0 = nullptr;    // element (0,0)
+    uint32_t origin = 0;   // byte offset of base in the arena
+    uint32_t ref() const { return origin; }
 };
 
 struct Slots {             // where u0 / u1 live after a run of stages
@@ -63,6 +65,9 @@ struct swm_ctx {
     double *dpart = nullptr;
     bool have_static = false, have_prog = false;
     std::vector<void *> allocs;
+    void *arena = nullptr;         // every field of the context; kernels address it through one buffer descriptor
+    size_t arena_bytes = 0, arena_used = 0;
+    ArenaRef aref() const { return ArenaRef{arena, (uint32_t)arena_bytes}; }
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -88,16 +93,21 @@ int pick(std::initializer_list<int> busy) {
     return -1;
 }
 
+// Carve a field from the arena (zeroed at creation).  A 2-D field has P*rows elements; a
+// 1-D per-row field `elems_1d`.
 int alloc_field(swm_ctx *ctx, Field &f, size_t elems_1d, bool two_d) {
     const size_t n = two_d ? ctx->elems : elems_1d;
-    void *p = nullptr;
-    HIPCHK(hipMalloc(&p, n * ctx->es));
-    HIPCHK(hipMemsetAsync(p, 0, n * ctx->es, ctx->stream));
-    ctx->allocs.push_back(p);
-    f.base = p;
-    f.p0 = (char *)p + (two_d ? ((size_t)OX + (size_t)OY * ctx->P) : (size_t)OY) * ctx->es;
+    const size_t at = (ctx->arena_used + 255) & ~(size_t)255;
+    if (at + n * ctx->es > ctx->arena_bytes) return ctx->fail(SWM_E_ARG, "arena exhausted");
+    ctx->arena_used = at + n * ctx->es;
+    f.base = (char *)ctx->arena + at;
+    f.origin = (uint32_t)at;
+    f.p0 = (char *)f.base + (two_d ? ((size_t)OX + (size_t)OY * ctx->P) : (size_t)OY) * ctx->es;
     return SWM_OK;
 }
+
+// the 1-D per-row layout of a field carved as 2-D (x-invariant static fields)
+void as_rows(swm_ctx *ctx, Field &f) { f.p0 = (char *)f.base + (size_t)OY * ctx->es; }
 
 // element (x, y) of a 2-D field, as a byte pointer
 inline char *elem(const swm_ctx *ctx, const Field &f, int x, int y) {
@@ -124,8 +134,7 @@ int put_static(swm_ctx *ctx, Field &f, int *sx, int *sy, const void *host, int m
         for (int i = 1; i < m && xinv; ++i)
             xinv = std::memcmp(h + ((size_t)j * m + i) * ctx->es, h + (size_t)j * m * ctx->es, ctx->es) == 0;
     if (xinv) {
-        int rc = alloc_field(ctx, f, (size_t)ctx->c.ny + 2 * OY + 16, false);
-        if (rc) return rc;
+        as_rows(ctx, f);
         std::vector<char> col((size_t)n * ctx->es);
         for (int j = 0; j < n; ++j) std::memcpy(&col[(size_t)j * ctx->es], h + (size_t)j * m * ctx->es, ctx->es);
         HIPCHK(hipMemcpyAsync(f.p0 + (ptrdiff_t)y0 * ctx->es, col.data(), col.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -134,30 +143,49 @@ int put_static(swm_ctx *ctx, Field &f, int *sx, int *sy, const void *host, int m
         *sy = 1;
         return SWM_OK;
     }
-    int rc = alloc_field(ctx, f, 0, true);
-    if (rc) return rc;
+    f.p0 = (char *)f.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
     *sx = 1;
     *sy = ctx->P;
     return put2d(ctx, f, host, m, n, x0, y0);
 }
 
-inline int nblocks_waves(int nwaves) { return (nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK; }
+// blocks for nwaves wavefronts, padded to a multiple of the XCD count (wave_tile's remap)
+inline int nblocks_waves(int nwaves) {
+    const int nb = (nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    return (nb + NXCD - 1) / NXCD * NXCD;
+}
 
 template <class R> struct Launch {
     static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
     static int nseg(const swm_ctx *ctx) { return (ctx->c.ny + ctx->L - 1) / ctx->L; }
-    static R *uf(swm_ctx *ctx, Field *f, int s) { return s < 0 ? nullptr : ctx->at<R>(f[s]); }
 
-    static Static<R> statics(const swm_ctx *ctx) {
-        Static<R> s;
-        s.H = ctx->at<R>(ctx->H);
-        s.Fx = ctx->at<R>(ctx->Fx); s.Fx_sx = ctx->Fx_sx; s.Fx_sy = ctx->Fx_sy;
-        s.Fy = ctx->at<R>(ctx->Fy); s.Fy_sx = ctx->Fy_sx; s.Fy_sy = ctx->Fy_sy;
-        s.fq = ctx->at<R>(ctx->fq); s.fq_sx = ctx->fq_sx; s.fq_sy = ctx->fq_sy;
-        s.eta_ref = ctx->at<R>(ctx->eta_ref);
-        s.Feta = ctx->at<R>(ctx->Feta);
+    // Rows per wave segment.  Each segment re-reads 4 halo rows, so longer segments move
+    // fewer bytes; more waves hide more latency.  Measured at 4096^2 on MI355X, throughput
+    // is flat from about two resident waves per slot down to one, and falls off beyond, so
+    // the segment is sized for about two waves per slot, and not below 8 rows.
+    template <bool PER> static int rows_per_wave(const swm_ctx *ctx) {
+        int cus = 0, bpc = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(&stage_kernel<R, PER, M_RK_MID>),
+                                                         WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
+            bpc = 1;
+        const long slots = (long)cus * bpc * WAVES_PER_BLOCK;
+        const long ns = nstrips(ctx), ny = ctx->c.ny;
+        const long L = (ny * ns + 2 * slots - 1) / (2 * slots);
+        return (int)std::max<long>(8, std::min<long>(L, std::max<long>(ny, 8)));
+    }
+
+    static StaticRefs statics(const swm_ctx *ctx) {
+        StaticRefs s;
+        s.H = ctx->H.ref();
+        s.Fx = ctx->Fx.ref(); s.Fx_2d = ctx->Fx_sx;
+        s.Fy = ctx->Fy.ref(); s.Fy_2d = ctx->Fy_sx;
+        s.fq = ctx->fq.ref(); s.fq_2d = ctx->fq_sx;
+        s.eta_ref = ctx->eta_ref.ref();
+        s.Feta = ctx->Feta.ref();
         return s;
     }
+    static uint32_t fr(const Field *f, int s) { return s < 0 ? 0u : f[s].ref(); }
 
     struct Times { R Fxt, Fyt, Fet; };
     static Times forcing_times(const swm_ctx *ctx, int64_t t) {     // Ftime, forcing.jl:215-217
@@ -173,12 +201,13 @@ template <class R> struct Launch {
                              const Times &T) {
         const swm_config &c = ctx->c;
         StageArgs<R> A;
-        A.in_u = uf(ctx, ctx->u, in.uv); A.in_v = uf(ctx, ctx->v, in.uv); A.in_e = uf(ctx, ctx->e, in.e);
-        A.pr_u = uf(ctx, ctx->u, pr.uv); A.pr_v = uf(ctx, ctx->v, pr.uv); A.pr_e = uf(ctx, ctx->e, pr.e);
-        A.ac_u = uf(ctx, ctx->u, ac.uv); A.ac_v = uf(ctx, ctx->v, ac.uv); A.ac_e = uf(ctx, ctx->e, ac.e);
-        A.o1_u = uf(ctx, ctx->u, o1.uv); A.o1_v = uf(ctx, ctx->v, o1.uv); A.o1_e = uf(ctx, ctx->e, o1.e);
-        A.o2_u = uf(ctx, ctx->u, o2.uv); A.o2_v = uf(ctx, ctx->v, o2.uv); A.o2_e = uf(ctx, ctx->e, o2.e);
-        A.qend = ctx->at<R>(ctx->qend);
+        A.ar = ctx->aref();
+        A.in_u = fr(ctx->u, in.uv); A.in_v = fr(ctx->v, in.uv); A.in_e = fr(ctx->e, in.e);
+        A.pr_u = fr(ctx->u, pr.uv); A.pr_v = fr(ctx->v, pr.uv); A.pr_e = fr(ctx->e, pr.e);
+        A.ac_u = fr(ctx->u, ac.uv); A.ac_v = fr(ctx->v, ac.uv); A.ac_e = fr(ctx->e, ac.e);
+        A.o1_u = fr(ctx->u, o1.uv); A.o1_v = fr(ctx->v, o1.uv); A.o1_e = fr(ctx->e, o1.e);
+        A.o2_u = fr(ctx->u, o2.uv); A.o2_v = fr(ctx->v, o2.uv); A.o2_e = fr(ctx->e, o2.e);
+        A.qend = ctx->qend.ref();
         A.st = statics(ctx);
         A.k = k;
         A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
@@ -214,8 +243,7 @@ template <class R> struct Launch {
         if (!slots_ok(m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
         StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
         if (per)
-            qend_kernel<R><<<(c.ny + 1 + 255) / 256, 256, 0, ctx->stream>>>(
-                A.in_u, A.in_v, A.in_e, A.st.H, A.st.fq, A.st.fq_sx, A.st.fq_sy, ctx->at<R>(ctx->qend), c.nx, c.ny, ctx->P);
+            qend_kernel<R><<<(c.ny + 1 + 255) / 256, 256, 0, ctx->stream>>>(A);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
@@ -330,9 +358,10 @@ template <class R> struct Launch {
     static int dissipation(swm_ctx *ctx, Slots in, int out_uv) {
         const swm_config &c = ctx->c;
         DissArgs<R> A;
-        A.u0 = ctx->at<R>(ctx->u[in.uv]); A.v0 = ctx->at<R>(ctx->v[in.uv]); A.e0 = ctx->at<R>(ctx->e[in.e]);
-        A.out_u = ctx->at<R>(ctx->u[out_uv]); A.out_v = ctx->at<R>(ctx->v[out_uv]);
-        A.H = ctx->at<R>(ctx->H);
+        A.ar = ctx->aref();
+        A.u0 = ctx->u[in.uv].ref(); A.v0 = ctx->v[in.uv].ref(); A.e0 = ctx->e[in.e].ref();
+        A.out_u = ctx->u[out_uv].ref(); A.out_v = ctx->v[out_uv].ref();
+        A.H = ctx->H.ref();
         A.cD = (R)c.cD; A.rD = (R)c.rD; A.cS = (R)c.cSmag; A.nuB = (R)c.nuB; A.dtd = (R)c.Dt_diff;
         A.oma = (R)c.one_minus_alpha;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
@@ -578,20 +607,34 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     ctx->P = ((c.nx + 2 * OX + 63) / 64) * 64;
     ctx->rows = c.ny + 2 * OY + 4;
     ctx->elems = (size_t)ctx->P * ctx->rows;
-    ctx->L = c.ny >= 2048 ? 64 : 32;
-    if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {   // tuning knob: rows each wavefront marches
-        const int l = atoi(s);
-        if (l >= 4 && l <= 4096) ctx->L = l;
-    }
     if (hipGetDevice(&ctx->dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         fprintf(stderr, "swm_create: no HIP device/stream\n");
         delete ctx;
         return SWM_E_HIP;
     }
+    ctx->L = c.T == SWM_F64 ? (c.bc == SWM_PERIODIC ? Launch<double>::rows_per_wave<true>(ctx) : Launch<double>::rows_per_wave<false>(ctx))
+                            : (c.bc == SWM_PERIODIC ? Launch<float>::rows_per_wave<true>(ctx) : Launch<float>::rows_per_wave<false>(ctx));
+    if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {   // tuning override: rows each wavefront marches
+        const int l = atoi(s);
+        if (l >= 4 && l <= 4096) ctx->L = l;
+    }
     auto bail = [&](int code) {
         swm_destroy(ctx);
         return code;
     };
+    // the arena: 12 state slots, 2 sst, um, vm, H, Fx, Fy, f_q (+ η_ref, Fη) and the q column
+    const int n2d = 22 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0);
+    const size_t n1d = (size_t)c.ny + 2 * OY + 16;
+    ctx->arena_bytes = (size_t)n2d * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256;
+    if (ctx->arena_bytes > 0xFFFF0000ull) {     // byte offsets into the arena are 32-bit
+        ctx->fail(SWM_E_UNSUPPORTED, "grid %dx%d needs %zu bytes of device fields; one context addresses at most 4 GiB",
+                  c.nx, c.ny, ctx->arena_bytes);
+        fprintf(stderr, "swm_create: %s\n", ctx->err.c_str());
+        return bail(SWM_E_UNSUPPORTED);
+    }
+    if (hipMalloc(&ctx->arena, ctx->arena_bytes) != hipSuccess) return bail(SWM_E_HIP);
+    ctx->allocs.push_back(ctx->arena);
+    if (hipMemsetAsync(ctx->arena, 0, ctx->arena_bytes, ctx->stream) != hipSuccess) return bail(SWM_E_HIP);
     for (int s = 0; s < 4; ++s) {
         if ((rc = alloc_field(ctx, ctx->u[s], 0, true)) || (rc = alloc_field(ctx, ctx->v[s], 0, true)) ||
             (rc = alloc_field(ctx, ctx->e[s], 0, true)))
@@ -599,7 +642,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     }
     if ((rc = alloc_field(ctx, ctx->sst[0], 0, true)) || (rc = alloc_field(ctx, ctx->sst[1], 0, true)) ||
         (rc = alloc_field(ctx, ctx->um, 0, true)) || (rc = alloc_field(ctx, ctx->vm, 0, true)) ||
-        (rc = alloc_field(ctx, ctx->H, 0, true)) || (rc = alloc_field(ctx, ctx->qend, (size_t)c.ny + 2 * OY + 16, false)))
+        (rc = alloc_field(ctx, ctx->H, 0, true)) || (rc = alloc_field(ctx, ctx->Fx, 0, true)) ||
+        (rc = alloc_field(ctx, ctx->Fy, 0, true)) || (rc = alloc_field(ctx, ctx->fq, 0, true)) ||
+        (rc = alloc_field(ctx, ctx->qend, n1d, false)))
         return bail(rc);
     if (c.continuity != SWM_CONT_ITSELF &&
         ((rc = alloc_field(ctx, ctx->eta_ref, 0, true)) || (rc = alloc_field(ctx, ctx->Feta, 0, true))))
