@@ -58,6 +58,18 @@ def cpu_baseline(swm, orc, sample_steps: int, nx: int):
             "seconds": dt}
 
 
+def measured_traffic(workload: str, dtype: str):
+    """HBM bytes per launch of the middle RK stage from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs kept in profiles/ (KiB per dispatch, each scaled by what the counter reads for a
+    plain copy of known size); None when no measurement of this workload is recorded."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{dtype}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    return 1024.0 * (m["fetch_size_kib"] * m["fetch_scale"] + m["write_size_kib"] * m["write_scale"])
+
+
 def copy_probe(mib: int, reps: int = 20):
     """A plain copy of `mib` MiB (read mib, write mib), timed with events: the calibration
     kernel for the HBM traffic counters and the practical copy bandwidth of the device."""
@@ -124,6 +136,7 @@ def main():
         mass, energy = sim.reduce()
 
     value = cells * args.steps / elapsed
+    traffic = measured_traffic(f"double_gyre_{G.nx}x{G.ny}", "f32" if es == 4 else "f64")
     bpc = bytes_per_cell_stage(es)
     alg_bytes = bpc["stage_mid"] * cells
     achieved = alg_bytes / (kms["stage_mid"] * 1e-3) / 1e9
@@ -146,7 +159,7 @@ def main():
                    "bottom_drag": "quadratic", "alpha": 0.0, "time_scheme": "RK4", "dt_s": G.dtint,
                    "tracer_every": G.nadvstep, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rk_stage_kernel (middle stage)", "bytes_per_cell": bpc["stage_mid"],
                      "kernel_ms": kms["stage_mid"]},
         "step_roofline": {"alg_bytes_per_cell_step": step_bytes / cells,

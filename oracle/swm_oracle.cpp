@@ -14,10 +14,13 @@
 // after each operation (float carries >= 2p+2 significand bits for both, so + - * / sqrt
 // are correctly rounded; Julia's Float16 arithmetic is likewise Float32-then-round).
 //
-// Parity pins (tests/test_oracle_*.py): the exact-zero known answer of test/runtests.jl:4-9,
-// closed forms of every stencil, exact mass conservation of continuity, x-shift
-// equivariance, and hand-derived single-cell values.  Julia is not installed here, so the
-// reference itself cannot be run; see the tests for what each pin covers.
+// Parity pins (tests/test_oracle.py, tests/test_setup.py): the exact-zero known answer of
+// test/runtests.jl:4-9, closed forms of every stencil, rest-state tendencies equal to the
+// wind, uniform zonal flow in Coriolis balance, exact mass conservation of continuity,
+// x-shift equivariance away from the periodic seam (where the reference's AHβ!/AHγ! wrap
+// breaks it), exact tracer shifts and bilinear reproduction of linear fields, and IEEE
+// binary16 / bfloat16 rounding.  Julia is not installed here, so the reference itself
+// cannot be run; see the tests for what each pin covers.
 #include <cmath>
 #include <cstdint>
 #include <cstring>

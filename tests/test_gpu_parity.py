@@ -53,6 +53,10 @@ CASES = {
                                      seasonal_wind_x=True, ωFx=400.0, A=3e-3),
     "np-f32-ssprk2-relax": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", surface_relax=True, t_relax=2.0,
                                 time_scheme="SSPRK2", RKs=3, cfl=1.2),
+    # tracer consumption (tracer_advection.jl:262-275)
+    "np-f32-consumption": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", tracer_consumption=True,
+                               topography="seamount", wind_forcing_x="double_gyre"),
+    "per-f64-consumption": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", tracer_consumption=True),
 }
 
 
@@ -189,3 +193,48 @@ def test_unsupported_options_fail_loudly(swm, gpu):
         S = swm.make_setup(P)
         with pytest.raises(Exception):
             swm.Simulation(S)
+
+
+def test_x_shift_equivariance_gpu(swm, gpu):
+    """The GPU step commutes with a circular x shift of the periodic flat channel away from
+    the seams (tests/test_oracle.py::check_shift_equivariance)."""
+    from test_oracle import check_shift_equivariance, shifted_channel
+    k = 48
+    P, S, prog, shifted = shifted_channel(swm, k)
+    out = []
+    for p in (prog, shifted):
+        with swm.Simulation(S) as sim:
+            sim.upload(p)
+            assert sim.step(1, 1, 0) == (1, 0)
+            g = sim.download()
+        out.append((g.u, g.v, g.η, g.sst))
+    check_shift_equivariance(swm, S, out[0], out[1], k)
+
+
+@pytest.mark.parametrize("case", ["np-f64-smag-quad", "per-f32-const-none"])
+def test_reduce_matches_fp64_sums(swm, gpu, case):
+    """swm_reduce: FP64 mass and energy of the current Prog against numpy FP64 sums over
+    the downloaded fields (mass = Σ η, energy = Σ ½gη² + ½h u² + ½h v² per T-cell with
+    u at the cell's right face, v at its top face, h = η + H)."""
+    P, S, prog = setup_case(swm, case)
+    G, F = S.grid, S.forcing
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        sim.step(3, 1, 0)
+        mass, energy = sim.reduce()
+        g = sim.download()
+    T = P.Tprog
+    u, v, e = (swm.to_float64(a, T) for a in (g.u, g.v, g.η))
+    H = swm.to_float64(F.H, P.T)
+    ep = 1 if P.bc == "periodic" else 0
+    eta = e[1:G.nx + 1, 1:G.ny + 1]
+    h = eta + H[1:G.nx + 1, 1:G.ny + 1]
+    # Julia u[x+2+ep, y+2] and v[x+2, y+2] are the faces of T-cell (x, y)
+    uf = u[1 + ep + 1:1 + ep + 1 + G.nx, 2:2 + G.ny]                # x = 1..nx (x = nx: zero wall halo if ep = 0)
+    vf = np.zeros_like(eta)
+    vf[:, :G.ny - 1] = v[2:2 + G.nx, 2:2 + G.ny - 1]                # y = 1..ny-1
+    g_ = float(swm.make_config(S).g)
+    m_ref = eta.sum()
+    e_ref = (0.5 * g_ * eta * eta + 0.5 * h * uf * uf + 0.5 * h * vf * vf).sum()
+    assert abs(mass - m_ref) <= 1e-12 * np.abs(eta).sum(), (mass, m_ref)
+    assert abs(energy - e_ref) <= 1e-12 * abs(e_ref), (energy, e_ref)

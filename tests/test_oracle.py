@@ -166,6 +166,56 @@ def test_mass_is_conserved(swm, orc, bc, topo):
     assert rel_l2(prog.η, r[2]) > 1e-2
 
 
+def shifted_channel(swm, k, nx=96, seed=3):
+    """Periodic channel on a flat bottom: every static field (H, Fx(y), Fy, f_q(y)) is
+    x-invariant.  Returns the seeded state and the same state circularly shifted by k
+    columns."""
+    P = swm.Parameter(T="Float64", nx=nx, L_ratio=3, bc="periodic", topography="flat", wind_forcing_x="channel",
+                      diffusion="Smagorinsky", bottom_drag="quadratic")
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=seed, σ=0.6)
+    ui, vi, ei, si = swm.remove_halo(prog.u, prog.v, prog.η, prog.sst, S)
+    roll = lambda a: np.roll(a, k, axis=0)
+    U, V, E, SS = swm.add_halo(roll(ui), roll(vi), roll(ei), roll(si), S, T="Float64")
+    shifted = swm.PrognosticVars(u=U, v=V, η=E, sst=SS, T="Float64")
+    return P, S, prog, shifted
+
+
+def check_shift_equivariance(swm, S, a, b, k, far=20):
+    """One step from the shifted state equals the shifted step, bitwise, in every column
+    at least `far` columns from both periodic seams; near a seam it does not.  The
+    reference's AHβ!/AHγ! take the left neighbour of their first column from q[end,·]
+    (PV_advection.jl:110,126), the corner x = nx rather than x = nx-1, so the seam is not
+    translation invariant.  Any other position-dependent error (a wrong ghost copy, a
+    mishandled strip or segment boundary of the GPU tiling) breaks the far columns."""
+    nx = S.grid.nx
+    ai = swm.remove_halo(*a, S)
+    bi = swm.remove_halo(*b, S)
+    # interior column c of u is x = c (ep = 1), of v, η, sst x = c + 1; seams at x = 0 and x = -k
+    seams = (0, (-k) % nx)
+    near = 0
+    for name, x, y, off in zip(("u", "v", "η"), ai, bi, (0, 1, 1)):
+        want = np.roll(x, k, axis=0)
+        for c in range(nx):
+            xo = (c + off - k) % nx                     # column in the unshifted frame
+            dist = min(min((xo - s0) % nx, (s0 - xo) % nx) for s0 in seams)
+            same = np.array_equal(want[c], y[c])
+            if dist >= far:
+                assert same, (name, c, xo, dist)
+            near += not same
+    assert near > 0        # the seam quirk is reproduced, not smoothed away
+    return near
+
+
+def test_x_shift_equivariance(swm, orc):
+    k = 48
+    P, S, prog, shifted = shifted_channel(swm, k)
+    cfg = swm.make_config(S)
+    a = orc.run(S, cfg, prog, 1)[:4]
+    b = orc.run(S, cfg, shifted, 1)[:4]
+    check_shift_equivariance(swm, S, a, b, k)
+
+
 # ------------------------------------------------------------------ tracer (tracer_advection.jl)
 
 def _tracer_setup(swm, bc):
