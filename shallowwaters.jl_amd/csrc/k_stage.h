@@ -15,7 +15,7 @@ namespace swm {
 //   SSP3_KN    o1 = x·w + y·ac + z·d                             dxaybzc!  :129
 //   S4         t = w + c·d,  o1 = h·(w + t)                      caxb!(u0,u1,Δt_Δ,du), cxab!  :172-175
 //   S4_3       ... and o2 = x·pr + y·o1                          cxayb!(u1,2/3,u,1/3,u1)  :187
-// w is the stage input at the point, pr the Prog value, ac the u0 value.
+// w is the stage input at the point, pr the Prog value, ac the u0 value; all in Tprog.
 enum { M_RK_FIRST = 0, M_RK_MID, M_RK_LAST, M_SSP2, M_SSP2_LAST, M_SSP3, M_SSP3_KN, M_S4, M_S4_3 };
 
 template <int M> struct Mode {
@@ -28,23 +28,28 @@ template <int M> struct Mode {
     static constexpr bool ghost2 = M != M_RK_FIRST && M != M_RK_MID;    // u0 between RK stages: interior only
 };
 
-template <class R> struct Coef {
-    R a, b, c, x, y, z, h;
+template <class Q> struct Coef {
+    Q a, b, c, x, y, z, h;
 };
 
-template <int M, class R>
-__device__ __forceinline__ void combine(R w, R pr, R ac, R d, const Coef<R> &k, R &o1, R &o2) {
+template <int M, class Q>
+__device__ __forceinline__ void combine(Q w, Q pr, Q ac, Q d, const Coef<Q> &k, Q &o1, Q &o2) {
     if constexpr (M == M_RK_FIRST) { o1 = w + k.b * d; o2 = w + k.a * d; }
     else if constexpr (M == M_RK_MID) { o1 = pr + k.b * d; o2 = ac + k.a * d; }
     else if constexpr (M == M_RK_LAST) { o2 = ac + k.a * d; }
     else if constexpr (M == M_SSP2 || M == M_SSP3) { o1 = w + k.c * d; }
     else if constexpr (M == M_SSP2_LAST) { o1 = w + k.c * d; o2 = k.x * pr + k.y * o1; }
     else if constexpr (M == M_SSP3_KN) { o1 = k.x * w + k.y * ac + k.z * d; }
-    else if constexpr (M == M_S4) { const R t = w + k.c * d; o1 = k.h * (w + t); }
-    else { const R t = w + k.c * d; o1 = k.h * (w + t); o2 = k.x * pr + k.y * o1; }
+    else if constexpr (M == M_S4) { const Q t = w + k.c * d; o1 = k.h * (w + t); }
+    else { const Q t = w + k.c * d; o1 = k.h * (w + t); o2 = k.x * pr + k.y * o1; }
 }
 
-template <class R> struct StageArgs {
+// Field formats: Prog, u0 and the second output live in Tprog (Q); the static fields and the
+// stage input u1 between RK stages in T (ST: the reference converts u1 to T before rhs!,
+// time_integration.jl:62-64, and uses it for nothing else).  The first stage reads Prog.
+template <class F> struct StageArgs {
+    using R = typename F::R;
+    using Q = typename F::Q;
     ArenaRef ar;                      // the context's fields; the members below are origins in it
     uint32_t in_u, in_v, in_e;        // stage input (u1 of the reference; Prog at stage 1)
     uint32_t pr_u, pr_v, pr_e;        // Prog
@@ -53,19 +58,22 @@ template <class R> struct StageArgs {
     uint32_t o2_u, o2_v, o2_e;        // second output (u0 / u1'')
     uint32_t qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap), 1-D
     StaticRefs st;
-    Coef<R> k;
-    R Fxt, Fyt, Fet, g, oma, gamma;
-    int cont;
+    Coef<Q> k;
+    R Fxt, Fyt, Fet, g, gamma, k12;   // T scalars (k12 = T(1/12))
+    Q oma;                            // one_minus_α (Tprog)
+    int cont, cm;                     // continuity variant, Tcomm
     int nx, ny, P, L, nstrips, nseg;
 };
 
+template <class F, int M> using StageIn = std::conditional_t<Mode<M>::first, typename F::Q, typename F::ST>;
+
 // dη at T-point (x, y) from the flux divergence, continuity.jl:3-68
-template <class R>
-__device__ __forceinline__ R continuity_rhs(R dUdx, R dVdy, R e, int cont, const Fld<R> &eref, const Fld<R> &feta,
+template <class Q, class R, class ST>
+__device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, const Fld<ST> &eref, const Fld<ST> &feta,
                                             uint32_t vo, uint32_t so, R gamma, R Fet) {
-    R d = -(dUdx + dVdy);
-    if (cont == CONT_RELAX) d = d + gamma * (eref.ld(vo, so) - e);
-    else if (cont == CONT_FORCING) d = d + Fet * feta.ld(vo, so);
+    Q d = -(toQ<Q>(dUdx) + toQ<Q>(dVdy));
+    if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (eref.template ldr<R>(vo, so) - e));
+    else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * feta.template ldr<R>(vo, so));
     return d;
 }
 
@@ -77,9 +85,14 @@ __device__ __forceinline__ R continuity_rhs(R dUdx, R dVdy, R e, int cont, const
 // ghost copies of what the next stage reads.  At input row r the wave forms U(r), then the
 // row-(r-1) quantities h_v, V, h_q, q, KE, p and the Arakawa-Hsu weights, and emits du (and
 // dη) at row r-1 and dv at row r-2.
-template <class R, bool PER, int M>
-__global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
+template <class F, bool PER, int M>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
     using MI = Mode<M>;
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+    using SI = StageIn<F, M>;
+    using VI = typename Sto<SI>::V;
     const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
@@ -90,21 +103,25 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
     const bool qfix = PER && x == 0;
     const GhostLanes gl = ghost_lanes<PER>(x, nx);
 
-    const Arena<R> ar(A.ar);
-    const Fld<R> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e}, fH{&ar, A.st.H};
-    const Fld<R> pr_u{&ar, A.pr_u}, pr_v{&ar, A.pr_v}, pr_e{&ar, A.pr_e};
-    const Fld<R> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
-    const Fld<R> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
-    const Fld<R> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
-    const Fld<R> fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy}, fqend{&ar, A.qend};
-    const Fld<R> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
-    const uint32_t vx = colb<R>(x), dso = (uint32_t)P * sizeof(R);
-    const uint32_t vfq = scolb<R>(x, A.st.fq_2d), vFx = scolb<R>(x, A.st.Fx_2d), vFy = scolb<R>(x, A.st.Fy_2d);
+    const Arena ar(A.ar);
+    const Fld<SI> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e};
+    const Fld<Q> pr_u{&ar, A.pr_u}, pr_v{&ar, A.pr_v}, pr_e{&ar, A.pr_e};
+    const Fld<Q> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
+    const Fld<ST> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
+    const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
+    const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy}, fqend{&ar, A.qend};
+    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    // column offsets and row pitches of the three storage formats (equal when T == Tprog)
+    const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
+    const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
+    const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
 
-    const R half = R(0.5), quarter = R(0.25), k12 = R(1.0 / 12.0);
+    const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
 
-    // row history (suffix _1 = r-1, _2 = r-2)
+    // row history (suffix _1 = r-1, _2 = r-2); the w operands of the combination keep the
+    // stage input in its own format
     R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
+    VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;
     R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
     R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
 
@@ -113,23 +130,25 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
     // while row r is computed.
     int r = t.ya - 1;
     const int rlast = t.yb + 1;
-    uint32_t so = rowb<R>(r, P);                               // byte offset of row r
-    R nu = in_u.ld(vx, so), nv = in_v.ld(vx, so), ne = in_e.ld(vx, so), nH = fH.ld(vx, so);
-    R npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
-    for (; r <= rlast; ++r, so += dso) {
-        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH;
-        const R pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+    uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
+    VI nu = in_u.ld(vxI, sI), nv = in_v.ld(vxI, sI), ne = in_e.ld(vxI, sI);
+    R nH = fH.template ldr<R>(vxT, sT);
+    Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
+    for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
+        const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
+        const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
+        const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         if (r < rlast) {
-            const uint32_t sn = so + dso;
-            nu = in_u.ld(vx, sn); nv = in_v.ld(vx, sn); ne = in_e.ld(vx, sn); nH = fH.ld(vx, sn);
+            nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
+            nH = fH.template ldr<R>(vxT, sT + dT);
         }
         if (r >= t.ya && r < t.yb) {                          // row r of u, η is emitted next iteration
-            if (MI::need_pr) { npu = pr_u.ld(vx, so); if (MI::rk) npe = pr_e.ld(vx, so); }
-            if (MI::need_ac) { nau = ac_u.ld(vx, so); if (MI::rk) nae = ac_e.ld(vx, so); }
+            if (MI::need_pr) { npu = pr_u.ld(vxQ, sQ); if (MI::rk) npe = pr_e.ld(vxQ, sQ); }
+            if (MI::need_ac) { nau = ac_u.ld(vxQ, sQ); if (MI::rk) nae = ac_e.ld(vxQ, sQ); }
         }
         if (r - 1 >= t.ya && r - 1 < t.yb) {                  // row r-1 of v is emitted next iteration
-            if (MI::need_pr) npv = pr_v.ld(vx, so - dso);
-            if (MI::need_ac) nav = ac_v.ld(vx, so - dso);
+            if (MI::need_pr) npv = pr_v.ld(vxQ, sQ - dQ);
+            if (MI::need_ac) nav = ac_v.ld(vxQ, sQ - dQ);
         }
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
@@ -142,7 +161,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
         const R dvdx_1 = sh_xp(v_1) - v_1;
         const R dudy_1 = u_0 - u_1;
-        const R q_1 = (fq.ld(vfq, srowb<R>(r - 1, P, A.st.fq_2d)) + dvdx_1 - dudy_1) / hq_1;
+        const R q_1 = (fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d)) + dvdx_1 - dudy_1) / hq_1;
         // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
         const R uu_1 = u_1 * u_1;
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
@@ -153,7 +172,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         R qxm_1 = sh_xm(q_1);
         R qend_1 = 0;
         if (PER) {
-            qend_1 = fqend.ld(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(R));
+            qend_1 = fqend.template ldr<R>(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(ST));
             if (qfix) qxm_1 = qend_1;
         }
         const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
@@ -167,49 +186,50 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
         const R Vxp_1 = sh_xp(V_1);
         const R p_xp1 = sh_xp(p_1);
         const int yu = r - 1;
-        const uint32_t so1 = so - dso;
         const bool row1 = yu >= t.ya && yu < t.yb;
         if (u_col && row1) {
             const R qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
-            const R du = (qhv - (p_xp1 - p_1)) + A.Fxt * fFx.ld(vFx, srowb<R>(yu, P, A.st.Fx_2d));
-            R o1 = 0, o2 = 0;
-            combine<M>(u_1, MI::first ? u_1 : pu_1, au_1, du, A.k, o1, o2);
-            if (MI::out1) put_u<R, PER>(o1_u, x, vx, yu, so1, o1, gl.u, nx, ny, P, A.oma);
+            const R Fx = fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d));
+            const Q du = (toQ<Q>(qhv) - toQ<Q>(p_xp1 - p_1)) + toQ<Q>(A.Fxt * Fx);
+            Q o1 = 0, o2 = 0;
+            combine<M>((Q)wu_1, MI::first ? (Q)wu_1 : pu_1, au_1, du, A.k, o1, o2);
+            if (MI::out1) put_u<ST, PER>(o1_u, x, vxT, yu, sT - dT, o1, gl.u, nx, ny, P, A.oma, A.cm);
             if (MI::out2) {
-                if (MI::ghost2) put_u<R, PER>(o2_u, x, vx, yu, so1, o2, gl.u, nx, ny, P, A.oma);
-                else o2_u.st(o2, vx, so1);
+                if (MI::ghost2) put_u<Q, PER>(o2_u, x, vxQ, yu, sQ - dQ, o2, gl.u, nx, ny, P, A.oma, A.cm);
+                else o2_u.st(o2, vxQ, sQ - dQ);
             }
         }
         // dη at row r-1: ∂x!(dUdx,U), ∂y!(dVdy,V), continuity!  continuity.jl:52-98
         const R Uxm_1 = sh_xm(U_1);
         if (MI::rk && e_col && row1) {
-            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vx, so1, A.gamma, A.Fet);
-            R o1 = 0, o2 = 0;
-            combine<M>(e_1, MI::first ? e_1 : pe_1, ae_1, deta, A.k, o1, o2);
-            if (MI::out1) put_e<R, PER>(o1_e, x, vx, yu, so1, o1, gl.e, nx, ny, P);
+            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
+            Q o1 = 0, o2 = 0;
+            combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2);
+            if (MI::out1) put_e<ST, PER>(o1_e, x, vxT, yu, sT - dT, o1, gl.e, nx, ny, P, A.cm);
             if (MI::out2) {
-                if (MI::ghost2) put_e<R, PER>(o2_e, x, vx, yu, so1, o2, gl.e, nx, ny, P);
-                else o2_e.st(o2, vx, so1);
+                if (MI::ghost2) put_e<Q, PER>(o2_e, x, vxQ, yu, sQ - dQ, o2, gl.e, nx, ny, P, A.cm);
+                else o2_e.st(o2, vxQ, sQ - dQ);
             }
         }
         // dv at row r-2: PV_ArakawaHsu! qhu  PV_advection.jl:81 ; momentum_v!  rhs.jl:271
         const R qdxm_1 = sh_xm(qd_1);
         const int yv = r - 2;
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
-            const uint32_t so2 = so - 2 * dso;
             const R qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
-            const R dv = -(qhu + (p_1 - p_2)) + A.Fyt * fFy.ld(vFy, srowb<R>(yv, P, A.st.Fy_2d));
-            R o1 = 0, o2 = 0;
-            combine<M>(v_2, MI::first ? v_2 : pv_2, av_2, dv, A.k, o1, o2);
-            if (MI::out1) put_v<R, PER>(o1_v, x, vx, yv, so2, o1, gl.v, nx, P, A.oma);
+            const R Fy = fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d));
+            const Q dv = -(toQ<Q>(qhu) + toQ<Q>(p_1 - p_2)) + toQ<Q>(A.Fyt * Fy);
+            Q o1 = 0, o2 = 0;
+            combine<M>((Q)wv_2, MI::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, o1, o2);
+            if (MI::out1) put_v<ST, PER>(o1_v, x, vxT, yv, sT - 2 * dT, o1, gl.v, nx, P, A.oma, A.cm);
             if (MI::out2) {
-                if (MI::ghost2) put_v<R, PER>(o2_v, x, vx, yv, so2, o2, gl.v, nx, P, A.oma);
-                else o2_v.st(o2, vx, so2);
+                if (MI::ghost2) put_v<Q, PER>(o2_v, x, vxQ, yv, sQ - 2 * dQ, o2, gl.v, nx, P, A.oma, A.cm);
+                else o2_v.st(o2, vxQ, sQ - 2 * dQ);
             }
         }
         // advance the row history
         const R qaxm_1 = sh_xm(qa_1);
         u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hx_1 = hx_0;
+        wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
         U_2 = U_1; U_1 = U_0; Uxm_2 = Uxm_1; V_2 = V_1; Vxp_2 = Vxp_1;
         q_2 = q_1; qxp_2 = qxp_1; qxm_2 = qxm_1; qaxm_2 = qaxm_1; qb_2 = qb_1; p_2 = p_1;
         qend_2 = qend_1;
@@ -220,36 +240,45 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<R> A) {
 //
 // continuity! of the SSPRK schemes (continuity.jl:72-98 with UVfluxes! recomputed from the
 // updated u1, v1 and the stage's η1, :83-85), followed by the η combination of the scheme
-// (time_integration.jl:103,143-146,183-189) and the η ghost copies.  Emits row r-1.
-template <class R, bool PER, int M>
-__global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
+// (time_integration.jl:103,143-146,183-189) and the η ghost copies.  Emits row r-1.  The
+// SSPRK stage variables are Tprog throughout.
+template <class F, bool PER, int M>
+__global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
     using MI = Mode<M>;
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
     const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
     const bool e_col = t.lane >= HALO && t.lane < WAVE - HALO && x >= 1 && x <= nx;
     const GhostLanes gl = ghost_lanes<PER>(x, nx);
-    const Arena<R> ar(A.ar);
-    const Fld<R> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e}, fH{&ar, A.st.H};
-    const Fld<R> pr_e{&ar, A.pr_e}, ac_e{&ar, A.ac_e}, o1_e{&ar, A.o1_e}, o2_e{&ar, A.o2_e};
-    const Fld<R> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
-    const uint32_t vx = colb<R>(x), dso = (uint32_t)P * sizeof(R);
+    const Arena ar(A.ar);
+    const Fld<Q> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e};
+    const Fld<ST> fH{&ar, A.st.H};
+    const Fld<Q> pr_e{&ar, A.pr_e}, ac_e{&ar, A.ac_e}, o1_e{&ar, A.o1_e}, o2_e{&ar, A.o2_e};
+    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);
+    const uint32_t dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     const R half = R(0.5);
 
     R v_1 = 0, e_1 = 0, h_1 = 0, U_1 = 0, V_2 = 0;
+    Q we_1 = 0;
     int r = t.ya - 1;
     const int rlast = t.yb;
-    uint32_t so = rowb<R>(r, P);
-    R nu = in_u.ld(vx, so), nv = in_v.ld(vx, so), ne = in_e.ld(vx, so), nH = fH.ld(vx, so), npe = 0, nae = 0;
-    for (; r <= rlast; ++r, so += dso) {
-        const R u_0 = nu, v_0 = nv, e_0 = ne, H_0 = nH, pe_1 = npe, ae_1 = nae;
+    uint32_t sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);
+    Q nu = in_u.ld(vxQ, sQ), nv = in_v.ld(vxQ, sQ), ne = in_e.ld(vxQ, sQ), npe = 0, nae = 0;
+    R nH = fH.template ldr<R>(vxT, sT);
+    for (; r <= rlast; ++r, sQ += dQ, sT += dT) {
+        const Q we_0 = ne, pe_1 = npe, ae_1 = nae;
+        const R u_0 = toR<R>(nu), v_0 = toR<R>(nv), e_0 = toR<R>(ne), H_0 = nH;
         if (r < rlast) {
-            const uint32_t sn = so + dso;
-            nu = in_u.ld(vx, sn); nv = in_v.ld(vx, sn); ne = in_e.ld(vx, sn); nH = fH.ld(vx, sn);
+            nu = in_u.ld(vxQ, sQ + dQ); nv = in_v.ld(vxQ, sQ + dQ); ne = in_e.ld(vxQ, sQ + dQ);
+            nH = fH.template ldr<R>(vxT, sT + dT);
         }
         if (r >= t.ya && r < t.yb) {
-            if (MI::need_pr) npe = pr_e.ld(vx, so);
-            if (MI::need_ac) nae = ac_e.ld(vx, so);
+            if (MI::need_pr) npe = pr_e.ld(vxQ, sQ);
+            if (MI::need_ac) nae = ac_e.ld(vxQ, sQ);
         }
         // UVfluxes!  rhs.jl:307-324 from the updated velocities
         const R h_0 = e_0 + H_0;
@@ -258,38 +287,44 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<R> A) {
         const R Uxm_1 = sh_xm(U_1);
         const int y = r - 1;
         if (e_col && y >= t.ya && y < t.yb) {
-            const uint32_t so1 = so - dso;
-            const R deta = continuity_rhs(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vx, so1, A.gamma, A.Fet);
-            R o1 = 0, o2 = 0;
-            combine<M>(e_1, pe_1, ae_1, deta, A.k, o1, o2);
-            if (M == M_S4_3) put_e<R, PER>(o1_e, x, vx, y, so1, o2, gl.e, nx, ny, P);   // only 2/3 η + 1/3 η1' lives on
+            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
+            Q o1 = 0, o2 = 0;
+            combine<M>(we_1, pe_1, ae_1, deta, A.k, o1, o2);
+            if (M == M_S4_3) put_e<Q, PER>(o1_e, x, vxQ, y, sQ - dQ, o2, gl.e, nx, ny, P, A.cm);   // only 2/3 η + 1/3 η1' lives on
             else {
-                put_e<R, PER>(o1_e, x, vx, y, so1, o1, gl.e, nx, ny, P);
-                if (MI::out2) put_e<R, PER>(o2_e, x, vx, y, so1, o2, gl.e, nx, ny, P);
+                put_e<Q, PER>(o1_e, x, vxQ, y, sQ - dQ, o1, gl.e, nx, ny, P, A.cm);
+                if (MI::out2) put_e<Q, PER>(o2_e, x, vxQ, y, sQ - dQ, o2, gl.e, nx, ny, P, A.cm);
             }
         }
-        v_1 = v_0; e_1 = e_0; h_1 = h_0; U_1 = U_0; V_2 = V_1;
+        v_1 = v_0; e_1 = e_0; we_1 = we_0; h_1 = h_0; U_1 = U_0; V_2 = V_1;
     }
 }
 
 // q at the corner column x = nx for every corner row y = 0..ny (PV!, PV_advection.jl:15):
 // the periodic AHβ!/AHγ! read q[end,·] for their first column (PV_advection.jl:110,126).
-template <class R>
-__global__ void qend_kernel(StageArgs<R> A) {
+// SI is the storage of the stage input.
+template <class F, class SI>
+__global__ void qend_kernel(StageArgs<F> A) {
+    using R = typename F::R;
+    using ST = typename F::ST;
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y > A.ny) return;
-    const Arena<R> ar(A.ar);
-    const Fld<R> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e}, H{&ar, A.st.H}, fq{&ar, A.st.fq}, qe{&ar, A.qend};
+    const Arena ar(A.ar);
+    const Fld<SI> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e};
+    const Fld<ST> H{&ar, A.st.H}, fq{&ar, A.st.fq}, qe{&ar, A.qend};
     const R quarter = R(0.25);
     const int x = A.nx, P = A.P;
-    const uint32_t c0 = colb<R>(x), c1 = colb<R>(x + 1), r0 = rowb<R>(y, P), r1 = rowb<R>(y + 1, P);
-    const R h00 = e.ld(c0, r0) + H.ld(c0, r0), h10 = e.ld(c1, r0) + H.ld(c1, r0);
-    const R h01 = e.ld(c0, r1) + H.ld(c0, r1), h11 = e.ld(c1, r1) + H.ld(c1, r1);
+    const uint32_t c0 = colb<SI>(x), c1 = colb<SI>(x + 1), r0 = rowb<SI>(y, P), r1 = rowb<SI>(y + 1, P);
+    const uint32_t t0 = colb<ST>(x), t1 = colb<ST>(x + 1), q0 = rowb<ST>(y, P), q1 = rowb<ST>(y + 1, P);
+    const R h00 = e.template ldr<R>(c0, r0) + H.template ldr<R>(t0, q0);
+    const R h10 = e.template ldr<R>(c1, r0) + H.template ldr<R>(t1, q0);
+    const R h01 = e.template ldr<R>(c0, r1) + H.template ldr<R>(t0, q1);
+    const R h11 = e.template ldr<R>(c1, r1) + H.template ldr<R>(t1, q1);
     const R hq = quarter * (h00 + h10) + quarter * (h01 + h11);
-    const R dvdx = v.ld(c1, r0) - v.ld(c0, r0);
-    const R dudy = u.ld(c0, r1) - u.ld(c0, r0);
-    const R f = fq.ld(scolb<R>(x, A.st.fq_2d), srowb<R>(y, P, A.st.fq_2d));
-    qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(R));
+    const R dvdx = v.template ldr<R>(c1, r0) - v.template ldr<R>(c0, r0);
+    const R dudy = u.template ldr<R>(c0, r1) - u.template ldr<R>(c0, r0);
+    const R f = fq.template ldr<R>(scolb<ST>(x, A.st.fq_2d), srowb<ST>(y, P, A.st.fq_2d));
+    qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST));
 }
 
 }  // namespace swm

@@ -7,8 +7,15 @@
 // nor a value still to be read.  Dissipation writes the new Prog into a free slot, so the
 // previous Prog stays intact until the step is accepted: when NaNs are detected the loop
 // stops before the copy-back exactly as the reference does (time_integration.jl:235-237).
+//
+// Number formats.  T = Tprog in {Float64, Float32}, or T in {Float16, BFloat16} with
+// Tprog = Float32 (the classic RK scheme).  In the mixed case the stage input u1 between
+// RK stages is held in T: the reference converts it to T before rhs! and uses it for
+// nothing else (time_integration.jl:57-64), so two extra slots of T storage replace the
+// Tprog u1 without changing a bit of the result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -37,12 +44,49 @@ struct Field {             // one padded device array, carved from the context's
     void *base = nullptr;  // start of the field
     char *p0 = nullptr;    // element (0,0)
     uint32_t origin = 0;   // byte offset of base in the arena
+    int es = 4;            // element size of its storage format
     uint32_t ref() const { return origin; }
 };
 
-struct Slots {             // where u0 / u1 live after a run of stages
+struct Slots {             // where a set (u, v, η) lives: slot indices, in the T pool if t
     int uv = -1, e = -1;
+    bool t = false;
 };
+
+// host rounding of a Float64 value to a number format (Julia's T(x), one rounding)
+float f32_round_odd_host(double x) {
+    float f = (float)x;
+    if ((double)f != x && std::isfinite(f)) {
+        uint32_t b;
+        std::memcpy(&b, &f, 4);
+        if (std::fabs((double)f) > std::fabs(x)) b -= 1u;
+        b |= 1u;
+        std::memcpy(&f, &b, 4);
+    }
+    return f;
+}
+double round_host(double x, int fmt) {
+    if (fmt == SWM_F64 || std::isnan(x)) return x;
+    if (fmt == SWM_F32) return (double)(float)x;
+    const float f = f32_round_odd_host(x);
+    if (fmt == SWM_BF16) {
+        uint32_t b;
+        std::memcpy(&b, &f, 4);
+        b += 0x7FFFu + ((b >> 16) & 1u);
+        b &= 0xFFFF0000u;
+        float r;
+        std::memcpy(&r, &b, 4);
+        return r;
+    }
+    // binary16: quantum 2^-24 below 2^-14, 2^(e-10) above; overflow to infinity
+    const double a = std::fabs((double)f);
+    if (std::isinf(f) || a >= 65520.0) return std::copysign(INFINITY, x);
+    int e;
+    std::frexp(a, &e);                             // a = m * 2^e, m in [0.5, 1)
+    const int q = std::max(e - 11, -24);           // log2 of the quantum
+    const double r = std::nearbyint(std::ldexp(a, -q));
+    return std::copysign(std::ldexp(r, q), x);
+}
 
 }  // namespace
 
@@ -51,12 +95,17 @@ struct swm_ctx {
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
-    int es = 4;            // element size of the compute format
+    int fmt = SWM_F32;     // format bundle: SWM_F64, SWM_F32 (T = Tprog), SWM_F16, SWM_BF16 (T, Tprog = F32)
+    int es = 4;            // element size of Tprog fields
+    int est = 4;           // element size of T fields
+    bool mixed = false;
     int P = 0, rows = 0;   // pitch (elements) and allocated rows of 2-D fields
     size_t elems = 0;
     int ep = 0, nux = 0, nuy = 0, nvx = 0, nvy = 0;
     int L = 64;            // rows per wave segment
-    Field u[4], v[4], e[4], sst[2], um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;
+    Field u[4], v[4], e[4], sst[2];                          // Tprog
+    Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
+    Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0;
     int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
     int rsst = 0;
@@ -69,16 +118,19 @@ struct swm_ctx {
     size_t arena_bytes = 0, arena_used = 0;
     ArenaRef aref() const { return ArenaRef{arena, (uint32_t)arena_bytes}; }
 
-    int fail(int code, const char *fmt, ...) {
+    int fail(int code, const char *fmt_, ...) {
         char buf[512];
         va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_start(ap, fmt_);
+        vsnprintf(buf, sizeof buf, fmt_, ap);
         va_end(ap);
         err = buf;
         return code;
     }
-    template <class R> R *at(const Field &f) const { return (R *)f.p0; }
+    template <class S> S *at(const Field &f) const { return (S *)f.p0; }
+    const Field &fu(Slots s) const { return s.t ? tu[s.uv] : u[s.uv]; }
+    const Field &fv(Slots s) const { return s.t ? tv[s.uv] : v[s.uv]; }
+    const Field &fe(Slots s) const { return s.t ? te[s.e] : e[s.e]; }
 };
 
 namespace {
@@ -93,57 +145,83 @@ int pick(std::initializer_list<int> busy) {
     return -1;
 }
 
-// Carve a field from the arena (zeroed at creation).  A 2-D field has P*rows elements; a
-// 1-D per-row field `elems_1d`.
-int alloc_field(swm_ctx *ctx, Field &f, size_t elems_1d, bool two_d) {
+// Carve a field from the arena (zeroed at creation).  A 2-D field has P*rows elements of
+// size es; a 1-D per-row field `elems_1d`.
+int alloc_field(swm_ctx *ctx, Field &f, size_t elems_1d, bool two_d, int es) {
     const size_t n = two_d ? ctx->elems : elems_1d;
     const size_t at = (ctx->arena_used + 255) & ~(size_t)255;
-    if (at + n * ctx->es > ctx->arena_bytes) return ctx->fail(SWM_E_ARG, "arena exhausted");
-    ctx->arena_used = at + n * ctx->es;
+    if (at + n * es > ctx->arena_bytes) return ctx->fail(SWM_E_ARG, "arena exhausted");
+    ctx->arena_used = at + n * es;
     f.base = (char *)ctx->arena + at;
     f.origin = (uint32_t)at;
-    f.p0 = (char *)f.base + (two_d ? ((size_t)OX + (size_t)OY * ctx->P) : (size_t)OY) * ctx->es;
+    f.es = es;
+    f.p0 = (char *)f.base + (two_d ? ((size_t)OX + (size_t)OY * ctx->P) : (size_t)OY) * es;
     return SWM_OK;
 }
 
 // the 1-D per-row layout of a field carved as 2-D (x-invariant static fields)
-void as_rows(swm_ctx *ctx, Field &f) { f.p0 = (char *)f.base + (size_t)OY * ctx->es; }
+void as_rows(Field &f) { f.p0 = (char *)f.base + (size_t)OY * f.es; }
 
 // element (x, y) of a 2-D field, as a byte pointer
 inline char *elem(const swm_ctx *ctx, const Field &f, int x, int y) {
-    return f.p0 + ((ptrdiff_t)x + (ptrdiff_t)y * ctx->P) * ctx->es;
+    return f.p0 + ((ptrdiff_t)x + (ptrdiff_t)y * ctx->P) * f.es;
 }
 
 // copy a Julia (m, n) host array whose element [1,1] sits at (x0, y0)
 int put2d(swm_ctx *ctx, const Field &f, const void *host, int m, int n, int x0, int y0) {
-    HIPCHK(hipMemcpy2DAsync(elem(ctx, f, x0, y0), (size_t)ctx->P * ctx->es, host, (size_t)m * ctx->es,
-                            (size_t)m * ctx->es, n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpy2DAsync(elem(ctx, f, x0, y0), (size_t)ctx->P * f.es, host, (size_t)m * f.es, (size_t)m * f.es, n,
+                            hipMemcpyHostToDevice, ctx->stream));
     return SWM_OK;
 }
 int get2d(swm_ctx *ctx, const Field &f, void *host, int m, int n, int x0, int y0) {
-    HIPCHK(hipMemcpy2DAsync(host, (size_t)m * ctx->es, elem(ctx, f, x0, y0), (size_t)ctx->P * ctx->es,
-                            (size_t)m * ctx->es, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpy2DAsync(host, (size_t)m * f.es, elem(ctx, f, x0, y0), (size_t)ctx->P * f.es, (size_t)m * f.es, n,
+                            hipMemcpyDeviceToHost, ctx->stream));
+    return SWM_OK;
+}
+
+// a T-format field downloaded into a Tprog (Float32) host array: exact widening
+int get2d_widen(swm_ctx *ctx, const Field &f, void *host, int m, int n, int x0, int y0) {
+    std::vector<uint16_t> tmp((size_t)m * n);
+    int rc = get2d(ctx, f, tmp.data(), m, n, x0, y0);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float *out = (float *)host;
+    for (size_t k = 0; k < tmp.size(); ++k) {
+        uint32_t b;
+        if (ctx->fmt == SWM_BF16) b = (uint32_t)tmp[k] << 16;
+        else {                                     // binary16 -> binary32
+            const uint32_t h = tmp[k], s = (h >> 15) << 31, ex = (h >> 10) & 0x1f, mt = h & 0x3ff;
+            if (ex == 0) {
+                const float r = std::ldexp((float)mt, -24);
+                std::memcpy(&b, &r, 4);
+                b |= s;
+            } else if (ex == 31) b = s | 0x7f800000u | (mt << 13);
+            else b = s | ((ex + 112) << 23) | (mt << 13);
+        }
+        std::memcpy(&out[k], &b, 4);
+    }
     return SWM_OK;
 }
 
 // A static field that does not vary along x is kept as one value per row (stride 0 in x).
 int put_static(swm_ctx *ctx, Field &f, int *sx, int *sy, const void *host, int m, int n, int x0, int y0) {
     const char *h = (const char *)host;
+    const int es = f.es;
     bool xinv = true;
     for (int j = 0; j < n && xinv; ++j)
         for (int i = 1; i < m && xinv; ++i)
-            xinv = std::memcmp(h + ((size_t)j * m + i) * ctx->es, h + (size_t)j * m * ctx->es, ctx->es) == 0;
+            xinv = std::memcmp(h + ((size_t)j * m + i) * es, h + (size_t)j * m * es, es) == 0;
     if (xinv) {
-        as_rows(ctx, f);
-        std::vector<char> col((size_t)n * ctx->es);
-        for (int j = 0; j < n; ++j) std::memcpy(&col[(size_t)j * ctx->es], h + (size_t)j * m * ctx->es, ctx->es);
-        HIPCHK(hipMemcpyAsync(f.p0 + (ptrdiff_t)y0 * ctx->es, col.data(), col.size(), hipMemcpyHostToDevice, ctx->stream));
+        as_rows(f);
+        std::vector<char> col((size_t)n * es);
+        for (int j = 0; j < n; ++j) std::memcpy(&col[(size_t)j * es], h + (size_t)j * m * es, es);
+        HIPCHK(hipMemcpyAsync(f.p0 + (ptrdiff_t)y0 * es, col.data(), col.size(), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *sx = 0;
         *sy = 1;
         return SWM_OK;
     }
-    f.p0 = (char *)f.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
+    f.p0 = (char *)f.base + ((size_t)OX + (size_t)OY * ctx->P) * es;
     *sx = 1;
     *sy = ctx->P;
     return put2d(ctx, f, host, m, n, x0, y0);
@@ -155,7 +233,21 @@ inline int nblocks_waves(int nwaves) {
     return (nb + NXCD - 1) / NXCD * NXCD;
 }
 
-template <class R> struct Launch {
+// a T-format scalar the host already rounded (config values marked (T)) as the kernel's R
+template <class R> R mkR(double d) {
+    if constexpr (std::is_same_v<R, double> || std::is_same_v<R, float>) return (R)d;
+    else {
+        R r;
+        r.v = (float)d;
+        return r;
+    }
+}
+
+template <class F> struct Launch {
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+
     static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
     static int nseg(const swm_ctx *ctx) { return (ctx->c.ny + ctx->L - 1) / ctx->L; }
 
@@ -166,7 +258,7 @@ template <class R> struct Launch {
     template <bool PER> static int rows_per_wave(const swm_ctx *ctx) {
         int cus = 0, bpc = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess || cus <= 0) cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(&stage_kernel<R, PER, M_RK_MID>),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>),
                                                          WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
             bpc = 1;
         const long slots = (long)cus * bpc * WAVES_PER_BLOCK;
@@ -185,75 +277,95 @@ template <class R> struct Launch {
         s.Feta = ctx->Feta.ref();
         return s;
     }
-    static uint32_t fr(const Field *f, int s) { return s < 0 ? 0u : f[s].ref(); }
+    static uint32_t ou(const swm_ctx *ctx, Slots s) { return s.uv < 0 ? 0u : ctx->fu(s).ref(); }
+    static uint32_t ov(const swm_ctx *ctx, Slots s) { return s.uv < 0 ? 0u : ctx->fv(s).ref(); }
+    static uint32_t oe(const swm_ctx *ctx, Slots s) { return s.e < 0 ? 0u : ctx->fe(s).ref(); }
 
     struct Times { R Fxt, Fyt, Fet; };
-    static Times forcing_times(const swm_ctx *ctx, int64_t t) {     // Ftime, forcing.jl:215-217
+    static Times forcing_times(const swm_ctx *ctx, int64_t t) {     // Ftime, forcing.jl:215-217: T(sin(ω t))
         const swm_config &c = ctx->c;
         Times T;
-        T.Fxt = c.seasonal_wind_x ? (R)std::sin(c.omegaFx * (double)t) : R(1);
-        T.Fyt = c.seasonal_wind_y ? (R)std::sin(c.omegaFy * (double)t) : R(1);
-        T.Fet = c.continuity == SWM_CONT_FORCING ? (R)std::sin(c.omegaFeta * (double)t) : R(0);
+        T.Fxt = mkR<R>(c.seasonal_wind_x ? round_host(std::sin(c.omegaFx * (double)t), c.T) : 1.0);
+        T.Fyt = mkR<R>(c.seasonal_wind_y ? round_host(std::sin(c.omegaFy * (double)t), c.T) : 1.0);
+        T.Fet = mkR<R>(c.continuity == SWM_CONT_FORCING ? round_host(std::sin(c.omegaFeta * (double)t), c.T) : 0.0);
         return T;
     }
 
-    static StageArgs<R> args(swm_ctx *ctx, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+    static StageArgs<F> args(swm_ctx *ctx, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
                              const Times &T) {
         const swm_config &c = ctx->c;
-        StageArgs<R> A;
+        StageArgs<F> A;
         A.ar = ctx->aref();
-        A.in_u = fr(ctx->u, in.uv); A.in_v = fr(ctx->v, in.uv); A.in_e = fr(ctx->e, in.e);
-        A.pr_u = fr(ctx->u, pr.uv); A.pr_v = fr(ctx->v, pr.uv); A.pr_e = fr(ctx->e, pr.e);
-        A.ac_u = fr(ctx->u, ac.uv); A.ac_v = fr(ctx->v, ac.uv); A.ac_e = fr(ctx->e, ac.e);
-        A.o1_u = fr(ctx->u, o1.uv); A.o1_v = fr(ctx->v, o1.uv); A.o1_e = fr(ctx->e, o1.e);
-        A.o2_u = fr(ctx->u, o2.uv); A.o2_v = fr(ctx->v, o2.uv); A.o2_e = fr(ctx->e, o2.e);
+        A.in_u = ou(ctx, in); A.in_v = ov(ctx, in); A.in_e = oe(ctx, in);
+        A.pr_u = ou(ctx, pr); A.pr_v = ov(ctx, pr); A.pr_e = oe(ctx, pr);
+        A.ac_u = ou(ctx, ac); A.ac_v = ov(ctx, ac); A.ac_e = oe(ctx, ac);
+        A.o1_u = ou(ctx, o1); A.o1_v = ov(ctx, o1); A.o1_e = oe(ctx, o1);
+        A.o2_u = ou(ctx, o2); A.o2_v = ov(ctx, o2); A.o2_e = oe(ctx, o2);
         A.qend = ctx->qend.ref();
         A.st = statics(ctx);
         A.k = k;
         A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
-        A.g = (R)c.g; A.oma = (R)c.one_minus_alpha; A.gamma = (R)c.gamma;
-        A.cont = c.continuity;
+        A.g = mkR<R>(c.g); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(1.0 / 12.0, c.T));
+        A.oma = (Q)c.one_minus_alpha;
+        A.cont = c.continuity; A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
         A.nstrips = nstrips(ctx); A.nseg = nseg(ctx);
         return A;
     }
 
-    // every buffer a mode reads or writes must name a slot (no launch through a null base)
-    static bool slots_ok(int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, bool cont) {
+    // Every buffer a mode reads or writes must name a slot (no launch through a null base),
+    // and the stage input / first output live in the pool the mode's formats expect.
+    static bool slots_ok(const swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, bool cont) {
         const bool rk = m <= M_RK_LAST;
         const bool need_pr = m == M_RK_MID || m == M_SSP2_LAST || m == M_S4_3;
         const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN;
         const bool out1 = m != M_RK_LAST;
         const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont);
-        auto ok = [](int s) { return s >= 0 && s < 4; };
+        auto ok = [](Slots s, bool uv, bool e, bool tpool) {
+            const int n = tpool ? 2 : 4;
+            if (s.t != tpool) return false;
+            return (!uv || (s.uv >= 0 && s.uv < n)) && (!e || (s.e >= 0 && s.e < n));
+        };
         const bool uv = !cont, e = rk || cont;            // which of (u,v) and η the kernel touches
-        if (!ok(in.uv) || !ok(in.e)) return false;
-        if (need_pr && ((uv && !ok(pr.uv)) || (e && !ok(pr.e)))) return false;
-        if (need_ac && ((uv && !ok(ac.uv)) || (e && !ok(ac.e)))) return false;
-        if (out1 && ((uv && !ok(o1.uv)) || (e && !ok(o1.e)))) return false;
-        if (out2 && ((uv && !ok(o2.uv)) || (e && !ok(o2.e)))) return false;
+        const bool tin = ctx->mixed && m != M_RK_FIRST;   // stage input u1 in the T pool
+        const bool tout = ctx->mixed;                     // first output u1 in the T pool
+        if (!ok(in, uv, e, tin)) return false;
+        if (need_pr && !ok(pr, uv, e, false)) return false;
+        if (need_ac && !ok(ac, uv, e, false)) return false;
+        if (out1 && !ok(o1, uv, e, tout)) return false;
+        if (out2 && !ok(o2, uv, e, false)) return false;
         return true;
     }
 
     // one stage kernel (momentum, and continuity for the RK modes) in mode m
-    static int stage(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+    static int stage(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
                      const Times &T) {
         const swm_config &c = ctx->c;
         const bool per = c.bc == SWM_PERIODIC;
-        if (!slots_ok(m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
-        StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
-        if (per)
-            qend_kernel<R><<<(c.ny + 1 + 255) / 256, 256, 0, ctx->stream>>>(A);
+        if (!slots_ok(ctx, m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
+        StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
+        if (per) {
+            const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
+            if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
+            else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
+        }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
-        if (per) hipLaunchKernelGGL((stage_kernel<R, true, M_>), grid, block, 0, ctx->stream, A);          \
-        else hipLaunchKernelGGL((stage_kernel<R, false, M_>), grid, block, 0, ctx->stream, A);             \
+        if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);          \
+        else hipLaunchKernelGGL((stage_kernel<F, false, M_>), grid, block, 0, ctx->stream, A);             \
         break;
-        switch (m) {
-            SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
-            SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
-        default: return ctx->fail(SWM_E_ARG, "stage mode %d", m);
+        if constexpr (F::mixed) {
+            switch (m) {
+                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST)
+            default: return ctx->fail(SWM_E_UNSUPPORTED, "stage mode %d with mixed formats", m);
+            }
+        } else {
+            switch (m) {
+                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
+                SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
+            default: return ctx->fail(SWM_E_ARG, "stage mode %d", m);
+            }
         }
 #undef SWM_K
         HIPCHK(hipGetLastError());
@@ -261,25 +373,29 @@ template <class R> struct Launch {
     }
 
     // semi-implicit continuity of the SSPRK schemes in mode m
-    static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<R> &k,
+    static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
                           const Times &T) {
-        const bool per = ctx->c.bc == SWM_PERIODIC;
-        if (m <= M_RK_LAST || !slots_ok(m, in, pr, ac, o1, o2, true))
-            return ctx->fail(SWM_E_STATE, "continuity mode %d: buffer slot missing", m);
-        StageArgs<R> A = args(ctx, in, pr, ac, o1, o2, k, T);
-        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+        if constexpr (F::mixed) {
+            return ctx->fail(SWM_E_UNSUPPORTED, "SSPRK schemes with mixed formats");
+        } else {
+            const bool per = ctx->c.bc == SWM_PERIODIC;
+            if (m <= M_RK_LAST || !slots_ok(ctx, m, in, pr, ac, o1, o2, true))
+                return ctx->fail(SWM_E_STATE, "continuity mode %d: buffer slot missing", m);
+            StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
+            const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
-        if (per) hipLaunchKernelGGL((continuity_kernel<R, true, M_>), grid, block, 0, ctx->stream, A);     \
-        else hipLaunchKernelGGL((continuity_kernel<R, false, M_>), grid, block, 0, ctx->stream, A);        \
+        if (per) hipLaunchKernelGGL((continuity_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);     \
+        else hipLaunchKernelGGL((continuity_kernel<F, false, M_>), grid, block, 0, ctx->stream, A);        \
         break;
-        switch (m) {
-            SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3) SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
-        default: return ctx->fail(SWM_E_ARG, "continuity mode %d", m);
-        }
+            switch (m) {
+                SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3) SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
+            default: return ctx->fail(SWM_E_ARG, "continuity mode %d", m);
+            }
 #undef SWM_K
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
+            HIPCHK(hipGetLastError());
+            return SWM_OK;
+        }
     }
 
     // The first n_stages stages of the configured scheme (time_integration.jl:53-196) from
@@ -294,23 +410,25 @@ template <class R> struct Launch {
         const Times T = forcing_times(ctx, t);
         const Slots P{ctx->pu, ctx->pe}, none{};
         Slots cur = P, z0 = P;
-        Coef<R> k{};
+        Coef<Q> k{};
         int rc = SWM_OK;
         if (c.time_scheme == SWM_RK) {                                     // :55-78
             const Slots a{pick({P.uv}), pick({P.e})};
             for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
                 const int m = s == 1 ? M_RK_FIRST : (s == c.RKo ? M_RK_LAST : M_RK_MID);
-                k.a = (R)c.RKaDt[s - 1];
-                k.b = s < c.RKo ? (R)c.RKbDt[s - 1] : R(0);
-                const Slots n{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
+                k.a = (Q)c.RKaDt[s - 1];
+                k.b = s < c.RKo ? (Q)c.RKbDt[s - 1] : Q(0);
+                // the next u1: a free Tprog slot, or the other T slot with mixed formats
+                const Slots n = ctx->mixed ? Slots{cur.t ? 1 - cur.uv : 0, cur.t ? 1 - cur.e : 0, true}
+                                           : Slots{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
                 rc = stage(ctx, m, cur, P, a, s < c.RKo ? n : none, a, k, T);
                 if (s < c.RKo) cur = n;
             }
             z0 = a;
         } else if (c.time_scheme == SWM_SSPRK2) {                          // :82-110
-            k.c = (R)c.Dt_Ds;
-            k.x = (R)(1.0 / c.RKs);
-            k.y = (R)((double)(c.RKs - 1) / c.RKs);
+            k.c = (Q)c.Dt_Ds;
+            k.x = (Q)(1.0 / c.RKs);
+            k.y = (Q)((double)(c.RKs - 1) / c.RKs);
             for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
                 const bool last = s == c.RKs;
                 const Slots n{pick({P.uv, cur.uv}), pick({P.e, cur.e})};
@@ -322,10 +440,10 @@ template <class R> struct Launch {
                 if (last) z0 = z;
             }
         } else if (c.time_scheme == SWM_SSPRK3) {                          // :114-156
-            k.c = (R)c.ssp_Dt_Dn;
-            k.x = (R)c.ssp_kna;
-            k.y = (R)c.ssp_knb;
-            k.z = (R)c.ssp_Dt_Dnc;
+            k.c = (Q)c.ssp_Dt_Dn;
+            k.x = (Q)c.ssp_kna;
+            k.y = (Q)c.ssp_knb;
+            k.z = (Q)c.ssp_Dt_Dnc;
             for (int s = 2; s <= n_stages + 1 && s <= c.ssp_s + 1 && rc == SWM_OK; ++s) {
                 const Slots n{pick({P.uv, cur.uv, z0.uv}), pick({P.e, cur.e, z0.e})};
                 const int m = s == c.ssp_kn ? M_SSP3_KN : M_SSP3;
@@ -335,10 +453,10 @@ template <class R> struct Launch {
                 if (s == c.ssp_mn || s == c.ssp_s + 1) z0 = n;            // copyto!(u0,u1) :150-155
             }
         } else {                                                            // 4SSPRK3 :160-195
-            k.c = (R)c.Dt_D;
-            k.h = R(0.5);
-            k.x = (R)(2.0 / 3.0);
-            k.y = (R)(1.0 / 3.0);
+            k.c = (Q)c.Dt_D;
+            k.h = Q(0.5);
+            k.x = (Q)(2.0 / 3.0);
+            k.y = (Q)(1.0 / 3.0);
             for (int s = 1; s <= n_stages && s <= 4 && rc == SWM_OK; ++s) {
                 const int n1 = pick({P.uv, cur.uv});
                 const int n2 = s == 3 ? pick({P.uv, cur.uv, n1}) : -1;
@@ -357,18 +475,19 @@ template <class R> struct Launch {
 
     static int dissipation(swm_ctx *ctx, Slots in, int out_uv) {
         const swm_config &c = ctx->c;
-        DissArgs<R> A;
+        DissArgs<F> A;
         A.ar = ctx->aref();
         A.u0 = ctx->u[in.uv].ref(); A.v0 = ctx->v[in.uv].ref(); A.e0 = ctx->e[in.e].ref();
         A.out_u = ctx->u[out_uv].ref(); A.out_v = ctx->v[out_uv].ref();
         A.H = ctx->H.ref();
-        A.cD = (R)c.cD; A.rD = (R)c.rD; A.cS = (R)c.cSmag; A.nuB = (R)c.nuB; A.dtd = (R)c.Dt_diff;
-        A.oma = (R)c.one_minus_alpha;
+        A.cD = mkR<R>(c.cD); A.rD = mkR<R>(c.rD); A.cS = mkR<R>(c.cSmag); A.nuB = mkR<R>(c.nuB);
+        A.dtd = (Q)c.Dt_diff; A.oma = (Q)c.one_minus_alpha;
+        A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
         A.nstrips = nstrips(ctx); A.nseg = nseg(ctx);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
-#define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<R, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
+#define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<F, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
 #define SWM_DISS_D(PER_, S_)                                   \
     switch (c.bottom_drag) {                                   \
     case SWM_DRAG_QUADRATIC: SWM_DISS(PER_, 0, S_); break;     \
@@ -390,11 +509,12 @@ template <class R> struct Launch {
         const swm_config &c = ctx->c;
         const dim3 grid((c.nx + 127) / 128, c.ny), block(128);
         const bool per = c.bc == SWM_PERIODIC;
-        auto k = per ? tracer_kernel<R, true> : tracer_kernel<R, false>;
-        hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<R>(ctx->u[uv_slot]), ctx->at<R>(ctx->v[uv_slot]),
-                           ctx->at<R>(ctx->um), ctx->at<R>(ctx->vm), ctx->at<R>(ctx->sst[src]), ctx->at<R>(ctx->sst[dst]),
-                           (R)c.half_dtadvu, (R)c.half_dtadvv, (R)c.dtadvu, (R)c.dtadvv, c.nx, c.ny, ctx->P, ctx->ep,
-                           c.tracer_consumption, (R)c.jSST, (R)c.SSTmin, ctx->dflags + 1);
+        auto k = per ? tracer_kernel<F, true> : tracer_kernel<F, false>;
+        hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<Q>(ctx->u[uv_slot]), ctx->at<Q>(ctx->v[uv_slot]),
+                           ctx->at<ST>(ctx->um), ctx->at<ST>(ctx->vm), ctx->at<Q>(ctx->sst[src]), ctx->at<Q>(ctx->sst[dst]),
+                           mkR<R>(c.half_dtadvu), mkR<R>(c.half_dtadvv), mkR<R>(c.dtadvu), mkR<R>(c.dtadvv), c.nx, c.ny,
+                           ctx->P, ctx->ep, c.tracer_consumption, mkR<R>(c.jSST), mkR<R>(c.SSTmin), c.Tcomm,
+                           ctx->dflags + 1);
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
@@ -404,9 +524,9 @@ template <class R> struct Launch {
         const swm_config &c = ctx->c;
         if (!c.tracer_advection) return SWM_OK;
         const size_t n = ctx->elems;
-        if ((i + c.nadvstep_half) % c.nadvstep == 0) {      // mid-point snapshot :12-17
-            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->um.base, (const R *)ctx->u[uv_slot].base, n);
-            copy_kernel<R><<<1024, 256, 0, ctx->stream>>>((R *)ctx->vm.base, (const R *)ctx->v[uv_slot].base, n);
+        if ((i + c.nadvstep_half) % c.nadvstep == 0) {      // mid-point snapshot um, vm ← T(u0), T(v0)  :12-17
+            snapshot_kernel<F><<<1024, 256, 0, ctx->stream>>>((ST *)ctx->um.base, (const Q *)ctx->u[uv_slot].base, n);
+            snapshot_kernel<F><<<1024, 256, 0, ctx->stream>>>((ST *)ctx->vm.base, (const Q *)ctx->v[uv_slot].base, n);
             HIPCHK(hipGetLastError());
         }
         if (i % c.nadvstep == 0) {                          // :19-42
@@ -421,13 +541,13 @@ template <class R> struct Launch {
     static int ghosts(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot) {
         const swm_config &c = ctx->c;
         const bool per = c.bc == SWM_PERIODIC;
-        auto k = per ? ghost_kernel<R, true> : ghost_kernel<R, false>;
-        R *u = ctx->at<R>(ctx->u[uv_slot]), *v = ctx->at<R>(ctx->v[uv_slot]), *e = ctx->at<R>(ctx->e[e_slot]);
-        R *s = ctx->at<R>(ctx->sst[sst_slot]);
+        auto k = per ? ghost_kernel<Q, true> : ghost_kernel<Q, false>;
+        Q *u = ctx->at<Q>(ctx->u[uv_slot]), *v = ctx->at<Q>(ctx->v[uv_slot]), *e = ctx->at<Q>(ctx->e[e_slot]);
+        Q *s = ctx->at<Q>(ctx->sst[sst_slot]);
         hipLaunchKernelGGL(k, dim3((c.ny + 4 + 127) / 128), dim3(128), 0, ctx->stream, u, v, e, s, c.nx, c.ny, ctx->P,
-                           (R)c.one_minus_alpha, 0);
+                           (Q)c.one_minus_alpha, c.Tcomm, 0);
         hipLaunchKernelGGL(k, dim3((c.nx + 5 + 127) / 128), dim3(128), 0, ctx->stream, u, v, e, s, c.nx, c.ny, ctx->P,
-                           (R)c.one_minus_alpha, 1);
+                           (Q)c.one_minus_alpha, c.Tcomm, 1);
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
@@ -436,14 +556,14 @@ template <class R> struct Launch {
     static int nan_check(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot, bool *found) {
         const swm_config &c = ctx->c;
         NanArgs A;
-        A.f[0] = ctx->at<R>(ctx->u[uv_slot]); A.x0[0] = -1 - ctx->ep; A.x1[0] = ctx->nux + 2 - ctx->ep; A.y0[0] = -1; A.y1[0] = c.ny + 2;
-        A.f[1] = ctx->at<R>(ctx->v[uv_slot]); A.x0[1] = -1; A.x1[1] = c.nx + 2; A.y0[1] = -1; A.y1[1] = c.ny + 1;
-        A.f[2] = ctx->at<R>(ctx->e[e_slot]); A.x0[2] = 0; A.x1[2] = c.nx + 1; A.y0[2] = 0; A.y1[2] = c.ny + 1;
-        A.f[3] = ctx->at<R>(ctx->sst[sst_slot]); A.x0[3] = 0; A.x1[3] = c.nx + 1; A.y0[3] = 1; A.y1[3] = c.ny;
+        A.f[0] = ctx->at<Q>(ctx->u[uv_slot]); A.x0[0] = -1 - ctx->ep; A.x1[0] = ctx->nux + 2 - ctx->ep; A.y0[0] = -1; A.y1[0] = c.ny + 2;
+        A.f[1] = ctx->at<Q>(ctx->v[uv_slot]); A.x0[1] = -1; A.x1[1] = c.nx + 2; A.y0[1] = -1; A.y1[1] = c.ny + 1;
+        A.f[2] = ctx->at<Q>(ctx->e[e_slot]); A.x0[2] = 0; A.x1[2] = c.nx + 1; A.y0[2] = 0; A.y1[2] = c.ny + 1;
+        A.f[3] = ctx->at<Q>(ctx->sst[sst_slot]); A.x0[3] = 0; A.x1[3] = c.nx + 1; A.y0[3] = 1; A.y1[3] = c.ny;
         A.P = ctx->P;
         A.nf = 4;
         HIPCHK(hipMemsetAsync(ctx->dflags, 0, sizeof(int), ctx->stream));
-        nan_kernel<R><<<dim3(1, c.ny + 4, 4), 256, 0, ctx->stream>>>(A, ctx->dflags);
+        nan_kernel<Q><<<dim3(1, c.ny + 4, 4), 256, 0, ctx->stream>>>(A, ctx->dflags);
         HIPCHK(hipGetLastError());
         int flag = 0;
         HIPCHK(hipMemcpyAsync(&flag, ctx->dflags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -482,20 +602,22 @@ template <class R> struct Launch {
         const swm_config &c = ctx->c;
         const Slots P{ctx->pu, ctx->pe}, none{};
         const Slots a{pick({P.uv}), pick({P.e})};
-        const Slots s1{pick({P.uv, a.uv}), pick({P.e, a.e})};
-        const Slots s2{pick({P.uv, a.uv, s1.uv}), pick({P.e, a.e, s1.e})};
-        const Times T{R(1), R(1), R(0)};
-        Coef<R> k{};
-        k.a = (R)c.RKaDt[0];
-        k.b = (R)c.RKbDt[0];
-        k.c = (R)c.Dt_D;
+        const bool tp = ctx->mixed;
+        const Slots s1 = tp ? Slots{0, 0, true} : Slots{pick({P.uv, a.uv}), pick({P.e, a.e})};
+        const Slots s2 = tp ? Slots{1, 1, true} : Slots{pick({P.uv, a.uv, s1.uv}), pick({P.e, a.e, s1.e})};
+        const int dq = pick({P.uv, a.uv});                   // dissipation output (Tprog)
+        const Times T{mkR<R>(1.0), mkR<R>(1.0), mkR<R>(0.0)};
+        Coef<Q> k{};
+        k.a = (Q)c.RKaDt[0];
+        k.b = (Q)c.RKbDt[0];
+        k.c = (Q)c.Dt_D;
         auto launch = [&]() -> int {
             switch (which) {
             case 0: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_MID : M_SSP2, s1, P, a, s2, a, k, T) ||
                            (c.time_scheme != SWM_RK && continuity(ctx, M_SSP2, Slots{s2.uv, s1.e}, P, none, s2, none, k, T));
             case 1: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_FIRST : M_SSP2, P, P, a, s1, a, k, T);
             case 2: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_LAST : M_SSP2, s1, P, a, none, a, k, T);
-            case 3: return dissipation(ctx, a, s2.uv);
+            case 3: return dissipation(ctx, a, dq);
             case 4: return tracer_advect(ctx, P.uv, ctx->rsst, 1 - ctx->rsst);
             }
             return ctx->fail(SWM_E_ARG, "unknown kernel %d", which);
@@ -511,22 +633,40 @@ template <class R> struct Launch {
         HIPCHK(hipEventSynchronize(e1));
         float tm = 0.0f;
         HIPCHK(hipEventElapsedTime(&tm, e0, e1));
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        HIPCHK(hipEventDestroy(e0));
+        HIPCHK(hipEventDestroy(e1));
         *ms = (double)tm / reps;
         return rc;
     }
 };
+
+// call fn with the format bundle of the context
+template <class Fn> int with_fmt(const swm_ctx *ctx, Fn fn) {
+    switch (ctx->fmt) {
+    case SWM_F64: return fn(FmtF64{});
+    case SWM_F32: return fn(FmtF32{});
+    case SWM_F16: return fn(FmtF16{});
+    default: return fn(FmtBF16{});
+    }
+}
 
 int check_config(swm_ctx *ctx) {
     const swm_config &c = ctx->c;
     if (c.abi_version != SWM_ABI_VERSION) return ctx->fail(SWM_E_ARG, "abi_version %d != %d", c.abi_version, SWM_ABI_VERSION);
     if (c.nx < 3 || c.ny < 3) return ctx->fail(SWM_E_ARG, "grid %dx%d too small (need nx, ny >= 3)", c.nx, c.ny);
     if (c.bc != SWM_PERIODIC && c.bc != SWM_NONPERIODIC) return ctx->fail(SWM_E_ARG, "bc %d", c.bc);
-    if (!(c.T == c.Tprog && c.Tprog == c.Tcomm) || (c.T != SWM_F64 && c.T != SWM_F32))
-        return ctx->fail(SWM_E_UNSUPPORTED, "formats T=%d Tprog=%d Tcomm=%d: this build supports T=Tprog=Tcomm in {Float64, Float32}",
+    if (c.T < SWM_F64 || c.T > SWM_BF16 || c.Tprog < SWM_F64 || c.Tprog > SWM_BF16 || c.Tcomm < SWM_F64 || c.Tcomm > SWM_BF16)
+        return ctx->fail(SWM_E_ARG, "number format codes T=%d Tprog=%d Tcomm=%d", c.T, c.Tprog, c.Tcomm);
+    const bool same = c.T == c.Tprog && (c.T == SWM_F64 || c.T == SWM_F32);
+    const bool mixed = (c.T == SWM_F16 || c.T == SWM_BF16) && c.Tprog == SWM_F32 &&
+                       (c.Tcomm == c.T || c.Tcomm == SWM_F32 || c.Tcomm == SWM_F64);
+    if (!same && !mixed)
+        return ctx->fail(SWM_E_UNSUPPORTED, "formats T=%d Tprog=%d Tcomm=%d: supported are T = Tprog in {Float64, Float32} "
+                         "(any Tcomm), and T in {Float16, BFloat16} with Tprog = Float32 and Tcomm in {T, Tprog}",
                          c.T, c.Tprog, c.Tcomm);
     if (c.time_scheme < SWM_RK || c.time_scheme > SWM_4SSPRK3) return ctx->fail(SWM_E_ARG, "time_scheme %d", c.time_scheme);
+    if (mixed && c.time_scheme != SWM_RK)
+        return ctx->fail(SWM_E_UNSUPPORTED, "T != Tprog is implemented for the classic RK scheme only");
     if (c.time_scheme == SWM_RK && (c.RKo < 2 || c.RKo > 4)) return ctx->fail(SWM_E_ARG, "RKo %d", c.RKo);
     if (c.time_scheme == SWM_SSPRK2 && c.RKs < 2) return ctx->fail(SWM_E_ARG, "RKs %d", c.RKs);
     if (c.time_scheme == SWM_SSPRK3 && (c.ssp_n < 2 || c.ssp_s != c.ssp_n * c.ssp_n)) return ctx->fail(SWM_E_ARG, "RKn %d", c.ssp_n);
@@ -537,15 +677,17 @@ int check_config(swm_ctx *ctx) {
     if (c.nstep_diff < 1 || (c.tracer_advection && c.nadvstep < 1)) return ctx->fail(SWM_E_ARG, "nstep_diff/nadvstep must be >= 1");
     if (c.bottom_drag < 0 || c.bottom_drag > 2 || c.diffusion < 0 || c.diffusion > 1)
         return ctx->fail(SWM_E_ARG, "bottom_drag %d / diffusion %d", c.bottom_drag, c.diffusion);
+    ctx->mixed = mixed;
+    ctx->fmt = c.T;
     return SWM_OK;
 }
 
-template <class R> int run_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *done, int64_t *nan_step) {
+template <class F> int run_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *done, int64_t *nan_step) {
     *done = 0;
     *nan_step = 0;
     for (int64_t k = 0; k < nsteps; ++k) {
         bool nan = false;
-        int rc = Launch<R>::step(ctx, i0 + k, t0 + k * ctx->c.dtint, &nan);
+        int rc = Launch<F>::step(ctx, i0 + k, t0 + k * ctx->c.dtint, &nan);
         if (rc) return rc;
         if (nan) {
             *nan_step = i0 + k;
@@ -598,7 +740,8 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         return rc;
     }
     const swm_config &c = ctx->c;
-    ctx->es = c.T == SWM_F64 ? 8 : 4;
+    ctx->es = c.Tprog == SWM_F64 ? 8 : 4;
+    ctx->est = c.T == SWM_F64 ? 8 : c.T == SWM_F32 ? 4 : 2;
     ctx->ep = c.bc == SWM_PERIODIC ? 1 : 0;
     ctx->nux = ctx->ep ? c.nx : c.nx - 1;
     ctx->nuy = c.ny;
@@ -612,8 +755,10 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         delete ctx;
         return SWM_E_HIP;
     }
-    ctx->L = c.T == SWM_F64 ? (c.bc == SWM_PERIODIC ? Launch<double>::rows_per_wave<true>(ctx) : Launch<double>::rows_per_wave<false>(ctx))
-                            : (c.bc == SWM_PERIODIC ? Launch<float>::rows_per_wave<true>(ctx) : Launch<float>::rows_per_wave<false>(ctx));
+    ctx->L = with_fmt(ctx, [&](auto f) {
+        using F = decltype(f);
+        return c.bc == SWM_PERIODIC ? Launch<F>::template rows_per_wave<true>(ctx) : Launch<F>::template rows_per_wave<false>(ctx);
+    });
     if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {   // tuning override: rows each wavefront marches
         const int l = atoi(s);
         if (l >= 4 && l <= 4096) ctx->L = l;
@@ -622,10 +767,13 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         swm_destroy(ctx);
         return code;
     };
-    // the arena: 12 state slots, 2 sst, um, vm, H, Fx, Fy, f_q (+ η_ref, Fη) and the q column
-    const int n2d = 22 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0);
+    // the arena: Tprog fields 12 state slots + 2 sst; T fields um, vm, H, Fx, Fy, f_q
+    // (+ η_ref, Fη; + 6 u1 slots with mixed formats) and the q column
+    const int nq = 14;
+    const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0);
     const size_t n1d = (size_t)c.ny + 2 * OY + 16;
-    ctx->arena_bytes = (size_t)n2d * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256;
+    ctx->arena_bytes = (size_t)nq * (ctx->elems * ctx->es + 256) + (size_t)nt * (ctx->elems * ctx->est + 256) +
+                       n1d * ctx->est + 256;
     if (ctx->arena_bytes > 0xFFFF0000ull) {     // byte offsets into the arena are 32-bit
         ctx->fail(SWM_E_UNSUPPORTED, "grid %dx%d needs %zu bytes of device fields; one context addresses at most 4 GiB",
                   c.nx, c.ny, ctx->arena_bytes);
@@ -635,19 +783,25 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     if (hipMalloc(&ctx->arena, ctx->arena_bytes) != hipSuccess) return bail(SWM_E_HIP);
     ctx->allocs.push_back(ctx->arena);
     if (hipMemsetAsync(ctx->arena, 0, ctx->arena_bytes, ctx->stream) != hipSuccess) return bail(SWM_E_HIP);
+    const int es = ctx->es, est = ctx->est;
     for (int s = 0; s < 4; ++s) {
-        if ((rc = alloc_field(ctx, ctx->u[s], 0, true)) || (rc = alloc_field(ctx, ctx->v[s], 0, true)) ||
-            (rc = alloc_field(ctx, ctx->e[s], 0, true)))
+        if ((rc = alloc_field(ctx, ctx->u[s], 0, true, es)) || (rc = alloc_field(ctx, ctx->v[s], 0, true, es)) ||
+            (rc = alloc_field(ctx, ctx->e[s], 0, true, es)))
             return bail(rc);
     }
-    if ((rc = alloc_field(ctx, ctx->sst[0], 0, true)) || (rc = alloc_field(ctx, ctx->sst[1], 0, true)) ||
-        (rc = alloc_field(ctx, ctx->um, 0, true)) || (rc = alloc_field(ctx, ctx->vm, 0, true)) ||
-        (rc = alloc_field(ctx, ctx->H, 0, true)) || (rc = alloc_field(ctx, ctx->Fx, 0, true)) ||
-        (rc = alloc_field(ctx, ctx->Fy, 0, true)) || (rc = alloc_field(ctx, ctx->fq, 0, true)) ||
-        (rc = alloc_field(ctx, ctx->qend, n1d, false)))
+    if (ctx->mixed)
+        for (int s = 0; s < 2; ++s)
+            if ((rc = alloc_field(ctx, ctx->tu[s], 0, true, est)) || (rc = alloc_field(ctx, ctx->tv[s], 0, true, est)) ||
+                (rc = alloc_field(ctx, ctx->te[s], 0, true, est)))
+                return bail(rc);
+    if ((rc = alloc_field(ctx, ctx->sst[0], 0, true, es)) || (rc = alloc_field(ctx, ctx->sst[1], 0, true, es)) ||
+        (rc = alloc_field(ctx, ctx->um, 0, true, est)) || (rc = alloc_field(ctx, ctx->vm, 0, true, est)) ||
+        (rc = alloc_field(ctx, ctx->H, 0, true, est)) || (rc = alloc_field(ctx, ctx->Fx, 0, true, est)) ||
+        (rc = alloc_field(ctx, ctx->Fy, 0, true, est)) || (rc = alloc_field(ctx, ctx->fq, 0, true, est)) ||
+        (rc = alloc_field(ctx, ctx->qend, n1d, false, est)))
         return bail(rc);
     if (c.continuity != SWM_CONT_ITSELF &&
-        ((rc = alloc_field(ctx, ctx->eta_ref, 0, true)) || (rc = alloc_field(ctx, ctx->Feta, 0, true))))
+        ((rc = alloc_field(ctx, ctx->eta_ref, 0, true, est)) || (rc = alloc_field(ctx, ctx->Feta, 0, true, est))))
         return bail(rc);
     void *p = nullptr;
     if (hipMalloc(&p, 4 * sizeof(int)) != hipSuccess) return bail(SWM_E_HIP);
@@ -664,9 +818,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
 
 int swm_destroy(swm_ctx *ctx) {
     if (!ctx) return SWM_OK;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (void *p : ctx->allocs) hipFree(p);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SWM_OK;
 }
@@ -731,7 +885,7 @@ int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta,
     if ((rc = put2d(ctx, ctx->v[pu], v, ctx->nvx + 4, ctx->nvy + 4, -1, -1))) return rc;
     if ((rc = put2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
     if ((rc = put2d(ctx, ctx->sst[ctx->rsst], sst, c.nx + 2, c.ny, 0, 1))) return rc;
-    rc = c.T == SWM_F64 ? Launch<double>::ghosts(ctx, pu, pe, ctx->rsst) : Launch<float>::ghosts(ctx, pu, pe, ctx->rsst);
+    rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::ghosts(ctx, pu, pe, ctx->rsst); });
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_prog = true;
@@ -756,8 +910,7 @@ int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *step
     if (!ctx || !steps_done || !nan_step) return SWM_E_ARG;
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     if (nsteps < 0 || i0 < 1) return ctx->fail(SWM_E_ARG, "nsteps >= 0 and i0 >= 1 required");
-    return ctx->c.T == SWM_F64 ? run_steps<double>(ctx, nsteps, i0, t0, steps_done, nan_step)
-                               : run_steps<float>(ctx, nsteps, i0, t0, steps_done, nan_step);
+    return with_fmt(ctx, [&](auto f) { return run_steps<decltype(f)>(ctx, nsteps, i0, t0, steps_done, nan_step); });
 }
 
 int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
@@ -768,8 +921,7 @@ int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
                     : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
     if (n_stages < 1 || n_stages > total) return ctx->fail(SWM_E_ARG, "n_stages %d out of 1..%d", n_stages, total);
     (void)i0;
-    int rc = c.T == SWM_F64 ? Launch<double>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1)
-                            : Launch<float>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1);
+    int rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1); });
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
@@ -781,12 +933,16 @@ int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void
     const swm_config &c = ctx->c;
     int rc;
     const int mu = ctx->nux + 4, nu = ctx->nuy + 4, mv = ctx->nvx + 4, nv = ctx->nvy + 4;
-    if (u0 && (rc = get2d(ctx, ctx->u[ctx->rk0.uv], u0, mu, nu, -1 - ctx->ep, -1))) return rc;
-    if (v0 && (rc = get2d(ctx, ctx->v[ctx->rk0.uv], v0, mv, nv, -1, -1))) return rc;
-    if (eta0 && (rc = get2d(ctx, ctx->e[ctx->rk0.e], eta0, c.nx + 2, c.ny + 2, 0, 0))) return rc;
-    if (u1 && (rc = get2d(ctx, ctx->u[ctx->rk1.uv], u1, mu, nu, -1 - ctx->ep, -1))) return rc;
-    if (v1 && (rc = get2d(ctx, ctx->v[ctx->rk1.uv], v1, mv, nv, -1, -1))) return rc;
-    if (eta1 && (rc = get2d(ctx, ctx->e[ctx->rk1.e], eta1, c.nx + 2, c.ny + 2, 0, 0))) return rc;
+    const Slots s0 = ctx->rk0, s1 = ctx->rk1;
+    auto get = [&](const Field &f, void *h, int m, int n, int x0, int y0, bool t) {
+        return t ? get2d_widen(ctx, f, h, m, n, x0, y0) : get2d(ctx, f, h, m, n, x0, y0);
+    };
+    if (u0 && (rc = get(ctx->fu(s0), u0, mu, nu, -1 - ctx->ep, -1, s0.t))) return rc;
+    if (v0 && (rc = get(ctx->fv(s0), v0, mv, nv, -1, -1, s0.t))) return rc;
+    if (eta0 && (rc = get(ctx->fe(s0), eta0, c.nx + 2, c.ny + 2, 0, 0, s0.t))) return rc;
+    if (u1 && (rc = get(ctx->fu(s1), u1, mu, nu, -1 - ctx->ep, -1, s1.t))) return rc;
+    if (v1 && (rc = get(ctx->fv(s1), v1, mv, nv, -1, -1, s1.t))) return rc;
+    if (eta1 && (rc = get(ctx->fe(s1), eta1, c.nx + 2, c.ny + 2, 0, 0, s1.t))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
 }
@@ -795,8 +951,7 @@ int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     if (!ctx || !ms_avg) return SWM_E_ARG;
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");
-    return ctx->c.T == SWM_F64 ? Launch<double>::time_kernel(ctx, which, reps, ms_avg)
-                               : Launch<float>::time_kernel(ctx, which, reps, ms_avg);
+    return with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::time_kernel(ctx, which, reps, ms_avg); });
 }
 
 int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
@@ -804,15 +959,17 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     const swm_config &c = ctx->c;
     const int pu = ctx->pu, pe = ctx->pe;
-    if (c.T == SWM_F64)
-        reduce_kernel<double><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<double>(ctx->u[pu]), ctx->at<double>(ctx->v[pu]),
-                                                             ctx->at<double>(ctx->e[pe]), ctx->at<double>(ctx->H), c.nx,
-                                                             c.ny, ctx->P, ctx->ep, c.g, ctx->dpart);
-    else
-        reduce_kernel<float><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<float>(ctx->u[pu]), ctx->at<float>(ctx->v[pu]),
-                                                            ctx->at<float>(ctx->e[pe]), ctx->at<float>(ctx->H), c.nx, c.ny,
+    int rc = with_fmt(ctx, [&](auto f) {
+        using F = decltype(f);
+        using Q = typename F::Q;
+        using ST = typename F::ST;
+        reduce_kernel<Q, ST><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<Q>(ctx->u[pu]), ctx->at<Q>(ctx->v[pu]),
+                                                            ctx->at<Q>(ctx->e[pe]), ctx->at<ST>(ctx->H), c.nx, c.ny,
                                                             ctx->P, ctx->ep, c.g, ctx->dpart);
-    HIPCHK(hipGetLastError());
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    });
+    if (rc) return rc;
     std::vector<double> part(2 * (size_t)c.ny);
     HIPCHK(hipMemcpyAsync(part.data(), ctx->dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

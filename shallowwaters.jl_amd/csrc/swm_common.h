@@ -1,5 +1,5 @@
-// swm_common.h -- device layout, field access, cross-lane moves, ghost copies and wave
-// tiling shared by the gfx950 kernels of the ShallowWaters.jl time step.
+// swm_common.h -- number formats, device layout, field access, cross-lane moves, ghost
+// copies and wave tiling shared by the gfx950 kernels of the ShallowWaters.jl time step.
 //
 // Device layout.  Every field, whatever its staggering, lives in one padded 2-D layout
 // indexed by T-cell coordinates (x, y): element (x, y) sits at element offset
@@ -13,11 +13,16 @@
 //   u[ii,jj] -> (ii-2-ep, jj-2)   v[ii,jj] -> (ii-2, jj-2)   η[ii,jj] -> (ii-1, jj-1)
 //   q[ii,jj] -> (ii-1, jj-1)      sst[ii,jj] -> (ii-1, jj)
 // The padding (x from -64, y from -4) keeps every lane's access in bounds; padding is zero
-// and never written.
+// and never written.  All fields share the element pitch P; their byte pitch follows their
+// storage format.
 //
-// Field access goes through buffer resources: a lane's byte offset of its column is a
-// loop-invariant VGPR and the row is a scalar offset, so marching a row costs one scalar
-// add instead of per-lane 64-bit address arithmetic.
+// Number formats (src/default_parameters.jl:3-6).  T is the format of the right-hand side,
+// Tprog that of the prognostic and Runge-Kutta variables, Tcomm that of ghost copies.  A
+// kernel is instantiated for a bundle Fmt<R, Q, ST>: R the arithmetic type of T, Q that of
+// Tprog, ST the storage of T-format fields (static fields, the stage input u1, the tracer
+// snapshots).  Binary16 and bfloat16 arithmetic is carried in float registers and rounded
+// after every operation (lo<K>): float has at least 2p+2 significand bits for both, so each
+// + - * / sqrt is the correctly rounded result of the narrow format, as in Julia.
 //
 // Kernel shape.  One wavefront owns a strip of 64 consecutive x (lane = x - xs) and a
 // segment of rows, and marches upward in y keeping the few rows each stencil needs in
@@ -28,11 +33,12 @@
 //
 // Arithmetic follows the reference term by term in the same order (compiled with
 // -ffp-contract=off; IEEE division and sqrt), so results are bitwise those of the CPU
-// restatement for Float64 and Float32.
+// restatement.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace swm {
 
@@ -43,6 +49,111 @@ constexpr int HALO = 2;      // x-radius of every fused stencil
 constexpr int WOUT = WAVE - 2 * HALO;   // output columns per strip
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int NXCD = 8;      // accelerator complex dies of an MI355X, each with its own L2
+
+// ------------------------------------------------------------------ number formats
+
+enum { FMT_F64 = 0, FMT_F32 = 1, FMT_F16 = 2, FMT_BF16 = 3 };   // the SWM_F64 ... codes
+
+struct h16 { uint16_t b; };  // storage of an IEEE binary16 value
+struct b16 { uint16_t b; };  // storage of a bfloat16 value
+
+__device__ __forceinline__ float round_f16(float x) { return (float)(_Float16)x; }
+__device__ __forceinline__ float round_bf16(float x) { return (float)(__bf16)x; }
+
+// float64 -> float32 rounding to odd: a second rounding to binary16 / bfloat16 is then
+// the correctly rounded result from float64
+__device__ __forceinline__ float round_odd_f32(double x) {
+    float f = (float)x;
+    if ((double)f != x && __builtin_isfinite(f)) {
+        uint32_t b = __builtin_bit_cast(uint32_t, f);
+        if (__builtin_fabs((double)f) > __builtin_fabs(x)) b -= 1u;
+        f = __builtin_bit_cast(float, b | 1u);
+    }
+    return f;
+}
+
+// Arithmetic in binary16 (K = FMT_F16) or bfloat16 (K = FMT_BF16): a float register holding
+// a value of the format, rounded to it after every operation.
+template <int K> struct lo {
+    float v;
+    __device__ __forceinline__ static float rnd(float x) { return K == FMT_F16 ? round_f16(x) : round_bf16(x); }
+    __device__ __forceinline__ static lo raw(float x) { lo r; r.v = x; return r; }
+    lo() = default;
+    __device__ __forceinline__ lo(int i) : v(rnd((float)i)) {}
+    // exact constants only (0.5, 0.25, 1.0 ...); inexact ones come pre-rounded from the host
+    __device__ __forceinline__ explicit lo(double d) : v(rnd((float)d)) {}
+    __device__ __forceinline__ friend lo operator+(lo a, lo b) { return raw(rnd(a.v + b.v)); }
+    __device__ __forceinline__ friend lo operator-(lo a, lo b) { return raw(rnd(a.v - b.v)); }
+    __device__ __forceinline__ friend lo operator*(lo a, lo b) { return raw(rnd(a.v * b.v)); }
+    __device__ __forceinline__ friend lo operator/(lo a, lo b) { return raw(rnd(a.v / b.v)); }
+    __device__ __forceinline__ friend lo operator-(lo a) { return raw(-a.v); }
+    __device__ __forceinline__ lo &operator+=(lo b) { v = rnd(v + b.v); return *this; }
+    __device__ __forceinline__ friend bool operator<(lo a, lo b) { return a.v < b.v; }
+    __device__ __forceinline__ friend bool operator>(lo a, lo b) { return a.v > b.v; }
+};
+
+// sqrt, floor and the finiteness test of every arithmetic type (IEEE sqrt, correctly rounded)
+__device__ __forceinline__ float nsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double nsqrt(double x) { return ::sqrt(x); }
+template <int K> __device__ __forceinline__ lo<K> nsqrt(lo<K> a) { return lo<K>::raw(lo<K>::rnd(sqrtf(a.v))); }
+__device__ __forceinline__ float nfloor(float x) { return floorf(x); }
+__device__ __forceinline__ double nfloor(double x) { return ::floor(x); }
+template <int K> __device__ __forceinline__ lo<K> nfloor(lo<K> a) { return lo<K>::raw(floorf(a.v)); }
+__device__ __forceinline__ bool nfinite(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool nfinite(double x) { return __builtin_isfinite(x); }
+template <int K> __device__ __forceinline__ bool nfinite(lo<K> a) { return __builtin_isfinite(a.v); }
+
+// the arithmetic type's value as float / double (exact)
+__device__ __forceinline__ double wide(double x) { return x; }
+__device__ __forceinline__ float wide(float x) { return x; }
+template <int K> __device__ __forceinline__ float wide(lo<K> x) { return x.v; }
+
+// T(x) for an integer-valued x of the arithmetic type's range (|x| < 2^24)
+template <class R> __device__ __forceinline__ R from_int(long i) {
+    if constexpr (std::is_same_v<R, double>) return (double)i;
+    else if constexpr (std::is_same_v<R, float>) return (float)i;
+    else return R::raw(R::rnd((float)i));
+}
+
+// A bundle of formats: R arithmetic of T, Q arithmetic and storage of Tprog, ST storage of
+// the T-format fields.
+template <class R_, class Q_, class ST_> struct Fmt {
+    using R = R_;
+    using Q = Q_;
+    using ST = ST_;
+    static constexpr bool mixed = !std::is_same_v<R_, Q_>;
+};
+using FmtF64 = Fmt<double, double, double>;
+using FmtF32 = Fmt<float, float, float>;
+using FmtF16 = Fmt<lo<FMT_F16>, float, h16>;     // T = Float16, Tprog = Float32
+using FmtBF16 = Fmt<lo<FMT_BF16>, float, b16>;   // T = BFloat16, Tprog = Float32
+
+// Tprog(x) of a T value (exact widening) and T(x) of a Tprog value (one rounding)
+template <class Q, class R> __device__ __forceinline__ Q toQ(R r) {
+    if constexpr (std::is_same_v<Q, R>) return r;
+    else return (Q)wide(r);
+}
+template <class R, class Q> __device__ __forceinline__ R toR(Q q) {
+    if constexpr (std::is_same_v<Q, R>) return q;
+    else if constexpr (std::is_same_v<R, double> || std::is_same_v<R, float>) return (R)q;
+    else if constexpr (std::is_same_v<Q, double>) return R::raw(R::rnd(round_odd_f32(q)));
+    else return R::raw(R::rnd(q));
+}
+
+// Tcomm(x) of a value held in format `own` (ghost_points.jl:65): a no-op unless Tcomm is
+// narrower; the caller's format then holds it exactly.
+__device__ __forceinline__ double comm_round(double x, int cm) {
+    if (cm == FMT_F32) return (double)(float)x;
+    if (cm == FMT_F16) return (double)round_f16(round_odd_f32(x));
+    if (cm == FMT_BF16) return (double)round_bf16(round_odd_f32(x));
+    return x;
+}
+__device__ __forceinline__ float comm_round(float x, int cm) {
+    if (cm == FMT_F16) return round_f16(x);
+    if (cm == FMT_BF16) return round_bf16(x);
+    return x;
+}
+template <int K> __device__ __forceinline__ lo<K> comm_round(lo<K> x, int) { return x; }   // Tcomm is T or Tprog
 
 // ------------------------------------------------------------------ field access
 
@@ -55,36 +166,85 @@ struct ArenaRef {
     uint32_t bytes;
 };
 
-template <class R> struct Arena {
+struct Arena {
     __amdgpu_buffer_rsrc_t rs;
     __device__ __forceinline__ explicit Arena(ArenaRef a)
         : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a.base), 0, (int)a.bytes, 0x00020000)) {}
-    // vo: per-lane byte offset, so: uniform byte offset (field origin + row)
-    __device__ __forceinline__ R ld(uint32_t vo, uint32_t so) const {
-        if constexpr (sizeof(R) == 4) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0));
-        else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0));
+};
+
+// Storage formats: the register value type V, loads and stores through the arena.
+template <class S> struct Sto;
+template <> struct Sto<double> {
+    using V = double;
+    __device__ __forceinline__ static V ld(const Arena &a, uint32_t vo, uint32_t so) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(a.rs, vo, so, 0));
     }
-    __device__ __forceinline__ void st(R v, uint32_t vo, uint32_t so) const {
-        if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, vo, so, 0);
-        else {
-            using u2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0));
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, vo, so, 0);
-        }
+    __device__ __forceinline__ static void st(const Arena &a, V v, uint32_t vo, uint32_t so) {
+        using u2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(a.rs, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), a.rs, vo, so, 0);
+    }
+};
+template <> struct Sto<float> {
+    using V = float;
+    __device__ __forceinline__ static V ld(const Arena &a, uint32_t vo, uint32_t so) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(a.rs, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(const Arena &a, V v, uint32_t vo, uint32_t so) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), a.rs, vo, so, 0);
+    }
+};
+template <> struct Sto<h16> {
+    using V = float;                     // a binary16 value, widened
+    __device__ __forceinline__ static V ld(const Arena &a, uint32_t vo, uint32_t so) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(a.rs, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(const Arena &a, V v, uint32_t vo, uint32_t so) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, (_Float16)v), a.rs, vo, so, 0);
+    }
+};
+template <> struct Sto<b16> {
+    using V = float;                     // a bfloat16 value, widened
+    __device__ __forceinline__ static V ld(const Arena &a, uint32_t vo, uint32_t so) {
+        const uint32_t b = (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(a.rs, vo, so, 0);
+        return __builtin_bit_cast(float, b << 16);
+    }
+    __device__ __forceinline__ static void st(const Arena &a, V v, uint32_t vo, uint32_t so) {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(__builtin_bit_cast(uint32_t, v) >> 16), a.rs, vo, so, 0);
     }
 };
 
-// a field of the arena: its origin and the arena it lives in
-template <class R> struct Fld {
-    const Arena<R> *a;
+// value of an arithmetic type to store in format S (rounding a wider value to S)
+template <class S, class X> __device__ __forceinline__ typename Sto<S>::V enc(X x) {
+    if constexpr (std::is_same_v<S, double>) return (double)wide(x);
+    else if constexpr (std::is_same_v<S, float>) return (float)wide(x);
+    else if constexpr (std::is_same_v<X, lo<FMT_F16>> || std::is_same_v<X, lo<FMT_BF16>>) return x.v;
+    else if constexpr (std::is_same_v<S, h16>) return round_f16(x);
+    else return round_bf16(x);
+}
+// arithmetic value R of a stored S value (exact for T-format storage, one rounding from Tprog)
+template <class R, class S> __device__ __forceinline__ R dec(typename Sto<S>::V v) {
+    if constexpr (std::is_same_v<S, h16> || std::is_same_v<S, b16>) {
+        if constexpr (std::is_same_v<R, float> || std::is_same_v<R, double>) return (R)v;
+        else return R::raw(v);
+    } else return toR<R>(v);
+}
+
+// a field of the arena in storage format S: its origin and the arena it lives in
+template <class S> struct Fld {
+    using V = typename Sto<S>::V;
+    const Arena *a;
     uint32_t o;
-    __device__ __forceinline__ R ld(uint32_t vo, uint32_t so) const { return a->ld(vo, o + so); }
-    __device__ __forceinline__ void st(R v, uint32_t vo, uint32_t so) const { a->st(v, vo, o + so); }
+    __device__ __forceinline__ V ld(uint32_t vo, uint32_t so) const { return Sto<S>::ld(*a, vo, o + so); }
+    template <class R> __device__ __forceinline__ R ldr(uint32_t vo, uint32_t so) const { return dec<R, S>(ld(vo, so)); }
+    template <class X> __device__ __forceinline__ void st(X x, uint32_t vo, uint32_t so) const {
+        Sto<S>::st(*a, enc<S>(x), vo, o + so);
+    }
 };
 
-// byte offsets of column x and row y in the padded layout
-template <class R> __device__ __forceinline__ uint32_t colb(int x) { return (uint32_t)(x + OX) * (uint32_t)sizeof(R); }
-template <class R> __device__ __forceinline__ uint32_t rowb(int y, int P) {
-    return (uint32_t)(y + OY) * (uint32_t)P * (uint32_t)sizeof(R);
+// byte offsets of column x and row y in the padded layout of a field stored as S
+template <class S> __device__ __forceinline__ uint32_t colb(int x) { return (uint32_t)(x + OX) * (uint32_t)sizeof(S); }
+template <class S> __device__ __forceinline__ uint32_t rowb(int y, int P) {
+    return (uint32_t)(y + OY) * (uint32_t)P * (uint32_t)sizeof(S);
 }
 
 // ------------------------------------------------------------------ cross-lane moves
@@ -98,14 +258,16 @@ template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
 }
 template <int CTRL> __device__ __forceinline__ double dpp_f(double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
-    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    const int lo_ = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi_ = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi_ << 32) | (uint32_t)lo_);
 }
 __device__ __forceinline__ float sh_xp(float v) { return dpp_f<DPP_WAVE_SHL1>(v); }
 __device__ __forceinline__ float sh_xm(float v) { return dpp_f<DPP_WAVE_SHR1>(v); }
 __device__ __forceinline__ double sh_xp(double v) { return dpp_f<DPP_WAVE_SHL1>(v); }
 __device__ __forceinline__ double sh_xm(double v) { return dpp_f<DPP_WAVE_SHR1>(v); }
+template <int K> __device__ __forceinline__ lo<K> sh_xp(lo<K> v) { return lo<K>::raw(sh_xp(v.v)); }
+template <int K> __device__ __forceinline__ lo<K> sh_xm(lo<K> v) { return lo<K>::raw(sh_xm(v.v)); }
 
 // ------------------------------------------------------------------ static fields
 
@@ -117,9 +279,9 @@ struct StaticRefs {
 };
 
 // byte offset of (x, y) in a static field that may be stored 1-D (one value per row)
-template <class R> __device__ __forceinline__ uint32_t scolb(int x, int is2d) { return is2d ? colb<R>(x) : 0u; }
-template <class R> __device__ __forceinline__ uint32_t srowb(int y, int P, int is2d) {
-    return is2d ? rowb<R>(y, P) : (uint32_t)(y + OY) * (uint32_t)sizeof(R);
+template <class S> __device__ __forceinline__ uint32_t scolb(int x, int is2d) { return is2d ? colb<S>(x) : 0u; }
+template <class S> __device__ __forceinline__ uint32_t srowb(int y, int P, int is2d) {
+    return is2d ? rowb<S>(y, P) : (uint32_t)(y + OY) * (uint32_t)sizeof(S);
 }
 
 // continuity variant (continuity.jl:91-97), a runtime switch uniform over the launch
@@ -128,7 +290,9 @@ enum { CONT_ITSELF = 0, CONT_RELAX = 1, CONT_FORCING = 2 };
 // ------------------------------------------------------------------ ghost copies on store
 // ghost_points.jl:47-133: each interior value is written together with the halo cells the
 // reference copies from it.  The common case (no ghost cell) costs one test of a flag
-// precomputed per lane (x-copies) and a uniform row test (y-copies).
+// precomputed per lane (x-copies) and a uniform row test (y-copies).  Values are in the
+// format of the field the reference updates (Tprog, arithmetic type Q); the store rounds
+// them to the field's storage S.  Periodic copies pass through Tcomm (cm).
 
 // lanes whose u, v or η value is the source of an x-direction ghost copy
 struct GhostLanes {
@@ -142,81 +306,85 @@ template <bool PER> __device__ __forceinline__ GhostLanes ghost_lanes(int x, int
     return g;
 }
 
-template <class R, bool PER>
-__device__ __forceinline__ void ghost_u(const Fld<R> &b, int x, int y, R w, int nx, int ny, int P, R oma) {
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void ghost_u(const Fld<S> &b, int x, int y, Q w, int nx, int ny, int P, Q oma, int cm) {
     if (PER) {                                   // ghost_points_u_periodic!  :60-73
-        const uint32_t so = rowb<R>(y, P);
-        if (x == nx - 2) b.st(w, colb<R>(-2), so);
-        if (x == nx - 1) b.st(w, colb<R>(-1), so);
-        if (x == 0) b.st(w, colb<R>(nx), so);
-        if (x == 1) b.st(w, colb<R>(nx + 1), so);
+        const Q gw = comm_round(w, cm);
+        const uint32_t so = rowb<S>(y, P);
+        if (x == nx - 2) b.st(gw, colb<S>(-2), so);
+        if (x == nx - 1) b.st(gw, colb<S>(-1), so);
+        if (x == 0) b.st(gw, colb<S>(nx), so);
+        if (x == 1) b.st(gw, colb<S>(nx + 1), so);
     } else {                                     // ghost_points_u_nonperiodic!  :47-57
-        const R gw = oma * w;
-        const uint32_t vo = colb<R>(x);
-        if (y == 1) b.st(gw, vo, rowb<R>(0, P));
-        if (y == 2) b.st(gw, vo, rowb<R>(-1, P));
-        if (y == ny) b.st(gw, vo, rowb<R>(ny + 1, P));
-        if (y == ny - 1) b.st(gw, vo, rowb<R>(ny + 2, P));
+        const Q gw = oma * w;
+        const uint32_t vo = colb<S>(x);
+        if (y == 1) b.st(gw, vo, rowb<S>(0, P));
+        if (y == 2) b.st(gw, vo, rowb<S>(-1, P));
+        if (y == ny) b.st(gw, vo, rowb<S>(ny + 1, P));
+        if (y == ny - 1) b.st(gw, vo, rowb<S>(ny + 2, P));
     }
 }
 
-template <class R, bool PER>
-__device__ __forceinline__ void ghost_v(const Fld<R> &b, int x, int y, R w, int nx, int P, R oma) {
-    const uint32_t so = rowb<R>(y, P);
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void ghost_v(const Fld<S> &b, int x, int y, Q w, int nx, int P, Q oma, int cm) {
+    const uint32_t so = rowb<S>(y, P);
     if (PER) {                                   // ghost_points_v_periodic!  :89-99
-        if (x == nx - 1) b.st(w, colb<R>(-1), so);
-        if (x == nx) b.st(w, colb<R>(0), so);
-        if (x == 1) b.st(w, colb<R>(nx + 1), so);
-        if (x == 2) b.st(w, colb<R>(nx + 2), so);
+        const Q gw = comm_round(w, cm);
+        if (x == nx - 1) b.st(gw, colb<S>(-1), so);
+        if (x == nx) b.st(gw, colb<S>(0), so);
+        if (x == 1) b.st(gw, colb<S>(nx + 1), so);
+        if (x == 2) b.st(gw, colb<S>(nx + 2), so);
     } else {                                     // ghost_points_v_nonperiodic!  :76-86
-        const R gw = oma * w;
-        if (x == 1) b.st(gw, colb<R>(0), so);
-        if (x == 2) b.st(gw, colb<R>(-1), so);
-        if (x == nx) b.st(gw, colb<R>(nx + 1), so);
-        if (x == nx - 1) b.st(gw, colb<R>(nx + 2), so);
+        const Q gw = oma * w;
+        if (x == 1) b.st(gw, colb<S>(0), so);
+        if (x == 2) b.st(gw, colb<S>(-1), so);
+        if (x == nx) b.st(gw, colb<S>(nx + 1), so);
+        if (x == nx - 1) b.st(gw, colb<S>(nx + 2), so);
     }
 }
 
-template <class R, bool PER>
-__device__ __forceinline__ void ghost_e(const Fld<R> &b, int x, int y, R w, int nx, int ny, int P) {
-    // ghost_points_η_(non)periodic!  :102-133: x-copies first, then y-copies of whole rows
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void ghost_e(const Fld<S> &b, int x, int y, Q w, int nx, int ny, int P, int cm) {
+    // ghost_points_η_(non)periodic!  :102-133: x-copies first (through Tcomm when
+    // periodic), then y-copies of whole rows
     const bool gy0 = y == 1, gy1 = y == ny;
-    const uint32_t s0 = rowb<R>(0, P), s1 = rowb<R>(ny + 1, P), so = rowb<R>(y, P);
-    const uint32_t vo = colb<R>(x);
+    const uint32_t s0 = rowb<S>(0, P), s1 = rowb<S>(ny + 1, P), so = rowb<S>(y, P);
+    const uint32_t vo = colb<S>(x);
     if (gy0) b.st(w, vo, s0);
     if (gy1) b.st(w, vo, s1);
+    const Q gw = PER ? comm_round(w, cm) : w;
     if (x == (PER ? nx : 1)) {
-        const uint32_t vg = colb<R>(0);
-        b.st(w, vg, so);
-        if (gy0) b.st(w, vg, s0);
-        if (gy1) b.st(w, vg, s1);
+        const uint32_t vg = colb<S>(0);
+        b.st(gw, vg, so);
+        if (gy0) b.st(gw, vg, s0);
+        if (gy1) b.st(gw, vg, s1);
     }
     if (x == (PER ? 1 : nx)) {
-        const uint32_t vg = colb<R>(nx + 1);
-        b.st(w, vg, so);
-        if (gy0) b.st(w, vg, s0);
-        if (gy1) b.st(w, vg, s1);
+        const uint32_t vg = colb<S>(nx + 1);
+        b.st(gw, vg, so);
+        if (gy0) b.st(gw, vg, s0);
+        if (gy1) b.st(gw, vg, s1);
     }
 }
 
 // store of an interior value plus its ghost copies
-template <class R, bool PER>
-__device__ __forceinline__ void put_u(const Fld<R> &b, int x, uint32_t vo, int y, uint32_t so, R w, bool glane, int nx,
-                                      int ny, int P, R oma) {
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void put_u(const Fld<S> &b, int x, uint32_t vo, int y, uint32_t so, Q w, bool glane, int nx,
+                                      int ny, int P, Q oma, int cm) {
     b.st(w, vo, so);
-    if (PER ? glane : (y <= 2 || y >= ny - 1)) ghost_u<R, PER>(b, x, y, w, nx, ny, P, oma);
+    if (PER ? glane : (y <= 2 || y >= ny - 1)) ghost_u<S, PER>(b, x, y, w, nx, ny, P, oma, cm);
 }
-template <class R, bool PER>
-__device__ __forceinline__ void put_v(const Fld<R> &b, int x, uint32_t vo, int y, uint32_t so, R w, bool glane, int nx,
-                                      int P, R oma) {
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void put_v(const Fld<S> &b, int x, uint32_t vo, int y, uint32_t so, Q w, bool glane, int nx,
+                                      int P, Q oma, int cm) {
     b.st(w, vo, so);
-    if (glane) ghost_v<R, PER>(b, x, y, w, nx, P, oma);
+    if (glane) ghost_v<S, PER>(b, x, y, w, nx, P, oma, cm);
 }
-template <class R, bool PER>
-__device__ __forceinline__ void put_e(const Fld<R> &b, int x, uint32_t vo, int y, uint32_t so, R w, bool glane, int nx,
-                                      int ny, int P) {
+template <class S, bool PER, class Q>
+__device__ __forceinline__ void put_e(const Fld<S> &b, int x, uint32_t vo, int y, uint32_t so, Q w, bool glane, int nx,
+                                      int ny, int P, int cm) {
     b.st(w, vo, so);
-    if (glane || y == 1 || y == ny) ghost_e<R, PER>(b, x, y, w, nx, ny, P);
+    if (glane || y == 1 || y == ny) ghost_e<S, PER>(b, x, y, w, nx, ny, P, cm);
 }
 
 // ------------------------------------------------------------------ wave -> (strip, segment)

@@ -53,6 +53,23 @@ CASES = {
                                      seasonal_wind_x=True, ωFx=400.0, A=3e-3),
     "np-f32-ssprk2-relax": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", surface_relax=True, t_relax=2.0,
                                 time_scheme="SSPRK2", RKs=3, cfl=1.2),
+    # mixed precision: T = Float16 / BFloat16 right-hand side with Tprog = Float32 state
+    # (time_integration.jl:57-64, 200-228), and ghost copies through a narrower Tcomm
+    "np-f16-f32-smag-quad": dict(T="Float16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5,
+                                 diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                 wind_forcing_x="double_gyre"),
+    "per-bf16-f32-comm-bf16": dict(T="BFloat16", Tprog="Float32", Tcomm="BFloat16", nx=66, L_ratio=1.5, bc="periodic",
+                                   diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                   wind_forcing_x="channel"),
+    "per-f16-f32-const-lin": dict(T="Float16", Tprog="Float32", Tcomm="Float16", nx=62, L_ratio=1.0, bc="periodic",
+                                  diffusion="constant", bottom_drag="linear", topography="ridge", RKo=3, cfl=0.6),
+    "np-bf16-f32-relax": dict(T="BFloat16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", surface_relax=True,
+                              t_relax=2.0, topography="seamount", wind_forcing_x="double_gyre",
+                              tracer_consumption=True),
+    "per-f32-comm-f16": dict(T="Float32", Tcomm="Float16", nx=64, L_ratio=2.0, bc="periodic", diffusion="Smagorinsky",
+                             topography="seamount", wind_forcing_x="channel"),
+    "per-f64-comm-bf16": dict(T="Float64", Tcomm="BFloat16", nx=60, L_ratio=1.0, bc="periodic", time_scheme="SSPRK2",
+                              RKs=3, cfl=1.2),
     # tracer consumption (tracer_advection.jl:262-275)
     "np-f32-consumption": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", tracer_consumption=True,
                                topography="seamount", wind_forcing_x="double_gyre"),
@@ -107,6 +124,8 @@ def test_rk_stages_bitwise(swm, orc, gpu, case):
             r_u0, r_v0, r_e0, r_u1, r_v1, r_e1 = rk[:6]
             # u1 enters the next stage after ghost_points! (time_integration.jl:57,84,118,162)
             r_u1, r_v1, r_e1 = ghosted(swm, S, r_u1, r_v1, r_e1)
+            if P.T != P.Tprog:      # u1rhs = T(u1): the GPU keeps only that (time_integration.jl:62-64)
+                r_u1, r_v1, r_e1 = (swm.to_float64(swm.to_storage(a, P.T), P.T) for a in (r_u1, r_v1, r_e1))
             assert_same(f"{case} stage {k} u1", r_u1, g_u1, T)
             assert_same(f"{case} stage {k} v1", r_v1, g_v1, T)
             assert_same(f"{case} stage {k} η1", r_e1, g_e1, T)
