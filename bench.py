@@ -26,21 +26,88 @@ HBM_PEAK_GBS = 8000.0
 NX = 4096
 
 
-def workload_parameter(swm, T="Float32", nx=NX):
-    return swm.Parameter(T=T, nx=nx, Lx=3840e3, L_ratio=1, bc="nonperiodic", α=0.0, topography="seamount",
+def workload_parameter(swm, T="Float32", nx=NX, Tprog=None):
+    return swm.Parameter(T=T, Tprog=Tprog or T, Tcomm=Tprog or T, nx=nx, Lx=3840e3, L_ratio=1, bc="nonperiodic", α=0.0, topography="seamount",
                          wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky", bottom_drag="quadratic",
                          tracer_advection=True, Ndays=10)
 
 
-def bytes_per_cell_stage(es: int) -> dict:
+def bytes_per_cell_stage(es: int, est: int = None) -> dict:
     """Algorithmic HBM bytes per cell: each kernel reads every field it needs once and
-    writes each output once (u, v, η per field; H static)."""
+    writes each output once.  es: bytes of a Tprog value (u, v, η, u0), est: of a T value
+    (H, and the stage input u1 between RK stages when T differs from Tprog)."""
+    est = es if est is None else est
     return {
-        "stage1": 10 * es,        # read u,v,η,H; write u0(3), u1(3)
-        "stage_mid": 16 * es,     # read u1(3),H,u(3),u0(3); write u0(3),u1(3)
-        "stage4": 10 * es,        # read u1(3),H,u0(3); write u0(3)
-        "dissipation": 6 * es,    # read u0,v0,η0,H; write u,v
+        "stage1": 6 * es + 4 * est,        # read u,v,η (Tprog), H; write u1 (T), u0 (Tprog)
+        "stage_mid": 9 * es + 7 * est,     # read u1 (T), H, u, u0; write u1 (T), u0
+        "stage4": 6 * es + 4 * est,        # read u1 (T), H, u0; write u0
+        "dissipation": 5 * es + est,       # read u0, v0, η0, H; write u, v
     }
+
+
+def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps):
+    """Time `steps` steps of the workload with right-hand side format T and state format
+    Tprog (state resident in HBM), then the four step kernels with HIP events."""
+    S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tprog))
+    G = S.grid
+    prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        done, nan = sim.step(warmup, 1, 0)
+        assert nan == 0
+        t0 = time.perf_counter()
+        done, nan = sim.step(steps, 1 + warmup, warmup * G.dtint)   # returns after a stream sync
+        elapsed = time.perf_counter() - t0
+        assert (done, nan) == (steps, 0), (done, nan)
+        kms = {name: sim.time_kernel(k, kernel_reps)
+               for name, k in (("stage_mid", 0), ("stage1", 1), ("stage4", 2), ("dissipation", 3))}
+        mass, energy = sim.reduce()
+    return S, elapsed, kms, mass, energy
+
+
+def precision_report(swm, nx, days_list):
+    """Errors of the Float32 and the Float16 / BFloat16 right-hand-side forms against the
+    Float64 run of the same seeded state after each of `days_list` model days: relative L2
+    of η, u, v over the interior, relative difference of total energy, and mass."""
+    import numpy as np
+    forms = [("Float64", "Float64"), ("Float32", "Float32"), ("BFloat16", "Float32"), ("Float16", "Float32")]
+    P0 = workload_parameter(swm, T="Float64", nx=nx)
+    G0 = swm.make_setup(P0).grid
+    marks = [int(np.ceil(d * 86400 / G0.dtint)) for d in days_list]
+    results = {}
+    for T, Tp in forms:
+        S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tp))
+        prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+        snaps = {}
+        t0 = time.perf_counter()
+        with swm.Simulation(S) as sim:
+            sim.upload(prog)
+            done = 0
+            for m in marks:
+                while done < m:
+                    n = min(2000, m - done)
+                    d, nan = sim.step(n, 1 + done, done * S.grid.dtint)
+                    assert nan == 0 and d == n, (T, d, nan)
+                    done += n
+                    print(f"# {T}/{Tp}: {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+                g = sim.download()
+                mass, energy = sim.reduce()
+                u, v, e, _ = swm.remove_halo(g.u, g.v, g.η, g.sst, S)
+                snaps[m] = (swm.to_float64(u, Tp), swm.to_float64(v, Tp), swm.to_float64(e, Tp), mass, energy)
+        results[(T, Tp)] = snaps
+    ref = results[("Float64", "Float64")]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    rows = []
+    for (T, Tp), snaps in results.items():
+        if T == "Float64":
+            continue
+        for d, m in zip(days_list, marks):
+            u, v, e, mass, energy = snaps[m]
+            ur, vr, er, mr, enr = ref[m]
+            rows.append({"T": T, "Tprog": Tp, "days": d, "steps": m, "rel_l2_eta": rel(e, er), "rel_l2_u": rel(u, ur),
+                         "rel_l2_v": rel(v, vr), "rel_energy": (energy - enr) / enr, "mass": mass, "mass_f64": mr})
+    for r in rows:
+        print(json.dumps(r), flush=True)
 
 
 def cpu_baseline(swm, orc, sample_steps: int, nx: int):
@@ -100,6 +167,11 @@ def main():
     ap.add_argument("--dtype", default="Float32", choices=["Float32", "Float64"])
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 skips)")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--form-steps", type=int, default=100,
+                    help="steps timed for each reduced-precision right-hand-side form (0 skips them)")
+    ap.add_argument("--precision-days", type=str, default="",
+                    help="comma-separated model days: only run the Float32 / Float16 / BFloat16 forms against "
+                         "Float64 and print their errors at those days, then exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (the reference kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -114,27 +186,15 @@ def main():
 
     from __graft_entry__ import load_oracle, load_package
     swm = load_package()
-    P = workload_parameter(swm, T=args.dtype, nx=args.nx)
-    S = swm.make_setup(P)
-    G = S.grid
-    prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    if args.precision_days:
+        precision_report(swm, args.nx, [float(d) for d in args.precision_days.split(",")])
+        return
     es = 4 if args.dtype == "Float32" else 8
+    # dominant kernel: middle RK stage, HIP events on the library stream
+    S, elapsed, kms, mass, energy = run_form(swm, args.dtype, args.dtype, args.nx, args.steps, args.warmup,
+                                             args.kernel_reps)
+    G = S.grid
     cells = G.nx * G.ny
-
-    with swm.Simulation(S) as sim:
-        sim.upload(prog)
-        done, nan = sim.step(args.warmup, 1, 0)
-        assert nan == 0
-        i0 = 1 + args.warmup
-        t0 = time.perf_counter()
-        done, nan = sim.step(args.steps, i0, args.warmup * G.dtint)   # returns after a stream sync
-        elapsed = time.perf_counter() - t0
-        assert (done, nan) == (args.steps, 0), (done, nan)
-        # dominant kernel: middle RK stage, HIP events on the library stream
-        kms = {name: sim.time_kernel(k, args.kernel_reps)
-               for name, k in (("stage_mid", 0), ("stage1", 1), ("stage4", 2), ("dissipation", 3))}
-        mass, energy = sim.reduce()
-
     value = cells * args.steps / elapsed
     traffic = measured_traffic(f"double_gyre_{G.nx}x{G.ny}", "f32" if es == 4 else "f64")
     bpc = bytes_per_cell_stage(es)
@@ -168,6 +228,16 @@ def main():
         "kernel_ms": kms,
         "diagnostics": {"mass": mass, "energy": energy},
     }
+    if args.form_steps > 0 and args.dtype == "Float32":
+        # the reduced-precision right-hand-side forms: Tprog = Float32 state, T = BFloat16 / Float16
+        out["forms"] = {}
+        for T in ("BFloat16", "Float16"):
+            Sf, el, km, _, _ = run_form(swm, T, "Float32", args.nx, args.form_steps, args.warmup, args.kernel_reps)
+            b = bytes_per_cell_stage(4, 2)
+            out["forms"][T] = {"value": cells * args.form_steps / el, "ms_per_step": el / args.form_steps * 1e3,
+                               "speedup_vs_f32": (elapsed / args.steps) / (el / args.form_steps),
+                               "kernel_ms": km, "bytes_per_cell_step": b["stage1"] + 2 * b["stage_mid"] + b["stage4"]
+                               + b["dissipation"], "stage_mid_GBs": b["stage_mid"] * cells / (km["stage_mid"] * 1e-3) / 1e9}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
     print(json.dumps(out), flush=True)

@@ -133,14 +133,33 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
     VI nu = in_u.ld(vxI, sI), nv = in_v.ld(vxI, sI), ne = in_e.ld(vxI, sI);
     R nH = fH.template ldr<R>(vxT, sT);
+    // The last three input rows (yb-1 .. yb+1) are also the first rows of the segment above,
+    // which reads them at its start.  Reading them here at the start too, while that wave
+    // does, keeps the second read in L2 instead of sending it to HBM at the end of the march.
+    VI tu[3], tv[3], te[3];
+    R tH[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t ri = rowb<SI>(t.yb - 1 + k, P), rt = rowb<ST>(t.yb - 1 + k, P);
+        tu[k] = in_u.ld(vxI, ri); tv[k] = in_v.ld(vxI, ri); te[k] = in_e.ld(vxI, ri);
+        tH[k] = fH.template ldr<R>(vxT, rt);
+    }
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
     for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         if (r < rlast) {
-            nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
-            nH = fH.template ldr<R>(vxT, sT + dT);
+            const int k = r + 1 - (t.yb - 1);                  // uniform
+            if (k >= 0) {
+                nu = k == 0 ? tu[0] : k == 1 ? tu[1] : tu[2];
+                nv = k == 0 ? tv[0] : k == 1 ? tv[1] : tv[2];
+                ne = k == 0 ? te[0] : k == 1 ? te[1] : te[2];
+                nH = k == 0 ? tH[0] : k == 1 ? tH[1] : tH[2];
+            } else {
+                nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
+                nH = fH.template ldr<R>(vxT, sT + dT);
+            }
         }
         if (r >= t.ya && r < t.yb) {                          // row r of u, η is emitted next iteration
             if (MI::need_pr) { npu = pr_u.ld(vxQ, sQ); if (MI::rk) npe = pr_e.ld(vxQ, sQ); }

@@ -102,7 +102,8 @@ struct swm_ctx {
     int P = 0, rows = 0;   // pitch (elements) and allocated rows of 2-D fields
     size_t elems = 0;
     int ep = 0, nux = 0, nuy = 0, nvx = 0, nvy = 0;
-    int L = 64;            // rows per wave segment
+    int L = 64;            // rows per wave segment of the stage kernels
+    int Ld = 64;           // rows per wave segment of the dissipation kernel
     Field u[4], v[4], e[4], sst[2];                          // Tprog
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
@@ -249,22 +250,24 @@ template <class F> struct Launch {
     using ST = typename F::ST;
 
     static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
-    static int nseg(const swm_ctx *ctx) { return (ctx->c.ny + ctx->L - 1) / ctx->L; }
+    static int nseg(const swm_ctx *ctx, int L) { return (ctx->c.ny + L - 1) / L; }
 
-    // Rows per wave segment.  Each segment re-reads 4 halo rows, so longer segments move
-    // fewer bytes; more waves hide more latency.  Measured at 4096^2 on MI355X, throughput
-    // is flat from about two resident waves per slot down to one, and falls off beyond, so
-    // the segment is sized for about two waves per slot, and not below 8 rows.
-    template <bool PER> static int rows_per_wave(const swm_ctx *ctx) {
+    // Rows per wave segment, for `waves_per_slot` resident waves per wave slot of the device
+    // (not below lmin rows).  A segment re-reads the halo rows it shares with its
+    // neighbours; the stage kernels read their top halo at the start, while the segment
+    // above reads the same rows, so the re-read stays in L2 and more, shorter segments pay:
+    // measured at 4096^2 on MI355X, about six waves per slot for the stage kernels and two
+    // for the dissipation kernel are best.
+    template <bool PER> static int rows_per_wave(const swm_ctx *ctx, int waves_per_slot, int lmin) {
         int cus = 0, bpc = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess || cus <= 0) cus = 256;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>),
                                                          WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
             bpc = 1;
-        const long slots = (long)cus * bpc * WAVES_PER_BLOCK;
+        const long slots = (long)cus * bpc * WAVES_PER_BLOCK * waves_per_slot;
         const long ns = nstrips(ctx), ny = ctx->c.ny;
-        const long L = (ny * ns + 2 * slots - 1) / (2 * slots);
-        return (int)std::max<long>(8, std::min<long>(L, std::max<long>(ny, 8)));
+        const long L = (ny * ns + slots - 1) / slots;
+        return (int)std::max<long>(lmin, std::min<long>(L, std::max<long>(ny, lmin)));
     }
 
     static StaticRefs statics(const swm_ctx *ctx) {
@@ -309,7 +312,7 @@ template <class F> struct Launch {
         A.oma = (Q)c.one_minus_alpha;
         A.cont = c.continuity; A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
-        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx);
+        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->L);
         return A;
     }
 
@@ -483,8 +486,8 @@ template <class F> struct Launch {
         A.cD = mkR<R>(c.cD); A.rD = mkR<R>(c.rD); A.cS = mkR<R>(c.cSmag); A.nuB = mkR<R>(c.nuB);
         A.dtd = (Q)c.Dt_diff; A.oma = (Q)c.one_minus_alpha;
         A.cm = c.Tcomm;
-        A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
-        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx);
+        A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->Ld;
+        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->Ld);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
 #define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<F, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
@@ -755,13 +758,21 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         delete ctx;
         return SWM_E_HIP;
     }
-    ctx->L = with_fmt(ctx, [&](auto f) {
+    with_fmt(ctx, [&](auto f) {
         using F = decltype(f);
-        return c.bc == SWM_PERIODIC ? Launch<F>::template rows_per_wave<true>(ctx) : Launch<F>::template rows_per_wave<false>(ctx);
+        const bool per = c.bc == SWM_PERIODIC;
+        ctx->L = per ? Launch<F>::template rows_per_wave<true>(ctx, 6, 4) : Launch<F>::template rows_per_wave<false>(ctx, 6, 4);
+        ctx->Ld = per ? Launch<F>::template rows_per_wave<true>(ctx, 2, 8) : Launch<F>::template rows_per_wave<false>(ctx, 2, 8);
+        return 0;
     });
-    if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {   // tuning override: rows each wavefront marches
+    // tuning overrides: rows each wavefront marches in the stage / dissipation kernels
+    if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {
         const int l = atoi(s);
         if (l >= 4 && l <= 4096) ctx->L = l;
+    }
+    if (const char *s = getenv("SWM_ROWS_PER_WAVE_DISS")) {
+        const int l = atoi(s);
+        if (l >= 4 && l <= 4096) ctx->Ld = l;
     }
     auto bail = [&](int code) {
         swm_destroy(ctx);
