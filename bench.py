@@ -45,6 +45,21 @@ def bytes_per_cell_stage(es: int, est: int = None) -> dict:
     }
 
 
+def fused_bytes_per_cell(es: int, est: int = None) -> dict:
+    """A fused pair of RK stages: the algorithmic bytes of the two stages it performs (each
+    stage's sync-point minimum, as for the separate kernels) and the minimum the fused
+    kernel itself moves (the intermediate stage stays in registers)."""
+    b = bytes_per_cell_stage(es, est)
+    est = es if est is None else est
+    return {"fused12": {"alg": b["stage1"] + b["stage_mid"], "min": 6 * es + 4 * est},     # read u,v,η,H; write u1,u0
+            "fused34": {"alg": b["stage_mid"] + b["stage4"], "min": 9 * es + 4 * est}}     # read u1,H,u,u0; write u0
+
+
+# kernels of one step: unfused RK4 (periodic) and with fused stage pairs (nonperiodic)
+STEP_KERNELS = {"unfused": ("stage1", "stage_mid", "stage_mid", "stage4", "dissipation"),
+                "fused": ("fused12", "fused34", "dissipation")}
+
+
 def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps):
     """Time `steps` steps of the workload with right-hand side format T and state format
     Tprog (state resident in HBM), then the four step kernels with HIP events."""
@@ -61,6 +76,8 @@ def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps):
         assert (done, nan) == (steps, 0), (done, nan)
         kms = {name: sim.time_kernel(k, kernel_reps)
                for name, k in (("stage_mid", 0), ("stage1", 1), ("stage4", 2), ("dissipation", 3))}
+        if S.parameters.bc == "nonperiodic":      # the step runs stages 1+2 and 3+4 as fused pairs
+            kms.update({name: sim.time_kernel(k, kernel_reps) for name, k in (("fused12", 5), ("fused34", 6))})
         mass, energy = sim.reduce()
     return S, elapsed, kms, mass, energy
 
@@ -125,16 +142,19 @@ def cpu_baseline(swm, orc, sample_steps: int, nx: int):
             "seconds": dt}
 
 
-def measured_traffic(workload: str, dtype: str):
-    """HBM bytes per launch of the middle RK stage from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    runs kept in profiles/ (KiB per dispatch, each scaled by what the counter reads for a
-    plain copy of known size); None when no measurement of this workload is recorded."""
+def measured_traffic(workload: str, dtype: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs kept
+    in profiles/ (KiB per dispatch, each scaled by what the counter reads for a plain copy
+    of known size); None when no measurement of this workload and kernel is recorded."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{dtype}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
-    return 1024.0 * (m["fetch_size_kib"] * m["fetch_scale"] + m["write_size_kib"] * m["write_scale"])
+    k = m.get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    return 1024.0 * (k["fetch_size_kib"] * m["fetch_scale"] + k["write_size_kib"] * m["write_scale"])
 
 
 def copy_probe(mib: int, reps: int = 20):
@@ -196,11 +216,16 @@ def main():
     G = S.grid
     cells = G.nx * G.ny
     value = cells * args.steps / elapsed
-    traffic = measured_traffic(f"double_gyre_{G.nx}x{G.ny}", "f32" if es == 4 else "f64")
     bpc = bytes_per_cell_stage(es)
-    alg_bytes = bpc["stage_mid"] * cells
-    achieved = alg_bytes / (kms["stage_mid"] * 1e-3) / 1e9
-    step_bytes = (bpc["stage1"] + 2 * bpc["stage_mid"] + bpc["stage4"] + bpc["dissipation"]) * cells
+    fb = fused_bytes_per_cell(es)
+    alg = {**bpc, **{k: v["alg"] for k, v in fb.items()}}                 # bytes of the stages a kernel performs
+    moved = {**bpc, **{k: v["min"] for k, v in fb.items()}}               # minimum bytes a kernel itself moves
+    step = STEP_KERNELS["fused" if "fused12" in kms else "unfused"]
+    dom = max(set(step), key=lambda k: kms[k] * step.count(k))          # largest share of the step
+    traffic = measured_traffic(f"double_gyre_{G.nx}x{G.ny}", "f32" if es == 4 else "f64", dom)
+    achieved = alg[dom] * cells / (kms[dom] * 1e-3) / 1e9
+    step_bytes = sum(moved[k] for k in step) * cells
+    ref_bytes = sum(bpc[k] for k in STEP_KERNELS["unfused"]) * cells   # per-stage sync points (232 B at FP32)
     out = {
         "metric": "RK4 cell-updates/s at 4096x4096 (nonperiodic double gyre)",
         "value": value,
@@ -219,12 +244,14 @@ def main():
                    "bottom_drag": "quadratic", "alpha": 0.0, "time_scheme": "RK4", "dt_s": G.dtint,
                    "tracer_every": G.nadvstep, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rk_stage_kernel (middle stage)", "bytes_per_cell": bpc["stage_mid"],
-                     "kernel_ms": kms["stage_mid"]},
-        "step_roofline": {"alg_bytes_per_cell_step": step_bytes / cells,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "bytes_per_cell": alg[dom], "min_bytes_moved_per_cell": moved[dom],
+                     "min_moved_GBs": moved[dom] * cells / (kms[dom] * 1e-3) / 1e9, "kernel_ms": kms[dom]},
+        "step_roofline": {"kernels": list(step), "alg_bytes_per_cell_step": step_bytes / cells,
                           "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
-                          "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS},
+                          "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+                          "per_stage_sync_bytes_per_cell_step": ref_bytes / cells,
+                          "per_stage_sync_equivalent_GBs": ref_bytes / (elapsed / args.steps) / 1e9},
         "kernel_ms": kms,
         "diagnostics": {"mass": mass, "energy": energy},
     }
@@ -234,10 +261,11 @@ def main():
         for T in ("BFloat16", "Float16"):
             Sf, el, km, _, _ = run_form(swm, T, "Float32", args.nx, args.form_steps, args.warmup, args.kernel_reps)
             b = bytes_per_cell_stage(4, 2)
+            fstep = STEP_KERNELS["fused" if "fused12" in km else "unfused"]
             out["forms"][T] = {"value": cells * args.form_steps / el, "ms_per_step": el / args.form_steps * 1e3,
                                "speedup_vs_f32": (elapsed / args.steps) / (el / args.form_steps),
-                               "kernel_ms": km, "bytes_per_cell_step": b["stage1"] + 2 * b["stage_mid"] + b["stage4"]
-                               + b["dissipation"], "stage_mid_GBs": b["stage_mid"] * cells / (km["stage_mid"] * 1e-3) / 1e9}
+                               "kernel_ms": km, "bytes_per_cell_step": sum(
+                                   {**b, **{k: v["min"] for k, v in fused_bytes_per_cell(4, 2).items()}}[k] for k in fstep)}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
     print(json.dumps(out), flush=True)

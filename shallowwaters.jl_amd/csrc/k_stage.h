@@ -58,7 +58,8 @@ template <class F> struct StageArgs {
     uint32_t o2_u, o2_v, o2_e;        // second output (u0 / u1'')
     uint32_t qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap), 1-D
     StaticRefs st;
-    Coef<Q> k;
+    Coef<Q> k;                        // stage coefficients (first stage of a fused pair)
+    Coef<Q> k2;                       // second stage of a fused pair
     R Fxt, Fyt, Fet, g, gamma, k12;   // T scalars (k12 = T(1/12))
     Q oma;                            // one_minus_α (Tprog)
     int cont, cm;                     // continuity variant, Tcomm
@@ -76,6 +77,73 @@ __device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, const
     else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * feta.template ldr<R>(vo, so));
     return d;
 }
+
+// ------------------------------------------------------------------ right-hand side of one row
+//
+// rhs_nonlinear!  rhs.jl:24-51 in the y-march of one wavefront: push() takes the stage
+// input at row r (u, v, η and H at row r, f_q at corner row r-1) and leaves the terms of
+// du and dη at row r-1 and of dv at row r-2 (qhv, the p difference ... ), with the rows
+// each stencil needs kept as history.  Arithmetic in T, term by term as the reference.
+template <class R, bool PER> struct RhsPipe {
+    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
+    R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
+    R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
+    // results of the last push
+    R qhv, px;            // row r-1: Arakawa-Hsu qhv and p(x+1) - p(x)  (momentum_u!)
+    R dUdx, dVdy, e_r1;   // row r-1: flux divergence terms and η  (continuity!)
+    R qhu, py;            // row r-2: Arakawa-Hsu qhu and p(y+1) - p(y)  (momentum_v!)
+
+    // qend_1: q at the corner (nx, r-1), read by the wrapped first column when qfix (periodic x = 0)
+    __device__ __forceinline__ void push(R u_0, R v_0, R e_0, R H_0, R fq_1, R qend_1, bool qfix, R half, R quarter,
+                                         R k12, R g) {
+        // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
+        const R h_0 = e_0 + H_0;
+        const R hx_0 = sh_xp(h_0);
+        const R U_0 = u_0 * (half * (hx_0 + h_0));
+        // row r-1: Iy!(h_v), Vflux!  rhs.jl:294-304
+        const R V_1 = v_1 * (half * (h_0 + h_1));
+        // Ixy!(h_q,h)  interpolations.jl:33-45
+        const R hq_1 = quarter * (h_1 + hx_1) + quarter * (h_0 + hx_0);
+        // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
+        const R dvdx_1 = sh_xp(v_1) - v_1;
+        const R dudy_1 = u_0 - u_1;
+        const R q_1 = (fq_1 + dvdx_1 - dudy_1) / hq_1;
+        // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
+        const R uu_1 = u_1 * u_1;
+        const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
+        const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
+        const R p_1 = half * (KEu_1 + KEv_1) + g * e_1;
+        // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
+        const R qxp_1 = sh_xp(q_1);
+        R qxm_1 = sh_xm(q_1);
+        if (PER && qfix) qxm_1 = qend_1;
+        const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
+        const R qb_1 = k12 * (q_2 + q_1 + qxm_1);
+        // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
+        // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
+        const R qg_1 = (PER && qfix) ? k12 * (q_2 + q_1 + qend_2) : k12 * (q_2 + qxm_2 + q_1);
+        const R qd_1 = k12 * (q_2 + q_1 + qxp_2);
+        // PV_ArakawaHsu! qhv  PV_advection.jl:73 and ∂x!(dpdx)  at row r-1
+        const R Vxp_1 = sh_xp(V_1);
+        qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
+        px = sh_xp(p_1) - p_1;
+        // ∂x!(dUdx,U), ∂y!(dVdy,V)  continuity.jl:52-68 at row r-1
+        const R Uxm_1 = sh_xm(U_1);
+        dUdx = U_1 - Uxm_1;
+        dVdy = V_1 - V_2;
+        e_r1 = e_1;
+        // PV_ArakawaHsu! qhu  PV_advection.jl:81 and ∂y!(dpdy)  at row r-2
+        const R qdxm_1 = sh_xm(qd_1);
+        qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
+        py = p_1 - p_2;
+        // advance the row history
+        const R qaxm_1 = sh_xm(qa_1);
+        u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hx_1 = hx_0;
+        U_2 = U_1; U_1 = U_0; Uxm_2 = Uxm_1; V_2 = V_1; Vxp_2 = Vxp_1;
+        q_2 = q_1; qxp_2 = qxp_1; qxm_2 = qxm_1; qaxm_2 = qaxm_1; qb_2 = qb_1; p_2 = p_1;
+        qend_2 = qend_1;
+    }
+};
 
 // ------------------------------------------------------------------ stage kernel (K1)
 //
@@ -118,12 +186,10 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
 
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
 
-    // row history (suffix _1 = r-1, _2 = r-2); the w operands of the combination keep the
-    // stage input in its own format
-    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
+    RhsPipe<R, PER> rhs;
+    // the w operands of the combination: the stage input in its own format at rows r-1 (u,
+    // η) and r-2 (v)
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;
-    R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
-    R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
 
     // Loads run one row ahead of their use: the stencil inputs of row r+1 and the
     // pointwise operands (Prog, u0) of the rows emitted next iteration are in flight
@@ -169,47 +235,16 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
             if (MI::need_pr) npv = pr_v.ld(vxQ, sQ - dQ);
             if (MI::need_ac) nav = ac_v.ld(vxQ, sQ - dQ);
         }
-        // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
-        const R h_0 = e_0 + H_0;
-        const R hx_0 = sh_xp(h_0);
-        const R U_0 = u_0 * (half * (hx_0 + h_0));
-        // row r-1: Iy!(h_v), Vflux!  rhs.jl:294-304
-        const R V_1 = v_1 * (half * (h_0 + h_1));
-        // Ixy!(h_q,h)  interpolations.jl:33-45
-        const R hq_1 = quarter * (h_1 + hx_1) + quarter * (h_0 + hx_0);
-        // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
-        const R dvdx_1 = sh_xp(v_1) - v_1;
-        const R dudy_1 = u_0 - u_1;
-        const R q_1 = (fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d)) + dvdx_1 - dudy_1) / hq_1;
-        // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
-        const R uu_1 = u_1 * u_1;
-        const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
-        const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
-        const R p_1 = half * (KEu_1 + KEv_1) + A.g * e_1;
-        // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
-        const R qxp_1 = sh_xp(q_1);
-        R qxm_1 = sh_xm(q_1);
-        R qend_1 = 0;
-        if (PER) {
-            qend_1 = fqend.template ldr<R>(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(ST));
-            if (qfix) qxm_1 = qend_1;
-        }
-        const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
-        const R qb_1 = k12 * (q_2 + q_1 + qxm_1);
-        // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
-        // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
-        const R qg_1 = (PER && qfix) ? k12 * (q_2 + q_1 + qend_2) : k12 * (q_2 + qxm_2 + q_1);
-        const R qd_1 = k12 * (q_2 + q_1 + qxp_2);
+        const R fq_1 = fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d));
+        const R qend_1 = PER ? fqend.template ldr<R>(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(ST)) : R(0);
+        rhs.push(u_0, v_0, e_0, H_0, fq_1, qend_1, qfix, half, quarter, k12, A.g);
 
-        // du at row r-1: PV_ArakawaHsu! qhv  PV_advection.jl:73 ; momentum_u!  rhs.jl:242
-        const R Vxp_1 = sh_xp(V_1);
-        const R p_xp1 = sh_xp(p_1);
+        // du at row r-1: momentum_u!  rhs.jl:242
         const int yu = r - 1;
         const bool row1 = yu >= t.ya && yu < t.yb;
         if (u_col && row1) {
-            const R qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
             const R Fx = fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d));
-            const Q du = (toQ<Q>(qhv) - toQ<Q>(p_xp1 - p_1)) + toQ<Q>(A.Fxt * Fx);
+            const Q du = (toQ<Q>(rhs.qhv) - toQ<Q>(rhs.px)) + toQ<Q>(A.Fxt * Fx);
             Q o1 = 0, o2 = 0;
             combine<M>((Q)wu_1, MI::first ? (Q)wu_1 : pu_1, au_1, du, A.k, o1, o2);
             if (MI::out1) put_u<ST, PER>(o1_u, x, vxT, yu, sT - dT, o1, gl.u, nx, ny, P, A.oma, A.cm);
@@ -218,10 +253,9 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
                 else o2_u.st(o2, vxQ, sQ - dQ);
             }
         }
-        // dη at row r-1: ∂x!(dUdx,U), ∂y!(dVdy,V), continuity!  continuity.jl:52-98
-        const R Uxm_1 = sh_xm(U_1);
+        // dη at row r-1: continuity!  continuity.jl:52-98
         if (MI::rk && e_col && row1) {
-            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
+            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
             Q o1 = 0, o2 = 0;
             combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2);
             if (MI::out1) put_e<ST, PER>(o1_e, x, vxT, yu, sT - dT, o1, gl.e, nx, ny, P, A.cm);
@@ -230,13 +264,11 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
                 else o2_e.st(o2, vxQ, sQ - dQ);
             }
         }
-        // dv at row r-2: PV_ArakawaHsu! qhu  PV_advection.jl:81 ; momentum_v!  rhs.jl:271
-        const R qdxm_1 = sh_xm(qd_1);
+        // dv at row r-2: momentum_v!  rhs.jl:271
         const int yv = r - 2;
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
-            const R qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
             const R Fy = fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d));
-            const Q dv = -(toQ<Q>(qhu) + toQ<Q>(p_1 - p_2)) + toQ<Q>(A.Fyt * Fy);
+            const Q dv = -(toQ<Q>(rhs.qhu) + toQ<Q>(rhs.py)) + toQ<Q>(A.Fyt * Fy);
             Q o1 = 0, o2 = 0;
             combine<M>((Q)wv_2, MI::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, o1, o2);
             if (MI::out1) put_v<ST, PER>(o1_v, x, vxT, yv, sT - 2 * dT, o1, gl.v, nx, P, A.oma, A.cm);
@@ -245,13 +277,167 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
                 else o2_v.st(o2, vxQ, sQ - 2 * dQ);
             }
         }
-        // advance the row history
-        const R qaxm_1 = sh_xm(qa_1);
-        u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hx_1 = hx_0;
         wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
-        U_2 = U_1; U_1 = U_0; Uxm_2 = Uxm_1; V_2 = V_1; Vxp_2 = Vxp_1;
-        q_2 = q_1; qxp_2 = qxp_1; qxm_2 = qxm_1; qaxm_2 = qaxm_1; qb_2 = qb_1; p_2 = p_1;
-        qend_2 = qend_1;
+    }
+}
+
+// ------------------------------------------------------------------ two fused RK stages
+//
+// Two consecutive stages of the classic RK scheme in one march (nonperiodic boundaries):
+// stage A (mode MA) computes u1, η1 at row r-1 and v1 at row r-2 from the input row r,
+// applies ghost_points!(u1,v1,η1) (ghost_points.jl:47-133, time_integration.jl:57) to them
+// in registers, and stage B (mode MB) consumes them two rows later as its input; A's second
+// output (u0) waits in registers for B's combination.  Only B's outputs are stored, so the
+// intermediate stage costs no HBM traffic.  Strips carry an x-halo of 4 lanes (A is valid on
+// lanes 2..61, B on 4..59).  Results are those of the two stage kernels, bit for bit.
+constexpr int HALO2 = 4;
+constexpr int WOUT2 = WAVE - 2 * HALO2;
+
+template <class F, int MA, int MB>
+__global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
+    using MIA = Mode<MA>;
+    using MIB = Mode<MB>;
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+    using SI = StageIn<F, MA>;
+    using VI = typename Sto<SI>::V;
+    static_assert(MIA::rk && MIB::rk && !MIB::first && MIA::out1, "fused pair of classic RK stages");
+    const WaveTile t = wave_tile<HALO2>(A.nstrips, A.nseg, A.L, A.ny);
+    if (!t.live) return;
+    const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
+    const bool laneB = lane >= HALO2 && lane < WAVE - HALO2;
+    const bool u_col = laneB && x >= 1 && x <= nx - 1;
+    const bool v_col = laneB && x >= 1 && x <= nx;
+    const bool e_col = v_col;
+    const GhostLanes gl = ghost_lanes<false>(x, nx);
+    // waves whose stage-A lanes (2..61) reach the x-halo apply the ghost copies across lanes
+    const bool xedge = __any((lane >= 2 && lane < WAVE - 2) && (x <= 0 || x >= nx + 1)) != 0;
+    const bool u_halo = x <= 0 || x >= nx;             // normal-velocity halo of u: never written (0)
+
+    const Arena ar(A.ar);
+    const Fld<SI> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e};
+    const Fld<Q> pr_u{&ar, A.pr_u}, pr_v{&ar, A.pr_v}, pr_e{&ar, A.pr_e};
+    const Fld<Q> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
+    const Fld<ST> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
+    const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
+    const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy};
+    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
+    const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
+    const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
+    const Q oma = A.oma;
+
+    RhsPipe<R, false> ra, rb;
+    VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
+    // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
+    Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
+    Q U0_0 = 0, U0_1 = 0, U0_2 = 0, E0_0 = 0, E0_1 = 0, E0_2 = 0, V0_0 = 0, V0_1 = 0, V0_2 = 0;
+    R H_1 = 0, H_2 = 0;                                // H at rows r-1, r-2
+
+    // A marches input rows ya-2 .. yb+3 (its outputs from row ya-1 on are valid); B takes
+    // A's row r-2 at iteration r and emits u, η at row r-3 and v at row r-4.
+    int r = t.ya - 2;
+    const int rlast = t.yb + 3;
+    VI nu = in_u.ld(vxI, rowb<SI>(r, P)), nv = in_v.ld(vxI, rowb<SI>(r, P)), ne = in_e.ld(vxI, rowb<SI>(r, P));
+    R nH = fH.template ldr<R>(vxT, rowb<ST>(r, P));
+    Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;      // A's Prog / u0 operands (rows r-1, r-2)
+    Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
+    const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
+    uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
+    for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
+        const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
+        const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
+        const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+        const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
+        if (r < rlast) {
+            nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
+            nH = fH.template ldr<R>(vxT, sT + dT);
+        }
+        {   // A's pointwise operands for the next iteration: rows r (u, η) and r-1 (v)
+            if (MIA::need_pr) { npu = pr_u.ld(vxQ, sQ); npe = pr_e.ld(vxQ, sQ); npv = pr_v.ld(vxQ, sQ - dQ); }
+            if (MIA::need_ac) { nau = ac_u.ld(vxQ, sQ); nae = ac_e.ld(vxQ, sQ); nav = ac_v.ld(vxQ, sQ - dQ); }
+            // B's Prog operands for the next iteration: rows r-2 (u, η) and r-3 (v)
+            if (MIB::need_pr) { bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ); }
+        }
+
+        // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
+        ra.push(u_0, v_0, e_0, H_0, fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d)), R(0), false, half,
+                quarter, k12, A.g);
+        Q cu1, cu0, ce1, ce0, cv1, cv0;
+        {
+            const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) +
+                         toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(r - 1, P, A.st.Fx_2d)));
+            combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0);
+            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
+            combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0);
+            const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) +
+                         toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(r - 2, P, A.st.Fy_2d)));
+            combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0);
+        }
+        // ghost_points!(u1, v1, η1) on A's first outputs (nonperiodic): u's normal halo is 0,
+        // its tangential rows mirror with (1-α); v's normal rows are 0, its columns mirror
+        // with (1-α); η copies its neighbour row / column.  Row 0 (first segment) copies row
+        // 1, which A has not made yet: B fixes it when it reads row 0.
+        if (u_halo) cu1 = Q(0);
+        if (r - 1 == ny + 1) cu1 = oma * U1_0;
+        if (r - 1 == ny + 2) cu1 = oma * U1_1;
+        if (r - 1 == ny + 1) ce1 = E1_0;
+        if (r - 2 <= 0 || r - 2 >= ny) cv1 = Q(0);
+        if (xedge) {
+            const Q vp1 = sh_xp(cv1), vm1 = sh_xm(cv1);
+            const Q vp3 = __shfl(cv1, lane + 3), vm3 = __shfl(cv1, lane - 3);
+            const Q ep1 = sh_xp(ce1), em1 = sh_xm(ce1);
+            if (x == 0) { cv1 = oma * vp1; ce1 = ep1; }
+            if (x == -1) cv1 = oma * vp3;
+            if (x == nx + 1) { cv1 = oma * vm1; ce1 = em1; }
+            if (x == nx + 2) cv1 = oma * vm3;
+        }
+        U1_1 = U1_0; U1_0 = cu1; E1_1 = E1_0; E1_0 = ce1; V1_0 = cv1;
+        U0_2 = U0_1; U0_1 = U0_0; U0_0 = cu0; E0_2 = E0_1; E0_1 = E0_0; E0_0 = ce0;
+        V0_2 = V0_1; V0_1 = V0_0; V0_0 = cv0;
+
+        // ---- stage B at its input row r-2
+        const int rbi = r - 2;
+        if (rbi >= t.ya - 1) {
+            const Q ub = rbi == 0 ? oma * U1_0 : U1_1;          // row 0: (1-α)·u1 of row 1
+            const Q eb = rbi == 0 ? E1_0 : E1_1;                // row 0: η1 of row 1
+            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, fq.template ldr<R>(vfq, srowb<ST>(rbi - 1, P, A.st.fq_2d)),
+                    R(0), false, half, quarter, k12, A.g);
+            const int yu = r - 3;
+            if (yu >= t.ya && yu < t.yb) {
+                if (u_col) {
+                    const Q du = (toQ<Q>(rb.qhv) - toQ<Q>(rb.px)) +
+                                 toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d)));
+                    Q o1 = 0, o2 = 0;
+                    combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2);
+                    if (MIB::out1) put_u<ST, false>(o1_u, x, vxT, yu, sT - 3 * dT, o1, gl.u, nx, ny, P, oma, A.cm);
+                    if (MIB::ghost2) put_u<Q, false>(o2_u, x, vxQ, yu, sQ - 3 * dQ, o2, gl.u, nx, ny, P, oma, A.cm);
+                    else o2_u.st(o2, vxQ, sQ - 3 * dQ);
+                }
+                if (e_col) {
+                    const Q deta = continuity_rhs<Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, eref, feta, vxT, sT - 3 * dT,
+                                                     A.gamma, A.Fet);
+                    Q o1 = 0, o2 = 0;
+                    combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2);
+                    if (MIB::out1) put_e<ST, false>(o1_e, x, vxT, yu, sT - 3 * dT, o1, gl.e, nx, ny, P, A.cm);
+                    if (MIB::ghost2) put_e<Q, false>(o2_e, x, vxQ, yu, sQ - 3 * dQ, o2, gl.e, nx, ny, P, A.cm);
+                    else o2_e.st(o2, vxQ, sQ - 3 * dQ);
+                }
+            }
+            const int yv = r - 4;
+            if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
+                const Q dv = -(toQ<Q>(rb.qhu) + toQ<Q>(rb.py)) +
+                             toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d)));
+                Q o1 = 0, o2 = 0;
+                combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2);
+                if (MIB::out1) put_v<ST, false>(o1_v, x, vxT, yv, sT - 4 * dT, o1, gl.v, nx, P, oma, A.cm);
+                if (MIB::ghost2) put_v<Q, false>(o2_v, x, vxQ, yv, sQ - 4 * dQ, o2, gl.v, nx, P, oma, A.cm);
+                else o2_v.st(o2, vxQ, sQ - 4 * dQ);
+            }
+        }
+        wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
+        H_2 = H_1; H_1 = H_0;
     }
 }
 

@@ -102,8 +102,10 @@ struct swm_ctx {
     int P = 0, rows = 0;   // pitch (elements) and allocated rows of 2-D fields
     size_t elems = 0;
     int ep = 0, nux = 0, nuy = 0, nvx = 0, nvy = 0;
+    bool fuse = true;      // run consecutive RK stages as fused pairs where supported
     int L = 64;            // rows per wave segment of the stage kernels
     int Ld = 64;           // rows per wave segment of the dissipation kernel
+    int Lf = 64;           // rows per wave segment of the fused stage pairs
     Field u[4], v[4], e[4], sst[2];                          // Tprog
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
@@ -250,24 +252,29 @@ template <class F> struct Launch {
     using ST = typename F::ST;
 
     static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
+    static int nstrips2(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT2 - 1) / WOUT2; }
     static int nseg(const swm_ctx *ctx, int L) { return (ctx->c.ny + L - 1) / L; }
 
-    // Rows per wave segment, for `waves_per_slot` resident waves per wave slot of the device
-    // (not below lmin rows).  A segment re-reads the halo rows it shares with its
-    // neighbours; the stage kernels read their top halo at the start, while the segment
-    // above reads the same rows, so the re-read stays in L2 and more, shorter segments pay:
-    // measured at 4096^2 on MI355X, about six waves per slot for the stage kernels and two
-    // for the dissipation kernel are best.
-    template <bool PER> static int rows_per_wave(const swm_ctx *ctx, int waves_per_slot, int lmin) {
+    // Rows per wave segment of `kernel` (strips of `nstr`), for `waves_per_slot` resident
+    // waves per wave slot the kernel's occupancy gives (not below lmin rows).  A segment
+    // re-reads the halo rows it shares with its neighbours; the stage kernels read their top
+    // halo at the start, while the segment above reads the same rows, so the re-read stays
+    // in L2 and more, shorter segments pay.  Measured at 4096^2 on MI355X: about six waves
+    // per slot for the stage kernels, five for the fused pairs, two for dissipation.
+    static int rows_per_wave(const swm_ctx *ctx, const void *kernel, int nstr, int waves_per_slot, int lmin) {
         int cus = 0, bpc = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess || cus <= 0) cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>),
-                                                         WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
             bpc = 1;
         const long slots = (long)cus * bpc * WAVES_PER_BLOCK * waves_per_slot;
-        const long ns = nstrips(ctx), ny = ctx->c.ny;
-        const long L = (ny * ns + slots - 1) / slots;
+        const long ny = ctx->c.ny;
+        const long L = (ny * nstr + slots - 1) / slots;
         return (int)std::max<long>(lmin, std::min<long>(L, std::max<long>(ny, lmin)));
+    }
+    template <bool PER> static void segment_lengths(swm_ctx *ctx) {
+        ctx->L = rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>), nstrips(ctx), 6, 4);
+        ctx->Ld = rows_per_wave(ctx, reinterpret_cast<const void *>(&dissipation_kernel<F, PER, 0, true>), nstrips(ctx), 2, 8);
+        ctx->Lf = rows_per_wave(ctx, reinterpret_cast<const void *>(&fused_kernel<F, M_RK_MID, M_RK_LAST>), nstrips2(ctx), 5, 8);
     }
 
     static StaticRefs statics(const swm_ctx *ctx) {
@@ -375,6 +382,35 @@ template <class F> struct Launch {
         return SWM_OK;
     }
 
+    // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
+    static bool fusable(const swm_ctx *ctx) {
+        const swm_config &c = ctx->c;
+        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK;
+    }
+    static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
+                     const Coef<Q> &kb, const Times &T) {
+        // the first stage's outputs stay in registers: check what the pair reads and writes
+        auto ok = [](Slots s, bool tpool) {
+            const int n = tpool ? 2 : 4;
+            return s.t == tpool && s.uv >= 0 && s.uv < n && s.e >= 0 && s.e < n;
+        };
+        const bool good = ok(in, ctx->mixed && ma != M_RK_FIRST) && ok(pr, false) && (ma == M_RK_FIRST || ok(ac, false)) &&
+                          (mb == M_RK_LAST || ok(o1, ctx->mixed)) && ok(o2, false);
+        if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
+        StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
+        A.k2 = kb;
+        A.nstrips = nstrips2(ctx);
+        A.L = ctx->Lf;
+        A.nseg = nseg(ctx, ctx->Lf);
+        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+        if (ma == M_RK_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
+        else if (ma == M_RK_MID && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RK_LAST>), grid, block, 0, ctx->stream, A);
+        else if (ma == M_RK_FIRST && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_LAST>), grid, block, 0, ctx->stream, A);
+        else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
     // semi-implicit continuity of the SSPRK schemes in mode m
     static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
                           const Times &T) {
@@ -407,7 +443,9 @@ template <class F> struct Launch {
         return c.time_scheme == SWM_RK ? c.RKo : c.time_scheme == SWM_SSPRK2 ? c.RKs : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
     }
 
-    static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1) {
+    // keep_u1: leave the last stage's u1 in a slot (a fused pair ending with the scheme's last
+    // stage keeps it in registers only)
+    static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1, bool keep_u1 = false) {
         const swm_config &c = ctx->c;
         if (n_stages > total_stages(c)) n_stages = total_stages(c);
         const Times T = forcing_times(ctx, t);
@@ -417,14 +455,27 @@ template <class F> struct Launch {
         int rc = SWM_OK;
         if (c.time_scheme == SWM_RK) {                                     // :55-78
             const Slots a{pick({P.uv}), pick({P.e})};
+            auto mode = [&](int s) { return s == 1 ? M_RK_FIRST : (s == c.RKo ? M_RK_LAST : M_RK_MID); };
+            auto coef = [&](int s) {
+                Coef<Q> kk{};
+                kk.a = (Q)c.RKaDt[s - 1];
+                kk.b = s < c.RKo ? (Q)c.RKbDt[s - 1] : Q(0);
+                return kk;
+            };
             for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
-                const int m = s == 1 ? M_RK_FIRST : (s == c.RKo ? M_RK_LAST : M_RK_MID);
-                k.a = (Q)c.RKaDt[s - 1];
-                k.b = s < c.RKo ? (Q)c.RKbDt[s - 1] : Q(0);
                 // the next u1: a free Tprog slot, or the other T slot with mixed formats
                 const Slots n = ctx->mixed ? Slots{cur.t ? 1 - cur.uv : 0, cur.t ? 1 - cur.e : 0, true}
                                            : Slots{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
-                rc = stage(ctx, m, cur, P, a, s < c.RKo ? n : none, a, k, T);
+                if (s + 1 <= n_stages && fusable(ctx) && !(keep_u1 && s + 1 == c.RKo)) {   // stages s, s+1 in one kernel
+                    const int s2 = s + 1;
+                    // the first stage reads u0 only after stage 1 wrote it
+                    rc = fused(ctx, mode(s), mode(s2), cur, P, s == 1 ? none : a, s2 < c.RKo ? n : none, a, coef(s),
+                               coef(s2), T);
+                    if (s2 < c.RKo) cur = n;
+                    ++s;
+                    continue;
+                }
+                rc = stage(ctx, mode(s), cur, P, a, s < c.RKo ? n : none, a, coef(s), T);
                 if (s < c.RKo) cur = n;
             }
             z0 = a;
@@ -622,6 +673,8 @@ template <class F> struct Launch {
             case 2: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_LAST : M_SSP2, s1, P, a, none, a, k, T);
             case 3: return dissipation(ctx, a, dq);
             case 4: return tracer_advect(ctx, P.uv, ctx->rsst, 1 - ctx->rsst);
+            case 5: return fused(ctx, M_RK_FIRST, M_RK_MID, P, P, none, s1, a, k, k, T);
+            case 6: return fused(ctx, M_RK_MID, M_RK_LAST, s1, P, a, none, a, k, k, T);
             }
             return ctx->fail(SWM_E_ARG, "unknown kernel %d", which);
         };
@@ -760,15 +813,19 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     }
     with_fmt(ctx, [&](auto f) {
         using F = decltype(f);
-        const bool per = c.bc == SWM_PERIODIC;
-        ctx->L = per ? Launch<F>::template rows_per_wave<true>(ctx, 6, 4) : Launch<F>::template rows_per_wave<false>(ctx, 6, 4);
-        ctx->Ld = per ? Launch<F>::template rows_per_wave<true>(ctx, 2, 8) : Launch<F>::template rows_per_wave<false>(ctx, 2, 8);
+        if (c.bc == SWM_PERIODIC) Launch<F>::template segment_lengths<true>(ctx);
+        else Launch<F>::template segment_lengths<false>(ctx);
         return 0;
     });
     // tuning overrides: rows each wavefront marches in the stage / dissipation kernels
     if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {
         const int l = atoi(s);
         if (l >= 4 && l <= 4096) ctx->L = l;
+    }
+    if (const char *s = getenv("SWM_FUSE")) ctx->fuse = atoi(s) != 0;   // 0: one kernel per stage
+    if (const char *s = getenv("SWM_ROWS_PER_WAVE_FUSED")) {
+        const int l = atoi(s);
+        if (l >= 4 && l <= 4096) ctx->Lf = l;
     }
     if (const char *s = getenv("SWM_ROWS_PER_WAVE_DISS")) {
         const int l = atoi(s);
@@ -932,7 +989,7 @@ int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
                     : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
     if (n_stages < 1 || n_stages > total) return ctx->fail(SWM_E_ARG, "n_stages %d out of 1..%d", n_stages, total);
     (void)i0;
-    int rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1); });
+    int rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1, true); });
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;

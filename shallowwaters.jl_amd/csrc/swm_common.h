@@ -394,6 +394,8 @@ struct WaveTile {
     bool live;
 };
 
+// strips of WAVE lanes with an x-halo of H lanes on each side: outputs in lanes H..WAVE-H-1
+template <int H = HALO>
 __device__ __forceinline__ WaveTile wave_tile(int nstrips, int nseg, int L, int ny) {
     WaveTile t;
     // Blocks are dispatched round-robin over the XCDs (block b to XCD b % NXCD).  The grid is
@@ -406,7 +408,7 @@ __device__ __forceinline__ WaveTile wave_tile(int nstrips, int nseg, int L, int 
     t.lane = (int)(threadIdx.x % WAVE);
     t.live = w < nstrips * nseg;
     const int s = w % nstrips, g = w / nstrips;
-    t.x = -HALO + WOUT * s + t.lane;
+    t.x = -H + (WAVE - 2 * H) * s + t.lane;
     t.ya = 1 + g * L;
     t.yb = min(t.ya + L, ny + 1);
     return t;

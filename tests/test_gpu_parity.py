@@ -257,3 +257,22 @@ def test_reduce_matches_fp64_sums(swm, gpu, case):
     e_ref = (0.5 * g_ * eta * eta + 0.5 * h * uf * uf + 0.5 * h * vf * vf).sum()
     assert abs(mass - m_ref) <= 1e-12 * np.abs(eta).sum(), (mass, m_ref)
     assert abs(energy - e_ref) <= 1e-12 * abs(e_ref), (energy, e_ref)
+
+
+@pytest.mark.parametrize("T,Tprog", [("Float32", "Float32"), ("Float64", "Float64"), ("BFloat16", "Float32")])
+def test_fused_stage_pairs_on_a_wider_grid(swm, orc, gpu, T, Tprog, monkeypatch):
+    """Consecutive RK stages run as fused pairs (nonperiodic); on a grid of several strips
+    and segments both the fused and the one-kernel-per-stage path equal the oracle bitwise."""
+    P = swm.Parameter(T=T, Tprog=Tprog, nx=181, L_ratio=1.7, bc="nonperiodic", α=0.7, diffusion="Smagorinsky",
+                      bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre", Uadv=20.0)
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=23, σ=0.8)
+    r = orc.run(S, swm.make_config(S), prog, 6)
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("SWM_FUSE", fuse)
+        with swm.Simulation(S) as sim:
+            sim.upload(prog)
+            assert sim.step(6, 1, 0) == (6, 0)
+            g = sim.download()
+        for name, a, b in zip(("u", "v", "η", "sst"), r[:4], (g.u, g.v, g.η, g.sst)):
+            assert_same(f"fuse={fuse} {name}", swm.to_float64(a, Tprog), swm.to_float64(b, Tprog), Tprog)
