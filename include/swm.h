@@ -138,9 +138,36 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy);
 /* Measurement hook (no counterpart in the reference): launch kernel `which` `reps` times
  * back to back on the context's stream between two HIP events and return the average
  * duration in milliseconds.  which: 0 middle RK stage (stage 2), 1 first stage, 2 last
- * stage, 3 dissipation, 4 tracer advection.  Operates on the current buffers; the
- * prognostic state is left unchanged except the RK scratch slots. */
+ * stage, 3 dissipation, 4 tracer advection, 5 fused stages 1+2, 6 fused stages 3+4
+ * (nonperiodic).  Operates on the current buffers; the prognostic state is left unchanged
+ * except the RK scratch slots. */
 int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg);
+
+/* ---- Ensembles (no counterpart in the reference, whose "ensemble mode" runs separate
+ * processes with run ids, README.md:88, src/output.jl:168-207).  An ensemble advances
+ * several contexts, of possibly different grid sizes, together: each kernel of a step
+ * (stage or fused stage pair, dissipation) is one batched launch over the tiles of every
+ * member, and pairs of steps without host work (tracer, NaN check) replay one captured
+ * hipGraph.  Members must share the number formats, boundary condition, time scheme
+ * (classic RK), drag, diffusion, continuity variant and device, use nstep_diff = 1, and
+ * have their static fields and a state uploaded; each member remains a context of its own
+ * (download, step) between ensemble calls.  Results equal each member's own swm_step bit
+ * for bit. */
+typedef struct swm_ens swm_ens;
+int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out);
+int swm_ens_destroy(swm_ens *ens);
+const char *swm_ens_last_error(const swm_ens *ens);
+/* Autotuner: time the batched step for candidate rows per wave segment (4..48) of the stage,
+ * fused-pair and dissipation kernels in turn and keep the fastest; the choices are returned
+ * (pointers may be NULL). */
+int swm_ens_tune(swm_ens *ens, int32_t reps, int32_t *rows_stage, int32_t *rows_fused, int32_t *rows_diss);
+/* Steps i0 .. i0+nsteps-1 of every member, member m at model time (i-1)*dtint of m.  NaNs
+ * are checked every nout steps of each member (feedback.jl:136-148); after a step in which
+ * any member has them, all members stop: members with NaNs keep their state from before that
+ * step and nan_steps[m] (n entries) receives the step, else 0.  *steps_done receives the
+ * steps completed, *graph_steps how many of them ran as graph replays. */
+int swm_ens_step(swm_ens *ens, int64_t nsteps, int64_t i0, int64_t *steps_done, int64_t *nan_steps,
+                 int64_t *graph_steps);
 
 #ifdef __cplusplus
 }

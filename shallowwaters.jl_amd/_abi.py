@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libswm_hip.so")
 # every symbol include/swm.h declares
 EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_last_error",
            "swm_set_static", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
-           "swm_download_rk", "swm_reduce", "swm_time_kernel"]
+           "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
+           "swm_ens_last_error", "swm_ens_tune", "swm_ens_step"]
 
 _lib = None
 
@@ -46,8 +47,14 @@ def load():
     L.swm_download_rk.argtypes = [vp] * 7
     L.swm_reduce.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.swm_time_kernel.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]
+    L.swm_ens_create.argtypes = [i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.swm_ens_destroy.argtypes = [vp]
+    L.swm_ens_last_error.argtypes = [vp]
+    L.swm_ens_last_error.restype = ctypes.c_char_p
+    L.swm_ens_tune.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.swm_ens_step.argtypes = [vp, i64, i64, pi64, pi64, pi64]
     for name in EXPORTS:
-        if name != "swm_last_error":
+        if name not in ("swm_last_error", "swm_ens_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -26,11 +26,11 @@ template <class F> struct DissArgs {
 // At input row r the wave forms the row r quantities (KE, sqrtKE, DT), the Laplacians and
 // DS of row r-1, ν and the viscous tensor of rows r-1/r-2, and emits u and v at row r-2.
 template <class F, bool PER, int DRAG, bool SMAG>
-__global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<F> A) {
+__device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
     using R = typename F::R;
     using Q = typename F::Q;
     using ST = typename F::ST;
-    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    const WaveTile t = tile_of<HALO>(w, A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
     const bool lane_ok = t.lane >= HALO && t.lane < WAVE - HALO;
@@ -133,6 +133,20 @@ __global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<F> A) {
         DT_1 = DT_0; DSq_2 = DSq_1; DSqxm_2 = DSqxm_1; nu_2 = nu_1; Lu_2 = Lu_1; Lv_2 = Lv_1; Lvxp_2 = Lvxp_1;
         S11_2 = S11_1; S12_3 = S12_2; S22_2 = S22_1;
     }
+}
+
+template <class F, bool PER, int DRAG, bool SMAG>
+__global__ void __launch_bounds__(256) dissipation_kernel(DissArgs<F> A) {
+    dissipation_body<F, PER, DRAG, SMAG>(A, wave_index());
+}
+
+template <class F, bool PER, int DRAG, bool SMAG>
+__global__ void __launch_bounds__(256) dissipation_batch(const DissArgs<F> *__restrict__ args,
+                                                         const int *__restrict__ start, int nmem) {
+    int m, wl;
+    if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
+    const DissArgs<F> A = args[m];
+    dissipation_body<F, PER, DRAG, SMAG>(A, wl);
 }
 
 }  // namespace swm

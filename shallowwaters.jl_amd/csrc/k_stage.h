@@ -154,14 +154,14 @@ template <class R, bool PER> struct RhsPipe {
 // row-(r-1) quantities h_v, V, h_q, q, KE, p and the Arakawa-Hsu weights, and emits du (and
 // dη) at row r-1 and dv at row r-2.
 template <class F, bool PER, int M>
-__global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
+__device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     using MI = Mode<M>;
     using R = typename F::R;
     using Q = typename F::Q;
     using ST = typename F::ST;
     using SI = StageIn<F, M>;
     using VI = typename Sto<SI>::V;
-    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    const WaveTile t = tile_of<HALO>(w, A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
     const bool lane_ok = t.lane >= HALO && t.lane < WAVE - HALO;
@@ -281,6 +281,21 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
     }
 }
 
+template <class F, bool PER, int M>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
+    stage_body<F, PER, M>(A, wave_index());
+}
+
+// the same stage for every member of an ensemble in one launch (tile table `start`)
+template <class F, bool PER, int M>
+__global__ void __launch_bounds__(256) stage_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start,
+                                                   int nmem) {
+    int m, wl;
+    if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
+    const StageArgs<F> A = args[m];
+    stage_body<F, PER, M>(A, wl);
+}
+
 // ------------------------------------------------------------------ two fused RK stages
 //
 // Two consecutive stages of the classic RK scheme in one march (nonperiodic boundaries):
@@ -294,7 +309,7 @@ constexpr int HALO2 = 4;
 constexpr int WOUT2 = WAVE - 2 * HALO2;
 
 template <class F, int MA, int MB>
-__global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
+__device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
     using R = typename F::R;
@@ -303,7 +318,7 @@ __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
     using SI = StageIn<F, MA>;
     using VI = typename Sto<SI>::V;
     static_assert(MIA::rk && MIB::rk && !MIB::first && MIA::out1, "fused pair of classic RK stages");
-    const WaveTile t = wave_tile<HALO2>(A.nstrips, A.nseg, A.L, A.ny);
+    const WaveTile t = tile_of<HALO2>(w, A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
     const bool laneB = lane >= HALO2 && lane < WAVE - HALO2;
@@ -441,6 +456,20 @@ __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
     }
 }
 
+template <class F, int MA, int MB>
+__global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
+    fused_body<F, MA, MB>(A, wave_index());
+}
+
+template <class F, int MA, int MB>
+__global__ void __launch_bounds__(256) fused_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start,
+                                                   int nmem) {
+    int m, wl;
+    if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
+    const StageArgs<F> A = args[m];
+    fused_body<F, MA, MB>(A, wl);
+}
+
 // ------------------------------------------------------------------ semi-implicit continuity
 //
 // continuity! of the SSPRK schemes (continuity.jl:72-98 with UVfluxes! recomputed from the
@@ -453,7 +482,7 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
     using R = typename F::R;
     using Q = typename F::Q;
     using ST = typename F::ST;
-    const WaveTile t = wave_tile(A.nstrips, A.nseg, A.L, A.ny);
+    const WaveTile t = tile_of<HALO>(wave_index(), A.nstrips, A.nseg, A.L, A.ny);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny;
     const bool e_col = t.lane >= HALO && t.lane < WAVE - HALO && x >= 1 && x <= nx;
@@ -509,10 +538,9 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
 // the periodic AHβ!/AHγ! read q[end,·] for their first column (PV_advection.jl:110,126).
 // SI is the storage of the stage input.
 template <class F, class SI>
-__global__ void qend_kernel(StageArgs<F> A) {
+__device__ __forceinline__ void qend_body(const StageArgs<F> &A, int y) {
     using R = typename F::R;
     using ST = typename F::ST;
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y > A.ny) return;
     const Arena ar(A.ar);
     const Fld<SI> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e};
@@ -530,6 +558,20 @@ __global__ void qend_kernel(StageArgs<F> A) {
     const R dudy = u.template ldr<R>(c0, r1) - u.template ldr<R>(c0, r0);
     const R f = fq.template ldr<R>(scolb<ST>(x, A.st.fq_2d), srowb<ST>(y, P, A.st.fq_2d));
     qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST));
+}
+
+template <class F, class SI>
+__global__ void qend_kernel(StageArgs<F> A) {
+    qend_body<F, SI>(A, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// all members: start[m] is member m's first block of 256 rows
+template <class F, class SI>
+__global__ void qend_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start, int nmem) {
+    int m, bl;
+    if (!batch_locate(start, nmem, (int)blockIdx.x, m, bl)) return;
+    const StageArgs<F> A = args[m];
+    qend_body<F, SI>(A, bl * (int)blockDim.x + (int)threadIdx.x);
 }
 
 }  // namespace swm

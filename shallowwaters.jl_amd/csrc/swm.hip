@@ -90,8 +90,23 @@ double round_host(double x, int fmt) {
 
 }  // namespace
 
+// Kernels of a step recorded instead of launched (ensemble planning): kind 0 single stage
+// (mode ma), 1 fused pair (ma, mb), 2 dissipation, 3 semi-implicit continuity; args bytes.
+struct RecItem {
+    int kind, ma, mb;
+    std::vector<char> args;
+};
+struct Recorder {
+    std::vector<RecItem> items;
+    void add(int kind, int ma, int mb, const void *a, size_t n) {
+        RecItem it{kind, ma, mb, std::vector<char>((const char *)a, (const char *)a + n)};
+        items.push_back(std::move(it));
+    }
+};
+
 struct swm_ctx {
     swm_config c;
+    Recorder *rec = nullptr;   // set while an ensemble plans a step
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -354,6 +369,10 @@ template <class F> struct Launch {
         const bool per = c.bc == SWM_PERIODIC;
         if (!slots_ok(ctx, m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
         StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
+        if (ctx->rec) {
+            ctx->rec->add(0, m, -1, &A, sizeof A);
+            return SWM_OK;
+        }
         if (per) {
             const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
             if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
@@ -402,6 +421,10 @@ template <class F> struct Launch {
         A.nstrips = nstrips2(ctx);
         A.L = ctx->Lf;
         A.nseg = nseg(ctx, ctx->Lf);
+        if (ctx->rec) {
+            ctx->rec->add(1, ma, mb, &A, sizeof A);
+            return SWM_OK;
+        }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         if (ma == M_RK_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
         else if (ma == M_RK_MID && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RK_LAST>), grid, block, 0, ctx->stream, A);
@@ -421,6 +444,10 @@ template <class F> struct Launch {
             if (m <= M_RK_LAST || !slots_ok(ctx, m, in, pr, ac, o1, o2, true))
                 return ctx->fail(SWM_E_STATE, "continuity mode %d: buffer slot missing", m);
             StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
+            if (ctx->rec) {
+                ctx->rec->add(3, m, -1, &A, sizeof A);
+                return SWM_OK;
+            }
             const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
@@ -539,6 +566,10 @@ template <class F> struct Launch {
         A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->Ld;
         A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->Ld);
+        if (ctx->rec) {
+            ctx->rec->add(2, 0, 0, &A, sizeof A);
+            return SWM_OK;
+        }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
 #define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<F, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
@@ -761,6 +792,358 @@ template <class F> int run_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64
     }
     return SWM_OK;
 }
+
+}  // namespace
+
+// ====================================================================== ensembles
+//
+// An ensemble advances several contexts together.  A step of every member is planned on
+// the host (the members' kernels recorded, not launched, with their slots and scalars);
+// all members run the same kernel sequence, so each kernel of the step becomes one batched
+// launch whose waves find their member in a table of tile counts.  The packed plan (every
+// member's arguments and the tables) lives in one of two device buffers: a context's slot
+// roles repeat every two steps, so two buffers hold the steady state, and two steps without
+// host work (tracer, NaN check) replay one captured hipGraph that reads them.
+
+struct swm_ens {
+    std::vector<swm_ctx *> m;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int fmt = SWM_F32;
+    bool per = false;
+    struct Plan {                      // one step of every member, packed for upload
+        std::vector<int> kind, ma, mb; // per kernel of the step
+        std::vector<size_t> args_off, tab_off, qtab_off;
+        std::vector<int> tiles, qblocks;
+        std::vector<char> blob;        // what the device buffer holds
+        std::vector<int> next_pu, next_pe, next_uv;   // member slots after the step
+    };
+    struct Buf {
+        Plan plan;
+        void *dev = nullptr;
+        size_t cap = 0;
+        bool valid = false;
+    };
+    Buf buf[2];
+    int last = 1;                      // buffer of the previous eager step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_ok = false;
+
+    int fail(int code, const char *fmt_, ...) {
+        char b[512];
+        va_list ap;
+        va_start(ap, fmt_);
+        vsnprintf(b, sizeof b, fmt_, ap);
+        va_end(ap);
+        err = b;
+        return code;
+    }
+};
+
+namespace {
+
+template <class F> struct Ens {
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+
+    // record step i of every member (time (i-1)*dtint of the member) into a packed plan
+    static int plan(swm_ens *E, int64_t i, swm_ens::Plan &pl) {
+        const size_t nm = E->m.size();
+        std::vector<Recorder> recs(nm);
+        pl = swm_ens::Plan();
+        for (size_t k = 0; k < nm; ++k) {
+            swm_ctx *ctx = E->m[k];
+            ctx->rec = &recs[k];
+            Slots u0, u1;
+            int rc = Launch<F>::stages(ctx, 1 << 20, (i - 1) * ctx->c.dtint, &u0, &u1);
+            const int out = pick({ctx->pu, u0.uv});
+            if (!rc) rc = Launch<F>::dissipation(ctx, u0, out);
+            ctx->rec = nullptr;
+            if (rc) return E->fail(rc, "member %zu: %s", k, ctx->err.c_str());
+            pl.next_pu.push_back(out);
+            pl.next_pe.push_back(u0.e);
+            pl.next_uv.push_back(out);
+        }
+        const size_t nk = recs[0].items.size();
+        for (size_t k = 1; k < nm; ++k) {
+            bool same = recs[k].items.size() == nk;
+            for (size_t j = 0; same && j < nk; ++j)
+                same = recs[k].items[j].kind == recs[0].items[j].kind && recs[k].items[j].ma == recs[0].items[j].ma &&
+                       recs[k].items[j].mb == recs[0].items[j].mb;
+            if (!same) return E->fail(SWM_E_ARG, "member %zu runs another kernel sequence than member 0", k);
+        }
+        auto align = [&]() { pl.blob.resize((pl.blob.size() + 255) & ~(size_t)255, 0); };
+        for (size_t j = 0; j < nk; ++j) {
+            const RecItem &it0 = recs[0].items[j];
+            pl.kind.push_back(it0.kind);
+            pl.ma.push_back(it0.ma);
+            pl.mb.push_back(it0.mb);
+            align();
+            pl.args_off.push_back(pl.blob.size());
+            for (size_t k = 0; k < nm; ++k) pl.blob.insert(pl.blob.end(), recs[k].items[j].args.begin(), recs[k].items[j].args.end());
+            // tile table: waves per member, and (periodic stages) blocks of the q column kernel
+            std::vector<int> start(1, 0), qstart(1, 0);
+            for (size_t k = 0; k < nm; ++k) {
+                int tiles, ny;
+                if (it0.kind == 2) {
+                    const DissArgs<F> *A = (const DissArgs<F> *)recs[k].items[j].args.data();
+                    tiles = A->nstrips * A->nseg;
+                    ny = A->ny;
+                } else {
+                    const StageArgs<F> *A = (const StageArgs<F> *)recs[k].items[j].args.data();
+                    tiles = A->nstrips * A->nseg;
+                    ny = A->ny;
+                }
+                start.push_back(start.back() + tiles);
+                qstart.push_back(qstart.back() + (ny + 1 + 255) / 256);
+            }
+            align();
+            pl.tab_off.push_back(pl.blob.size());
+            pl.blob.insert(pl.blob.end(), (const char *)start.data(), (const char *)(start.data() + start.size()));
+            align();
+            pl.qtab_off.push_back(pl.blob.size());
+            pl.blob.insert(pl.blob.end(), (const char *)qstart.data(), (const char *)(qstart.data() + qstart.size()));
+            pl.tiles.push_back(start.back());
+            pl.qblocks.push_back(qstart.back());
+        }
+        return SWM_OK;
+    }
+
+    // make buffer b hold plan pl (upload only when it differs)
+    static int load(swm_ens *E, int b, const swm_ens::Plan &pl) {
+        swm_ens::Buf &B = E->buf[b];
+        if (B.valid && B.plan.blob == pl.blob) return SWM_OK;
+        if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
+        if (pl.blob.size() > B.cap) {
+            if (B.dev) (void)hipFree(B.dev);
+            B.cap = std::max<size_t>(pl.blob.size() * 2, 1 << 16);
+            if (hipMalloc(&B.dev, B.cap) != hipSuccess) return E->fail(SWM_E_HIP, "ensemble plan buffer");
+        }
+        if (hipMemcpy(B.dev, pl.blob.data(), pl.blob.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return E->fail(SWM_E_HIP, "ensemble plan upload");
+        B.plan = pl;
+        B.valid = true;
+        return SWM_OK;
+    }
+
+    // the batched launches of buffer b's step on the ensemble stream
+    template <bool PER> static int launch(swm_ens *E, int b) {
+        const swm_ens::Buf &B = E->buf[b];
+        const swm_ens::Plan &pl = B.plan;
+        const int nm = (int)E->m.size();
+        const swm_config &c0 = E->m[0]->c;
+        char *dev = (char *)B.dev;
+        const dim3 block(WAVE * WAVES_PER_BLOCK);
+        for (size_t j = 0; j < pl.kind.size(); ++j) {
+            const int *tab = (const int *)(dev + pl.tab_off[j]), *qtab = (const int *)(dev + pl.qtab_off[j]);
+            const dim3 grid(nblocks_waves(pl.tiles[j]));
+            if (pl.kind[j] == 0) {
+                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
+                if (PER) {
+                    if (pl.ma[j] == M_RK_FIRST) hipLaunchKernelGGL((qend_batch<F, Q>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
+                    else hipLaunchKernelGGL((qend_batch<F, ST>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
+                }
+                switch (pl.ma[j]) {
+                case M_RK_FIRST: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_FIRST>), grid, block, 0, E->stream, a, tab, nm); break;
+                case M_RK_MID: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_MID>), grid, block, 0, E->stream, a, tab, nm); break;
+                case M_RK_LAST: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm); break;
+                default: return E->fail(SWM_E_UNSUPPORTED, "stage mode %d in an ensemble", pl.ma[j]);
+                }
+            } else if (pl.kind[j] == 1) {
+                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
+                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID)
+                    hipLaunchKernelGGL((fused_batch<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, E->stream, a, tab, nm);
+                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST)
+                    hipLaunchKernelGGL((fused_batch<F, M_RK_MID, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm);
+                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST)
+                    hipLaunchKernelGGL((fused_batch<F, M_RK_FIRST, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm);
+                else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
+            } else if (pl.kind[j] == 2) {
+                const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
+                const bool smag = c0.diffusion == SWM_DIFF_SMAGORINSKY;
+#define SWM_DB(D_, S_) hipLaunchKernelGGL((dissipation_batch<F, PER, D_, S_>), grid, block, 0, E->stream, a, tab, nm)
+                if (c0.bottom_drag == SWM_DRAG_QUADRATIC) { if (smag) SWM_DB(0, true); else SWM_DB(0, false); }
+                else if (c0.bottom_drag == SWM_DRAG_LINEAR) { if (smag) SWM_DB(1, true); else SWM_DB(1, false); }
+                else { if (smag) SWM_DB(2, true); else SWM_DB(2, false); }
+#undef SWM_DB
+            } else return E->fail(SWM_E_UNSUPPORTED, "kernel kind %d in an ensemble", pl.kind[j]);
+        }
+        if (hipGetLastError() != hipSuccess) return E->fail(SWM_E_HIP, "batched launch");
+        return SWM_OK;
+    }
+    static int launch_buf(swm_ens *E, int b) { return E->per ? launch<true>(E, b) : launch<false>(E, b); }
+
+    // host work a member has at step i: tracer snapshot or advection, NaN check
+    static bool events(const swm_ens *E, int64_t i) {
+        for (const swm_ctx *ctx : E->m) {
+            const swm_config &c = ctx->c;
+            if (c.tracer_advection && ((i + c.nadvstep_half) % c.nadvstep == 0 || i % c.nadvstep == 0)) return true;
+            if (c.nout > 0 && i % c.nout == 0) return true;
+        }
+        return false;
+    }
+
+    static int capture(swm_ens *E) {
+        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
+        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
+        if (hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return E->fail(SWM_E_HIP, "graph capture");
+        int rc = launch_buf(E, 0);
+        if (!rc) rc = launch_buf(E, 1);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(E->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return E->fail(SWM_E_HIP, "graph capture end");
+        E->graph = g;
+        if (hipGraphInstantiate(&E->exec, g, nullptr, nullptr, 0) != hipSuccess) return E->fail(SWM_E_HIP, "graph instantiate");
+        E->graph_ok = true;
+        return SWM_OK;
+    }
+
+    static void advance(swm_ens *E, const swm_ens::Plan &pl) {
+        for (size_t k = 0; k < E->m.size(); ++k) {
+            E->m[k]->pu = pl.next_pu[k];
+            E->m[k]->pe = pl.next_pe[k];
+        }
+    }
+
+    static int step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, int64_t *gsteps) {
+        const size_t nm = E->m.size();
+        *done = 0;
+        *gsteps = 0;
+        for (size_t k = 0; k < nm; ++k) nan_steps[k] = 0;
+        swm_ens::Plan p0, p1;
+        for (int64_t k = 0; k < nsteps;) {
+            const int64_t i = i0 + k;
+            int rc = plan(E, i, p0);
+            if (rc) return rc;
+            // two steps as one graph replay: both plans already in the buffers, no host work
+            if (k + 1 < nsteps && !events(E, i) && !events(E, i + 1) && E->buf[0].valid && E->buf[1].valid &&
+                p0.blob == E->buf[0].plan.blob) {
+                std::vector<int> pu(nm), pe(nm);
+                for (size_t q = 0; q < nm; ++q) { pu[q] = E->m[q]->pu; pe[q] = E->m[q]->pe; }
+                advance(E, p0);
+                rc = plan(E, i + 1, p1);
+                for (size_t q = 0; q < nm; ++q) { E->m[q]->pu = pu[q]; E->m[q]->pe = pe[q]; }
+                if (rc) return rc;
+                if (p1.blob == E->buf[1].plan.blob) {
+                    if (!E->graph_ok && (rc = capture(E))) return rc;
+                    if (hipGraphLaunch(E->exec, E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "graph launch");
+                    advance(E, p0);
+                    advance(E, p1);
+                    k += 2;
+                    *gsteps += 2;
+                    *done = k;
+                    continue;
+                }
+            }
+            // one eager step: the plan into the buffer it matches, else the one not used last
+            int b = E->buf[0].valid && E->buf[0].plan.blob == p0.blob ? 0
+                  : E->buf[1].valid && E->buf[1].plan.blob == p0.blob ? 1 : 1 - E->last;
+            if ((rc = load(E, b, p0))) return rc;
+            if (b != E->last) { /* alternate */ }
+            E->last = b;
+            if ((rc = launch_buf(E, b))) return rc;
+            // per-member host work, in the member's own order (time_integration.jl:226-237)
+            bool any_nan = false;
+            for (size_t q = 0; q < nm; ++q) {
+                swm_ctx *ctx = E->m[q];
+                if ((rc = Launch<F>::tracer(ctx, i, p0.next_uv[q]))) return E->fail(rc, "member %zu: %s", q, ctx->err.c_str());
+                if (ctx->c.nout > 0 && i % ctx->c.nout == 0) {
+                    bool nan = false;
+                    if ((rc = Launch<F>::nan_check(ctx, ctx->pu, ctx->pe, ctx->rsst, &nan)))
+                        return E->fail(rc, "member %zu: %s", q, ctx->err.c_str());
+                    if (nan) { nan_steps[q] = i; any_nan = true; }
+                }
+            }
+            for (size_t q = 0; q < nm; ++q)
+                if (!nan_steps[q]) { E->m[q]->pu = p0.next_pu[q]; E->m[q]->pe = p0.next_pe[q]; }
+            ++k;
+            *done = k;
+            if (any_nan) break;
+        }
+        if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
+        int flags[2] = {0, 0};
+        for (size_t q = 0; q < nm; ++q) {
+            swm_ctx *ctx = E->m[q];
+            if (hipMemcpy(flags, ctx->dflags, sizeof flags, hipMemcpyDeviceToHost) != hipSuccess) return E->fail(SWM_E_HIP, "flags");
+            if (flags[1]) {
+                int z = 0;
+                (void)hipMemcpy(ctx->dflags + 1, &z, sizeof z, hipMemcpyHostToDevice);
+                return E->fail(SWM_E_ARG, "member %zu: tracer departure point is NaN or outside the grid", q);
+            }
+        }
+        return SWM_OK;
+    }
+
+    // time `reps` batched steps (plan of the current state, into buffer 0) with HIP events
+    static int time_step(swm_ens *E, int reps, double *ms) {
+        swm_ens::Plan pl;
+        int rc = plan(E, 1, pl);
+        if (rc) return rc;
+        E->buf[0].valid = false;
+        if ((rc = load(E, 0, pl))) return rc;
+        if ((rc = launch_buf(E, 0))) return rc;                 // warm-up
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return E->fail(SWM_E_HIP, "event");
+        (void)hipEventRecord(e0, E->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = launch_buf(E, 0);
+        (void)hipEventRecord(e1, E->stream);
+        (void)hipEventSynchronize(e1);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = t / reps;
+        return rc;
+    }
+
+    // Autotuner: for each kernel class (stage, fused pair, dissipation) try segment lengths
+    // and keep the one with the fastest batched step; the others held at their choice.
+    static int tune(swm_ens *E, int reps, int *Ls, int *Lf, int *Ld) {
+        static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48};
+        for (int cls = 0; cls < 3; ++cls) {
+            double best = 1e300;
+            int bestL = 0;
+            for (int L : cand) {
+                for (swm_ctx *ctx : E->m) (cls == 0 ? ctx->L : cls == 1 ? ctx->Lf : ctx->Ld) = (cls == 0 ? std::max(L, 4) : std::max(L, 8));
+                double ms = 0;
+                int rc = time_step(E, reps, &ms);
+                if (rc) return rc;
+                if (ms < best) { best = ms; bestL = cls == 0 ? std::max(L, 4) : std::max(L, 8); }
+            }
+            for (swm_ctx *ctx : E->m) (cls == 0 ? ctx->L : cls == 1 ? ctx->Lf : ctx->Ld) = bestL;
+        }
+        *Ls = E->m[0]->L;
+        *Lf = E->m[0]->Lf;
+        *Ld = E->m[0]->Ld;
+        E->buf[0].valid = E->buf[1].valid = false;     // plans (and the graph's launch sizes) changed
+        E->graph_ok = false;
+        return SWM_OK;
+    }
+};
+
+template <class Fn> int ens_with_fmt(const swm_ens *E, Fn fn) {
+    switch (E->fmt) {
+    case SWM_F64: return fn(FmtF64{});
+    case SWM_F32: return fn(FmtF32{});
+    case SWM_F16: return fn(FmtF16{});
+    default: return fn(FmtBF16{});
+    }
+}
+
+// the members' work goes to the ensemble's stream while it runs
+struct StreamSwap {
+    swm_ens *E;
+    std::vector<hipStream_t> saved;
+    explicit StreamSwap(swm_ens *e) : E(e) {
+        for (swm_ctx *c : E->m) { saved.push_back(c->stream); c->stream = E->stream; }
+    }
+    ~StreamSwap() {
+        for (size_t k = 0; k < E->m.size(); ++k) E->m[k]->stream = saved[k];
+    }
+};
 
 }  // namespace
 
@@ -1054,6 +1437,76 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     *mass = m + cm;
     *energy = e + ce;
     return SWM_OK;
+}
+
+int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
+    if (!out || n < 1 || !members) return SWM_E_ARG;
+    *out = nullptr;
+    swm_ctx *m0 = members[0];
+    if (!m0) return SWM_E_ARG;
+    const swm_config &c0 = m0->c;
+    for (int k = 0; k < n; ++k) {
+        swm_ctx *ctx = members[k];
+        if (!ctx) return SWM_E_ARG;
+        const swm_config &c = ctx->c;
+        const char *why = nullptr;
+        if (!ctx->have_static || !ctx->have_prog) why = "static fields and a state must be uploaded";
+        else if (ctx->dev != m0->dev) why = "all members must live on one device";
+        else if (c.T != c0.T || c.Tprog != c0.Tprog || c.Tcomm != c0.Tcomm) why = "number formats differ";
+        else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
+        else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
+        else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
+        else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";
+        if (why) {
+            fprintf(stderr, "swm_ens_create: member %d: %s\n", k, why);
+            return SWM_E_UNSUPPORTED;
+        }
+        for (int j = 0; j < k; ++j)
+            if (members[j] == ctx) return SWM_E_ARG;
+    }
+    swm_ens *E = new swm_ens();
+    E->m.assign(members, members + n);
+    E->fmt = m0->fmt;
+    E->per = c0.bc == SWM_PERIODIC;
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete E;
+        return SWM_E_HIP;
+    }
+    *out = E;
+    return SWM_OK;
+}
+
+int swm_ens_destroy(swm_ens *E) {
+    if (!E) return SWM_OK;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    if (E->exec) (void)hipGraphExecDestroy(E->exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    for (auto &b : E->buf) if (b.dev) (void)hipFree(b.dev);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+    return SWM_OK;
+}
+
+const char *swm_ens_last_error(const swm_ens *E) { return E ? E->err.c_str() : "null ensemble"; }
+
+int swm_ens_tune(swm_ens *E, int32_t reps, int32_t *rows_stage, int32_t *rows_fused, int32_t *rows_diss) {
+    if (!E || reps < 1) return SWM_E_ARG;
+    for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
+    StreamSwap sw(E);
+    int Ls = 0, Lf = 0, Ld = 0;
+    int rc = ens_with_fmt(E, [&](auto f) { return Ens<decltype(f)>::tune(E, reps, &Ls, &Lf, &Ld); });
+    if (rc) return rc;
+    if (rows_stage) *rows_stage = Ls;
+    if (rows_fused) *rows_fused = Lf;
+    if (rows_diss) *rows_diss = Ld;
+    return SWM_OK;
+}
+
+int swm_ens_step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *steps_done, int64_t *nan_steps, int64_t *graph_steps) {
+    if (!E || !steps_done || !nan_steps || !graph_steps || nsteps < 0 || i0 < 1) return SWM_E_ARG;
+    for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
+    StreamSwap sw(E);
+    return ens_with_fmt(E, [&](auto f) { return Ens<decltype(f)>::step(E, nsteps, i0, steps_done, nan_steps, graph_steps); });
 }
 
 }  // extern "C"

@@ -394,24 +394,44 @@ struct WaveTile {
     bool live;
 };
 
-// strips of WAVE lanes with an x-halo of H lanes on each side: outputs in lanes H..WAVE-H-1
-template <int H = HALO>
-__device__ __forceinline__ WaveTile wave_tile(int nstrips, int nseg, int L, int ny) {
-    WaveTile t;
-    // Blocks are dispatched round-robin over the XCDs (block b to XCD b % NXCD).  The grid is
-    // a multiple of NXCD blocks; renumbering gives each XCD a contiguous run of tiles, so the
-    // halo rows two segments share sit in the same L2.
+// The launch's wave index.  Blocks are dispatched round-robin over the XCDs (block b to
+// XCD b % NXCD).  The grid is a multiple of NXCD blocks; renumbering gives each XCD a
+// contiguous run of tiles, so the halo rows two segments share sit in the same L2.  The
+// index is uniform over the wave: readfirstlane makes it (and the rows) scalar.
+__device__ __forceinline__ int wave_index() {
     const int per = (int)gridDim.x / NXCD;
     const int b = ((int)blockIdx.x % NXCD) * per + (int)blockIdx.x / NXCD;
-    // the wave index is uniform over the wave: readfirstlane makes it (and the rows) scalar
-    const int w = b * WAVES_PER_BLOCK + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    return b * WAVES_PER_BLOCK + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+}
+
+// Tile w of a field: strips of WAVE lanes with an x-halo of H lanes on each side (outputs in
+// lanes H..WAVE-H-1) times segments of L rows.
+template <int H = HALO>
+__device__ __forceinline__ WaveTile tile_of(int w, int nstrips, int nseg, int L, int ny) {
+    WaveTile t;
     t.lane = (int)(threadIdx.x % WAVE);
-    t.live = w < nstrips * nseg;
+    t.live = w >= 0 && w < nstrips * nseg;
     const int s = w % nstrips, g = w / nstrips;
     t.x = -H + (WAVE - 2 * H) * s + t.lane;
     t.ya = 1 + g * L;
     t.yb = min(t.ya + L, ny + 1);
     return t;
+}
+
+// A batched launch over the members of an ensemble: start[m] is the first tile of member
+// m, start[nmem] the total.  Sets the member and the member's own tile index of tile w;
+// false past the last tile.  Uniform over the wave (scalar loads).
+__device__ __forceinline__ bool batch_locate(const int *__restrict__ start, int nmem, int w, int &m, int &wl) {
+    if (w >= start[nmem]) return false;
+    int lo = 0, hi = nmem;                 // start[lo] <= w < start[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    m = lo;
+    wl = w - start[lo];
+    return true;
 }
 
 }  // namespace swm

@@ -118,6 +118,59 @@ class Simulation:
         return m.value, e.value
 
 
+class Ensemble:
+    """Several ``Simulation`` s advanced together (``swm_ens_*``): one batched launch per
+    kernel of a step for all members, graph replays of step pairs, an autotuner for the
+    segment lengths.  The members stay usable on their own (``download`` ...)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        L = _abi.load()
+        arr = (ctypes.c_void_p * len(self.members))(*[m.ctx for m in self.members])
+        ens = ctypes.c_void_p()
+        rc = L.swm_ens_create(len(self.members), arr, ctypes.byref(ens))
+        if rc != 0:
+            raise _abi.SwmError(f"swm_ens_create failed with status {rc}")
+        self.ens = ens
+
+    def _check(self, rc, what):
+        if rc != 0:
+            m = _abi.load().swm_ens_last_error(self.ens)
+            raise _abi.SwmError(f"{what} failed with status {rc}: {m.decode() if m else ''}")
+
+    def tune(self, reps: int = 5):
+        """Autotune the rows per wave segment; returns (stage, fused, dissipation)."""
+        a, b, c = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        self._check(_abi.load().swm_ens_tune(self.ens, reps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "swm_ens_tune")
+        return a.value, b.value, c.value
+
+    def step(self, nsteps: int, i0: int = 1):
+        """Steps ``i0 .. i0+nsteps-1`` of every member; returns (steps_done, nan_steps, graph_steps)."""
+        done, graph = ctypes.c_int64(0), ctypes.c_int64(0)
+        nan = (ctypes.c_int64 * len(self.members))()
+        self._check(_abi.load().swm_ens_step(self.ens, nsteps, i0, ctypes.byref(done), nan, ctypes.byref(graph)),
+                    "swm_ens_step")
+        return done.value, list(nan), graph.value
+
+    def close(self):
+        if getattr(self, "ens", None):
+            _abi.load().swm_ens_destroy(self.ens)
+            self.ens = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def time_integration(Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup) -> PrognosticVars:
     """``src/time_integration.jl:2-250`` on the GPU."""
     G = S.grid

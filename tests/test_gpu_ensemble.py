@@ -1,0 +1,101 @@
+"""Ensembles (swm_ens_*): members of different grid sizes advanced by batched launches and
+graph replays equal their own single-context runs bit for bit."""
+import numpy as np
+import pytest
+
+from conftest import first_mismatch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(40, 1.0), (64, 2.0), (70, 0.8), (48, 1.5), (96, 1.0)]
+
+
+def members(swm, bc, T="Float32", Tprog=None, tracer=True, Uadv=0.25, output_dt=1000.0, seed=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (nx, ratio) in enumerate(SIZES):
+        P = swm.Parameter(T=T, Tprog=Tprog or T, nx=nx, L_ratio=ratio, bc=bc, Fx0=float(rng.uniform(0.08, 0.16)),
+                          H=float(rng.uniform(400, 600)), topography="seamount", wind_forcing_x="double_gyre",
+                          diffusion="Smagorinsky", bottom_drag="quadratic", tracer_advection=tracer, Uadv=Uadv,
+                          output_dt=output_dt)
+        S = swm.make_setup(P)
+        out.append((S, swm.seeded_initial_conditions(S, seed=100 + k, σ=0.6)))
+    return out
+
+
+def run_alone(swm, S, prog, n):
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        assert sim.step(n, 1, 0) == (n, 0)
+        return sim.download()
+
+
+def assert_equal(S, a, b, tag):
+    T = S.parameters.Tprog
+    import shallowwaters_amd as _  # noqa: F401  (package import for to_float64)
+    for name, x, y in zip(("u", "v", "η", "sst"), (a.u, a.v, a.η, a.sst), (b.u, b.v, b.η, b.sst)):
+        mm = first_mismatch(np.asarray(x, dtype=np.float64) if x.dtype != np.uint16 else x.astype(np.float64),
+                            np.asarray(y, dtype=np.float64) if y.dtype != np.uint16 else y.astype(np.float64))
+        assert mm is None, f"{tag} {name}: {mm}"
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("events", [False, True])
+def test_ensemble_matches_single_runs(swm, gpu, bc, events):
+    """Without tracer / NaN-check steps the ensemble runs step pairs as graph replays; with
+    them (tracer every few steps, NaN check every step) it runs them eagerly in between."""
+    n = 14
+    ms = members(swm, bc, Uadv=30.0 if events else 0.25, output_dt=0.02 if events else 1000.0)
+    if events:
+        assert any(S.grid.nadvstep <= 5 for S, _ in ms)
+    ref = [run_alone(swm, S, p, n) for S, p in ms]
+    sims = [swm.Simulation(S) for S, _ in ms]
+    try:
+        for sim, (S, p) in zip(sims, ms):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            done, nan, graph = ens.step(n, 1)
+            assert done == n and not any(nan)
+        for sim, (S, _), r in zip(sims, ms, ref):
+            assert_equal(S, r, sim.download(), f"{bc} nx={S.grid.nx}")
+        if not events:      # the first steps settle the slot roles; then step pairs replay the graph
+            assert graph >= n - 4, graph
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+@pytest.mark.parametrize("bc,T,Tprog", [("nonperiodic", "Float32", "Float32"), ("periodic", "Float64", "Float64"),
+                                        ("nonperiodic", "BFloat16", "Float32")])
+def test_tuned_ensemble_in_two_calls(swm, gpu, bc, T, Tprog):
+    """After the autotuner picks segment lengths, steps issued in two calls still equal the
+    single runs."""
+    ms = members(swm, bc, T=T, Tprog=Tprog, tracer=True, Uadv=20.0)
+    ref = [run_alone(swm, S, p, 9) for S, p in ms]
+    sims = [swm.Simulation(S) for S, _ in ms]
+    try:
+        for sim, (S, p) in zip(sims, ms):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            Ls, Lf, Ld = ens.tune(reps=2)
+            assert Ls >= 4 and Lf >= 8 and Ld >= 8
+            assert ens.step(4, 1)[0] == 4
+            assert ens.step(5, 5)[0] == 5
+        for sim, (S, _), r in zip(sims, ms, ref):
+            assert_equal(S, r, sim.download(), f"{bc} {T} nx={S.grid.nx}")
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+def test_ensemble_rejects_mixed_options(swm, gpu):
+    a = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, bc="periodic")))
+    b = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, bc="nonperiodic")))
+    try:
+        for sim in (a, b):
+            sim.upload(swm.seeded_initial_conditions(sim.S, seed=1, σ=0.1))
+        with pytest.raises(Exception):
+            swm.Ensemble([a, b])
+    finally:
+        a.close()
+        b.close()
