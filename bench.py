@@ -82,6 +82,49 @@ def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps):
     return S, elapsed, kms, mass, energy
 
 
+def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
+    """The batched ensemble: nmem members with grid sizes drawn from {128, 256, 512, 1024}^2
+    (L_ratio = 1), Fx0 ~ U(0.08, 0.16) Pa, H ~ U(400, 600) m, seeded smoothed η noise
+    (σ = 0.05 m), Float32, RK4, the other parameters at their defaults.  Times `steps`
+    steps of the tuned ensemble and, as the unbatched comparison, `seq_steps` steps of each
+    member on its own, one after the other."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 256, 512, 1024], size=nmem)
+    setups = []
+    for k, n in enumerate(sizes):
+        P = swm.Parameter(T="Float32", nx=int(n), L_ratio=1, Fx0=float(rng.uniform(0.08, 0.16)),
+                          H=float(rng.uniform(400, 600)))
+        S = swm.make_setup(P)
+        setups.append((S, swm.seeded_initial_conditions(S, seed=1000 + k, σ=0.05)))
+    cells = sum(S.grid.nx * S.grid.ny for S, _ in setups)
+    sims = [swm.Simulation(S) for S, _ in setups]
+    try:
+        for sim, (_, p) in zip(sims, setups):
+            sim.upload(p)
+        # unbatched: each member alone
+        t0 = time.perf_counter()
+        for sim in sims:
+            sim.step(seq_steps, 1, 0)
+        seq = time.perf_counter() - t0
+        for sim, (_, p) in zip(sims, setups):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            tuned = ens.tune(reps=3)
+            ens.step(10, 1)
+            t0 = time.perf_counter()
+            done, nan, graph = ens.step(steps, 11)
+            el = time.perf_counter() - t0
+        assert done == steps and not any(nan), (done, nan)
+    finally:
+        for sim in sims:
+            sim.close()
+    return {"members": nmem, "sizes": {str(int(n)): int((sizes == n).sum()) for n in (128, 256, 512, 1024)},
+            "cells_per_step": cells, "steps": steps, "value": cells * steps / el, "unit": "cell-updates/s",
+            "ms_per_step": el / steps * 1e3, "graph_steps": graph, "rows_per_wave_tuned": list(tuned),
+            "unbatched_value": cells * seq_steps / seq, "speedup_vs_unbatched": (cells * steps / el) / (cells * seq_steps / seq)}
+
+
 def precision_report(swm, nx, days_list):
     """Errors of the Float32 and the Float16 / BFloat16 right-hand-side forms against the
     Float64 run of the same seeded state after each of `days_list` model days: relative L2
@@ -192,6 +235,10 @@ def main():
     ap.add_argument("--precision-days", type=str, default="",
                     help="comma-separated model days: only run the Float32 / Float16 / BFloat16 forms against "
                          "Float64 and print their errors at those days, then exit")
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="steps timed for the 64-member batched ensemble (0 skips it)")
+    ap.add_argument("--ensemble-seq-steps", type=int, default=20,
+                    help="steps of each ensemble member run on its own, for the unbatched comparison")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (the reference kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -266,6 +313,8 @@ def main():
                                "speedup_vs_f32": (elapsed / args.steps) / (el / args.form_steps),
                                "kernel_ms": km, "bytes_per_cell_step": sum(
                                    {**b, **{k: v["min"] for k, v in fused_bytes_per_cell(4, 2).items()}}[k] for k in fstep)}
+    if args.ensemble_steps > 0 and args.dtype == "Float32":
+        out["ensemble"] = run_ensemble(swm, 64, args.ensemble_steps, args.ensemble_seq_steps)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
     print(json.dumps(out), flush=True)
