@@ -114,3 +114,66 @@ def arith(fmt_code, op, a, b):
     out = np.empty_like(a)
     lib().swmo_arith(fmt_code, op, _ptr(a), _ptr(b), _ptr(out), a.size)
     return out
+
+
+# ------------------------------------------------------------------ tangent-linear model
+
+def _statics64(S):
+    F, G = S.forcing, S.grid
+    return [np.asfortranarray(a, dtype=np.float64) if a is not None else None
+            for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
+
+
+def _out_shapes(S, op):
+    G = S.grid
+    su, sv, se, ss = (G.nux + 4, G.nuy + 4), (G.nvx + 4, G.nvy + 4), (G.nx + 2, G.ny + 2), (G.nx + 2, G.ny)
+    return {0: [su, sv, se, ss], 1: [su, sv, se], 2: [su, sv], 3: [su, sv, ss]}[op]
+
+
+def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0):
+    """Tangent-linear model of the oracle (Float64 values with one directional derivative).
+
+    ``op`` 0: ``nsteps`` steps of the loop from x = (u, v, η, sst) after the upload's ghost
+    copies; 1: rhs + continuity of a classic RK stage at x = (u, v, η) -> (du, dv, dη);
+    2: drag + diffusion + ghost_points_uv! of x = (u0, v0, η0) -> (u0, v0);
+    3: tracer! at step i0 of x = (u, v, um, vm, sst) -> (um, vm, sst).
+    Returns (y, dy) as lists of float64 Julia-layout arrays."""
+    x = [np.asfortranarray(a, dtype=np.float64) for a in x]
+    dx = None if dx is None else [np.asfortranarray(a, dtype=np.float64) for a in dx]
+    shapes = _out_shapes(S, op)
+    y = [np.zeros(s, order="F") for s in shapes]
+    dy = [np.zeros(s, order="F") for s in shapes]
+    vp = ctypes.c_void_p
+    L = lib()
+    L.swmo_tlm.argtypes = [vp] * 7 + [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [vp] * 4
+    arr = lambda lst: (vp * len(lst))(*[a.ctypes.data if a is not None else None for a in lst])
+    xs, ys, dys = arr(x), arr(y), arr(dy)
+    dxs = arr(dx) if dx is not None else None
+    st = _statics64(S)
+    rc = L.swmo_tlm(ctypes.byref(cfg), *[_ptr(a) for a in st], op, nsteps, i0, t0, xs, dxs, ys, dys)
+    if rc != 0:
+        raise OracleError("tangent-linear run failed")
+    return y, dy
+
+
+def tape(S, cfg, i, t, u, v, e, sst, um, vm):
+    """Intermediate states of one Float64 step (classic RK) from the given state: returns a
+    dict with 's' (list of the RK stage inputs (u1, v1, η1)), 'pre' (u0, v0, η0 after the
+    stages' ghost copies), 'post' (u0, v0 after the dissipation), 'mid' (um, vm after the
+    step) and 'new' (u, v, η, sst)."""
+    G = S.grid
+    su, sv, se, ss = (G.nux + 4, G.nuy + 4), (G.nvx + 4, G.nvy + 4), (G.nx + 2, G.ny + 2), (G.nx + 2, G.ny)
+    RKo = S.parameters.RKo
+    shapes = [su, sv, se] * RKo + [su, sv, se] + [su, sv] + [su, sv] + [su, sv, se, ss]
+    out = [np.zeros(s, order="F") for s in shapes]
+    vp = ctypes.c_void_p
+    L = lib()
+    L.swmo_tape.argtypes = [vp] * 7 + [ctypes.c_int64, ctypes.c_int64] + [vp] * 7
+    ins = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, e, sst, um, vm)]
+    ptrs = (vp * len(out))(*[a.ctypes.data for a in out])
+    st = _statics64(S)
+    if L.swmo_tape(ctypes.byref(cfg), *[_ptr(a) for a in st], i, t, *[_ptr(a) for a in ins], ptrs) != 0:
+        raise OracleError("tape failed")
+    k = 3 * RKo
+    return {"s": [tuple(out[3 * j:3 * j + 3]) for j in range(RKo)], "pre": tuple(out[k:k + 3]),
+            "post": tuple(out[k + 3:k + 5]), "mid": tuple(out[k + 5:k + 7]), "new": tuple(out[k + 7:k + 11])}

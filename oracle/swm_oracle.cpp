@@ -97,9 +97,40 @@ struct Low {
 typedef Low<0> f16;
 typedef Low<1> bf16;
 
+// A float64 value carrying one directional derivative (forward-mode differentiation).  The
+// model instantiated on Dual returns its outputs together with the tangent-linear map applied
+// to the input tangents, operation by operation in the model's own order.  Kinks follow fixed
+// conventions: sqrt'(0) = 0, floor and the integer indices derived from it are constants, a
+// clipped departure coordinate is a constant.
+struct Dual {
+    double v, d;
+    Dual() : v(0.0), d(0.0) {}
+    explicit Dual(double x) : v(x), d(0.0) {}
+    explicit Dual(float x) : v(x), d(0.0) {}
+    explicit Dual(int x) : v(x), d(0.0) {}
+    Dual(double x, double dx) : v(x), d(dx) {}
+    explicit operator double() const { return v; }
+    friend Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+    friend Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+    friend Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+    friend Dual operator/(Dual a, Dual b) {
+        const double q = a.v / b.v;
+        return Dual(q, (a.d - q * b.d) / b.v);
+    }
+    friend Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+    Dual &operator+=(Dual b) { v += b.v; d += b.d; return *this; }
+    friend bool operator<(Dual a, Dual b) { return a.v < b.v; }
+    friend bool operator>(Dual a, Dual b) { return a.v > b.v; }
+    friend bool operator==(Dual a, Dual b) { return a.v == b.v; }
+};
+
 template <class T> static inline T num_sqrt(T x) { return std::sqrt(x); }
 template <> inline f16 num_sqrt<f16>(f16 x) { return f16::raw(f16::rnd(std::sqrt(x.v))); }
 template <> inline bf16 num_sqrt<bf16>(bf16 x) { return bf16::raw(bf16::rnd(std::sqrt(x.v))); }
+template <> inline Dual num_sqrt<Dual>(Dual x) {
+    const double s = std::sqrt(x.v);
+    return Dual(s, s > 0.0 ? x.d / (2.0 * s) : 0.0);
+}
 template <class T> static inline double to_f64(T x) { return (double)x; }
 template <class T> static inline bool is_nan(T x) { return std::isnan((double)x); }
 
@@ -270,6 +301,13 @@ template <class T, class Tp, class Tc> struct Model {
     Arr<T> DT, DS, nuSmag, DS_q, nuSmag_q, S12, S21, S11, S22, LLu1, LLu2, LLv1, LLv2;
     // SemiLagrangeVars{T} preallocate.jl:388-436
     Arr<T> xd, yd, um, vm, u_T, um_T, v_T, vm_T, uinterp, vinterp, ssti;
+    // intermediate states of one step (set by swmo_tape): the RK stage inputs, u0 before and
+    // after the dissipation, and the tracer's mid-point velocities after the step
+    struct Tape {
+        std::vector<Arr<Tp>> s;
+        Arr<Tp> au, av, ae, du, dv;
+        Arr<T> um, vm;
+    } *tape = nullptr;
 
     explicit Model(const swm_config &cfg) : c(cfg) {
         nx = c.nx;
@@ -794,7 +832,8 @@ template <class T, class Tp, class Tc> struct Model {
     // there with the Tprog array η: Julia promotes η[i]+H[i] to the wider format and
     // rounds the sum to T on the store.
     void init() {
-        using W = std::conditional_t<std::is_same_v<T, double> || std::is_same_v<Tp, double>, double, float>;
+        using W = std::conditional_t<std::is_same_v<T, Dual>, Dual,
+                                     std::conditional_t<std::is_same_v<T, double> || std::is_same_v<Tp, double>, double, float>>;
         for (size_t k = 0; k < h.size(); ++k) h.a[k] = cvt<T>(cvt<W>(eta.a[k]) + cvt<W>(H.a[k]));
         Ix(h_u, h);
         Iy(h_v, h);
@@ -822,6 +861,7 @@ template <class T, class Tp, class Tc> struct Model {
         if (c.time_scheme == SWM_RK) {                         // :53-78
             for (int rki = 1; rki <= c.RKo; ++rki) {
                 if (rki > 1) ghost_points(u1, v1, eta1);
+                if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
                 const Arr<T> &a = rhs_view(u1, ur), &b = rhs_view(v1, vr), &e = rhs_view(eta1, etar);
                 rhs(a, b, e, t);
                 continuity(a, b, e, t);
@@ -910,6 +950,7 @@ template <class T, class Tp, class Tc> struct Model {
 
         if (stop_stage > 0) return false;                      // all stages run: state before :198
         ghost_points(u0, v0, eta0);                            // :198
+        if (tape) { tape->au = u0; tape->av = v0; tape->ae = eta0; }
         const Arr<T> &a0 = rhs_view(u0, ur), &b0 = rhs_view(v0, vr), &e0 = rhs_view(eta0, etar);
         if (c.nstep_advcor > 0 && i % c.nstep_advcor == 0) {   // :208-211
             UVfluxes(a0, b0, e0);
@@ -922,7 +963,9 @@ template <class T, class Tp, class Tc> struct Model {
             ghost_points_uv(u0, v0);
         }
         // :226-228 tracer with the post-dissipation velocities
+        if (tape) { tape->du = u0; tape->dv = v0; }
         tracer(i, rhs_view(u0, ur), rhs_view(v0, vr));
+        if (tape) { tape->um = um; tape->vm = vm; }
         if (c.nout > 0 && i % c.nout == 0 && has_nan()) return true;  // feedback.jl:136-148
         copyto(u, u0);                                         // :240-242
         copyto(v, v0);
@@ -1043,6 +1086,138 @@ int swmo_tracer(const swm_config *c, int64_t i, const double *u, const double *v
         std::memcpy(um, M.um.a.data(), sizeof(double) * M.um.size());
         std::memcpy(vm, M.vm.a.data(), sizeof(double) * M.vm.size());
         std::memcpy(sst, M.sst.a.data(), sizeof(double) * M.sst.size());
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ tangent-linear model
+// The model on Dual (T = Tprog = Tcomm = Float64 plus one directional derivative).  Arrays
+// are float64 in Julia layout; x / dx are input values / tangents, y / dy receive output
+// values / tangents (dx, dy entries may be NULL for zero / unwanted tangents).
+//   op 0  nsteps steps of the time loop (time_integration.jl:45-243) from (u, v, η, sst)
+//         after the ghost copies swm_upload_prog applies; x, y = (u, v, η, sst)
+//   op 1  rhs! and continuity! of a classic RK stage (rhs.jl:2-51, continuity.jl:72-98) at
+//         the stage input as given (no ghost copies); x = (u, v, η), y = (du, dv, dη) with the
+//         shapes of u, v, η
+//   op 2  bottom_drag!, diffusion!, add_drag_diff_tendencies! and ghost_points_uv!
+//         (time_integration.jl:216-221) of (u0, v0, η0) as given; x = (u0, v0, η0), y = (u0, v0)
+//   op 3  tracer! at step i0 (tracer_advection.jl:1-44); x = (u, v, um, vm, sst), y = (um, vm, sst)
+static void load_dual(Arr<Dual> &A, const double *v, const double *d) {
+    for (size_t k = 0; k < A.size(); ++k) A.a[k] = Dual(v[k], d ? d[k] : 0.0);
+}
+static void store_dual(const Arr<Dual> &A, double *v, double *d) {
+    for (size_t k = 0; k < A.size(); ++k) {
+        if (v) v[k] = A.a[k].v;
+        if (d) d[k] = A.a[k].d;
+    }
+}
+
+int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
+             const double *eta_ref, const double *Feta, int32_t op, int64_t nsteps, int64_t i0, int64_t t0,
+             const double *const *x, const double *const *dx, double *const *y, double *const *dy) {
+    auto dxk = [&](int k) { return dx ? dx[k] : nullptr; };
+    auto dyk = [&](int k) { return dy ? dy[k] : nullptr; };
+    try {
+        Model<Dual, Dual, Dual> M(*c);
+        load_dual(M.H, H, nullptr);
+        load_dual(M.Fx, Fx, nullptr);
+        load_dual(M.Fy, Fy, nullptr);
+        load_dual(M.f_q, f_q, nullptr);
+        if (eta_ref) load_dual(M.eta_ref, eta_ref, nullptr);
+        if (Feta) load_dual(M.Feta, Feta, nullptr);
+        if (op == 0) {
+            load_dual(M.u, x[0], dxk(0));
+            load_dual(M.v, x[1], dxk(1));
+            load_dual(M.eta, x[2], dxk(2));
+            load_dual(M.sst, x[3], dxk(3));
+            M.ghost_points(M.u, M.v, M.eta);
+            M.ghost_sst_any(M.sst);
+            M.init();
+            int64_t t = t0;
+            for (int64_t k = 0; k < nsteps; ++k, t += c->dtint) M.step(i0 + k, t, 0);
+            store_dual(M.u, y[0], dyk(0));
+            store_dual(M.v, y[1], dyk(1));
+            store_dual(M.eta, y[2], dyk(2));
+            store_dual(M.sst, y[3], dyk(3));
+        } else if (op == 1) {
+            Arr<Dual> uu(M.u.m, M.u.n), vv(M.v.m, M.v.n), ee(M.eta.m, M.eta.n);
+            load_dual(uu, x[0], dxk(0));
+            load_dual(vv, x[1], dxk(1));
+            load_dual(ee, x[2], dxk(2));
+            M.rhs(uu, vv, ee, t0);
+            M.continuity(uu, vv, ee, t0);
+            store_dual(M.du, y[0], dyk(0));
+            store_dual(M.dv, y[1], dyk(1));
+            store_dual(M.deta, y[2], dyk(2));
+        } else if (op == 2) {
+            load_dual(M.u0, x[0], dxk(0));
+            load_dual(M.v0, x[1], dxk(1));
+            load_dual(M.eta0, x[2], dxk(2));
+            M.bottom_drag(M.u0, M.v0, M.eta0);
+            M.diffusion(M.u0, M.v0);
+            M.add_drag_diff_tendencies(M.u0, M.v0);
+            M.ghost_points_uv(M.u0, M.v0);
+            store_dual(M.u0, y[0], dyk(0));
+            store_dual(M.v0, y[1], dyk(1));
+        } else if (op == 3) {
+            Arr<Dual> uu(M.u.m, M.u.n), vv(M.v.m, M.v.n);
+            load_dual(uu, x[0], dxk(0));
+            load_dual(vv, x[1], dxk(1));
+            load_dual(M.um, x[2], dxk(2));
+            load_dual(M.vm, x[3], dxk(3));
+            load_dual(M.sst, x[4], dxk(4));
+            M.tracer(i0, uu, vv);
+            store_dual(M.um, y[0], dyk(0));
+            store_dual(M.vm, y[1], dyk(1));
+            store_dual(M.sst, y[2], dyk(2));
+        } else {
+            return -1;
+        }
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return 0;
+}
+
+// One step (Float64) at step i, time t from the state (u, v, η, sst, um, vm) as given (Prog
+// with its ghost copies; um, vm the tracer's mid-point velocities), returning its
+// intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo),
+// then u0, v0, η0 after ghost_points! (:198), then u0, v0 after the dissipation, then um, vm
+// after the step, then the new u, v, η, sst.  Classic RK only.
+int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
+              const double *eta_ref, const double *Feta, int64_t i, int64_t t, const double *u, const double *v,
+              const double *eta, const double *sst, const double *um, const double *vm, double *const *out) {
+    if (c->time_scheme != SWM_RK) return -1;
+    try {
+        Model<double, double, double> M(*c);
+        typename Model<double, double, double>::Tape tp;
+        std::memcpy(M.H.a.data(), H, sizeof(double) * M.H.size());
+        std::memcpy(M.Fx.a.data(), Fx, sizeof(double) * M.Fx.size());
+        std::memcpy(M.Fy.a.data(), Fy, sizeof(double) * M.Fy.size());
+        std::memcpy(M.f_q.a.data(), f_q, sizeof(double) * M.f_q.size());
+        if (eta_ref) std::memcpy(M.eta_ref.a.data(), eta_ref, sizeof(double) * M.eta_ref.size());
+        if (Feta) std::memcpy(M.Feta.a.data(), Feta, sizeof(double) * M.Feta.size());
+        std::memcpy(M.u.a.data(), u, sizeof(double) * M.u.size());
+        std::memcpy(M.v.a.data(), v, sizeof(double) * M.v.size());
+        std::memcpy(M.eta.a.data(), eta, sizeof(double) * M.eta.size());
+        std::memcpy(M.sst.a.data(), sst, sizeof(double) * M.sst.size());
+        std::memcpy(M.um.a.data(), um, sizeof(double) * M.um.size());
+        std::memcpy(M.vm.a.data(), vm, sizeof(double) * M.vm.size());
+        M.init();
+        M.tape = &tp;
+        M.step(i, t, 0);
+        int k = 0;
+        auto put = [&](const Arr<double> &A) {
+            if (out[k]) std::memcpy(out[k], A.a.data(), sizeof(double) * A.size());
+            ++k;
+        };
+        for (auto &A : tp.s) put(A);
+        put(tp.au); put(tp.av); put(tp.ae);
+        put(tp.du); put(tp.dv);
+        put(tp.um); put(tp.vm);
+        put(M.u); put(M.v); put(M.eta); put(M.sst);
     } catch (ShapeError &) {
         return -1;
     }
