@@ -169,6 +169,35 @@ int swm_ens_tune(swm_ens *ens, int32_t reps, int32_t *rows_stage, int32_t *rows_
 int swm_ens_step(swm_ens *ens, int64_t nsteps, int64_t i0, int64_t *steps_done, int64_t *nan_steps,
                  int64_t *graph_steps);
 
+/* ---- Reverse mode (no counterpart in the reference, which has no derivatives; it is what a
+ * ChainRulesCore.rrule of time_integration, src/time_integration.jl:2-250, would call).
+ * Classic RK scheme with T = Tprog = Tcomm in {Float64, Float32}.  Derivatives are those of
+ * the exact-arithmetic linearisation of each operation; at the kinks: sqrt'(0) = 0
+ * (Smagorinsky ν, drag √KE), floor / clip / wrap of the tracer's departure points held
+ * constant (tracer_advection.jl:211-239).
+ *
+ * swm_adj_forward: run steps i0 .. i0+nsteps-1 from the current state as swm_step does
+ * (the NaN check stops it with SWM_E_STATE) and keep device snapshots of full states
+ * (u, v, η, sst and the tracer's mid-point velocities) where a binomial checkpointing
+ * schedule with `snapshots` snapshots places them (revolve: the steps are recomputed at most
+ * r times for the least r with nsteps <= C(snapshots + r, snapshots); 0 snapshots recompute
+ * from the start for every step).  Leaves the final state, as swm_step would. */
+int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots);
+/* swm_adj_pullback: the vector-Jacobian product of the recorded steps.  ybar_* are the
+ * cotangents of the final Prog (Julia layout and format Tprog, halo included, as
+ * swm_download_prog returns it); xbar_* (nullable) receive the gradient of <ybar, y> with
+ * respect to the Prog given to the forward run, halo cells zero (upload applies the ghost
+ * copies, so the gradient lives on the interior).  The tracer's mid-point velocities at i0
+ * are constants (never read when i0 = 1).  *forward_steps (nullable) receives the forward
+ * steps recomputed, taped steps included.  The context is left at the final state. */
+int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
+                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps);
+/* Measurement hook: average duration in ms of `reps` launches of: 0 a forward step, 1 a taped
+ * forward step (unfused stages, intermediates kept), 2 the reverse of one step (all its
+ * adjoint kernels), 3 one stage adjoint kernel, 4 the dissipation adjoint kernel.  The state
+ * is restored afterwards. */
+int swm_adj_time(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1047,6 +1047,32 @@ static int dispatch_prog(const swm_config *c, const OracleStatic *s, void *u, vo
     return -1;
 }
 
+template <class T>
+static int tape_typed(const swm_config *c, const double *const *st, int64_t i, int64_t t, const double *const *x,
+                      double *const *out) {
+    Model<T, T, T> M(*c);
+    typename Model<T, T, T>::Tape tp;
+    auto in = [](Arr<T> &A, const double *p) {
+        if (p) for (size_t k = 0; k < A.size(); ++k) A.a[k] = (T)p[k];
+    };
+    in(M.H, st[0]); in(M.Fx, st[1]); in(M.Fy, st[2]); in(M.f_q, st[3]); in(M.eta_ref, st[4]); in(M.Feta, st[5]);
+    in(M.u, x[0]); in(M.v, x[1]); in(M.eta, x[2]); in(M.sst, x[3]); in(M.um, x[4]); in(M.vm, x[5]);
+    M.init();
+    M.tape = &tp;
+    M.step(i, t, 0);
+    int k = 0;
+    auto put = [&](const Arr<T> &A) {
+        if (out[k]) for (size_t q = 0; q < A.size(); ++q) out[k][q] = (double)A.a[q];
+        ++k;
+    };
+    for (auto &A : tp.s) put(A);
+    put(tp.au); put(tp.av); put(tp.ae);
+    put(tp.du); put(tp.dv);
+    put(tp.um); put(tp.vm);
+    put(M.u); put(M.v); put(M.eta); put(M.sst);
+    return 0;
+}
+
 extern "C" {
 
 // Run nsteps steps of the time loop from Prog (u, v, eta, sst; format Tprog, Julia layout,
@@ -1181,47 +1207,25 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
     return 0;
 }
 
-// One step (Float64) at step i, time t from the state (u, v, η, sst, um, vm) as given (Prog
-// with its ghost copies; um, vm the tracer's mid-point velocities), returning its
+// One step at step i, time t from the state (u, v, η, sst, um, vm) as given (Prog with its
+// ghost copies; um, vm the tracer's mid-point velocities), T = Tprog = Tcomm = Float64 or
+// Float32 (c->T; arrays are float64 holding values of that format), returning its
 // intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo),
 // then u0, v0, η0 after ghost_points! (:198), then u0, v0 after the dissipation, then um, vm
 // after the step, then the new u, v, η, sst.  Classic RK only.
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
               const double *eta_ref, const double *Feta, int64_t i, int64_t t, const double *u, const double *v,
               const double *eta, const double *sst, const double *um, const double *vm, double *const *out) {
-    if (c->time_scheme != SWM_RK) return -1;
+    if (c->time_scheme != SWM_RK || c->T != c->Tprog || c->Tcomm != c->Tprog) return -1;
+    const double *st[6] = {H, Fx, Fy, f_q, eta_ref, Feta};
+    const double *x[6] = {u, v, eta, sst, um, vm};
     try {
-        Model<double, double, double> M(*c);
-        typename Model<double, double, double>::Tape tp;
-        std::memcpy(M.H.a.data(), H, sizeof(double) * M.H.size());
-        std::memcpy(M.Fx.a.data(), Fx, sizeof(double) * M.Fx.size());
-        std::memcpy(M.Fy.a.data(), Fy, sizeof(double) * M.Fy.size());
-        std::memcpy(M.f_q.a.data(), f_q, sizeof(double) * M.f_q.size());
-        if (eta_ref) std::memcpy(M.eta_ref.a.data(), eta_ref, sizeof(double) * M.eta_ref.size());
-        if (Feta) std::memcpy(M.Feta.a.data(), Feta, sizeof(double) * M.Feta.size());
-        std::memcpy(M.u.a.data(), u, sizeof(double) * M.u.size());
-        std::memcpy(M.v.a.data(), v, sizeof(double) * M.v.size());
-        std::memcpy(M.eta.a.data(), eta, sizeof(double) * M.eta.size());
-        std::memcpy(M.sst.a.data(), sst, sizeof(double) * M.sst.size());
-        std::memcpy(M.um.a.data(), um, sizeof(double) * M.um.size());
-        std::memcpy(M.vm.a.data(), vm, sizeof(double) * M.vm.size());
-        M.init();
-        M.tape = &tp;
-        M.step(i, t, 0);
-        int k = 0;
-        auto put = [&](const Arr<double> &A) {
-            if (out[k]) std::memcpy(out[k], A.a.data(), sizeof(double) * A.size());
-            ++k;
-        };
-        for (auto &A : tp.s) put(A);
-        put(tp.au); put(tp.av); put(tp.ae);
-        put(tp.du); put(tp.dv);
-        put(tp.um); put(tp.vm);
-        put(M.u); put(M.v); put(M.eta); put(M.sst);
+        if (c->T == SWM_F64) return tape_typed<double>(c, st, i, t, x, out);
+        if (c->T == SWM_F32) return tape_typed<float>(c, st, i, t, x, out);
     } catch (ShapeError &) {
         return -1;
     }
-    return 0;
+    return -1;
 }
 
 // Single-operator entry points on float64 arrays (closed-form pins of the stencils).

@@ -15,7 +15,8 @@ from .forcing import Forcing, make_forcing
 from .model_setup import ModelSetup, make_setup, make_config, swm_config
 from .initial_conditions import (PrognosticVars, initial_conditions, add_halo, remove_halo,
                                  seeded_initial_conditions, zero_prognostic)
-from .time_integration import (time_integration, run_model, preallocate, DiagnosticVars, Simulation, Ensemble)
+from .time_integration import (time_integration, run_model, preallocate, DiagnosticVars, Simulation, Ensemble,
+                               rrule, NoTangent)
 
 __all__ = ["Parameter", "run_model", "time_integration", "preallocate", "DiagnosticVars",
            "PrognosticVars", "ModelSetup", "make_setup", "make_config", "initial_conditions",

@@ -601,7 +601,9 @@ template <class R> struct Bilin {
 template <class R, bool PER>
 SWM_HD Bilin<R> locate(R xx, R yy, int x, int y, int mx, int my, int xh1, int xh2, int yh1, int cx_of_k, int cy_of_l) {
     Bilin<R> b;
-    const R fx = floor_r(xx), fy = floor_r(yy);
+    R fx = floor_r(xx), fy = floor_r(yy);
+    if (!(fx == fx) || fx > R(1e6) || fx < R(-1e6)) fx = R(0);     // NaN / runaway: the forward has failed
+    if (!(fy == fy) || fy > R(1e6) || fy < R(-1e6)) fy = R(0);
     const long xi = (long)fx, yi = (long)fy;
     long k = xi + x + xh1, l = yi + y + yh1;
     b.x0 = xx - (R)xi;
@@ -610,6 +612,9 @@ SWM_HD Bilin<R> locate(R xx, R yy, int x, int y, int mx, int my, int xh1, int xh
     if (PER) {
         if (k < 1 + xh1) k = k + mx - xh1 - xh2;
         else if (k > mx - 1) k = k - mx + xh1 + xh2;
+        // a departure point farther than one period (the forward reports it as an error)
+        // must still address the array
+        if (k < 1 + xh1 || k > mx - 1) k = k < 1 + xh1 ? 1 + xh1 : mx - 1;
     } else {
         if (k < 1 + xh1) { k = 1 + xh1; b.x0 = R(0); b.clipx = true; }
         else if (k > mx - 1) { k = mx - 1; b.x0 = R(1); b.clipx = true; }

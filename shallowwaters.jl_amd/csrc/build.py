@@ -13,7 +13,7 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libswm_hip.so")
 SOURCES = [os.path.join(HERE, "swm.hip")]
-DEPS = SOURCES + [os.path.join(HERE, h) for h in ("swm_common.h", "k_stage.h", "k_dissipation.h", "k_misc.h")] + [
+DEPS = SOURCES + [os.path.join(HERE, h) for h in ("swm_common.h", "k_stage.h", "k_dissipation.h", "k_misc.h", "k_adjoint.h", "adj_core.h")] + [
     os.path.join(ROOT, "include", "swm.h")]
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

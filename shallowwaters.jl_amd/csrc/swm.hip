@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/swm.h"
+#include "k_adjoint.h"
 #include "k_dissipation.h"
 #include "k_misc.h"
 #include "k_stage.h"
@@ -121,7 +122,7 @@ struct swm_ctx {
     int L = 64;            // rows per wave segment of the stage kernels
     int Ld = 64;           // rows per wave segment of the dissipation kernel
     int Lf = 64;           // rows per wave segment of the fused stage pairs
-    Field u[4], v[4], e[4], sst[2];                          // Tprog
+    Field u[6], v[6], e[6], sst[2];                          // Tprog (slots 4, 5: the adjoint's taped step)
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0;
@@ -131,6 +132,9 @@ struct swm_ctx {
     int *dflags = nullptr; // [0] NaN flag, [1] tracer index error
     double *dpart = nullptr;
     bool have_static = false, have_prog = false;
+    int nslots = 4;                // Tprog slot sets in use (6 once the adjoint is set up)
+    int ens_refs = 0;              // ensembles holding this context (their plans point into the arena)
+    struct AdjState *adj = nullptr;
     std::vector<void *> allocs;
     void *arena = nullptr;         // every field of the context; kernels address it through one buffer descriptor
     size_t arena_bytes = 0, arena_used = 0;
@@ -346,8 +350,8 @@ template <class F> struct Launch {
         const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN;
         const bool out1 = m != M_RK_LAST;
         const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont);
-        auto ok = [](Slots s, bool uv, bool e, bool tpool) {
-            const int n = tpool ? 2 : 4;
+        auto ok = [ctx](Slots s, bool uv, bool e, bool tpool) {
+            const int n = tpool ? 2 : ctx->nslots;
             if (s.t != tpool) return false;
             return (!uv || (s.uv >= 0 && s.uv < n)) && (!e || (s.e >= 0 && s.e < n));
         };
@@ -791,6 +795,516 @@ template <class F> int run_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64
         return ctx->fail(SWM_E_ARG, "tracer departure point is NaN or outside the grid (InexactError/BoundsError in the reference)");
     }
     return SWM_OK;
+}
+
+}  // namespace
+
+// ====================================================================== adjoint
+//
+// Reverse mode of time_integration (classic RK, T = Tprog = Tcomm in {Float64, Float32}).
+// swm_adj_forward runs the steps as swm_step does and keeps device snapshots of full states
+// (u, v, η, sst and the tracer's um, vm) at the positions a binomial checkpointing schedule
+// chooses for the given number of snapshots (Griewank's revolve: with s snapshots, n steps
+// are reversed with the least repetition number r, n <= C(s+r, s)).  swm_adj_pullback
+// reverses the steps last first: it restores the nearest snapshot, advances to the step,
+// runs that step once more keeping its intermediate states (the taped step: every RK stage
+// input, u0 before and after the dissipation, the tracer state it read), and applies the
+// reverse step of adj_core.h.  The result is the gradient of <ȳ, y> with respect to the
+// state given to swm_adj_forward (interior cells; the halo of the gradient is zero).
+
+struct AdjState {
+    swm::adj::AdjFields F{};       // adjoint fields (origins); roles rotate during a pullback
+    swm::adj::AdjFields F0{};
+    int La = 64;                   // rows per wave segment of the adjoint marches
+    std::vector<char *> snap;      // snapshots: [0] the initial state, [1..s] the schedule's, [s+1] the final state
+    std::vector<int64_t> at;       // step held by each schedule snapshot (-1: free)
+    size_t snap_bytes = 0;
+    int *dmax = nullptr;           // the tracer adjoint's gather window (device, 2 ints)
+    int64_t i0 = 0, t0 = 0, n = 0;
+    int nsnap = 0;
+    bool recorded = false;
+    int64_t fwd_steps = 0;         // forward steps run by the last pullback (taped steps included)
+    void release() {
+        for (char *p : snap) if (p) (void)hipFree(p);
+        snap.clear();
+        at.clear();
+        if (dmax) (void)hipFree(dmax);
+        dmax = nullptr;
+    }
+};
+
+namespace {
+
+// the fields of a context, for re-basing after the arena moves
+std::vector<Field *> all_fields(swm_ctx *ctx) {
+    std::vector<Field *> f;
+    for (int s = 0; s < 6; ++s) { f.push_back(&ctx->u[s]); f.push_back(&ctx->v[s]); f.push_back(&ctx->e[s]); }
+    for (int s = 0; s < 2; ++s) {
+        f.push_back(&ctx->sst[s]); f.push_back(&ctx->tu[s]); f.push_back(&ctx->tv[s]); f.push_back(&ctx->te[s]);
+    }
+    for (Field *g : {&ctx->um, &ctx->vm, &ctx->H, &ctx->Fx, &ctx->Fy, &ctx->fq, &ctx->eta_ref, &ctx->Feta, &ctx->qend})
+        f.push_back(g);
+    return f;
+}
+
+// Enlarge the arena by `extra` bytes, keeping every field at its byte offset.
+int grow_arena(swm_ctx *ctx, size_t extra) {
+    const size_t nb = ctx->arena_bytes + extra;
+    if (nb > 0xFFFF0000ull)
+        return ctx->fail(SWM_E_UNSUPPORTED, "grid %dx%d with the adjoint needs %zu bytes of device fields; one context "
+                         "addresses at most 4 GiB", ctx->c.nx, ctx->c.ny, nb);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    void *p = nullptr;
+    if (hipMalloc(&p, nb) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc of %zu bytes failed", nb);
+    HIPCHK(hipMemsetAsync(p, 0, nb, ctx->stream));
+    HIPCHK(hipMemcpyAsync(p, ctx->arena, ctx->arena_used, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    char *old = (char *)ctx->arena;
+    for (Field *f : all_fields(ctx))
+        if (f->base) {
+            f->p0 = (char *)p + (f->p0 - old);
+            f->base = (char *)p + f->origin;
+        }
+    for (auto &a : ctx->allocs)
+        if (a == ctx->arena) a = p;
+    (void)hipFree(ctx->arena);
+    ctx->arena = p;
+    ctx->arena_bytes = nb;
+    return SWM_OK;
+}
+
+int64_t binom_beta(int s, int t) {               // C(s+t, s), saturating
+    double b = 1.0;
+    for (int k = 1; k <= s; ++k) b = b * (double)(t + k) / (double)k;
+    return b > 4e18 ? (int64_t)4e18 : (int64_t)(b + 0.5);
+}
+// where the schedule stores the next snapshot when reversing steps [a, b) with s free
+// snapshots: the right part is reversible with s-1 snapshots and the left part with s at
+// one repetition less (β(s,r) = β(s-1,r) + β(s,r-1))
+int64_t revolve_split(int64_t a, int64_t b, int s) {
+    const int64_t n = b - a;
+    int r = 0;
+    while (binom_beta(s, r) < n) ++r;
+    const int64_t left = std::max<int64_t>(1, n - binom_beta(s - 1, r));
+    return a + left;
+}
+
+template <class Fm> struct Adj {
+    using R = typename Fm::R;
+    using Q = typename Fm::Q;
+    using LF = Launch<Fm>;
+
+    static int check(swm_ctx *ctx) {
+        const swm_config &c = ctx->c;
+        if (c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK scheme only");
+        if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
+        if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+        if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
+        return SWM_OK;
+    }
+
+    // carve the adjoint's fields: two more slot sets and the adjoint fields (21), zeroed
+    static int init(swm_ctx *ctx) {
+        if (ctx->adj) return SWM_OK;
+        int rc = check(ctx);
+        if (rc) return rc;
+        const int nf2 = 6 + 21;
+        const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
+        if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
+        AdjState *st = new AdjState();
+        const int es = ctx->es;
+        for (int s = 4; s < 6; ++s)
+            if ((rc = alloc_field(ctx, ctx->u[s], 0, true, es)) || (rc = alloc_field(ctx, ctx->v[s], 0, true, es)) ||
+                (rc = alloc_field(ctx, ctx->e[s], 0, true, es))) {
+                delete st;
+                return rc;
+            }
+        auto one = [&](uint32_t &o, bool two_d) {
+            Field f;
+            if (!rc) rc = alloc_field(ctx, f, two_d ? 0 : n1d, two_d, es);
+            o = f.origin;
+        };
+        auto uve = [&](adj::UVE &x) { one(x.u, true); one(x.v, true); one(x.e, true); };
+        adj::AdjFields &F = st->F;
+        uve(F.L); uve(F.Z); uve(F.Ma); uve(F.Mb); uve(F.A);
+        one(F.Lsst, true); one(F.Lsst2, true); one(F.Lum, true); one(F.Lvm, true);
+        one(F.tr0, true); one(F.tr1, true);
+        one(F.lqend, false);
+        if (rc) {
+            delete st;
+            return rc;
+        }
+        F.H = ctx->H.ref();
+        F.fq = ctx->fq.ref();
+        F.fq_2d = ctx->fq_sx;
+        F.qend = ctx->qend.ref();
+        st->F0 = F;
+        // rows per wave: about four resident waves per wave slot
+        st->La = LF::rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_adj_kernel<R, false, true, false, false>),
+                                   adj::nstrips_for<adj::RT_HALO>(ctx->c.nx + 2), 4, 8);
+        if (const char *e = getenv("SWM_ROWS_PER_WAVE_ADJ")) {
+            const int l = atoi(e);
+            if (l >= 4 && l <= 4096) st->La = l;
+        }
+        ctx->adj = st;
+        ctx->nslots = 6;
+        return SWM_OK;
+    }
+
+    static adj::AdjConsts<R> consts(const swm_ctx *ctx) {
+        const swm_config &c = ctx->c;
+        adj::AdjConsts<R> k{};
+        for (int j = 0; j < 4; ++j) {
+            k.a[j] = (R)c.RKaDt[j];
+            k.b[j] = j < 3 ? (R)c.RKbDt[j] : R(0);
+        }
+        k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
+        k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
+        k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
+        k.RKo = c.RKo; k.cont = c.continuity; k.drag = c.bottom_drag; k.smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
+        k.per = c.bc == SWM_PERIODIC; k.consume = c.tracer_consumption; k.tracer = c.tracer_advection;
+        k.nadvstep = (int)c.nadvstep; k.nadvstep_half = (int)c.nadvstep_half; k.nstep_diff = c.nstep_diff;
+        k.nx = c.nx; k.ny = c.ny; k.P = ctx->P; k.ep = ctx->ep;
+        return k;
+    }
+
+    // ---- kernel launcher of adj::reverse_step
+    struct K {
+        swm_ctx *ctx;
+        AdjState *st;
+        int fail_launch() {
+            hipError_t e = hipGetLastError();
+            return e == hipSuccess ? SWM_OK : ctx->fail(SWM_E_HIP, "adjoint kernel launch: %s", hipGetErrorString(e));
+        }
+        template <bool PER, bool NEXT, bool AL0, bool PRE> void launch_stage(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE>), dim3(nblocks_waves(S.nstrips * S.nseg)),
+                               dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), S);
+        }
+        template <bool PER> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
+            if (!next) launch_stage<PER, false, false, false>(S);
+            else if (al0 && pre) launch_stage<PER, true, true, true>(S);
+            else if (al0) launch_stage<PER, true, true, false>(S);
+            else if (pre) launch_stage<PER, true, false, true>(S);
+            else launch_stage<PER, true, false, false>(S);
+        }
+        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre) {
+            S.L = st->La;
+            S.nstrips = adj::nstrips_for<adj::RT_HALO>(S.nx + 2);
+            S.nseg = (adj::adj_y1(S.ny) - adj::ADJ_Y0 + S.L - 1) / S.L;
+            if (per) stage_p<true>(S, next, al0, pre);
+            else stage_p<false>(S, next, al0, pre);
+            return fail_launch();
+        }
+        int qend(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((qend_fwd_kernel<R>), dim3((S.ny + 1 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S);
+            return fail_launch();
+        }
+        int qend_adj(const adj::StageAdjArgs<R> &S, const adj::UVE &mu) {
+            hipLaunchKernelGGL((qend_adj_kernel<R>), dim3((S.ny + 2 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S,
+                               mu);
+            return fail_launch();
+        }
+        template <bool PER, int DRAG, bool SMAG> void launch_diss(const adj::DissAdjArgs<R> &D) {
+            hipLaunchKernelGGL((diss_adj_kernel<R, PER, DRAG, SMAG>), dim3(nblocks_waves(D.nstrips * D.nseg)),
+                               dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), D);
+        }
+        template <bool PER> void diss_p(const adj::DissAdjArgs<R> &D, int drag, int smag) {
+            if (drag == 0) { if (smag) launch_diss<PER, 0, true>(D); else launch_diss<PER, 0, false>(D); }
+            else if (drag == 1) { if (smag) launch_diss<PER, 1, true>(D); else launch_diss<PER, 1, false>(D); }
+            else { if (smag) launch_diss<PER, 2, true>(D); else launch_diss<PER, 2, false>(D); }
+        }
+        int diss(adj::DissAdjArgs<R> D, int per, int drag, int smag) {
+            D.L = st->La;
+            D.nstrips = adj::nstrips_for<adj::DT_HALO>(D.nx + 2);
+            D.nseg = (adj::adj_y1(D.ny) - adj::ADJ_Y0 + D.L - 1) / D.L;
+            if (per) diss_p<true>(D, drag, smag);
+            else diss_p<false>(D, drag, smag);
+            return fail_launch();
+        }
+        int tracer(const adj::TracerAdjArgs<R> &T) {
+            hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
+            const dim3 g1((T.nx + 127) / 128, T.ny), g2((T.nx + 5 + 127) / 128, T.ny + 4);
+            if (T.ep) {
+                hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+            } else {
+                hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+            }
+            return fail_launch();
+        }
+        int snapshot(uint32_t lu, uint32_t lum, uint32_t lv, uint32_t lvm) {
+            char *b = (char *)ctx->arena;
+            hipLaunchKernelGGL((snapshot_adj_kernel<R>), dim3(1024), dim3(256), 0, ctx->stream, (R *)(b + lu), (R *)(b + lum),
+                               (R *)(b + lv), (R *)(b + lvm), ctx->elems);
+            return fail_launch();
+        }
+    };
+
+    // ---- snapshots of full states (u, v, η, sst, um, vm)
+    static int snap_alloc(swm_ctx *ctx, int count) {
+        AdjState *st = ctx->adj;
+        st->release();
+        st->snap_bytes = 6 * ctx->elems * ctx->es;
+        for (int k = 0; k < count; ++k) {
+            void *p = nullptr;
+            if (hipMalloc(&p, st->snap_bytes) != hipSuccess) {
+                st->release();
+                return ctx->fail(SWM_E_HIP, "hipMalloc of snapshot %d (%zu bytes) failed", k, st->snap_bytes);
+            }
+            st->snap.push_back((char *)p);
+        }
+        if (hipMalloc(&st->dmax, 2 * sizeof(int)) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc");
+        st->at.assign(count, -1);
+        return SWM_OK;
+    }
+    static int copy_state(swm_ctx *ctx, int k, bool save) {
+        AdjState *st = ctx->adj;
+        const size_t fb = ctx->elems * ctx->es;
+        const Field *f[6] = {&ctx->u[ctx->pu], &ctx->v[ctx->pu], &ctx->e[ctx->pe], &ctx->sst[ctx->rsst], &ctx->um, &ctx->vm};
+        for (int j = 0; j < 6; ++j) {
+            char *s = st->snap[k] + j * fb;
+            HIPCHK(hipMemcpyAsync(save ? s : f[j]->base, save ? f[j]->base : s, fb, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return SWM_OK;
+    }
+
+    // one forward step from Prog at step i (0-based k = i - i0)
+    static int advance(swm_ctx *ctx, int64_t k) {
+        AdjState *st = ctx->adj;
+        bool nan = false;
+        const int64_t i = st->i0 + k;
+        int rc = LF::step(ctx, i, st->t0 + k * ctx->c.dtint, &nan);
+        if (rc) return rc;
+        if (nan) return ctx->fail(SWM_E_STATE, "NaN in the state at step %lld: no gradient", (long long)i);
+        st->fwd_steps += 1;
+        return SWM_OK;
+    }
+
+    // The taped step k: step i = i0 + k from Prog without copy-back, every intermediate kept
+    // in its own slot (the stage inputs s_1..s_RKo, u0 before / after the dissipation).
+    static int tape_step(swm_ctx *ctx, int64_t k, adj::AdjFields &F) {
+        AdjState *st = ctx->adj;
+        const swm_config &c = ctx->c;
+        const int64_t i = st->i0 + k, t = st->t0 + k * c.dtint;
+        const typename LF::Times T = LF::forcing_times(ctx, t);
+        int uv[6], ee[6], nu = 0, ne = 0;
+        uv[nu++] = ctx->pu;
+        ee[ne++] = ctx->pe;
+        for (int s = 0; s < 6; ++s) {
+            if (s != ctx->pu) uv[nu++] = s;
+            if (s != ctx->pe) ee[ne++] = s;
+        }
+        // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo, [5] the dissipation output
+        const Slots P{uv[0], ee[0]}, a{uv[1], ee[1]}, none{};
+        auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
+        int rc = SWM_OK;
+        for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
+            Coef<Q> kk{};
+            kk.a = (Q)c.RKaDt[q - 1];
+            kk.b = q < c.RKo ? (Q)c.RKbDt[q - 1] : Q(0);
+            const int m = q == 1 ? M_RK_FIRST : (q == c.RKo ? M_RK_LAST : M_RK_MID);
+            rc = LF::stage(ctx, m, sk(q), P, a, q < c.RKo ? sk(q + 1) : none, a, kk, T);
+        }
+        if (rc) return rc;
+        int post = a.uv;
+        if (i % c.nstep_diff == 0) {
+            post = uv[5];
+            if ((rc = LF::dissipation(ctx, a, post))) return rc;
+        }
+        const int sst_old = ctx->rsst;
+        if ((rc = LF::tracer(ctx, i, post))) return rc;
+        for (int q = 1; q <= c.RKo; ++q) {
+            const Slots s = sk(q);
+            F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
+        }
+        F.pre = adj::UVE{ctx->u[a.uv].ref(), ctx->v[a.uv].ref(), ctx->e[a.e].ref()};
+        F.post_u = ctx->u[post].ref();
+        F.post_v = ctx->v[post].ref();
+        F.um = ctx->um.ref();
+        F.vm = ctx->vm.ref();
+        F.sst = ctx->sst[sst_old].ref();
+        st->fwd_steps += 1;
+        return SWM_OK;
+    }
+
+    static int reverse_one(swm_ctx *ctx, int64_t k) {
+        AdjState *st = ctx->adj;
+        int rc = tape_step(ctx, k, st->F);
+        if (rc) return rc;
+        K kl{ctx, st};
+        return adj::reverse_step<R>(kl, st->F, consts(ctx), (long)(st->i0 + k));
+    }
+
+    // reverse steps [a, b) with the state at a in snapshot slot sa and s free snapshots
+    static int reverse(swm_ctx *ctx, int64_t a, int64_t b, int sa, int s) {
+        AdjState *st = ctx->adj;
+        int rc;
+        auto restore_advance = [&](int64_t to) {
+            int r = copy_state(ctx, sa, false);
+            for (int64_t q = a; q < to && !r; ++q) r = advance(ctx, q);
+            return r;
+        };
+        if (b - a == 1 || s == 0) {
+            for (int64_t k = b - 1; k >= a; --k) {
+                if ((rc = restore_advance(k)) || (rc = reverse_one(ctx, k))) return rc;
+            }
+            return SWM_OK;
+        }
+        const int64_t m = revolve_split(a, b, s);
+        int sm = -1;
+        for (size_t q = 1; q + 1 < st->snap.size(); ++q)
+            if (st->at[q] == m) sm = (int)q;
+        if (sm < 0) {                                            // not stored by the forward sweep
+            for (size_t q = 1; q + 1 < st->snap.size() && sm < 0; ++q)
+                if (st->at[q] < 0) sm = (int)q;
+            if (sm < 0) return ctx->fail(SWM_E_STATE, "revolve: no free snapshot");
+            if ((rc = restore_advance(m)) || (rc = copy_state(ctx, sm, true))) return rc;
+            st->at[sm] = m;
+        }
+        if ((rc = reverse(ctx, m, b, sm, s - 1))) return rc;
+        st->at[sm] = -1;
+        return reverse(ctx, a, m, sa, s);
+    }
+
+    static int forward(swm_ctx *ctx, int64_t n, int64_t i0, int64_t t0, int s) {
+        int rc = init(ctx);
+        if (rc) return rc;
+        AdjState *st = ctx->adj;
+        if ((rc = snap_alloc(ctx, s + 2))) return rc;
+        st->i0 = i0; st->t0 = t0; st->n = n; st->nsnap = s; st->recorded = false; st->fwd_steps = 0;
+        if ((rc = copy_state(ctx, 0, true))) return rc;
+        // the snapshots the schedule takes first: the chain of splits toward the last step
+        std::vector<int64_t> chain;
+        for (int64_t a = 0, sf = s; n - a > 1 && sf > 0; --sf) {
+            a = revolve_split(a, n, (int)sf);
+            chain.push_back(a);
+        }
+        size_t next = 0;
+        for (int64_t k = 0; k < n; ++k) {
+            if (next < chain.size() && chain[next] == k) {
+                if ((rc = copy_state(ctx, (int)next + 1, true))) return rc;
+                st->at[next + 1] = k;
+                ++next;
+            }
+            if ((rc = advance(ctx, k))) return rc;
+        }
+        if ((rc = copy_state(ctx, s + 1, true))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        st->recorded = true;
+        return SWM_OK;
+    }
+
+    static int zero_adjoint(swm_ctx *ctx) {
+        AdjState *st = ctx->adj;
+        const size_t from = std::min<size_t>(st->F0.L.u, ctx->u[4].origin);
+        const size_t words = (ctx->arena_used - from) / 4;
+        hipLaunchKernelGGL(zero_words_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t *)((char *)ctx->arena + from),
+                           words);
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
+    static int pullback(swm_ctx *ctx, const void *yu, const void *yv, const void *ye, const void *ys, void *gu, void *gv,
+                        void *ge, void *gs) {
+        AdjState *st = ctx->adj;
+        const swm_config &c = ctx->c;
+        int rc;
+        st->F = st->F0;
+        st->fwd_steps = 0;
+        if ((rc = zero_adjoint(ctx))) return rc;
+        Field tmp;
+        auto put = [&](uint32_t o, const void *h, int m, int nn, int x0, int y0) {
+            tmp.base = (char *)ctx->arena + o;
+            tmp.origin = o;
+            tmp.es = ctx->es;
+            tmp.p0 = (char *)tmp.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
+            return put2d(ctx, tmp, h, m, nn, x0, y0);
+        };
+        const int mu = ctx->nux + 4, nuu = ctx->nuy + 4, mv = ctx->nvx + 4, nv = ctx->nvy + 4;
+        if ((rc = put(st->F.L.u, yu, mu, nuu, -1 - ctx->ep, -1)) || (rc = put(st->F.L.v, yv, mv, nv, -1, -1)) ||
+            (rc = put(st->F.L.e, ye, c.nx + 2, c.ny + 2, 0, 0)) || (rc = put(st->F.Lsst, ys, c.nx + 2, c.ny, 0, 1)))
+            return rc;
+        if ((rc = reverse(ctx, 0, st->n, 0, st->nsnap))) return rc;
+        // the gradient: ghost copies of the uploaded state folded in (swm_upload_prog applies them)
+        adj::FoldOutArgs O{st->F.L, st->F.Z, st->F.Lsst, st->F.Lsst2, c.nx, c.ny, c.bc == SWM_PERIODIC};
+        hipLaunchKernelGGL((fold_out_kernel<R>), dim3((c.nx + 5 + 127) / 128, c.ny + 4), dim3(128), 0, ctx->stream,
+                           ctx->aref(), O, (R)c.one_minus_alpha, ctx->P);
+        HIPCHK(hipGetLastError());
+        auto get = [&](uint32_t o, void *h, int m, int nn, int x0, int y0) {
+            tmp.base = (char *)ctx->arena + o;
+            tmp.origin = o;
+            tmp.p0 = (char *)tmp.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
+            return get2d(ctx, tmp, h, m, nn, x0, y0);
+        };
+        if ((gu && (rc = get(st->F.Z.u, gu, mu, nuu, -1 - ctx->ep, -1))) || (gv && (rc = get(st->F.Z.v, gv, mv, nv, -1, -1))) ||
+            (ge && (rc = get(st->F.Z.e, ge, c.nx + 2, c.ny + 2, 0, 0))) || (gs && (rc = get(st->F.Lsst2, gs, c.nx + 2, c.ny, 0, 1))))
+            return rc;
+        // leave the context at the final state, as after swm_adj_forward
+        if ((rc = copy_state(ctx, st->nsnap + 1, false))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return SWM_OK;
+    }
+
+    // measurement: which 0 forward step, 1 taped step, 2 reverse step (kernels of one step,
+    // tape in place), 3 stage adjoint (middle stage), 4 dissipation adjoint
+    static int time(swm_ctx *ctx, int which, int reps, double *ms) {
+        int rc = init(ctx);
+        if (rc) return rc;
+        AdjState *st = ctx->adj;
+        if (st->snap.empty() && (rc = snap_alloc(ctx, 2))) return rc;
+        st->i0 = 1; st->t0 = 0;
+        if ((rc = copy_state(ctx, 0, true))) return rc;
+        st->F = st->F0;
+        adj::AdjFields Ft = st->F0;
+        if ((rc = tape_step(ctx, 0, Ft))) return rc;
+        K kl{ctx, st};
+        const adj::AdjConsts<R> k = consts(ctx);
+        adj::StageAdjArgs<R> S{};
+        S.s = Ft.s[1]; S.H = Ft.H; S.fq = Ft.fq; S.fq_2d = Ft.fq_2d; S.qend = Ft.qend; S.lqend = Ft.lqend;
+        S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.a_in = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
+        S.a = k.a[1]; S.b = k.b[1]; S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = k.cont;
+        S.nx = k.nx; S.ny = k.ny; S.P = k.P;
+        adj::DissAdjArgs<R> D{};
+        D.s = Ft.pre; D.H = Ft.H; D.lin = Ft.L; D.out = Ft.Z;
+        D.cD = k.cD; D.rD = k.rD; D.cS = k.cS; D.nuB = k.nuB; D.dtd = k.dtd; D.oma = k.oma;
+        D.nx = k.nx; D.ny = k.ny; D.P = k.P;
+        auto launch = [&]() -> int {
+            switch (which) {
+            case 0: { bool nan = false; return LF::step(ctx, 1, 0, &nan); }
+            case 1: { adj::AdjFields G = st->F0; return tape_step(ctx, 0, G); }
+            case 2: { st->F = Ft; return adj::reverse_step<R>(kl, st->F, k, 1); }
+            case 3: return kl.stage(S, k.per, 1, 0, 0);
+            case 4: return kl.diss(D, k.per, k.drag, k.smag);
+            }
+            return ctx->fail(SWM_E_ARG, "unknown adjoint kernel %d", which);
+        };
+        if ((rc = launch())) return rc;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps && rc == SWM_OK; ++r) rc = launch();
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float tm = 0.0f;
+        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
+        HIPCHK(hipEventDestroy(e0));
+        HIPCHK(hipEventDestroy(e1));
+        *ms = (double)tm / reps;
+        // the state the measurement started from
+        if (!rc) rc = copy_state(ctx, 0, false);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        st->recorded = false;
+        return rc;
+    }
+};
+
+template <class Fn> int adj_with_fmt(swm_ctx *ctx, Fn fn) {
+    if (ctx->fmt == SWM_F64 && ctx->c.Tprog == SWM_F64) return fn(FmtF64{});
+    if (ctx->fmt == SWM_F32 && ctx->c.Tprog == SWM_F32) return fn(FmtF32{});
+    return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
 }
 
 }  // namespace
@@ -1270,6 +1784,10 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
 int swm_destroy(swm_ctx *ctx) {
     if (!ctx) return SWM_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->adj) {
+        ctx->adj->release();
+        delete ctx->adj;
+    }
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1466,6 +1984,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
     }
     swm_ens *E = new swm_ens();
     E->m.assign(members, members + n);
+    for (swm_ctx *c : E->m) c->ens_refs += 1;
     E->fmt = m0->fmt;
     E->per = c0.bc == SWM_PERIODIC;
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1483,6 +2002,7 @@ int swm_ens_destroy(swm_ens *E) {
     if (E->graph) (void)hipGraphDestroy(E->graph);
     for (auto &b : E->buf) if (b.dev) (void)hipFree(b.dev);
     if (E->stream) (void)hipStreamDestroy(E->stream);
+    for (swm_ctx *c : E->m) c->ens_refs -= 1;
     delete E;
     return SWM_OK;
 }
@@ -1507,6 +2027,32 @@ int swm_ens_step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *steps_done, in
     for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
     StreamSwap sw(E);
     return ens_with_fmt(E, [&](auto f) { return Ens<decltype(f)>::step(E, nsteps, i0, steps_done, nan_steps, graph_steps); });
+}
+
+// ---- reverse mode
+
+int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots) {
+    if (!ctx) return SWM_E_ARG;
+    if (nsteps < 1 || i0 < 1 || snapshots < 0) return ctx->fail(SWM_E_ARG, "nsteps >= 1, i0 >= 1 and snapshots >= 0 required");
+    return adj_with_fmt(ctx, [&](auto f) { return Adj<decltype(f)>::forward(ctx, nsteps, i0, t0, snapshots); });
+}
+
+int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
+                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps) {
+    if (!ctx) return SWM_E_ARG;
+    if (!ybar_u || !ybar_v || !ybar_eta || !ybar_sst) return ctx->fail(SWM_E_ARG, "null cotangent array");
+    if (!ctx->adj || !ctx->adj->recorded) return ctx->fail(SWM_E_STATE, "call swm_adj_forward first");
+    const int rc = adj_with_fmt(ctx, [&](auto f) {
+        return Adj<decltype(f)>::pullback(ctx, ybar_u, ybar_v, ybar_eta, ybar_sst, xbar_u, xbar_v, xbar_eta, xbar_sst);
+    });
+    if (forward_steps) *forward_steps = ctx->adj->fwd_steps;
+    return rc;
+}
+
+int swm_adj_time(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
+    if (!ctx || !ms_avg) return SWM_E_ARG;
+    if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");
+    return adj_with_fmt(ctx, [&](auto f) { return Adj<decltype(f)>::time(ctx, which, reps, ms_avg); });
 }
 
 }  // extern "C"

@@ -117,6 +117,34 @@ class Simulation:
         _abi.check(_abi.load().swm_reduce(self.ctx, ctypes.byref(m), ctypes.byref(e)), self.ctx, "swm_reduce")
         return m.value, e.value
 
+    # ---- reverse mode (swm_adj_*): classic RK, T = Tprog = Tcomm in {Float64, Float32}
+    def adjoint_forward(self, nsteps: int, i0: int = 1, t0: int = 0, snapshots: int = 20):
+        """Steps ``i0 .. i0+nsteps-1`` recorded for a pullback, with ``snapshots`` device
+        checkpoints placed by the binomial (revolve) schedule."""
+        _abi.check(_abi.load().swm_adj_forward(self.ctx, nsteps, i0, t0, snapshots), self.ctx, "swm_adj_forward")
+
+    def pullback(self, ybar: PrognosticVars):
+        """Gradient of <ybar, y> with respect to the state given to ``adjoint_forward`` (a
+        PrognosticVars in Tprog storage, halo cells zero); also returns the number of
+        forward steps the reverse sweep recomputed."""
+        G, dt = self.S.grid, storage_dtype(self.S.parameters.Tprog)
+        ins = [np.asfortranarray(a, dtype=dt) for a in (ybar.u, ybar.v, ybar.η, ybar.sst)]
+        z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
+        out = PrognosticVars(u=z(G.nux + 4, G.nuy + 4), v=z(G.nvx + 4, G.nvy + 4), η=z(G.nx + 2, G.ny + 2),
+                             sst=z(G.nx + 2, G.ny), T=self.S.parameters.Tprog)
+        nf = ctypes.c_int64(0)
+        _abi.check(_abi.load().swm_adj_pullback(self.ctx, *[_abi.ptr(a) for a in ins],
+                                                *[_abi.ptr(a) for a in (out.u, out.v, out.η, out.sst)], ctypes.byref(nf)),
+                   self.ctx, "swm_adj_pullback")
+        return out, nf.value
+
+    def adjoint_time(self, which: int, reps: int) -> float:
+        """Average duration (ms): 0 forward step, 1 taped step, 2 reverse step, 3 stage
+        adjoint kernel, 4 dissipation adjoint kernel."""
+        ms = ctypes.c_double(0)
+        _abi.check(_abi.load().swm_adj_time(self.ctx, which, reps, ctypes.byref(ms)), self.ctx, "swm_adj_time")
+        return ms.value
+
 
 class Ensemble:
     """Several ``Simulation`` s advanced together (``swm_ens_*``): one batched launch per
@@ -182,6 +210,46 @@ def time_integration(Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup) 
         print(f" NaNs detected at time step {nan_step}")
     u, v, η, sst = remove_halo(Prog.u, Prog.v, Prog.η, Prog.sst, S)
     return PrognosticVars(u=u, v=v, η=η, sst=sst, T=S.parameters.Tprog)
+
+
+class NoTangent:
+    """ChainRulesCore.NoTangent(): the argument has no derivative."""
+
+    def __repr__(self):
+        return "NoTangent()"
+
+
+def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshots: int = 20):
+    """``ChainRulesCore.rrule(::typeof(time_integration), Prog, Diag, S)``: runs
+    ``time_integration`` on the GPU keeping ``snapshots`` device checkpoints (revolve) and
+    returns ``(y, pullback)``; ``pullback(ȳ)`` takes the cotangent of the returned interior
+    fields (a PrognosticVars without halo) and returns ``(NoTangent(), Proḡ, NoTangent(),
+    NoTangent())`` with Proḡ the gradient with respect to ``Prog`` (halo cells zero: the
+    upload's ghost copies make the halo a function of the interior)."""
+    if f is not time_integration:
+        raise TypeError("rrule is defined for time_integration only")
+    G, T = S.grid, S.parameters.Tprog
+    sim = Simulation(S)
+    sim.upload(Prog)
+    sim.adjoint_forward(G.nt, 1, 0, snapshots)
+    Y = sim.download()
+    u, v, η, sst = remove_halo(Y.u, Y.v, Y.η, Y.sst, S)
+    y = PrognosticVars(u=u, v=v, η=η, sst=sst, T=T)
+    dt = storage_dtype(T)
+
+    def pullback(ybar: PrognosticVars):
+        z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
+        yb = PrognosticVars(u=z(G.nux + 4, G.nuy + 4), v=z(G.nvx + 4, G.nvy + 4), η=z(G.nx + 2, G.ny + 2),
+                            sst=z(G.nx + 2, G.ny), T=T)
+        yb.u[2:-2, 2:-2] = ybar.u          # remove_halo keeps the interior (initial_conditions.jl:170-181)
+        yb.v[2:-2, 2:-2] = ybar.v
+        yb.η[1:-1, 1:-1] = ybar.η
+        yb.sst[1:-1, :] = ybar.sst
+        xbar, _ = sim.pullback(yb)
+        return NoTangent(), xbar, NoTangent(), NoTangent()
+
+    pullback.simulation = sim
+    return y, pullback
 
 
 def run_model(T=None, P: Parameter | None = None, **kwargs) -> PrognosticVars:

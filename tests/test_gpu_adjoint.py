@@ -1,0 +1,147 @@
+"""The adjoint of the time step on the GPU (swm_adj_forward / swm_adj_pullback).
+
+Pins:
+  * bitwise: the GPU gradient equals the host build of the same adjoint kernels (tests/emu)
+    driven by the oracle's taped steps, which is itself exact against the transposed
+    Jacobian of the oracle's tangent-linear model (tests/test_adjoint_cpu.py);
+  * the FP64 dot-product identity <J dx, ȳ> = <dx, J^T ȳ> with J from the CPU
+    tangent-linear model, relative error <= 1e-11;
+  * central finite differences of the GPU forward run (FP64), relative error <= 1e-6;
+  * revolve: the gradient is bitwise the same for any number of snapshots, and the forward
+    steps recomputed follow the schedule (0 snapshots: n(n+1)/2).
+"""
+import numpy as np
+import pytest
+
+import adjemu
+from test_adjoint_cpu import ghosted_state, interior_masks, reverse_steps
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(swm, bc, T="Float64", nx=70, **kw):
+    d = dict(T=T, nx=nx, L_ratio=0.6, bc=bc, α=0.5, diffusion="Smagorinsky", bottom_drag="quadratic",
+             topography="seamount", wind_forcing_x="double_gyre", Uadv=20.0, tracer_consumption=True)
+    d.update(kw)
+    P = swm.Parameter(**d)
+    S = swm.make_setup(P)
+    return P, S, swm.make_config(S), swm.seeded_initial_conditions(S, seed=5, σ=0.8)
+
+
+def cotangent(S, seed, dtype):
+    rng = np.random.default_rng(seed)
+    return [np.asfortranarray(rng.standard_normal(m.shape).astype(dtype)) for m in interior_masks(S)]
+
+
+def gpu_gradient(swm, S, prog, ybar, n, snapshots):
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        sim.adjoint_forward(n, 1, 0, snapshots)
+        y = sim.download()
+        g, nf = sim.pullback(swm.PrognosticVars(u=ybar[0], v=ybar[1], η=ybar[2], sst=ybar[3], T=S.parameters.Tprog))
+        y2 = sim.download()
+    for a, b in zip((y.u, y.v, y.η, y.sst), (y2.u, y2.v, y2.η, y2.sst)):
+        assert np.array_equal(a, b), "the context must be left at the final state"
+    return [g.u, g.v, g.η, g.sst], nf
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("T", ["Float64", "Float32"])
+def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T):
+    monkeypatch.setenv("SWM_ROWS_PER_WAVE_ADJ", "8")        # many segments on a small grid
+    P, S, cfg, prog = setup(swm, bc, T=T)
+    assert (S.grid.nadvstep, S.grid.nadvstep_half) == (3, 1)
+    n = 3                                                     # snapshot at step 2, advection at step 3
+    dt = np.float64 if T == "Float64" else np.float32
+    ybar = cotangent(S, 1, dt)
+    print(f"pullback {bc} {T} nx={S.grid.nx} ny={S.grid.ny}", flush=True)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=1)
+    k = adjemu.config(S, cfg, L=8, fmt=8 if T == "Float64" else 4)
+    ref, _, _ = reverse_steps(swm, orc, S, cfg, prog, n, [a.astype(np.float64) for a in ybar], k)
+    for name, r, a in zip(("u", "v", "η", "sst"), ref, g):
+        a = np.asarray(a, dtype=np.float64)
+        bad = np.argwhere(r != a)
+        assert len(bad) == 0, f"{name}: {len(bad)} cells differ, first {bad[0].tolist()}: {r[tuple(bad[0])]!r} vs {a[tuple(bad[0])]!r}"
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+def test_dot_product_identity_fp64(swm, orc, gpu, bc):
+    P, S, cfg, prog = setup(swm, bc, nx=90)
+    n = 7
+    ybar = cotangent(S, 2, np.float64)
+    print(f"dot product {bc} nx={S.grid.nx} ny={S.grid.ny} n={n}", flush=True)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=2)
+    x = list(ghosted_state(swm, S, prog))
+    rng = np.random.default_rng(3)
+    for m in range(3):
+        dx = [np.where(mk, rng.standard_normal(mk.shape), 0.0) for mk in interior_masks(S)]
+        dxg = list(ghosted_state(swm, S, type(prog)(*dx)))
+        _, dy = orc.tlm(S, cfg, 0, x, dxg, nsteps=n)
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(dy, ybar))
+        rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, g))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dx, ȳ> = {lhs!r}, <dx, J^T ȳ> = {rhs!r}"
+
+
+def test_central_differences_of_the_gpu_forward(swm, gpu):
+    P, S, cfg, prog = setup(swm, "nonperiodic", nx=64)
+    n = 6
+    ybar = cotangent(S, 4, np.float64)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=3)
+    rng = np.random.default_rng(5)
+    dx = [np.where(mk, rng.standard_normal(mk.shape), 0.0) for mk in interior_masks(S)]
+    eps = 1e-6
+
+    def J(sign):
+        p = type(prog)(*(a + sign * eps * b for a, b in zip((prog.u, prog.v, prog.η, prog.sst), dx)))
+        with swm.Simulation(S) as sim:
+            sim.upload(p)
+            assert sim.step(n, 1, 0) == (n, 0)
+            y = sim.download()
+        return sum(float(np.sum(a * b)) for a, b in zip((y.u, y.v, y.η, y.sst), ybar))
+
+    fd = (J(1) - J(-1)) / (2 * eps)
+    ad = sum(float(np.sum(a * b)) for a, b in zip(dx, g))
+    assert abs(fd - ad) <= 1e-6 * abs(ad), f"finite differences {fd!r} vs adjoint {ad!r}"
+
+
+def test_revolve_snapshots_do_not_change_the_gradient(swm, gpu):
+    P, S, cfg, prog = setup(swm, "periodic", T="Float32", nx=66)
+    n = 7
+    ybar = cotangent(S, 6, np.float32)
+    runs = {s: gpu_gradient(swm, S, prog, ybar, n, snapshots=s) for s in (0, 1, 2, 3, 7)}
+    g0, nf0 = runs[0]
+    assert nf0 == n * (n + 1) // 2
+    counts = [runs[s][1] for s in (0, 1, 2, 3, 7)]
+    assert counts == sorted(counts, reverse=True) and counts[-1] <= 2 * n, counts
+    for s, (g, _) in runs.items():
+        for name, a, b in zip(("u", "v", "η", "sst"), g0, g):
+            assert np.array_equal(a, b), f"{s} snapshots: ∇{name} differs"
+
+
+def test_rrule_of_time_integration(swm, gpu):
+    P, S, cfg, prog = setup(swm, "nonperiodic", T="Float32", nx=64, Ndays=0.05)
+    G = S.grid
+    ref = swm.time_integration(type(prog)(*(a.copy(order="F") for a in (prog.u, prog.v, prog.η, prog.sst))),
+                               None, S)
+    y, pullback = swm.rrule(swm.time_integration, prog, None, S, snapshots=4)
+    for a, b in zip((ref.u, ref.v, ref.η, ref.sst), (y.u, y.v, y.η, y.sst)):
+        assert np.array_equal(a, b)
+    ybar = type(y)(*(np.ones_like(a) for a in (y.u, y.v, y.η, y.sst)))
+    nt, pbar, nd, ns = pullback(ybar)
+    assert isinstance(nt, swm.NoTangent) and isinstance(nd, swm.NoTangent) and isinstance(ns, swm.NoTangent)
+    for a, m in zip((pbar.u, pbar.v, pbar.η, pbar.sst), interior_masks(S)):
+        assert a.shape == m.shape and np.all(a[~m] == 0) and np.all(np.isfinite(a))
+    assert G.nt >= 2
+
+
+def test_adjoint_rejects_unsupported_configurations(swm, gpu):
+    P, S, cfg, prog = setup(swm, "nonperiodic", T="BFloat16", Tprog="Float32")
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        with pytest.raises(swm.SwmError if hasattr(swm, "SwmError") else RuntimeError, match="adjoint"):
+            sim.adjoint_forward(2)
+    P, S, cfg, prog = setup(swm, "nonperiodic", T="Float32")
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        with pytest.raises(RuntimeError, match="swm_adj_forward first"):
+            sim.pullback(prog)
