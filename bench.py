@@ -125,6 +125,55 @@ def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
             "unbatched_value": cells * seq_steps / seq, "speedup_vs_unbatched": (cells * steps / el) / (cells * seq_steps / seq)}
 
 
+def adjoint_bytes_per_cell(es: int) -> dict:
+    """Algorithmic HBM bytes per cell of the reverse step's kernels (each field read or
+    written once): the last RK stage's adjoint reads the stage input, H and λ0 and writes
+    μ; the middle ones also read μ of the later stage and the running sum A and write A;
+    the first writes the step input's adjoint; the dissipation adjoint reads the state
+    before the dissipation, H and λ after it, and writes λ before it."""
+    return {"stage_adj_last": 10 * es, "stage_adj_next": 16 * es, "stage_adj_mid": 19 * es, "stage_adj_first": 16 * es,
+            "diss_adj": 10 * es}
+
+
+def run_adjoint(swm, T, nx, nsteps, snapshots, reps):
+    """The reverse mode on the workload: the step's kernels timed with HIP events, then a
+    forward run of `nsteps` steps recorded with `snapshots` revolve checkpoints and its
+    pullback (the cotangent of the sum of the final state), device time of the sweep."""
+    import numpy as np
+    S = swm.make_setup(workload_parameter(swm, T=T, nx=nx))
+    G = S.grid
+    prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    es = 4 if T == "Float32" else 8
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        names = ("forward_step", "taped_step", "reverse_step", "stage_adjoint_mid", "dissipation_adjoint")
+        t = {n: sim.adjoint_time(w, reps) for w, n in enumerate(names)}
+        t0 = time.perf_counter()
+        sim.adjoint_forward(nsteps, 1, 0, snapshots)
+        fwd = time.perf_counter() - t0
+        y = sim.download()
+        ybar = type(y)(*(np.ones_like(a) for a in (y.u, y.v, y.η, y.sst)))
+        t0 = time.perf_counter()
+        g, nf = sim.pullback(ybar)
+        wall = time.perf_counter() - t0
+        sweep = sim.sweep_ms
+    finite = all(bool(np.isfinite(a).all()) for a in (g.u, g.v, g.η, g.sst))
+    cells = G.nx * G.ny
+    b = adjoint_bytes_per_cell(es)
+    step_bytes = sum(b.values()) * cells
+    return {"kernel_ms": t, "reverse_over_forward": t["reverse_step"] / t["forward_step"],
+            "taped_plus_reverse_over_forward": (t["taped_step"] + t["reverse_step"]) / t["forward_step"],
+            "reverse_step_alg_bytes_per_cell": sum(b.values()),
+            "reverse_step_GBs": step_bytes / (t["reverse_step"] * 1e-3) / 1e9,
+            "stage_adjoint_mid_GBs": b["stage_adj_mid"] * cells / (t["stage_adjoint_mid"] * 1e-3) / 1e9,
+            "dissipation_adjoint_GBs": b["diss_adj"] * cells / (t["dissipation_adjoint"] * 1e-3) / 1e9,
+            "revolve": {"steps": nsteps, "snapshots": snapshots, "forward_steps_recomputed": nf,
+                        "forward_ms_per_step": fwd / nsteps * 1e3, "reverse_sweep_ms_per_step": sweep / nsteps,
+                        "gradient_ms_per_step": fwd / nsteps * 1e3 + sweep / nsteps,
+                        "gradient_over_forward": (fwd / nsteps * 1e3 + sweep / nsteps) / t["forward_step"],
+                        "pullback_wall_s_incl_host_transfers": wall, "gradient_finite": finite}}
+
+
 def precision_report(swm, nx, days_list):
     """Errors of the Float32 and the Float16 / BFloat16 right-hand-side forms against the
     Float64 run of the same seeded state after each of `days_list` model days: relative L2
@@ -239,6 +288,9 @@ def main():
                     help="steps timed for the 64-member batched ensemble (0 skips it)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=20,
                     help="steps of each ensemble member run on its own, for the unbatched comparison")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the revolve pullback measurement (0 skips the reverse-mode line)")
+    ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (the reference kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -315,6 +367,9 @@ def main():
                                    {**b, **{k: v["min"] for k, v in fused_bytes_per_cell(4, 2).items()}}[k] for k in fstep)}
     if args.ensemble_steps > 0 and args.dtype == "Float32":
         out["ensemble"] = run_ensemble(swm, 64, args.ensemble_steps, args.ensemble_seq_steps)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = run_adjoint(swm, args.dtype, args.nx, args.adjoint_steps, args.adjoint_snapshots,
+                                     max(5, args.kernel_reps // 2))
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
     print(json.dumps(out), flush=True)

@@ -189,9 +189,11 @@ int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_
  * respect to the Prog given to the forward run, halo cells zero (upload applies the ghost
  * copies, so the gradient lives on the interior).  The tracer's mid-point velocities at i0
  * are constants (never read when i0 = 1).  *forward_steps (nullable) receives the forward
- * steps recomputed, taped steps included.  The context is left at the final state. */
+ * steps recomputed, taped steps included; *sweep_ms (nullable) the device time of the
+ * reverse sweep (host transfers excluded).  The context is left at the final state. */
 int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
-                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps);
+                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps,
+                     double *sweep_ms);
 /* Measurement hook: average duration in ms of `reps` launches of: 0 a forward step, 1 a taped
  * forward step (unfused stages, intermediates kept), 2 the reverse of one step (all its
  * adjoint kernels), 3 one stage adjoint kernel, 4 the dissipation adjoint kernel.  The state

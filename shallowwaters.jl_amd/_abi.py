@@ -55,7 +55,7 @@ def load():
     L.swm_ens_tune.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.swm_ens_step.argtypes = [vp, i64, i64, pi64, pi64, pi64]
     L.swm_adj_forward.argtypes = [vp, i64, i64, i64, i32]
-    L.swm_adj_pullback.argtypes = [vp] * 9 + [pi64]
+    L.swm_adj_pullback.argtypes = [vp] * 9 + [pi64, ctypes.POINTER(ctypes.c_double)]
     L.swm_adj_time.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTS:
         if name not in ("swm_last_error", "swm_ens_last_error"):

@@ -824,6 +824,7 @@ struct AdjState {
     int nsnap = 0;
     bool recorded = false;
     int64_t fwd_steps = 0;         // forward steps run by the last pullback (taped steps included)
+    double sweep_ms = 0.0;         // device time of the last reverse sweep (HIP events)
     void release() {
         for (char *p : snap) if (p) (void)hipFree(p);
         snap.clear();
@@ -1226,7 +1227,23 @@ template <class Fm> struct Adj {
         if ((rc = put(st->F.L.u, yu, mu, nuu, -1 - ctx->ep, -1)) || (rc = put(st->F.L.v, yv, mv, nv, -1, -1)) ||
             (rc = put(st->F.L.e, ye, c.nx + 2, c.ny + 2, 0, 0)) || (rc = put(st->F.Lsst, ys, c.nx + 2, c.ny, 0, 1)))
             return rc;
-        if ((rc = reverse(ctx, 0, st->n, 0, st->nsnap))) return rc;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        rc = reverse(ctx, 0, st->n, 0, st->nsnap);
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        HIPCHK(hipEventSynchronize(e1));
+        float tm = 0.0f;
+        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
+        st->sweep_ms = tm;
+        HIPCHK(hipEventDestroy(e0));
+        HIPCHK(hipEventDestroy(e1));
         // the gradient: ghost copies of the uploaded state folded in (swm_upload_prog applies them)
         adj::FoldOutArgs O{st->F.L, st->F.Z, st->F.Lsst, st->F.Lsst2, c.nx, c.ny, c.bc == SWM_PERIODIC};
         hipLaunchKernelGGL((fold_out_kernel<R>), dim3((c.nx + 5 + 127) / 128, c.ny + 4), dim3(128), 0, ctx->stream,
@@ -2038,7 +2055,7 @@ int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_
 }
 
 int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
-                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps) {
+                     void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps, double *sweep_ms) {
     if (!ctx) return SWM_E_ARG;
     if (!ybar_u || !ybar_v || !ybar_eta || !ybar_sst) return ctx->fail(SWM_E_ARG, "null cotangent array");
     if (!ctx->adj || !ctx->adj->recorded) return ctx->fail(SWM_E_STATE, "call swm_adj_forward first");
@@ -2046,6 +2063,7 @@ int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const
         return Adj<decltype(f)>::pullback(ctx, ybar_u, ybar_v, ybar_eta, ybar_sst, xbar_u, xbar_v, xbar_eta, xbar_sst);
     });
     if (forward_steps) *forward_steps = ctx->adj->fwd_steps;
+    if (sweep_ms) *sweep_ms = ctx->adj->sweep_ms;
     return rc;
 }
 

@@ -126,16 +126,19 @@ class Simulation:
     def pullback(self, ybar: PrognosticVars):
         """Gradient of <ybar, y> with respect to the state given to ``adjoint_forward`` (a
         PrognosticVars in Tprog storage, halo cells zero); also returns the number of
-        forward steps the reverse sweep recomputed."""
+        forward steps the reverse sweep recomputed.  ``self.sweep_ms`` receives the device
+        time of the reverse sweep."""
         G, dt = self.S.grid, storage_dtype(self.S.parameters.Tprog)
         ins = [np.asfortranarray(a, dtype=dt) for a in (ybar.u, ybar.v, ybar.η, ybar.sst)]
         z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
         out = PrognosticVars(u=z(G.nux + 4, G.nuy + 4), v=z(G.nvx + 4, G.nvy + 4), η=z(G.nx + 2, G.ny + 2),
                              sst=z(G.nx + 2, G.ny), T=self.S.parameters.Tprog)
-        nf = ctypes.c_int64(0)
+        nf, ms = ctypes.c_int64(0), ctypes.c_double(0)
         _abi.check(_abi.load().swm_adj_pullback(self.ctx, *[_abi.ptr(a) for a in ins],
-                                                *[_abi.ptr(a) for a in (out.u, out.v, out.η, out.sst)], ctypes.byref(nf)),
+                                                *[_abi.ptr(a) for a in (out.u, out.v, out.η, out.sst)], ctypes.byref(nf),
+                                                ctypes.byref(ms)),
                    self.ctx, "swm_adj_pullback")
+        self.sweep_ms = ms.value
         return out, nf.value
 
     def adjoint_time(self, which: int, reps: int) -> float:
