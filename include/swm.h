@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWM_ABI_VERSION 1
+#define SWM_ABI_VERSION 2
 
 /* status codes */
 #define SWM_OK 0
@@ -67,7 +67,9 @@ typedef struct swm_config {
     int32_t adv_scheme, dynamics, bottom_drag, diffusion, continuity;
     int32_t tracer_advection, tracer_consumption;
     int32_t seasonal_wind_x, seasonal_wind_y;
-    int32_t pad0;
+    /* u, v are carried inside the library as 2^scale_log2 · u, v (0: unscaled); the arrays
+     * at this ABI stay unscaled.  A low-precision option that is not in the reference. */
+    int32_t scale_log2;
     int64_t dtint, nadvstep, nadvstep_half, nout;   /* grid.jl:70,77-78,86 */
     /* (Tprog) constants.jl:69-75,81,86 and grid.jl:74 */
     double RKaDt[4], RKbDt[3];
@@ -81,6 +83,10 @@ typedef struct swm_config {
     double jSST, SSTmin;
     /* (F64) constants.jl:108-110 */
     double omegaFx, omegaFy, omegaFeta;
+    /* classic RK: sum the stage increments apart and add them to the state with Kahan's
+     * compensated summation (a persistent compensation per variable; not in the reference) */
+    int32_t compensated;
+    int32_t pad1;
 } swm_config;
 
 typedef struct swm_ctx swm_ctx;

@@ -309,7 +309,20 @@ template <class T, class Tp, class Tc> struct Model {
         Arr<T> um, vm;
     } *tape = nullptr;
 
+    // Low-precision options (not in the reference; defined by this library): u, v carried as
+    // s·u, s·v with s = 2^scale_log2, the factor 1/s absorbed where the equations are not
+    // linear in u, v (PV weights, Bernoulli potential, divergence, drag speed, Smagorinsky
+    // coefficient, tracer departure); with compensated, the RK increments are summed apart
+    // and added to the state with Kahan's compensated summation (a persistent compensation
+    // per variable).
+    double sc = 1.0, sci = 1.0;
+    bool comp = false;
+    Arr<Tp> cu, cv, ce;                 // Kahan compensations of u, v, η
+
     explicit Model(const swm_config &cfg) : c(cfg) {
+        sc = std::ldexp(1.0, c.scale_log2);
+        sci = std::ldexp(1.0, -c.scale_log2);
+        comp = c.compensated != 0;
         nx = c.nx;
         ny = c.ny;
         periodic = c.bc == SWM_PERIODIC;
@@ -324,9 +337,9 @@ template <class T, class Tp, class Tc> struct Model {
         f_q = Arr<T>(nx + 1, ny + 1);
         eta_ref = Arr<T>(nx, ny);
         Feta = Arr<T>(nx, ny);
-        u = u0 = u1 = du = Arr<Tp>(nux + 4, nuy + 4);
-        v = v0 = v1 = dv = Arr<Tp>(nvx + 4, nvy + 4);
-        eta = eta0 = eta1 = deta = Arr<Tp>(nx + 2, ny + 2);
+        u = u0 = u1 = du = cu = Arr<Tp>(nux + 4, nuy + 4);
+        v = v0 = v1 = dv = cv = Arr<Tp>(nvx + 4, nvy + 4);
+        eta = eta0 = eta1 = deta = ce = Arr<Tp>(nx + 2, ny + 2);
         sst = Arr<Tp>(nx + 2, ny);
         ur = u2 = um = Arr<T>(nux + 4, nuy + 4);
         vr = v2 = vm = Arr<T>(nvx + 4, nvy + 4);
@@ -482,26 +495,26 @@ template <class T, class Tp, class Tc> struct Model {
                 q(i, j) = (f_q(i, j) + dvdx(i + 1, j + 1) - dudy(i + 1 + ep, j + 1)) / h_q(i, j);
     }
     void AHa() {                                               // PV_advection.jl:88-99
-        const T k = T(1.0 / 12.0);
+        const T k = T(1.0 / 12.0) * T(sci);
         for (int j = 1; j <= qa.n; ++j)
             for (int i = 1; i <= qa.m; ++i) qa(i, j) = k * (q(i, j) + q(i, j + 1) + q(i + 1, j + 1));
     }
     void AHb() {                                               // PV_advection.jl:103-115
-        const T k = T(1.0 / 12.0);
+        const T k = T(1.0 / 12.0) * T(sci);
         for (int j = 1; j <= qb.n; ++j) {
             qb(1, j) = k * (q(1, j) + q(1, j + 1) + q(q.m, j + 1));
             for (int i = 2; i <= qb.m; ++i) qb(i, j) = k * (q(i, j) + q(i, j + 1) + q(i - 1, j + 1));
         }
     }
     void AHg() {                                               // PV_advection.jl:119-131
-        const T k = T(1.0 / 12.0);
+        const T k = T(1.0 / 12.0) * T(sci);
         for (int j = 1; j <= qg.n; ++j) {
             qg(1, j) = k * (q(1, j) + q(1, j + 1) + q(q.m, j));
             for (int i = 2; i <= qg.m; ++i) qg(i, j) = k * (q(i, j) + q(i - 1, j) + q(i, j + 1));
         }
     }
     void AHd() {                                               // PV_advection.jl:135-146
-        const T k = T(1.0 / 12.0);
+        const T k = T(1.0 / 12.0) * T(sci);
         for (int j = 1; j <= qd.n; ++j)
             for (int i = 1; i <= qd.m; ++i) qd(i, j) = k * (q(i, j) + q(i, j + 1) + q(i + 1, j));
     }
@@ -531,7 +544,7 @@ template <class T, class Tp, class Tc> struct Model {
     }
     void bernoulli(const Arr<T> &e) {                          // rhs.jl:162-180
         CHECK(p.m + ep == KEu.m && p.n + 2 == KEu.n && p.m + 2 == KEv.m && p.n == KEv.n);
-        const T half = T(0.5), g = T(c.g);
+        const T half = T(0.5) * T(sci), g = T(c.g) * T(sc);
         for (int j = 1; j <= p.n; ++j)
             for (int i = 1; i <= p.m; ++i) p(i, j) = half * (KEu(i + ep, j + 1) + KEv(i + 1, j)) + g * e(i, j);
     }
@@ -575,16 +588,16 @@ template <class T, class Tp, class Tc> struct Model {
             const T gam = T(c.gamma);
             for (int j = 1; j <= n; ++j)
                 for (int i = 1; i <= m; ++i)
-                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) +
+                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) * Tp(sci) +
                                          cvt<Tp>(gam * (eta_ref(i, j) - e(i + 1, j + 1)));
         } else if (c.continuity == SWM_CONT_FORCING) {         // continuity.jl:27-49
             const T Fet = Ftime(t, c.omegaFeta);
             for (int j = 1; j <= n; ++j)
                 for (int i = 1; i <= m; ++i)
-                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) + cvt<Tp>(Fet * Feta(i, j));
+                    deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) * Tp(sci) + cvt<Tp>(Fet * Feta(i, j));
         } else {                                               // continuity.jl:52-68
             for (int j = 1; j <= n; ++j)
-                for (int i = 1; i <= m; ++i) deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j)));
+                for (int i = 1; i <= m; ++i) deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) * Tp(sci);
         }
     }
 
@@ -600,7 +613,7 @@ template <class T, class Tp, class Tc> struct Model {
             Iy(KEv, v2);
             CHECK(sqrtKE.m + ep == KEu.m && sqrtKE.n + 2 == KEu.n);
             for (int j = 1; j <= sqrtKE.n; ++j)
-                for (int i = 1; i <= sqrtKE.m; ++i) sqrtKE(i, j) = num_sqrt(KEu(i + ep, j + 1) + KEv(i + 1, j));
+                for (int i = 1; i <= sqrtKE.m; ++i) sqrtKE(i, j) = num_sqrt(KEu(i + ep, j + 1) + KEv(i + 1, j)) * T(sci);
             Ix(sqrtKE_u, sqrtKE);
             Iy(sqrtKE_v, sqrtKE);
             for (int j = 1; j <= Bu.n; ++j)
@@ -644,7 +657,7 @@ template <class T, class Tp, class Tc> struct Model {
                 DT(i, j) = s * s;
             }
         for (int j = 1; j <= nuSmag.n; ++j)
-            for (int i = 1; i <= nuSmag.m; ++i) nuSmag(i, j) = cS * num_sqrt(DS(i, j) + DT(i, j));
+            for (int i = 1; i <= nuSmag.m; ++i) nuSmag(i, j) = cS * (num_sqrt(DS(i, j) + DT(i, j)) * T(sci));
         Ixy(nuSmag_q, nuSmag);
     }
     void viscous_tensor_smagorinsky() {                        // diffusion.jl:126-171
@@ -756,13 +769,17 @@ template <class T, class Tp, class Tc> struct Model {
     }
     void departure(const Arr<T> &uu, const Arr<T> &vv) {       // tracer_advection.jl:52-85
         Ix(u_T, uu);
+        for (auto &x : u_T.a) x = x * T(sci);
         Ix(um_T, um);
         Iy(v_T, vv);
+        for (auto &x : v_T.a) x = x * T(sci);
         Iy(vm_T, vm);
         backtraj(xd, T(c.half_dtadvu), u_T);
         backtraj(yd, T(c.half_dtadvv), v_T);
         interp_uv(uinterp, um_T, xd, yd);
         interp_uv(vinterp, vm_T, xd, yd);
+        for (auto &x : uinterp.a) x = x * T(sci);
+        for (auto &x : vinterp.a) x = x * T(sci);
         backtraj(xd, T(c.dtadvu), uinterp);
         backtraj(yd, T(c.dtadvv), vinterp);
     }
@@ -820,6 +837,16 @@ template <class T, class Tp, class Tc> struct Model {
         for (size_t k = 0; k < a.size(); ++k) d.a[k] = xT * a.a[k] + yT * b.a[k] + zT * cc.a[k];
     }
 
+    // d = x + d with Kahan's compensated summation: y = d - c, t = x + y, c = (t - x) - y
+    static void kahan(Arr<Tp> &d, const Arr<Tp> &x, Arr<Tp> &cc) {
+        for (size_t k = 0; k < d.size(); ++k) {
+            const Tp y = d.a[k] - cc.a[k];
+            const Tp t = x.a[k] + y;
+            cc.a[k] = (t - x.a[k]) - y;
+            d.a[k] = t;
+        }
+    }
+
     bool has_nan() const {                                    // feedback.jl:61-72
         for (auto &x : u.a) if (is_nan(x)) return true;
         for (auto &x : v.a) if (is_nan(x)) return true;
@@ -859,6 +886,11 @@ template <class T, class Tp, class Tc> struct Model {
         auto stop = [&]() { return stop_stage > 0 && stages_run >= stop_stage; };
 
         if (c.time_scheme == SWM_RK) {                         // :53-78
+            if (comp) {                                        // u0 holds the increment Σ a_k du_k
+                for (auto &x : u0.a) x = Tp(0.0);
+                for (auto &x : v0.a) x = Tp(0.0);
+                for (auto &x : eta0.a) x = Tp(0.0);
+            }
             for (int rki = 1; rki <= c.RKo; ++rki) {
                 if (rki > 1) ghost_points(u1, v1, eta1);
                 if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
@@ -873,6 +905,11 @@ template <class T, class Tp, class Tc> struct Model {
                 axb(u0, Tp(c.RKaDt[rki - 1]), du);
                 axb(v0, Tp(c.RKaDt[rki - 1]), dv);
                 axb(eta0, Tp(c.RKaDt[rki - 1]), deta);
+                if (comp && rki == c.RKo) {                    // u0 = u + Σ, compensated
+                    kahan(u0, u, cu);
+                    kahan(v0, v, cv);
+                    kahan(eta0, eta, ce);
+                }
                 ++stages_run;
                 if (stop()) return false;
             }
@@ -995,6 +1032,12 @@ static int run_typed(const swm_config *c, const OracleStatic *s, void *u, void *
     load_arr(M.v, v);
     load_arr(M.eta, eta);
     load_arr(M.sst, sst);
+    if (c->scale_log2 != 0) {                   // u, v and their forcing carried as s·u, s·v
+        for (auto *A : {&M.Fx, &M.Fy, &M.f_q})
+            for (auto &x : A->a) x = x * T(M.sc);
+        for (auto *A : {&M.u, &M.v})
+            for (auto &x : A->a) x = x * Tp(M.sc);
+    }
     M.init();
     *steps_done = 0;
     *nan_step = 0;
@@ -1011,6 +1054,9 @@ static int run_typed(const swm_config *c, const OracleStatic *s, void *u, void *
         if (ss > 0) break;
         t += c->dtint;
     }
+    if (c->scale_log2 != 0)
+        for (auto *A : {&M.u, &M.v, &M.u0, &M.v0, &M.u1, &M.v1, &M.du, &M.dv})
+            for (auto &x : A->a) x = x * Tp(M.sci);
     store_arr(M.u, u);
     store_arr(M.v, v);
     store_arr(M.eta, eta);
@@ -1143,6 +1189,7 @@ static void store_dual(const Arr<Dual> &A, double *v, double *d) {
 int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
              const double *eta_ref, const double *Feta, int32_t op, int64_t nsteps, int64_t i0, int64_t t0,
              const double *const *x, const double *const *dx, double *const *y, double *const *dy) {
+    if (c->scale_log2 != 0 || c->compensated) return -1;   // the tangent-linear model is of the plain step
     auto dxk = [&](int k) { return dx ? dx[k] : nullptr; };
     auto dyk = [&](int k) { return dy ? dy[k] : nullptr; };
     try {
@@ -1216,7 +1263,8 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
               const double *eta_ref, const double *Feta, int64_t i, int64_t t, const double *u, const double *v,
               const double *eta, const double *sst, const double *um, const double *vm, double *const *out) {
-    if (c->time_scheme != SWM_RK || c->T != c->Tprog || c->Tcomm != c->Tprog) return -1;
+    if (c->time_scheme != SWM_RK || c->T != c->Tprog || c->Tcomm != c->Tprog || c->scale_log2 != 0 || c->compensated)
+        return -1;
     const double *st[6] = {H, Fx, Fy, f_q, eta_ref, Feta};
     const double *x[6] = {u, v, eta, sst, um, vm};
     try {

@@ -13,6 +13,7 @@ template <class F> struct DissArgs {
     uint32_t out_u, out_v;            // u0 + Δt_diff*(drag + diffusion), with ghosts, Tprog
     uint32_t H;                       // T
     R cD, rD, cS, nuB;                // T
+    R sinv;                           // 1/s for velocities carried scaled by s (1 unscaled)
     Q dtd, oma;                       // Δt_diff, one_minus_α (Tprog)
     int cm;                           // Tcomm
     int nx, ny, P, L, nstrips, nseg;
@@ -69,7 +70,7 @@ __device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
             h_0 = e_0 + H_0;
             const R KEu_0 = half * (u_0 * u_0 + uxm_0 * uxm_0);
             const R KEv_0 = half * (v_0 * v_0 + v_1 * v_1);
-            sk_0 = nsqrt(KEu_0 + KEv_0);
+            sk_0 = nsqrt(KEu_0 + KEv_0) * A.sinv;
         }
         // stress_tensor!  diffusion.jl:114-122: ∇²!  gradients.jl:29-42 at row r-1
         const R uxp_1 = sh_xp(u_1);
@@ -86,7 +87,7 @@ __device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
             DSq_1 = s1 * s1;
             DSqxm_1 = sh_xm(DSq_1);
             const R DS_1 = quarter * (DSqxm_2 + DSq_2) + quarter * (DSqxm_1 + DSq_1);
-            nu_1 = A.cS * nsqrt(DS_1 + DT_1);
+            nu_1 = A.cS * (nsqrt(DS_1 + DT_1) * A.sinv);
             const R nuxp_1 = sh_xp(nu_1);
             nuq_2 = quarter * (nu_2 + nuxp_2) + quarter * (nu_1 + nuxp_1);
             nuxp_2 = nuxp_1;

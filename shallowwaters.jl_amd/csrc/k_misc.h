@@ -33,7 +33,7 @@ __global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, co
                               const typename F::ST *vm, const typename F::Q *sst, typename F::Q *ssti,
                               typename F::R hdu, typename F::R hdv, typename F::R du_, typename F::R dv_, int nx,
                               int ny, int P, int ep, int consume, typename F::R jSST, typename F::R SSTmin, int cm,
-                              int *bad) {
+                              typename F::R sinv, int *bad) {
     using R = typename F::R;
     using Q = typename F::Q;
     using ST = typename F::ST;
@@ -44,8 +44,9 @@ __global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, co
     auto U = [&](int cx, int cy) { return toR<R>(u[cx + cy * P]); };
     auto V = [&](int cx, int cy) { return toR<R>(v[cx + cy * P]); };
     // Ix!(u_T,u): u_T(x,y) = ½(u(x) + u(x-1)); Iy!(v_T,v): v_T(x,y) = ½(v(y) + v(y-1))
-    const R uT = half * (U(x, y) + U(x - 1, y));
-    const R vT = half * (V(x, y) + V(x, y - 1));
+    // (velocities carried scaled by s: the departure distances use u/s, v/s)
+    const R uT = half * (U(x, y) + U(x - 1, y)) * sinv;
+    const R vT = half * (V(x, y) + V(x, y - 1)) * sinv;
     const R xd0 = -hdu * uT, yd0 = -hdv * vT;
     // interp_uv! on um_T (x-index k <-> T-column k-1-ep, y-index l <-> row l-2) and
     // vm_T (k <-> column k-2, l <-> row l-1)
@@ -77,8 +78,8 @@ __global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, co
         const R f00 = fT(k, l), f10 = fT(k + 1, l), f01 = fT(k, l + 1), f11 = fT(k + 1, l + 1);
         return f00 * (one - x0) * (one - y0) + f10 * x0 * (one - y0) + f01 * (one - x0) * y0 + f11 * x0 * y0;
     };
-    const R ui = interp(um, true, xd0, yd0);
-    const R vi = interp(vm, false, xd0, yd0);
+    const R ui = interp(um, true, xd0, yd0) * sinv;
+    const R vi = interp(vm, false, xd0, yd0) * sinv;
     const R xd = -du_ * ui, yd = -dv_ * vi;
     // adv_sst!  (sst index k <-> column k-1, l <-> row l; m = nx+2, n = ny, halo (1,0))
     const R fx = nfloor(xd), fy = nfloor(yd);
@@ -109,6 +110,16 @@ __global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, co
 }
 
 // ------------------------------------------------------------------ small kernels
+
+// x ← x·f for a power of two f, n elements of storage S (velocities and their forcing
+// carried scaled)
+template <class S> __global__ void scale_kernel(S *p, size_t n, float f) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if constexpr (std::is_same_v<S, double>) p[i] = p[i] * (double)f;
+        else if constexpr (std::is_same_v<S, float>) p[i] = p[i] * f;
+        else stp(p, (long)i, ldp(p, (long)i) * f);
+    }
+}
 
 // ghost_points! of the uploaded state (ghost_points.jl:182-242), in the reference order:
 // phase 0 writes the x-copies, phase 1 the y-copies (which read the x-copies for η).
@@ -205,7 +216,7 @@ __global__ void nan_kernel(NanArgs A, int *flag) {
 // FP64 mass and energy over the interior (swm_reduce): block partial sums in double.
 template <class Q, class ST>
 __global__ void reduce_kernel(const Q *u, const Q *v, const Q *e, const ST *H, int nx, int ny, int P, int ep, double g,
-                              double *part) {
+                              double sinv, double *part) {
     __shared__ double sm[2][256];
     double mass = 0.0, en = 0.0;
     const int y = 1 + blockIdx.x;
@@ -214,7 +225,7 @@ __global__ void reduce_kernel(const Q *u, const Q *v, const Q *e, const ST *H, i
         const double h = ee + (double)ldp(H, x + (long)y * P);
         mass += ee;
         en += 0.5 * g * ee * ee;
-        const double uu = (double)u[x + y * P], vv = (double)v[x + y * P];
+        const double uu = (double)u[x + y * P] * sinv, vv = (double)v[x + y * P] * sinv;
         if (ep || x <= nx - 1) en += 0.5 * h * uu * uu;
         if (y <= ny - 1) en += 0.5 * h * vv * vv;
     }

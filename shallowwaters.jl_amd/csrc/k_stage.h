@@ -15,17 +15,22 @@ namespace swm {
 //   SSP3_KN    o1 = x·w + y·ac + z·d                             dxaybzc!  :129
 //   S4         t = w + c·d,  o1 = h·(w + t)                      caxb!(u0,u1,Δt_Δ,du), cxab!  :172-175
 //   S4_3       ... and o2 = x·pr + y·o1                          cxayb!(u1,2/3,u,1/3,u1)  :187
+//   RKC_FIRST  o1 = w + b·d,  o2 = a·d                 compensated RK: o2 holds Σ a_k·d_k
+//   RKC_LAST   Σ = ac + a·d, then o2 = pr + Σ with Kahan's compensated summation (compensation cc)
 // w is the stage input at the point, pr the Prog value, ac the u0 value; all in Tprog.
-enum { M_RK_FIRST = 0, M_RK_MID, M_RK_LAST, M_SSP2, M_SSP2_LAST, M_SSP3, M_SSP3_KN, M_S4, M_S4_3 };
+enum { M_RK_FIRST = 0, M_RK_MID, M_RK_LAST, M_SSP2, M_SSP2_LAST, M_SSP3, M_SSP3_KN, M_S4, M_S4_3, M_RKC_FIRST,
+       M_RKC_LAST };
 
 template <int M> struct Mode {
-    static constexpr bool rk = M <= M_RK_LAST;   // rhs! and continuity! in one kernel
-    static constexpr bool first = M == M_RK_FIRST;
-    static constexpr bool need_pr = M == M_RK_MID || M == M_SSP2_LAST || M == M_S4_3;
-    static constexpr bool need_ac = M == M_RK_MID || M == M_RK_LAST || M == M_SSP3_KN;
-    static constexpr bool out1 = M != M_RK_LAST;
-    static constexpr bool out2 = M == M_RK_FIRST || M == M_RK_MID || M == M_RK_LAST || M == M_SSP2_LAST || M == M_S4_3;
-    static constexpr bool ghost2 = M != M_RK_FIRST && M != M_RK_MID;    // u0 between RK stages: interior only
+    static constexpr bool rk = M <= M_RK_LAST || M == M_RKC_FIRST || M == M_RKC_LAST;   // rhs! and continuity! in one kernel
+    static constexpr bool first = M == M_RK_FIRST || M == M_RKC_FIRST;
+    static constexpr bool comp = M == M_RKC_LAST;           // reads and writes the compensation
+    static constexpr bool need_pr = M == M_RK_MID || M == M_SSP2_LAST || M == M_S4_3 || M == M_RKC_LAST;
+    static constexpr bool need_ac = M == M_RK_MID || M == M_RK_LAST || M == M_SSP3_KN || M == M_RKC_LAST;
+    static constexpr bool out1 = M != M_RK_LAST && M != M_RKC_LAST;
+    static constexpr bool out2 = M == M_RK_FIRST || M == M_RK_MID || M == M_RK_LAST || M == M_SSP2_LAST || M == M_S4_3 ||
+                                 M == M_RKC_FIRST || M == M_RKC_LAST;
+    static constexpr bool ghost2 = M != M_RK_FIRST && M != M_RK_MID && M != M_RKC_FIRST;   // u0 between RK stages: interior only
 };
 
 template <class Q> struct Coef {
@@ -33,8 +38,16 @@ template <class Q> struct Coef {
 };
 
 template <int M, class Q>
-__device__ __forceinline__ void combine(Q w, Q pr, Q ac, Q d, const Coef<Q> &k, Q &o1, Q &o2) {
+__device__ __forceinline__ void combine(Q w, Q pr, Q ac, Q d, const Coef<Q> &k, Q &o1, Q &o2, Q &cc) {
     if constexpr (M == M_RK_FIRST) { o1 = w + k.b * d; o2 = w + k.a * d; }
+    else if constexpr (M == M_RKC_FIRST) { o1 = w + k.b * d; o2 = k.a * d; }
+    else if constexpr (M == M_RKC_LAST) {
+        const Q s = ac + k.a * d;
+        const Q y = s - cc;
+        const Q t = pr + y;
+        cc = (t - pr) - y;
+        o2 = t;
+    }
     else if constexpr (M == M_RK_MID) { o1 = pr + k.b * d; o2 = ac + k.a * d; }
     else if constexpr (M == M_RK_LAST) { o2 = ac + k.a * d; }
     else if constexpr (M == M_SSP2 || M == M_SSP3) { o1 = w + k.c * d; }
@@ -58,21 +71,26 @@ template <class F> struct StageArgs {
     uint32_t o2_u, o2_v, o2_e;        // second output (u0 / u1'')
     uint32_t qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap), 1-D
     StaticRefs st;
+    uint32_t c_u, c_v, c_e;           // compensations of the compensated RK modes (Tprog)
     Coef<Q> k;                        // stage coefficients (first stage of a fused pair)
     Coef<Q> k2;                       // second stage of a fused pair
-    R Fxt, Fyt, Fet, g, gamma, k12;   // T scalars (k12 = T(1/12))
-    Q oma;                            // one_minus_α (Tprog)
-    int cont, cm;                     // continuity variant, Tcomm
+    // T scalars: k12 = T(1/12)/s, g·s and the ½ of the Bernoulli potential /s for velocities
+    // carried scaled by s (= 1 unscaled)
+    R Fxt, Fyt, Fet, g, gamma, k12, halfp;
+    Q oma, sinv;                      // one_minus_α, 1/s (Tprog)
+    int cont, cm, scaled;             // continuity variant, Tcomm, s != 1
     int nx, ny, P, L, nstrips, nseg;
 };
 
 template <class F, int M> using StageIn = std::conditional_t<Mode<M>::first, typename F::Q, typename F::ST>;
 
 // dη at T-point (x, y) from the flux divergence, continuity.jl:3-68
+// (the divergence of fluxes of velocities carried scaled by s is divided by s)
 template <class Q, class R, class ST>
 __device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, const Fld<ST> &eref, const Fld<ST> &feta,
-                                            uint32_t vo, uint32_t so, R gamma, R Fet) {
+                                            uint32_t vo, uint32_t so, R gamma, R Fet, int scaled, Q sinv) {
     Q d = -(toQ<Q>(dUdx) + toQ<Q>(dVdy));
+    if (scaled) d = d * sinv;
     if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (eref.template ldr<R>(vo, so) - e));
     else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * feta.template ldr<R>(vo, so));
     return d;
@@ -95,7 +113,7 @@ template <class R, bool PER> struct RhsPipe {
 
     // qend_1: q at the corner (nx, r-1), read by the wrapped first column when qfix (periodic x = 0)
     __device__ __forceinline__ void push(R u_0, R v_0, R e_0, R H_0, R fq_1, R qend_1, bool qfix, R half, R quarter,
-                                         R k12, R g) {
+                                         R k12, R g, R halfp) {
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
         const R hx_0 = sh_xp(h_0);
@@ -112,7 +130,7 @@ template <class R, bool PER> struct RhsPipe {
         const R uu_1 = u_1 * u_1;
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
         const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
-        const R p_1 = half * (KEu_1 + KEv_1) + g * e_1;
+        const R p_1 = halfp * (KEu_1 + KEv_1) + g * e_1;
         // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
         const R qxp_1 = sh_xp(q_1);
         R qxm_1 = sh_xm(q_1);
@@ -179,6 +197,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
     const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy}, fqend{&ar, A.qend};
     const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
     // column offsets and row pitches of the three storage formats (equal when T == Tprog)
     const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
@@ -237,7 +256,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         }
         const R fq_1 = fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d));
         const R qend_1 = PER ? fqend.template ldr<R>(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(ST)) : R(0);
-        rhs.push(u_0, v_0, e_0, H_0, fq_1, qend_1, qfix, half, quarter, k12, A.g);
+        rhs.push(u_0, v_0, e_0, H_0, fq_1, qend_1, qfix, half, quarter, k12, A.g, A.halfp);
 
         // du at row r-1: momentum_u!  rhs.jl:242
         const int yu = r - 1;
@@ -245,8 +264,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         if (u_col && row1) {
             const R Fx = fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d));
             const Q du = (toQ<Q>(rhs.qhv) - toQ<Q>(rhs.px)) + toQ<Q>(A.Fxt * Fx);
-            Q o1 = 0, o2 = 0;
-            combine<M>((Q)wu_1, MI::first ? (Q)wu_1 : pu_1, au_1, du, A.k, o1, o2);
+            Q o1 = 0, o2 = 0, cc = 0;
+            if (MI::comp) cc = cmp_u.ld(vxQ, sQ - dQ);
+            combine<M>((Q)wu_1, MI::first ? (Q)wu_1 : pu_1, au_1, du, A.k, o1, o2, cc);
+            if (MI::comp) cmp_u.st(cc, vxQ, sQ - dQ);
             if (MI::out1) put_u<ST, PER>(o1_u, x, vxT, yu, sT - dT, o1, gl.u, nx, ny, P, A.oma, A.cm);
             if (MI::out2) {
                 if (MI::ghost2) put_u<Q, PER>(o2_u, x, vxQ, yu, sQ - dQ, o2, gl.u, nx, ny, P, A.oma, A.cm);
@@ -255,9 +276,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         }
         // dη at row r-1: continuity!  continuity.jl:52-98
         if (MI::rk && e_col && row1) {
-            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
-            Q o1 = 0, o2 = 0;
-            combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2);
+            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
+                                               A.scaled, A.sinv);
+            Q o1 = 0, o2 = 0, cc = 0;
+            if (MI::comp) cc = cmp_e.ld(vxQ, sQ - dQ);
+            combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2, cc);
+            if (MI::comp) cmp_e.st(cc, vxQ, sQ - dQ);
             if (MI::out1) put_e<ST, PER>(o1_e, x, vxT, yu, sT - dT, o1, gl.e, nx, ny, P, A.cm);
             if (MI::out2) {
                 if (MI::ghost2) put_e<Q, PER>(o2_e, x, vxQ, yu, sQ - dQ, o2, gl.e, nx, ny, P, A.cm);
@@ -269,8 +293,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
             const R Fy = fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d));
             const Q dv = -(toQ<Q>(rhs.qhu) + toQ<Q>(rhs.py)) + toQ<Q>(A.Fyt * Fy);
-            Q o1 = 0, o2 = 0;
-            combine<M>((Q)wv_2, MI::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, o1, o2);
+            Q o1 = 0, o2 = 0, cc = 0;
+            if (MI::comp) cc = cmp_v.ld(vxQ, sQ - 2 * dQ);
+            combine<M>((Q)wv_2, MI::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, o1, o2, cc);
+            if (MI::comp) cmp_v.st(cc, vxQ, sQ - 2 * dQ);
             if (MI::out1) put_v<ST, PER>(o1_v, x, vxT, yv, sT - 2 * dT, o1, gl.v, nx, P, A.oma, A.cm);
             if (MI::out2) {
                 if (MI::ghost2) put_v<Q, PER>(o2_v, x, vxQ, yv, sQ - 2 * dQ, o2, gl.v, nx, P, A.oma, A.cm);
@@ -338,6 +364,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
     const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy};
     const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
     const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
     const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
@@ -378,17 +405,19 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
 
         // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
         ra.push(u_0, v_0, e_0, H_0, fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d)), R(0), false, half,
-                quarter, k12, A.g);
+                quarter, k12, A.g, A.halfp);
         Q cu1, cu0, ce1, ce0, cv1, cv0;
         {
             const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) +
                          toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(r - 1, P, A.st.Fx_2d)));
-            combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0);
-            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
-            combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0);
+            Q nocc = 0;
+            combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0, nocc);
+            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
+                                               A.scaled, A.sinv);
+            combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0, nocc);
             const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) +
                          toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(r - 2, P, A.st.Fy_2d)));
-            combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0);
+            combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
         }
         // ghost_points!(u1, v1, η1) on A's first outputs (nonperiodic): u's normal halo is 0,
         // its tangential rows mirror with (1-α); v's normal rows are 0, its columns mirror
@@ -418,23 +447,27 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             const Q ub = rbi == 0 ? oma * U1_0 : U1_1;          // row 0: (1-α)·u1 of row 1
             const Q eb = rbi == 0 ? E1_0 : E1_1;                // row 0: η1 of row 1
             rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, fq.template ldr<R>(vfq, srowb<ST>(rbi - 1, P, A.st.fq_2d)),
-                    R(0), false, half, quarter, k12, A.g);
+                    R(0), false, half, quarter, k12, A.g, A.halfp);
             const int yu = r - 3;
             if (yu >= t.ya && yu < t.yb) {
                 if (u_col) {
                     const Q du = (toQ<Q>(rb.qhv) - toQ<Q>(rb.px)) +
                                  toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d)));
-                    Q o1 = 0, o2 = 0;
-                    combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2);
+                    Q o1 = 0, o2 = 0, cc = 0;
+                    if (MIB::comp) cc = cmp_u.ld(vxQ, sQ - 3 * dQ);
+                    combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2, cc);
+                    if (MIB::comp) cmp_u.st(cc, vxQ, sQ - 3 * dQ);
                     if (MIB::out1) put_u<ST, false>(o1_u, x, vxT, yu, sT - 3 * dT, o1, gl.u, nx, ny, P, oma, A.cm);
                     if (MIB::ghost2) put_u<Q, false>(o2_u, x, vxQ, yu, sQ - 3 * dQ, o2, gl.u, nx, ny, P, oma, A.cm);
                     else o2_u.st(o2, vxQ, sQ - 3 * dQ);
                 }
                 if (e_col) {
                     const Q deta = continuity_rhs<Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, eref, feta, vxT, sT - 3 * dT,
-                                                     A.gamma, A.Fet);
-                    Q o1 = 0, o2 = 0;
-                    combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2);
+                                                     A.gamma, A.Fet, A.scaled, A.sinv);
+                    Q o1 = 0, o2 = 0, cc = 0;
+                    if (MIB::comp) cc = cmp_e.ld(vxQ, sQ - 3 * dQ);
+                    combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2, cc);
+                    if (MIB::comp) cmp_e.st(cc, vxQ, sQ - 3 * dQ);
                     if (MIB::out1) put_e<ST, false>(o1_e, x, vxT, yu, sT - 3 * dT, o1, gl.e, nx, ny, P, A.cm);
                     if (MIB::ghost2) put_e<Q, false>(o2_e, x, vxQ, yu, sQ - 3 * dQ, o2, gl.e, nx, ny, P, A.cm);
                     else o2_e.st(o2, vxQ, sQ - 3 * dQ);
@@ -444,8 +477,10 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
                 const Q dv = -(toQ<Q>(rb.qhu) + toQ<Q>(rb.py)) +
                              toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d)));
-                Q o1 = 0, o2 = 0;
-                combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2);
+                Q o1 = 0, o2 = 0, cc = 0;
+                if (MIB::comp) cc = cmp_v.ld(vxQ, sQ - 4 * dQ);
+                combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2, cc);
+                if (MIB::comp) cmp_v.st(cc, vxQ, sQ - 4 * dQ);
                 if (MIB::out1) put_v<ST, false>(o1_v, x, vxT, yv, sT - 4 * dT, o1, gl.v, nx, P, oma, A.cm);
                 if (MIB::ghost2) put_v<Q, false>(o2_v, x, vxQ, yv, sQ - 4 * dQ, o2, gl.v, nx, P, oma, A.cm);
                 else o2_v.st(o2, vxQ, sQ - 4 * dQ);
@@ -521,9 +556,11 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
         const R Uxm_1 = sh_xm(U_1);
         const int y = r - 1;
         if (e_col && y >= t.ya && y < t.yb) {
-            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet);
+            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
+                                               A.scaled, A.sinv);
             Q o1 = 0, o2 = 0;
-            combine<M>(we_1, pe_1, ae_1, deta, A.k, o1, o2);
+            Q nocc = 0;
+            combine<M>(we_1, pe_1, ae_1, deta, A.k, o1, o2, nocc);
             if (M == M_S4_3) put_e<Q, PER>(o1_e, x, vxQ, y, sQ - dQ, o2, gl.e, nx, ny, P, A.cm);   // only 2/3 η + 1/3 η1' lives on
             else {
                 put_e<Q, PER>(o1_e, x, vxQ, y, sQ - dQ, o1, gl.e, nx, ny, P, A.cm);

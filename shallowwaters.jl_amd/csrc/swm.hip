@@ -125,6 +125,8 @@ struct swm_ctx {
     Field u[6], v[6], e[6], sst[2];                          // Tprog (slots 4, 5: the adjoint's taped step)
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
+    Field cu, cv, ce;                                        // Kahan compensations (compensated RK), Tprog
+    double sc = 1.0, sci = 1.0;                              // u, v carried as sc·u, sc·v
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0;
     int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
     int rsst = 0;
@@ -334,8 +336,13 @@ template <class F> struct Launch {
         A.st = statics(ctx);
         A.k = k;
         A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
-        A.g = mkR<R>(c.g); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(1.0 / 12.0, c.T));
+        // scaled velocities: PV weights /s, Bernoulli ½ /s and g·s (powers of two, exact)
+        A.g = mkR<R>(c.g * ctx->sc); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(1.0 / 12.0, c.T) * ctx->sci);
+        A.halfp = mkR<R>(0.5 * ctx->sci);
         A.oma = (Q)c.one_minus_alpha;
+        A.sinv = (Q)ctx->sci;
+        A.scaled = c.scale_log2 != 0;
+        A.c_u = ctx->cu.ref(); A.c_v = ctx->cv.ref(); A.c_e = ctx->ce.ref();
         A.cont = c.continuity; A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
         A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->L);
@@ -345,18 +352,20 @@ template <class F> struct Launch {
     // Every buffer a mode reads or writes must name a slot (no launch through a null base),
     // and the stage input / first output live in the pool the mode's formats expect.
     static bool slots_ok(const swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, bool cont) {
-        const bool rk = m <= M_RK_LAST;
-        const bool need_pr = m == M_RK_MID || m == M_SSP2_LAST || m == M_S4_3;
-        const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN;
-        const bool out1 = m != M_RK_LAST;
-        const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont);
+        const bool rk = m <= M_RK_LAST || m == M_RKC_FIRST || m == M_RKC_LAST;
+        const bool need_pr = m == M_RK_MID || m == M_SSP2_LAST || m == M_S4_3 || m == M_RKC_LAST;
+        const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN || m == M_RKC_LAST;
+        const bool out1 = m != M_RK_LAST && m != M_RKC_LAST;
+        const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont) ||
+                          m == M_RKC_FIRST || m == M_RKC_LAST;
+        if (m == M_RKC_LAST && !ctx->cu.base) return false;
         auto ok = [ctx](Slots s, bool uv, bool e, bool tpool) {
             const int n = tpool ? 2 : ctx->nslots;
             if (s.t != tpool) return false;
             return (!uv || (s.uv >= 0 && s.uv < n)) && (!e || (s.e >= 0 && s.e < n));
         };
         const bool uv = !cont, e = rk || cont;            // which of (u,v) and η the kernel touches
-        const bool tin = ctx->mixed && m != M_RK_FIRST;   // stage input u1 in the T pool
+        const bool tin = ctx->mixed && m != M_RK_FIRST && m != M_RKC_FIRST;   // stage input u1 in the T pool
         const bool tout = ctx->mixed;                     // first output u1 in the T pool
         if (!ok(in, uv, e, tin)) return false;
         if (need_pr && !ok(pr, uv, e, false)) return false;
@@ -390,12 +399,13 @@ template <class F> struct Launch {
         break;
         if constexpr (F::mixed) {
             switch (m) {
-                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST)
+                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_RKC_FIRST) SWM_K(M_RKC_LAST)
             default: return ctx->fail(SWM_E_UNSUPPORTED, "stage mode %d with mixed formats", m);
             }
         } else {
             switch (m) {
-                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
+                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_RKC_FIRST) SWM_K(M_RKC_LAST) SWM_K(M_SSP2)
+                SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
                 SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
             default: return ctx->fail(SWM_E_ARG, "stage mode %d", m);
             }
@@ -417,8 +427,9 @@ template <class F> struct Launch {
             const int n = tpool ? 2 : 4;
             return s.t == tpool && s.uv >= 0 && s.uv < n && s.e >= 0 && s.e < n;
         };
-        const bool good = ok(in, ctx->mixed && ma != M_RK_FIRST) && ok(pr, false) && (ma == M_RK_FIRST || ok(ac, false)) &&
-                          (mb == M_RK_LAST || ok(o1, ctx->mixed)) && ok(o2, false);
+        const bool afirst = ma == M_RK_FIRST || ma == M_RKC_FIRST, blast = mb == M_RK_LAST || mb == M_RKC_LAST;
+        const bool good = ok(in, ctx->mixed && !afirst) && ok(pr, false) && (afirst || ok(ac, false)) &&
+                          (blast || ok(o1, ctx->mixed)) && ok(o2, false) && (mb != M_RKC_LAST || ctx->cu.base);
         if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
         StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
         A.k2 = kb;
@@ -433,6 +444,9 @@ template <class F> struct Launch {
         if (ma == M_RK_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
         else if (ma == M_RK_MID && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RK_LAST>), grid, block, 0, ctx->stream, A);
         else if (ma == M_RK_FIRST && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_LAST>), grid, block, 0, ctx->stream, A);
+        else if (ma == M_RKC_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RKC_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
+        else if (ma == M_RK_MID && mb == M_RKC_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RKC_LAST>), grid, block, 0, ctx->stream, A);
+        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RKC_FIRST, M_RKC_LAST>), grid, block, 0, ctx->stream, A);
         else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
         HIPCHK(hipGetLastError());
         return SWM_OK;
@@ -486,7 +500,10 @@ template <class F> struct Launch {
         int rc = SWM_OK;
         if (c.time_scheme == SWM_RK) {                                     // :55-78
             const Slots a{pick({P.uv}), pick({P.e})};
-            auto mode = [&](int s) { return s == 1 ? M_RK_FIRST : (s == c.RKo ? M_RK_LAST : M_RK_MID); };
+            const bool cp = c.compensated != 0;                            // increments summed apart, Kahan add
+            auto mode = [&](int s) {
+                return s == 1 ? (cp ? M_RKC_FIRST : M_RK_FIRST) : (s == c.RKo ? (cp ? M_RKC_LAST : M_RK_LAST) : M_RK_MID);
+            };
             auto coef = [&](int s) {
                 Coef<Q> kk{};
                 kk.a = (Q)c.RKaDt[s - 1];
@@ -566,6 +583,7 @@ template <class F> struct Launch {
         A.out_u = ctx->u[out_uv].ref(); A.out_v = ctx->v[out_uv].ref();
         A.H = ctx->H.ref();
         A.cD = mkR<R>(c.cD); A.rD = mkR<R>(c.rD); A.cS = mkR<R>(c.cSmag); A.nuB = mkR<R>(c.nuB);
+        A.sinv = mkR<R>(ctx->sci);
         A.dtd = (Q)c.Dt_diff; A.oma = (Q)c.one_minus_alpha;
         A.cm = c.Tcomm;
         A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->Ld;
@@ -602,7 +620,7 @@ template <class F> struct Launch {
         hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<Q>(ctx->u[uv_slot]), ctx->at<Q>(ctx->v[uv_slot]),
                            ctx->at<ST>(ctx->um), ctx->at<ST>(ctx->vm), ctx->at<Q>(ctx->sst[src]), ctx->at<Q>(ctx->sst[dst]),
                            mkR<R>(c.half_dtadvu), mkR<R>(c.half_dtadvv), mkR<R>(c.dtadvu), mkR<R>(c.dtadvv), c.nx, c.ny,
-                           ctx->P, ctx->ep, c.tracer_consumption, mkR<R>(c.jSST), mkR<R>(c.SSTmin), c.Tcomm,
+                           ctx->P, ctx->ep, c.tracer_consumption, mkR<R>(c.jSST), mkR<R>(c.SSTmin), c.Tcomm, mkR<R>(ctx->sci),
                            ctx->dflags + 1);
         HIPCHK(hipGetLastError());
         return SWM_OK;
@@ -762,6 +780,8 @@ int check_config(swm_ctx *ctx) {
     if (c.time_scheme == SWM_SSPRK2 && c.RKs < 2) return ctx->fail(SWM_E_ARG, "RKs %d", c.RKs);
     if (c.time_scheme == SWM_SSPRK3 && (c.ssp_n < 2 || c.ssp_s != c.ssp_n * c.ssp_n)) return ctx->fail(SWM_E_ARG, "RKn %d", c.ssp_n);
     if (c.nstep_advcor != 0) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
+    if (c.scale_log2 < -20 || c.scale_log2 > 20) return ctx->fail(SWM_E_ARG, "scale_log2 %d out of -20..20", c.scale_log2);
+    if (c.compensated && c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "compensated needs the classic RK scheme");
     if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) return ctx->fail(SWM_E_UNSUPPORTED, "adv_scheme Sadourny is not implemented");
     if (c.dynamics != SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "linear dynamics is not implemented");
     if (c.continuity < SWM_CONT_ITSELF || c.continuity > SWM_CONT_FORCING) return ctx->fail(SWM_E_ARG, "continuity %d", c.continuity);
@@ -843,7 +863,8 @@ std::vector<Field *> all_fields(swm_ctx *ctx) {
     for (int s = 0; s < 2; ++s) {
         f.push_back(&ctx->sst[s]); f.push_back(&ctx->tu[s]); f.push_back(&ctx->tv[s]); f.push_back(&ctx->te[s]);
     }
-    for (Field *g : {&ctx->um, &ctx->vm, &ctx->H, &ctx->Fx, &ctx->Fy, &ctx->fq, &ctx->eta_ref, &ctx->Feta, &ctx->qend})
+    for (Field *g : {&ctx->um, &ctx->vm, &ctx->H, &ctx->Fx, &ctx->Fy, &ctx->fq, &ctx->eta_ref, &ctx->Feta, &ctx->qend,
+                     &ctx->cu, &ctx->cv, &ctx->ce})
         f.push_back(g);
     return f;
 }
@@ -900,6 +921,8 @@ template <class Fm> struct Adj {
         if (c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK scheme only");
         if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
+        if (c.scale_log2 != 0 || c.compensated)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -1717,6 +1740,8 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     ctx->nuy = c.ny;
     ctx->nvx = c.nx;
     ctx->nvy = c.ny - 1;
+    ctx->sc = std::ldexp(1.0, c.scale_log2);
+    ctx->sci = std::ldexp(1.0, -c.scale_log2);
     ctx->P = ((c.nx + 2 * OX + 63) / 64) * 64;
     ctx->rows = c.ny + 2 * OY + 4;
     ctx->elems = (size_t)ctx->P * ctx->rows;
@@ -1751,7 +1776,7 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     };
     // the arena: Tprog fields 12 state slots + 2 sst; T fields um, vm, H, Fx, Fy, f_q
     // (+ η_ref, Fη; + 6 u1 slots with mixed formats) and the q column
-    const int nq = 14;
+    const int nq = 14 + (c.compensated ? 3 : 0);
     const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0);
     const size_t n1d = (size_t)c.ny + 2 * OY + 16;
     ctx->arena_bytes = (size_t)nq * (ctx->elems * ctx->es + 256) + (size_t)nt * (ctx->elems * ctx->est + 256) +
@@ -1781,6 +1806,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         (rc = alloc_field(ctx, ctx->H, 0, true, est)) || (rc = alloc_field(ctx, ctx->Fx, 0, true, est)) ||
         (rc = alloc_field(ctx, ctx->Fy, 0, true, est)) || (rc = alloc_field(ctx, ctx->fq, 0, true, est)) ||
         (rc = alloc_field(ctx, ctx->qend, n1d, false, est)))
+        return bail(rc);
+    if (c.compensated && ((rc = alloc_field(ctx, ctx->cu, 0, true, es)) || (rc = alloc_field(ctx, ctx->cv, 0, true, es)) ||
+                          (rc = alloc_field(ctx, ctx->ce, 0, true, es))))
         return bail(rc);
     if (c.continuity != SWM_CONT_ITSELF &&
         ((rc = alloc_field(ctx, ctx->eta_ref, 0, true, est)) || (rc = alloc_field(ctx, ctx->Feta, 0, true, est))))
@@ -1822,6 +1850,17 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
     if ((rc = put_static(ctx, ctx->Fx, &ctx->Fx_sx, &ctx->Fx_sy, Fx, ctx->nux, ctx->nuy, 1 - ctx->ep, 1))) return rc;
     if ((rc = put_static(ctx, ctx->Fy, &ctx->Fy_sx, &ctx->Fy_sy, Fy, ctx->nvx, ctx->nvy, 1, 1))) return rc;
     if ((rc = put_static(ctx, ctx->fq, &ctx->fq_sx, &ctx->fq_sy, f_q, c.nx + 1, c.ny + 1, 0, 0))) return rc;
+    if (c.scale_log2 != 0) {                     // the forcing and f of velocities carried scaled: ×s (exact)
+        rc = with_fmt(ctx, [&](auto f) {
+            using ST = typename decltype(f)::ST;
+            for (Field *g : {&ctx->Fx, &ctx->Fy, &ctx->fq})
+                hipLaunchKernelGGL((scale_kernel<ST>), dim3(1024), dim3(256), 0, ctx->stream, (ST *)g->base, ctx->elems,
+                                   (float)ctx->sc);
+            HIPCHK(hipGetLastError());
+            return SWM_OK;
+        });
+        if (rc) return rc;
+    }
     if (c.continuity == SWM_CONT_RELAX) {
         if (!eta_ref) return ctx->fail(SWM_E_ARG, "eta_ref is required for surface relaxation");
         if ((rc = put2d(ctx, ctx->eta_ref, eta_ref, c.nx, c.ny, 1, 1))) return rc;
@@ -1860,6 +1899,13 @@ static int check_never_written(swm_ctx *ctx, const void *u, const void *v) {
     return SWM_OK;
 }
 
+// host arrays of velocities carried scaled: ×1/s in their Tprog storage (exact)
+static void unscale_host(const swm_ctx *ctx, void *a, size_t n) {
+    if (!a || ctx->c.scale_log2 == 0) return;
+    if (ctx->es == 8) for (size_t k = 0; k < n; ++k) ((double *)a)[k] *= ctx->sci;
+    else for (size_t k = 0; k < n; ++k) ((float *)a)[k] *= (float)ctx->sci;
+}
+
 int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta, const void *sst) {
     if (!ctx) return SWM_E_ARG;
     if (!u || !v || !eta || !sst) return ctx->fail(SWM_E_ARG, "null array");
@@ -1871,7 +1917,19 @@ int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta,
     if ((rc = put2d(ctx, ctx->v[pu], v, ctx->nvx + 4, ctx->nvy + 4, -1, -1))) return rc;
     if ((rc = put2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
     if ((rc = put2d(ctx, ctx->sst[ctx->rsst], sst, c.nx + 2, c.ny, 0, 1))) return rc;
-    rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::ghosts(ctx, pu, pe, ctx->rsst); });
+    rc = with_fmt(ctx, [&](auto f) {
+        using Q = typename decltype(f)::Q;
+        if (c.scale_log2 != 0)
+            for (Field *g : {&ctx->u[pu], &ctx->v[pu]})
+                hipLaunchKernelGGL((scale_kernel<Q>), dim3(1024), dim3(256), 0, ctx->stream, (Q *)g->base, ctx->elems,
+                                   (float)ctx->sc);
+        if (c.compensated)                       // a new state: no compensation carried over
+            for (Field *g : {&ctx->cu, &ctx->cv, &ctx->ce})
+                hipLaunchKernelGGL(zero_words_kernel, dim3(1024), dim3(256), 0, ctx->stream, (uint32_t *)g->base,
+                                   ctx->elems * ctx->es / 4);
+        HIPCHK(hipGetLastError());
+        return Launch<decltype(f)>::ghosts(ctx, pu, pe, ctx->rsst);
+    });
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_prog = true;
@@ -1889,6 +1947,8 @@ int swm_download_prog(swm_ctx *ctx, void *u, void *v, void *eta, void *sst) {
     if (eta && (rc = get2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
     if (sst && (rc = get2d(ctx, ctx->sst[ctx->rsst], sst, c.nx + 2, c.ny, 0, 1))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    unscale_host(ctx, u, (size_t)(ctx->nux + 4) * (ctx->nuy + 4));
+    unscale_host(ctx, v, (size_t)(ctx->nvx + 4) * (ctx->nvy + 4));
     return SWM_OK;
 }
 
@@ -1930,6 +1990,8 @@ int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void
     if (v1 && (rc = get(ctx->fv(s1), v1, mv, nv, -1, -1, s1.t))) return rc;
     if (eta1 && (rc = get(ctx->fe(s1), eta1, c.nx + 2, c.ny + 2, 0, 0, s1.t))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (void *a : {u0, u1}) unscale_host(ctx, a, (size_t)mu * nu);
+    for (void *a : {v0, v1}) unscale_host(ctx, a, (size_t)mv * nv);
     return SWM_OK;
 }
 
@@ -1951,7 +2013,7 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
         using ST = typename F::ST;
         reduce_kernel<Q, ST><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<Q>(ctx->u[pu]), ctx->at<Q>(ctx->v[pu]),
                                                             ctx->at<Q>(ctx->e[pe]), ctx->at<ST>(ctx->H), c.nx, c.ny,
-                                                            ctx->P, ctx->ep, c.g, ctx->dpart);
+                                                            ctx->P, ctx->ep, c.g, ctx->sci, ctx->dpart);
         HIPCHK(hipGetLastError());
         return SWM_OK;
     });
@@ -1991,6 +2053,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
+        else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";
         if (why) {
             fprintf(stderr, "swm_ens_create: member %d: %s\n", k, why);

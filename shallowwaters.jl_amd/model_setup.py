@@ -8,6 +8,7 @@ specialised on those enums instead of branching on strings per cell.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 
 from .constants import Constants, make_constants
@@ -16,7 +17,7 @@ from .grid import Grid, make_grid
 from .numtypes import TYPE_CODES
 from .parameter import Parameter
 
-SWM_ABI_VERSION = 1
+SWM_ABI_VERSION = 2
 SWM_PERIODIC, SWM_NONPERIODIC = 0, 1
 SCHEMES = {"RK": 0, "SSPRK2": 1, "SSPRK3": 2, "4SSPRK3": 3}
 DRAGS = {"quadratic": 0, "linear": 1, "none": 2}
@@ -38,7 +39,7 @@ class swm_config(ctypes.Structure):
         ("diffusion", ctypes.c_int32), ("continuity", ctypes.c_int32),
         ("tracer_advection", ctypes.c_int32), ("tracer_consumption", ctypes.c_int32),
         ("seasonal_wind_x", ctypes.c_int32), ("seasonal_wind_y", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("scale_log2", ctypes.c_int32),
         ("dtint", ctypes.c_int64), ("nadvstep", ctypes.c_int64), ("nadvstep_half", ctypes.c_int64),
         ("nout", ctypes.c_int64),
         ("RKaDt", ctypes.c_double * 4), ("RKbDt", ctypes.c_double * 3),
@@ -53,6 +54,7 @@ class swm_config(ctypes.Structure):
         ("half_dtadvu", ctypes.c_double), ("half_dtadvv", ctypes.c_double),
         ("jSST", ctypes.c_double), ("SSTmin", ctypes.c_double),
         ("omegaFx", ctypes.c_double), ("omegaFy", ctypes.c_double), ("omegaFeta", ctypes.c_double),
+        ("compensated", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 
@@ -101,4 +103,6 @@ def make_config(S: ModelSetup) -> swm_config:
     c.dtadvu, c.dtadvv, c.half_dtadvu, c.half_dtadvv = G.dtadvu, G.dtadvv, G.half_dtadvu, G.half_dtadvv
     c.jSST, c.SSTmin = C.jSST, C.SSTmin
     c.omegaFx, c.omegaFy, c.omegaFeta = C.ωFx, C.ωFy, C.ωFη
+    c.scale_log2 = int(round(math.log2(P.scale)))
+    c.compensated = int(P.compensated)
     return c

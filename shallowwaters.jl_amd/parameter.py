@@ -97,6 +97,13 @@ class Parameter:
     νB: float = 500.0
     cSmag: float = 0.15
 
+    # LOW-PRECISION OPTIONS (not in this version of the reference; the name `scale` follows
+    # later ShallowWaters.jl versions): u, v are carried as scale·u, scale·v inside the time
+    # step (a power of two, so Float32 / Float64 results are unchanged); compensated sums
+    # the RK increments apart and adds them with Kahan summation (classic RK only)
+    scale: float = 1.0
+    compensated: bool = False
+
     # TRACER ADVECTION
     tracer_advection: bool = True
     tracer_relaxation: bool = False
@@ -181,6 +188,10 @@ class Parameter:
         assert self.init_starti > 0 or self.init_starti == -1, \
             f"Start index, init_starti, has to be >0 || -1, {self.init_starti} given."
         assert self.get_id_mode in ["continue", "fill", "specific"], f"get_id_mode {self.get_id_mode} unsupported."
+        k = math.log2(self.scale) if self.scale > 0 else 0.5
+        assert self.scale > 0 and k == int(k) and -20 <= k <= 20, \
+            f"scale has to be a power of two in 2^-20 .. 2^20, {self.scale} given."
+        assert not self.compensated or self.time_scheme == "RK", "compensated requires the classic RK scheme"
         # rejected here although the reference accepts them (see module docstring)
         assert self.wind_forcing_y in ["constant", "none"], \
             f"wind_forcing_y '{self.wind_forcing_y}' builds Fy on the u-grid; only 'constant' and 'none' are supported"

@@ -32,7 +32,7 @@ def test_library_loads_and_exports_every_declared_symbol(swm):
     assert not missing, missing
     assert sorted(_abi.EXPORTS) == declared_symbols()
     v = ctypes.c_int32(0)
-    assert L.swm_version(ctypes.byref(v)) == 0 and v.value == 1
+    assert L.swm_version(ctypes.byref(v)) == 0 and v.value == 2
 
 
 def test_config_struct_matches_header(swm):

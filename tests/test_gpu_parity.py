@@ -70,6 +70,18 @@ CASES = {
                              topography="seamount", wind_forcing_x="channel"),
     "per-f64-comm-bf16": dict(T="Float64", Tcomm="BFloat16", nx=60, L_ratio=1.0, bc="periodic", time_scheme="SSPRK2",
                               RKs=3, cfl=1.2),
+    # low-precision options: u, v carried scaled (Float16: s·u·h must stay below 65504, so
+    # s = 8 for these velocities), Kahan-compensated RK increments
+    "np-f16-f32-scaled-comp": dict(T="Float16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5,
+                                   diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                   wind_forcing_x="double_gyre", scale=8.0, compensated=True),
+    "per-bf16-f32-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
+                                diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                wind_forcing_x="channel", scale=64.0, tracer_consumption=True),
+    "np-f32-comp-rk3": dict(T="Float32", nx=61, L_ratio=0.8, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                            topography="seamount", wind_forcing_x="double_gyre", compensated=True, RKo=3, cfl=0.6),
+    "per-f64-scaled-comp-ssp": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", scale=64.0,
+                                    time_scheme="SSPRK2", RKs=3, cfl=1.2),
     # tracer consumption (tracer_advection.jl:262-275)
     "np-f32-consumption": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", tracer_consumption=True,
                                topography="seamount", wind_forcing_x="double_gyre"),
@@ -125,7 +137,10 @@ def test_rk_stages_bitwise(swm, orc, gpu, case):
             # u1 enters the next stage after ghost_points! (time_integration.jl:57,84,118,162)
             r_u1, r_v1, r_e1 = ghosted(swm, S, r_u1, r_v1, r_e1)
             if P.T != P.Tprog:      # u1rhs = T(u1): the GPU keeps only that (time_integration.jl:62-64)
-                r_u1, r_v1, r_e1 = (swm.to_float64(swm.to_storage(a, P.T), P.T) for a in (r_u1, r_v1, r_e1))
+                # (velocities carried scaled are rounded to T as s·u, then unscaled)
+                sc = P.scale
+                r_u1, r_v1 = (swm.to_float64(swm.to_storage(a * sc, P.T), P.T) / sc for a in (r_u1, r_v1))
+                r_e1 = swm.to_float64(swm.to_storage(r_e1, P.T), P.T)
             assert_same(f"{case} stage {k} u1", r_u1, g_u1, T)
             assert_same(f"{case} stage {k} v1", r_v1, g_v1, T)
             assert_same(f"{case} stage {k} η1", r_e1, g_e1, T)

@@ -279,3 +279,43 @@ def test_golden_fixtures(swm, orc):
         got = run_case(swm, orc, name)
         for k in ("u", "v", "eta", "sst"):
             assert np.array_equal(ref[k], got[k]), (name, k)
+
+
+# ------------------------------------------------------------------ low-precision options
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("T", ["Float64", "Float32"])
+def test_scaled_velocities_change_nothing_at_float32_and_float64(swm, orc, bc, T):
+    """u, v carried as 2^6·u, 2^6·v: every factor 1/s sits on an exact operation (powers of
+    two), so the run is bitwise the unscaled one wherever nothing under- or overflows.  A
+    misplaced or missing factor breaks this (PV weights, Bernoulli ½ and g, divergence,
+    drag speed, Smagorinsky ν, tracer departure, forcing, f)."""
+    kw = dict(T=T, nx=40, L_ratio=0.8, bc=bc, α=0.5, diffusion="Smagorinsky", bottom_drag="quadratic",
+              topography="seamount", wind_forcing_x="double_gyre", Uadv=20.0, tracer_consumption=True)
+    runs = []
+    for sc in (1.0, 64.0):
+        S = swm.make_setup(swm.Parameter(scale=sc, **kw))
+        prog = swm.seeded_initial_conditions(S, seed=3, σ=0.5)
+        assert S.grid.nadvstep <= 5
+        runs.append(orc.run(S, swm.make_config(S), prog, 10)[:4])
+    for name, a, b in zip(("u", "v", "η", "sst"), *runs):
+        assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+def test_compensated_increments_track_float64_closer(swm, orc, bc):
+    """Kahan-compensated RK increments: a Float32 run stays closer to the Float64 run than
+    the plain Float32 run (200 steps; the remaining error is the Float32 right-hand side)."""
+    kw = dict(nx=48, L_ratio=0.8, bc=bc, α=0.5, diffusion="Smagorinsky", bottom_drag="quadratic",
+              topography="seamount", wind_forcing_x="double_gyre", Uadv=20.0)
+    n = 200
+    S64 = swm.make_setup(swm.Parameter(T="Float64", **kw))
+    r64 = orc.run(S64, swm.make_config(S64), swm.seeded_initial_conditions(S64, seed=3, σ=0.5), n)[:3]
+    err = {}
+    for comp in (False, True):
+        S = swm.make_setup(swm.Parameter(T="Float32", compensated=comp, **kw))
+        r = orc.run(S, swm.make_config(S), swm.seeded_initial_conditions(S, seed=3, σ=0.5), n)[:3]
+        err[comp] = [np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b) for a, b in zip(r, r64)]
+    for k, name in enumerate(("u", "v", "η")):
+        assert err[True][k] < err[False][k], f"{name}: compensated {err[True][k]:.2e} vs plain {err[False][k]:.2e}"
+    assert err[True][0] < 0.8 * err[False][0]
