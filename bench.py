@@ -26,10 +26,10 @@ HBM_PEAK_GBS = 8000.0
 NX = 4096
 
 
-def workload_parameter(swm, T="Float32", nx=NX, Tprog=None):
+def workload_parameter(swm, T="Float32", nx=NX, Tprog=None, **kw):
     return swm.Parameter(T=T, Tprog=Tprog or T, Tcomm=Tprog or T, nx=nx, Lx=3840e3, L_ratio=1, bc="nonperiodic", α=0.0, topography="seamount",
                          wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky", bottom_drag="quadratic",
-                         tracer_advection=True, Ndays=10)
+                         tracer_advection=True, Ndays=10, **kw)
 
 
 def bytes_per_cell_stage(es: int, est: int = None) -> dict:
@@ -60,10 +60,10 @@ STEP_KERNELS = {"unfused": ("stage1", "stage_mid", "stage_mid", "stage4", "dissi
                 "fused": ("fused12", "fused34", "dissipation")}
 
 
-def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps):
+def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps, **kw):
     """Time `steps` steps of the workload with right-hand side format T and state format
     Tprog (state resident in HBM), then the four step kernels with HIP events."""
-    S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tprog))
+    S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tprog, **kw))
     G = S.grid
     prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
     with swm.Simulation(S) as sim:
@@ -179,13 +179,17 @@ def precision_report(swm, nx, days_list):
     Float64 run of the same seeded state after each of `days_list` model days: relative L2
     of η, u, v over the interior, relative difference of total energy, and mass."""
     import numpy as np
-    forms = [("Float64", "Float64"), ("Float32", "Float32"), ("BFloat16", "Float32"), ("Float16", "Float32")]
+    # (T, Tprog, options): the scale of u, v and compensated RK increments are the
+    # library's low-precision options
+    forms = [("Float64", "Float64", {}), ("Float32", "Float32", {}), ("Float32", "Float32", {"compensated": True}),
+             ("BFloat16", "Float32", {}), ("BFloat16", "Float32", {"scale": 64.0, "compensated": True}),
+             ("Float16", "Float32", {}), ("Float16", "Float32", {"scale": 64.0, "compensated": True})]
     P0 = workload_parameter(swm, T="Float64", nx=nx)
     G0 = swm.make_setup(P0).grid
     marks = [int(np.ceil(d * 86400 / G0.dtint)) for d in days_list]
     results = {}
-    for T, Tp in forms:
-        S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tp))
+    for T, Tp, opt in forms:
+        S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tp, **opt))
         prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
         snaps = {}
         t0 = time.perf_counter()
@@ -196,24 +200,25 @@ def precision_report(swm, nx, days_list):
                 while done < m:
                     n = min(2000, m - done)
                     d, nan = sim.step(n, 1 + done, done * S.grid.dtint)
-                    assert nan == 0 and d == n, (T, d, nan)
+                    assert nan == 0 and d == n, (T, opt, d, nan)
                     done += n
-                    print(f"# {T}/{Tp}: {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+                    print(f"# {T}/{Tp} {opt}: {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
                 g = sim.download()
                 mass, energy = sim.reduce()
                 u, v, e, _ = swm.remove_halo(g.u, g.v, g.η, g.sst, S)
                 snaps[m] = (swm.to_float64(u, Tp), swm.to_float64(v, Tp), swm.to_float64(e, Tp), mass, energy)
-        results[(T, Tp)] = snaps
-    ref = results[("Float64", "Float64")]
+        results[(T, Tp, tuple(sorted(opt.items())))] = snaps
+    ref = results[("Float64", "Float64", ())]
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     rows = []
-    for (T, Tp), snaps in results.items():
+    for (T, Tp, opt), snaps in results.items():
         if T == "Float64":
             continue
         for d, m in zip(days_list, marks):
             u, v, e, mass, energy = snaps[m]
             ur, vr, er, mr, enr = ref[m]
-            rows.append({"T": T, "Tprog": Tp, "days": d, "steps": m, "rel_l2_eta": rel(e, er), "rel_l2_u": rel(u, ur),
+            rows.append({"T": T, "Tprog": Tp, "options": dict(opt), "days": d, "steps": m, "rel_l2_eta": rel(e, er),
+                         "rel_l2_u": rel(u, ur),
                          "rel_l2_v": rel(v, vr), "rel_energy": (energy - enr) / enr, "mass": mass, "mass_f64": mr})
     for r in rows:
         print(json.dumps(r), flush=True)
@@ -357,11 +362,13 @@ def main():
     if args.form_steps > 0 and args.dtype == "Float32":
         # the reduced-precision right-hand-side forms: Tprog = Float32 state, T = BFloat16 / Float16
         out["forms"] = {}
-        for T in ("BFloat16", "Float16"):
-            Sf, el, km, _, _ = run_form(swm, T, "Float32", args.nx, args.form_steps, args.warmup, args.kernel_reps)
+        for name, T, opt in (("BFloat16", "BFloat16", {}), ("Float16", "Float16", {}),
+                             ("BFloat16+scale+compensated", "BFloat16", {"scale": 64.0, "compensated": True}),
+                             ("Float16+scale+compensated", "Float16", {"scale": 64.0, "compensated": True})):
+            Sf, el, km, _, _ = run_form(swm, T, "Float32", args.nx, args.form_steps, args.warmup, args.kernel_reps, **opt)
             b = bytes_per_cell_stage(4, 2)
             fstep = STEP_KERNELS["fused" if "fused12" in km else "unfused"]
-            out["forms"][T] = {"value": cells * args.form_steps / el, "ms_per_step": el / args.form_steps * 1e3,
+            out["forms"][name] = {"value": cells * args.form_steps / el, "ms_per_step": el / args.form_steps * 1e3,
                                "speedup_vs_f32": (elapsed / args.steps) / (el / args.form_steps),
                                "kernel_ms": km, "bytes_per_cell_step": sum(
                                    {**b, **{k: v["min"] for k, v in fused_bytes_per_cell(4, 2).items()}}[k] for k in fstep)}

@@ -73,21 +73,35 @@ __device__ __forceinline__ float round_odd_f32(double x) {
 }
 
 // Arithmetic in binary16 (K = FMT_F16) or bfloat16 (K = FMT_BF16): a float register holding
-// a value of the format, rounded to it after every operation.
+// a value of the format, rounded to it after every operation.  Binary16 + - * use the
+// native half-precision instructions (correctly rounded: the same bits as the float
+// operation rounded once, since float carries more than 2p+2 significand bits); the
+// compiler then drops the conversions between consecutive half operations.  Division and
+// sqrt go through float and one rounding.
 template <int K> struct lo {
     float v;
     __device__ __forceinline__ static float rnd(float x) { return K == FMT_F16 ? round_f16(x) : round_bf16(x); }
     __device__ __forceinline__ static lo raw(float x) { lo r; r.v = x; return r; }
+    __device__ __forceinline__ static _Float16 h(float x) { return (_Float16)x; }
     lo() = default;
     __device__ __forceinline__ lo(int i) : v(rnd((float)i)) {}
     // exact constants only (0.5, 0.25, 1.0 ...); inexact ones come pre-rounded from the host
     __device__ __forceinline__ explicit lo(double d) : v(rnd((float)d)) {}
-    __device__ __forceinline__ friend lo operator+(lo a, lo b) { return raw(rnd(a.v + b.v)); }
-    __device__ __forceinline__ friend lo operator-(lo a, lo b) { return raw(rnd(a.v - b.v)); }
-    __device__ __forceinline__ friend lo operator*(lo a, lo b) { return raw(rnd(a.v * b.v)); }
+    __device__ __forceinline__ friend lo operator+(lo a, lo b) {
+        if constexpr (K == FMT_F16) return raw((float)(h(a.v) + h(b.v)));
+        else return raw(rnd(a.v + b.v));
+    }
+    __device__ __forceinline__ friend lo operator-(lo a, lo b) {
+        if constexpr (K == FMT_F16) return raw((float)(h(a.v) - h(b.v)));
+        else return raw(rnd(a.v - b.v));
+    }
+    __device__ __forceinline__ friend lo operator*(lo a, lo b) {
+        if constexpr (K == FMT_F16) return raw((float)(h(a.v) * h(b.v)));
+        else return raw(rnd(a.v * b.v));
+    }
     __device__ __forceinline__ friend lo operator/(lo a, lo b) { return raw(rnd(a.v / b.v)); }
     __device__ __forceinline__ friend lo operator-(lo a) { return raw(-a.v); }
-    __device__ __forceinline__ lo &operator+=(lo b) { v = rnd(v + b.v); return *this; }
+    __device__ __forceinline__ lo &operator+=(lo b) { *this = *this + b; return *this; }
     __device__ __forceinline__ friend bool operator<(lo a, lo b) { return a.v < b.v; }
     __device__ __forceinline__ friend bool operator>(lo a, lo b) { return a.v > b.v; }
 };
@@ -266,8 +280,20 @@ __device__ __forceinline__ float sh_xp(float v) { return dpp_f<DPP_WAVE_SHL1>(v)
 __device__ __forceinline__ float sh_xm(float v) { return dpp_f<DPP_WAVE_SHR1>(v); }
 __device__ __forceinline__ double sh_xp(double v) { return dpp_f<DPP_WAVE_SHL1>(v); }
 __device__ __forceinline__ double sh_xm(double v) { return dpp_f<DPP_WAVE_SHR1>(v); }
-template <int K> __device__ __forceinline__ lo<K> sh_xp(lo<K> v) { return lo<K>::raw(sh_xp(v.v)); }
-template <int K> __device__ __forceinline__ lo<K> sh_xm(lo<K> v) { return lo<K>::raw(sh_xm(v.v)); }
+// binary16 values move as their 16-bit pattern (no conversion around the lane shift)
+template <int CTRL> __device__ __forceinline__ float dpp_h(float v) {
+    const int b = (int)__builtin_bit_cast(uint16_t, (_Float16)v);
+    const int r = __builtin_amdgcn_update_dpp(0, b, CTRL, 0xF, 0xF, true);
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)r);
+}
+template <int K> __device__ __forceinline__ lo<K> sh_xp(lo<K> v) {
+    if constexpr (K == FMT_F16) return lo<K>::raw(dpp_h<DPP_WAVE_SHL1>(v.v));
+    else return lo<K>::raw(sh_xp(v.v));
+}
+template <int K> __device__ __forceinline__ lo<K> sh_xm(lo<K> v) {
+    if constexpr (K == FMT_F16) return lo<K>::raw(dpp_h<DPP_WAVE_SHR1>(v.v));
+    else return lo<K>::raw(sh_xm(v.v));
+}
 
 // ------------------------------------------------------------------ static fields
 
