@@ -177,51 +177,55 @@ def run_adjoint(swm, T, nx, nsteps, snapshots, reps):
 def precision_report(swm, nx, days_list):
     """Errors of the Float32 and the Float16 / BFloat16 right-hand-side forms against the
     Float64 run of the same seeded state after each of `days_list` model days: relative L2
-    of η, u, v over the interior, relative difference of total energy, and mass."""
+    of η, u, v over the interior, relative difference of total energy, and mass.  One JSON
+    line per form and day, printed as each form finishes; a form whose run stops (NaN, a
+    departure point off the grid) prints its error instead."""
     import numpy as np
     # (T, Tprog, options): the scale of u, v and compensated RK increments are the
-    # library's low-precision options
+    # library's low-precision options.  Float16 carries s·u·h in the Uflux, which must stay
+    # below 65504: s = 2^4 keeps |u| up to 8 m/s over 500 m; BFloat16 has the Float32 range.
     forms = [("Float64", "Float64", {}), ("Float32", "Float32", {}), ("Float32", "Float32", {"compensated": True}),
              ("BFloat16", "Float32", {}), ("BFloat16", "Float32", {"scale": 64.0, "compensated": True}),
-             ("Float16", "Float32", {}), ("Float16", "Float32", {"scale": 64.0, "compensated": True})]
+             ("Float16", "Float32", {}), ("Float16", "Float32", {"scale": 16.0, "compensated": True})]
     P0 = workload_parameter(swm, T="Float64", nx=nx)
     G0 = swm.make_setup(P0).grid
     marks = [int(np.ceil(d * 86400 / G0.dtint)) for d in days_list]
-    results = {}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    ref = None
     for T, Tp, opt in forms:
         S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tp, **opt))
         prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
         snaps = {}
         t0 = time.perf_counter()
-        with swm.Simulation(S) as sim:
-            sim.upload(prog)
-            done = 0
-            for m in marks:
-                while done < m:
-                    n = min(2000, m - done)
-                    d, nan = sim.step(n, 1 + done, done * S.grid.dtint)
-                    assert nan == 0 and d == n, (T, opt, d, nan)
-                    done += n
-                    print(f"# {T}/{Tp} {opt}: {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-                g = sim.download()
-                mass, energy = sim.reduce()
-                u, v, e, _ = swm.remove_halo(g.u, g.v, g.η, g.sst, S)
-                snaps[m] = (swm.to_float64(u, Tp), swm.to_float64(v, Tp), swm.to_float64(e, Tp), mass, energy)
-        results[(T, Tp, tuple(sorted(opt.items())))] = snaps
-    ref = results[("Float64", "Float64", ())]
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    rows = []
-    for (T, Tp, opt), snaps in results.items():
+        try:
+            with swm.Simulation(S) as sim:
+                sim.upload(prog)
+                done = 0
+                for m in marks:
+                    while done < m:
+                        n = min(2000, m - done)
+                        d, nan = sim.step(n, 1 + done, done * S.grid.dtint)
+                        if nan or d != n:
+                            raise RuntimeError(f"NaN after {done + d} steps")
+                        done += n
+                        print(f"# {T}/{Tp} {opt}: {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                              flush=True)
+                    g = sim.download()
+                    mass, energy = sim.reduce()
+                    u, v, e, _ = swm.remove_halo(g.u, g.v, g.η, g.sst, S)
+                    snaps[m] = (swm.to_float64(u, Tp), swm.to_float64(v, Tp), swm.to_float64(e, Tp), mass, energy)
+        except RuntimeError as err:                     # includes the library's SwmError
+            print(json.dumps({"T": T, "Tprog": Tp, "options": opt, "error": str(err)}), flush=True)
+            continue
         if T == "Float64":
+            ref = snaps
             continue
         for d, m in zip(days_list, marks):
             u, v, e, mass, energy = snaps[m]
             ur, vr, er, mr, enr = ref[m]
-            rows.append({"T": T, "Tprog": Tp, "options": dict(opt), "days": d, "steps": m, "rel_l2_eta": rel(e, er),
-                         "rel_l2_u": rel(u, ur),
-                         "rel_l2_v": rel(v, vr), "rel_energy": (energy - enr) / enr, "mass": mass, "mass_f64": mr})
-    for r in rows:
-        print(json.dumps(r), flush=True)
+            print(json.dumps({"T": T, "Tprog": Tp, "options": opt, "days": d, "steps": m, "rel_l2_eta": rel(e, er),
+                              "rel_l2_u": rel(u, ur), "rel_l2_v": rel(v, vr), "rel_energy": (energy - enr) / enr,
+                              "mass": mass, "mass_f64": mr}), flush=True)
 
 
 def cpu_baseline(swm, orc, sample_steps: int, nx: int):

@@ -84,16 +84,40 @@ template <class F> struct StageArgs {
 
 template <class F, int M> using StageIn = std::conditional_t<Mode<M>::first, typename F::Q, typename F::ST>;
 
-// dη at T-point (x, y) from the flux divergence, continuity.jl:3-68
+// dη at T-point (x, y) from the flux divergence, continuity.jl:3-68; cs is the static of
+// the variant at the point (η_ref for the relaxation, Fη for the forcing)
 // (the divergence of fluxes of velocities carried scaled by s is divided by s)
-template <class Q, class R, class ST>
-__device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, const Fld<ST> &eref, const Fld<ST> &feta,
-                                            uint32_t vo, uint32_t so, R gamma, R Fet, int scaled, Q sinv) {
+template <class Q, class R>
+__device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, R cs, R gamma, R Fet, int scaled, Q sinv) {
     Q d = -(toQ<Q>(dUdx) + toQ<Q>(dVdy));
     if (scaled) d = d * sinv;
-    if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (eref.template ldr<R>(vo, so) - e));
-    else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * feta.template ldr<R>(vo, so));
+    if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (cs - e));
+    else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * cs);
     return d;
+}
+
+// The static operands a march takes per row, loaded a row ahead of their use like the
+// fields: f_q at corner row y, Fx at u-row y, Fy at v-row y-1, the continuity static at
+// T-row y and (periodic) q at the corner (nx, y).  A load that is waited for in the row that
+// issues it would also wait for every load issued before it (the counter is in order).
+template <class R> struct RowStatics {
+    R fq, Fx, Fy, cs, qend;
+};
+
+template <class F, bool PER>
+__device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs<F> &A, const Arena &ar, uint32_t vfq,
+                                                                 uint32_t vFx, uint32_t vFy, uint32_t vxT, int y) {
+    using R = typename F::R;
+    using ST = typename F::ST;
+    const int P = A.P;
+    RowStatics<R> s;
+    s.fq = Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
+    s.Fx = Fld<ST>{&ar, A.st.Fx}.template ldr<R>(vFx, srowb<ST>(y, P, A.st.Fx_2d));
+    s.Fy = Fld<ST>{&ar, A.st.Fy}.template ldr<R>(vFy, srowb<ST>(y - 1, P, A.st.Fy_2d));
+    s.cs = A.cont == CONT_ITSELF ? R(0)
+                                 : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
+    s.qend = PER ? Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST)) : R(0);
+    return s;
 }
 
 // ------------------------------------------------------------------ right-hand side of one row
@@ -195,8 +219,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     const Fld<Q> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
     const Fld<ST> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
-    const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy}, fqend{&ar, A.qend};
-    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const Fld<ST> fH{&ar, A.st.H};
     const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
     // column offsets and row pitches of the three storage formats (equal when T == Tprog)
     const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
@@ -230,10 +253,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         tH[k] = fH.template ldr<R>(vxT, rt);
     }
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
+    RowStatics<R> nst = row_statics<F, PER>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
     for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+        const RowStatics<R> sa = nst;
+        nst = row_statics<F, PER>(A, ar, vfq, vFx, vFy, vxT, r);
         if (r < rlast) {
             const int k = r + 1 - (t.yb - 1);                  // uniform
             if (k >= 0) {
@@ -254,16 +280,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
             if (MI::need_pr) npv = pr_v.ld(vxQ, sQ - dQ);
             if (MI::need_ac) nav = ac_v.ld(vxQ, sQ - dQ);
         }
-        const R fq_1 = fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d));
-        const R qend_1 = PER ? fqend.template ldr<R>(0, (uint32_t)(r - 1 + OY) * (uint32_t)sizeof(ST)) : R(0);
-        rhs.push(u_0, v_0, e_0, H_0, fq_1, qend_1, qfix, half, quarter, k12, A.g, A.halfp);
+        rhs.push(u_0, v_0, e_0, H_0, sa.fq, sa.qend, qfix, half, quarter, k12, A.g, A.halfp);
 
         // du at row r-1: momentum_u!  rhs.jl:242
         const int yu = r - 1;
         const bool row1 = yu >= t.ya && yu < t.yb;
         if (u_col && row1) {
-            const R Fx = fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d));
-            const Q du = (toQ<Q>(rhs.qhv) - toQ<Q>(rhs.px)) + toQ<Q>(A.Fxt * Fx);
+            const Q du = (toQ<Q>(rhs.qhv) - toQ<Q>(rhs.px)) + toQ<Q>(A.Fxt * sa.Fx);
             Q o1 = 0, o2 = 0, cc = 0;
             if (MI::comp) cc = cmp_u.ld(vxQ, sQ - dQ);
             combine<M>((Q)wu_1, MI::first ? (Q)wu_1 : pu_1, au_1, du, A.k, o1, o2, cc);
@@ -276,8 +299,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         }
         // dη at row r-1: continuity!  continuity.jl:52-98
         if (MI::rk && e_col && row1) {
-            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
-                                               A.scaled, A.sinv);
+            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.scaled, A.sinv);
             Q o1 = 0, o2 = 0, cc = 0;
             if (MI::comp) cc = cmp_e.ld(vxQ, sQ - dQ);
             combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2, cc);
@@ -291,8 +313,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         // dv at row r-2: momentum_v!  rhs.jl:271
         const int yv = r - 2;
         if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
-            const R Fy = fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d));
-            const Q dv = -(toQ<Q>(rhs.qhu) + toQ<Q>(rhs.py)) + toQ<Q>(A.Fyt * Fy);
+            const Q dv = -(toQ<Q>(rhs.qhu) + toQ<Q>(rhs.py)) + toQ<Q>(A.Fyt * sa.Fy);
             Q o1 = 0, o2 = 0, cc = 0;
             if (MI::comp) cc = cmp_v.ld(vxQ, sQ - 2 * dQ);
             combine<M>((Q)wv_2, MI::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, o1, o2, cc);
@@ -362,8 +383,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const Fld<Q> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
     const Fld<ST> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
-    const Fld<ST> fH{&ar, A.st.H}, fq{&ar, A.st.fq}, fFx{&ar, A.st.Fx}, fFy{&ar, A.st.Fy};
-    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    const Fld<ST> fH{&ar, A.st.H};
     const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
     const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
     const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
@@ -385,13 +405,24 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     R nH = fH.template ldr<R>(vxT, rowb<ST>(r, P));
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;      // A's Prog / u0 operands (rows r-1, r-2)
     Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
+    // A's statics a row ahead (f_q, Fx, η static at row r-1, Fy at r-2); B takes those A had
+    // two iterations before (rows r-3, r-4)
+    RowStatics<R> nst = row_statics<F, false>(A, ar, vfq, vFx, vFy, vxT, r - 1);
+    RowStatics<R> st_1 = {0, 0, 0, 0, 0}, st_2 = {0, 0, 0, 0, 0};
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
-    for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
+    // Two rows per trip: the row history is then in the same registers at the top of every
+    // trip, and the compiler has no copies to make for it.  An odd count of rows gets one
+    // more at the end, which stores nothing (u, η at row yb+1, v at yb) and reads rows up
+    // to yb+4 of the padded fields.
+    const int rend = rlast + ((rlast - r + 1) & 1);
+    auto row = [&](const int r, const uint32_t sI, const uint32_t sQ, const uint32_t sT) __attribute__((always_inline)) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
+        const RowStatics<R> sa = nst, sb = st_2;
+        nst = row_statics<F, false>(A, ar, vfq, vFx, vFy, vxT, r);
         if (r < rlast) {
             nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
             nH = fH.template ldr<R>(vxT, sT + dT);
@@ -404,19 +435,15 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         }
 
         // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
-        ra.push(u_0, v_0, e_0, H_0, fq.template ldr<R>(vfq, srowb<ST>(r - 1, P, A.st.fq_2d)), R(0), false, half,
-                quarter, k12, A.g, A.halfp);
+        ra.push(u_0, v_0, e_0, H_0, sa.fq, R(0), false, half, quarter, k12, A.g, A.halfp);
         Q cu1, cu0, ce1, ce0, cv1, cv0;
         {
-            const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) +
-                         toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(r - 1, P, A.st.Fx_2d)));
+            const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) + toQ<Q>(A.Fxt * sa.Fx);
             Q nocc = 0;
             combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0, nocc);
-            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
-                                               A.scaled, A.sinv);
+            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.scaled, A.sinv);
             combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0, nocc);
-            const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) +
-                         toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(r - 2, P, A.st.Fy_2d)));
+            const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) + toQ<Q>(A.Fyt * sa.Fy);
             combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
         }
         // ghost_points!(u1, v1, η1) on A's first outputs (nonperiodic): u's normal halo is 0,
@@ -441,18 +468,19 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         U0_2 = U0_1; U0_1 = U0_0; U0_0 = cu0; E0_2 = E0_1; E0_1 = E0_0; E0_0 = ce0;
         V0_2 = V0_1; V0_1 = V0_0; V0_0 = cv0;
 
-        // ---- stage B at its input row r-2
+        // ---- stage B at its input row r-2.  B pushes from the first iteration on, so that its
+        // row history is carried in the same registers on every iteration; the rows it takes
+        // before ya-1 are A's halo rows, and what they give reaches no stored output (u and η
+        // from B's input row ya+1 on, v from ya+2 on, each reading two rows of history).
         const int rbi = r - 2;
-        if (rbi >= t.ya - 1) {
+        {
             const Q ub = rbi == 0 ? oma * U1_0 : U1_1;          // row 0: (1-α)·u1 of row 1
             const Q eb = rbi == 0 ? E1_0 : E1_1;                // row 0: η1 of row 1
-            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, fq.template ldr<R>(vfq, srowb<ST>(rbi - 1, P, A.st.fq_2d)),
-                    R(0), false, half, quarter, k12, A.g, A.halfp);
+            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, R(0), false, half, quarter, k12, A.g, A.halfp);
             const int yu = r - 3;
             if (yu >= t.ya && yu < t.yb) {
                 if (u_col) {
-                    const Q du = (toQ<Q>(rb.qhv) - toQ<Q>(rb.px)) +
-                                 toQ<Q>(A.Fxt * fFx.template ldr<R>(vFx, srowb<ST>(yu, P, A.st.Fx_2d)));
+                    const Q du = (toQ<Q>(rb.qhv) - toQ<Q>(rb.px)) + toQ<Q>(A.Fxt * sb.Fx);
                     Q o1 = 0, o2 = 0, cc = 0;
                     if (MIB::comp) cc = cmp_u.ld(vxQ, sQ - 3 * dQ);
                     combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2, cc);
@@ -462,8 +490,8 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                     else o2_u.st(o2, vxQ, sQ - 3 * dQ);
                 }
                 if (e_col) {
-                    const Q deta = continuity_rhs<Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, eref, feta, vxT, sT - 3 * dT,
-                                                     A.gamma, A.Fet, A.scaled, A.sinv);
+                    const Q deta = continuity_rhs<Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, sb.cs, A.gamma, A.Fet, A.scaled,
+                                                     A.sinv);
                     Q o1 = 0, o2 = 0, cc = 0;
                     if (MIB::comp) cc = cmp_e.ld(vxQ, sQ - 3 * dQ);
                     combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2, cc);
@@ -475,8 +503,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             }
             const int yv = r - 4;
             if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
-                const Q dv = -(toQ<Q>(rb.qhu) + toQ<Q>(rb.py)) +
-                             toQ<Q>(A.Fyt * fFy.template ldr<R>(vFy, srowb<ST>(yv, P, A.st.Fy_2d)));
+                const Q dv = -(toQ<Q>(rb.qhu) + toQ<Q>(rb.py)) + toQ<Q>(A.Fyt * sb.Fy);
                 Q o1 = 0, o2 = 0, cc = 0;
                 if (MIB::comp) cc = cmp_v.ld(vxQ, sQ - 4 * dQ);
                 combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2, cc);
@@ -488,6 +515,11 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         }
         wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
         H_2 = H_1; H_1 = H_0;
+        st_2 = st_1; st_1 = sa;
+    };
+    for (; r <= rend; r += 2, sI += 2 * dI, sQ += 2 * dQ, sT += 2 * dT) {
+        row(r, sI, sQ, sT);
+        row(r + 1, sI + dI, sQ + dQ, sT + dT);
     }
 }
 
@@ -526,7 +558,8 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
     const Fld<Q> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e};
     const Fld<ST> fH{&ar, A.st.H};
     const Fld<Q> pr_e{&ar, A.pr_e}, ac_e{&ar, A.ac_e}, o1_e{&ar, A.o1_e}, o2_e{&ar, A.o2_e};
-    const Fld<ST> eref{&ar, A.st.eta_ref}, feta{&ar, A.st.Feta};
+    // the static of the continuity variant, a row ahead (row r-1 of the iteration that uses it)
+    const Fld<ST> fcs{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta};
     const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);
     const uint32_t dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     const R half = R(0.5);
@@ -538,9 +571,11 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
     uint32_t sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);
     Q nu = in_u.ld(vxQ, sQ), nv = in_v.ld(vxQ, sQ), ne = in_e.ld(vxQ, sQ), npe = 0, nae = 0;
     R nH = fH.template ldr<R>(vxT, sT);
+    R ncs = A.cont == CONT_ITSELF ? R(0) : fcs.template ldr<R>(vxT, sT - dT);
     for (; r <= rlast; ++r, sQ += dQ, sT += dT) {
         const Q we_0 = ne, pe_1 = npe, ae_1 = nae;
-        const R u_0 = toR<R>(nu), v_0 = toR<R>(nv), e_0 = toR<R>(ne), H_0 = nH;
+        const R u_0 = toR<R>(nu), v_0 = toR<R>(nv), e_0 = toR<R>(ne), H_0 = nH, cs_1 = ncs;
+        if (A.cont != CONT_ITSELF) ncs = fcs.template ldr<R>(vxT, sT);
         if (r < rlast) {
             nu = in_u.ld(vxQ, sQ + dQ); nv = in_v.ld(vxQ, sQ + dQ); ne = in_e.ld(vxQ, sQ + dQ);
             nH = fH.template ldr<R>(vxT, sT + dT);
@@ -556,8 +591,7 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
         const R Uxm_1 = sh_xm(U_1);
         const int y = r - 1;
         if (e_col && y >= t.ya && y < t.yb) {
-            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, eref, feta, vxT, sT - dT, A.gamma, A.Fet,
-                                               A.scaled, A.sinv);
+            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, cs_1, A.gamma, A.Fet, A.scaled, A.sinv);
             Q o1 = 0, o2 = 0;
             Q nocc = 0;
             combine<M>(we_1, pe_1, ae_1, deta, A.k, o1, o2, nocc);
