@@ -182,11 +182,11 @@ def precision_report(swm, nx, days_list):
     departure point off the grid) prints its error instead."""
     import numpy as np
     # (T, Tprog, options): the scale of u, v and compensated RK increments are the
-    # library's low-precision options.  Float16 carries s·u·h in the Uflux, which must stay
-    # below 65504: s = 2^4 keeps |u| up to 8 m/s over 500 m; BFloat16 has the Float32 range.
+    # library's low-precision options.  Float16 intermediates of this state overflow for
+    # s >= 8 (see scale_probe); BFloat16 has the Float32 range.
     forms = [("Float64", "Float64", {}), ("Float32", "Float32", {}), ("Float32", "Float32", {"compensated": True}),
              ("BFloat16", "Float32", {}), ("BFloat16", "Float32", {"scale": 64.0, "compensated": True}),
-             ("Float16", "Float32", {}), ("Float16", "Float32", {"scale": 16.0, "compensated": True})]
+             ("Float16", "Float32", {}), ("Float16", "Float32", {"scale": 4.0, "compensated": True})]
     P0 = workload_parameter(swm, T="Float64", nx=nx)
     G0 = swm.make_setup(P0).grid
     marks = [int(np.ceil(d * 86400 / G0.dtint)) for d in days_list]
@@ -226,6 +226,32 @@ def precision_report(swm, nx, days_list):
             print(json.dumps({"T": T, "Tprog": Tp, "options": opt, "days": d, "steps": m, "rel_l2_eta": rel(e, er),
                               "rel_l2_u": rel(u, ur), "rel_l2_v": rel(v, vr), "rel_energy": (energy - enr) / enr,
                               "mass": mass, "mass_f64": mr}), flush=True)
+
+
+def scale_probe(swm, nx):
+    """How far the 16-bit right-hand sides run at `nx`² with velocities carried scaled by s,
+    with and without compensated increments: the step count reached before the state turns
+    NaN or a tracer departure point leaves the grid (one model day at most)."""
+    day = int(round(86400 / swm.make_setup(workload_parameter(swm, nx=nx)).grid.dtint))
+    for T, scales in (("Float16", (1.0, 2.0, 4.0, 8.0, 16.0, 64.0)), ("BFloat16", (1.0, 64.0))):
+        for sc in scales:
+            for comp in (False, True):
+                S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog="Float32", scale=sc, compensated=comp))
+                prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+                done, status = 0, "ok"
+                with swm.Simulation(S) as sim:
+                    sim.upload(prog)
+                    try:
+                        for chunk in (1, 9, 90, 900, day - 1000):
+                            d, nan = sim.step(chunk, 1 + done, done * S.grid.dtint)
+                            done += d
+                            if nan or d != chunk:
+                                status = f"NaN by step {done}"
+                                break
+                    except RuntimeError as err:             # includes the library's SwmError
+                        status = f"stopped after step {done}: {err}"
+                print(json.dumps({"T": T, "Tprog": "Float32", "scale": sc, "compensated": comp, "steps_run": done,
+                                  "status": status}), flush=True)
 
 
 def cpu_baseline(swm, orc, sample_steps: int, nx: int):
@@ -293,6 +319,9 @@ def main():
     ap.add_argument("--precision-days", type=str, default="",
                     help="comma-separated model days: only run the Float32 / Float16 / BFloat16 forms against "
                          "Float64 and print their errors at those days, then exit")
+    ap.add_argument("--scale-probe", action="store_true",
+                    help="only report how far the Float16 / BFloat16 right-hand sides run with velocities scaled "
+                         "by s = 1 .. 64, then exit")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps timed for the 64-member batched ensemble (0 skips it)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=20,
@@ -316,6 +345,9 @@ def main():
     swm = load_package()
     if args.precision_days:
         precision_report(swm, args.nx, [float(d) for d in args.precision_days.split(",")])
+        return
+    if args.scale_probe:
+        scale_probe(swm, args.nx)
         return
     es = 4 if args.dtype == "Float32" else 8
     # dominant kernel: middle RK stage, HIP events on the library stream
@@ -368,7 +400,7 @@ def main():
         out["forms"] = {}
         for name, T, opt in (("BFloat16", "BFloat16", {}), ("Float16", "Float16", {}),
                              ("BFloat16+scale+compensated", "BFloat16", {"scale": 64.0, "compensated": True}),
-                             ("Float16+scale+compensated", "Float16", {"scale": 64.0, "compensated": True})):
+                             ("Float16+scale+compensated", "Float16", {"scale": 4.0, "compensated": True})):
             Sf, el, km, _, _ = run_form(swm, T, "Float32", args.nx, args.form_steps, args.warmup, args.kernel_reps, **opt)
             b = bytes_per_cell_stage(4, 2)
             fstep = STEP_KERNELS["fused" if "fused12" in km else "unfused"]

@@ -42,7 +42,7 @@ __device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
     const Fld<Q> fu{&ar, A.u0}, fv{&ar, A.v0}, fe{&ar, A.e0}, ou{&ar, A.out_u}, ov{&ar, A.out_v};
     const Fld<ST> fH{&ar, A.H};
     const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);
-    const uint32_t dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
+    const uint32_t dQ = (uint32_t)P * sizeof(Q);
     const R half = R(0.5), quarter = R(0.25);
 
     R u_1 = 0, u_2 = 0, uxm_1 = 0, v_1 = 0, v_2 = 0, h_1 = 0, h_2 = 0, sk_1 = 0, sk_2 = 0;
@@ -56,12 +56,16 @@ __device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
     Q nu = fu.ld(vxQ, sQ), nv = fv.ld(vxQ, sQ), ne = 0;   // row r+1 is loaded while row r is computed
     R nH = 0;
     if (DRAG == 0) { ne = fe.ld(vxQ, sQ); nH = fH.template ldr<R>(vxT, sT); }
-    for (; r <= rlast; ++r, sQ += dQ, sT += dT) {
+    // Two rows per trip (the row history stays in the same registers, no copies); an odd
+    // count of rows gets one more at the end, which stores nothing (row yb).
+    const int rend = rlast + ((rlast - r + 1) & 1);
+    auto row = [&](const int r, const uint32_t sQ) __attribute__((always_inline)) {
         const Q wu_0 = nu, wv_0 = nv;
         const R u_0 = toR<R>(nu), v_0 = toR<R>(nv), e_0 = toR<R>(ne), H_0 = nH;
-        if (r < rlast) {
-            nu = fu.ld(vxQ, sQ + dQ); nv = fv.ld(vxQ, sQ + dQ);
-            if (DRAG == 0) { ne = fe.ld(vxQ, sQ + dQ); nH = fH.template ldr<R>(vxT, sT + dT); }
+        {   // row r+1, clamped to the last row the march reads: a load on every row
+            const int rn = min(r + 1, rlast);
+            nu = fu.ld(vxQ, rowb<Q>(rn, P)); nv = fv.ld(vxQ, rowb<Q>(rn, P));
+            if (DRAG == 0) { ne = fe.ld(vxQ, rowb<Q>(rn, P)); nH = fH.template ldr<R>(vxT, rowb<ST>(rn, P)); }
         }
         const R uxm_0 = sh_xm(u_0);
         R h_0 = 0, sk_0 = 0;
@@ -133,6 +137,10 @@ __device__ __forceinline__ void dissipation_body(const DissArgs<F> &A, int w) {
         wu_2 = wu_1; wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0;
         DT_1 = DT_0; DSq_2 = DSq_1; DSqxm_2 = DSqxm_1; nu_2 = nu_1; Lu_2 = Lu_1; Lv_2 = Lv_1; Lvxp_2 = Lvxp_1;
         S11_2 = S11_1; S12_3 = S12_2; S22_2 = S22_1;
+    };
+    for (; r <= rend; r += 2, sQ += 2 * dQ) {
+        row(r, sQ);
+        row(r + 1, sQ + dQ);
     }
 }
 

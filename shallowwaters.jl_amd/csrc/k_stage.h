@@ -423,9 +423,10 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
         const RowStatics<R> sa = nst, sb = st_2;
         nst = row_statics<F, false>(A, ar, vfq, vFx, vFy, vxT, r);
-        if (r < rlast) {
-            nu = in_u.ld(vxI, sI + dI); nv = in_v.ld(vxI, sI + dI); ne = in_e.ld(vxI, sI + dI);
-            nH = fH.template ldr<R>(vxT, sT + dT);
+        {   // the input row r+1, clamped to the last row the march reads
+            const int rn = min(r + 1, rlast);
+            nu = in_u.ld(vxI, rowb<SI>(rn, P)); nv = in_v.ld(vxI, rowb<SI>(rn, P)); ne = in_e.ld(vxI, rowb<SI>(rn, P));
+            nH = fH.template ldr<R>(vxT, rowb<ST>(rn, P));
         }
         {   // A's pointwise operands for the next iteration: rows r (u, η) and r-1 (v)
             if (MIA::need_pr) { npu = pr_u.ld(vxQ, sQ); npe = pr_e.ld(vxQ, sQ); npv = pr_v.ld(vxQ, sQ - dQ); }
@@ -572,17 +573,20 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
     Q nu = in_u.ld(vxQ, sQ), nv = in_v.ld(vxQ, sQ), ne = in_e.ld(vxQ, sQ), npe = 0, nae = 0;
     R nH = fH.template ldr<R>(vxT, sT);
     R ncs = A.cont == CONT_ITSELF ? R(0) : fcs.template ldr<R>(vxT, sT - dT);
-    for (; r <= rlast; ++r, sQ += dQ, sT += dT) {
+    // two rows per trip, and loads on every row at rows clamped to the segment (as in the
+    // fused pairs); an odd count of rows gets one more at the end, which stores nothing
+    const int rend = rlast + ((rlast - r + 1) & 1);
+    auto row = [&](const int r, const uint32_t sQ, const uint32_t sT) __attribute__((always_inline)) {
         const Q we_0 = ne, pe_1 = npe, ae_1 = nae;
         const R u_0 = toR<R>(nu), v_0 = toR<R>(nv), e_0 = toR<R>(ne), H_0 = nH, cs_1 = ncs;
         if (A.cont != CONT_ITSELF) ncs = fcs.template ldr<R>(vxT, sT);
-        if (r < rlast) {
-            nu = in_u.ld(vxQ, sQ + dQ); nv = in_v.ld(vxQ, sQ + dQ); ne = in_e.ld(vxQ, sQ + dQ);
-            nH = fH.template ldr<R>(vxT, sT + dT);
-        }
-        if (r >= t.ya && r < t.yb) {
-            if (MI::need_pr) npe = pr_e.ld(vxQ, sQ);
-            if (MI::need_ac) nae = ac_e.ld(vxQ, sQ);
+        {
+            const int rn = min(r + 1, rlast);
+            nu = in_u.ld(vxQ, rowb<Q>(rn, P)); nv = in_v.ld(vxQ, rowb<Q>(rn, P)); ne = in_e.ld(vxQ, rowb<Q>(rn, P));
+            nH = fH.template ldr<R>(vxT, rowb<ST>(rn, P));
+            const uint32_t se = rowb<Q>(min(max(r, t.ya), t.yb - 1), P);
+            if (MI::need_pr) npe = pr_e.ld(vxQ, se);
+            if (MI::need_ac) nae = ac_e.ld(vxQ, se);
         }
         // UVfluxes!  rhs.jl:307-324 from the updated velocities
         const R h_0 = e_0 + H_0;
@@ -602,6 +606,10 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
             }
         }
         v_1 = v_0; e_1 = e_0; we_1 = we_0; h_1 = h_0; U_1 = U_0; V_2 = V_1;
+    };
+    for (; r <= rend; r += 2, sQ += 2 * dQ, sT += 2 * dT) {
+        row(r, sQ, sT);
+        row(r + 1, sQ + dQ, sT + dT);
     }
 }
 
