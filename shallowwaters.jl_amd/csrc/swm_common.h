@@ -58,7 +58,14 @@ struct h16 { uint16_t b; };  // storage of an IEEE binary16 value
 struct b16 { uint16_t b; };  // storage of a bfloat16 value
 
 __device__ __forceinline__ float round_f16(float x) { return (float)(_Float16)x; }
-__device__ __forceinline__ float round_bf16(float x) { return (float)(__bf16)x; }
+// bfloat16 rounding (RNE) of x, as a float: converting the pair (0, x) packs bf16(x) into the
+// high half over a zero low half, which is the bit pattern of the float bf16(x) itself, so
+// the conversion needs no shift back
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float round_bf16(float x) {
+    return __builtin_bit_cast(float, __builtin_convertvector((f32x2){0.0f, x}, bf16x2));
+}
 
 // float64 -> float32 rounding to odd: a second rounding to binary16 / bfloat16 is then
 // the correctly rounded result from float64
