@@ -85,14 +85,18 @@ template <class F> struct StageArgs {
 template <class F, int M> using StageIn = std::conditional_t<Mode<M>::first, typename F::Q, typename F::ST>;
 
 // dη at T-point (x, y) from the flux divergence, continuity.jl:3-68; cs is the static of
-// the variant at the point (η_ref for the relaxation, Fη for the forcing)
-// (the divergence of fluxes of velocities carried scaled by s is divided by s)
-template <class Q, class R>
-__device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, R cs, R gamma, R Fet, int scaled, Q sinv) {
+// the variant at the point (η_ref for the relaxation, Fη for the forcing).  The divergence
+// of fluxes of velocities carried scaled by s is divided by s (sinv = 1/s, exactly 1 when
+// unscaled, which changes no value).  CS: the variant may have a static term (cont is then
+// read at run time); without it the code has no branch.
+template <bool CS, class Q, class R>
+__device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, R cs, R gamma, R Fet, Q sinv) {
     Q d = -(toQ<Q>(dUdx) + toQ<Q>(dVdy));
-    if (scaled) d = d * sinv;
-    if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (cs - e));
-    else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * cs);
+    d = d * sinv;
+    if constexpr (CS) {
+        if (cont == CONT_RELAX) d = d + toQ<Q>(gamma * (cs - e));
+        else if (cont == CONT_FORCING) d = d + toQ<Q>(Fet * cs);
+    }
     return d;
 }
 
@@ -104,7 +108,7 @@ template <class R> struct RowStatics {
     R fq, Fx, Fy, cs, qend;
 };
 
-template <class F, bool PER>
+template <class F, bool PER, bool CS>
 __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs<F> &A, const Arena &ar, uint32_t vfq,
                                                                  uint32_t vFx, uint32_t vFy, uint32_t vxT, int y) {
     using R = typename F::R;
@@ -114,8 +118,9 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
     s.fq = Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
     s.Fx = Fld<ST>{&ar, A.st.Fx}.template ldr<R>(vFx, srowb<ST>(y, P, A.st.Fx_2d));
     s.Fy = Fld<ST>{&ar, A.st.Fy}.template ldr<R>(vFy, srowb<ST>(y - 1, P, A.st.Fy_2d));
-    s.cs = A.cont == CONT_ITSELF ? R(0)
-                                 : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
+    s.cs = (!CS || A.cont == CONT_ITSELF)
+               ? R(0)
+               : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
     s.qend = PER ? Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST)) : R(0);
     return s;
 }
@@ -127,25 +132,29 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
 // du and dη at row r-1 and of dv at row r-2 (qhv, the p difference ... ), with the rows
 // each stencil needs kept as history.  Arithmetic in T, term by term as the reference.
 template <class R, bool PER> struct RhsPipe {
-    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hx_1 = 0;
-    R U_1 = 0, U_2 = 0, Uxm_2 = 0, V_2 = 0, Vxp_2 = 0;
-    R q_2 = 0, qxp_2 = 0, qxm_2 = 0, qaxm_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
+    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
+    R U_1 = 0, U_2 = 0, V_2 = 0;
+    R q_2 = 0, qaU_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
     // results of the last push
     R qhv, px;            // row r-1: Arakawa-Hsu qhv and p(x+1) - p(x)  (momentum_u!)
     R dUdx, dVdy, e_r1;   // row r-1: flux divergence terms and η  (continuity!)
     R qhu, py;            // row r-2: Arakawa-Hsu qhu and p(y+1) - p(y)  (momentum_v!)
 
+    // Every x-neighbour is taken where it is consumed, as the shifted operand of one add,
+    // subtract or multiply (a lane move the compiler folds into that instruction), and the
+    // products of two shifted factors are formed unshifted and moved once:
+    // q(x-1)·U(x-1) is the shift of q·U.  The values are those of the formulas as written.
     // qend_1: q at the corner (nx, r-1), read by the wrapped first column when qfix (periodic x = 0)
     __device__ __forceinline__ void push(R u_0, R v_0, R e_0, R H_0, R fq_1, R qend_1, bool qfix, R half, R quarter,
                                          R k12, R g, R halfp) {
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
-        const R hx_0 = sh_xp(h_0);
-        const R U_0 = u_0 * (half * (hx_0 + h_0));
+        const R hs_0 = sh_xp(h_0) + h_0;                   // h(x+1) + h, read by Ix!(h_u) and Ixy!(h_q)
+        const R U_0 = u_0 * (half * hs_0);
         // row r-1: Iy!(h_v), Vflux!  rhs.jl:294-304
         const R V_1 = v_1 * (half * (h_0 + h_1));
         // Ixy!(h_q,h)  interpolations.jl:33-45
-        const R hq_1 = quarter * (h_1 + hx_1) + quarter * (h_0 + hx_0);
+        const R hq_1 = quarter * hs_1 + quarter * hs_0;
         // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
         const R dvdx_1 = sh_xp(v_1) - v_1;
         const R dudy_1 = u_0 - u_1;
@@ -155,34 +164,33 @@ template <class R, bool PER> struct RhsPipe {
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
         const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
         const R p_1 = halfp * (KEu_1 + KEv_1) + g * e_1;
-        // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0)
-        const R qxp_1 = sh_xp(q_1);
-        R qxm_1 = sh_xm(q_1);
-        if (PER && qfix) qxm_1 = qend_1;
-        const R qa_1 = k12 * (q_2 + q_1 + qxp_1);
-        const R qb_1 = k12 * (q_2 + q_1 + qxm_1);
+        // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0).
+        // The x-1 neighbours are moved before the per-lane choice of the wrapped column: a
+        // lane move inside a branch taken by some lanes only would read 0 from the others.
+        const R s_1 = q_2 + q_1;
+        const R qxm_1 = sh_xm(q_1), qxm_2 = sh_xm(q_2);
+        const R qa_1 = k12 * (s_1 + sh_xp(q_1));
+        const R qb_1 = k12 * (s_1 + ((PER && qfix) ? qend_1 : qxm_1));
         // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
         // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
-        const R qg_1 = (PER && qfix) ? k12 * (q_2 + q_1 + qend_2) : k12 * (q_2 + qxm_2 + q_1);
-        const R qd_1 = k12 * (q_2 + q_1 + qxp_2);
+        const R qg_1 = (PER && qfix) ? k12 * (s_1 + qend_2) : k12 * ((q_2 + qxm_2) + q_1);
+        const R qd_1 = k12 * (s_1 + sh_xp(q_2));
         // PV_ArakawaHsu! qhv  PV_advection.jl:73 and ∂x!(dpdx)  at row r-1
-        const R Vxp_1 = sh_xp(V_1);
-        qhv = qa_1 * Vxp_1 + qb_1 * V_1 + qg_1 * V_2 + qd_1 * Vxp_2;
+        qhv = qa_1 * sh_xp(V_1) + qb_1 * V_1 + qg_1 * V_2 + qd_1 * sh_xp(V_2);
         px = sh_xp(p_1) - p_1;
         // ∂x!(dUdx,U), ∂y!(dVdy,V)  continuity.jl:52-68 at row r-1
-        const R Uxm_1 = sh_xm(U_1);
-        dUdx = U_1 - Uxm_1;
+        dUdx = U_1 - sh_xm(U_1);
         dVdy = V_1 - V_2;
         e_r1 = e_1;
-        // PV_ArakawaHsu! qhu  PV_advection.jl:81 and ∂y!(dpdy)  at row r-2
-        const R qdxm_1 = sh_xm(qd_1);
-        qhu = qaxm_2 * Uxm_2 + qb_2 * U_2 + qg_1 * U_1 + qdxm_1 * Uxm_1;
+        // PV_ArakawaHsu! qhu  PV_advection.jl:81 and ∂y!(dpdy)  at row r-2:
+        // qα(x-1)·U(x-1) of row r-2 and qδ(x-1)·U(x-1) of row r-1
+        const R qaU_1 = qa_1 * U_1;
+        qhu = sh_xm(qaU_2) + qb_2 * U_2 + qg_1 * U_1 + sh_xm(qd_1 * U_1);
         py = p_1 - p_2;
         // advance the row history
-        const R qaxm_1 = sh_xm(qa_1);
-        u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hx_1 = hx_0;
-        U_2 = U_1; U_1 = U_0; Uxm_2 = Uxm_1; V_2 = V_1; Vxp_2 = Vxp_1;
-        q_2 = q_1; qxp_2 = qxp_1; qxm_2 = qxm_1; qaxm_2 = qaxm_1; qb_2 = qb_1; p_2 = p_1;
+        u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hs_1 = hs_0;
+        U_2 = U_1; U_1 = U_0; V_2 = V_1;
+        q_2 = q_1; qaU_2 = qaU_1; qb_2 = qb_1; p_2 = p_1;
         qend_2 = qend_1;
     }
 };
@@ -253,13 +261,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         tH[k] = fH.template ldr<R>(vxT, rt);
     }
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
-    RowStatics<R> nst = row_statics<F, PER>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
+    RowStatics<R> nst = row_statics<F, PER, true>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
     for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const RowStatics<R> sa = nst;
-        nst = row_statics<F, PER>(A, ar, vfq, vFx, vFy, vxT, r);
+        nst = row_statics<F, PER, true>(A, ar, vfq, vFx, vFy, vxT, r);
         if (r < rlast) {
             const int k = r + 1 - (t.yb - 1);                  // uniform
             if (k >= 0) {
@@ -299,7 +307,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         }
         // dη at row r-1: continuity!  continuity.jl:52-98
         if (MI::rk && e_col && row1) {
-            const Q deta = continuity_rhs<Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.scaled, A.sinv);
+            const Q deta = continuity_rhs<true, Q>(rhs.dUdx, rhs.dVdy, rhs.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.sinv);
             Q o1 = 0, o2 = 0, cc = 0;
             if (MI::comp) cc = cmp_e.ld(vxQ, sQ - dQ);
             combine<M>((Q)we_1, MI::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, o1, o2, cc);
@@ -352,10 +360,12 @@ __global__ void __launch_bounds__(256) stage_batch(const StageArgs<F> *__restric
 // output (u0) waits in registers for B's combination.  Only B's outputs are stored, so the
 // intermediate stage costs no HBM traffic.  Strips carry an x-halo of 4 lanes (A is valid on
 // lanes 2..61, B on 4..59).  Results are those of the two stage kernels, bit for bit.
+// CS: the continuity variant has a static term (relaxation or forcing); the plain variant
+// runs without the branches that would split the march's body.
 constexpr int HALO2 = 4;
 constexpr int WOUT2 = WAVE - 2 * HALO2;
 
-template <class F, int MA, int MB>
+template <class F, int MA, int MB, bool CS>
 __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
@@ -407,7 +417,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
     // A's statics a row ahead (f_q, Fx, η static at row r-1, Fy at r-2); B takes those A had
     // two iterations before (rows r-3, r-4)
-    RowStatics<R> nst = row_statics<F, false>(A, ar, vfq, vFx, vFy, vxT, r - 1);
+    RowStatics<R> nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r - 1);
     RowStatics<R> st_1 = {0, 0, 0, 0, 0}, st_2 = {0, 0, 0, 0, 0};
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
@@ -422,7 +432,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
         const RowStatics<R> sa = nst, sb = st_2;
-        nst = row_statics<F, false>(A, ar, vfq, vFx, vFy, vxT, r);
+        nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r);
         {   // the input row r+1, clamped to the last row the march reads
             const int rn = min(r + 1, rlast);
             nu = in_u.ld(vxI, rowb<SI>(rn, P)); nv = in_v.ld(vxI, rowb<SI>(rn, P)); ne = in_e.ld(vxI, rowb<SI>(rn, P));
@@ -442,7 +452,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) + toQ<Q>(A.Fxt * sa.Fx);
             Q nocc = 0;
             combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0, nocc);
-            const Q deta = continuity_rhs<Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.scaled, A.sinv);
+            const Q deta = continuity_rhs<CS, Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.sinv);
             combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0, nocc);
             const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) + toQ<Q>(A.Fyt * sa.Fy);
             combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
@@ -491,8 +501,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                     else o2_u.st(o2, vxQ, sQ - 3 * dQ);
                 }
                 if (e_col) {
-                    const Q deta = continuity_rhs<Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, sb.cs, A.gamma, A.Fet, A.scaled,
-                                                     A.sinv);
+                    const Q deta = continuity_rhs<CS, Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, sb.cs, A.gamma, A.Fet, A.sinv);
                     Q o1 = 0, o2 = 0, cc = 0;
                     if (MIB::comp) cc = cmp_e.ld(vxQ, sQ - 3 * dQ);
                     combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2, cc);
@@ -524,18 +533,18 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     }
 }
 
-template <class F, int MA, int MB>
+template <class F, int MA, int MB, bool CS>
 __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
-    fused_body<F, MA, MB>(A, wave_index());
+    fused_body<F, MA, MB, CS>(A, wave_index());
 }
 
-template <class F, int MA, int MB>
+template <class F, int MA, int MB, bool CS>
 __global__ void __launch_bounds__(256) fused_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start,
                                                    int nmem) {
     int m, wl;
     if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
     const StageArgs<F> A = args[m];
-    fused_body<F, MA, MB>(A, wl);
+    fused_body<F, MA, MB, CS>(A, wl);
 }
 
 // ------------------------------------------------------------------ semi-implicit continuity
@@ -595,7 +604,7 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
         const R Uxm_1 = sh_xm(U_1);
         const int y = r - 1;
         if (e_col && y >= t.ya && y < t.yb) {
-            const Q deta = continuity_rhs<Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, cs_1, A.gamma, A.Fet, A.scaled, A.sinv);
+            const Q deta = continuity_rhs<true, Q>(U_1 - Uxm_1, V_1 - V_2, e_1, A.cont, cs_1, A.gamma, A.Fet, A.sinv);
             Q o1 = 0, o2 = 0;
             Q nocc = 0;
             combine<M>(we_1, pe_1, ae_1, deta, A.k, o1, o2, nocc);

@@ -295,7 +295,8 @@ template <class F> struct Launch {
     template <bool PER> static void segment_lengths(swm_ctx *ctx) {
         ctx->L = rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>), nstrips(ctx), 6, 4);
         ctx->Ld = rows_per_wave(ctx, reinterpret_cast<const void *>(&dissipation_kernel<F, PER, 0, true>), nstrips(ctx), 2, 8);
-        ctx->Lf = rows_per_wave(ctx, reinterpret_cast<const void *>(&fused_kernel<F, M_RK_MID, M_RK_LAST>), nstrips2(ctx), 5, 8);
+        ctx->Lf = rows_per_wave(ctx, reinterpret_cast<const void *>(&fused_kernel<F, M_RK_MID, M_RK_LAST, false>),
+                                nstrips2(ctx), 5, 8);
     }
 
     static StaticRefs statics(const swm_ctx *ctx) {
@@ -441,13 +442,18 @@ template <class F> struct Launch {
             return SWM_OK;
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-        if (ma == M_RK_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
-        else if (ma == M_RK_MID && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RK_LAST>), grid, block, 0, ctx->stream, A);
-        else if (ma == M_RK_FIRST && mb == M_RK_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_FIRST, M_RK_LAST>), grid, block, 0, ctx->stream, A);
-        else if (ma == M_RKC_FIRST && mb == M_RK_MID) hipLaunchKernelGGL((fused_kernel<F, M_RKC_FIRST, M_RK_MID>), grid, block, 0, ctx->stream, A);
-        else if (ma == M_RK_MID && mb == M_RKC_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RK_MID, M_RKC_LAST>), grid, block, 0, ctx->stream, A);
-        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) hipLaunchKernelGGL((fused_kernel<F, M_RKC_FIRST, M_RKC_LAST>), grid, block, 0, ctx->stream, A);
+        const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
+#define SWM_FK(MA_, MB_)                                                                                   \
+    if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true>), grid, block, 0, ctx->stream, A);         \
+    else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false>), grid, block, 0, ctx->stream, A)
+        if (ma == M_RK_FIRST && mb == M_RK_MID) SWM_FK(M_RK_FIRST, M_RK_MID);
+        else if (ma == M_RK_MID && mb == M_RK_LAST) SWM_FK(M_RK_MID, M_RK_LAST);
+        else if (ma == M_RK_FIRST && mb == M_RK_LAST) SWM_FK(M_RK_FIRST, M_RK_LAST);
+        else if (ma == M_RKC_FIRST && mb == M_RK_MID) SWM_FK(M_RKC_FIRST, M_RK_MID);
+        else if (ma == M_RK_MID && mb == M_RKC_LAST) SWM_FK(M_RK_MID, M_RKC_LAST);
+        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) SWM_FK(M_RKC_FIRST, M_RKC_LAST);
         else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
+#undef SWM_FK
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
@@ -1506,13 +1512,15 @@ template <class F> struct Ens {
                 }
             } else if (pl.kind[j] == 1) {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID)
-                    hipLaunchKernelGGL((fused_batch<F, M_RK_FIRST, M_RK_MID>), grid, block, 0, E->stream, a, tab, nm);
-                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST)
-                    hipLaunchKernelGGL((fused_batch<F, M_RK_MID, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm);
-                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST)
-                    hipLaunchKernelGGL((fused_batch<F, M_RK_FIRST, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm);
+                const bool cs = c0.continuity != SWM_CONT_ITSELF;
+#define SWM_FB(MA_, MB_)                                                                                        \
+    if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true>), grid, block, 0, E->stream, a, tab, nm);        \
+    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false>), grid, block, 0, E->stream, a, tab, nm)
+                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) SWM_FB(M_RK_FIRST, M_RK_MID);
+                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_MID, M_RK_LAST);
+                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_FIRST, M_RK_LAST);
                 else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
+#undef SWM_FB
             } else if (pl.kind[j] == 2) {
                 const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
                 const bool smag = c0.diffusion == SWM_DIFF_SMAGORINSKY;
