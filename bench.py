@@ -228,6 +228,48 @@ def precision_report(swm, nx, days_list):
                               "mass": mass, "mass_f64": mr}), flush=True)
 
 
+def cpu_protocol(swm, orc, config2_cpu_steps: int = 100):
+    """The CPU-beside-GPU table: the oracle (one thread) and the GPU on the same setups, one
+    JSON line per row.  4096^2 workload: 5 timed steps after one warm-up step, Float32 and
+    Float64; the README's 100x100 double gyre over 100 days (30,639 steps from rest, no
+    tracer); the 2048x1024 periodic channel (tracer every 300 steps, seeded state),
+    `config2_cpu_steps` steps on the CPU and 1000 on the GPU."""
+    def row(name, S, prog, steps, cpu_steps, warm=1):
+        cfg = swm.make_config(S)
+        G = S.grid
+        out = {"row": name, "nx": G.nx, "ny": G.ny, "T": S.parameters.T, "steps_gpu": steps, "steps_cpu": cpu_steps}
+        if cpu_steps > 0:
+            if warm:
+                orc.run(S, cfg, prog, warm)
+            t = time.perf_counter()
+            orc.run(S, cfg, prog, cpu_steps)
+            dt = time.perf_counter() - t
+            out.update(cpu_cell_updates_per_s=G.nx * G.ny * cpu_steps / dt, cpu_seconds=dt, cpu_cores=1)
+        with swm.Simulation(S) as sim:
+            sim.upload(prog)
+            sim.step(min(warm * 10, steps), 1, 0)
+            sim.upload(prog)
+            t = time.perf_counter()
+            d, nan = sim.step(steps, 1, 0)
+            dt = time.perf_counter() - t
+        out.update(gpu_cell_updates_per_s=G.nx * G.ny * steps / dt, gpu_seconds=dt, gpu_steps_done=d, nan=nan)
+        print(json.dumps(out), flush=True)
+
+    for T in ("Float32", "Float64"):
+        S = swm.make_setup(workload_parameter(swm, T=T))
+        row(f"double_gyre_4096x4096 {T}", S, swm.seeded_initial_conditions(S, seed=2024, σ=0.1), 200, 5)
+    P1 = swm.Parameter(T="Float32", nx=100, L_ratio=1, Lx=2000e3, H=500, g=10, bc="nonperiodic", α=0.0,
+                       topography="seamount", wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky",
+                       bottom_drag="quadratic", tracer_advection=False, Ndays=100, cfl=1.0)
+    S1 = swm.make_setup(P1)
+    row("readme_double_gyre_100x100 100 days", S1, swm.initial_conditions("Float32", S1), S1.grid.nt, S1.grid.nt)
+    P2 = swm.Parameter(T="Float32", nx=2048, L_ratio=2, bc="periodic", topography="flat", wind_forcing_x="channel",
+                       Fx0=0.12)
+    S2 = swm.make_setup(P2)
+    row("channel_2048x1024 periodic", S2, swm.seeded_initial_conditions(S2, seed=2024, σ=0.1), 1000,
+        config2_cpu_steps)
+
+
 def scale_probe(swm, nx):
     """How far the 16-bit right-hand sides run at `nx`² with velocities carried scaled by s,
     with and without compensated increments: the step count reached before the state turns
@@ -319,6 +361,10 @@ def main():
     ap.add_argument("--precision-days", type=str, default="",
                     help="comma-separated model days: only run the Float32 / Float16 / BFloat16 forms against "
                          "Float64 and print their errors at those days, then exit")
+    ap.add_argument("--cpu-protocol", action="store_true",
+                    help="only print the CPU-beside-GPU rows (oracle on one thread and the GPU on the 4096^2 "
+                         "workload in Float32 and Float64, the README's 100x100 100-day run, the 2048x1024 "
+                         "periodic channel), then exit")
     ap.add_argument("--scale-probe", action="store_true",
                     help="only report how far the Float16 / BFloat16 right-hand sides run with velocities scaled "
                          "by s = 1 .. 64, then exit")
@@ -326,6 +372,9 @@ def main():
                     help="steps timed for the 64-member batched ensemble (0 skips it)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=20,
                     help="steps of each ensemble member run on its own, for the unbatched comparison")
+    ap.add_argument("--adjoint-nx", type=str, default="1024,4096",
+                    help="grid sizes (comma-separated) of the reverse-mode measurement: the adjoint/forward "
+                         "ratio is quoted at 1024^2 (200 steps, revolve), 4096^2 is the forward workload")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the revolve pullback measurement (0 skips the reverse-mode line)")
     ap.add_argument("--adjoint-snapshots", type=int, default=20)
@@ -348,6 +397,9 @@ def main():
         return
     if args.scale_probe:
         scale_probe(swm, args.nx)
+        return
+    if args.cpu_protocol:
+        cpu_protocol(swm, load_oracle())
         return
     es = 4 if args.dtype == "Float32" else 8
     # dominant kernel: middle RK stage, HIP events on the library stream
@@ -411,8 +463,9 @@ def main():
     if args.ensemble_steps > 0 and args.dtype == "Float32":
         out["ensemble"] = run_ensemble(swm, 64, args.ensemble_steps, args.ensemble_seq_steps)
     if args.adjoint_steps > 0:
-        out["adjoint"] = run_adjoint(swm, args.dtype, args.nx, args.adjoint_steps, args.adjoint_snapshots,
-                                     max(5, args.kernel_reps // 2))
+        out["adjoint"] = {f"{n}x{n}": run_adjoint(swm, args.dtype, n, args.adjoint_steps, args.adjoint_snapshots,
+                                                  max(5, args.kernel_reps // 2))
+                          for n in (int(a) for a in args.adjoint_nx.split(","))}
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(swm, load_oracle(), args.cpu_steps, args.nx)
     print(json.dumps(out), flush=True)

@@ -115,12 +115,11 @@ template <class R> struct DissAdjArgs {
 // 2, ny, ny-1.  u (periodic): columns -2, -1, nx, nx+1 copy columns nx-2, nx-1, 0, 1.  v:
 // columns 0, -1, nx+1, nx+2 copy (1-α)·v of columns 1, 2, nx, nx-1 (nonperiodic) or columns
 // nx, nx-1, 1, 2 (periodic).  η: the halo ring copies its nearest interior cell (periodic:
-// wrapped in x).  Tcomm = Tprog: the periodic copies are exact.
+// wrapped in x).  Tcomm = Tprog: the periodic copies are exact.  `l` is the cell's own value
+// (row y of the field at o), loaded by the caller ahead of the fold.
 
 template <class W, bool PER>
-SWM_HD typename W::V fold_u(const W &w, uint32_t o, int y, typename W::R oma, int nx, int ny) {
-    using V = typename W::V;
-    V l = w.ld(o, y);
+SWM_HD typename W::V fold_u(const W &w, uint32_t o, int y, typename W::V l, typename W::R oma, int nx, int ny) {
     if (PER) {
         const auto mlo = (w.x >= nx - 2) & (w.x <= nx - 1);
         const auto mhi = (w.x >= 0) & (w.x <= 1);
@@ -136,9 +135,7 @@ SWM_HD typename W::V fold_u(const W &w, uint32_t o, int y, typename W::R oma, in
 }
 
 template <class W, bool PER>
-SWM_HD typename W::V fold_v(const W &w, uint32_t o, int y, typename W::R oma, int nx) {
-    using V = typename W::V;
-    V l = w.ld(o, y);
+SWM_HD typename W::V fold_v(const W &w, uint32_t o, int y, typename W::V l, typename W::R oma, int nx) {
     const auto m1 = w.x == 1, m2 = w.x == 2, m3 = w.x == nx, m4 = w.x == nx - 1;
     if (PER) {
         const auto mlo = m3 | m4, mhi = m1 | m2;      // x - nx in {0, -1}; x + nx in {nx+1, nx+2}
@@ -153,9 +150,8 @@ SWM_HD typename W::V fold_v(const W &w, uint32_t o, int y, typename W::R oma, in
 }
 
 template <class W, bool PER>
-SWM_HD typename W::V fold_e(const W &w, uint32_t o, int y, int nx, int ny) {
+SWM_HD typename W::V fold_e(const W &w, uint32_t o, int y, typename W::V l, int nx, int ny) {
     using V = typename W::V;
-    V l = w.ld(o, y);
     const auto mx0 = w.x == (PER ? nx : 1), mx1 = w.x == (PER ? 1 : nx);
     const auto mx = mx0 | mx1;
     const auto xa = W::seli(mx0, 0, nx + 1);
@@ -167,6 +163,40 @@ SWM_HD typename W::V fold_e(const W &w, uint32_t o, int y, int nx, int ny) {
         l = l + c;
     }
     return l;
+}
+
+// ------------------------------------------------------------------ loads a row ahead
+//
+// The marches issue the loads of their next row while the current row is computed: a load
+// waited for in the row that issues it would also wait for every load issued before it.
+// Each row is clamped to the rows the march reads, so every row issues the same loads.
+
+template <class V> struct StageLoads {
+    V u, v, e, H, fq, qe;      // stage input at row r; f_q and q(nx) at corner row r-1
+    V lu, lv, le, mu, mv, me;  // cells of λ0 and μ_{k+1} at row r-1 (the folds add their halo)
+    V au, av, ae;              // running sum A at the row emitted, r-2
+};
+
+template <class W, bool PER, bool NEXT, bool A_L0>
+SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<typename W::R> &A, int r, int ya, int yb) {
+    using V = typename W::V;
+    using R = typename W::R;
+    const V zero = V(R(0));
+    StageLoads<V> p;
+    const int rs = amin(r, yb + 1), ny = A.ny;
+    p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs); p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs);
+    p.fq = w.ld_s(A.fq, rs - 1, A.fq_2d);
+    p.qe = PER ? w.ld_row1(A.qend, rs - 1) : zero;
+    const int yu = amin(amax(rs - 1, 1), ny), yv = amin(amax(rs - 1, 1), amax(ny - 1, 1));
+    p.lu = w.ld(A.l0.u, yu); p.lv = w.ld(A.l0.v, yv); p.le = w.ld(A.l0.e, yu);
+    p.mu = p.mv = p.me = zero;
+    if (NEXT) { p.mu = w.ld(A.mu_in.u, yu); p.mv = w.ld(A.mu_in.v, yv); p.me = w.ld(A.mu_in.e, yu); }
+    p.au = p.av = p.ae = zero;
+    if (NEXT && !A_L0) {
+        const int ye = amin(amax(rs - 2, ya), yb - 1);
+        p.au = w.ld(A.a_in.u, ye); p.av = w.ld(A.a_in.v, ye); p.ae = w.ld(A.a_in.e, ye);
+    }
+    return p;
 }
 
 // ------------------------------------------------------------------ stage adjoint
@@ -221,26 +251,29 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     V su_2 = zero, sv_2 = zero, se_2 = zero;           // λs_{k+1} of row r-2
     V zu_2 = zero, zv_2 = zero, ze_2 = zero;           // folded λ0 of row r-2 (A_L0)
 
+    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0>(w, A, ya - 2, ya, yb);
     for (int r = ya - 2; r <= yb + 1; ++r) {
+        const StageLoads<V> c = nl;
+        nl = stage_loads<W, PER, NEXT, A_L0>(w, A, r + 1, ya, yb);
         // ---- stage input row r
-        const V u_0 = w.ld(A.s.u, r), v_0 = w.ld(A.s.v, r), e_0 = w.ld(A.s.e, r), H_0 = w.ld(A.H, r);
-        const V fq_1 = w.ld_s(A.fq, r - 1, A.fq_2d);
-        const V qend_1 = PER ? w.ld_row1(A.qend, r - 1) : zero;
+        const V u_0 = c.u, v_0 = c.v, e_0 = c.e, H_0 = c.H;
+        const V fq_1 = c.fq;
+        const V qend_1 = c.qe;
         // ---- tendency adjoint row r-1
         const int yd = r - 1;
         const bool ydu = yd >= 1 && yd <= ny, ydv = yd >= 1 && yd <= ny - 1;
         V zu = zero, zv = zero, ze = zero, su = zero, sv = zero, se = zero;
         if (ydu) {
-            zu = fold_u<W, PER>(w, A.l0.u, yd, A.oma, nx, ny);
-            ze = fold_e<W, PER>(w, A.l0.e, yd, nx, ny);
+            zu = fold_u<W, PER>(w, A.l0.u, yd, c.lu, A.oma, nx, ny);
+            ze = fold_e<W, PER>(w, A.l0.e, yd, c.le, nx, ny);
             if (NEXT) {
-                su = fold_u<W, PER>(w, A.mu_in.u, yd, A.oma, nx, ny);
-                se = fold_e<W, PER>(w, A.mu_in.e, yd, nx, ny);
+                su = fold_u<W, PER>(w, A.mu_in.u, yd, c.mu, A.oma, nx, ny);
+                se = fold_e<W, PER>(w, A.mu_in.e, yd, c.me, nx, ny);
             }
         }
         if (ydv) {
-            zv = fold_v<W, PER>(w, A.l0.v, yd, A.oma, nx);
-            if (NEXT) sv = fold_v<W, PER>(w, A.mu_in.v, yd, A.oma, nx);
+            zv = fold_v<W, PER>(w, A.l0.v, yd, c.lv, A.oma, nx);
+            if (NEXT) sv = fold_v<W, PER>(w, A.mu_in.v, yd, c.mv, A.oma, nx);
         }
         const V gu_1 = W::sel(ucol & ydu, NEXT ? A.a * zu + A.b * su : A.a * zu, zero);
         const V gv_1 = W::sel(vcol & ydv, NEXT ? A.a * zv + A.b * sv : A.a * zv, zero);
@@ -303,9 +336,9 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             const bool yv_st = yv_in, ye_st = y >= 0 && y <= ny + 1;
             const M iu = ucol & yu_in, iv = vcol & yv_in, ie = vcol & yu_in;
             if (NEXT) {
-                const V au = A_L0 ? zu_2 : w.ld(A.a_in.u, y);
-                const V av = A_L0 ? zv_2 : w.ld(A.a_in.v, y);
-                const V ae = A_L0 ? ze_2 : w.ld(A.a_in.e, y);
+                const V au = A_L0 ? zu_2 : c.au;
+                const V av = A_L0 ? zv_2 : c.av;
+                const V ae = A_L0 ? ze_2 : c.ae;
                 const V Au = au + su_2, Av = av + sv_2, Ae = ae + se_2;     // A_k on interior cells
                 if (PRE) {
                     w.st(A.a_out.u, y, W::sel(iu, Au + lu_2, lu_2), st_u & yu_st);
@@ -422,16 +455,34 @@ SWM_HD void diss_adjoint(W &w, const DissAdjArgs<typename W::R> &A, int ya, int 
     V lLu_3 = zero, lLu_4 = zero, lLv_3 = zero, lLv_4 = zero;
     V lKE_3 = zero, lskv_3 = zero, lhu_3 = zero, lhv_3 = zero, lhv_4 = zero, bdu_3 = zero, bdv_3 = zero;
 
+    // loads a row ahead (rows clamped to the rows the march reads): the state at row r, the
+    // cells of λ after the dissipation at row r-1 and η's at the row emitted, r-3
+    struct Loads {
+        V u, v, e, H, lu, lv, le;
+    };
+    auto loads = [&](int r) {
+        Loads p;
+        const int rs = amin(r, yb + 2);
+        p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs);
+        p.e = p.H = zero;
+        if (DRAG == 0) { p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs); }
+        p.lu = w.ld(A.lin.u, amin(amax(rs - 1, 1), ny));
+        p.lv = w.ld(A.lin.v, amin(amax(rs - 1, 1), amax(ny - 1, 1)));
+        p.le = w.ld(A.lin.e, amin(amax(rs - 3, ya), yb - 1));
+        return p;
+    };
+    Loads nl = loads(ya - 3);
     for (int r = ya - 3; r <= yb + 2; ++r) {
+        const Loads c = nl;
+        nl = loads(r + 1);
         // ---- state row r
-        const V u_0 = w.ld(A.s.u, r), v_0 = w.ld(A.s.v, r);
-        V e_0 = zero, H_0 = zero;
-        if (DRAG == 0) { e_0 = w.ld(A.s.e, r); H_0 = w.ld(A.H, r); }
+        const V u_0 = c.u, v_0 = c.v;
+        const V e_0 = c.e, H_0 = c.H;
         // ---- adjoint after the dissipation, row r-1 (interior increments only)
         const int yi = r - 1;
         const bool yiu = yi >= 1 && yi <= ny, yiv = yi >= 1 && yi <= ny - 1;
-        const V lin_u_1 = yiu ? W::sel(ucol, fold_u<W, PER>(w, A.lin.u, yi, A.oma, nx, ny), zero) : zero;
-        const V lin_v_1 = yiv ? W::sel(vcol, fold_v<W, PER>(w, A.lin.v, yi, A.oma, nx), zero) : zero;
+        const V lin_u_1 = yiu ? W::sel(ucol, fold_u<W, PER>(w, A.lin.u, yi, c.lu, A.oma, nx, ny), zero) : zero;
+        const V lin_v_1 = yiv ? W::sel(vcol, fold_v<W, PER>(w, A.lin.v, yi, c.lv, A.oma, nx), zero) : zero;
         const V gu_1 = dtd * lin_u_1, gv_1 = dtd * lin_v_1;
 
         // ---- forward (as the dissipation kernel): row r quantities, Laplacians of row r-1
@@ -498,15 +549,17 @@ SWM_HD void diss_adjoint(W &w, const DissAdjArgs<typename W::R> &A, int ya, int 
         if (DRAG == 0) {
             const V sku_2 = half * (W::xp(sk_2) + sk_2), hu_2 = half * (W::xp(h_2) + h_2);
             const V skv_2 = half * (sk_1 + sk_2), hv_2 = half * (h_1 + h_2);
-            const V hu_s = W::sel(hu_2 != zero, hu_2, V(R(1))), hv_s = W::sel(hv_2 != zero, hv_2, V(R(1)));
-            const V Bu_2 = cD * sku_2 * u_2 / hu_s;
-            const V Bv_2 = cD * skv_2 * v_2 / hv_s;
-            bdu_2 = gu_2 * ((cD * sku_2) / hu_s);
-            bdv_2 = gv_2 * ((cD * skv_2) / hv_s);
-            const V lsku_2 = gu_2 * (cD * u_2 / hu_s);
-            lskv_2 = gv_2 * (cD * v_2 / hv_s);
-            lhu_2 = -(gu_2 * Bu_2 / hu_s);
-            lhv_2 = -(gv_2 * Bv_2 / hv_s);
+            // the derivatives of cD·sk·u/h share 1/h: one division per point (the products
+            // round differently from the forward quotient; the linearisation is the same)
+            const V ihu = V(R(1)) / W::sel(hu_2 != zero, hu_2, V(R(1))), ihv = V(R(1)) / W::sel(hv_2 != zero, hv_2, V(R(1)));
+            const V Bu_2 = cD * sku_2 * u_2 * ihu;
+            const V Bv_2 = cD * skv_2 * v_2 * ihv;
+            bdu_2 = gu_2 * ((cD * sku_2) * ihu);
+            bdv_2 = gv_2 * ((cD * skv_2) * ihv);
+            const V lsku_2 = gu_2 * (cD * u_2 * ihu);
+            lskv_2 = gv_2 * (cD * v_2 * ihv);
+            lhu_2 = -(gu_2 * Bu_2 * ihu);
+            lhv_2 = -(gv_2 * Bv_2 * ihv);
             const V lsk_2 = half * (lsku_2 + W::xm(lsku_2)) + half * (lskv_2 + lskv_3);
             lKE_2 = W::sel(sk_2 > zero, lsk_2 * half / W::sel(sk_2 > zero, sk_2, V(R(1))), zero);
         } else if (DRAG == 1) {
@@ -524,7 +577,7 @@ SWM_HD void diss_adjoint(W &w, const DissAdjArgs<typename W::R> &A, int ya, int 
                 lu = lu + ((ls0_3 - W::xp(ls0_3)) + (ls1_4 - ls1_3));
                 lv = lv + ((ls0_3 - ls0_2) + (W::xm(ls1_3) - ls1_3));
             }
-            V le = w.ld(A.lin.e, y);
+            V le = c.le;
             if (DRAG == 0) {
                 lu = lu + (bdu_3 + u_3 * (lKE_3 + W::xp(lKE_3)));
                 lv = lv + (bdv_3 + v_3 * (lKE_3 + lKE_2));
