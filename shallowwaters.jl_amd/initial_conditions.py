@@ -5,8 +5,8 @@ Mirrors ``src/initial_conditions.jl:6-164`` (rest start and the analytic sst fie
 (``add_halo``); inside the time loop the same copies are made on the GPU by the kernels
 that produce each field.
 
-The restart path ``initial_cond="ncfile"`` (NetCDF reader plus regridding) is not part of
-this package and raises ``NotImplementedError``.  ``seeded_initial_conditions`` is an
+The restart path ``initial_cond="ncfile"`` reads a run folder's NetCDF files (``output.py``).
+``seeded_initial_conditions`` is an
 addition of this package: the seeded, smoothed and geostrophically balanced start used by
 the channel, ensemble and mixed-precision workloads, injected through
 ``time_integration(Prog, Diag, S)`` as the reference allows for any ``Prog``.
@@ -205,15 +205,29 @@ def sst_initial_field(S) -> np.ndarray:
 
 
 def initial_conditions(Tprog, S) -> PrognosticVars:
-    """``initial_conditions(T,S)``, ``src/initial_conditions.jl:26-164`` (rest start)."""
+    """``initial_conditions(T,S)``, ``src/initial_conditions.jl:26-164``: rest, or a restart
+    from a run folder's NetCDF files (``initial_cond="ncfile"``)."""
     P, G = S.parameters, S.grid
     Tprog = canonical_type(Tprog)
-    if P.initial_cond != "rest":
-        raise NotImplementedError("initial_cond='ncfile' (NetCDF restart) is not supported")
-    u = np.zeros((G.nux, G.nuy), order="F")
-    v = np.zeros((G.nvx, G.nvy), order="F")
-    η = np.zeros((G.nx, G.ny), order="F")
-    sst = sst_initial_field(S)
+    sst = None
+    if P.initial_cond == "ncfile":                         # :39-120, the run folder's NetCDF files
+        from .output import read_initial_conditions
+        u, v, η, sst = read_initial_conditions(S)
+        for name, a, shape in (("u", u, (G.nux, G.nuy)), ("v", v, (G.nvx, G.nvy)), ("η", η, (G.nx, G.ny))):
+            if a.shape != shape:
+                raise ValueError(f"restart {name} of shape {a.shape}, the grid needs {shape}")
+        u, v, η = (np.asfortranarray(a) for a in (u, v, η))
+    else:
+        u = np.zeros((G.nux, G.nuy), order="F")
+        v = np.zeros((G.nvx, G.nvy), order="F")
+        η = np.zeros((G.nx, G.ny), order="F")
+    if sst is None:
+        if P.sst_initial == "restart":
+            raise ValueError("sst_initial='restart' needs initial_cond='ncfile'")
+        sst = sst_initial_field(S)
+    elif sst.shape != (G.nx, G.ny):
+        raise ValueError(f"restart sst of shape {sst.shape}, the grid needs {(G.nx, G.ny)}")
+    sst = np.asfortranarray(sst)
     u, v, η, sst = (to_storage(a, Tprog) for a in (u, v, η, sst))
     u, v, η, sst = add_halo(u, v, η, sst, S, T=Tprog)
     return PrognosticVars(u=u, v=v, η=η, sst=sst, T=Tprog)

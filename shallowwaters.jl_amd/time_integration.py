@@ -202,15 +202,55 @@ class Ensemble:
         self.close()
 
 
+def _stops(S, fb, ncs) -> list:
+    """The steps at which the host takes part: NetCDF output (every ``nout`` steps, and the
+    step before it when tendencies are written), the duration estimate (after the tracer ran
+    once and twice, ``feedback.jl:125-131``) and the progress percentages
+    (``feedback.jl:168-181``, after step 100); the last step."""
+    G, P = S.grid, S.parameters
+    nt = G.nt
+    stops = {nt}
+    if P.output:
+        stops.update(k * G.nout for k in range(1, nt // G.nout + 1))
+        if ncs.wants_tendencies:
+            stops.update(k * G.nout - 1 for k in range(1, nt // G.nout + 1) if k * G.nout > 1)
+        stops.update((G.nadvstep, 2 * G.nadvstep))
+    stops.update(i for i in range(101, nt + 1) if ((i + 1) / nt * 100 % 1) < (i / nt * 100 % 1))
+    return sorted(i for i in stops if 1 <= i <= nt)
+
+
 def time_integration(Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup) -> PrognosticVars:
-    """``src/time_integration.jl:2-250`` on the GPU."""
-    G = S.grid
-    with Simulation(S) as sim:
-        sim.upload(Prog)
-        done, nan_step = sim.step(G.nt, 1, 0)
-        sim.download(Prog)
-    if nan_step:
-        print(f" NaNs detected at time step {nan_step}")
+    """``src/time_integration.jl:2-250`` on the GPU: the steps run on the device in chunks
+    between the steps where the host writes output (``output_nc!``) or feedback."""
+    from .output import NcFiles, feedback, feedback_end, feedback_init
+    G, P = S.grid, S.parameters
+    fb = feedback_init(S)                                   # :38-41
+    ncs = NcFiles(fb, S)
+    try:
+        with Simulation(S) as sim:
+            sim.upload(Prog)
+            cur = remove_halo(Prog.u, Prog.v, Prog.η, Prog.sst, S)
+            ncs.output(0, cur, cur[:3], S)
+            before, before_i = cur, 0                       # the last state downloaded, and its step
+            i = 0
+            for stop in _stops(S, fb, ncs):
+                done, nan_step = sim.step(stop - i, i + 1, i * G.dtint)
+                i += done
+                fb.i = i
+                if nan_step:
+                    feedback(fb, S, nan_step)
+                    break
+                feedback(fb, S)
+                if P.output and (i % G.nout == 0 or (ncs.wants_tendencies and (i + 1) % G.nout == 0)):
+                    g = sim.download()
+                    now = remove_halo(g.u, g.v, g.η, g.sst, S)
+                    if i % G.nout == 0:                     # tendencies: the state after step i-1
+                        ncs.output(i, now, before[:3] if before_i == i - 1 else now[:3], S)
+                    before, before_i = now, i
+            sim.download(Prog)
+    finally:
+        ncs.close()
+    feedback_end(fb)
     u, v, η, sst = remove_halo(Prog.u, Prog.v, Prog.η, Prog.sst, S)
     return PrognosticVars(u=u, v=v, η=η, sst=sst, T=S.parameters.Tprog)
 
