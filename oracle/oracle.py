@@ -87,6 +87,22 @@ def tracer(cfg, i, u, v, um, vm, sst):
     return um, vm, sst
 
 
+def pv_advection(cfg, q, U, V):
+    """The PV advection terms alone (Float64) of the configured scheme: from q at the corners
+    (nx+1, ny+1) and the mass fluxes U (nx+1, ny+2), V (nx+2, ny+1) -> (qhv, qhu) of shapes
+    (nux, nuy), (nvx, nvy).  q's scheme-specific combinations are formed inside."""
+    q, U, V = (np.asfortranarray(a, dtype=np.float64) for a in (q, U, V))
+    nx, ny = q.shape[0] - 1, q.shape[1] - 1
+    nux = nx if cfg.bc == 0 else nx - 1                     # SWM_PERIODIC = 0
+    qhv = np.zeros((nux, ny), order="F")
+    qhu = np.zeros((nx, ny - 1), order="F")
+    L = lib()
+    L.swmo_pv_advection.argtypes = [ctypes.c_void_p] * 6
+    if L.swmo_pv_advection(ctypes.byref(cfg), *[_ptr(a) for a in (q, U, V, qhv, qhu)]) != 0:
+        raise OracleError("shape error")
+    return qhv, qhu
+
+
 _STENCIL = {"dx": (0, -1, 0), "dy": (1, 0, -1), "lap": (2, -2, -2), "Ix": (3, -1, 0), "Iy": (4, 0, -1),
             "Ixy": (5, -1, -1)}
 

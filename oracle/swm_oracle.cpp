@@ -18,7 +18,9 @@
 // test/runtests.jl:4-9, closed forms of every stencil, rest-state tendencies equal to the
 // wind, uniform zonal flow in Coriolis balance, exact mass conservation of continuity,
 // x-shift equivariance away from the periodic seam (where the reference's AHβ!/AHγ! wrap
-// breaks it), exact tracer shifts and bilinear reproduction of linear fields, and IEEE
+// breaks it), the PV advection schemes' exact conservation identities on non-divergent
+// fluxes (enstrophy for Sadourny's, enstrophy and energy for Arakawa-Hsu's), exact tracer
+// shifts and bilinear reproduction of linear fields, and IEEE
 // binary16 / bfloat16 rounding.  Julia is not installed here, so the reference itself
 // cannot be run; see the tests for what each pin covers.
 #include <cmath>
@@ -295,6 +297,8 @@ template <class T, class Tp, class Tc> struct Model {
     Arr<T> sqrtKE, sqrtKE_u, sqrtKE_v, Bu, Bv;
     // ArakawaHsuVars{T}   preallocate.jl:253-288
     Arr<T> qa, qb, qg, qd;
+    // VorticityVars{T} of the Sadourny scheme  preallocate.jl:141-145
+    Arr<T> q_u, V_u, q_v, U_v;
     // LaplaceVars{T}      preallocate.jl:291-329
     Arr<T> Lu, Lv, dLudx, dLudy, dLvdx, dLvdy;
     // SmagorinskyVars{T}  preallocate.jl:332-385
@@ -358,6 +362,8 @@ template <class T, class Tp, class Tc> struct Model {
         dvdy = KEv = v_T = vm_T = Arr<T>(nvx + 4, nvy + 3);
         qa = qd = xd = yd = uinterp = vinterp = Arr<T>(nx, ny);
         qb = qg = Arr<T>(nx + 1, ny);
+        q_u = V_u = Arr<T>(nx + 1, ny);
+        q_v = U_v = Arr<T>(nx, ny + 1);
         Lu = Arr<T>(nux + 2, nuy + 2);
         Lv = Arr<T>(nvx + 2, nvy + 2);
         dLudx = S11 = Arr<T>(nux + 1, nuy + 2);
@@ -526,11 +532,34 @@ template <class T, class Tp, class Tc> struct Model {
         Ix(KEu, u2);
         Iy(KEv, v2);
         PV();
-        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) throw ShapeError();
+        PV_combinations();
+    }
+    void PV_combinations() {                                   // rhs.jl:117-128
+        if (c.adv_scheme == SWM_ADV_SADOURNY) {                // Iy!(q_u,q), Ix!(q_v,q)
+            const T half = T(0.5) * T(sci);                    // (1/s: q and the fluxes both carry s)
+            for (int j = 1; j <= q_u.n; ++j)
+                for (int i = 1; i <= q_u.m; ++i) q_u(i, j) = half * (q(i, j + 1) + q(i, j));
+            for (int j = 1; j <= q_v.n; ++j)
+                for (int i = 1; i <= q_v.m; ++i) q_v(i, j) = half * (q(i + 1, j) + q(i, j));
+            return;
+        }
         AHa();
         AHb();
         AHg();
         AHd();
+    }
+    void PV_advection() {                                      // PV_advection.jl:21-27
+        if (c.adv_scheme == SWM_ADV_SADOURNY) PV_Sadourny();
+        else PV_ArakawaHsu();
+    }
+    void PV_Sadourny() {                                       // PV_advection.jl:31-59
+        Ixy(V_u, V);
+        Ixy(U_v, U);
+        CHECK(qhv.m + 2 - ep == q_u.m && qhv.n == q_u.n && qhu.m == q_v.m && qhu.n + 2 == q_v.n);
+        for (int j = 1; j <= qhv.n; ++j)
+            for (int i = 1; i <= qhv.m; ++i) qhv(i, j) = q_u(i + 1 - ep, j) * V_u(i + 1 - ep, j);
+        for (int j = 1; j <= qhu.n; ++j)
+            for (int i = 1; i <= qhu.m; ++i) qhu(i, j) = q_v(i, j + 1) * U_v(i, j + 1);
     }
     void PV_ArakawaHsu() {                                     // PV_advection.jl:63-84
         for (int j = 1; j <= qhv.n; ++j)
@@ -569,7 +598,7 @@ template <class T, class Tp, class Tc> struct Model {
         if (c.dynamics != SWM_NONLINEAR) throw ShapeError();
         UVfluxes(uu, vv, e);
         if (c.nstep_advcor == 0) advection_coriolis(uu, vv);
-        PV_ArakawaHsu();
+        PV_advection();
         bernoulli(e);
         dx(dpdx, p);
         dy(dpdy, p);
@@ -868,7 +897,7 @@ template <class T, class Tp, class Tc> struct Model {
         const Arr<T> &ur_ = rhs_view(u, ur);
         const Arr<T> &vr_ = rhs_view(v, vr);
         advection_coriolis(ur_, vr_);
-        PV_ArakawaHsu();
+        PV_advection();
         copyto(u0, u);
         copyto(v0, v);
         copyto(eta0, eta);
@@ -1158,6 +1187,27 @@ int swmo_tracer(const swm_config *c, int64_t i, const double *u, const double *v
         std::memcpy(um, M.um.a.data(), sizeof(double) * M.um.size());
         std::memcpy(vm, M.vm.a.data(), sizeof(double) * M.vm.size());
         std::memcpy(sst, M.sst.a.data(), sizeof(double) * M.sst.size());
+    } catch (ShapeError &) {
+        return -1;
+    }
+    return 0;
+}
+
+// The PV advection terms alone for T = Float64 (PV_advection.jl:22-146 with the q
+// interpolations of rhs.jl:117-128): from q at the corners (nx+1, ny+1) and the mass fluxes U
+// (nx+1, ny+2), V (nx+2, ny+1) given as they are, qhv (nux, nuy) and qhu (nvx, nvy) of the
+// configured scheme.  Pins the schemes' conservation properties on fluxes chosen freely.
+int swmo_pv_advection(const swm_config *c, const double *q, const double *U, const double *V, double *qhv,
+                      double *qhu) {
+    try {
+        Model<double, double, double> M(*c);
+        std::memcpy(M.q.a.data(), q, sizeof(double) * M.q.size());
+        std::memcpy(M.U.a.data(), U, sizeof(double) * M.U.size());
+        std::memcpy(M.V.a.data(), V, sizeof(double) * M.V.size());
+        M.PV_combinations();
+        M.PV_advection();
+        std::memcpy(qhv, M.qhv.a.data(), sizeof(double) * M.qhv.size());
+        std::memcpy(qhu, M.qhu.a.data(), sizeof(double) * M.qhu.size());
     } catch (ShapeError &) {
         return -1;
     }

@@ -131,7 +131,10 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
 // input at row r (u, v, η and H at row r, f_q at corner row r-1) and leaves the terms of
 // du and dη at row r-1 and of dv at row r-2 (qhv, the p difference ... ), with the rows
 // each stencil needs kept as history.  Arithmetic in T, term by term as the reference.
-template <class R, bool PER> struct RhsPipe {
+// SAD: Sadourny's enstrophy-conserving scheme (PV_advection.jl:31-59) in place of Arakawa-Hsu's;
+// then k12 carries 1/2 (the weight of Iy!(q_u,q) and Ix!(q_v,q), rhs.jl:117-120) and no
+// column wraps (q_u and q_v read no q[end,·]).
+template <class R, bool PER, bool SAD = false> struct RhsPipe {
     R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
     R U_1 = 0, U_2 = 0, V_2 = 0;
     R q_2 = 0, qaU_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
@@ -164,6 +167,23 @@ template <class R, bool PER> struct RhsPipe {
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
         const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
         const R p_1 = halfp * (KEu_1 + KEv_1) + g * e_1;
+        if constexpr (SAD) {
+            // qhv = q_u·V_u at row r-1 (q_u = Iy!(q), V_u = Ixy!(V)) and qhu = q_v·U_v at row
+            // r-2 (q_v = Ix!(q), U_v = Ixy!(U)); lane x holds the q corner x, the V point
+            // x-1/2 and the U point x
+            const R qxm_2 = sh_xm(q_2);
+            qhv = (k12 * (q_1 + q_2)) * (quarter * (V_2 + sh_xp(V_2)) + quarter * (V_1 + sh_xp(V_1)));
+            qhu = (k12 * (q_2 + qxm_2)) * (quarter * (sh_xm(U_2) + U_2) + quarter * (sh_xm(U_1) + U_1));
+            px = sh_xp(p_1) - p_1;
+            dUdx = U_1 - sh_xm(U_1);
+            dVdy = V_1 - V_2;
+            e_r1 = e_1;
+            py = p_1 - p_2;
+            u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hs_1 = hs_0;
+            U_2 = U_1; U_1 = U_0; V_2 = V_1;
+            q_2 = q_1; p_2 = p_1;
+            return;
+        }
         // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0).
         // The x-1 neighbours are moved before the per-lane choice of the wrapped column: a
         // lane move inside a branch taken by some lanes only would read 0 from the others.
@@ -203,7 +223,7 @@ template <class R, bool PER> struct RhsPipe {
 // ghost copies of what the next stage reads.  At input row r the wave forms U(r), then the
 // row-(r-1) quantities h_v, V, h_q, q, KE, p and the Arakawa-Hsu weights, and emits du (and
 // dη) at row r-1 and dv at row r-2.
-template <class F, bool PER, int M>
+template <class F, bool PER, int M, bool SAD = false>
 __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     using MI = Mode<M>;
     using R = typename F::R;
@@ -236,7 +256,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
 
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
 
-    RhsPipe<R, PER> rhs;
+    RhsPipe<R, PER, SAD> rhs;
     // the w operands of the combination: the stage input in its own format at rows r-1 (u,
     // η) and r-2 (v)
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;
@@ -261,13 +281,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         tH[k] = fH.template ldr<R>(vxT, rt);
     }
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
-    RowStatics<R> nst = row_statics<F, PER, true>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
+    RowStatics<R> nst = row_statics<F, PER && !SAD, true>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
     for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const RowStatics<R> sa = nst;
-        nst = row_statics<F, PER, true>(A, ar, vfq, vFx, vFy, vxT, r);
+        nst = row_statics<F, PER && !SAD, true>(A, ar, vfq, vFx, vFy, vxT, r);
         if (r < rlast) {
             const int k = r + 1 - (t.yb - 1);                  // uniform
             if (k >= 0) {
@@ -336,9 +356,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     }
 }
 
-template <class F, bool PER, int M>
+template <class F, bool PER, int M, bool SAD = false>
 __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
-    stage_body<F, PER, M>(A, wave_index());
+    stage_body<F, PER, M, SAD>(A, wave_index());
 }
 
 // the same stage for every member of an ensemble in one launch (tile table `start`)

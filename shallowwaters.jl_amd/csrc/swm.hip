@@ -338,7 +338,7 @@ template <class F> struct Launch {
         A.k = k;
         A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
         // scaled velocities: PV weights /s, Bernoulli ½ /s and g·s (powers of two, exact)
-        A.g = mkR<R>(c.g * ctx->sc); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(1.0 / 12.0, c.T) * ctx->sci);
+        A.g = mkR<R>(c.g * ctx->sc); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(c.adv_scheme == SWM_ADV_SADOURNY ? 0.5 : 1.0 / 12.0, c.T) * ctx->sci);
         A.halfp = mkR<R>(0.5 * ctx->sci);
         A.oma = (Q)c.one_minus_alpha;
         A.sinv = (Q)ctx->sci;
@@ -387,7 +387,8 @@ template <class F> struct Launch {
             ctx->rec->add(0, m, -1, &A, sizeof A);
             return SWM_OK;
         }
-        if (per) {
+        const bool sad = c.adv_scheme == SWM_ADV_SADOURNY;
+        if (per && !sad) {
             const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
             if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
             else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
@@ -395,7 +396,10 @@ template <class F> struct Launch {
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
-        if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);          \
+        if (sad) {                                                                                         \
+            if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_, true>), grid, block, 0, ctx->stream, A);  \
+            else hipLaunchKernelGGL((stage_kernel<F, false, M_, true>), grid, block, 0, ctx->stream, A);     \
+        } else if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);   \
         else hipLaunchKernelGGL((stage_kernel<F, false, M_>), grid, block, 0, ctx->stream, A);             \
         break;
         if constexpr (F::mixed) {
@@ -419,7 +423,7 @@ template <class F> struct Launch {
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
     static bool fusable(const swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK;
+        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && c.adv_scheme == SWM_ADV_ARAKAWAHSU;
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -788,7 +792,7 @@ int check_config(swm_ctx *ctx) {
     if (c.nstep_advcor != 0) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
     if (c.scale_log2 < -20 || c.scale_log2 > 20) return ctx->fail(SWM_E_ARG, "scale_log2 %d out of -20..20", c.scale_log2);
     if (c.compensated && c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "compensated needs the classic RK scheme");
-    if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) return ctx->fail(SWM_E_UNSUPPORTED, "adv_scheme Sadourny is not implemented");
+    if (c.adv_scheme != SWM_ADV_ARAKAWAHSU && c.adv_scheme != SWM_ADV_SADOURNY) return ctx->fail(SWM_E_ARG, "adv_scheme %d", c.adv_scheme);
     if (c.dynamics != SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "linear dynamics is not implemented");
     if (c.continuity < SWM_CONT_ITSELF || c.continuity > SWM_CONT_FORCING) return ctx->fail(SWM_E_ARG, "continuity %d", c.continuity);
     if (c.nstep_diff < 1 || (c.tracer_advection && c.nadvstep < 1)) return ctx->fail(SWM_E_ARG, "nstep_diff/nadvstep must be >= 1");
@@ -929,6 +933,7 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
+        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the Arakawa-Hsu scheme only");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -2060,6 +2065,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         else if (c.T != c0.T || c.Tprog != c0.Tprog || c.Tcomm != c0.Tcomm) why = "number formats differ";
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
+        else if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) why = "ensembles run the Arakawa-Hsu scheme only";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
         else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";

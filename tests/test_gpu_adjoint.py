@@ -135,11 +135,12 @@ def test_rrule_of_time_integration(swm, gpu):
 
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
-    P, S, cfg, prog = setup(swm, "nonperiodic", T="BFloat16", Tprog="Float32")
-    with swm.Simulation(S) as sim:
-        sim.upload(prog)
-        with pytest.raises(swm.SwmError if hasattr(swm, "SwmError") else RuntimeError, match="adjoint"):
-            sim.adjoint_forward(2)
+    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", adv_scheme="Sadourny")):
+        P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
+        with swm.Simulation(S) as sim:
+            sim.upload(prog)
+            with pytest.raises(swm.SwmError if hasattr(swm, "SwmError") else RuntimeError, match="adjoint"):
+                sim.adjoint_forward(2)
     P, S, cfg, prog = setup(swm, "nonperiodic", T="Float32")
     with swm.Simulation(S) as sim:
         sim.upload(prog)

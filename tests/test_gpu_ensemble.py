@@ -88,9 +88,12 @@ def test_tuned_ensemble_in_two_calls(swm, gpu, bc, T, Tprog):
             sim.close()
 
 
-def test_ensemble_rejects_mixed_options(swm, gpu):
+@pytest.mark.parametrize("kb", [dict(bc="nonperiodic"), dict(bc="periodic", adv_scheme="Sadourny")])
+def test_ensemble_rejects_mixed_options(swm, gpu, kb):
+    """Members must share formats, boundary condition and scheme; the batched kernels are of
+    the Arakawa-Hsu scheme, so a Sadourny member is refused."""
     a = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, bc="periodic")))
-    b = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, bc="nonperiodic")))
+    b = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, **kb)))
     try:
         for sim in (a, b):
             sim.upload(swm.seeded_initial_conditions(sim.S, seed=1, σ=0.1))

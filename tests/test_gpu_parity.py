@@ -86,6 +86,20 @@ CASES = {
     "np-f32-consumption": dict(T="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", tracer_consumption=True,
                                topography="seamount", wind_forcing_x="double_gyre"),
     "per-f64-consumption": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", tracer_consumption=True),
+    # Sadourny's enstrophy-conserving PV advection (PV_advection.jl:31-59, rhs.jl:117-120)
+    "np-f64-sadourny": dict(T="Float64", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                            bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre",
+                            adv_scheme="Sadourny"),
+    "per-f32-sadourny": dict(T="Float32", nx=66, L_ratio=1.5, bc="periodic", diffusion="Smagorinsky",
+                             bottom_drag="quadratic", topography="seamount", wind_forcing_x="channel",
+                             adv_scheme="Sadourny"),
+    "np-f32-sadourny-rk3": dict(T="Float32", nx=61, L_ratio=0.8, bc="nonperiodic", α=2.0, diffusion="constant",
+                                bottom_drag="linear", topography="ridge", adv_scheme="Sadourny", RKo=3, cfl=0.6),
+    "per-f64-sadourny-ssprk3": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", time_scheme="SSPRK3", RKn=2,
+                                    cfl=1.2, adv_scheme="Sadourny"),
+    "np-f16-f32-sadourny-scaled": dict(T="Float16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5,
+                                       diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                       wind_forcing_x="double_gyre", scale=8.0, adv_scheme="Sadourny"),
 }
 
 
@@ -222,7 +236,7 @@ def test_upload_rejects_nonzero_unwritten_halo(swm, gpu):
 
 
 def test_unsupported_options_fail_loudly(swm, gpu):
-    for kw in (dict(nstep_advcor=1), dict(dynamics="linear"), dict(adv_scheme="Sadourny")):
+    for kw in (dict(nstep_advcor=1), dict(dynamics="linear")):
         P = swm.Parameter(nx=40, **kw)
         S = swm.make_setup(P)
         with pytest.raises(Exception):

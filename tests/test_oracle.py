@@ -319,3 +319,70 @@ def test_compensated_increments_track_float64_closer(swm, orc, bc):
     for k, name in enumerate(("u", "v", "η")):
         assert err[True][k] < err[False][k], f"{name}: compensated {err[True][k]:.2e} vs plain {err[False][k]:.2e}"
     assert err[True][0] < 0.8 * err[False][0]
+
+
+# ------------------------------------------------------------------ PV advection schemes (PV_advection.jl)
+
+def _nondivergent_fluxes(nx, ny, seed, margin=3):
+    """q at the corners (nx+1, ny+1) and mass fluxes from a streamfunction ψ at the corners,
+    U = -δy ψ on the u-points (nx+1, ny+2), V = δx ψ on the v-points (nx+2, ny+1), so the
+    discrete divergence δx U + δy V vanishes exactly in every cell.  ψ is integer-valued and
+    zero within ``margin`` cells of the walls, so U, V and both PV flux terms vanish there."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros((nx + 5, ny + 6))                     # ψ at corner x = -2..nx+2, y = -3..ny+2
+    P[2 + margin:nx + 3 - margin, 3 + margin:ny + 4 - margin] = rng.integers(-8, 9, (nx + 1 - 2 * margin, ny + 1 - 2 * margin))
+    psi = lambda x, y: P[x + 2, y + 3]
+    iU, jU = np.meshgrid(np.arange(1, nx + 2), np.arange(1, ny + 3), indexing="ij")   # U(i,j) at (i-1, j-1.5)
+    U = -(psi(iU - 1, jU - 1) - psi(iU - 1, jU - 2))
+    iV, jV = np.meshgrid(np.arange(1, nx + 3), np.arange(1, ny + 2), indexing="ij")   # V(i,j) at (i-1.5, j-1)
+    V = psi(iV - 1, jV - 1) - psi(iV - 2, jV - 1)
+    q = rng.standard_normal((nx + 1, ny + 1))                                          # q(i,j) at (i-1, j-1)
+    return q, U, V
+
+
+def _pv_invariants(q, qhv, qhu, U, V, ep):
+    """(Σ q ζ_t, scale) and (Σ U·qhv - Σ V·qhu, scale) for du = qhv, dv = -qhu: the PV flux
+    term's enstrophy and energy tendencies.  qhv(i,j) sits at u-point (i-ep, j-0.5), qhu(i,j)
+    at v-point (i-0.5, j)."""
+    nx, ny = q.shape[0] - 1, q.shape[1] - 1
+    a, b = np.meshgrid(np.arange(1, nx), np.arange(1, ny), indexing="ij")             # interior corners
+    zt = -(qhu[a, b - 1] - qhu[a - 1, b - 1]) - (qhv[a + ep - 1, b] - qhv[a + ep - 1, b - 1])
+    qc = q[a, b]
+    ens = (np.sum(qc * zt), np.sum(np.abs(qc * zt)))
+    Uu = U[1 - ep:1 - ep + qhv.shape[0], 1:1 + qhv.shape[1]]                           # U at qhv's points
+    Vv = V[1:1 + qhu.shape[0], 1:1 + qhu.shape[1]]                                     # V at qhu's points
+    en = (np.sum(Uu * qhv) - np.sum(Vv * qhu), np.sum(np.abs(Uu * qhv)) + np.sum(np.abs(Vv * qhu)))
+    return ens, en
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+def test_pv_advection_conservation(swm, orc, bc):
+    """Sadourny (1975)'s scheme conserves enstrophy, Arakawa & Hsu (1990)'s conserves enstrophy
+    and energy, for a non-divergent mass flux: Σ q ζ_t = 0 (and Σ u·(qhv, -qhu) = 0) up to
+    rounding, exact identities of each scheme's stencil.  Sadourny's does not conserve energy,
+    and a q combination shifted by one point breaks the enstrophy identity; both are checked so
+    the sums are shown to see the stencils' indexing."""
+    nx, ny = 24, 18
+    q, U, V = _nondivergent_fluxes(nx, ny, seed=11)
+    ep = 1 if bc == "periodic" else 0
+    assert np.all(np.diff(U, axis=0)[:, 1:-1] + np.diff(V, axis=1)[1:-1, :] == 0)   # δx U + δy V = 0
+    out = {}
+    for scheme in ("Sadourny", "ArakawaHsu"):
+        S = swm.make_setup(swm.Parameter(T="Float64", nx=nx, L_ratio=nx / ny, bc=bc, adv_scheme=scheme))
+        assert (S.grid.nx, S.grid.ny) == (nx, ny)
+        qhv, qhu = orc.pv_advection(swm.make_config(S), q, U, V)
+        assert qhv.shape == (S.grid.nux, S.grid.nuy) and qhu.shape == (S.grid.nvx, S.grid.nvy)
+        out[scheme] = (qhv, qhu)
+        (e, es), (k, ks) = _pv_invariants(q, qhv, qhu, U, V, ep)
+        assert es > 1 and abs(e) < 1e-13 * es, (scheme, e, es)
+        if scheme == "ArakawaHsu":
+            assert ks > 1 and abs(k) < 1e-13 * ks, (k, ks)
+        else:
+            assert abs(k) > 1e-3 * ks                                                  # energy not conserved
+    qhv, qhu = out["Sadourny"]
+    (e, es), _ = _pv_invariants(q, np.roll(qhv, 1, axis=1), qhu, U, V, ep)
+    assert abs(e) > 1e-3 * es
+    # Sadourny's closed form on a uniform q and uniform fluxes: qhv = q·V, qhu = q·U
+    S = swm.make_setup(swm.Parameter(T="Float64", nx=nx, L_ratio=nx / ny, bc=bc, adv_scheme="Sadourny"))
+    qhv, qhu = orc.pv_advection(swm.make_config(S), np.full_like(q, 3.0), np.full_like(U, 2.0), np.full_like(V, -5.0))
+    assert np.all(qhv == -15.0) and np.all(qhu == 6.0)
