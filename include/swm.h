@@ -110,6 +110,13 @@ const char *swm_last_error(const swm_ctx *ctx);
 int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy,
                    const void *f_q, const void *eta_ref, const void *Feta);
 
+/* The Coriolis parameter on the u- and v-grid, read by the linear dynamics only
+ * (rhs_linear!, src/rhs.jl:55-86; Grid.f_u, f_v, src/grid.jl:93-94), format T, Julia layout:
+ *   f_u   (nux, nuy)
+ *   f_v   (nvx, nvy)
+ * Required once, after swm_create, when dynamics is SWM_LINEAR; an error otherwise. */
+int swm_set_static_linear(swm_ctx *ctx, const void *f_u, const void *f_v);
+
 /* Prognostic variables in format Tprog, halo included (PrognosticVars,
  * src/initial_conditions.jl:6-11):
  *   u (nux+4, nuy+4), v (nvx+4, nvy+4), eta (nx+2, ny+2), sst (nx+2, ny).

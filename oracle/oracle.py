@@ -31,7 +31,7 @@ def lib():
             raise RuntimeError(f"oracle library {LIB_PATH} is not built (run `make -C oracle`)")
         L = ctypes.CDLL(LIB_PATH)
         vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
-        L.swmo_run.argtypes = [vp] * 11 + [i64, i64, i64, i32, ctypes.POINTER(vp),
+        L.swmo_run.argtypes = [vp] * 13 + [i64, i64, i64, i32, ctypes.POINTER(vp),
                                            ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.swmo_run.restype = ctypes.c_int
         L.swmo_stencil.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
@@ -67,7 +67,7 @@ def run(S, cfg, prog, nsteps, i0=1, t0=0, stop_stage=0, want_rk=False):
         rk_ptrs = (ctypes.c_void_p * 9)(*[x.ctypes.data for x in rk])
     sd, ns = ctypes.c_int64(0), ctypes.c_int64(0)
     statics = [_fortran(F.H), _fortran(F.Fx), _fortran(F.Fy), _fortran(G.f_q),
-               _fortran(F.η_ref), _fortran(F.Fη)]
+               _fortran(F.η_ref), _fortran(F.Fη), _fortran(G.f_u), _fortran(G.f_v)]
     rc = lib().swmo_run(ctypes.byref(cfg), *[_ptr(a) for a in statics], _ptr(u), _ptr(v), _ptr(e), _ptr(s),
                         nsteps, i0, t0, stop_stage, rk_ptrs, ctypes.byref(sd), ctypes.byref(ns))
     if rc != 0:

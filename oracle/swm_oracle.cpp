@@ -279,8 +279,8 @@ template <class T, class Tp, class Tc> struct Model {
     const swm_config &c;
     int nx, ny, nux, nuy, nvx, nvy, ep;
     bool periodic;
-    // static fields (Forcing{T}, Grid.f_q)
-    Arr<T> H, Fx, Fy, f_q, eta_ref, Feta;
+    // static fields (Forcing{T}, Grid.f_q; Grid.f_u, f_v for the linear dynamics)
+    Arr<T> H, Fx, Fy, f_q, eta_ref, Feta, f_u, f_v;
     // PrognosticVars{Tprog}
     Arr<Tp> u, v, eta, sst;
     // RungeKuttaVars{Tprog}, TendencyVars{Tprog}   preallocate.jl:2-73
@@ -299,6 +299,9 @@ template <class T, class Tp, class Tc> struct Model {
     Arr<T> qa, qb, qg, qd;
     // VorticityVars{T} of the Sadourny scheme  preallocate.jl:141-145
     Arr<T> q_u, V_u, q_v, U_v;
+    // VorticityVars{T} of the linear dynamics (preallocate.jl:150-151) and the array g*η that
+    // rhs_linear! allocates
+    Arr<T> u_v, v_u, geta;
     // LaplaceVars{T}      preallocate.jl:291-329
     Arr<T> Lu, Lv, dLudx, dLudy, dLvdx, dLvdy;
     // SmagorinskyVars{T}  preallocate.jl:332-385
@@ -337,7 +340,8 @@ template <class T, class Tp, class Tc> struct Model {
         nvy = ny - 1;
         H = Arr<T>(nx + 2, ny + 2);
         Fx = Arr<T>(nux, nuy);
-        Fy = Arr<T>(nvx, nvy);
+        Fy = f_v = Arr<T>(nvx, nvy);
+        f_u = Arr<T>(nux, nuy);
         f_q = Arr<T>(nx + 1, ny + 1);
         eta_ref = Arr<T>(nx, ny);
         Feta = Arr<T>(nx, ny);
@@ -347,7 +351,9 @@ template <class T, class Tp, class Tc> struct Model {
         sst = Arr<Tp>(nx + 2, ny);
         ur = u2 = um = Arr<T>(nux + 4, nuy + 4);
         vr = v2 = vm = Arr<T>(nvx + 4, nvy + 4);
-        etar = h = p = sqrtKE = DT = DS = nuSmag = Arr<T>(nx + 2, ny + 2);
+        etar = h = p = sqrtKE = DT = DS = nuSmag = geta = Arr<T>(nx + 2, ny + 2);
+        u_v = Arr<T>(nux + 3, nuy + 3);
+        v_u = Arr<T>(nvx + 3, nvy + 3);
         sstr = ssti = Arr<T>(nx + 2, ny);
         h_u = U = dpdx = sqrtKE_u = Bu = Arr<T>(nx + 1, ny + 2);
         h_v = V = dpdy = sqrtKE_v = Bv = Arr<T>(nx + 2, ny + 1);
@@ -594,8 +600,26 @@ template <class T, class Tp, class Tc> struct Model {
             for (int i = 1; i <= m; ++i)
                 dv(i + 2, j + 2) = -(cvt<Tp>(qhu(i, j)) + cvt<Tp>(dpdy(i + 1, j + 1))) + cvt<Tp>(Fyt * Fy(i, j));
     }
+    // rhs_linear!  rhs.jl:55-86: f·v and f·u on the u- and v-grids (fv!, fu!  rhs.jl:183-215)
+    // and the gradient of g·η alone; U and V are not formed (continuity! under the classic RK
+    // scheme then reads the zero fluxes it was allocated with).
+    void rhs_linear(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e, int64_t t) {
+        const T g = T(c.g) * T(sc);
+        for (size_t k = 0; k < geta.size(); ++k) geta.a[k] = g * e.a[k];
+        dx(dpdx, geta);
+        dy(dpdy, geta);
+        Ixy(v_u, vv);
+        Ixy(u_v, uu);
+        CHECK(v_u.m == qhv.m + 4 - ep && v_u.n == qhv.n + 2 && u_v.m == qhu.m + 2 + ep && u_v.n == qhu.n + 4);
+        for (int j = 1; j <= qhv.n; ++j)
+            for (int i = 1; i <= qhv.m; ++i) qhv(i, j) = f_u(i, j) * v_u(i + 2 - ep, j + 1);
+        for (int j = 1; j <= qhu.n; ++j)
+            for (int i = 1; i <= qhu.m; ++i) qhu(i, j) = f_v(i, j) * u_v(i + 1 + ep, j + 2);
+        momentum_u(t);
+        momentum_v(t);
+    }
     void rhs(const Arr<T> &uu, const Arr<T> &vv, const Arr<T> &e, int64_t t) {  // rhs.jl:2-51
-        if (c.dynamics != SWM_NONLINEAR) throw ShapeError();
+        if (c.dynamics == SWM_LINEAR) return rhs_linear(uu, vv, e, t);
         UVfluxes(uu, vv, e);
         if (c.nstep_advcor == 0) advection_coriolis(uu, vv);
         PV_advection();
@@ -1018,7 +1042,7 @@ template <class T, class Tp, class Tc> struct Model {
         ghost_points(u0, v0, eta0);                            // :198
         if (tape) { tape->au = u0; tape->av = v0; tape->ae = eta0; }
         const Arr<T> &a0 = rhs_view(u0, ur), &b0 = rhs_view(v0, vr), &e0 = rhs_view(eta0, etar);
-        if (c.nstep_advcor > 0 && i % c.nstep_advcor == 0) {   // :208-211
+        if (c.dynamics == SWM_NONLINEAR && c.nstep_advcor > 0 && i % c.nstep_advcor == 0) {   // :208-211
             UVfluxes(a0, b0, e0);
             advection_coriolis(a0, b0);
         }
@@ -1043,7 +1067,7 @@ template <class T, class Tp, class Tc> struct Model {
 // ------------------------------------------------------------------ C entry points
 
 struct OracleStatic {
-    const void *H, *Fx, *Fy, *f_q, *eta_ref, *Feta;
+    const void *H, *Fx, *Fy, *f_q, *eta_ref, *Feta, *f_u, *f_v;
 };
 
 template <class T, class Tp, class Tc>
@@ -1057,6 +1081,11 @@ static int run_typed(const swm_config *c, const OracleStatic *s, void *u, void *
     load_arr(M.f_q, s->f_q);
     if (s->eta_ref) load_arr(M.eta_ref, s->eta_ref);
     if (s->Feta) load_arr(M.Feta, s->Feta);
+    if (c->dynamics == SWM_LINEAR) {
+        if (!s->f_u || !s->f_v) throw ShapeError();
+        load_arr(M.f_u, s->f_u);
+        load_arr(M.f_v, s->f_v);
+    }
     load_arr(M.u, u);
     load_arr(M.v, v);
     load_arr(M.eta, eta);
@@ -1155,9 +1184,10 @@ extern "C" {
 // du dv deta of the last step.  Returns 0, or -1 on a shape/option error or an index
 // the reference would throw on (NaN in the tracer, departure outside the arrays).
 int swmo_run(const swm_config *c, const void *H, const void *Fx, const void *Fy, const void *f_q,
-             const void *eta_ref, const void *Feta, void *u, void *v, void *eta, void *sst, int64_t nsteps,
-             int64_t i0, int64_t t0, int32_t stop_stage, void **rk_out, int64_t *steps_done, int64_t *nan_step) {
-    OracleStatic s{H, Fx, Fy, f_q, eta_ref, Feta};
+             const void *eta_ref, const void *Feta, const void *f_u, const void *f_v, void *u, void *v, void *eta,
+             void *sst, int64_t nsteps, int64_t i0, int64_t t0, int32_t stop_stage, void **rk_out, int64_t *steps_done,
+             int64_t *nan_step) {
+    OracleStatic s{H, Fx, Fy, f_q, eta_ref, Feta, f_u, f_v};
     try {
         switch (c->T) {
         case SWM_F64: return dispatch_prog<double>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);

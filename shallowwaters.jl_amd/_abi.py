@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libswm_hip.so")
 
 # every symbol include/swm.h declares
 EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_last_error",
-           "swm_set_static", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
+           "swm_set_static", "swm_set_static_linear", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
            "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
            "swm_adj_time"]
@@ -41,6 +41,7 @@ def load():
     L.swm_last_error.argtypes = [vp]
     L.swm_last_error.restype = ctypes.c_char_p
     L.swm_set_static.argtypes = [vp] * 7
+    L.swm_set_static_linear.argtypes = [vp] * 3
     L.swm_upload_prog.argtypes = [vp] * 5
     L.swm_download_prog.argtypes = [vp] * 5
     L.swm_step.argtypes = [vp, i64, i64, i64, pi64, pi64]

@@ -104,24 +104,30 @@ __device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, R cs,
 // fields: f_q at corner row y, Fx at u-row y, Fy at v-row y-1, the continuity static at
 // T-row y and (periodic) q at the corner (nx, y).  A load that is waited for in the row that
 // issues it would also wait for every load issued before it (the counter is in order).
+// The linear dynamics read f on the u-grid at row y and on the v-grid at row y-1 instead of
+// f_q (fu, fv; column offsets vfu, vfv), and no q column.
 template <class R> struct RowStatics {
-    R fq, Fx, Fy, cs, qend;
+    R fq, Fx, Fy, cs, qend, fu, fv;
 };
 
-template <class F, bool PER, bool CS>
+template <class F, bool PER, bool CS, int ADV = ADV_AH>
 __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs<F> &A, const Arena &ar, uint32_t vfq,
-                                                                 uint32_t vFx, uint32_t vFy, uint32_t vxT, int y) {
+                                                                 uint32_t vFx, uint32_t vFy, uint32_t vxT, int y,
+                                                                 uint32_t vfu = 0, uint32_t vfv = 0) {
     using R = typename F::R;
     using ST = typename F::ST;
     const int P = A.P;
     RowStatics<R> s;
-    s.fq = Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
+    s.fq = ADV == ADV_LIN ? R(0) : Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
     s.Fx = Fld<ST>{&ar, A.st.Fx}.template ldr<R>(vFx, srowb<ST>(y, P, A.st.Fx_2d));
     s.Fy = Fld<ST>{&ar, A.st.Fy}.template ldr<R>(vFy, srowb<ST>(y - 1, P, A.st.Fy_2d));
     s.cs = (!CS || A.cont == CONT_ITSELF)
                ? R(0)
                : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
-    s.qend = PER ? Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST)) : R(0);
+    s.qend = (PER && ADV == ADV_AH) ? Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST))
+                                    : R(0);
+    s.fu = ADV == ADV_LIN ? Fld<ST>{&ar, A.st.fu}.template ldr<R>(vfu, srowb<ST>(y, P, A.st.fu_2d)) : R(0);
+    s.fv = ADV == ADV_LIN ? Fld<ST>{&ar, A.st.fv}.template ldr<R>(vfv, srowb<ST>(y - 1, P, A.st.fv_2d)) : R(0);
     return s;
 }
 
@@ -131,11 +137,14 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
 // input at row r (u, v, η and H at row r, f_q at corner row r-1) and leaves the terms of
 // du and dη at row r-1 and of dv at row r-2 (qhv, the p difference ... ), with the rows
 // each stencil needs kept as history.  Arithmetic in T, term by term as the reference.
-// SAD: Sadourny's enstrophy-conserving scheme (PV_advection.jl:31-59) in place of Arakawa-Hsu's;
-// then k12 carries 1/2 (the weight of Iy!(q_u,q) and Ix!(q_v,q), rhs.jl:117-120) and no
-// column wraps (q_u and q_v read no q[end,·]).
-template <class R, bool PER, bool SAD = false> struct RhsPipe {
-    R u_1 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
+// ADV_SAD: Sadourny's enstrophy-conserving scheme (PV_advection.jl:31-59) in place of
+// Arakawa-Hsu's; then k12 carries 1/2 (the weight of Iy!(q_u,q) and Ix!(q_v,q), rhs.jl:117-120)
+// and no column wraps (q_u and q_v read no q[end,·]).  ADV_LIN: rhs_linear! (rhs.jl:55-86),
+// the Coriolis terms f·v, f·u of the bilinearly interpolated velocities and the gradient of
+// g·η; it forms no mass fluxes, so the divergence it leaves for continuity! is zero (the
+// reference's U, V stay as allocated under the classic RK scheme, continuity.jl:83-85).
+template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
+    R u_1 = 0, u_2 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
     R U_1 = 0, U_2 = 0, V_2 = 0;
     R q_2 = 0, qaU_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
     // results of the last push
@@ -148,8 +157,22 @@ template <class R, bool PER, bool SAD = false> struct RhsPipe {
     // products of two shifted factors are formed unshifted and moved once:
     // q(x-1)·U(x-1) is the shift of q·U.  The values are those of the formulas as written.
     // qend_1: q at the corner (nx, r-1), read by the wrapped first column when qfix (periodic x = 0)
+    // fu_1, fv_2: f on the u-grid at row r-1 and on the v-grid at row r-2 (ADV_LIN)
     __device__ __forceinline__ void push(R u_0, R v_0, R e_0, R H_0, R fq_1, R qend_1, bool qfix, R half, R quarter,
-                                         R k12, R g, R halfp) {
+                                         R k12, R g, R halfp, R fu_1 = 0, R fv_2 = 0) {
+        if constexpr (ADV == ADV_LIN) {
+            // ∂x!(dpdx,g*η), ∂y!(dpdy,g*η) ; Ixy!(v_u,v), fv! at row r-1 ; Ixy!(u_v,u), fu! at row r-2
+            const R p_1 = g * e_1;
+            qhv = fu_1 * (quarter * (v_2 + sh_xp(v_2)) + quarter * (v_1 + sh_xp(v_1)));
+            qhu = fv_2 * (quarter * (sh_xm(u_2) + u_2) + quarter * (sh_xm(u_1) + u_1));
+            px = sh_xp(p_1) - p_1;
+            py = p_1 - p_2;
+            dUdx = R(0);
+            dVdy = R(0);
+            e_r1 = e_1;
+            u_2 = u_1; u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; p_2 = p_1;
+            return;
+        }
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
         const R h_0 = e_0 + H_0;
         const R hs_0 = sh_xp(h_0) + h_0;                   // h(x+1) + h, read by Ix!(h_u) and Ixy!(h_q)
@@ -167,7 +190,7 @@ template <class R, bool PER, bool SAD = false> struct RhsPipe {
         const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
         const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
         const R p_1 = halfp * (KEu_1 + KEv_1) + g * e_1;
-        if constexpr (SAD) {
+        if constexpr (ADV == ADV_SAD) {
             // qhv = q_u·V_u at row r-1 (q_u = Iy!(q), V_u = Ixy!(V)) and qhu = q_v·U_v at row
             // r-2 (q_v = Ix!(q), U_v = Ixy!(U)); lane x holds the q corner x, the V point
             // x-1/2 and the U point x
@@ -223,7 +246,7 @@ template <class R, bool PER, bool SAD = false> struct RhsPipe {
 // ghost copies of what the next stage reads.  At input row r the wave forms U(r), then the
 // row-(r-1) quantities h_v, V, h_q, q, KE, p and the Arakawa-Hsu weights, and emits du (and
 // dη) at row r-1 and dv at row r-2.
-template <class F, bool PER, int M, bool SAD = false>
+template <class F, bool PER, int M, int ADV = ADV_AH>
 __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     using MI = Mode<M>;
     using R = typename F::R;
@@ -253,10 +276,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
+    const uint32_t vfu = scolb<ST>(x, A.st.fu_2d), vfv = scolb<ST>(x, A.st.fv_2d);
 
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
 
-    RhsPipe<R, PER, SAD> rhs;
+    RhsPipe<R, PER, ADV> rhs;
     // the w operands of the combination: the stage input in its own format at rows r-1 (u,
     // η) and r-2 (v)
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;
@@ -281,13 +305,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
         tH[k] = fH.template ldr<R>(vxT, rt);
     }
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;   // operands of rows r-1 (u, η) and r-2 (v)
-    RowStatics<R> nst = row_statics<F, PER && !SAD, true>(A, ar, vfq, vFx, vFy, vxT, r - 1);   // f_q, Fx, η static at r-1, Fy at r-2
+    RowStatics<R> nst = row_statics<F, PER, true, ADV>(A, ar, vfq, vFx, vFy, vxT, r - 1, vfu, vfv);   // f_q, Fx, η static at r-1, Fy at r-2
     for (; r <= rlast; ++r, sI += dI, sQ += dQ, sT += dT) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const RowStatics<R> sa = nst;
-        nst = row_statics<F, PER && !SAD, true>(A, ar, vfq, vFx, vFy, vxT, r);
+        nst = row_statics<F, PER, true, ADV>(A, ar, vfq, vFx, vFy, vxT, r, vfu, vfv);
         if (r < rlast) {
             const int k = r + 1 - (t.yb - 1);                  // uniform
             if (k >= 0) {
@@ -308,7 +332,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
             if (MI::need_pr) npv = pr_v.ld(vxQ, sQ - dQ);
             if (MI::need_ac) nav = ac_v.ld(vxQ, sQ - dQ);
         }
-        rhs.push(u_0, v_0, e_0, H_0, sa.fq, sa.qend, qfix, half, quarter, k12, A.g, A.halfp);
+        rhs.push(u_0, v_0, e_0, H_0, sa.fq, sa.qend, qfix, half, quarter, k12, A.g, A.halfp, sa.fu, sa.fv);
 
         // du at row r-1: momentum_u!  rhs.jl:242
         const int yu = r - 1;
@@ -356,9 +380,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
     }
 }
 
-template <class F, bool PER, int M, bool SAD = false>
+template <class F, bool PER, int M, int ADV = ADV_AH>
 __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
-    stage_body<F, PER, M, SAD>(A, wave_index());
+    stage_body<F, PER, M, ADV>(A, wave_index());
 }
 
 // the same stage for every member of an ensemble in one launch (tile table `start`)

@@ -125,9 +125,11 @@ struct swm_ctx {
     Field u[6], v[6], e[6], sst[2];                          // Tprog (slots 4, 5: the adjoint's taped step)
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
+    Field f_u, f_v;                                          // T: f on the u-, v-grid (linear dynamics)
     Field cu, cv, ce;                                        // Kahan compensations (compensated RK), Tprog
     double sc = 1.0, sci = 1.0;                              // u, v carried as sc·u, sc·v
-    int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0;
+    int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0, fu_sx = 1, fu_sy = 0, fv_sx = 1, fv_sy = 0;
+    bool have_fuv = false;   // swm_set_static_linear done
     int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
     int rsst = 0;
     Slots rk0, rk1;        // u0 / u1 after swm_stages
@@ -307,6 +309,8 @@ template <class F> struct Launch {
         s.fq = ctx->fq.ref(); s.fq_2d = ctx->fq_sx;
         s.eta_ref = ctx->eta_ref.ref();
         s.Feta = ctx->Feta.ref();
+        s.fu = ctx->f_u.ref(); s.fu_2d = ctx->fu_sx;
+        s.fv = ctx->f_v.ref(); s.fv_2d = ctx->fv_sx;
         return s;
     }
     static uint32_t ou(const swm_ctx *ctx, Slots s) { return s.uv < 0 ? 0u : ctx->fu(s).ref(); }
@@ -387,20 +391,21 @@ template <class F> struct Launch {
             ctx->rec->add(0, m, -1, &A, sizeof A);
             return SWM_OK;
         }
-        const bool sad = c.adv_scheme == SWM_ADV_SADOURNY;
-        if (per && !sad) {
+        const int adv = c.dynamics == SWM_LINEAR ? ADV_LIN : c.adv_scheme == SWM_ADV_SADOURNY ? ADV_SAD : ADV_AH;
+        if (per && adv == ADV_AH) {
             const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
             if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
             else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+#define SWM_KA(M_, ADV_)                                                                                   \
+    if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_, ADV_>), grid, block, 0, ctx->stream, A);        \
+    else hipLaunchKernelGGL((stage_kernel<F, false, M_, ADV_>), grid, block, 0, ctx->stream, A);
 #define SWM_K(M_)                                                                                          \
     case M_:                                                                                               \
-        if (sad) {                                                                                         \
-            if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_, true>), grid, block, 0, ctx->stream, A);  \
-            else hipLaunchKernelGGL((stage_kernel<F, false, M_, true>), grid, block, 0, ctx->stream, A);     \
-        } else if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);   \
-        else hipLaunchKernelGGL((stage_kernel<F, false, M_>), grid, block, 0, ctx->stream, A);             \
+        if (adv == ADV_SAD) { SWM_KA(M_, ADV_SAD) }                                                        \
+        else if (adv == ADV_LIN) { SWM_KA(M_, ADV_LIN) }                                                   \
+        else { SWM_KA(M_, ADV_AH) }                                                                        \
         break;
         if constexpr (F::mixed) {
             switch (m) {
@@ -416,6 +421,7 @@ template <class F> struct Launch {
             }
         }
 #undef SWM_K
+#undef SWM_KA
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
@@ -423,7 +429,8 @@ template <class F> struct Launch {
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
     static bool fusable(const swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && c.adv_scheme == SWM_ADV_ARAKAWAHSU;
+        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && c.adv_scheme == SWM_ADV_ARAKAWAHSU &&
+               c.dynamics == SWM_NONLINEAR;
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -789,11 +796,11 @@ int check_config(swm_ctx *ctx) {
     if (c.time_scheme == SWM_RK && (c.RKo < 2 || c.RKo > 4)) return ctx->fail(SWM_E_ARG, "RKo %d", c.RKo);
     if (c.time_scheme == SWM_SSPRK2 && c.RKs < 2) return ctx->fail(SWM_E_ARG, "RKs %d", c.RKs);
     if (c.time_scheme == SWM_SSPRK3 && (c.ssp_n < 2 || c.ssp_s != c.ssp_n * c.ssp_n)) return ctx->fail(SWM_E_ARG, "RKn %d", c.ssp_n);
-    if (c.nstep_advcor != 0) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
+    if (c.nstep_advcor != 0 && c.dynamics == SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
     if (c.scale_log2 < -20 || c.scale_log2 > 20) return ctx->fail(SWM_E_ARG, "scale_log2 %d out of -20..20", c.scale_log2);
     if (c.compensated && c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "compensated needs the classic RK scheme");
     if (c.adv_scheme != SWM_ADV_ARAKAWAHSU && c.adv_scheme != SWM_ADV_SADOURNY) return ctx->fail(SWM_E_ARG, "adv_scheme %d", c.adv_scheme);
-    if (c.dynamics != SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "linear dynamics is not implemented");
+    if (c.dynamics != SWM_NONLINEAR && c.dynamics != SWM_LINEAR) return ctx->fail(SWM_E_ARG, "dynamics %d", c.dynamics);
     if (c.continuity < SWM_CONT_ITSELF || c.continuity > SWM_CONT_FORCING) return ctx->fail(SWM_E_ARG, "continuity %d", c.continuity);
     if (c.nstep_diff < 1 || (c.tracer_advection && c.nadvstep < 1)) return ctx->fail(SWM_E_ARG, "nstep_diff/nadvstep must be >= 1");
     if (c.bottom_drag < 0 || c.bottom_drag > 2 || c.diffusion < 0 || c.diffusion > 1)
@@ -933,7 +940,8 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the Arakawa-Hsu scheme only");
+        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU || c.dynamics != SWM_NONLINEAR)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear Arakawa-Hsu right-hand side only");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -1790,7 +1798,7 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     // the arena: Tprog fields 12 state slots + 2 sst; T fields um, vm, H, Fx, Fy, f_q
     // (+ η_ref, Fη; + 6 u1 slots with mixed formats) and the q column
     const int nq = 14 + (c.compensated ? 3 : 0);
-    const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0);
+    const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0) + (c.dynamics == SWM_LINEAR ? 2 : 0);
     const size_t n1d = (size_t)c.ny + 2 * OY + 16;
     ctx->arena_bytes = (size_t)nq * (ctx->elems * ctx->es + 256) + (size_t)nt * (ctx->elems * ctx->est + 256) +
                        n1d * ctx->est + 256;
@@ -1825,6 +1833,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         return bail(rc);
     if (c.continuity != SWM_CONT_ITSELF &&
         ((rc = alloc_field(ctx, ctx->eta_ref, 0, true, est)) || (rc = alloc_field(ctx, ctx->Feta, 0, true, est))))
+        return bail(rc);
+    if (c.dynamics == SWM_LINEAR &&
+        ((rc = alloc_field(ctx, ctx->f_u, 0, true, est)) || (rc = alloc_field(ctx, ctx->f_v, 0, true, est))))
         return bail(rc);
     void *p = nullptr;
     if (hipMalloc(&p, 4 * sizeof(int)) != hipSuccess) return bail(SWM_E_HIP);
@@ -1884,6 +1895,20 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_static = true;
+    return SWM_OK;
+}
+
+int swm_set_static_linear(swm_ctx *ctx, const void *f_u, const void *f_v) {
+    if (!ctx) return SWM_E_ARG;
+    if (ctx->c.dynamics != SWM_LINEAR) return ctx->fail(SWM_E_STATE, "f_u, f_v are read by the linear dynamics only");
+    if (!f_u || !f_v) return ctx->fail(SWM_E_ARG, "f_u and f_v are required");
+    if (ctx->have_fuv) return ctx->fail(SWM_E_STATE, "f_u, f_v already set");
+    int rc;
+    // f multiplies the velocities carried scaled, so it is not scaled itself
+    if ((rc = put_static(ctx, ctx->f_u, &ctx->fu_sx, &ctx->fu_sy, f_u, ctx->nux, ctx->nuy, 1 - ctx->ep, 1))) return rc;
+    if ((rc = put_static(ctx, ctx->f_v, &ctx->fv_sx, &ctx->fv_sy, f_v, ctx->nvx, ctx->nvy, 1, 1))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_fuv = true;
     return SWM_OK;
 }
 
@@ -1965,16 +1990,24 @@ int swm_download_prog(swm_ctx *ctx, void *u, void *v, void *eta, void *sst) {
     return SWM_OK;
 }
 
+// the fields a step reads are all set
+static int step_ready(swm_ctx *ctx) {
+    if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+    if (ctx->c.dynamics == SWM_LINEAR && !ctx->have_fuv)
+        return ctx->fail(SWM_E_STATE, "the linear dynamics need f_u, f_v: swm_set_static_linear first");
+    return SWM_OK;
+}
+
 int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *steps_done, int64_t *nan_step) {
     if (!ctx || !steps_done || !nan_step) return SWM_E_ARG;
-    if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+    if (int rc = step_ready(ctx)) return rc;
     if (nsteps < 0 || i0 < 1) return ctx->fail(SWM_E_ARG, "nsteps >= 0 and i0 >= 1 required");
     return with_fmt(ctx, [&](auto f) { return run_steps<decltype(f)>(ctx, nsteps, i0, t0, steps_done, nan_step); });
 }
 
 int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
     if (!ctx) return SWM_E_ARG;
-    if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+    if (int rc = step_ready(ctx)) return rc;
     const swm_config &c = ctx->c;
     const int total = c.time_scheme == SWM_RK ? c.RKo : c.time_scheme == SWM_SSPRK2 ? c.RKs
                     : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
@@ -2010,7 +2043,7 @@ int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void
 
 int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     if (!ctx || !ms_avg) return SWM_E_ARG;
-    if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+    if (int rc = step_ready(ctx)) return rc;
     if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");
     return with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::time_kernel(ctx, which, reps, ms_avg); });
 }
@@ -2065,7 +2098,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         else if (c.T != c0.T || c.Tprog != c0.Tprog || c.Tcomm != c0.Tcomm) why = "number formats differ";
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
-        else if (c.adv_scheme != SWM_ADV_ARAKAWAHSU) why = "ensembles run the Arakawa-Hsu scheme only";
+        else if (c.adv_scheme != SWM_ADV_ARAKAWAHSU || c.dynamics != SWM_NONLINEAR) why = "ensembles run the nonlinear Arakawa-Hsu right-hand side only";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
         else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";

@@ -309,7 +309,12 @@ struct StaticRefs {
     uint32_t Fx, Fy, fq;               // U-, V-, Q-point; 1-D per row when x-invariant
     int Fx_2d, Fy_2d, fq_2d;           // 1: 2-D layout, 0: one value per row
     uint32_t eta_ref, Feta;            // T-point interior (continuity relax / forcing)
+    uint32_t fu, fv;                   // f on the u- and v-grid (linear dynamics), layout of Fx, Fy
+    int fu_2d, fv_2d;
 };
+
+// PV advection / Coriolis form of the right-hand side (rhs.jl:2-16, PV_advection.jl:21-27)
+enum { ADV_AH = 0, ADV_SAD = 1, ADV_LIN = 2 };
 
 // byte offset of (x, y) in a static field that may be stored 1-D (one value per row)
 template <class S> __device__ __forceinline__ uint32_t scolb(int x, int is2d) { return is2d ? colb<S>(x) : 0u; }

@@ -50,6 +50,10 @@ class Simulation:
         F, G = S.forcing, S.grid
         self._static = [np.asfortranarray(a) for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
         _abi.check(L.swm_set_static(ctx, *[_abi.ptr(a) for a in self._static]), ctx, "swm_set_static")
+        if S.parameters.dynamics == "linear":
+            self._static += [np.asfortranarray(a) for a in (G.f_u, G.f_v)]
+            _abi.check(L.swm_set_static_linear(ctx, *[_abi.ptr(a) for a in self._static[-2:]]), ctx,
+                       "swm_set_static_linear")
 
     def close(self):
         if getattr(self, "ctx", None):

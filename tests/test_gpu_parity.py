@@ -100,6 +100,22 @@ CASES = {
     "np-f16-f32-sadourny-scaled": dict(T="Float16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5,
                                        diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
                                        wind_forcing_x="double_gyre", scale=8.0, adv_scheme="Sadourny"),
+    # linear dynamics (rhs.jl:55-86): under the classic RK scheme η moves by relaxation or
+    # forcing only (no mass fluxes are formed), under SSPRK continuity! forms them
+    "np-f64-linear": dict(T="Float64", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                          bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre",
+                          dynamics="linear", nstep_advcor=2),
+    "per-f32-linear-relax": dict(T="Float32", nx=66, L_ratio=1.5, bc="periodic", diffusion="constant",
+                                 bottom_drag="linear", topography="ridge", wind_forcing_x="channel", dynamics="linear",
+                                 surface_relax=True, t_relax=2.0),
+    "np-f32-linear-ssprk2": dict(T="Float32", nx=62, L_ratio=0.9, bc="nonperiodic", α=1.0, diffusion="Smagorinsky",
+                                 topography="seamount", wind_forcing_x="double_gyre", time_scheme="SSPRK2", RKs=3,
+                                 cfl=1.2, dynamics="linear"),
+    "per-f64-linear-4ssprk3": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", time_scheme="4SSPRK3", cfl=1.0,
+                                   dynamics="linear", surface_forcing=True),
+    "per-bf16-f32-linear-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
+                                       diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                                       wind_forcing_x="channel", scale=64.0, dynamics="linear"),
 }
 
 
@@ -236,7 +252,7 @@ def test_upload_rejects_nonzero_unwritten_halo(swm, gpu):
 
 
 def test_unsupported_options_fail_loudly(swm, gpu):
-    for kw in (dict(nstep_advcor=1), dict(dynamics="linear")):
+    for kw in (dict(nstep_advcor=1),):
         P = swm.Parameter(nx=40, **kw)
         S = swm.make_setup(P)
         with pytest.raises(Exception):

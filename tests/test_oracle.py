@@ -386,3 +386,59 @@ def test_pv_advection_conservation(swm, orc, bc):
     S = swm.make_setup(swm.Parameter(T="Float64", nx=nx, L_ratio=nx / ny, bc=bc, adv_scheme="Sadourny"))
     qhv, qhu = orc.pv_advection(swm.make_config(S), np.full_like(q, 3.0), np.full_like(U, 2.0), np.full_like(V, -5.0))
     assert np.all(qhv == -15.0) and np.all(qhu == 6.0)
+
+
+# ------------------------------------------------------------------ linear dynamics (rhs.jl:55-86)
+
+@pytest.mark.parametrize("bc", ["periodic", "nonperiodic"])
+def test_linear_dynamics_closed_forms(swm, orc, bc):
+    """rhs_linear!: du = f_u·Ixy(v) - ∂x(gη) + Fx, dv = -f_v·Ixy(u) - ∂y(gη) + Fy on the
+    β-plane.  A uniform v (u = η = 0) gives du = f_u·V0 exactly away from the walls, a uniform
+    u gives dv = -f_v·U0 exactly, a tilted η gives the pressure gradient -g·a; continuity!
+    under the classic RK scheme reads the zero mass fluxes it was allocated with (rhs_linear!
+    forms no U, V), so dη = 0, while SSPRK2's continuity! forms them and moves η."""
+    P = swm.Parameter(T="Float64", nx=24, L_ratio=1.5, bc=bc, dynamics="linear", topography="flat",
+                      wind_forcing_x="none", wind_forcing_y="none", Fx0=0.0)
+    S = swm.make_setup(P)
+    G = S.grid
+    assert G.β != 0 and np.ptp(G.f_u) > 0
+
+    def rk1(u, v, e, P=P, S=S):
+        prog = swm.PrognosticVars(*swm.add_halo(u, v, e, np.zeros((G.nx, G.ny)), S, T="Float64"), T="Float64")
+        r = orc.run(S, swm.make_config(S), prog, 1, stop_stage=1, want_rk=True)
+        return r[4][6:9]
+
+    z = lambda m, n: np.zeros((m, n))
+    V0, U0 = 0.41, -0.23
+    du, dv, de = rk1(z(G.nux, G.nuy), np.full((G.nvx, G.nvy), V0), z(G.nx, G.ny))
+    assert np.array_equal(du[2:-2, 3:-3], G.f_u[:, 1:-1] * V0)          # rows 2..ny-1: interior v either side
+    assert np.all(dv == 0) and np.all(de == 0)
+    # v = j/2 on v row j (y = j): Ixy!(v) on u row j (y = j - 1/2) is j/2 - 1/4 exactly
+    jv = np.arange(1, G.nvy + 1)[None, :] * 0.5 + np.zeros((G.nvx, 1))
+    du, _, _ = rk1(z(G.nux, G.nuy), jv, z(G.nx, G.ny))
+    ju = np.arange(1, G.nuy + 1)[None, :]
+    assert np.array_equal(du[2:-2, 3:-3], (G.f_u * (0.5 * ju - 0.25))[:, 1:-1])
+    if bc == "nonperiodic":                                             # u = x/2: Ixy!(u) at x - 1/2
+        iu = np.arange(1, G.nux + 1)[:, None] * 0.5 + np.zeros((1, G.nuy))
+        _, dv, _ = rk1(iu, z(G.nvx, G.nvy), z(G.nx, G.ny))
+        iv = np.arange(1, G.nvx + 1)[:, None]
+        assert np.array_equal(dv[2:-3, 2:-2], -(G.f_v * (0.5 * iv - 0.25))[:-1])
+    du, dv, de = rk1(np.full((G.nux, G.nuy), U0), z(G.nvx, G.nvy), z(G.nx, G.ny))
+    c = slice(0, None) if bc == "periodic" else slice(1, -1)            # next to a wall u_v sees u = 0
+    assert np.array_equal(dv[2:-2, 2:-2][c], -(G.f_v * U0)[c]) and np.all(de == 0)
+    assert np.all(du == 0)
+    a, b = 1.7e-3, -0.6e-3
+    i, j = np.meshgrid(np.arange(G.nx, dtype=float), np.arange(G.ny, dtype=float), indexing="ij")
+    du, dv, de = rk1(z(G.nux, G.nuy), z(G.nvx, G.nvy), a * i + b * j)
+    g = S.constants.g
+    seam = 1 if bc == "periodic" else 0                                  # the periodic u column x = 0 spans the wrap
+    assert np.allclose(du[2 + seam:-2, 2:-2], -g * a, rtol=1e-9, atol=0)
+    assert np.allclose(dv[2:-2, 2:-2], -g * b, rtol=1e-9, atol=0)
+    assert np.all(de == 0)
+    # SSPRK2 forms U, V in continuity!: a uniform u over a tilted η converges mass
+    P2 = swm.Parameter(T="Float64", nx=24, L_ratio=1.5, bc=bc, dynamics="linear", topography="flat",
+                       wind_forcing_x="none", wind_forcing_y="none", Fx0=0.0, time_scheme="SSPRK2", RKs=2)
+    S2 = swm.make_setup(P2)
+    r = orc.run(S2, swm.make_config(S2), swm.PrognosticVars(*swm.add_halo(
+        np.full((G.nux, G.nuy), U0), z(G.nvx, G.nvy), a * i, np.zeros((G.nx, G.ny)), S2, T="Float64"), T="Float64"), 1)
+    assert np.any(r[2][1:-1, 1:-1] != a * i)
