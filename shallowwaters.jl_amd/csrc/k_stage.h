@@ -70,6 +70,7 @@ template <class F> struct StageArgs {
     uint32_t o1_u, o1_v, o1_e;        // first output (u1 / u1')
     uint32_t o2_u, o2_v, o2_e;        // second output (u0 / u1'')
     uint32_t qend;                    // periodic: q at corner x=nx per row (AHβ/AHγ wrap), 1-D
+    uint32_t qs, ks;                  // nstep_advcor > 0: q (corners) and KEu+KEv (T points) kept, 2-D T
     StaticRefs st;
     uint32_t c_u, c_v, c_e;           // compensations of the compensated RK modes (Tprog)
     Coef<Q> k;                        // stage coefficients (first stage of a fused pair)
@@ -105,9 +106,11 @@ __device__ __forceinline__ Q continuity_rhs(R dUdx, R dVdy, R e, int cont, R cs,
 // T-row y and (periodic) q at the corner (nx, y).  A load that is waited for in the row that
 // issues it would also wait for every load issued before it (the counter is in order).
 // The linear dynamics read f on the u-grid at row y and on the v-grid at row y-1 instead of
-// f_q (fu, fv; column offsets vfu, vfv), and no q column.
+// f_q (fu, fv; column offsets vfu, vfv), and no q column.  With ADV_KEEP the kept q at the
+// corner (x, y) and KEu+KEv at the T point (x, y) are read (qs, ks), and the periodic wrap
+// column from the kept q at x = nx.
 template <class R> struct RowStatics {
-    R fq, Fx, Fy, cs, qend, fu, fv;
+    R fq, Fx, Fy, cs, qend, fu, fv, qs, ks;
 };
 
 template <class F, bool PER, bool CS, int ADV = ADV_AH>
@@ -116,18 +119,23 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
                                                                  uint32_t vfu = 0, uint32_t vfv = 0) {
     using R = typename F::R;
     using ST = typename F::ST;
+    constexpr int SCH = ADV & 3;
+    constexpr bool KEEP = (ADV & ADV_KEEP) != 0;
     const int P = A.P;
     RowStatics<R> s;
-    s.fq = ADV == ADV_LIN ? R(0) : Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
+    s.fq = (SCH == ADV_LIN || KEEP) ? R(0) : Fld<ST>{&ar, A.st.fq}.template ldr<R>(vfq, srowb<ST>(y, P, A.st.fq_2d));
     s.Fx = Fld<ST>{&ar, A.st.Fx}.template ldr<R>(vFx, srowb<ST>(y, P, A.st.Fx_2d));
     s.Fy = Fld<ST>{&ar, A.st.Fy}.template ldr<R>(vFy, srowb<ST>(y - 1, P, A.st.Fy_2d));
     s.cs = (!CS || A.cont == CONT_ITSELF)
                ? R(0)
                : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
-    s.qend = (PER && ADV == ADV_AH) ? Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST))
-                                    : R(0);
-    s.fu = ADV == ADV_LIN ? Fld<ST>{&ar, A.st.fu}.template ldr<R>(vfu, srowb<ST>(y, P, A.st.fu_2d)) : R(0);
-    s.fv = ADV == ADV_LIN ? Fld<ST>{&ar, A.st.fv}.template ldr<R>(vfv, srowb<ST>(y - 1, P, A.st.fv_2d)) : R(0);
+    s.qend = !(PER && SCH == ADV_AH) ? R(0)
+             : KEEP ? Fld<ST>{&ar, A.qs}.template ldr<R>(colb<ST>(A.nx), rowb<ST>(y, P))
+                    : Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST));
+    s.fu = SCH == ADV_LIN ? Fld<ST>{&ar, A.st.fu}.template ldr<R>(vfu, srowb<ST>(y, P, A.st.fu_2d)) : R(0);
+    s.fv = SCH == ADV_LIN ? Fld<ST>{&ar, A.st.fv}.template ldr<R>(vfv, srowb<ST>(y - 1, P, A.st.fv_2d)) : R(0);
+    s.qs = KEEP ? Fld<ST>{&ar, A.qs}.template ldr<R>(vxT, rowb<ST>(y, P)) : R(0);
+    s.ks = KEEP ? Fld<ST>{&ar, A.ks}.template ldr<R>(vxT, rowb<ST>(y, P)) : R(0);
     return s;
 }
 
@@ -144,6 +152,8 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
 // g·η; it forms no mass fluxes, so the divergence it leaves for continuity! is zero (the
 // reference's U, V stay as allocated under the classic RK scheme, continuity.jl:83-85).
 template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
+    static constexpr int SCH = ADV & 3;
+    static constexpr bool KEEP = (ADV & ADV_KEEP) != 0;
     R u_1 = 0, u_2 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
     R U_1 = 0, U_2 = 0, V_2 = 0;
     R q_2 = 0, qaU_2 = 0, qb_2 = 0, p_2 = 0, qend_2 = 0;
@@ -157,10 +167,11 @@ template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
     // products of two shifted factors are formed unshifted and moved once:
     // q(x-1)·U(x-1) is the shift of q·U.  The values are those of the formulas as written.
     // qend_1: q at the corner (nx, r-1), read by the wrapped first column when qfix (periodic x = 0)
-    // fu_1, fv_2: f on the u-grid at row r-1 and on the v-grid at row r-2 (ADV_LIN)
+    // fu_1, fv_2: f on the u-grid at row r-1 and on the v-grid at row r-2 (ADV_LIN); qs_1,
+    // ks_1: the kept q and KEu+KEv at row r-1 (ADV_KEEP)
     __device__ __forceinline__ void push(R u_0, R v_0, R e_0, R H_0, R fq_1, R qend_1, bool qfix, R half, R quarter,
-                                         R k12, R g, R halfp, R fu_1 = 0, R fv_2 = 0) {
-        if constexpr (ADV == ADV_LIN) {
+                                         R k12, R g, R halfp, R fu_1 = 0, R fv_2 = 0, R qs_1 = 0, R ks_1 = 0) {
+        if constexpr (SCH == ADV_LIN) {
             // ∂x!(dpdx,g*η), ∂y!(dpdy,g*η) ; Ixy!(v_u,v), fv! at row r-1 ; Ixy!(u_v,u), fu! at row r-2
             const R p_1 = g * e_1;
             qhv = fu_1 * (quarter * (v_2 + sh_xp(v_2)) + quarter * (v_1 + sh_xp(v_1)));
@@ -179,18 +190,26 @@ template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
         const R U_0 = u_0 * (half * hs_0);
         // row r-1: Iy!(h_v), Vflux!  rhs.jl:294-304
         const R V_1 = v_1 * (half * (h_0 + h_1));
-        // Ixy!(h_q,h)  interpolations.jl:33-45
-        const R hq_1 = quarter * hs_1 + quarter * hs_0;
-        // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
-        const R dvdx_1 = sh_xp(v_1) - v_1;
-        const R dudy_1 = u_0 - u_1;
-        const R q_1 = (fq_1 + dvdx_1 - dudy_1) / hq_1;
-        // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109 ; bernoulli!  rhs.jl:162-180
-        const R uu_1 = u_1 * u_1;
-        const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
-        const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
-        const R p_1 = halfp * (KEu_1 + KEv_1) + g * e_1;
-        if constexpr (ADV == ADV_SAD) {
+        R q_1, ke_1;
+        if constexpr (KEEP) {
+            q_1 = qs_1;
+            ke_1 = ks_1;
+        } else {
+            // Ixy!(h_q,h)  interpolations.jl:33-45
+            const R hq_1 = quarter * hs_1 + quarter * hs_0;
+            // ∂x!(dvdx,v), ∂y!(dudy,u)  gradients.jl:3-25 ; PV!  PV_advection.jl:2-18
+            const R dvdx_1 = sh_xp(v_1) - v_1;
+            const R dudy_1 = u_0 - u_1;
+            q_1 = (fq_1 + dvdx_1 - dudy_1) / hq_1;
+            // speed!, Ix!(KEu), Iy!(KEv)  rhs.jl:107-109
+            const R uu_1 = u_1 * u_1;
+            const R KEu_1 = half * (uu_1 + sh_xm(uu_1));
+            const R KEv_1 = half * (v_1 * v_1 + v_2 * v_2);
+            ke_1 = KEu_1 + KEv_1;
+        }
+        // bernoulli!  rhs.jl:162-180
+        const R p_1 = halfp * ke_1 + g * e_1;
+        if constexpr (SCH == ADV_SAD) {
             // qhv = q_u·V_u at row r-1 (q_u = Iy!(q), V_u = Ixy!(V)) and qhu = q_v·U_v at row
             // r-2 (q_v = Ix!(q), U_v = Ixy!(U)); lane x holds the q corner x, the V point
             // x-1/2 and the U point x
@@ -332,7 +351,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<F> &A, int w) {
             if (MI::need_pr) npv = pr_v.ld(vxQ, sQ - dQ);
             if (MI::need_ac) nav = ac_v.ld(vxQ, sQ - dQ);
         }
-        rhs.push(u_0, v_0, e_0, H_0, sa.fq, sa.qend, qfix, half, quarter, k12, A.g, A.halfp, sa.fu, sa.fv);
+        rhs.push(u_0, v_0, e_0, H_0, sa.fq, sa.qend, qfix, half, quarter, k12, A.g, A.halfp, sa.fu, sa.fv, sa.qs, sa.ks);
 
         // du at row r-1: momentum_u!  rhs.jl:242
         const int yu = r - 1;
@@ -664,6 +683,53 @@ __global__ void __launch_bounds__(256) continuity_kernel(StageArgs<F> A) {
         row(r, sQ, sT);
         row(r + 1, sQ + dQ, sT + dT);
     }
+}
+
+// ------------------------------------------------------------------ kept advection terms
+//
+// nstep_advcor > 0 (time_integration.jl:204-211, rhs.jl:36-38): q and KEu+KEv, the part of
+// the Bernoulli potential the kinetic energy makes, are formed from a state (u, v, η) in
+// format SI and kept for the stages of the next steps.  One thread per T point (x, y) of
+// the η grid with its ghost ring (x = 0..nx+1, y = 0..ny+1) forms KEu(x,y) + KEv(x,y) as
+// bernoulli! reads them (Ix!(KEu,u²), Iy!(KEv,v²), rhs.jl:107-109), and for x <= nx, y <= ny
+// also q at the corner (x, y) (PV!, PV_advection.jl:2-18, with Ixy!(h_q,h)); the same
+// operations in the same order as the stage kernels' right-hand side.  WQ: also q (the
+// update of advection_coriolis!; bottom_drag_quadratic! refreshes KEu, KEv alone,
+// bottom_drag.jl:25-38).  INIT: the thickness of the initial state, thickness!(h,η,H) on the
+// Tprog η (time_integration.jl:21): η + H in the wider of T and Tprog, rounded to T.
+template <class F, class SI, bool WQ, bool INIT>
+__global__ void __launch_bounds__(256) advcor_kernel(StageArgs<F> A) {
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x), y = (int)blockIdx.y;
+    if (x > A.nx + 1 || y > A.ny + 1) return;
+    const Arena ar(A.ar);
+    const Fld<SI> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e};
+    const Fld<ST> H{&ar, A.st.H}, fq{&ar, A.st.fq}, qs{&ar, A.qs}, ks{&ar, A.ks};
+    const R half = R(0.5), quarter = R(0.25);
+    const int P = A.P;
+    const uint32_t cm = colb<SI>(x - 1), c0 = colb<SI>(x), r0 = rowb<SI>(y, P), rm = rowb<SI>(y - 1, P);
+    const R u0 = u.template ldr<R>(c0, r0), um = u.template ldr<R>(cm, r0);
+    const R v0 = v.template ldr<R>(c0, r0), vm = v.template ldr<R>(c0, rm);
+    const R uu0 = u0 * u0, uum = um * um;
+    const R KEu = half * (uu0 + uum);
+    const R KEv = half * (v0 * v0 + vm * vm);
+    ks.st(KEu + KEv, colb<ST>(x), rowb<ST>(y, P));
+    if (!WQ || x > A.nx || y > A.ny) return;
+    const uint32_t c1 = colb<SI>(x + 1), r1 = rowb<SI>(y + 1, P);
+    const uint32_t t0 = colb<ST>(x), t1 = colb<ST>(x + 1), q0 = rowb<ST>(y, P), q1 = rowb<ST>(y + 1, P);
+    auto thick = [&](uint32_t ce, uint32_t re, uint32_t ch, uint32_t rh) -> R {
+        if constexpr (INIT) return toR<R>(e.template ldr<Q>(ce, re) + toQ<Q>(H.template ldr<R>(ch, rh)));
+        else return e.template ldr<R>(ce, re) + H.template ldr<R>(ch, rh);
+    };
+    const R h00 = thick(c0, r0, t0, q0), h10 = thick(c1, r0, t1, q0);
+    const R h01 = thick(c0, r1, t0, q1), h11 = thick(c1, r1, t1, q1);
+    const R hq = quarter * (h10 + h00) + quarter * (h11 + h01);
+    const R dvdx = v.template ldr<R>(c1, r0) - v0;
+    const R dudy = u.template ldr<R>(c0, r1) - u0;
+    const R f = fq.template ldr<R>(scolb<ST>(x, A.st.fq_2d), srowb<ST>(y, P, A.st.fq_2d));
+    qs.st((f + dvdx - dudy) / hq, t0, q0);
 }
 
 // q at the corner column x = nx for every corner row y = 0..ny (PV!, PV_advection.jl:15):

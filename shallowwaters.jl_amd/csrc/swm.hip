@@ -126,6 +126,8 @@ struct swm_ctx {
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
     Field f_u, f_v;                                          // T: f on the u-, v-grid (linear dynamics)
+    Field qs, ks;                                            // T: kept q and KEu+KEv (nstep_advcor > 0)
+    bool advcor_pending = false;   // a new state: q, KEu+KEv to be formed from it before its first stage
     Field cu, cv, ce;                                        // Kahan compensations (compensated RK), Tprog
     double sc = 1.0, sci = 1.0;                              // u, v carried as sc·u, sc·v
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0, fu_sx = 1, fu_sy = 0, fv_sx = 1, fv_sy = 0;
@@ -259,6 +261,15 @@ inline int nblocks_waves(int nwaves) {
     return (nb + NXCD - 1) / NXCD * NXCD;
 }
 
+// the right-hand side's form: the PV scheme or the linear dynamics, and whether q and the
+// kinetic part of the Bernoulli potential are kept between updates (nstep_advcor > 0 with the
+// nonlinear dynamics, rhs.jl:36-38, time_integration.jl:208)
+inline int adv_form(const swm_config &c) {
+    if (c.dynamics == SWM_LINEAR) return ADV_LIN;
+    const int s = c.adv_scheme == SWM_ADV_SADOURNY ? ADV_SAD : ADV_AH;
+    return c.nstep_advcor > 0 ? (s | ADV_KEEP) : s;
+}
+
 // a T-format scalar the host already rounded (config values marked (T)) as the kernel's R
 template <class R> R mkR(double d) {
     if constexpr (std::is_same_v<R, double> || std::is_same_v<R, float>) return (R)d;
@@ -338,6 +349,7 @@ template <class F> struct Launch {
         A.o1_u = ou(ctx, o1); A.o1_v = ov(ctx, o1); A.o1_e = oe(ctx, o1);
         A.o2_u = ou(ctx, o2); A.o2_v = ov(ctx, o2); A.o2_e = oe(ctx, o2);
         A.qend = ctx->qend.ref();
+        A.qs = ctx->qs.ref(); A.ks = ctx->ks.ref();
         A.st = statics(ctx);
         A.k = k;
         A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
@@ -391,7 +403,7 @@ template <class F> struct Launch {
             ctx->rec->add(0, m, -1, &A, sizeof A);
             return SWM_OK;
         }
-        const int adv = c.dynamics == SWM_LINEAR ? ADV_LIN : c.adv_scheme == SWM_ADV_SADOURNY ? ADV_SAD : ADV_AH;
+        const int adv = adv_form(c);
         if (per && adv == ADV_AH) {
             const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
             if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
@@ -405,6 +417,8 @@ template <class F> struct Launch {
     case M_:                                                                                               \
         if (adv == ADV_SAD) { SWM_KA(M_, ADV_SAD) }                                                        \
         else if (adv == ADV_LIN) { SWM_KA(M_, ADV_LIN) }                                                   \
+        else if (adv == (ADV_AH | ADV_KEEP)) { SWM_KA(M_, ADV_AH | ADV_KEEP) }                             \
+        else if (adv == (ADV_SAD | ADV_KEEP)) { SWM_KA(M_, ADV_SAD | ADV_KEEP) }                           \
         else { SWM_KA(M_, ADV_AH) }                                                                        \
         break;
         if constexpr (F::mixed) {
@@ -426,11 +440,24 @@ template <class F> struct Launch {
         return SWM_OK;
     }
 
+    // q and KEu+KEv kept for nstep_advcor > 0, formed from the state in slots s (Tprog
+    // storage): wq also q; init with the initial state's thickness
+    static int advcor(swm_ctx *ctx, Slots s, bool wq, bool init) {
+        const swm_config &c = ctx->c;
+        const Times T{mkR<R>(1.0), mkR<R>(1.0), mkR<R>(0.0)};
+        const StageArgs<F> A = args(ctx, s, Slots{}, Slots{}, Slots{}, Slots{}, Coef<Q>{}, T);
+        const dim3 g((c.nx + 2 + 255) / 256, c.ny + 2), b(256);
+        if (init) hipLaunchKernelGGL((advcor_kernel<F, Q, true, true>), g, b, 0, ctx->stream, A);
+        else if (wq) hipLaunchKernelGGL((advcor_kernel<F, Q, true, false>), g, b, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((advcor_kernel<F, Q, false, false>), g, b, 0, ctx->stream, A);
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
     static bool fusable(const swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && c.adv_scheme == SWM_ADV_ARAKAWAHSU &&
-               c.dynamics == SWM_NONLINEAR;
+        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -510,6 +537,11 @@ template <class F> struct Launch {
     static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1, bool keep_u1 = false) {
         const swm_config &c = ctx->c;
         if (n_stages > total_stages(c)) n_stages = total_stages(c);
+        if (ctx->advcor_pending) {                           // advection_coriolis! on the initial state (:20-31)
+            int rc = advcor(ctx, Slots{ctx->pu, ctx->pe}, true, true);
+            if (rc) return rc;
+            ctx->advcor_pending = false;
+        }
         const Times T = forcing_times(ctx, t);
         const Slots P{ctx->pu, ctx->pe}, none{};
         Slots cur = P, z0 = P;
@@ -703,6 +735,11 @@ template <class F> struct Launch {
         Slots u0, u1;
         int rc = stages(ctx, 1 << 20, t, &u0, &u1);
         if (rc) return rc;
+        if (adv_form(c) & ADV_KEEP) {                        // :208-211, and bottom_drag_quadratic!'s KEu, KEv
+            const bool due = i % c.nstep_advcor == 0;
+            if (due || (c.bottom_drag == SWM_DRAG_QUADRATIC && i % c.nstep_diff == 0))
+                if ((rc = advcor(ctx, u0, due, false))) return rc;
+        }
         int new_uv = u0.uv;
         if (i % c.nstep_diff == 0) {                         // :216-221
             const int out = pick({ctx->pu, u0.uv});
@@ -796,7 +833,7 @@ int check_config(swm_ctx *ctx) {
     if (c.time_scheme == SWM_RK && (c.RKo < 2 || c.RKo > 4)) return ctx->fail(SWM_E_ARG, "RKo %d", c.RKo);
     if (c.time_scheme == SWM_SSPRK2 && c.RKs < 2) return ctx->fail(SWM_E_ARG, "RKs %d", c.RKs);
     if (c.time_scheme == SWM_SSPRK3 && (c.ssp_n < 2 || c.ssp_s != c.ssp_n * c.ssp_n)) return ctx->fail(SWM_E_ARG, "RKn %d", c.ssp_n);
-    if (c.nstep_advcor != 0 && c.dynamics == SWM_NONLINEAR) return ctx->fail(SWM_E_UNSUPPORTED, "nstep_advcor > 0 is not implemented");
+    if (c.nstep_advcor < 0) return ctx->fail(SWM_E_ARG, "nstep_advcor %d", c.nstep_advcor);
     if (c.scale_log2 < -20 || c.scale_log2 > 20) return ctx->fail(SWM_E_ARG, "scale_log2 %d out of -20..20", c.scale_log2);
     if (c.compensated && c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "compensated needs the classic RK scheme");
     if (c.adv_scheme != SWM_ADV_ARAKAWAHSU && c.adv_scheme != SWM_ADV_SADOURNY) return ctx->fail(SWM_E_ARG, "adv_scheme %d", c.adv_scheme);
@@ -881,7 +918,7 @@ std::vector<Field *> all_fields(swm_ctx *ctx) {
         f.push_back(&ctx->sst[s]); f.push_back(&ctx->tu[s]); f.push_back(&ctx->tv[s]); f.push_back(&ctx->te[s]);
     }
     for (Field *g : {&ctx->um, &ctx->vm, &ctx->H, &ctx->Fx, &ctx->Fy, &ctx->fq, &ctx->eta_ref, &ctx->Feta, &ctx->qend,
-                     &ctx->cu, &ctx->cv, &ctx->ce})
+                     &ctx->cu, &ctx->cv, &ctx->ce, &ctx->f_u, &ctx->f_v, &ctx->qs, &ctx->ks})
         f.push_back(g);
     return f;
 }
@@ -940,8 +977,9 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (c.adv_scheme != SWM_ADV_ARAKAWAHSU || c.dynamics != SWM_NONLINEAR)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear Arakawa-Hsu right-hand side only");
+        if (adv_form(c) != ADV_AH)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear Arakawa-Hsu right-hand side only "
+                                                "(nstep_advcor = 0)");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -1798,7 +1836,8 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     // the arena: Tprog fields 12 state slots + 2 sst; T fields um, vm, H, Fx, Fy, f_q
     // (+ η_ref, Fη; + 6 u1 slots with mixed formats) and the q column
     const int nq = 14 + (c.compensated ? 3 : 0);
-    const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0) + (c.dynamics == SWM_LINEAR ? 2 : 0);
+    const int nt = 6 + (c.continuity != SWM_CONT_ITSELF ? 2 : 0) + (ctx->mixed ? 6 : 0) + (c.dynamics == SWM_LINEAR ? 2 : 0) +
+                   ((adv_form(c) & ADV_KEEP) ? 2 : 0);
     const size_t n1d = (size_t)c.ny + 2 * OY + 16;
     ctx->arena_bytes = (size_t)nq * (ctx->elems * ctx->es + 256) + (size_t)nt * (ctx->elems * ctx->est + 256) +
                        n1d * ctx->est + 256;
@@ -1836,6 +1875,9 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         return bail(rc);
     if (c.dynamics == SWM_LINEAR &&
         ((rc = alloc_field(ctx, ctx->f_u, 0, true, est)) || (rc = alloc_field(ctx, ctx->f_v, 0, true, est))))
+        return bail(rc);
+    if ((adv_form(c) & ADV_KEEP) &&
+        ((rc = alloc_field(ctx, ctx->qs, 0, true, est)) || (rc = alloc_field(ctx, ctx->ks, 0, true, est))))
         return bail(rc);
     void *p = nullptr;
     if (hipMalloc(&p, 4 * sizeof(int)) != hipSuccess) return bail(SWM_E_HIP);
@@ -1971,6 +2013,7 @@ int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta,
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_prog = true;
+    ctx->advcor_pending = (adv_form(c) & ADV_KEEP) != 0;
     return SWM_OK;
 }
 
@@ -2098,7 +2141,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         else if (c.T != c0.T || c.Tprog != c0.Tprog || c.Tcomm != c0.Tcomm) why = "number formats differ";
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
-        else if (c.adv_scheme != SWM_ADV_ARAKAWAHSU || c.dynamics != SWM_NONLINEAR) why = "ensembles run the nonlinear Arakawa-Hsu right-hand side only";
+        else if (adv_form(c) != ADV_AH) why = "ensembles run the nonlinear Arakawa-Hsu right-hand side only (nstep_advcor = 0)";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
         else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";

@@ -313,8 +313,10 @@ struct StaticRefs {
     int fu_2d, fv_2d;
 };
 
-// PV advection / Coriolis form of the right-hand side (rhs.jl:2-16, PV_advection.jl:21-27)
-enum { ADV_AH = 0, ADV_SAD = 1, ADV_LIN = 2 };
+// PV advection / Coriolis form of the right-hand side (rhs.jl:2-16, PV_advection.jl:21-27);
+// ADV_KEEP (with ADV_AH or ADV_SAD): q and the kinetic part of the Bernoulli potential are
+// read from the last update instead of formed (nstep_advcor > 0, rhs.jl:36-38)
+enum { ADV_AH = 0, ADV_SAD = 1, ADV_LIN = 2, ADV_KEEP = 4 };
 
 // byte offset of (x, y) in a static field that may be stored 1-D (one value per row)
 template <class S> __device__ __forceinline__ uint32_t scolb(int x, int is2d) { return is2d ? colb<S>(x) : 0u; }

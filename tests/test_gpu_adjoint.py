@@ -135,7 +135,8 @@ def test_rrule_of_time_integration(swm, gpu):
 
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
-    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", adv_scheme="Sadourny")):
+    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", adv_scheme="Sadourny"),
+               dict(T="Float32", dynamics="linear"), dict(T="Float32", nstep_advcor=2)):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
             sim.upload(prog)

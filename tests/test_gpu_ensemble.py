@@ -88,7 +88,8 @@ def test_tuned_ensemble_in_two_calls(swm, gpu, bc, T, Tprog):
             sim.close()
 
 
-@pytest.mark.parametrize("kb", [dict(bc="nonperiodic"), dict(bc="periodic", adv_scheme="Sadourny")])
+@pytest.mark.parametrize("kb", [dict(bc="nonperiodic"), dict(bc="periodic", adv_scheme="Sadourny"),
+                                dict(bc="periodic", dynamics="linear"), dict(bc="periodic", nstep_advcor=2)])
 def test_ensemble_rejects_mixed_options(swm, gpu, kb):
     """Members must share formats, boundary condition and scheme; the batched kernels are of
     the Arakawa-Hsu scheme, so a Sadourny member is refused."""

@@ -116,6 +116,27 @@ CASES = {
     "per-bf16-f32-linear-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
                                        diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
                                        wind_forcing_x="channel", scale=64.0, dynamics="linear"),
+    # advection and Coriolis terms kept between updates every nstep_advcor steps
+    # (time_integration.jl:204-211; the quadratic drag refreshes the kinetic part each
+    # diffusion step)
+    "np-f64-advcor3": dict(T="Float64", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
+                           bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre",
+                           nstep_advcor=3),
+    "per-f64-advcor1": dict(T="Float64", nx=72, L_ratio=1.1, bc="periodic", α=1.0, diffusion="Smagorinsky",
+                            bottom_drag="quadratic", topography="seamount", wind_forcing_x="double_gyre",
+                            nstep_advcor=1),
+    "per-f32-advcor2-sadourny": dict(T="Float32", nx=66, L_ratio=1.5, bc="periodic", diffusion="constant",
+                                     bottom_drag="linear", topography="ridge", wind_forcing_x="channel",
+                                     adv_scheme="Sadourny", nstep_advcor=2),
+    "np-f32-advcor4-ssprk2": dict(T="Float32", nx=62, L_ratio=0.9, bc="nonperiodic", α=1.0, diffusion="Smagorinsky",
+                                  topography="seamount", wind_forcing_x="double_gyre", time_scheme="SSPRK2", RKs=3,
+                                  cfl=1.2, nstep_advcor=4, nstep_diff=2),
+    "np-f16-f32-advcor2": dict(T="Float16", Tprog="Float32", nx=64, L_ratio=1.0, bc="nonperiodic", α=0.5,
+                               diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
+                               wind_forcing_x="double_gyre", nstep_advcor=2),
+    "per-bf16-f32-advcor3-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
+                                        diffusion="Smagorinsky", bottom_drag="none", topography="seamount",
+                                        wind_forcing_x="channel", scale=64.0, nstep_advcor=3),
 }
 
 
@@ -252,7 +273,7 @@ def test_upload_rejects_nonzero_unwritten_halo(swm, gpu):
 
 
 def test_unsupported_options_fail_loudly(swm, gpu):
-    for kw in (dict(nstep_advcor=1),):
+    for kw in (dict(T="Float16", Tprog="Float16"), dict(T="BFloat16", Tprog="Float32", time_scheme="SSPRK2")):
         P = swm.Parameter(nx=40, **kw)
         S = swm.make_setup(P)
         with pytest.raises(Exception):

@@ -442,3 +442,37 @@ def test_linear_dynamics_closed_forms(swm, orc, bc):
     r = orc.run(S2, swm.make_config(S2), swm.PrognosticVars(*swm.add_halo(
         np.full((G.nux, G.nuy), U0), z(G.nvx, G.nvy), a * i, np.zeros((G.nx, G.ny)), S2, T="Float64"), T="Float64"), 1)
     assert np.any(r[2][1:-1, 1:-1] != a * i)
+
+
+# ------------------------------------------------------------------ nstep_advcor > 0 (rhs.jl:36-38, time_integration.jl:204-211)
+
+def test_nstep_advcor_schedule(swm, orc):
+    """With nstep_advcor = N the stages read q and KEu, KEv as last formed: from the initial
+    state (time_integration.jl:30), after the stages of every N-th step (:208-211), and KEu,
+    KEv again by bottom_drag_quadratic! every diffusion step (bottom_drag.jl:35-38, the same
+    arrays).  Exact consequences: stage 1 of step 1 equals the every-stage form; N = 2 and
+    N = 7 agree through step 2 and N = 1 does not; a quadratic drag with cD = 0 (no force)
+    differs from no drag only through the KE refresh, so only when N > 1."""
+    base = dict(T="Float64", nx=40, L_ratio=1.25, bc="nonperiodic", topography="seamount",
+                wind_forcing_x="double_gyre", diffusion="Smagorinsky", α=0.5, tracer_advection=False)
+    def run(nsteps, stop_stage=0, **kw):
+        P = swm.Parameter(**{**base, **kw})
+        S = swm.make_setup(P)
+        prog = swm.seeded_initial_conditions(S, seed=9, σ=0.6)
+        return orc.run(S, swm.make_config(S), prog, nsteps, stop_stage=stop_stage, want_rk=stop_stage > 0)
+
+    d0 = run(1, stop_stage=1, nstep_advcor=0)[4][6:9]
+    d5 = run(1, stop_stage=1, nstep_advcor=5)[4][6:9]
+    assert all(np.array_equal(a, b) for a, b in zip(d0, d5))
+    d0 = run(1, stop_stage=2, nstep_advcor=0)[4][6:9]
+    d5 = run(1, stop_stage=2, nstep_advcor=5)[4][6:9]
+    assert not np.array_equal(d0[0], d5[0])                             # stage 2 reads the kept q
+    same = lambda a, b: all(np.array_equal(x, y) for x, y in zip(a[:3], b[:3]))
+    r1, r2, r7 = (run(2, nstep_advcor=n) for n in (1, 2, 7))
+    assert same(r2, r7) and not same(r1, r7)
+    assert not same(run(1, nstep_advcor=0), run(1, nstep_advcor=7))
+    assert same(run(1, nstep_advcor=1), run(1, nstep_advcor=7))         # the update follows the stages
+    for n, differ in ((0, False), (1, False), (3, True)):
+        a = run(2, nstep_advcor=n, bottom_drag="quadratic", cD=0.0)
+        b = run(2, nstep_advcor=n, bottom_drag="none")
+        assert same(a, b) != differ, n
