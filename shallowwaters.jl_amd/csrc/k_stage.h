@@ -151,7 +151,7 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
 // the Coriolis terms f·v, f·u of the bilinearly interpolated velocities and the gradient of
 // g·η; it forms no mass fluxes, so the divergence it leaves for continuity! is zero (the
 // reference's U, V stay as allocated under the classic RK scheme, continuity.jl:83-85).
-template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
+template <class R, bool PER, int ADV = ADV_AH, bool QOWN = false> struct RhsPipe {
     static constexpr int SCH = ADV & 3;
     static constexpr bool KEEP = (ADV & ADV_KEEP) != 0;
     R u_1 = 0, u_2 = 0, v_1 = 0, v_2 = 0, e_1 = 0, h_1 = 0, hs_1 = 0;
@@ -229,13 +229,20 @@ template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
         // AHα!..AHδ!  PV_advection.jl:88-146 at row r-1 (β, γ wrap to q[end,·] at x=0).
         // The x-1 neighbours are moved before the per-lane choice of the wrapped column: a
         // lane move inside a branch taken by some lanes only would read 0 from the others.
+        // The wrapped β, γ of the first column are read by qhv at the u column 0 only; qhu
+        // reads β, γ at the v columns 1..nx, unwrapped (the v column nx too, whose lane also
+        // stands for x = 0 when QOWN).  QOWN: q[end,·] is the lane's own q (a march whose
+        // lanes wrap around the periodic seam, where q at the corner nx equals q at 0).
         const R s_1 = q_2 + q_1;
         const R qxm_1 = sh_xm(q_1), qxm_2 = sh_xm(q_2);
         const R qa_1 = k12 * (s_1 + sh_xp(q_1));
-        const R qb_1 = k12 * (s_1 + ((PER && qfix) ? qend_1 : qxm_1));
+        const R qbn_1 = k12 * (s_1 + qxm_1);
         // the wrapped first column of AHγ! sums (q[1,j] + q[1,j+1]) + q[end,j] (:126), the
         // other columns (q[i,j] + q[i-1,j]) + q[i,j+1] (:128)
-        const R qg_1 = (PER && qfix) ? k12 * (s_1 + qend_2) : k12 * ((q_2 + qxm_2) + q_1);
+        const R qgn_1 = k12 * ((q_2 + qxm_2) + q_1);
+        const R qe_1 = QOWN ? q_1 : qend_1, qe_2 = QOWN ? q_2 : qend_2;
+        const R qb_1 = (PER && qfix) ? k12 * (s_1 + qe_1) : qbn_1;
+        const R qg_1 = (PER && qfix) ? k12 * (s_1 + qe_2) : qgn_1;
         const R qd_1 = k12 * (s_1 + sh_xp(q_2));
         // PV_ArakawaHsu! qhv  PV_advection.jl:73 and ∂x!(dpdx)  at row r-1
         qhv = qa_1 * sh_xp(V_1) + qb_1 * V_1 + qg_1 * V_2 + qd_1 * sh_xp(V_2);
@@ -247,12 +254,12 @@ template <class R, bool PER, int ADV = ADV_AH> struct RhsPipe {
         // PV_ArakawaHsu! qhu  PV_advection.jl:81 and ∂y!(dpdy)  at row r-2:
         // qα(x-1)·U(x-1) of row r-2 and qδ(x-1)·U(x-1) of row r-1
         const R qaU_1 = qa_1 * U_1;
-        qhu = sh_xm(qaU_2) + qb_2 * U_2 + qg_1 * U_1 + sh_xm(qd_1 * U_1);
+        qhu = sh_xm(qaU_2) + qb_2 * U_2 + qgn_1 * U_1 + sh_xm(qd_1 * U_1);
         py = p_1 - p_2;
         // advance the row history
         u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; h_1 = h_0; hs_1 = hs_0;
         U_2 = U_1; U_1 = U_0; V_2 = V_1;
-        q_2 = q_1; qaU_2 = qaU_1; qb_2 = qb_1; p_2 = p_1;
+        q_2 = q_1; qaU_2 = qaU_1; qb_2 = qbn_1; p_2 = p_1;
         qend_2 = qend_1;
     }
 };
@@ -428,7 +435,7 @@ __global__ void __launch_bounds__(256) stage_batch(const StageArgs<F> *__restric
 constexpr int HALO2 = 4;
 constexpr int WOUT2 = WAVE - 2 * HALO2;
 
-template <class F, int MA, int MB, bool CS>
+template <class F, int MA, int MB, bool CS, bool PER = false>
 __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
@@ -442,10 +449,18 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
     const bool laneB = lane >= HALO2 && lane < WAVE - HALO2;
-    const bool u_col = laneB && x >= 1 && x <= nx - 1;
+    const bool u_col = laneB && (PER ? (x >= 0 && x <= nx - 1) : (x >= 1 && x <= nx - 1));
     const bool v_col = laneB && x >= 1 && x <= nx;
     const bool e_col = v_col;
-    const GhostLanes gl = ghost_lanes<false>(x, nx);
+    const GhostLanes gl = ghost_lanes<PER>(x, nx);
+    // Periodic: every lane computes stage A at its column taken around the seam (u columns
+    // into 0..nx-1, v and η columns into 1..nx), so the lanes beyond the seam hold the values
+    // the reference's ghost copies carry, and stage B reads them across lanes.  This needs the
+    // ghost copies to be exact (Tcomm no narrower than the conversion to T), H equal on the
+    // two sides of the seam and f_q independent of x, which the host checks.
+    const int xu = PER ? ((x % nx) + nx) % nx : x;
+    const int xt = PER ? xu + (xu == 0 ? nx : 0) : x;
+    const bool qfix = PER && xu == 0;                  // lanes of the u column 0 (AHβ/AHγ wrap)
     // waves whose stage-A lanes (2..61) reach the x-halo apply the ghost copies across lanes
     const bool xedge = __any((lane >= 2 && lane < WAVE - 2) && (x <= 0 || x >= nx + 1)) != 0;
     const bool u_halo = x <= 0 || x >= nx;             // normal-velocity halo of u: never written (0)
@@ -458,12 +473,13 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
     const Fld<ST> fH{&ar, A.st.H};
     const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
-    const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
-    const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
+    const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);                 // stores (output lanes)
+    const uint32_t vIu = colb<SI>(xu), vIt = colb<SI>(xt), vQu = colb<Q>(xu), vQt = colb<Q>(xt), vTt = colb<ST>(xt);
+    const uint32_t vfq = scolb<ST>(xu, A.st.fq_2d), vFx = scolb<ST>(xu, A.st.Fx_2d), vFy = scolb<ST>(xt, A.st.Fy_2d);
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
     const Q oma = A.oma;
 
-    RhsPipe<R, false> ra, rb;
+    RhsPipe<R, PER, ADV_AH, PER> ra, rb;
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
     // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
@@ -474,13 +490,13 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     // A's row r-2 at iteration r and emits u, η at row r-3 and v at row r-4.
     int r = t.ya - 2;
     const int rlast = t.yb + 3;
-    VI nu = in_u.ld(vxI, rowb<SI>(r, P)), nv = in_v.ld(vxI, rowb<SI>(r, P)), ne = in_e.ld(vxI, rowb<SI>(r, P));
-    R nH = fH.template ldr<R>(vxT, rowb<ST>(r, P));
+    VI nu = in_u.ld(vIu, rowb<SI>(r, P)), nv = in_v.ld(vIt, rowb<SI>(r, P)), ne = in_e.ld(vIt, rowb<SI>(r, P));
+    R nH = fH.template ldr<R>(vTt, rowb<ST>(r, P));
     Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;      // A's Prog / u0 operands (rows r-1, r-2)
     Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
     // A's statics a row ahead (f_q, Fx, η static at row r-1, Fy at r-2); B takes those A had
     // two iterations before (rows r-3, r-4)
-    RowStatics<R> nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r - 1);
+    RowStatics<R> nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vTt, r - 1);
     RowStatics<R> st_1 = {0, 0, 0, 0, 0}, st_2 = {0, 0, 0, 0, 0};
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
@@ -495,21 +511,21 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
         const RowStatics<R> sa = nst, sb = st_2;
-        nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r);
+        nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vTt, r);
         {   // the input row r+1, clamped to the last row the march reads
             const int rn = min(r + 1, rlast);
-            nu = in_u.ld(vxI, rowb<SI>(rn, P)); nv = in_v.ld(vxI, rowb<SI>(rn, P)); ne = in_e.ld(vxI, rowb<SI>(rn, P));
-            nH = fH.template ldr<R>(vxT, rowb<ST>(rn, P));
+            nu = in_u.ld(vIu, rowb<SI>(rn, P)); nv = in_v.ld(vIt, rowb<SI>(rn, P)); ne = in_e.ld(vIt, rowb<SI>(rn, P));
+            nH = fH.template ldr<R>(vTt, rowb<ST>(rn, P));
         }
         {   // A's pointwise operands for the next iteration: rows r (u, η) and r-1 (v)
-            if (MIA::need_pr) { npu = pr_u.ld(vxQ, sQ); npe = pr_e.ld(vxQ, sQ); npv = pr_v.ld(vxQ, sQ - dQ); }
-            if (MIA::need_ac) { nau = ac_u.ld(vxQ, sQ); nae = ac_e.ld(vxQ, sQ); nav = ac_v.ld(vxQ, sQ - dQ); }
+            if (MIA::need_pr) { npu = pr_u.ld(vQu, sQ); npe = pr_e.ld(vQt, sQ); npv = pr_v.ld(vQt, sQ - dQ); }
+            if (MIA::need_ac) { nau = ac_u.ld(vQu, sQ); nae = ac_e.ld(vQt, sQ); nav = ac_v.ld(vQt, sQ - dQ); }
             // B's Prog operands for the next iteration: rows r-2 (u, η) and r-3 (v)
             if (MIB::need_pr) { bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ); }
         }
 
         // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
-        ra.push(u_0, v_0, e_0, H_0, sa.fq, R(0), false, half, quarter, k12, A.g, A.halfp);
+        ra.push(u_0, v_0, e_0, H_0, sa.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp);
         Q cu1, cu0, ce1, ce0, cv1, cv0;
         {
             const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) + toQ<Q>(A.Fxt * sa.Fx);
@@ -520,16 +536,18 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) + toQ<Q>(A.Fyt * sa.Fy);
             combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
         }
-        // ghost_points!(u1, v1, η1) on A's first outputs (nonperiodic): u's normal halo is 0,
-        // its tangential rows mirror with (1-α); v's normal rows are 0, its columns mirror
-        // with (1-α); η copies its neighbour row / column.  Row 0 (first segment) copies row
-        // 1, which A has not made yet: B fixes it when it reads row 0.
-        if (u_halo) cu1 = Q(0);
-        if (r - 1 == ny + 1) cu1 = oma * U1_0;
-        if (r - 1 == ny + 2) cu1 = oma * U1_1;
+        // ghost_points!(u1, v1, η1) on A's first outputs.  Nonperiodic: u's normal halo is
+        // 0, its tangential rows mirror with (1-α); v's normal rows are 0, its columns mirror
+        // with (1-α); η copies its neighbour row / column.  Periodic: the columns beyond the
+        // seam hold the copies already, u's rows beyond the walls are never written (0).
+        // Row 0 (first segment) copies row 1, which A has not made yet: B fixes it when it
+        // reads row 0.
+        if (!PER && u_halo) cu1 = Q(0);
+        if (r - 1 == ny + 1) cu1 = PER ? Q(0) : oma * U1_0;
+        if (r - 1 == ny + 2) cu1 = PER ? Q(0) : oma * U1_1;
         if (r - 1 == ny + 1) ce1 = E1_0;
         if (r - 2 <= 0 || r - 2 >= ny) cv1 = Q(0);
-        if (xedge) {
+        if (!PER && xedge) {
             const Q vp1 = sh_xp(cv1), vm1 = sh_xm(cv1);
             const Q vp3 = __shfl(cv1, lane + 3), vm3 = __shfl(cv1, lane - 3);
             const Q ep1 = sh_xp(ce1), em1 = sh_xm(ce1);
@@ -548,9 +566,9 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         // from B's input row ya+1 on, v from ya+2 on, each reading two rows of history).
         const int rbi = r - 2;
         {
-            const Q ub = rbi == 0 ? oma * U1_0 : U1_1;          // row 0: (1-α)·u1 of row 1
+            const Q ub = rbi == 0 ? (PER ? Q(0) : oma * U1_0) : U1_1;   // row 0: (1-α)·u1 of row 1 (periodic: 0)
             const Q eb = rbi == 0 ? E1_0 : E1_1;                // row 0: η1 of row 1
-            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, R(0), false, half, quarter, k12, A.g, A.halfp);
+            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp);
             const int yu = r - 3;
             if (yu >= t.ya && yu < t.yb) {
                 if (u_col) {
@@ -559,8 +577,8 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                     if (MIB::comp) cc = cmp_u.ld(vxQ, sQ - 3 * dQ);
                     combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2, cc);
                     if (MIB::comp) cmp_u.st(cc, vxQ, sQ - 3 * dQ);
-                    if (MIB::out1) put_u<ST, false>(o1_u, x, vxT, yu, sT - 3 * dT, o1, gl.u, nx, ny, P, oma, A.cm);
-                    if (MIB::ghost2) put_u<Q, false>(o2_u, x, vxQ, yu, sQ - 3 * dQ, o2, gl.u, nx, ny, P, oma, A.cm);
+                    if (MIB::out1) put_u<ST, PER>(o1_u, x, vxT, yu, sT - 3 * dT, o1, gl.u, nx, ny, P, oma, A.cm);
+                    if (MIB::ghost2) put_u<Q, PER>(o2_u, x, vxQ, yu, sQ - 3 * dQ, o2, gl.u, nx, ny, P, oma, A.cm);
                     else o2_u.st(o2, vxQ, sQ - 3 * dQ);
                 }
                 if (e_col) {
@@ -569,8 +587,8 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                     if (MIB::comp) cc = cmp_e.ld(vxQ, sQ - 3 * dQ);
                     combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2, cc);
                     if (MIB::comp) cmp_e.st(cc, vxQ, sQ - 3 * dQ);
-                    if (MIB::out1) put_e<ST, false>(o1_e, x, vxT, yu, sT - 3 * dT, o1, gl.e, nx, ny, P, A.cm);
-                    if (MIB::ghost2) put_e<Q, false>(o2_e, x, vxQ, yu, sQ - 3 * dQ, o2, gl.e, nx, ny, P, A.cm);
+                    if (MIB::out1) put_e<ST, PER>(o1_e, x, vxT, yu, sT - 3 * dT, o1, gl.e, nx, ny, P, A.cm);
+                    if (MIB::ghost2) put_e<Q, PER>(o2_e, x, vxQ, yu, sQ - 3 * dQ, o2, gl.e, nx, ny, P, A.cm);
                     else o2_e.st(o2, vxQ, sQ - 3 * dQ);
                 }
             }
@@ -581,8 +599,8 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                 if (MIB::comp) cc = cmp_v.ld(vxQ, sQ - 4 * dQ);
                 combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2, cc);
                 if (MIB::comp) cmp_v.st(cc, vxQ, sQ - 4 * dQ);
-                if (MIB::out1) put_v<ST, false>(o1_v, x, vxT, yv, sT - 4 * dT, o1, gl.v, nx, P, oma, A.cm);
-                if (MIB::ghost2) put_v<Q, false>(o2_v, x, vxQ, yv, sQ - 4 * dQ, o2, gl.v, nx, P, oma, A.cm);
+                if (MIB::out1) put_v<ST, PER>(o1_v, x, vxT, yv, sT - 4 * dT, o1, gl.v, nx, P, oma, A.cm);
+                if (MIB::ghost2) put_v<Q, PER>(o2_v, x, vxQ, yv, sQ - 4 * dQ, o2, gl.v, nx, P, oma, A.cm);
                 else o2_v.st(o2, vxQ, sQ - 4 * dQ);
             }
         }
@@ -596,9 +614,9 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     }
 }
 
-template <class F, int MA, int MB, bool CS>
+template <class F, int MA, int MB, bool CS, bool PER = false>
 __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
-    fused_body<F, MA, MB, CS>(A, wave_index());
+    fused_body<F, MA, MB, CS, PER>(A, wave_index());
 }
 
 template <class F, int MA, int MB, bool CS>

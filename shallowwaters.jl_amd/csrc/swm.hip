@@ -132,6 +132,7 @@ struct swm_ctx {
     double sc = 1.0, sci = 1.0;                              // u, v carried as sc·u, sc·v
     int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0, fu_sx = 1, fu_sy = 0, fv_sx = 1, fv_sy = 0;
     bool have_fuv = false;   // swm_set_static_linear done
+    bool seam_ok = false;    // periodic: the fused stage pairs may wrap around the seam (set_static)
     int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
     int rsst = 0;
     Slots rk0, rk1;        // u0 / u1 after swm_stages
@@ -457,7 +458,7 @@ template <class F> struct Launch {
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
     static bool fusable(const swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        return ctx->fuse && c.bc == SWM_NONPERIODIC && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
+        return ctx->fuse && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -470,6 +471,11 @@ template <class F> struct Launch {
         const bool good = ok(in, ctx->mixed && !afirst) && ok(pr, false) && (afirst || ok(ac, false)) &&
                           (blast || ok(o1, ctx->mixed)) && ok(o2, false) && (mb != M_RKC_LAST || ctx->cu.base);
         if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
+        const bool per = ctx->c.bc == SWM_PERIODIC;
+        if ((per && !ctx->seam_ok) || adv_form(ctx->c) != ADV_AH)
+            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages need the Arakawa-Hsu right-hand side and, periodic, a "
+                                                "seam the lanes can wrap around (H equal on both sides, f_q independent "
+                                                "of x, exact ghost copies)");
         StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
         A.k2 = kb;
         A.nstrips = nstrips2(ctx);
@@ -482,7 +488,10 @@ template <class F> struct Launch {
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
 #define SWM_FK(MA_, MB_)                                                                                   \
-    if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true>), grid, block, 0, ctx->stream, A);         \
+    if (per) {                                                                                             \
+        if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true>), grid, block, 0, ctx->stream, A); \
+        else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true>), grid, block, 0, ctx->stream, A);   \
+    } else if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true>), grid, block, 0, ctx->stream, A);  \
     else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false>), grid, block, 0, ctx->stream, A)
         if (ma == M_RK_FIRST && mb == M_RK_MID) SWM_FK(M_RK_FIRST, M_RK_MID);
         else if (ma == M_RK_MID && mb == M_RK_LAST) SWM_FK(M_RK_MID, M_RK_LAST);
@@ -1926,6 +1935,21 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
             return SWM_OK;
         });
         if (rc) return rc;
+    }
+    if (c.bc == SWM_PERIODIC) {
+        // The periodic fused stage pairs compute the columns beyond the seam from the columns
+        // they copy; that gives the reference's values where the copies are exact (Tcomm at
+        // least as wide as the conversion to T needs), H's ghost columns equal the columns
+        // they face across the seam, and f_q does not vary along x.
+        const bool exact = c.Tcomm == c.T || c.Tcomm == c.Tprog || c.Tcomm == SWM_F64 ||
+                           (c.Tcomm == SWM_F32 && c.Tprog != SWM_F64);
+        const char *h = (const char *)H;
+        const size_t es = (size_t)ctx->est, m = (size_t)c.nx + 2;
+        bool same = true;
+        for (size_t j = 0; j < (size_t)c.ny + 2 && same; ++j)
+            same = std::memcmp(h + (j * m + 0) * es, h + (j * m + c.nx) * es, es) == 0 &&
+                   std::memcmp(h + (j * m + c.nx + 1) * es, h + (j * m + 1) * es, es) == 0;
+        ctx->seam_ok = exact && same && ctx->fq_sx == 0;
     }
     if (c.continuity == SWM_CONT_RELAX) {
         if (!eta_ref) return ctx->fail(SWM_E_ARG, "eta_ref is required for surface relaxation");

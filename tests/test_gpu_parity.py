@@ -116,6 +116,15 @@ CASES = {
     "per-bf16-f32-linear-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
                                        diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
                                        wind_forcing_x="channel", scale=64.0, dynamics="linear"),
+    # periodic fused stage pairs: the lanes wrap around the seam (flat bottom, exact copies)
+    "per-f64-flat-fused": dict(T="Float64", nx=70, L_ratio=1.1, bc="periodic", diffusion="Smagorinsky",
+                               bottom_drag="quadratic", topography="flat", wind_forcing_x="channel"),
+    "per-f32-flat-comp-relax": dict(T="Float32", nx=64, L_ratio=1.0, bc="periodic", topography="flat",
+                                    wind_forcing_x="shear", compensated=True, surface_relax=True, t_relax=2.0),
+    "per-f64-flat-rk2-forcing": dict(T="Float64", nx=58, L_ratio=0.9, bc="periodic", topography="flat",
+                                     wind_forcing_x="double_gyre", RKo=2, cfl=0.5, surface_forcing=True),
+    "per-f32-flat-narrow": dict(T="Float32", nx=24, L_ratio=0.5, bc="periodic", topography="flat",
+                                wind_forcing_x="channel", diffusion="Smagorinsky"),
     # advection and Coriolis terms kept between updates every nstep_advcor steps
     # (time_integration.jl:204-211; the quadratic drag refreshes the kinetic part each
     # diffusion step)
