@@ -263,11 +263,33 @@ def cpu_protocol(swm, orc, config2_cpu_steps: int = 100):
                        bottom_drag="quadratic", tracer_advection=False, Ndays=100, cfl=1.0)
     S1 = swm.make_setup(P1)
     row("readme_double_gyre_100x100 100 days", S1, swm.initial_conditions("Float32", S1), S1.grid.nt, S1.grid.nt)
-    P2 = swm.Parameter(T="Float32", nx=2048, L_ratio=2, bc="periodic", topography="flat", wind_forcing_x="channel",
-                       Fx0=0.12)
-    S2 = swm.make_setup(P2)
+    S2 = swm.make_setup(channel_2048(swm))
     row("channel_2048x1024 periodic", S2, swm.seeded_initial_conditions(S2, seed=2024, σ=0.1), 1000,
         config2_cpu_steps)
+
+
+def channel_2048(swm, T="Float32"):
+    """The 2048x1024 periodic channel (flat bottom, channel wind, tracer every 300 steps)."""
+    return swm.Parameter(T=T, nx=2048, L_ratio=2, bc="periodic", topography="flat", wind_forcing_x="channel",
+                         Fx0=0.12)
+
+
+def channel_run(swm, T, steps, warmup, kernel_reps):
+    """The periodic channel alone on the GPU: `steps` timed steps from the seeded state and the
+    step kernels timed with HIP events (the step runs its RK stages as fused pairs wrapping
+    around the seam).  One JSON line."""
+    S = swm.make_setup(channel_2048(swm, T))
+    G = S.grid
+    with swm.Simulation(S) as sim:
+        sim.upload(swm.seeded_initial_conditions(S, seed=2024, σ=0.1))
+        sim.step(warmup, 1, 0)
+        t = time.perf_counter()
+        d, nan = sim.step(steps, 1 + warmup, warmup * G.dtint)
+        dt = time.perf_counter() - t
+        kms = {name: sim.time_kernel(k, kernel_reps) for name, k in (("fused12", 5), ("fused34", 6), ("dissipation", 3))}
+    print(json.dumps({"workload": f"channel_{G.nx}x{G.ny}_periodic", "T": T, "steps": steps, "steps_done": d,
+                      "nan": nan, "ms_per_step": dt / steps * 1e3, "cell_updates_per_s": G.nx * G.ny * steps / dt,
+                      "kernel_ms": kms}), flush=True)
 
 
 def scale_probe(swm, nx):
@@ -365,6 +387,8 @@ def main():
                     help="only print the CPU-beside-GPU rows (oracle on one thread and the GPU on the 4096^2 "
                          "workload in Float32 and Float64, the README's 100x100 100-day run, the 2048x1024 "
                          "periodic channel), then exit")
+    ap.add_argument("--channel", action="store_true",
+                    help="only time the 2048x1024 periodic channel (--steps, --warmup, --dtype) and its kernels")
     ap.add_argument("--scale-probe", action="store_true",
                     help="only report how far the Float16 / BFloat16 right-hand sides run with velocities scaled "
                          "by s = 1 .. 64, then exit")
@@ -400,6 +424,9 @@ def main():
         return
     if args.cpu_protocol:
         cpu_protocol(swm, load_oracle())
+        return
+    if args.channel:
+        channel_run(swm, args.dtype, args.steps, args.warmup, args.kernel_reps)
         return
     es = 4 if args.dtype == "Float32" else 8
     # dominant kernel: middle RK stage, HIP events on the library stream
