@@ -1574,8 +1574,8 @@ template <class F> struct Ens {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 const bool cs = c0.continuity != SWM_CONT_ITSELF;
 #define SWM_FB(MA_, MB_)                                                                                        \
-    if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true>), grid, block, 0, E->stream, a, tab, nm);        \
-    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false>), grid, block, 0, E->stream, a, tab, nm)
+    if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, PER>), grid, block, 0, E->stream, a, tab, nm);   \
+    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER>), grid, block, 0, E->stream, a, tab, nm)
                 if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) SWM_FB(M_RK_FIRST, M_RK_MID);
                 else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_MID, M_RK_LAST);
                 else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_FIRST, M_RK_LAST);
@@ -2178,13 +2178,19 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
     }
     swm_ens *E = new swm_ens();
     E->m.assign(members, members + n);
-    for (swm_ctx *c : E->m) c->ens_refs += 1;
     E->fmt = m0->fmt;
     E->per = c0.bc == SWM_PERIODIC;
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
         delete E;
         return SWM_E_HIP;
     }
+    for (swm_ctx *c : E->m) c->ens_refs += 1;
+    // periodic members run fused stage pairs only if every member's seam allows it (one
+    // kernel sequence for all); the two paths give the same results
+    bool seam_all = true;
+    for (swm_ctx *c : E->m) seam_all = seam_all && c->seam_ok;
+    if (!seam_all)
+        for (swm_ctx *c : E->m) c->seam_ok = false;
     *out = E;
     return SWM_OK;
 }

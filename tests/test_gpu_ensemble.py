@@ -10,12 +10,17 @@ pytestmark = pytest.mark.gpu
 SIZES = [(40, 1.0), (64, 2.0), (70, 0.8), (48, 1.5), (96, 1.0)]
 
 
-def members(swm, bc, T="Float32", Tprog=None, tracer=True, Uadv=0.25, output_dt=1000.0, seed=5):
+def members(swm, bc, T="Float32", Tprog=None, tracer=True, Uadv=0.25, output_dt=1000.0, seed=5, topo=None):
+    """bc "periodic-flat": periodic members on a flat bottom, whose stage pairs fuse around the
+    seam; topo: one topography per member (default seamount)."""
     rng = np.random.default_rng(seed)
     out = []
+    if bc == "periodic-flat":
+        bc, topo = "periodic", ["flat"] * len(SIZES)
     for k, (nx, ratio) in enumerate(SIZES):
         P = swm.Parameter(T=T, Tprog=Tprog or T, nx=nx, L_ratio=ratio, bc=bc, Fx0=float(rng.uniform(0.08, 0.16)),
-                          H=float(rng.uniform(400, 600)), topography="seamount", wind_forcing_x="double_gyre",
+                          H=float(rng.uniform(400, 600)), topography=topo[k] if topo else "seamount",
+                          wind_forcing_x="double_gyre",
                           diffusion="Smagorinsky", bottom_drag="quadratic", tracer_advection=tracer, Uadv=Uadv,
                           output_dt=output_dt)
         S = swm.make_setup(P)
@@ -39,13 +44,17 @@ def assert_equal(S, a, b, tag):
         assert mm is None, f"{tag} {name}: {mm}"
 
 
-@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat", "periodic-mixed-seams"])
 @pytest.mark.parametrize("events", [False, True])
 def test_ensemble_matches_single_runs(swm, gpu, bc, events):
     """Without tracer / NaN-check steps the ensemble runs step pairs as graph replays; with
-    them (tracer every few steps, NaN check every step) it runs them eagerly in between."""
+    them (tracer every few steps, NaN check every step) it runs them eagerly in between.
+    Periodic members on a flat bottom run fused stage pairs around the seam; with one member
+    whose seam does not allow it, every member runs one kernel per stage."""
     n = 14
-    ms = members(swm, bc, Uadv=30.0 if events else 0.25, output_dt=0.02 if events else 1000.0)
+    topo = ["flat", "flat", "ridge", "flat", "flat"] if bc == "periodic-mixed-seams" else None
+    ms = members(swm, "periodic" if bc == "periodic-mixed-seams" else bc, Uadv=30.0 if events else 0.25,
+                 output_dt=0.02 if events else 1000.0, topo=topo)
     if events:
         assert any(S.grid.nadvstep <= 5 for S, _ in ms)
     ref = [run_alone(swm, S, p, n) for S, p in ms]
