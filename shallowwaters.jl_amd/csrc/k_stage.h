@@ -81,7 +81,11 @@ template <class F> struct StageArgs {
     Q oma, sinv;                      // one_minus_α, 1/s (Tprog)
     int cont, cm, scaled;             // continuity variant, Tcomm, s != 1
     int nx, ny, P, L, nstrips, nseg;
+    // fused pairs: strip s of the launch is strip s + sfirst of the field, plus sjump from
+    // strip sjump_at on (a launch over the strips at the periodic seam skips the others)
+    int sfirst = 0, sjump_at = 1 << 30, sjump = 0;
 };
+
 
 template <class F, int M> using StageIn = std::conditional_t<Mode<M>::first, typename F::Q, typename F::ST>;
 
@@ -184,7 +188,18 @@ template <class R, bool PER, int ADV = ADV_AH, bool QOWN = false> struct RhsPipe
             u_2 = u_1; u_1 = u_0; v_2 = v_1; v_1 = v_0; e_1 = e_0; p_2 = p_1;
             return;
         }
+        front(u_0, v_0, e_0, H_0, fq_1, half, quarter, g, halfp, qs_1, ks_1);
+        back(qend_1, qfix, quarter, k12, g);
+    }
+
+    // The first part of a push: thickness, fluxes, q and the Bernoulli potential of the row
+    // (n_q1 is q at the corner row r-1); back() forms the PV fluxes and the outputs.  Split so
+    // that two marches can exchange q between them (the wrap's q[end,·]).
+    R n_u0, n_v0, n_e0, n_h0, n_hs0, n_U0, n_V1, n_q1, n_p1;
+    __device__ __forceinline__ void front(R u_0, R v_0, R e_0, R H_0, R fq_1, R half, R quarter, R g, R halfp,
+                                          R qs_1 = 0, R ks_1 = 0) {
         // thickness!  rhs.jl:131-139 ; Ix!(h_u)  interpolations.jl:3-14 ; Uflux!  rhs.jl:277-291
+        n_u0 = u_0; n_v0 = v_0; n_e0 = e_0;
         const R h_0 = e_0 + H_0;
         const R hs_0 = sh_xp(h_0) + h_0;                   // h(x+1) + h, read by Ix!(h_u) and Ixy!(h_q)
         const R U_0 = u_0 * (half * hs_0);
@@ -209,6 +224,11 @@ template <class R, bool PER, int ADV = ADV_AH, bool QOWN = false> struct RhsPipe
         }
         // bernoulli!  rhs.jl:162-180
         const R p_1 = halfp * ke_1 + g * e_1;
+        n_h0 = h_0; n_hs0 = hs_0; n_U0 = U_0; n_V1 = V_1; n_q1 = q_1; n_p1 = p_1;
+    }
+    __device__ __forceinline__ void back(R qend_1, bool qfix, R quarter, R k12, R g) {
+        const R u_0 = n_u0, v_0 = n_v0, e_0 = n_e0, h_0 = n_h0, hs_0 = n_hs0, U_0 = n_U0, V_1 = n_V1, q_1 = n_q1,
+                p_1 = n_p1;
         if constexpr (SCH == ADV_SAD) {
             // qhv = q_u·V_u at row r-1 (q_u = Iy!(q), V_u = Ixy!(V)) and qhu = q_v·U_v at row
             // r-2 (q_v = Ix!(q), U_v = Ixy!(U)); lane x holds the q corner x, the V point
@@ -435,6 +455,15 @@ __global__ void __launch_bounds__(256) stage_batch(const StageArgs<F> *__restric
 constexpr int HALO2 = 4;
 constexpr int WOUT2 = WAVE - 2 * HALO2;
 
+// tile w of a fused pair's launch (strips of WAVE - 2·HALO2 output columns)
+template <class F> __device__ __forceinline__ WaveTile fused_tile(const StageArgs<F> &A, int w) {
+    WaveTile t = tile_of<HALO2>(w, A.nstrips, A.nseg, A.L, A.ny);
+    int s = w % A.nstrips + A.sfirst;
+    s += s >= A.sjump_at ? A.sjump : 0;
+    t.x = -HALO2 + (WAVE - 2 * HALO2) * s + t.lane;
+    return t;
+}
+
 template <class F, int MA, int MB, bool CS, bool PER = false>
 __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
@@ -445,7 +474,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using SI = StageIn<F, MA>;
     using VI = typename Sto<SI>::V;
     static_assert(MIA::rk && MIB::rk && !MIB::first && MIA::out1, "fused pair of classic RK stages");
-    const WaveTile t = tile_of<HALO2>(w, A.nstrips, A.nseg, A.L, A.ny);
+    const WaveTile t = fused_tile(A, w);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
     const bool laneB = lane >= HALO2 && lane < WAVE - HALO2;
@@ -612,6 +641,209 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         row(r, sI, sQ, sT);
         row(r + 1, sI + dI, sQ + dQ, sT + dT);
     }
+}
+
+// A fused pair on the strips that reach the periodic seam, for the seams the lanes cannot
+// simply wrap around: H's ghost columns hold the topography of the halo grid (grid.jl:59),
+// not the columns they face, and a Tcomm narrower than T rounds the ghost copies.  Every
+// lane runs stage A twice, as its own column x over the stored fields (ghost columns
+// included: the reference's view of its interior columns) and as the column xs = x ± nx
+// across the seam (the view of the columns the ghost copies come from).  A's outputs at
+// the ghost lanes are the second view's, rounded through Tcomm as ghost_points! rounds its
+// copies; B runs on those in the first view, and the q[end,·] of its AHβ/AHγ wrap (q at
+// the corner nx of B's input) is formed from B's inputs as the second view sees them.  A's
+// own q[end,·] is the q the other view forms at the same lane (the corner nx).
+template <class F, int MA, int MB, bool CS>
+__device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
+    using MIA = Mode<MA>;
+    using MIB = Mode<MB>;
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+    using SI = StageIn<F, MA>;
+    using VI = typename Sto<SI>::V;
+    static_assert(MIA::rk && MIB::rk && !MIB::first && MIA::out1, "fused pair of classic RK stages");
+    const WaveTile t = fused_tile(A, w);
+    if (!t.live) return;
+    const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
+    const bool laneB = lane >= HALO2 && lane < WAVE - HALO2;
+    const bool u_col = laneB && x >= 0 && x <= nx - 1;
+    const bool v_col = laneB && x >= 1 && x <= nx;
+    const bool e_col = v_col;
+    const GhostLanes gl = ghost_lanes<true>(x, nx);
+    const int xs = x < nx / 2 ? x + nx : x - nx;       // the column across the seam
+    // ghost lanes of column x (u: beyond 0..nx-1, v and η: beyond 1..nx); they are the
+    // interior lanes of column xs, and the other way round
+    const bool ug = x <= -1 || x >= nx, vg = x <= 0 || x >= nx + 1;
+    const bool qfix = x == 0, qfixs = xs == 0;        // u column 0: the AHβ/AHγ wrap
+
+    const Arena ar(A.ar);
+    const Fld<SI> in_u{&ar, A.in_u}, in_v{&ar, A.in_v}, in_e{&ar, A.in_e};
+    const Fld<Q> pr_u{&ar, A.pr_u}, pr_v{&ar, A.pr_v}, pr_e{&ar, A.pr_e};
+    const Fld<Q> ac_u{&ar, A.ac_u}, ac_v{&ar, A.ac_v}, ac_e{&ar, A.ac_e};
+    const Fld<ST> o1_u{&ar, A.o1_u}, o1_v{&ar, A.o1_v}, o1_e{&ar, A.o1_e};
+    const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
+    const Fld<ST> fH{&ar, A.st.H};
+    const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
+    const uint32_t vxI = colb<SI>(x), vxQ = colb<Q>(x), vxT = colb<ST>(x);
+    const uint32_t vsI = colb<SI>(xs), vsQ = colb<Q>(xs), vsT = colb<ST>(xs);
+    const uint32_t vfq = scolb<ST>(x, A.st.fq_2d), vFx = scolb<ST>(x, A.st.Fx_2d), vFy = scolb<ST>(x, A.st.Fy_2d);
+    const uint32_t sfq = scolb<ST>(xs, A.st.fq_2d), sFx = scolb<ST>(xs, A.st.Fx_2d), sFy = scolb<ST>(xs, A.st.Fy_2d);
+    const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
+    const int cm = A.cm;
+
+    RhsPipe<R, true> ra, ras, rb;                      // A as column x, A as column xs, B
+    VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0, su_1 = 0, sv_1 = 0, sv_2 = 0, se_1 = 0;
+    Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;       // B's input as column x sees it
+    Q W1_0 = 0, W1_1 = 0, F1_0 = 0, F1_1 = 0, X1_0 = 0;       // ... and as column xs sees it (u, η, v)
+    Q U0_0 = 0, U0_1 = 0, U0_2 = 0, E0_0 = 0, E0_1 = 0, E0_2 = 0, V0_0 = 0, V0_1 = 0, V0_2 = 0;
+    R H_1 = 0, H_2 = 0, Hs_1 = 0, Hs_2 = 0;
+    // B's inputs as the column xs sees them, and its q at the corner nx (lane x = 0)
+    R bu_1 = 0, bv_1 = 0, bhs_1 = 0;
+
+    int r = t.ya - 2;
+    const int rlast = t.yb + 3;
+    VI nu = in_u.ld(vxI, rowb<SI>(r, P)), nv = in_v.ld(vxI, rowb<SI>(r, P)), ne = in_e.ld(vxI, rowb<SI>(r, P));
+    VI mu = in_u.ld(vsI, rowb<SI>(r, P)), mv = in_v.ld(vsI, rowb<SI>(r, P)), me = in_e.ld(vsI, rowb<SI>(r, P));
+    R nH = fH.template ldr<R>(vxT, rowb<ST>(r, P)), mH = fH.template ldr<R>(vsT, rowb<ST>(r, P));
+    Q npu = 0, nau = 0, npe = 0, nae = 0, npv = 0, nav = 0;
+    Q mpu = 0, mau = 0, mpe = 0, mae = 0, mpv = 0, mav = 0;
+    Q bpu = 0, bpe = 0, bpv = 0;
+    RowStatics<R> nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r - 1);
+    RowStatics<R> mst = row_statics<F, false, CS>(A, ar, sfq, sFx, sFy, vsT, r - 1);
+    RowStatics<R> st_1 = {}, st_2 = {}, ss_1 = {}, ss_2 = {};
+    const uint32_t dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
+    uint32_t sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);
+    for (; r <= rlast; ++r, sQ += dQ, sT += dT) {
+        const VI wu_0 = nu, wv_0 = nv, we_0 = ne, su_0 = mu, sv_0 = mv, se_0 = me;
+        const R H_0 = nH, Hs_0 = mH;
+        const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
+        const Q qpu_1 = mpu, qau_1 = mau, qpe_1 = mpe, qae_1 = mae, qpv_2 = mpv, qav_2 = mav;
+        const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
+        const RowStatics<R> sa = nst, sas = mst, sb = st_2, sbs = ss_2;
+        nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r);
+        mst = row_statics<F, false, CS>(A, ar, sfq, sFx, sFy, vsT, r);
+        {
+            const int rn = min(r + 1, rlast);
+            const uint32_t so = rowb<SI>(rn, P), to = rowb<ST>(rn, P);
+            nu = in_u.ld(vxI, so); nv = in_v.ld(vxI, so); ne = in_e.ld(vxI, so); nH = fH.template ldr<R>(vxT, to);
+            mu = in_u.ld(vsI, so); mv = in_v.ld(vsI, so); me = in_e.ld(vsI, so); mH = fH.template ldr<R>(vsT, to);
+        }
+        if (MIA::need_pr) {
+            npu = pr_u.ld(vxQ, sQ); npe = pr_e.ld(vxQ, sQ); npv = pr_v.ld(vxQ, sQ - dQ);
+            mpu = pr_u.ld(vsQ, sQ); mpe = pr_e.ld(vsQ, sQ); mpv = pr_v.ld(vsQ, sQ - dQ);
+        }
+        if (MIA::need_ac) {
+            nau = ac_u.ld(vxQ, sQ); nae = ac_e.ld(vxQ, sQ); nav = ac_v.ld(vxQ, sQ - dQ);
+            mau = ac_u.ld(vsQ, sQ); mae = ac_e.ld(vsQ, sQ); mav = ac_v.ld(vsQ, sQ - dQ);
+        }
+        if (MIB::need_pr) { bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ); }
+
+        // ---- stage A at input row r in both views.  The wrap's q[end,·] of one view is q at
+        // the corner nx, which the other view forms at the same lane.
+        ra.front(dec<R, SI>(wu_0), dec<R, SI>(wv_0), dec<R, SI>(we_0), H_0, sa.fq, half, quarter, A.g, A.halfp);
+        ras.front(dec<R, SI>(su_0), dec<R, SI>(sv_0), dec<R, SI>(se_0), Hs_0, sas.fq, half, quarter, A.g, A.halfp);
+        {
+            const R qa = ra.n_q1, qs = ras.n_q1;
+            ra.back(qs, qfix, quarter, k12, A.g);
+            ras.back(qa, qfixs, quarter, k12, A.g);
+        }
+        Q cu1, cu0, ce1, ce0, cv1, cv0, du1, du0, de1, de0, dv1, dv0, nocc = 0;
+        {
+            const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) + toQ<Q>(A.Fxt * sa.Fx);
+            combine<MA>((Q)wu_1, MIA::first ? (Q)wu_1 : pu_1, au_1, du, A.k, cu1, cu0, nocc);
+            const Q deta = continuity_rhs<CS, Q>(ra.dUdx, ra.dVdy, ra.e_r1, A.cont, sa.cs, A.gamma, A.Fet, A.sinv);
+            combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0, nocc);
+            const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) + toQ<Q>(A.Fyt * sa.Fy);
+            combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
+        }
+        {
+            const Q du = (toQ<Q>(ras.qhv) - toQ<Q>(ras.px)) + toQ<Q>(A.Fxt * sas.Fx);
+            combine<MA>((Q)su_1, MIA::first ? (Q)su_1 : qpu_1, qau_1, du, A.k, du1, du0, nocc);
+            const Q deta = continuity_rhs<CS, Q>(ras.dUdx, ras.dVdy, ras.e_r1, A.cont, sas.cs, A.gamma, A.Fet, A.sinv);
+            combine<MA>((Q)se_1, MIA::first ? (Q)se_1 : qpe_1, qae_1, deta, A.k, de1, de0, nocc);
+            const Q dv = -(toQ<Q>(ras.qhu) + toQ<Q>(ras.py)) + toQ<Q>(A.Fyt * sas.Fy);
+            combine<MA>((Q)sv_2, MIA::first ? (Q)sv_2 : qpv_2, qav_2, dv, A.k, dv1, dv0, nocc);
+        }
+        // A's first outputs as B reads them (ghost_points!): as column x, the ghost lanes hold
+        // the copies, through Tcomm, of the columns across the seam; as column xs, the lanes
+        // that are ghosts there hold the copies of column x.  u's rows beyond the walls are
+        // never written (0), v's wall rows are 0, η's row ny+1 copies row ny.
+        Q wu1 = ug ? du1 : comm_round(cu1, cm), wv1 = vg ? dv1 : comm_round(cv1, cm), we1 = vg ? de1 : comm_round(ce1, cm);
+        if (ug) cu1 = comm_round(du1, cm);
+        if (vg) { cv1 = comm_round(dv1, cm); ce1 = comm_round(de1, cm); }
+        if (r - 1 == ny + 1 || r - 1 == ny + 2) { cu1 = Q(0); wu1 = Q(0); }
+        if (r - 1 == ny + 1) { ce1 = E1_0; we1 = F1_0; }
+        if (r - 2 <= 0 || r - 2 >= ny) { cv1 = Q(0); wv1 = Q(0); }
+        U1_1 = U1_0; U1_0 = cu1; E1_1 = E1_0; E1_0 = ce1; V1_0 = cv1;
+        W1_1 = W1_0; W1_0 = wu1; F1_1 = F1_0; F1_0 = we1; X1_0 = wv1;
+        U0_2 = U0_1; U0_1 = U0_0; U0_0 = cu0; E0_2 = E0_1; E0_1 = E0_0; E0_0 = ce0;
+        V0_2 = V0_1; V0_1 = V0_0; V0_0 = cv0;
+
+        // ---- stage B at its input row r-2
+        const int rbi = r - 2;
+        {
+            const Q ub = rbi == 0 ? Q(0) : U1_1;
+            const Q eb = rbi == 0 ? E1_0 : E1_1;
+            // q at the corner nx of B's input (the wrap's q[end,·]) as the column xs sees it
+            {
+                const R us = toR<R>(rbi == 0 ? Q(0) : W1_1);
+                const R vs = toR<R>(X1_0);
+                const R es = toR<R>(rbi == 0 ? F1_0 : F1_1);
+                const R hs0 = es + Hs_2;
+                const R hss_0 = sh_xp(hs0) + hs0;
+                const R hq = quarter * bhs_1 + quarter * hss_0;
+                const R dvdx = sh_xp(bv_1) - bv_1;
+                const R dudy = us - bu_1;
+                const R qe = (sbs.fq + dvdx - dudy) / hq;
+                rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, qe, qfix, half, quarter, k12, A.g, A.halfp);
+                bu_1 = us; bv_1 = vs; bhs_1 = hss_0;
+            }
+            const int yu = r - 3;
+            if (yu >= t.ya && yu < t.yb) {
+                if (u_col) {
+                    const Q du = (toQ<Q>(rb.qhv) - toQ<Q>(rb.px)) + toQ<Q>(A.Fxt * sb.Fx);
+                    Q o1 = 0, o2 = 0, cc = 0;
+                    if (MIB::comp) cc = cmp_u.ld(vxQ, sQ - 3 * dQ);
+                    combine<MB>(Q(0), qpu_3, U0_2, du, A.k2, o1, o2, cc);
+                    if (MIB::comp) cmp_u.st(cc, vxQ, sQ - 3 * dQ);
+                    if (MIB::out1) put_u<ST, true>(o1_u, x, vxT, yu, sT - 3 * dT, o1, gl.u, nx, ny, P, A.oma, cm);
+                    if (MIB::ghost2) put_u<Q, true>(o2_u, x, vxQ, yu, sQ - 3 * dQ, o2, gl.u, nx, ny, P, A.oma, cm);
+                    else o2_u.st(o2, vxQ, sQ - 3 * dQ);
+                }
+                if (e_col) {
+                    const Q deta = continuity_rhs<CS, Q>(rb.dUdx, rb.dVdy, rb.e_r1, A.cont, sb.cs, A.gamma, A.Fet, A.sinv);
+                    Q o1 = 0, o2 = 0, cc = 0;
+                    if (MIB::comp) cc = cmp_e.ld(vxQ, sQ - 3 * dQ);
+                    combine<MB>(Q(0), qpe_3, E0_2, deta, A.k2, o1, o2, cc);
+                    if (MIB::comp) cmp_e.st(cc, vxQ, sQ - 3 * dQ);
+                    if (MIB::out1) put_e<ST, true>(o1_e, x, vxT, yu, sT - 3 * dT, o1, gl.e, nx, ny, P, cm);
+                    if (MIB::ghost2) put_e<Q, true>(o2_e, x, vxQ, yu, sQ - 3 * dQ, o2, gl.e, nx, ny, P, cm);
+                    else o2_e.st(o2, vxQ, sQ - 3 * dQ);
+                }
+            }
+            const int yv = r - 4;
+            if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1) {
+                const Q dv = -(toQ<Q>(rb.qhu) + toQ<Q>(rb.py)) + toQ<Q>(A.Fyt * sb.Fy);
+                Q o1 = 0, o2 = 0, cc = 0;
+                if (MIB::comp) cc = cmp_v.ld(vxQ, sQ - 4 * dQ);
+                combine<MB>(Q(0), qpv_4, V0_2, dv, A.k2, o1, o2, cc);
+                if (MIB::comp) cmp_v.st(cc, vxQ, sQ - 4 * dQ);
+                if (MIB::out1) put_v<ST, true>(o1_v, x, vxT, yv, sT - 4 * dT, o1, gl.v, nx, P, A.oma, cm);
+                if (MIB::ghost2) put_v<Q, true>(o2_v, x, vxQ, yv, sQ - 4 * dQ, o2, gl.v, nx, P, A.oma, cm);
+                else o2_v.st(o2, vxQ, sQ - 4 * dQ);
+            }
+        }
+        wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
+        su_1 = su_0; sv_2 = sv_1; sv_1 = sv_0; se_1 = se_0;
+        H_2 = H_1; H_1 = H_0; Hs_2 = Hs_1; Hs_1 = Hs_0;
+        st_2 = st_1; st_1 = sa; ss_2 = ss_1; ss_1 = sas;
+    }
+}
+
+template <class F, int MA, int MB, bool CS>
+__global__ void __launch_bounds__(256) fused_seam_kernel(StageArgs<F> A) {
+    fused_seam_body<F, MA, MB, CS>(A, wave_index());
 }
 
 template <class F, int MA, int MB, bool CS, bool PER = false>

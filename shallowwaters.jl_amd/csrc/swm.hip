@@ -111,6 +111,8 @@ struct swm_ctx {
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;   // periodic: the seam strips of a fused pair, beside the others
+    hipEvent_t fork = nullptr, join = nullptr;
     int fmt = SWM_F32;     // format bundle: SWM_F64, SWM_F32 (T = Tprog), SWM_F16, SWM_BF16 (T, Tprog = F32)
     int es = 4;            // element size of Tprog fields
     int est = 4;           // element size of T fields
@@ -342,7 +344,11 @@ template <class F> struct Launch {
     static StageArgs<F> args(swm_ctx *ctx, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
                              const Times &T) {
         const swm_config &c = ctx->c;
+        // every byte set (padding included): an ensemble compares the recorded arguments of
+        // consecutive steps byte for byte to replay them as one graph
         StageArgs<F> A;
+        std::memset((void *)&A, 0, sizeof A);
+        A.sfirst = 0; A.sjump_at = 1 << 30; A.sjump = 0;
         A.ar = ctx->aref();
         A.in_u = ou(ctx, in); A.in_v = ov(ctx, in); A.in_e = oe(ctx, in);
         A.pr_u = ou(ctx, pr); A.pr_v = ov(ctx, pr); A.pr_e = oe(ctx, pr);
@@ -407,7 +413,7 @@ template <class F> struct Launch {
         const int adv = adv_form(c);
         if (per && adv == ADV_AH) {
             const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
-            if (m == M_RK_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
+            if (m == M_RK_FIRST || m == M_RKC_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
             else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
@@ -458,7 +464,10 @@ template <class F> struct Launch {
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
     static bool fusable(const swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        return ctx->fuse && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
+        // periodic: a seam the lanes can wrap around (one launch), or the seam strips on their
+        // own kernel; an ensemble's batched pairs take the first kind only
+        const bool per_ok = c.bc == SWM_NONPERIODIC || ctx->seam_ok || !ctx->rec;
+        return ctx->fuse && per_ok && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -472,10 +481,9 @@ template <class F> struct Launch {
                           (blast || ok(o1, ctx->mixed)) && ok(o2, false) && (mb != M_RKC_LAST || ctx->cu.base);
         if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
         const bool per = ctx->c.bc == SWM_PERIODIC;
-        if ((per && !ctx->seam_ok) || adv_form(ctx->c) != ADV_AH)
-            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages need the Arakawa-Hsu right-hand side and, periodic, a "
-                                                "seam the lanes can wrap around (H equal on both sides, f_q independent "
-                                                "of x, exact ghost copies)");
+        if (adv_form(ctx->c) != ADV_AH || (per && !ctx->seam_ok && ctx->rec))
+            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages need the Arakawa-Hsu right-hand side (and in an ensemble, "
+                                                "periodic, a seam the lanes can wrap around)");
         StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
         A.k2 = kb;
         A.nstrips = nstrips2(ctx);
@@ -485,6 +493,7 @@ template <class F> struct Launch {
             ctx->rec->add(1, ma, mb, &A, sizeof A);
             return SWM_OK;
         }
+        if (per && !ctx->seam_ok) return fused_seam(ctx, ma, mb, A);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
 #define SWM_FK(MA_, MB_)                                                                                   \
@@ -502,6 +511,56 @@ template <class F> struct Launch {
         else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
 #undef SWM_FK
         HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
+    // A periodic fused pair whose seam the lanes cannot wrap around: the strips clear of
+    // the seam on the wrapping kernel (no lane of theirs wraps), and beside them, on the
+    // context's second stream, the strips at the seam on fused_seam_kernel.  The two write
+    // disjoint columns and read only what the launches before the pair wrote.
+    static int fused_seam(swm_ctx *ctx, int ma, int mb, StageArgs<F> A) {
+        const swm_config &c = ctx->c;
+        const int ns = A.nstrips, wo = WAVE - 2 * HALO2;
+        // a strip is clear of the seam if all its lanes lie in the columns 1..nx-1
+        auto clear = [&](int s) { return wo * s - HALO2 >= 1 && wo * s + WAVE - HALO2 - 1 <= c.nx - 1; };
+        int a = 0;
+        while (a < ns && !clear(a)) ++a;
+        int b = a;
+        while (b < ns && clear(b)) ++b;
+        const dim3 block(WAVE * WAVES_PER_BLOCK);
+        const bool cs = c.continuity != SWM_CONT_ITSELF;
+        StageArgs<F> I = A, E = A;
+        I.nstrips = b - a; I.sfirst = a;
+        E.nstrips = a + (ns - b); E.sfirst = 0; E.sjump_at = a; E.sjump = b - a;
+        hipStream_t s2 = I.nstrips > 0 ? ctx->stream2 : ctx->stream;
+        if (s2 != ctx->stream) {                             // fork
+            HIPCHK(hipEventRecord(ctx->fork, ctx->stream));
+            HIPCHK(hipStreamWaitEvent(s2, ctx->fork, 0));
+        }
+#define SWM_FS(MA_, MB_)                                                                                       \
+    {                                                                                                          \
+        const dim3 g(nblocks_waves(E.nstrips * E.nseg));                                                       \
+        if (cs) hipLaunchKernelGGL((fused_seam_kernel<F, MA_, MB_, true>), g, block, 0, s2, E);                \
+        else hipLaunchKernelGGL((fused_seam_kernel<F, MA_, MB_, false>), g, block, 0, s2, E);                  \
+    }                                                                                                          \
+    if (I.nstrips > 0) {                                                                                       \
+        const dim3 g(nblocks_waves(I.nstrips * I.nseg));                                                       \
+        if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true>), g, block, 0, ctx->stream, I);      \
+        else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true>), g, block, 0, ctx->stream, I);        \
+    }
+        if (ma == M_RK_FIRST && mb == M_RK_MID) { SWM_FS(M_RK_FIRST, M_RK_MID) }
+        else if (ma == M_RK_MID && mb == M_RK_LAST) { SWM_FS(M_RK_MID, M_RK_LAST) }
+        else if (ma == M_RK_FIRST && mb == M_RK_LAST) { SWM_FS(M_RK_FIRST, M_RK_LAST) }
+        else if (ma == M_RKC_FIRST && mb == M_RK_MID) { SWM_FS(M_RKC_FIRST, M_RK_MID) }
+        else if (ma == M_RK_MID && mb == M_RKC_LAST) { SWM_FS(M_RK_MID, M_RKC_LAST) }
+        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) { SWM_FS(M_RKC_FIRST, M_RKC_LAST) }
+        else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
+#undef SWM_FS
+        HIPCHK(hipGetLastError());
+        if (s2 != ctx->stream) {                             // join: the step goes on after both
+            HIPCHK(hipEventRecord(ctx->join, s2));
+            HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+        }
         return SWM_OK;
     }
 
@@ -636,6 +695,7 @@ template <class F> struct Launch {
     static int dissipation(swm_ctx *ctx, Slots in, int out_uv) {
         const swm_config &c = ctx->c;
         DissArgs<F> A;
+        std::memset((void *)&A, 0, sizeof A);   // padding too (compared byte for byte by ensembles)
         A.ar = ctx->aref();
         A.u0 = ctx->u[in.uv].ref(); A.v0 = ctx->v[in.uv].ref(); A.e0 = ctx->e[in.e].ref();
         A.out_u = ctx->u[out_uv].ref(); A.out_v = ctx->v[out_uv].ref();
@@ -1818,6 +1878,14 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         delete ctx;
         return SWM_E_HIP;
     }
+    if (c.bc == SWM_PERIODIC &&
+        (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess)) {
+        fprintf(stderr, "swm_create: no second HIP stream\n");
+        swm_destroy(ctx);
+        return SWM_E_HIP;
+    }
     with_fmt(ctx, [&](auto f) {
         using F = decltype(f);
         if (c.bc == SWM_PERIODIC) Launch<F>::template segment_lengths<true>(ctx);
@@ -1908,8 +1976,12 @@ int swm_destroy(swm_ctx *ctx) {
         ctx->adj->release();
         delete ctx->adj;
     }
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
     delete ctx;
     return SWM_OK;
 }

@@ -125,6 +125,18 @@ CASES = {
                                      wind_forcing_x="double_gyre", RKo=2, cfl=0.5, surface_forcing=True),
     "per-f32-flat-narrow": dict(T="Float32", nx=24, L_ratio=0.5, bc="periodic", topography="flat",
                                 wind_forcing_x="channel", diffusion="Smagorinsky"),
+    # periodic fused stage pairs whose seam the lanes cannot wrap around (H of the halo grid
+    # differs across it, or Tcomm rounds the ghost copies): seam strips on their own kernel
+    "per-f32-ridge-comp": dict(T="Float32", nx=130, L_ratio=1.2, bc="periodic", topography="ridge",
+                               wind_forcing_x="channel", diffusion="Smagorinsky", compensated=True),
+    "per-f64-ridges-rk2-relax": dict(T="Float64", nx=118, L_ratio=1.0, bc="periodic", topography="ridges",
+                                     wind_forcing_x="double_gyre", RKo=2, cfl=0.5, surface_relax=True, t_relax=2.0),
+    "per-f32-ridge-narrow": dict(T="Float32", nx=30, L_ratio=0.6, bc="periodic", topography="ridge",
+                                 wind_forcing_x="channel", diffusion="Smagorinsky", bottom_drag="quadratic"),
+    "per-f64-seamount-comm-f32": dict(T="Float64", Tcomm="Float32", nx=124, L_ratio=1.0, bc="periodic",
+                                      topography="seamount", wind_forcing_x="double_gyre", diffusion="Smagorinsky"),
+    "per-f16-f32-ridges": dict(T="Float16", Tprog="Float32", nx=126, L_ratio=1.0, bc="periodic", topography="ridges",
+                               wind_forcing_x="channel", diffusion="Smagorinsky", bottom_drag="quadratic"),
     # advection and Coriolis terms kept between updates every nstep_advcor steps
     # (time_integration.jl:204-211; the quadratic drag refreshes the kinetic part each
     # diffusion step)
