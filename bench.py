@@ -268,17 +268,18 @@ def cpu_protocol(swm, orc, config2_cpu_steps: int = 100):
         config2_cpu_steps)
 
 
-def channel_2048(swm, T="Float32"):
+def channel_2048(swm, T="Float32", topography="flat"):
     """The 2048x1024 periodic channel (flat bottom, channel wind, tracer every 300 steps)."""
-    return swm.Parameter(T=T, nx=2048, L_ratio=2, bc="periodic", topography="flat", wind_forcing_x="channel",
+    return swm.Parameter(T=T, nx=2048, L_ratio=2, bc="periodic", topography=topography, wind_forcing_x="channel",
                          Fx0=0.12)
 
 
-def channel_run(swm, T, steps, warmup, kernel_reps):
+def channel_run(swm, T, steps, warmup, kernel_reps, topography="flat"):
     """The periodic channel alone on the GPU: `steps` timed steps from the seeded state and the
     step kernels timed with HIP events (the step runs its RK stages as fused pairs wrapping
-    around the seam).  One JSON line."""
-    S = swm.make_setup(channel_2048(swm, T))
+    around the seam; with a topography whose H differs across the seam, the seam strips run
+    on their own kernel beside them).  One JSON line."""
+    S = swm.make_setup(channel_2048(swm, T, topography))
     G = S.grid
     with swm.Simulation(S) as sim:
         sim.upload(swm.seeded_initial_conditions(S, seed=2024, σ=0.1))
@@ -287,7 +288,7 @@ def channel_run(swm, T, steps, warmup, kernel_reps):
         d, nan = sim.step(steps, 1 + warmup, warmup * G.dtint)
         dt = time.perf_counter() - t
         kms = {name: sim.time_kernel(k, kernel_reps) for name, k in (("fused12", 5), ("fused34", 6), ("dissipation", 3))}
-    print(json.dumps({"workload": f"channel_{G.nx}x{G.ny}_periodic", "T": T, "steps": steps, "steps_done": d,
+    print(json.dumps({"workload": f"channel_{G.nx}x{G.ny}_periodic_{topography}", "T": T, "steps": steps, "steps_done": d,
                       "nan": nan, "ms_per_step": dt / steps * 1e3, "cell_updates_per_s": G.nx * G.ny * steps / dt,
                       "kernel_ms": kms}), flush=True)
 
@@ -389,6 +390,8 @@ def main():
                          "periodic channel), then exit")
     ap.add_argument("--channel", action="store_true",
                     help="only time the 2048x1024 periodic channel (--steps, --warmup, --dtype) and its kernels")
+    ap.add_argument("--topography", default="flat", choices=["flat", "ridge", "ridges", "seamount"],
+                    help="topography of --channel")
     ap.add_argument("--scale-probe", action="store_true",
                     help="only report how far the Float16 / BFloat16 right-hand sides run with velocities scaled "
                          "by s = 1 .. 64, then exit")
@@ -426,7 +429,7 @@ def main():
         cpu_protocol(swm, load_oracle())
         return
     if args.channel:
-        channel_run(swm, args.dtype, args.steps, args.warmup, args.kernel_reps)
+        channel_run(swm, args.dtype, args.steps, args.warmup, args.kernel_reps, args.topography)
         return
     es = 4 if args.dtype == "Float32" else 8
     # dominant kernel: middle RK stage, HIP events on the library stream
