@@ -464,7 +464,7 @@ template <class F> __device__ __forceinline__ WaveTile fused_tile(const StageArg
     return t;
 }
 
-template <class F, int MA, int MB, bool CS, bool PER = false>
+template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH>
 __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
@@ -505,10 +505,11 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);                 // stores (output lanes)
     const uint32_t vIu = colb<SI>(xu), vIt = colb<SI>(xt), vQu = colb<Q>(xu), vQt = colb<Q>(xt), vTt = colb<ST>(xt);
     const uint32_t vfq = scolb<ST>(xu, A.st.fq_2d), vFx = scolb<ST>(xu, A.st.Fx_2d), vFy = scolb<ST>(xt, A.st.Fy_2d);
+    const uint32_t vfu = scolb<ST>(xu, A.st.fu_2d), vfv = scolb<ST>(xt, A.st.fv_2d);
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
     const Q oma = A.oma;
 
-    RhsPipe<R, PER, ADV_AH, PER> ra, rb;
+    RhsPipe<R, PER, ADV, PER> ra, rb;
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
     // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
@@ -525,7 +526,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
     // A's statics a row ahead (f_q, Fx, η static at row r-1, Fy at r-2); B takes those A had
     // two iterations before (rows r-3, r-4)
-    RowStatics<R> nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vTt, r - 1);
+    RowStatics<R> nst = row_statics<F, false, CS, ADV>(A, ar, vfq, vFx, vFy, vTt, r - 1, vfu, vfv);
     RowStatics<R> st_1 = {0, 0, 0, 0, 0}, st_2 = {0, 0, 0, 0, 0};
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
@@ -540,7 +541,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
         const RowStatics<R> sa = nst, sb = st_2;
-        nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vTt, r);
+        nst = row_statics<F, false, CS, ADV>(A, ar, vfq, vFx, vFy, vTt, r, vfu, vfv);
         {   // the input row r+1, clamped to the last row the march reads
             const int rn = min(r + 1, rlast);
             nu = in_u.ld(vIu, rowb<SI>(rn, P)); nv = in_v.ld(vIt, rowb<SI>(rn, P)); ne = in_e.ld(vIt, rowb<SI>(rn, P));
@@ -554,7 +555,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         }
 
         // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
-        ra.push(u_0, v_0, e_0, H_0, sa.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp);
+        ra.push(u_0, v_0, e_0, H_0, sa.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp, sa.fu, sa.fv, sa.qs, sa.ks);
         Q cu1, cu0, ce1, ce0, cv1, cv0;
         {
             const Q du = (toQ<Q>(ra.qhv) - toQ<Q>(ra.px)) + toQ<Q>(A.Fxt * sa.Fx);
@@ -597,7 +598,8 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         {
             const Q ub = rbi == 0 ? (PER ? Q(0) : oma * U1_0) : U1_1;   // row 0: (1-α)·u1 of row 1 (periodic: 0)
             const Q eb = rbi == 0 ? E1_0 : E1_1;                // row 0: η1 of row 1
-            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp);
+            rb.push(toR<R>(ub), toR<R>(V1_0), toR<R>(eb), H_2, sb.fq, R(0), qfix, half, quarter, k12, A.g, A.halfp,
+                    sb.fu, sb.fv, sb.qs, sb.ks);
             const int yu = r - 3;
             if (yu >= t.ya && yu < t.yb) {
                 if (u_col) {
@@ -846,9 +848,9 @@ __global__ void __launch_bounds__(256) fused_seam_kernel(StageArgs<F> A) {
     fused_seam_body<F, MA, MB, CS>(A, wave_index());
 }
 
-template <class F, int MA, int MB, bool CS, bool PER = false>
+template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH>
 __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
-    fused_body<F, MA, MB, CS, PER>(A, wave_index());
+    fused_body<F, MA, MB, CS, PER, ADV>(A, wave_index());
 }
 
 template <class F, int MA, int MB, bool CS, bool PER = false>

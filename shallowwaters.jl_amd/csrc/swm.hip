@@ -462,12 +462,17 @@ template <class F> struct Launch {
     }
 
     // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
-    static bool fusable(const swm_ctx *ctx) {
+    static bool fusable(const swm_ctx *ctx, int ma, int mb) { return ctx->fuse && can_fuse(ctx, ma, mb); }
+    static bool can_fuse(const swm_ctx *ctx, int ma, int mb) {
         const swm_config &c = ctx->c;
+        if (c.time_scheme != SWM_RK) return false;
         // periodic: a seam the lanes can wrap around (one launch), or the seam strips on their
         // own kernel; an ensemble's batched pairs take the first kind only
-        const bool per_ok = c.bc == SWM_NONPERIODIC || ctx->seam_ok || !ctx->rec;
-        return ctx->fuse && per_ok && c.time_scheme == SWM_RK && adv_form(c) == ADV_AH;
+        if (adv_form(c) == ADV_AH) return c.bc == SWM_NONPERIODIC || ctx->seam_ok || !ctx->rec;
+        // Sadourny, the linear dynamics, kept q and KE: the pairs of RK4 with T = Tprog, no
+        // continuity static, on a seam the lanes can wrap around
+        return !ctx->mixed && c.continuity == SWM_CONT_ITSELF && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) &&
+               !ctx->rec && ((ma == M_RK_FIRST && mb == M_RK_MID) || (ma == M_RK_MID && mb == M_RK_LAST));
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -481,9 +486,8 @@ template <class F> struct Launch {
                           (blast || ok(o1, ctx->mixed)) && ok(o2, false) && (mb != M_RKC_LAST || ctx->cu.base);
         if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
         const bool per = ctx->c.bc == SWM_PERIODIC;
-        if (adv_form(ctx->c) != ADV_AH || (per && !ctx->seam_ok && ctx->rec))
-            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages need the Arakawa-Hsu right-hand side (and in an ensemble, "
-                                                "periodic, a seam the lanes can wrap around)");
+        if (!can_fuse(ctx, ma, mb)) return ctx->fail(SWM_E_UNSUPPORTED, "stages %d+%d do not fuse in this configuration", ma, mb);
+        const int adv = adv_form(ctx->c);
         StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
         A.k2 = kb;
         A.nstrips = nstrips2(ctx);
@@ -496,6 +500,26 @@ template <class F> struct Launch {
         if (per && !ctx->seam_ok) return fused_seam(ctx, ma, mb, A);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
+        if (adv != ADV_AH) {
+            if constexpr (!F::mixed) {
+#define SWM_FV(MA_, MB_, ADV_)                                                                             \
+    if (per) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true, ADV_>), grid, block, 0, ctx->stream, A); \
+    else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, false, ADV_>), grid, block, 0, ctx->stream, A)
+#define SWM_FVA(ADV_)                                                                                      \
+    if (ma == M_RK_FIRST) SWM_FV(M_RK_FIRST, M_RK_MID, ADV_);                                              \
+    else SWM_FV(M_RK_MID, M_RK_LAST, ADV_)
+                if (adv == ADV_SAD) { SWM_FVA(ADV_SAD); }
+                else if (adv == ADV_LIN) { SWM_FVA(ADV_LIN); }
+                else if (adv == (ADV_AH | ADV_KEEP)) { SWM_FVA(ADV_AH | ADV_KEEP); }
+                else if (adv == (ADV_SAD | ADV_KEEP)) { SWM_FVA(ADV_SAD | ADV_KEEP); }
+                else return ctx->fail(SWM_E_ARG, "right-hand side form %d", adv);
+#undef SWM_FVA
+#undef SWM_FV
+                HIPCHK(hipGetLastError());
+                return SWM_OK;
+            }
+            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages of this right-hand side with mixed formats");
+        }
 #define SWM_FK(MA_, MB_)                                                                                   \
     if (per) {                                                                                             \
         if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true>), grid, block, 0, ctx->stream, A); \
@@ -631,7 +655,7 @@ template <class F> struct Launch {
                 // the next u1: a free Tprog slot, or the other T slot with mixed formats
                 const Slots n = ctx->mixed ? Slots{cur.t ? 1 - cur.uv : 0, cur.t ? 1 - cur.e : 0, true}
                                            : Slots{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
-                if (s + 1 <= n_stages && fusable(ctx) && !(keep_u1 && s + 1 == c.RKo)) {   // stages s, s+1 in one kernel
+                if (s + 1 <= n_stages && fusable(ctx, mode(s), mode(s + 1)) && !(keep_u1 && s + 1 == c.RKo)) {   // stages s, s+1 in one kernel
                     const int s2 = s + 1;
                     // the first stage reads u0 only after stage 1 wrote it
                     rc = fused(ctx, mode(s), mode(s2), cur, P, s == 1 ? none : a, s2 < c.RKo ? n : none, a, coef(s),

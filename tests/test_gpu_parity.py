@@ -137,6 +137,14 @@ CASES = {
                                       topography="seamount", wind_forcing_x="double_gyre", diffusion="Smagorinsky"),
     "per-f16-f32-ridges": dict(T="Float16", Tprog="Float32", nx=126, L_ratio=1.0, bc="periodic", topography="ridges",
                                wind_forcing_x="channel", diffusion="Smagorinsky", bottom_drag="quadratic"),
+    # fused stage pairs of the other right-hand-side forms (RK4, T = Tprog, a wrapping seam)
+    "per-f32-flat-sadourny-advcor2": dict(T="Float32", nx=66, L_ratio=1.0, bc="periodic", topography="flat",
+                                          wind_forcing_x="channel", diffusion="Smagorinsky", bottom_drag="quadratic",
+                                          adv_scheme="Sadourny", nstep_advcor=2),
+    "per-f64-flat-linear": dict(T="Float64", nx=70, L_ratio=1.2, bc="periodic", topography="flat",
+                                wind_forcing_x="channel", dynamics="linear"),
+    "np-f32-sadourny-advcor3": dict(T="Float32", nx=118, L_ratio=0.8, bc="nonperiodic", α=1.0, topography="seamount",
+                                    wind_forcing_x="double_gyre", adv_scheme="Sadourny", nstep_advcor=3),
     # advection and Coriolis terms kept between updates every nstep_advcor steps
     # (time_integration.jl:204-211; the quadratic drag refreshes the kinetic part each
     # diffusion step)
