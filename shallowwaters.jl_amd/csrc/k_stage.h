@@ -848,6 +848,16 @@ __global__ void __launch_bounds__(256) fused_seam_kernel(StageArgs<F> A) {
     fused_seam_body<F, MA, MB, CS>(A, wave_index());
 }
 
+// the seam strips of every member of an ensemble in one launch
+template <class F, int MA, int MB, bool CS>
+__global__ void __launch_bounds__(256) fused_seam_batch(const StageArgs<F> *__restrict__ args,
+                                                        const int *__restrict__ start, int nmem) {
+    int m, wl;
+    if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
+    const StageArgs<F> A = args[m];
+    fused_seam_body<F, MA, MB, CS>(A, wl);
+}
+
 template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH>
 __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
     fused_body<F, MA, MB, CS, PER, ADV>(A, wave_index());

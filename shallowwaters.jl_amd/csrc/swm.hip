@@ -468,7 +468,7 @@ template <class F> struct Launch {
         if (c.time_scheme != SWM_RK) return false;
         // periodic: a seam the lanes can wrap around (one launch), or the seam strips on their
         // own kernel; an ensemble's batched pairs take the first kind only
-        if (adv_form(c) == ADV_AH) return c.bc == SWM_NONPERIODIC || ctx->seam_ok || !ctx->rec;
+        if (adv_form(c) == ADV_AH) return true;
         // Sadourny, the linear dynamics, kept q and KE: the pairs of RK4 with T = Tprog, no
         // continuity static, on a seam the lanes can wrap around
         return !ctx->mixed && c.continuity == SWM_CONT_ITSELF && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) &&
@@ -493,11 +493,11 @@ template <class F> struct Launch {
         A.nstrips = nstrips2(ctx);
         A.L = ctx->Lf;
         A.nseg = nseg(ctx, ctx->Lf);
+        if (per && !ctx->seam_ok) return fused_seam(ctx, ma, mb, A);
         if (ctx->rec) {
             ctx->rec->add(1, ma, mb, &A, sizeof A);
             return SWM_OK;
         }
-        if (per && !ctx->seam_ok) return fused_seam(ctx, ma, mb, A);
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
         if (adv != ADV_AH) {
@@ -556,6 +556,11 @@ template <class F> struct Launch {
         StageArgs<F> I = A, E = A;
         I.nstrips = b - a; I.sfirst = a;
         E.nstrips = a + (ns - b); E.sfirst = 0; E.sjump_at = a; E.sjump = b - a;
+        if (ctx->rec) {                                      // an ensemble's plan: the two launches as items
+            ctx->rec->add(4, ma, mb, &I, sizeof I);
+            ctx->rec->add(5, ma, mb, &E, sizeof E);
+            return SWM_OK;
+        }
         hipStream_t s2 = I.nstrips > 0 ? ctx->stream2 : ctx->stream;
         if (s2 != ctx->stream) {                             // fork
             HIPCHK(hipEventRecord(ctx->fork, ctx->stream));
@@ -1665,6 +1670,21 @@ template <class F> struct Ens {
                 else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_FIRST, M_RK_LAST);
                 else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
 #undef SWM_FB
+            } else if (pl.kind[j] == 4 || pl.kind[j] == 5) {      // periodic pairs: strips clear of the seam / at it
+                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
+                const bool cs = c0.continuity != SWM_CONT_ITSELF;
+                if (pl.tiles[j] == 0) continue;
+#define SWM_FB2(MA_, MB_)                                                                                         \
+    if (pl.kind[j] == 4) {                                                                                        \
+        if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, true>), grid, block, 0, E->stream, a, tab, nm); \
+        else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, true>), grid, block, 0, E->stream, a, tab, nm);  \
+    } else if (cs) hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, true>), grid, block, 0, E->stream, a, tab, nm); \
+    else hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, false>), grid, block, 0, E->stream, a, tab, nm)
+                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FB2(M_RK_FIRST, M_RK_MID); }
+                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_MID, M_RK_LAST); }
+                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_FIRST, M_RK_LAST); }
+                else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
+#undef SWM_FB2
             } else if (pl.kind[j] == 2) {
                 const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
                 const bool smag = c0.diffusion == SWM_DIFF_SMAGORINSKY;
