@@ -2301,8 +2301,9 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         return SWM_E_HIP;
     }
     for (swm_ctx *c : E->m) c->ens_refs += 1;
-    // periodic members run fused stage pairs only if every member's seam allows it (one
-    // kernel sequence for all); the two paths give the same results
+    // periodic members run the fused pairs that wrap around the seam only if every member's
+    // seam allows it, else all run the form with the seam strips on their own kernel (one
+    // kernel sequence for all members); the two forms give the same results
     bool seam_all = true;
     for (swm_ctx *c : E->m) seam_all = seam_all && c->seam_ok;
     if (!seam_all)

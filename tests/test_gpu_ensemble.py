@@ -49,8 +49,9 @@ def assert_equal(S, a, b, tag):
 def test_ensemble_matches_single_runs(swm, gpu, bc, events):
     """Without tracer / NaN-check steps the ensemble runs step pairs as graph replays; with
     them (tracer every few steps, NaN check every step) it runs them eagerly in between.
-    Periodic members on a flat bottom run fused stage pairs around the seam; with one member
-    whose seam does not allow it, every member runs one kernel per stage."""
+    Periodic members on a flat bottom run fused stage pairs wrapping around the seam; with
+    one member whose seam does not allow that, every member takes the form with the seam
+    strips on their own kernel (as do the seamount members of "periodic")."""
     n = 14
     topo = ["flat", "flat", "ridge", "flat", "flat"] if bc == "periodic-mixed-seams" else None
     ms = members(swm, "periodic" if bc == "periodic-mixed-seams" else bc, Uadv=30.0 if events else 0.25,
