@@ -371,3 +371,35 @@ def test_fused_stage_pairs_on_a_wider_grid(swm, orc, gpu, T, Tprog, monkeypatch)
             g = sim.download()
         for name, a, b in zip(("u", "v", "η", "sst"), r[:4], (g.u, g.v, g.η, g.sst)):
             assert_same(f"fuse={fuse} {name}", swm.to_float64(a, Tprog), swm.to_float64(b, Tprog), Tprog)
+
+
+def test_linear_dynamics_needs_f_on_the_u_and_v_grids(swm, gpu):
+    """The C ABI: a step of the linear dynamics without swm_set_static_linear fails loudly
+    (it would read f_u, f_v), and a nonlinear context refuses the call."""
+    import ctypes
+    from shallowwaters_amd import _abi
+    L = _abi.load()
+    for dyn in ("linear", "nonlinear"):
+        P, S, prog = setup_case(swm, "np-f32-const-lin", dynamics=dyn)
+        cfg = swm.make_config(S)
+        ctx = ctypes.c_void_p()
+        _abi.check(L.swm_create(ctypes.byref(cfg), ctypes.byref(ctx)), None, "swm_create")
+        try:
+            F, G = S.forcing, S.grid
+            st = [np.asfortranarray(a) for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
+            _abi.check(L.swm_set_static(ctx, *[_abi.ptr(a) for a in st]), ctx, "swm_set_static")
+            arrs = [np.asfortranarray(a) for a in (prog.u, prog.v, prog.η, prog.sst)]
+            _abi.check(L.swm_upload_prog(ctx, *[_abi.ptr(a) for a in arrs]), ctx, "swm_upload_prog")
+            fuv = [np.asfortranarray(a) for a in (G.f_u, G.f_v)]
+            if dyn == "linear":
+                done, nan = ctypes.c_int64(0), ctypes.c_int64(0)
+                with pytest.raises(_abi.SwmError, match="swm_set_static_linear first"):
+                    _abi.check(L.swm_step(ctx, 1, 1, 0, ctypes.byref(done), ctypes.byref(nan)), ctx, "swm_step")
+                _abi.check(L.swm_set_static_linear(ctx, *[_abi.ptr(a) for a in fuv]), ctx, "swm_set_static_linear")
+                _abi.check(L.swm_step(ctx, 1, 1, 0, ctypes.byref(done), ctypes.byref(nan)), ctx, "swm_step")
+                assert (done.value, nan.value) == (1, 0)
+            else:
+                with pytest.raises(_abi.SwmError, match="linear dynamics only"):
+                    _abi.check(L.swm_set_static_linear(ctx, *[_abi.ptr(a) for a in fuv]), ctx, "swm_set_static_linear")
+        finally:
+            L.swm_destroy(ctx)
