@@ -556,6 +556,12 @@ template <class F> struct Launch {
         StageArgs<F> I = A, E = A;
         I.nstrips = b - a; I.sfirst = a;
         E.nstrips = a + (ns - b); E.sfirst = 0; E.sjump_at = a; E.sjump = b - a;
+        // Segments of 4 rows for the seam strips: their waves run alone on their SIMDs, so the
+        // launch's time is the march of one segment (4 rows: 2048x1024 over the ridge 0.113 ms
+        // per step against 0.126 with the pairs' own 8 rows, 0.215 with 32; 4096² 0.565
+        // against 0.577).
+        E.L = 4;
+        E.nseg = nseg(ctx, E.L);
         if (ctx->rec) {                                      // an ensemble's plan: the two launches as items
             ctx->rec->add(4, ma, mb, &I, sizeof I);
             ctx->rec->add(5, ma, mb, &E, sizeof E);
