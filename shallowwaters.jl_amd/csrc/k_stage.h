@@ -432,13 +432,13 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<F> A) {
 }
 
 // the same stage for every member of an ensemble in one launch (tile table `start`)
-template <class F, bool PER, int M>
+template <class F, bool PER, int M, int ADV = ADV_AH>
 __global__ void __launch_bounds__(256) stage_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start,
                                                    int nmem) {
     int m, wl;
     if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
     const StageArgs<F> A = args[m];
-    stage_body<F, PER, M>(A, wl);
+    stage_body<F, PER, M, ADV>(A, wl);
 }
 
 // ------------------------------------------------------------------ two fused RK stages

@@ -1655,16 +1655,22 @@ template <class F> struct Ens {
             const dim3 grid(nblocks_waves(pl.tiles[j]));
             if (pl.kind[j] == 0) {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                if (PER) {
+                const int adv = adv_form(c0);                     // one form for all members
+                if (PER && adv == ADV_AH) {
                     if (pl.ma[j] == M_RK_FIRST) hipLaunchKernelGGL((qend_batch<F, Q>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
                     else hipLaunchKernelGGL((qend_batch<F, ST>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
                 }
+#define SWM_SB(M_)                                                                                                  \
+    if (adv == ADV_SAD) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm);  \
+    else if (adv == ADV_LIN) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm); \
+    else hipLaunchKernelGGL((stage_batch<F, PER, M_>), grid, block, 0, E->stream, a, tab, nm);
                 switch (pl.ma[j]) {
-                case M_RK_FIRST: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_FIRST>), grid, block, 0, E->stream, a, tab, nm); break;
-                case M_RK_MID: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_MID>), grid, block, 0, E->stream, a, tab, nm); break;
-                case M_RK_LAST: hipLaunchKernelGGL((stage_batch<F, PER, M_RK_LAST>), grid, block, 0, E->stream, a, tab, nm); break;
+                case M_RK_FIRST: SWM_SB(M_RK_FIRST) break;
+                case M_RK_MID: SWM_SB(M_RK_MID) break;
+                case M_RK_LAST: SWM_SB(M_RK_LAST) break;
                 default: return E->fail(SWM_E_UNSUPPORTED, "stage mode %d in an ensemble", pl.ma[j]);
                 }
+#undef SWM_SB
             } else if (pl.kind[j] == 1) {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 const bool cs = c0.continuity != SWM_CONT_ITSELF;
@@ -2283,11 +2289,13 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         const swm_config &c = ctx->c;
         const char *why = nullptr;
         if (!ctx->have_static || !ctx->have_prog) why = "static fields and a state must be uploaded";
+        else if (c.dynamics == SWM_LINEAR && !ctx->have_fuv) why = "the linear dynamics need f_u, f_v (swm_set_static_linear)";
         else if (ctx->dev != m0->dev) why = "all members must live on one device";
         else if (c.T != c0.T || c.Tprog != c0.Tprog || c.Tcomm != c0.Tcomm) why = "number formats differ";
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
-        else if (adv_form(c) != ADV_AH) why = "ensembles run the nonlinear Arakawa-Hsu right-hand side only (nstep_advcor = 0)";
+        else if (adv_form(c) != adv_form(c0)) why = "right-hand side form differs (scheme, dynamics, nstep_advcor)";
+        else if (adv_form(c) & ADV_KEEP) why = "nstep_advcor > 0 is not batched (its update kernel runs per context)";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
         else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";
