@@ -863,13 +863,13 @@ __global__ void __launch_bounds__(256) fused_kernel(StageArgs<F> A) {
     fused_body<F, MA, MB, CS, PER, ADV>(A, wave_index());
 }
 
-template <class F, int MA, int MB, bool CS, bool PER = false>
+template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH>
 __global__ void __launch_bounds__(256) fused_batch(const StageArgs<F> *__restrict__ args, const int *__restrict__ start,
                                                    int nmem) {
     int m, wl;
     if (!batch_locate(start, nmem, wave_index(), m, wl)) return;
     const StageArgs<F> A = args[m];
-    fused_body<F, MA, MB, CS, PER>(A, wl);
+    fused_body<F, MA, MB, CS, PER, ADV>(A, wl);
 }
 
 // ------------------------------------------------------------------ semi-implicit continuity

@@ -472,7 +472,7 @@ template <class F> struct Launch {
         // Sadourny, the linear dynamics, kept q and KE: the pairs of RK4 with T = Tprog, no
         // continuity static, on a seam the lanes can wrap around
         return !ctx->mixed && c.continuity == SWM_CONT_ITSELF && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) &&
-               !ctx->rec && ((ma == M_RK_FIRST && mb == M_RK_MID) || (ma == M_RK_MID && mb == M_RK_LAST));
+               ((ma == M_RK_FIRST && mb == M_RK_MID) || (ma == M_RK_MID && mb == M_RK_LAST));
     }
     static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
                      const Coef<Q> &kb, const Times &T) {
@@ -1671,6 +1671,20 @@ template <class F> struct Ens {
                 default: return E->fail(SWM_E_UNSUPPORTED, "stage mode %d in an ensemble", pl.ma[j]);
                 }
 #undef SWM_SB
+            } else if (pl.kind[j] == 1 && adv_form(c0) != ADV_AH) {   // Sadourny / linear members (RK4, T = Tprog)
+                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
+                if constexpr (!F::mixed) {
+                    const int adv = adv_form(c0);
+#define SWM_FBV(MA_, MB_)                                                                                          \
+    if (adv == ADV_SAD) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm); \
+    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm)
+                    if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FBV(M_RK_FIRST, M_RK_MID); }
+                    else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FBV(M_RK_MID, M_RK_LAST); }
+                    else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
+#undef SWM_FBV
+                } else {
+                    return E->fail(SWM_E_UNSUPPORTED, "fused stages of this right-hand side with mixed formats");
+                }
             } else if (pl.kind[j] == 1) {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 const bool cs = c0.continuity != SWM_CONT_ITSELF;

@@ -116,16 +116,18 @@ def test_ensemble_rejects_mixed_options(swm, gpu, kb):
 
 
 @pytest.mark.parametrize("kw", [dict(adv_scheme="Sadourny"), dict(dynamics="linear")])
-@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat"])
 def test_ensemble_of_other_right_hand_sides(swm, gpu, kw, bc):
-    """Members with Sadourny's scheme or the linear dynamics (batched one kernel per stage)
-    equal their own single-context runs bit for bit."""
+    """Members with Sadourny's scheme or the linear dynamics equal their own single-context
+    runs bit for bit: fused batched pairs (nonperiodic, and periodic on a flat bottom, whose
+    seams the lanes wrap around), batched single stages (periodic over a seamount)."""
     n = 8
     rng = np.random.default_rng(3)
     ms = []
+    topo = "flat" if bc == "periodic-flat" else "seamount"
     for k, (nx, ratio) in enumerate(SIZES[:3]):
-        P = swm.Parameter(T="Float32", nx=nx, L_ratio=ratio, bc=bc, Fx0=float(rng.uniform(0.08, 0.16)),
-                          topography="seamount", wind_forcing_x="double_gyre", Uadv=30.0, **kw)
+        P = swm.Parameter(T="Float32", nx=nx, L_ratio=ratio, bc=bc.split("-")[0], Fx0=float(rng.uniform(0.08, 0.16)),
+                          topography=topo, wind_forcing_x="double_gyre", Uadv=30.0, **kw)
         S = swm.make_setup(P)
         ms.append((S, swm.seeded_initial_conditions(S, seed=200 + k, σ=0.6)))
     ref = [run_alone(swm, S, p, n) for S, p in ms]
