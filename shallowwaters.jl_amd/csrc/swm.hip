@@ -92,7 +92,9 @@ double round_host(double x, int fmt) {
 }  // namespace
 
 // Kernels of a step recorded instead of launched (ensemble planning): kind 0 single stage
-// (mode ma), 1 fused pair (ma, mb), 2 dissipation, 3 semi-implicit continuity; args bytes.
+// (mode ma), 1 fused pair (ma, mb), 2 dissipation, 3 semi-implicit continuity, 4 / 5 periodic
+// pair strips clear of the seam / at it, 6 update of the kept q and KEu+KEv (ma: bit 0 q
+// too, bit 1 from the initial state); args bytes.
 struct RecItem {
     int kind, ma, mb;
     std::vector<char> args;
@@ -453,6 +455,10 @@ template <class F> struct Launch {
         const swm_config &c = ctx->c;
         const Times T{mkR<R>(1.0), mkR<R>(1.0), mkR<R>(0.0)};
         const StageArgs<F> A = args(ctx, s, Slots{}, Slots{}, Slots{}, Slots{}, Coef<Q>{}, T);
+        if (ctx->rec) {
+            ctx->rec->add(6, (init ? 2 : 0) | (wq ? 1 : 0), -1, &A, sizeof A);
+            return SWM_OK;
+        }
         const dim3 g((c.nx + 2 + 255) / 256, c.ny + 2), b(256);
         if (init) hipLaunchKernelGGL((advcor_kernel<F, Q, true, true>), g, b, 0, ctx->stream, A);
         else if (wq) hipLaunchKernelGGL((advcor_kernel<F, Q, true, false>), g, b, 0, ctx->stream, A);
@@ -1572,6 +1578,10 @@ template <class F> struct Ens {
             ctx->rec = &recs[k];
             Slots u0, u1;
             int rc = Launch<F>::stages(ctx, 1 << 20, (i - 1) * ctx->c.dtint, &u0, &u1);
+            if (!rc && (adv_form(ctx->c) & ADV_KEEP)) {          // time_integration.jl:208-211; the quadratic
+                const bool due = i % ctx->c.nstep_advcor == 0;   // drag's KEu, KEv every step (nstep_diff = 1)
+                if (due || ctx->c.bottom_drag == SWM_DRAG_QUADRATIC) rc = Launch<F>::advcor(ctx, u0, due, false);
+            }
             const int out = pick({ctx->pu, u0.uv});
             if (!rc) rc = Launch<F>::dissipation(ctx, u0, out);
             ctx->rec = nullptr;
@@ -1622,6 +1632,14 @@ template <class F> struct Ens {
             pl.tiles.push_back(start.back());
             pl.qblocks.push_back(qstart.back());
         }
+        // the kernel sequence itself closes the plan: two steps whose arguments are the same
+        // bytes but whose kernels differ (the kept fields' update with or without q) must not
+        // compare equal, or a buffer or graph of the one would run for the other
+        align();
+        for (size_t j = 0; j < nk; ++j) {
+            const int t[3] = {pl.kind[j], pl.ma[j], pl.mb[j]};
+            pl.blob.insert(pl.blob.end(), (const char *)t, (const char *)(t + 3));
+        }
         return SWM_OK;
     }
 
@@ -1663,6 +1681,8 @@ template <class F> struct Ens {
 #define SWM_SB(M_)                                                                                                  \
     if (adv == ADV_SAD) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm);  \
     else if (adv == ADV_LIN) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm); \
+    else if (adv == (ADV_AH | ADV_KEEP)) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_AH | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
+    else if (adv == (ADV_SAD | ADV_KEEP)) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_SAD | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
     else hipLaunchKernelGGL((stage_batch<F, PER, M_>), grid, block, 0, E->stream, a, tab, nm);
                 switch (pl.ma[j]) {
                 case M_RK_FIRST: SWM_SB(M_RK_FIRST) break;
@@ -1675,9 +1695,13 @@ template <class F> struct Ens {
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 if constexpr (!F::mixed) {
                     const int adv = adv_form(c0);
+                    if (adv != ADV_SAD && adv != ADV_LIN && adv != (ADV_AH | ADV_KEEP) && adv != (ADV_SAD | ADV_KEEP))
+                        return E->fail(SWM_E_UNSUPPORTED, "right-hand side form %d of fused stages in an ensemble", adv);
 #define SWM_FBV(MA_, MB_)                                                                                          \
     if (adv == ADV_SAD) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm); \
-    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm)
+    else if (adv == ADV_LIN) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm); \
+    else if (adv == (ADV_AH | ADV_KEEP)) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_AH | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
+    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_SAD | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm)
                     if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FBV(M_RK_FIRST, M_RK_MID); }
                     else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FBV(M_RK_MID, M_RK_LAST); }
                     else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
@@ -1719,6 +1743,16 @@ template <class F> struct Ens {
                 else if (c0.bottom_drag == SWM_DRAG_LINEAR) { if (smag) SWM_DB(1, true); else SWM_DB(1, false); }
                 else { if (smag) SWM_DB(2, true); else SWM_DB(2, false); }
 #undef SWM_DB
+            } else if (pl.kind[j] == 6) {                        // kept q and KEu+KEv: one launch per member
+                for (int k = 0; k < nm; ++k) {
+                    StageArgs<F> A;
+                    memcpy(&A, pl.blob.data() + pl.args_off[j] + (size_t)k * sizeof A, sizeof A);
+                    const swm_config &c = E->m[k]->c;
+                    const dim3 g((c.nx + 2 + 255) / 256, c.ny + 2), b(256);
+                    if (pl.ma[j] & 2) hipLaunchKernelGGL((advcor_kernel<F, Q, true, true>), g, b, 0, E->stream, A);
+                    else if (pl.ma[j] & 1) hipLaunchKernelGGL((advcor_kernel<F, Q, true, false>), g, b, 0, E->stream, A);
+                    else hipLaunchKernelGGL((advcor_kernel<F, Q, false, false>), g, b, 0, E->stream, A);
+                }
             } else return E->fail(SWM_E_UNSUPPORTED, "kernel kind %d in an ensemble", pl.kind[j]);
         }
         if (hipGetLastError() != hipSuccess) return E->fail(SWM_E_HIP, "batched launch");
@@ -2309,7 +2343,7 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
         else if (c.bc != c0.bc || c.time_scheme != SWM_RK || c.RKo != c0.RKo) why = "boundary condition or time scheme differs (classic RK only)";
         else if (c.bottom_drag != c0.bottom_drag || c.diffusion != c0.diffusion || c.continuity != c0.continuity) why = "drag, diffusion or continuity differs";
         else if (adv_form(c) != adv_form(c0)) why = "right-hand side form differs (scheme, dynamics, nstep_advcor)";
-        else if (adv_form(c) & ADV_KEEP) why = "nstep_advcor > 0 is not batched (its update kernel runs per context)";
+        else if (c.nstep_advcor != c0.nstep_advcor) why = "nstep_advcor differs";
         else if (c.nstep_diff != 1) why = "nstep_diff must be 1";
         else if (c.compensated || c.scale_log2 != c0.scale_log2) why = "compensated members, or members of different scale";
         else if (ctx->fuse != m0->fuse) why = "stage fusion setting differs";

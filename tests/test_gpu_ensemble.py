@@ -101,8 +101,8 @@ def test_tuned_ensemble_in_two_calls(swm, gpu, bc, T, Tprog):
 @pytest.mark.parametrize("kb", [dict(bc="nonperiodic"), dict(bc="periodic", adv_scheme="Sadourny"),
                                 dict(bc="periodic", dynamics="linear"), dict(bc="periodic", nstep_advcor=2)])
 def test_ensemble_rejects_mixed_options(swm, gpu, kb):
-    """Members must share formats, boundary condition and scheme; the batched kernels are of
-    the Arakawa-Hsu scheme, so a Sadourny member is refused."""
+    """Members must share formats, boundary condition and right-hand-side form (scheme,
+    dynamics, nstep_advcor > 0): one batched kernel sequence serves them all."""
     a = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, bc="periodic")))
     b = swm.Simulation(swm.make_setup(swm.Parameter(nx=40, **kb)))
     try:
@@ -115,12 +115,16 @@ def test_ensemble_rejects_mixed_options(swm, gpu, kb):
         b.close()
 
 
-@pytest.mark.parametrize("kw", [dict(adv_scheme="Sadourny"), dict(dynamics="linear")])
+@pytest.mark.parametrize("kw", [dict(adv_scheme="Sadourny"), dict(dynamics="linear"), dict(nstep_advcor=2),
+                                dict(adv_scheme="Sadourny", nstep_advcor=3)])
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat"])
 def test_ensemble_of_other_right_hand_sides(swm, gpu, kw, bc):
-    """Members with Sadourny's scheme or the linear dynamics equal their own single-context
-    runs bit for bit: fused batched pairs (nonperiodic, and periodic on a flat bottom, whose
-    seams the lanes wrap around), batched single stages (periodic over a seamount)."""
+    """Members with Sadourny's scheme, the linear dynamics or q and KEu+KEv kept between
+    nstep_advcor updates equal their own single-context runs bit for bit: fused batched
+    pairs (nonperiodic, and periodic on a flat bottom, whose seams the lanes wrap around),
+    batched single stages (periodic over a seamount); the kept fields' updates (from the
+    initial state, every nstep_advcor-th step, KEu+KEv after each quadratic-drag step) run
+    per member inside the step's plan."""
     n = 8
     rng = np.random.default_rng(3)
     ms = []
