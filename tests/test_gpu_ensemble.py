@@ -115,6 +115,19 @@ def test_ensemble_rejects_mixed_options(swm, gpu, kb):
         b.close()
 
 
+def test_ensemble_rejects_different_nstep_advcor(swm, gpu):
+    """Members keeping q and KEu+KEv update them on the same steps: one kernel sequence."""
+    sims = [swm.Simulation(swm.make_setup(swm.Parameter(nx=40, nstep_advcor=n))) for n in (2, 3)]
+    try:
+        for sim in sims:
+            sim.upload(swm.seeded_initial_conditions(sim.S, seed=1, σ=0.1))
+        with pytest.raises(Exception):
+            swm.Ensemble(sims)
+    finally:
+        for sim in sims:
+            sim.close()
+
+
 @pytest.mark.parametrize("kw", [dict(adv_scheme="Sadourny"), dict(dynamics="linear"), dict(nstep_advcor=2),
                                 dict(adv_scheme="Sadourny", nstep_advcor=3)])
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat"])
