@@ -1887,7 +1887,35 @@ template <class F> struct Ens {
 
     // Autotuner: for each kernel class (stage, fused pair, dissipation) try segment lengths
     // and keep the one with the fastest batched step; the others held at their choice.
+    // The timed steps leave each member's state in its slots, but they also update the kept
+    // q and KEu+KEv (nstep_advcor > 0) and take the pending update from the initial state:
+    // both are saved before timing and put back after it.
     static int tune(swm_ens *E, int reps, int *Ls, int *Lf, int *Ld) {
+        struct Kept { swm_ctx *ctx; char *buf; size_t n; bool pending; };
+        std::vector<Kept> kept;
+        int rc = hipStreamSynchronize(E->stream) == hipSuccess ? SWM_OK : E->fail(SWM_E_HIP, "stream synchronize");
+        for (swm_ctx *ctx : E->m) {
+            if (rc || !(adv_form(ctx->c) & ADV_KEEP)) continue;
+            Kept k{ctx, nullptr, ctx->elems * (size_t)ctx->qs.es, ctx->advcor_pending};
+            if (hipMalloc((void **)&k.buf, 2 * k.n) != hipSuccess) { rc = E->fail(SWM_E_HIP, "kept fields' copy"); break; }
+            kept.push_back(k);
+            if (hipMemcpy(k.buf, ctx->qs.base, k.n, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(k.buf + k.n, ctx->ks.base, k.n, hipMemcpyDeviceToDevice) != hipSuccess)
+                rc = E->fail(SWM_E_HIP, "kept fields' copy");
+        }
+        if (!rc) rc = tune_segments(E, reps, Ls, Lf, Ld);
+        if (hipStreamSynchronize(E->stream) != hipSuccess && !rc) rc = E->fail(SWM_E_HIP, "stream synchronize");
+        for (const Kept &k : kept) {
+            if ((hipMemcpy(k.ctx->qs.base, k.buf, k.n, hipMemcpyDeviceToDevice) != hipSuccess ||
+                 hipMemcpy(k.ctx->ks.base, k.buf + k.n, k.n, hipMemcpyDeviceToDevice) != hipSuccess) && !rc)
+                rc = E->fail(SWM_E_HIP, "kept fields' restore");
+            k.ctx->advcor_pending = k.pending;
+            (void)hipFree(k.buf);
+        }
+        return rc;
+    }
+
+    static int tune_segments(swm_ens *E, int reps, int *Ls, int *Lf, int *Ld) {
         static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48};
         for (int cls = 0; cls < 3; ++cls) {
             double best = 1e300;
