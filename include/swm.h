@@ -172,7 +172,10 @@ int swm_ens_destroy(swm_ens *ens);
 const char *swm_ens_last_error(const swm_ens *ens);
 /* Autotuner: time the batched step for candidate rows per wave segment (4..48) of the stage,
  * fused-pair and dissipation kernels in turn and keep the fastest; the choices are returned
- * (pointers may be NULL). */
+ * (pointers may be NULL). The timed steps leave every member as it was (state, and the kept
+ * q and KEu+KEv of nstep_advcor > 0), so it may run before the first step or between steps.
+ * Tuning changes only which rows per segment the kernels use. A step's results are the same
+ * for any segment length. */
 int swm_ens_tune(swm_ens *ens, int32_t reps, int32_t *rows_stage, int32_t *rows_fused, int32_t *rows_diss);
 /* Steps i0 .. i0+nsteps-1 of every member, member m at model time (i-1)*dtint of m.  NaNs
  * are checked every nout steps of each member (feedback.jl:136-148); after a step in which
