@@ -120,7 +120,7 @@ template <class R> __global__ void fold_out_kernel(ArenaRef ar, adj::FoldOutArgs
 }
 
 // zero n elements of size 4 (a fill kernel: usable where a memset node is not)
-__global__ void zero_words_kernel(uint32_t *p, size_t n) {
+static __global__ void zero_words_kernel(uint32_t *p, size_t n) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0u;
 }
 

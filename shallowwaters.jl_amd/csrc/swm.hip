@@ -13,915 +13,10 @@
 // RK stages is held in T: the reference converts it to T before rhs! and uses it for
 // nothing else (time_integration.jl:57-64), so two extra slots of T storage replace the
 // Tprog u1 without changing a bit of the result.
-#include <hip/hip_runtime.h>
 
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <initializer_list>
-#include <string>
-#include <vector>
-
-#include "../../include/swm.h"
-#include "k_adjoint.h"
-#include "k_dissipation.h"
-#include "k_misc.h"
-#include "k_stage.h"
-
-using namespace swm;
-
-#define HIPCHK(expr)                                                                                \
-    do {                                                                                            \
-        hipError_t e_ = (expr);                                                                     \
-        if (e_ != hipSuccess) return ctx->fail(SWM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+#include "swm_host.h"
 
 namespace {
-
-struct Field {             // one padded device array, carved from the context's arena
-    void *base = nullptr;  // start of the field
-    char *p0 = nullptr;    // element (0,0)
-    uint32_t origin = 0;   // byte offset of base in the arena
-    int es = 4;            // element size of its storage format
-    uint32_t ref() const { return origin; }
-};
-
-struct Slots {             // where a set (u, v, η) lives: slot indices, in the T pool if t
-    int uv = -1, e = -1;
-    bool t = false;
-};
-
-// host rounding of a Float64 value to a number format (Julia's T(x), one rounding)
-float f32_round_odd_host(double x) {
-    float f = (float)x;
-    if ((double)f != x && std::isfinite(f)) {
-        uint32_t b;
-        std::memcpy(&b, &f, 4);
-        if (std::fabs((double)f) > std::fabs(x)) b -= 1u;
-        b |= 1u;
-        std::memcpy(&f, &b, 4);
-    }
-    return f;
-}
-double round_host(double x, int fmt) {
-    if (fmt == SWM_F64 || std::isnan(x)) return x;
-    if (fmt == SWM_F32) return (double)(float)x;
-    const float f = f32_round_odd_host(x);
-    if (fmt == SWM_BF16) {
-        uint32_t b;
-        std::memcpy(&b, &f, 4);
-        b += 0x7FFFu + ((b >> 16) & 1u);
-        b &= 0xFFFF0000u;
-        float r;
-        std::memcpy(&r, &b, 4);
-        return r;
-    }
-    // binary16: quantum 2^-24 below 2^-14, 2^(e-10) above; overflow to infinity
-    const double a = std::fabs((double)f);
-    if (std::isinf(f) || a >= 65520.0) return std::copysign(INFINITY, x);
-    int e;
-    std::frexp(a, &e);                             // a = m * 2^e, m in [0.5, 1)
-    const int q = std::max(e - 11, -24);           // log2 of the quantum
-    const double r = std::nearbyint(std::ldexp(a, -q));
-    return std::copysign(std::ldexp(r, q), x);
-}
-
-}  // namespace
-
-// Kernels of a step recorded instead of launched (ensemble planning): kind 0 single stage
-// (mode ma), 1 fused pair (ma, mb), 2 dissipation, 3 semi-implicit continuity, 4 / 5 periodic
-// pair strips clear of the seam / at it, 6 update of the kept q and KEu+KEv (ma: bit 0 q
-// too, bit 1 from the initial state); args bytes.
-struct RecItem {
-    int kind, ma, mb;
-    std::vector<char> args;
-};
-struct Recorder {
-    std::vector<RecItem> items;
-    void add(int kind, int ma, int mb, const void *a, size_t n) {
-        RecItem it{kind, ma, mb, std::vector<char>((const char *)a, (const char *)a + n)};
-        items.push_back(std::move(it));
-    }
-};
-
-struct swm_ctx {
-    swm_config c;
-    Recorder *rec = nullptr;   // set while an ensemble plans a step
-    std::string err;
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;   // periodic: the seam strips of a fused pair, beside the others
-    hipEvent_t fork = nullptr, join = nullptr;
-    int fmt = SWM_F32;     // format bundle: SWM_F64, SWM_F32 (T = Tprog), SWM_F16, SWM_BF16 (T, Tprog = F32)
-    int es = 4;            // element size of Tprog fields
-    int est = 4;           // element size of T fields
-    bool mixed = false;
-    int P = 0, rows = 0;   // pitch (elements) and allocated rows of 2-D fields
-    size_t elems = 0;
-    int ep = 0, nux = 0, nuy = 0, nvx = 0, nvy = 0;
-    bool fuse = true;      // run consecutive RK stages as fused pairs where supported
-    int L = 64;            // rows per wave segment of the stage kernels
-    int Ld = 64;           // rows per wave segment of the dissipation kernel
-    int Lf = 64;           // rows per wave segment of the fused stage pairs
-    Field u[6], v[6], e[6], sst[2];                          // Tprog (slots 4, 5: the adjoint's taped step)
-    Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
-    Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
-    Field f_u, f_v;                                          // T: f on the u-, v-grid (linear dynamics)
-    Field qs, ks;                                            // T: kept q and KEu+KEv (nstep_advcor > 0)
-    bool advcor_pending = false;   // a new state: q, KEu+KEv to be formed from it before its first stage
-    Field cu, cv, ce;                                        // Kahan compensations (compensated RK), Tprog
-    double sc = 1.0, sci = 1.0;                              // u, v carried as sc·u, sc·v
-    int Fx_sx = 1, Fx_sy = 0, Fy_sx = 1, Fy_sy = 0, fq_sx = 1, fq_sy = 0, fu_sx = 1, fu_sy = 0, fv_sx = 1, fv_sy = 0;
-    bool have_fuv = false;   // swm_set_static_linear done
-    bool seam_ok = false;    // periodic: the fused stage pairs may wrap around the seam (set_static)
-    int pu = 0, pe = 0;    // slots holding Prog u,v and Prog η
-    int rsst = 0;
-    Slots rk0, rk1;        // u0 / u1 after swm_stages
-    int *dflags = nullptr; // [0] NaN flag, [1] tracer index error
-    double *dpart = nullptr;
-    bool have_static = false, have_prog = false;
-    int nslots = 4;                // Tprog slot sets in use (6 once the adjoint is set up)
-    int ens_refs = 0;              // ensembles holding this context (their plans point into the arena)
-    struct AdjState *adj = nullptr;
-    std::vector<void *> allocs;
-    void *arena = nullptr;         // every field of the context; kernels address it through one buffer descriptor
-    size_t arena_bytes = 0, arena_used = 0;
-    ArenaRef aref() const { return ArenaRef{arena, (uint32_t)arena_bytes}; }
-
-    int fail(int code, const char *fmt_, ...) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt_);
-        vsnprintf(buf, sizeof buf, fmt_, ap);
-        va_end(ap);
-        err = buf;
-        return code;
-    }
-    template <class S> S *at(const Field &f) const { return (S *)f.p0; }
-    const Field &fu(Slots s) const { return s.t ? tu[s.uv] : u[s.uv]; }
-    const Field &fv(Slots s) const { return s.t ? tv[s.uv] : v[s.uv]; }
-    const Field &fe(Slots s) const { return s.t ? te[s.e] : e[s.e]; }
-};
-
-namespace {
-
-// the lowest slot not in `busy` (four slots always leave one free: at most three are live)
-int pick(std::initializer_list<int> busy) {
-    for (int s = 0; s < 4; ++s) {
-        bool used = false;
-        for (int b : busy) used |= b == s;
-        if (!used) return s;
-    }
-    return -1;
-}
-
-// Carve a field from the arena (zeroed at creation).  A 2-D field has P*rows elements of
-// size es; a 1-D per-row field `elems_1d`.
-int alloc_field(swm_ctx *ctx, Field &f, size_t elems_1d, bool two_d, int es) {
-    const size_t n = two_d ? ctx->elems : elems_1d;
-    const size_t at = (ctx->arena_used + 255) & ~(size_t)255;
-    if (at + n * es > ctx->arena_bytes) return ctx->fail(SWM_E_ARG, "arena exhausted");
-    ctx->arena_used = at + n * es;
-    f.base = (char *)ctx->arena + at;
-    f.origin = (uint32_t)at;
-    f.es = es;
-    f.p0 = (char *)f.base + (two_d ? ((size_t)OX + (size_t)OY * ctx->P) : (size_t)OY) * es;
-    return SWM_OK;
-}
-
-// the 1-D per-row layout of a field carved as 2-D (x-invariant static fields)
-void as_rows(Field &f) { f.p0 = (char *)f.base + (size_t)OY * f.es; }
-
-// element (x, y) of a 2-D field, as a byte pointer
-inline char *elem(const swm_ctx *ctx, const Field &f, int x, int y) {
-    return f.p0 + ((ptrdiff_t)x + (ptrdiff_t)y * ctx->P) * f.es;
-}
-
-// copy a Julia (m, n) host array whose element [1,1] sits at (x0, y0)
-int put2d(swm_ctx *ctx, const Field &f, const void *host, int m, int n, int x0, int y0) {
-    HIPCHK(hipMemcpy2DAsync(elem(ctx, f, x0, y0), (size_t)ctx->P * f.es, host, (size_t)m * f.es, (size_t)m * f.es, n,
-                            hipMemcpyHostToDevice, ctx->stream));
-    return SWM_OK;
-}
-int get2d(swm_ctx *ctx, const Field &f, void *host, int m, int n, int x0, int y0) {
-    HIPCHK(hipMemcpy2DAsync(host, (size_t)m * f.es, elem(ctx, f, x0, y0), (size_t)ctx->P * f.es, (size_t)m * f.es, n,
-                            hipMemcpyDeviceToHost, ctx->stream));
-    return SWM_OK;
-}
-
-// a T-format field downloaded into a Tprog (Float32) host array: exact widening
-int get2d_widen(swm_ctx *ctx, const Field &f, void *host, int m, int n, int x0, int y0) {
-    std::vector<uint16_t> tmp((size_t)m * n);
-    int rc = get2d(ctx, f, tmp.data(), m, n, x0, y0);
-    if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    float *out = (float *)host;
-    for (size_t k = 0; k < tmp.size(); ++k) {
-        uint32_t b;
-        if (ctx->fmt == SWM_BF16) b = (uint32_t)tmp[k] << 16;
-        else {                                     // binary16 -> binary32
-            const uint32_t h = tmp[k], s = (h >> 15) << 31, ex = (h >> 10) & 0x1f, mt = h & 0x3ff;
-            if (ex == 0) {
-                const float r = std::ldexp((float)mt, -24);
-                std::memcpy(&b, &r, 4);
-                b |= s;
-            } else if (ex == 31) b = s | 0x7f800000u | (mt << 13);
-            else b = s | ((ex + 112) << 23) | (mt << 13);
-        }
-        std::memcpy(&out[k], &b, 4);
-    }
-    return SWM_OK;
-}
-
-// A static field that does not vary along x is kept as one value per row (stride 0 in x).
-int put_static(swm_ctx *ctx, Field &f, int *sx, int *sy, const void *host, int m, int n, int x0, int y0) {
-    const char *h = (const char *)host;
-    const int es = f.es;
-    bool xinv = true;
-    for (int j = 0; j < n && xinv; ++j)
-        for (int i = 1; i < m && xinv; ++i)
-            xinv = std::memcmp(h + ((size_t)j * m + i) * es, h + (size_t)j * m * es, es) == 0;
-    if (xinv) {
-        as_rows(f);
-        std::vector<char> col((size_t)n * es);
-        for (int j = 0; j < n; ++j) std::memcpy(&col[(size_t)j * es], h + (size_t)j * m * es, es);
-        HIPCHK(hipMemcpyAsync(f.p0 + (ptrdiff_t)y0 * es, col.data(), col.size(), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        *sx = 0;
-        *sy = 1;
-        return SWM_OK;
-    }
-    f.p0 = (char *)f.base + ((size_t)OX + (size_t)OY * ctx->P) * es;
-    *sx = 1;
-    *sy = ctx->P;
-    return put2d(ctx, f, host, m, n, x0, y0);
-}
-
-// blocks for nwaves wavefronts, padded to a multiple of the XCD count (wave_tile's remap)
-inline int nblocks_waves(int nwaves) {
-    const int nb = (nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-    return (nb + NXCD - 1) / NXCD * NXCD;
-}
-
-// the right-hand side's form: the PV scheme or the linear dynamics, and whether q and the
-// kinetic part of the Bernoulli potential are kept between updates (nstep_advcor > 0 with the
-// nonlinear dynamics, rhs.jl:36-38, time_integration.jl:208)
-inline int adv_form(const swm_config &c) {
-    if (c.dynamics == SWM_LINEAR) return ADV_LIN;
-    const int s = c.adv_scheme == SWM_ADV_SADOURNY ? ADV_SAD : ADV_AH;
-    return c.nstep_advcor > 0 ? (s | ADV_KEEP) : s;
-}
-
-// a T-format scalar the host already rounded (config values marked (T)) as the kernel's R
-template <class R> R mkR(double d) {
-    if constexpr (std::is_same_v<R, double> || std::is_same_v<R, float>) return (R)d;
-    else {
-        R r;
-        r.v = (float)d;
-        return r;
-    }
-}
-
-template <class F> struct Launch {
-    using R = typename F::R;
-    using Q = typename F::Q;
-    using ST = typename F::ST;
-
-    static int nstrips(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT - 1) / WOUT; }
-    static int nstrips2(const swm_ctx *ctx) { return (ctx->c.nx + 1 + WOUT2 - 1) / WOUT2; }
-    static int nseg(const swm_ctx *ctx, int L) { return (ctx->c.ny + L - 1) / L; }
-
-    // Rows per wave segment of `kernel` (strips of `nstr`), for `waves_per_slot` resident
-    // waves per wave slot the kernel's occupancy gives (not below lmin rows).  A segment
-    // re-reads the halo rows it shares with its neighbours; the stage kernels read their top
-    // halo at the start, while the segment above reads the same rows, so the re-read stays
-    // in L2 and more, shorter segments pay.  Measured at 4096^2 on MI355X: about six waves
-    // per slot for the stage kernels, five for the fused pairs, two for dissipation.
-    static int rows_per_wave(const swm_ctx *ctx, const void *kernel, int nstr, int waves_per_slot, int lmin) {
-        int cus = 0, bpc = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->dev) != hipSuccess || cus <= 0) cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel, WAVE * WAVES_PER_BLOCK, 0) != hipSuccess || bpc <= 0)
-            bpc = 1;
-        const long slots = (long)cus * bpc * WAVES_PER_BLOCK * waves_per_slot;
-        const long ny = ctx->c.ny;
-        const long L = (ny * nstr + slots - 1) / slots;
-        return (int)std::max<long>(lmin, std::min<long>(L, std::max<long>(ny, lmin)));
-    }
-    template <bool PER> static void segment_lengths(swm_ctx *ctx) {
-        ctx->L = rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_kernel<F, PER, M_RK_MID>), nstrips(ctx), 6, 4);
-        ctx->Ld = rows_per_wave(ctx, reinterpret_cast<const void *>(&dissipation_kernel<F, PER, 0, true>), nstrips(ctx), 2, 8);
-        ctx->Lf = rows_per_wave(ctx, reinterpret_cast<const void *>(&fused_kernel<F, M_RK_MID, M_RK_LAST, false>),
-                                nstrips2(ctx), 5, 8);
-    }
-
-    static StaticRefs statics(const swm_ctx *ctx) {
-        StaticRefs s;
-        s.H = ctx->H.ref();
-        s.Fx = ctx->Fx.ref(); s.Fx_2d = ctx->Fx_sx;
-        s.Fy = ctx->Fy.ref(); s.Fy_2d = ctx->Fy_sx;
-        s.fq = ctx->fq.ref(); s.fq_2d = ctx->fq_sx;
-        s.eta_ref = ctx->eta_ref.ref();
-        s.Feta = ctx->Feta.ref();
-        s.fu = ctx->f_u.ref(); s.fu_2d = ctx->fu_sx;
-        s.fv = ctx->f_v.ref(); s.fv_2d = ctx->fv_sx;
-        return s;
-    }
-    static uint32_t ou(const swm_ctx *ctx, Slots s) { return s.uv < 0 ? 0u : ctx->fu(s).ref(); }
-    static uint32_t ov(const swm_ctx *ctx, Slots s) { return s.uv < 0 ? 0u : ctx->fv(s).ref(); }
-    static uint32_t oe(const swm_ctx *ctx, Slots s) { return s.e < 0 ? 0u : ctx->fe(s).ref(); }
-
-    struct Times { R Fxt, Fyt, Fet; };
-    static Times forcing_times(const swm_ctx *ctx, int64_t t) {     // Ftime, forcing.jl:215-217: T(sin(ω t))
-        const swm_config &c = ctx->c;
-        Times T;
-        T.Fxt = mkR<R>(c.seasonal_wind_x ? round_host(std::sin(c.omegaFx * (double)t), c.T) : 1.0);
-        T.Fyt = mkR<R>(c.seasonal_wind_y ? round_host(std::sin(c.omegaFy * (double)t), c.T) : 1.0);
-        T.Fet = mkR<R>(c.continuity == SWM_CONT_FORCING ? round_host(std::sin(c.omegaFeta * (double)t), c.T) : 0.0);
-        return T;
-    }
-
-    static StageArgs<F> args(swm_ctx *ctx, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
-                             const Times &T) {
-        const swm_config &c = ctx->c;
-        // every byte set (padding included): an ensemble compares the recorded arguments of
-        // consecutive steps byte for byte to replay them as one graph
-        StageArgs<F> A;
-        std::memset((void *)&A, 0, sizeof A);
-        A.sfirst = 0; A.sjump_at = 1 << 30; A.sjump = 0;
-        A.ar = ctx->aref();
-        A.in_u = ou(ctx, in); A.in_v = ov(ctx, in); A.in_e = oe(ctx, in);
-        A.pr_u = ou(ctx, pr); A.pr_v = ov(ctx, pr); A.pr_e = oe(ctx, pr);
-        A.ac_u = ou(ctx, ac); A.ac_v = ov(ctx, ac); A.ac_e = oe(ctx, ac);
-        A.o1_u = ou(ctx, o1); A.o1_v = ov(ctx, o1); A.o1_e = oe(ctx, o1);
-        A.o2_u = ou(ctx, o2); A.o2_v = ov(ctx, o2); A.o2_e = oe(ctx, o2);
-        A.qend = ctx->qend.ref();
-        A.qs = ctx->qs.ref(); A.ks = ctx->ks.ref();
-        A.st = statics(ctx);
-        A.k = k;
-        A.Fxt = T.Fxt; A.Fyt = T.Fyt; A.Fet = T.Fet;
-        // scaled velocities: PV weights /s, Bernoulli ½ /s and g·s (powers of two, exact)
-        A.g = mkR<R>(c.g * ctx->sc); A.gamma = mkR<R>(c.gamma); A.k12 = mkR<R>(round_host(c.adv_scheme == SWM_ADV_SADOURNY ? 0.5 : 1.0 / 12.0, c.T) * ctx->sci);
-        A.halfp = mkR<R>(0.5 * ctx->sci);
-        A.oma = (Q)c.one_minus_alpha;
-        A.sinv = (Q)ctx->sci;
-        A.scaled = c.scale_log2 != 0;
-        A.c_u = ctx->cu.ref(); A.c_v = ctx->cv.ref(); A.c_e = ctx->ce.ref();
-        A.cont = c.continuity; A.cm = c.Tcomm;
-        A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->L;
-        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->L);
-        return A;
-    }
-
-    // Every buffer a mode reads or writes must name a slot (no launch through a null base),
-    // and the stage input / first output live in the pool the mode's formats expect.
-    static bool slots_ok(const swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, bool cont) {
-        const bool rk = m <= M_RK_LAST || m == M_RKC_FIRST || m == M_RKC_LAST;
-        const bool need_pr = m == M_RK_MID || m == M_SSP2_LAST || m == M_S4_3 || m == M_RKC_LAST;
-        const bool need_ac = m == M_RK_MID || m == M_RK_LAST || m == M_SSP3_KN || m == M_RKC_LAST;
-        const bool out1 = m != M_RK_LAST && m != M_RKC_LAST;
-        const bool out2 = m == M_RK_FIRST || m == M_RK_MID || m == M_RK_LAST || m == M_SSP2_LAST || (m == M_S4_3 && !cont) ||
-                          m == M_RKC_FIRST || m == M_RKC_LAST;
-        if (m == M_RKC_LAST && !ctx->cu.base) return false;
-        auto ok = [ctx](Slots s, bool uv, bool e, bool tpool) {
-            const int n = tpool ? 2 : ctx->nslots;
-            if (s.t != tpool) return false;
-            return (!uv || (s.uv >= 0 && s.uv < n)) && (!e || (s.e >= 0 && s.e < n));
-        };
-        const bool uv = !cont, e = rk || cont;            // which of (u,v) and η the kernel touches
-        const bool tin = ctx->mixed && m != M_RK_FIRST && m != M_RKC_FIRST;   // stage input u1 in the T pool
-        const bool tout = ctx->mixed;                     // first output u1 in the T pool
-        if (!ok(in, uv, e, tin)) return false;
-        if (need_pr && !ok(pr, uv, e, false)) return false;
-        if (need_ac && !ok(ac, uv, e, false)) return false;
-        if (out1 && !ok(o1, uv, e, tout)) return false;
-        if (out2 && !ok(o2, uv, e, false)) return false;
-        return true;
-    }
-
-    // one stage kernel (momentum, and continuity for the RK modes) in mode m
-    static int stage(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
-                     const Times &T) {
-        const swm_config &c = ctx->c;
-        const bool per = c.bc == SWM_PERIODIC;
-        if (!slots_ok(ctx, m, in, pr, ac, o1, o2, false)) return ctx->fail(SWM_E_STATE, "stage mode %d: buffer slot missing", m);
-        StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
-        if (ctx->rec) {
-            ctx->rec->add(0, m, -1, &A, sizeof A);
-            return SWM_OK;
-        }
-        const int adv = adv_form(c);
-        if (per && adv == ADV_AH) {
-            const dim3 qg((c.ny + 1 + 255) / 256), qb(256);
-            if (m == M_RK_FIRST || m == M_RKC_FIRST) hipLaunchKernelGGL((qend_kernel<F, Q>), qg, qb, 0, ctx->stream, A);
-            else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
-        }
-        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-#define SWM_KA(M_, ADV_)                                                                                   \
-    if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_, ADV_>), grid, block, 0, ctx->stream, A);        \
-    else hipLaunchKernelGGL((stage_kernel<F, false, M_, ADV_>), grid, block, 0, ctx->stream, A);
-#define SWM_K(M_)                                                                                          \
-    case M_:                                                                                               \
-        if (adv == ADV_SAD) { SWM_KA(M_, ADV_SAD) }                                                        \
-        else if (adv == ADV_LIN) { SWM_KA(M_, ADV_LIN) }                                                   \
-        else if (adv == (ADV_AH | ADV_KEEP)) { SWM_KA(M_, ADV_AH | ADV_KEEP) }                             \
-        else if (adv == (ADV_SAD | ADV_KEEP)) { SWM_KA(M_, ADV_SAD | ADV_KEEP) }                           \
-        else { SWM_KA(M_, ADV_AH) }                                                                        \
-        break;
-        if constexpr (F::mixed) {
-            switch (m) {
-                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_RKC_FIRST) SWM_K(M_RKC_LAST)
-            default: return ctx->fail(SWM_E_UNSUPPORTED, "stage mode %d with mixed formats", m);
-            }
-        } else {
-            switch (m) {
-                SWM_K(M_RK_FIRST) SWM_K(M_RK_MID) SWM_K(M_RK_LAST) SWM_K(M_RKC_FIRST) SWM_K(M_RKC_LAST) SWM_K(M_SSP2)
-                SWM_K(M_SSP2_LAST) SWM_K(M_SSP3)
-                SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
-            default: return ctx->fail(SWM_E_ARG, "stage mode %d", m);
-            }
-        }
-#undef SWM_K
-#undef SWM_KA
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    // q and KEu+KEv kept for nstep_advcor > 0, formed from the state in slots s (Tprog
-    // storage): wq also q; init with the initial state's thickness
-    static int advcor(swm_ctx *ctx, Slots s, bool wq, bool init) {
-        const swm_config &c = ctx->c;
-        const Times T{mkR<R>(1.0), mkR<R>(1.0), mkR<R>(0.0)};
-        const StageArgs<F> A = args(ctx, s, Slots{}, Slots{}, Slots{}, Slots{}, Coef<Q>{}, T);
-        if (ctx->rec) {
-            ctx->rec->add(6, (init ? 2 : 0) | (wq ? 1 : 0), -1, &A, sizeof A);
-            return SWM_OK;
-        }
-        const dim3 g((c.nx + 2 + 255) / 256, c.ny + 2), b(256);
-        if (init) hipLaunchKernelGGL((advcor_kernel<F, Q, true, true>), g, b, 0, ctx->stream, A);
-        else if (wq) hipLaunchKernelGGL((advcor_kernel<F, Q, true, false>), g, b, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((advcor_kernel<F, Q, false, false>), g, b, 0, ctx->stream, A);
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    // two consecutive classic RK stages in one kernel (nonperiodic): mode ma then mb
-    static bool fusable(const swm_ctx *ctx, int ma, int mb) { return ctx->fuse && can_fuse(ctx, ma, mb); }
-    static bool can_fuse(const swm_ctx *ctx, int ma, int mb) {
-        const swm_config &c = ctx->c;
-        if (c.time_scheme != SWM_RK) return false;
-        // periodic: a seam the lanes can wrap around (one launch), or the seam strips on their
-        // own kernel; an ensemble's batched pairs take the first kind only
-        if (adv_form(c) == ADV_AH) return true;
-        // Sadourny, the linear dynamics, kept q and KE: the pairs of RK4 with T = Tprog, no
-        // continuity static, on a seam the lanes can wrap around
-        return !ctx->mixed && c.continuity == SWM_CONT_ITSELF && (c.bc == SWM_NONPERIODIC || ctx->seam_ok) &&
-               ((ma == M_RK_FIRST && mb == M_RK_MID) || (ma == M_RK_MID && mb == M_RK_LAST));
-    }
-    static int fused(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &ka,
-                     const Coef<Q> &kb, const Times &T) {
-        // the first stage's outputs stay in registers: check what the pair reads and writes
-        auto ok = [](Slots s, bool tpool) {
-            const int n = tpool ? 2 : 4;
-            return s.t == tpool && s.uv >= 0 && s.uv < n && s.e >= 0 && s.e < n;
-        };
-        const bool afirst = ma == M_RK_FIRST || ma == M_RKC_FIRST, blast = mb == M_RK_LAST || mb == M_RKC_LAST;
-        const bool good = ok(in, ctx->mixed && !afirst) && ok(pr, false) && (afirst || ok(ac, false)) &&
-                          (blast || ok(o1, ctx->mixed)) && ok(o2, false) && (mb != M_RKC_LAST || ctx->cu.base);
-        if (!good) return ctx->fail(SWM_E_STATE, "fused stages %d+%d: buffer slot missing", ma, mb);
-        const bool per = ctx->c.bc == SWM_PERIODIC;
-        if (!can_fuse(ctx, ma, mb)) return ctx->fail(SWM_E_UNSUPPORTED, "stages %d+%d do not fuse in this configuration", ma, mb);
-        const int adv = adv_form(ctx->c);
-        StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
-        A.k2 = kb;
-        A.nstrips = nstrips2(ctx);
-        A.L = ctx->Lf;
-        A.nseg = nseg(ctx, ctx->Lf);
-        if (per && !ctx->seam_ok) return fused_seam(ctx, ma, mb, A);
-        if (ctx->rec) {
-            ctx->rec->add(1, ma, mb, &A, sizeof A);
-            return SWM_OK;
-        }
-        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-        const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
-        if (adv != ADV_AH) {
-            if constexpr (!F::mixed) {
-#define SWM_FV(MA_, MB_, ADV_)                                                                             \
-    if (per) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true, ADV_>), grid, block, 0, ctx->stream, A); \
-    else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, false, ADV_>), grid, block, 0, ctx->stream, A)
-#define SWM_FVA(ADV_)                                                                                      \
-    if (ma == M_RK_FIRST) SWM_FV(M_RK_FIRST, M_RK_MID, ADV_);                                              \
-    else SWM_FV(M_RK_MID, M_RK_LAST, ADV_)
-                if (adv == ADV_SAD) { SWM_FVA(ADV_SAD); }
-                else if (adv == ADV_LIN) { SWM_FVA(ADV_LIN); }
-                else if (adv == (ADV_AH | ADV_KEEP)) { SWM_FVA(ADV_AH | ADV_KEEP); }
-                else if (adv == (ADV_SAD | ADV_KEEP)) { SWM_FVA(ADV_SAD | ADV_KEEP); }
-                else return ctx->fail(SWM_E_ARG, "right-hand side form %d", adv);
-#undef SWM_FVA
-#undef SWM_FV
-                HIPCHK(hipGetLastError());
-                return SWM_OK;
-            }
-            return ctx->fail(SWM_E_UNSUPPORTED, "fused stages of this right-hand side with mixed formats");
-        }
-#define SWM_FK(MA_, MB_)                                                                                   \
-    if (per) {                                                                                             \
-        if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true>), grid, block, 0, ctx->stream, A); \
-        else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true>), grid, block, 0, ctx->stream, A);   \
-    } else if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true>), grid, block, 0, ctx->stream, A);  \
-    else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false>), grid, block, 0, ctx->stream, A)
-        if (ma == M_RK_FIRST && mb == M_RK_MID) SWM_FK(M_RK_FIRST, M_RK_MID);
-        else if (ma == M_RK_MID && mb == M_RK_LAST) SWM_FK(M_RK_MID, M_RK_LAST);
-        else if (ma == M_RK_FIRST && mb == M_RK_LAST) SWM_FK(M_RK_FIRST, M_RK_LAST);
-        else if (ma == M_RKC_FIRST && mb == M_RK_MID) SWM_FK(M_RKC_FIRST, M_RK_MID);
-        else if (ma == M_RK_MID && mb == M_RKC_LAST) SWM_FK(M_RK_MID, M_RKC_LAST);
-        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) SWM_FK(M_RKC_FIRST, M_RKC_LAST);
-        else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
-#undef SWM_FK
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    // A periodic fused pair whose seam the lanes cannot wrap around: the strips clear of
-    // the seam on the wrapping kernel (no lane of theirs wraps), and beside them, on the
-    // context's second stream, the strips at the seam on fused_seam_kernel.  The two write
-    // disjoint columns and read only what the launches before the pair wrote.
-    static int fused_seam(swm_ctx *ctx, int ma, int mb, StageArgs<F> A) {
-        const swm_config &c = ctx->c;
-        const int ns = A.nstrips, wo = WAVE - 2 * HALO2;
-        // a strip is clear of the seam if all its lanes lie in the columns 1..nx-1
-        auto clear = [&](int s) { return wo * s - HALO2 >= 1 && wo * s + WAVE - HALO2 - 1 <= c.nx - 1; };
-        int a = 0;
-        while (a < ns && !clear(a)) ++a;
-        int b = a;
-        while (b < ns && clear(b)) ++b;
-        const dim3 block(WAVE * WAVES_PER_BLOCK);
-        const bool cs = c.continuity != SWM_CONT_ITSELF;
-        StageArgs<F> I = A, E = A;
-        I.nstrips = b - a; I.sfirst = a;
-        E.nstrips = a + (ns - b); E.sfirst = 0; E.sjump_at = a; E.sjump = b - a;
-        // Segments of 4 rows for the seam strips: their waves run alone on their SIMDs, so the
-        // launch's time is the march of one segment (4 rows: 2048x1024 over the ridge 0.113 ms
-        // per step against 0.126 with the pairs' own 8 rows, 0.215 with 32; 4096² 0.565
-        // against 0.577).
-        E.L = 4;
-        E.nseg = nseg(ctx, E.L);
-        if (ctx->rec) {                                      // an ensemble's plan: the two launches as items
-            ctx->rec->add(4, ma, mb, &I, sizeof I);
-            ctx->rec->add(5, ma, mb, &E, sizeof E);
-            return SWM_OK;
-        }
-        hipStream_t s2 = I.nstrips > 0 ? ctx->stream2 : ctx->stream;
-        if (s2 != ctx->stream) {                             // fork
-            HIPCHK(hipEventRecord(ctx->fork, ctx->stream));
-            HIPCHK(hipStreamWaitEvent(s2, ctx->fork, 0));
-        }
-#define SWM_FS(MA_, MB_)                                                                                       \
-    {                                                                                                          \
-        const dim3 g(nblocks_waves(E.nstrips * E.nseg));                                                       \
-        if (cs) hipLaunchKernelGGL((fused_seam_kernel<F, MA_, MB_, true>), g, block, 0, s2, E);                \
-        else hipLaunchKernelGGL((fused_seam_kernel<F, MA_, MB_, false>), g, block, 0, s2, E);                  \
-    }                                                                                                          \
-    if (I.nstrips > 0) {                                                                                       \
-        const dim3 g(nblocks_waves(I.nstrips * I.nseg));                                                       \
-        if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true>), g, block, 0, ctx->stream, I);      \
-        else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true>), g, block, 0, ctx->stream, I);        \
-    }
-        if (ma == M_RK_FIRST && mb == M_RK_MID) { SWM_FS(M_RK_FIRST, M_RK_MID) }
-        else if (ma == M_RK_MID && mb == M_RK_LAST) { SWM_FS(M_RK_MID, M_RK_LAST) }
-        else if (ma == M_RK_FIRST && mb == M_RK_LAST) { SWM_FS(M_RK_FIRST, M_RK_LAST) }
-        else if (ma == M_RKC_FIRST && mb == M_RK_MID) { SWM_FS(M_RKC_FIRST, M_RK_MID) }
-        else if (ma == M_RK_MID && mb == M_RKC_LAST) { SWM_FS(M_RK_MID, M_RKC_LAST) }
-        else if (ma == M_RKC_FIRST && mb == M_RKC_LAST) { SWM_FS(M_RKC_FIRST, M_RKC_LAST) }
-        else return ctx->fail(SWM_E_ARG, "fused stages %d+%d", ma, mb);
-#undef SWM_FS
-        HIPCHK(hipGetLastError());
-        if (s2 != ctx->stream) {                             // join: the step goes on after both
-            HIPCHK(hipEventRecord(ctx->join, s2));
-            HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
-        }
-        return SWM_OK;
-    }
-
-    // semi-implicit continuity of the SSPRK schemes in mode m
-    static int continuity(swm_ctx *ctx, int m, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, const Coef<Q> &k,
-                          const Times &T) {
-        if constexpr (F::mixed) {
-            return ctx->fail(SWM_E_UNSUPPORTED, "SSPRK schemes with mixed formats");
-        } else {
-            const bool per = ctx->c.bc == SWM_PERIODIC;
-            if (m <= M_RK_LAST || !slots_ok(ctx, m, in, pr, ac, o1, o2, true))
-                return ctx->fail(SWM_E_STATE, "continuity mode %d: buffer slot missing", m);
-            StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, k, T);
-            if (ctx->rec) {
-                ctx->rec->add(3, m, -1, &A, sizeof A);
-                return SWM_OK;
-            }
-            const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-#define SWM_K(M_)                                                                                          \
-    case M_:                                                                                               \
-        if (per) hipLaunchKernelGGL((continuity_kernel<F, true, M_>), grid, block, 0, ctx->stream, A);     \
-        else hipLaunchKernelGGL((continuity_kernel<F, false, M_>), grid, block, 0, ctx->stream, A);        \
-        break;
-            switch (m) {
-                SWM_K(M_SSP2) SWM_K(M_SSP2_LAST) SWM_K(M_SSP3) SWM_K(M_SSP3_KN) SWM_K(M_S4) SWM_K(M_S4_3)
-            default: return ctx->fail(SWM_E_ARG, "continuity mode %d", m);
-            }
-#undef SWM_K
-            HIPCHK(hipGetLastError());
-            return SWM_OK;
-        }
-    }
-
-    // The first n_stages stages of the configured scheme (time_integration.jl:53-196) from
-    // Prog.  On return *u0 and *u1 name the slots of the reference's u0 and u1.
-    static int total_stages(const swm_config &c) {
-        return c.time_scheme == SWM_RK ? c.RKo : c.time_scheme == SWM_SSPRK2 ? c.RKs : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
-    }
-
-    // keep_u1: leave the last stage's u1 in a slot (a fused pair ending with the scheme's last
-    // stage keeps it in registers only)
-    static int stages(swm_ctx *ctx, int n_stages, int64_t t, Slots *u0, Slots *u1, bool keep_u1 = false) {
-        const swm_config &c = ctx->c;
-        if (n_stages > total_stages(c)) n_stages = total_stages(c);
-        if (ctx->advcor_pending) {                           // advection_coriolis! on the initial state (:20-31)
-            int rc = advcor(ctx, Slots{ctx->pu, ctx->pe}, true, true);
-            if (rc) return rc;
-            ctx->advcor_pending = false;
-        }
-        const Times T = forcing_times(ctx, t);
-        const Slots P{ctx->pu, ctx->pe}, none{};
-        Slots cur = P, z0 = P;
-        Coef<Q> k{};
-        int rc = SWM_OK;
-        if (c.time_scheme == SWM_RK) {                                     // :55-78
-            const Slots a{pick({P.uv}), pick({P.e})};
-            const bool cp = c.compensated != 0;                            // increments summed apart, Kahan add
-            auto mode = [&](int s) {
-                return s == 1 ? (cp ? M_RKC_FIRST : M_RK_FIRST) : (s == c.RKo ? (cp ? M_RKC_LAST : M_RK_LAST) : M_RK_MID);
-            };
-            auto coef = [&](int s) {
-                Coef<Q> kk{};
-                kk.a = (Q)c.RKaDt[s - 1];
-                kk.b = s < c.RKo ? (Q)c.RKbDt[s - 1] : Q(0);
-                return kk;
-            };
-            for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
-                // the next u1: a free Tprog slot, or the other T slot with mixed formats
-                const Slots n = ctx->mixed ? Slots{cur.t ? 1 - cur.uv : 0, cur.t ? 1 - cur.e : 0, true}
-                                           : Slots{pick({P.uv, a.uv, cur.uv}), pick({P.e, a.e, cur.e})};
-                if (s + 1 <= n_stages && fusable(ctx, mode(s), mode(s + 1)) && !(keep_u1 && s + 1 == c.RKo)) {   // stages s, s+1 in one kernel
-                    const int s2 = s + 1;
-                    // the first stage reads u0 only after stage 1 wrote it
-                    rc = fused(ctx, mode(s), mode(s2), cur, P, s == 1 ? none : a, s2 < c.RKo ? n : none, a, coef(s),
-                               coef(s2), T);
-                    if (s2 < c.RKo) cur = n;
-                    ++s;
-                    continue;
-                }
-                rc = stage(ctx, mode(s), cur, P, a, s < c.RKo ? n : none, a, coef(s), T);
-                if (s < c.RKo) cur = n;
-            }
-            z0 = a;
-        } else if (c.time_scheme == SWM_SSPRK2) {                          // :82-110
-            k.c = (Q)c.Dt_Ds;
-            k.x = (Q)(1.0 / c.RKs);
-            k.y = (Q)((double)(c.RKs - 1) / c.RKs);
-            for (int s = 1; s <= n_stages && rc == SWM_OK; ++s) {
-                const bool last = s == c.RKs;
-                const Slots n{pick({P.uv, cur.uv}), pick({P.e, cur.e})};
-                const Slots z = last ? Slots{pick({P.uv, cur.uv, n.uv}), pick({P.e, cur.e, n.e})} : none;
-                const int m = last ? M_SSP2_LAST : M_SSP2;
-                rc = stage(ctx, m, cur, P, none, n, z, k, T);
-                if (!rc) rc = continuity(ctx, m, Slots{n.uv, cur.e}, P, none, n, z, k, T);
-                cur = n;
-                if (last) z0 = z;
-            }
-        } else if (c.time_scheme == SWM_SSPRK3) {                          // :114-156
-            k.c = (Q)c.ssp_Dt_Dn;
-            k.x = (Q)c.ssp_kna;
-            k.y = (Q)c.ssp_knb;
-            k.z = (Q)c.ssp_Dt_Dnc;
-            for (int s = 2; s <= n_stages + 1 && s <= c.ssp_s + 1 && rc == SWM_OK; ++s) {
-                const Slots n{pick({P.uv, cur.uv, z0.uv}), pick({P.e, cur.e, z0.e})};
-                const int m = s == c.ssp_kn ? M_SSP3_KN : M_SSP3;
-                rc = stage(ctx, m, cur, P, z0, n, none, k, T);
-                if (!rc) rc = continuity(ctx, m, Slots{n.uv, cur.e}, P, z0, n, none, k, T);
-                cur = n;
-                if (s == c.ssp_mn || s == c.ssp_s + 1) z0 = n;            // copyto!(u0,u1) :150-155
-            }
-        } else {                                                            // 4SSPRK3 :160-195
-            k.c = (Q)c.Dt_D;
-            k.h = Q(0.5);
-            k.x = (Q)(2.0 / 3.0);
-            k.y = (Q)(1.0 / 3.0);
-            for (int s = 1; s <= n_stages && s <= 4 && rc == SWM_OK; ++s) {
-                const int n1 = pick({P.uv, cur.uv});
-                const int n2 = s == 3 ? pick({P.uv, cur.uv, n1}) : -1;
-                const int ne = pick({P.e, cur.e});
-                const int m = s == 3 ? M_S4_3 : M_S4;
-                rc = stage(ctx, m, cur, P, none, Slots{n1, -1}, Slots{n2, -1}, k, T);
-                if (!rc) rc = continuity(ctx, m, Slots{n1, cur.e}, P, none, Slots{-1, ne}, none, k, T);
-                cur = Slots{s == 3 ? n2 : n1, ne};
-                if (s == 4) z0 = cur;                                       // copyto!(u0,u1) :191-193
-            }
-        }
-        *u0 = z0;
-        *u1 = cur;
-        return rc;
-    }
-
-    static int dissipation(swm_ctx *ctx, Slots in, int out_uv) {
-        const swm_config &c = ctx->c;
-        DissArgs<F> A;
-        std::memset((void *)&A, 0, sizeof A);   // padding too (compared byte for byte by ensembles)
-        A.ar = ctx->aref();
-        A.u0 = ctx->u[in.uv].ref(); A.v0 = ctx->v[in.uv].ref(); A.e0 = ctx->e[in.e].ref();
-        A.out_u = ctx->u[out_uv].ref(); A.out_v = ctx->v[out_uv].ref();
-        A.H = ctx->H.ref();
-        A.cD = mkR<R>(c.cD); A.rD = mkR<R>(c.rD); A.cS = mkR<R>(c.cSmag); A.nuB = mkR<R>(c.nuB);
-        A.sinv = mkR<R>(ctx->sci);
-        A.dtd = (Q)c.Dt_diff; A.oma = (Q)c.one_minus_alpha;
-        A.cm = c.Tcomm;
-        A.nx = c.nx; A.ny = c.ny; A.P = ctx->P; A.L = ctx->Ld;
-        A.nstrips = nstrips(ctx); A.nseg = nseg(ctx, ctx->Ld);
-        if (ctx->rec) {
-            ctx->rec->add(2, 0, 0, &A, sizeof A);
-            return SWM_OK;
-        }
-        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
-        const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
-#define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<F, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
-#define SWM_DISS_D(PER_, S_)                                   \
-    switch (c.bottom_drag) {                                   \
-    case SWM_DRAG_QUADRATIC: SWM_DISS(PER_, 0, S_); break;     \
-    case SWM_DRAG_LINEAR: SWM_DISS(PER_, 1, S_); break;        \
-    default: SWM_DISS(PER_, 2, S_); break;                     \
-    }
-        if (per) {
-            if (smag) { SWM_DISS_D(true, true) } else { SWM_DISS_D(true, false) }
-        } else {
-            if (smag) { SWM_DISS_D(false, true) } else { SWM_DISS_D(false, false) }
-        }
-#undef SWM_DISS_D
-#undef SWM_DISS
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    static int tracer_advect(swm_ctx *ctx, int uv_slot, int src, int dst) {
-        const swm_config &c = ctx->c;
-        const dim3 grid((c.nx + 127) / 128, c.ny), block(128);
-        const bool per = c.bc == SWM_PERIODIC;
-        auto k = per ? tracer_kernel<F, true> : tracer_kernel<F, false>;
-        hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<Q>(ctx->u[uv_slot]), ctx->at<Q>(ctx->v[uv_slot]),
-                           ctx->at<ST>(ctx->um), ctx->at<ST>(ctx->vm), ctx->at<Q>(ctx->sst[src]), ctx->at<Q>(ctx->sst[dst]),
-                           mkR<R>(c.half_dtadvu), mkR<R>(c.half_dtadvv), mkR<R>(c.dtadvu), mkR<R>(c.dtadvv), c.nx, c.ny,
-                           ctx->P, ctx->ep, c.tracer_consumption, mkR<R>(c.jSST), mkR<R>(c.SSTmin), c.Tcomm, mkR<R>(ctx->sci),
-                           ctx->dflags + 1);
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    // tracer!  tracer_advection.jl:1-44 with the post-dissipation velocities (:226-228)
-    static int tracer(swm_ctx *ctx, int64_t i, int uv_slot) {
-        const swm_config &c = ctx->c;
-        if (!c.tracer_advection) return SWM_OK;
-        const size_t n = ctx->elems;
-        if ((i + c.nadvstep_half) % c.nadvstep == 0) {      // mid-point snapshot um, vm ← T(u0), T(v0)  :12-17
-            snapshot_kernel<F><<<1024, 256, 0, ctx->stream>>>((ST *)ctx->um.base, (const Q *)ctx->u[uv_slot].base, n);
-            snapshot_kernel<F><<<1024, 256, 0, ctx->stream>>>((ST *)ctx->vm.base, (const Q *)ctx->v[uv_slot].base, n);
-            HIPCHK(hipGetLastError());
-        }
-        if (i % c.nadvstep == 0) {                          // :19-42
-            const int src = ctx->rsst, dst = 1 - ctx->rsst;
-            int rc = tracer_advect(ctx, uv_slot, src, dst);
-            if (rc) return rc;
-            ctx->rsst = dst;
-        }
-        return SWM_OK;
-    }
-
-    static int ghosts(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot) {
-        const swm_config &c = ctx->c;
-        const bool per = c.bc == SWM_PERIODIC;
-        auto k = per ? ghost_kernel<Q, true> : ghost_kernel<Q, false>;
-        Q *u = ctx->at<Q>(ctx->u[uv_slot]), *v = ctx->at<Q>(ctx->v[uv_slot]), *e = ctx->at<Q>(ctx->e[e_slot]);
-        Q *s = ctx->at<Q>(ctx->sst[sst_slot]);
-        hipLaunchKernelGGL(k, dim3((c.ny + 4 + 127) / 128), dim3(128), 0, ctx->stream, u, v, e, s, c.nx, c.ny, ctx->P,
-                           (Q)c.one_minus_alpha, c.Tcomm, 0);
-        hipLaunchKernelGGL(k, dim3((c.nx + 5 + 127) / 128), dim3(128), 0, ctx->stream, u, v, e, s, c.nx, c.ny, ctx->P,
-                           (Q)c.one_minus_alpha, c.Tcomm, 1);
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    // NaN check of Prog (u, v of slot uv, η of slot e) and sst, feedback.jl:61-72
-    static int nan_check(swm_ctx *ctx, int uv_slot, int e_slot, int sst_slot, bool *found) {
-        const swm_config &c = ctx->c;
-        NanArgs A;
-        A.f[0] = ctx->at<Q>(ctx->u[uv_slot]); A.x0[0] = -1 - ctx->ep; A.x1[0] = ctx->nux + 2 - ctx->ep; A.y0[0] = -1; A.y1[0] = c.ny + 2;
-        A.f[1] = ctx->at<Q>(ctx->v[uv_slot]); A.x0[1] = -1; A.x1[1] = c.nx + 2; A.y0[1] = -1; A.y1[1] = c.ny + 1;
-        A.f[2] = ctx->at<Q>(ctx->e[e_slot]); A.x0[2] = 0; A.x1[2] = c.nx + 1; A.y0[2] = 0; A.y1[2] = c.ny + 1;
-        A.f[3] = ctx->at<Q>(ctx->sst[sst_slot]); A.x0[3] = 0; A.x1[3] = c.nx + 1; A.y0[3] = 1; A.y1[3] = c.ny;
-        A.P = ctx->P;
-        A.nf = 4;
-        HIPCHK(hipMemsetAsync(ctx->dflags, 0, sizeof(int), ctx->stream));
-        nan_kernel<Q><<<dim3(1, c.ny + 4, 4), 256, 0, ctx->stream>>>(A, ctx->dflags);
-        HIPCHK(hipGetLastError());
-        int flag = 0;
-        HIPCHK(hipMemcpyAsync(&flag, ctx->dflags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        *found = flag != 0;
-        return SWM_OK;
-    }
-
-    // one pass of the loop body, time_integration.jl:45-243
-    static int step(swm_ctx *ctx, int64_t i, int64_t t, bool *nan) {
-        const swm_config &c = ctx->c;
-        *nan = false;
-        Slots u0, u1;
-        int rc = stages(ctx, 1 << 20, t, &u0, &u1);
-        if (rc) return rc;
-        if (adv_form(c) & ADV_KEEP) {                        // :208-211, and bottom_drag_quadratic!'s KEu, KEv
-            const bool due = i % c.nstep_advcor == 0;
-            if (due || (c.bottom_drag == SWM_DRAG_QUADRATIC && i % c.nstep_diff == 0))
-                if ((rc = advcor(ctx, u0, due, false))) return rc;
-        }
-        int new_uv = u0.uv;
-        if (i % c.nstep_diff == 0) {                         // :216-221
-            const int out = pick({ctx->pu, u0.uv});
-            rc = dissipation(ctx, u0, out);
-            if (rc) return rc;
-            new_uv = out;
-        }
-        rc = tracer(ctx, i, new_uv);                         // :226-228
-        if (rc) return rc;
-        if (c.nout > 0 && i % c.nout == 0) {                 // feedback.jl:136-148
-            rc = nan_check(ctx, ctx->pu, ctx->pe, ctx->rsst, nan);
-            if (rc) return rc;
-            if (*nan) return SWM_OK;                         // keep Prog: no copy-back
-        }
-        ctx->pu = new_uv;                                    // copy-back (:240-242)
-        ctx->pe = u0.e;
-        return SWM_OK;
-    }
-
-    static int time_kernel(swm_ctx *ctx, int which, int reps, double *ms) {
-        const swm_config &c = ctx->c;
-        const Slots P{ctx->pu, ctx->pe}, none{};
-        const Slots a{pick({P.uv}), pick({P.e})};
-        const bool tp = ctx->mixed;
-        const Slots s1 = tp ? Slots{0, 0, true} : Slots{pick({P.uv, a.uv}), pick({P.e, a.e})};
-        const Slots s2 = tp ? Slots{1, 1, true} : Slots{pick({P.uv, a.uv, s1.uv}), pick({P.e, a.e, s1.e})};
-        const int dq = pick({P.uv, a.uv});                   // dissipation output (Tprog)
-        const Times T{mkR<R>(1.0), mkR<R>(1.0), mkR<R>(0.0)};
-        Coef<Q> k{};
-        k.a = (Q)c.RKaDt[0];
-        k.b = (Q)c.RKbDt[0];
-        k.c = (Q)c.Dt_D;
-        auto launch = [&]() -> int {
-            switch (which) {
-            case 0: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_MID : M_SSP2, s1, P, a, s2, a, k, T) ||
-                           (c.time_scheme != SWM_RK && continuity(ctx, M_SSP2, Slots{s2.uv, s1.e}, P, none, s2, none, k, T));
-            case 1: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_FIRST : M_SSP2, P, P, a, s1, a, k, T);
-            case 2: return stage(ctx, c.time_scheme == SWM_RK ? M_RK_LAST : M_SSP2, s1, P, a, none, a, k, T);
-            case 3: return dissipation(ctx, a, dq);
-            case 4: return tracer_advect(ctx, P.uv, ctx->rsst, 1 - ctx->rsst);
-            case 5: return fused(ctx, M_RK_FIRST, M_RK_MID, P, P, none, s1, a, k, k, T);
-            case 6: return fused(ctx, M_RK_MID, M_RK_LAST, s1, P, a, none, a, k, k, T);
-            }
-            return ctx->fail(SWM_E_ARG, "unknown kernel %d", which);
-        };
-        int rc = launch();                                   // warm-up
-        if (rc) return rc;
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < reps && rc == SWM_OK; ++r) rc = launch();
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float tm = 0.0f;
-        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
-        HIPCHK(hipEventDestroy(e0));
-        HIPCHK(hipEventDestroy(e1));
-        *ms = (double)tm / reps;
-        return rc;
-    }
-};
-
-// call fn with the format bundle of the context
-template <class Fn> int with_fmt(const swm_ctx *ctx, Fn fn) {
-    switch (ctx->fmt) {
-    case SWM_F64: return fn(FmtF64{});
-    case SWM_F32: return fn(FmtF32{});
-    case SWM_F16: return fn(FmtF16{});
-    default: return fn(FmtBF16{});
-    }
-}
 
 int check_config(swm_ctx *ctx) {
     const swm_config &c = ctx->c;
@@ -957,997 +52,13 @@ int check_config(swm_ctx *ctx) {
     return SWM_OK;
 }
 
-template <class F> int run_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *done, int64_t *nan_step) {
-    *done = 0;
-    *nan_step = 0;
-    for (int64_t k = 0; k < nsteps; ++k) {
-        bool nan = false;
-        int rc = Launch<F>::step(ctx, i0 + k, t0 + k * ctx->c.dtint, &nan);
-        if (rc) return rc;
-        if (nan) {
-            *nan_step = i0 + k;
-            break;
-        }
-        *done = k + 1;
-    }
-    int flags[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(flags, ctx->dflags, sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (flags[1]) {
-        int z = 0;
-        HIPCHK(hipMemcpy(ctx->dflags + 1, &z, sizeof z, hipMemcpyHostToDevice));
-        return ctx->fail(SWM_E_ARG, "tracer departure point is NaN or outside the grid (InexactError/BoundsError in the reference)");
-    }
-    return SWM_OK;
-}
-
-}  // namespace
-
-// ====================================================================== adjoint
-//
-// Reverse mode of time_integration (classic RK, T = Tprog = Tcomm in {Float64, Float32}).
-// swm_adj_forward runs the steps as swm_step does and keeps device snapshots of full states
-// (u, v, η, sst and the tracer's um, vm) at the positions a binomial checkpointing schedule
-// chooses for the given number of snapshots (Griewank's revolve: with s snapshots, n steps
-// are reversed with the least repetition number r, n <= C(s+r, s)).  swm_adj_pullback
-// reverses the steps last first: it restores the nearest snapshot, advances to the step,
-// runs that step once more keeping its intermediate states (the taped step: every RK stage
-// input, u0 before and after the dissipation, the tracer state it read), and applies the
-// reverse step of adj_core.h.  The result is the gradient of <ȳ, y> with respect to the
-// state given to swm_adj_forward (interior cells; the halo of the gradient is zero).
-
-struct AdjState {
-    swm::adj::AdjFields F{};       // adjoint fields (origins); roles rotate during a pullback
-    swm::adj::AdjFields F0{};
-    int La = 64;                   // rows per wave segment of the adjoint marches
-    std::vector<char *> snap;      // snapshots: [0] the initial state, [1..s] the schedule's, [s+1] the final state
-    std::vector<int64_t> at;       // step held by each schedule snapshot (-1: free)
-    size_t snap_bytes = 0;
-    int *dmax = nullptr;           // the tracer adjoint's gather window (device, 2 ints)
-    int64_t i0 = 0, t0 = 0, n = 0;
-    int nsnap = 0;
-    bool recorded = false;
-    int64_t fwd_steps = 0;         // forward steps run by the last pullback (taped steps included)
-    double sweep_ms = 0.0;         // device time of the last reverse sweep (HIP events)
-    void release() {
-        for (char *p : snap) if (p) (void)hipFree(p);
-        snap.clear();
-        at.clear();
-        if (dmax) (void)hipFree(dmax);
-        dmax = nullptr;
-    }
-};
-
-namespace {
-
-// the fields of a context, for re-basing after the arena moves
-std::vector<Field *> all_fields(swm_ctx *ctx) {
-    std::vector<Field *> f;
-    for (int s = 0; s < 6; ++s) { f.push_back(&ctx->u[s]); f.push_back(&ctx->v[s]); f.push_back(&ctx->e[s]); }
-    for (int s = 0; s < 2; ++s) {
-        f.push_back(&ctx->sst[s]); f.push_back(&ctx->tu[s]); f.push_back(&ctx->tv[s]); f.push_back(&ctx->te[s]);
-    }
-    for (Field *g : {&ctx->um, &ctx->vm, &ctx->H, &ctx->Fx, &ctx->Fy, &ctx->fq, &ctx->eta_ref, &ctx->Feta, &ctx->qend,
-                     &ctx->cu, &ctx->cv, &ctx->ce, &ctx->f_u, &ctx->f_v, &ctx->qs, &ctx->ks})
-        f.push_back(g);
-    return f;
-}
-
-// Enlarge the arena by `extra` bytes, keeping every field at its byte offset.
-int grow_arena(swm_ctx *ctx, size_t extra) {
-    const size_t nb = ctx->arena_bytes + extra;
-    if (nb > 0xFFFF0000ull)
-        return ctx->fail(SWM_E_UNSUPPORTED, "grid %dx%d with the adjoint needs %zu bytes of device fields; one context "
-                         "addresses at most 4 GiB", ctx->c.nx, ctx->c.ny, nb);
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    void *p = nullptr;
-    if (hipMalloc(&p, nb) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc of %zu bytes failed", nb);
-    HIPCHK(hipMemsetAsync(p, 0, nb, ctx->stream));
-    HIPCHK(hipMemcpyAsync(p, ctx->arena, ctx->arena_used, hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    char *old = (char *)ctx->arena;
-    for (Field *f : all_fields(ctx))
-        if (f->base) {
-            f->p0 = (char *)p + (f->p0 - old);
-            f->base = (char *)p + f->origin;
-        }
-    for (auto &a : ctx->allocs)
-        if (a == ctx->arena) a = p;
-    (void)hipFree(ctx->arena);
-    ctx->arena = p;
-    ctx->arena_bytes = nb;
-    return SWM_OK;
-}
-
-int64_t binom_beta(int s, int t) {               // C(s+t, s), saturating
-    double b = 1.0;
-    for (int k = 1; k <= s; ++k) b = b * (double)(t + k) / (double)k;
-    return b > 4e18 ? (int64_t)4e18 : (int64_t)(b + 0.5);
-}
-// where the schedule stores the next snapshot when reversing steps [a, b) with s free
-// snapshots: the right part is reversible with s-1 snapshots and the left part with s at
-// one repetition less (β(s,r) = β(s-1,r) + β(s,r-1))
-int64_t revolve_split(int64_t a, int64_t b, int s) {
-    const int64_t n = b - a;
-    int r = 0;
-    while (binom_beta(s, r) < n) ++r;
-    const int64_t left = std::max<int64_t>(1, n - binom_beta(s - 1, r));
-    return a + left;
-}
-
-template <class Fm> struct Adj {
-    using R = typename Fm::R;
-    using Q = typename Fm::Q;
-    using LF = Launch<Fm>;
-
-    static int check(swm_ctx *ctx) {
-        const swm_config &c = ctx->c;
-        if (c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK scheme only");
-        if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
-        if (c.scale_log2 != 0 || c.compensated)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (adv_form(c) != ADV_AH)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear Arakawa-Hsu right-hand side only "
-                                                "(nstep_advcor = 0)");
-        if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
-        if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
-        return SWM_OK;
-    }
-
-    // carve the adjoint's fields: two more slot sets and the adjoint fields (21), zeroed
-    static int init(swm_ctx *ctx) {
-        if (ctx->adj) return SWM_OK;
-        int rc = check(ctx);
-        if (rc) return rc;
-        const int nf2 = 6 + 21;
-        const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
-        if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
-        AdjState *st = new AdjState();
-        const int es = ctx->es;
-        for (int s = 4; s < 6; ++s)
-            if ((rc = alloc_field(ctx, ctx->u[s], 0, true, es)) || (rc = alloc_field(ctx, ctx->v[s], 0, true, es)) ||
-                (rc = alloc_field(ctx, ctx->e[s], 0, true, es))) {
-                delete st;
-                return rc;
-            }
-        auto one = [&](uint32_t &o, bool two_d) {
-            Field f;
-            if (!rc) rc = alloc_field(ctx, f, two_d ? 0 : n1d, two_d, es);
-            o = f.origin;
-        };
-        auto uve = [&](adj::UVE &x) { one(x.u, true); one(x.v, true); one(x.e, true); };
-        adj::AdjFields &F = st->F;
-        uve(F.L); uve(F.Z); uve(F.Ma); uve(F.Mb); uve(F.A);
-        one(F.Lsst, true); one(F.Lsst2, true); one(F.Lum, true); one(F.Lvm, true);
-        one(F.tr0, true); one(F.tr1, true);
-        one(F.lqend, false);
-        if (rc) {
-            delete st;
-            return rc;
-        }
-        F.H = ctx->H.ref();
-        F.fq = ctx->fq.ref();
-        F.fq_2d = ctx->fq_sx;
-        F.qend = ctx->qend.ref();
-        st->F0 = F;
-        // rows per wave: about four resident waves per wave slot
-        st->La = LF::rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_adj_kernel<R, false, true, false, false>),
-                                   adj::nstrips_for<adj::RT_HALO>(ctx->c.nx + 2), 4, 8);
-        if (const char *e = getenv("SWM_ROWS_PER_WAVE_ADJ")) {
-            const int l = atoi(e);
-            if (l >= 4 && l <= 4096) st->La = l;
-        }
-        ctx->adj = st;
-        ctx->nslots = 6;
-        return SWM_OK;
-    }
-
-    static adj::AdjConsts<R> consts(const swm_ctx *ctx) {
-        const swm_config &c = ctx->c;
-        adj::AdjConsts<R> k{};
-        for (int j = 0; j < 4; ++j) {
-            k.a[j] = (R)c.RKaDt[j];
-            k.b[j] = j < 3 ? (R)c.RKbDt[j] : R(0);
-        }
-        k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
-        k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
-        k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
-        k.RKo = c.RKo; k.cont = c.continuity; k.drag = c.bottom_drag; k.smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
-        k.per = c.bc == SWM_PERIODIC; k.consume = c.tracer_consumption; k.tracer = c.tracer_advection;
-        k.nadvstep = (int)c.nadvstep; k.nadvstep_half = (int)c.nadvstep_half; k.nstep_diff = c.nstep_diff;
-        k.nx = c.nx; k.ny = c.ny; k.P = ctx->P; k.ep = ctx->ep;
-        return k;
-    }
-
-    // ---- kernel launcher of adj::reverse_step
-    struct K {
-        swm_ctx *ctx;
-        AdjState *st;
-        int fail_launch() {
-            hipError_t e = hipGetLastError();
-            return e == hipSuccess ? SWM_OK : ctx->fail(SWM_E_HIP, "adjoint kernel launch: %s", hipGetErrorString(e));
-        }
-        template <bool PER, bool NEXT, bool AL0, bool PRE> void launch_stage(const adj::StageAdjArgs<R> &S) {
-            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE>), dim3(nblocks_waves(S.nstrips * S.nseg)),
-                               dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), S);
-        }
-        template <bool PER> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
-            if (!next) launch_stage<PER, false, false, false>(S);
-            else if (al0 && pre) launch_stage<PER, true, true, true>(S);
-            else if (al0) launch_stage<PER, true, true, false>(S);
-            else if (pre) launch_stage<PER, true, false, true>(S);
-            else launch_stage<PER, true, false, false>(S);
-        }
-        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre) {
-            S.L = st->La;
-            S.nstrips = adj::nstrips_for<adj::RT_HALO>(S.nx + 2);
-            S.nseg = (adj::adj_y1(S.ny) - adj::ADJ_Y0 + S.L - 1) / S.L;
-            if (per) stage_p<true>(S, next, al0, pre);
-            else stage_p<false>(S, next, al0, pre);
-            return fail_launch();
-        }
-        int qend(const adj::StageAdjArgs<R> &S) {
-            hipLaunchKernelGGL((qend_fwd_kernel<R>), dim3((S.ny + 1 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S);
-            return fail_launch();
-        }
-        int qend_adj(const adj::StageAdjArgs<R> &S, const adj::UVE &mu) {
-            hipLaunchKernelGGL((qend_adj_kernel<R>), dim3((S.ny + 2 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S,
-                               mu);
-            return fail_launch();
-        }
-        template <bool PER, int DRAG, bool SMAG> void launch_diss(const adj::DissAdjArgs<R> &D) {
-            hipLaunchKernelGGL((diss_adj_kernel<R, PER, DRAG, SMAG>), dim3(nblocks_waves(D.nstrips * D.nseg)),
-                               dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), D);
-        }
-        template <bool PER> void diss_p(const adj::DissAdjArgs<R> &D, int drag, int smag) {
-            if (drag == 0) { if (smag) launch_diss<PER, 0, true>(D); else launch_diss<PER, 0, false>(D); }
-            else if (drag == 1) { if (smag) launch_diss<PER, 1, true>(D); else launch_diss<PER, 1, false>(D); }
-            else { if (smag) launch_diss<PER, 2, true>(D); else launch_diss<PER, 2, false>(D); }
-        }
-        int diss(adj::DissAdjArgs<R> D, int per, int drag, int smag) {
-            D.L = st->La;
-            D.nstrips = adj::nstrips_for<adj::DT_HALO>(D.nx + 2);
-            D.nseg = (adj::adj_y1(D.ny) - adj::ADJ_Y0 + D.L - 1) / D.L;
-            if (per) diss_p<true>(D, drag, smag);
-            else diss_p<false>(D, drag, smag);
-            return fail_launch();
-        }
-        int tracer(const adj::TracerAdjArgs<R> &T) {
-            hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
-            const dim3 g1((T.nx + 127) / 128, T.ny), g2((T.nx + 5 + 127) / 128, T.ny + 4);
-            if (T.ep) {
-                hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-            } else {
-                hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-            }
-            return fail_launch();
-        }
-        int snapshot(uint32_t lu, uint32_t lum, uint32_t lv, uint32_t lvm) {
-            char *b = (char *)ctx->arena;
-            hipLaunchKernelGGL((snapshot_adj_kernel<R>), dim3(1024), dim3(256), 0, ctx->stream, (R *)(b + lu), (R *)(b + lum),
-                               (R *)(b + lv), (R *)(b + lvm), ctx->elems);
-            return fail_launch();
-        }
-    };
-
-    // ---- snapshots of full states (u, v, η, sst, um, vm)
-    static int snap_alloc(swm_ctx *ctx, int count) {
-        AdjState *st = ctx->adj;
-        st->release();
-        st->snap_bytes = 6 * ctx->elems * ctx->es;
-        for (int k = 0; k < count; ++k) {
-            void *p = nullptr;
-            if (hipMalloc(&p, st->snap_bytes) != hipSuccess) {
-                st->release();
-                return ctx->fail(SWM_E_HIP, "hipMalloc of snapshot %d (%zu bytes) failed", k, st->snap_bytes);
-            }
-            st->snap.push_back((char *)p);
-        }
-        if (hipMalloc(&st->dmax, 2 * sizeof(int)) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc");
-        st->at.assign(count, -1);
-        return SWM_OK;
-    }
-    static int copy_state(swm_ctx *ctx, int k, bool save) {
-        AdjState *st = ctx->adj;
-        const size_t fb = ctx->elems * ctx->es;
-        const Field *f[6] = {&ctx->u[ctx->pu], &ctx->v[ctx->pu], &ctx->e[ctx->pe], &ctx->sst[ctx->rsst], &ctx->um, &ctx->vm};
-        for (int j = 0; j < 6; ++j) {
-            char *s = st->snap[k] + j * fb;
-            HIPCHK(hipMemcpyAsync(save ? s : f[j]->base, save ? f[j]->base : s, fb, hipMemcpyDeviceToDevice, ctx->stream));
-        }
-        return SWM_OK;
-    }
-
-    // one forward step from Prog at step i (0-based k = i - i0)
-    static int advance(swm_ctx *ctx, int64_t k) {
-        AdjState *st = ctx->adj;
-        bool nan = false;
-        const int64_t i = st->i0 + k;
-        int rc = LF::step(ctx, i, st->t0 + k * ctx->c.dtint, &nan);
-        if (rc) return rc;
-        if (nan) return ctx->fail(SWM_E_STATE, "NaN in the state at step %lld: no gradient", (long long)i);
-        st->fwd_steps += 1;
-        return SWM_OK;
-    }
-
-    // The taped step k: step i = i0 + k from Prog without copy-back, every intermediate kept
-    // in its own slot (the stage inputs s_1..s_RKo, u0 before / after the dissipation).
-    static int tape_step(swm_ctx *ctx, int64_t k, adj::AdjFields &F) {
-        AdjState *st = ctx->adj;
-        const swm_config &c = ctx->c;
-        const int64_t i = st->i0 + k, t = st->t0 + k * c.dtint;
-        const typename LF::Times T = LF::forcing_times(ctx, t);
-        int uv[6], ee[6], nu = 0, ne = 0;
-        uv[nu++] = ctx->pu;
-        ee[ne++] = ctx->pe;
-        for (int s = 0; s < 6; ++s) {
-            if (s != ctx->pu) uv[nu++] = s;
-            if (s != ctx->pe) ee[ne++] = s;
-        }
-        // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo, [5] the dissipation output
-        const Slots P{uv[0], ee[0]}, a{uv[1], ee[1]}, none{};
-        auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
-        int rc = SWM_OK;
-        for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
-            Coef<Q> kk{};
-            kk.a = (Q)c.RKaDt[q - 1];
-            kk.b = q < c.RKo ? (Q)c.RKbDt[q - 1] : Q(0);
-            const int m = q == 1 ? M_RK_FIRST : (q == c.RKo ? M_RK_LAST : M_RK_MID);
-            rc = LF::stage(ctx, m, sk(q), P, a, q < c.RKo ? sk(q + 1) : none, a, kk, T);
-        }
-        if (rc) return rc;
-        int post = a.uv;
-        if (i % c.nstep_diff == 0) {
-            post = uv[5];
-            if ((rc = LF::dissipation(ctx, a, post))) return rc;
-        }
-        const int sst_old = ctx->rsst;
-        if ((rc = LF::tracer(ctx, i, post))) return rc;
-        for (int q = 1; q <= c.RKo; ++q) {
-            const Slots s = sk(q);
-            F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
-        }
-        F.pre = adj::UVE{ctx->u[a.uv].ref(), ctx->v[a.uv].ref(), ctx->e[a.e].ref()};
-        F.post_u = ctx->u[post].ref();
-        F.post_v = ctx->v[post].ref();
-        F.um = ctx->um.ref();
-        F.vm = ctx->vm.ref();
-        F.sst = ctx->sst[sst_old].ref();
-        st->fwd_steps += 1;
-        return SWM_OK;
-    }
-
-    static int reverse_one(swm_ctx *ctx, int64_t k) {
-        AdjState *st = ctx->adj;
-        int rc = tape_step(ctx, k, st->F);
-        if (rc) return rc;
-        K kl{ctx, st};
-        return adj::reverse_step<R>(kl, st->F, consts(ctx), (long)(st->i0 + k));
-    }
-
-    // reverse steps [a, b) with the state at a in snapshot slot sa and s free snapshots
-    static int reverse(swm_ctx *ctx, int64_t a, int64_t b, int sa, int s) {
-        AdjState *st = ctx->adj;
-        int rc;
-        auto restore_advance = [&](int64_t to) {
-            int r = copy_state(ctx, sa, false);
-            for (int64_t q = a; q < to && !r; ++q) r = advance(ctx, q);
-            return r;
-        };
-        if (b - a == 1 || s == 0) {
-            for (int64_t k = b - 1; k >= a; --k) {
-                if ((rc = restore_advance(k)) || (rc = reverse_one(ctx, k))) return rc;
-            }
-            return SWM_OK;
-        }
-        const int64_t m = revolve_split(a, b, s);
-        int sm = -1;
-        for (size_t q = 1; q + 1 < st->snap.size(); ++q)
-            if (st->at[q] == m) sm = (int)q;
-        if (sm < 0) {                                            // not stored by the forward sweep
-            for (size_t q = 1; q + 1 < st->snap.size() && sm < 0; ++q)
-                if (st->at[q] < 0) sm = (int)q;
-            if (sm < 0) return ctx->fail(SWM_E_STATE, "revolve: no free snapshot");
-            if ((rc = restore_advance(m)) || (rc = copy_state(ctx, sm, true))) return rc;
-            st->at[sm] = m;
-        }
-        if ((rc = reverse(ctx, m, b, sm, s - 1))) return rc;
-        st->at[sm] = -1;
-        return reverse(ctx, a, m, sa, s);
-    }
-
-    static int forward(swm_ctx *ctx, int64_t n, int64_t i0, int64_t t0, int s) {
-        int rc = init(ctx);
-        if (rc) return rc;
-        AdjState *st = ctx->adj;
-        if ((rc = snap_alloc(ctx, s + 2))) return rc;
-        st->i0 = i0; st->t0 = t0; st->n = n; st->nsnap = s; st->recorded = false; st->fwd_steps = 0;
-        if ((rc = copy_state(ctx, 0, true))) return rc;
-        // the snapshots the schedule takes first: the chain of splits toward the last step
-        std::vector<int64_t> chain;
-        for (int64_t a = 0, sf = s; n - a > 1 && sf > 0; --sf) {
-            a = revolve_split(a, n, (int)sf);
-            chain.push_back(a);
-        }
-        size_t next = 0;
-        for (int64_t k = 0; k < n; ++k) {
-            if (next < chain.size() && chain[next] == k) {
-                if ((rc = copy_state(ctx, (int)next + 1, true))) return rc;
-                st->at[next + 1] = k;
-                ++next;
-            }
-            if ((rc = advance(ctx, k))) return rc;
-        }
-        if ((rc = copy_state(ctx, s + 1, true))) return rc;
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        st->recorded = true;
-        return SWM_OK;
-    }
-
-    static int zero_adjoint(swm_ctx *ctx) {
-        AdjState *st = ctx->adj;
-        const size_t from = std::min<size_t>(st->F0.L.u, ctx->u[4].origin);
-        const size_t words = (ctx->arena_used - from) / 4;
-        hipLaunchKernelGGL(zero_words_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t *)((char *)ctx->arena + from),
-                           words);
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    }
-
-    static int pullback(swm_ctx *ctx, const void *yu, const void *yv, const void *ye, const void *ys, void *gu, void *gv,
-                        void *ge, void *gs) {
-        AdjState *st = ctx->adj;
-        const swm_config &c = ctx->c;
-        int rc;
-        st->F = st->F0;
-        st->fwd_steps = 0;
-        if ((rc = zero_adjoint(ctx))) return rc;
-        Field tmp;
-        auto put = [&](uint32_t o, const void *h, int m, int nn, int x0, int y0) {
-            tmp.base = (char *)ctx->arena + o;
-            tmp.origin = o;
-            tmp.es = ctx->es;
-            tmp.p0 = (char *)tmp.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
-            return put2d(ctx, tmp, h, m, nn, x0, y0);
-        };
-        const int mu = ctx->nux + 4, nuu = ctx->nuy + 4, mv = ctx->nvx + 4, nv = ctx->nvy + 4;
-        if ((rc = put(st->F.L.u, yu, mu, nuu, -1 - ctx->ep, -1)) || (rc = put(st->F.L.v, yv, mv, nv, -1, -1)) ||
-            (rc = put(st->F.L.e, ye, c.nx + 2, c.ny + 2, 0, 0)) || (rc = put(st->F.Lsst, ys, c.nx + 2, c.ny, 0, 1)))
-            return rc;
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
-        rc = reverse(ctx, 0, st->n, 0, st->nsnap);
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        if (rc) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return rc;
-        }
-        HIPCHK(hipEventSynchronize(e1));
-        float tm = 0.0f;
-        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
-        st->sweep_ms = tm;
-        HIPCHK(hipEventDestroy(e0));
-        HIPCHK(hipEventDestroy(e1));
-        // the gradient: ghost copies of the uploaded state folded in (swm_upload_prog applies them)
-        adj::FoldOutArgs O{st->F.L, st->F.Z, st->F.Lsst, st->F.Lsst2, c.nx, c.ny, c.bc == SWM_PERIODIC};
-        hipLaunchKernelGGL((fold_out_kernel<R>), dim3((c.nx + 5 + 127) / 128, c.ny + 4), dim3(128), 0, ctx->stream,
-                           ctx->aref(), O, (R)c.one_minus_alpha, ctx->P);
-        HIPCHK(hipGetLastError());
-        auto get = [&](uint32_t o, void *h, int m, int nn, int x0, int y0) {
-            tmp.base = (char *)ctx->arena + o;
-            tmp.origin = o;
-            tmp.p0 = (char *)tmp.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
-            return get2d(ctx, tmp, h, m, nn, x0, y0);
-        };
-        if ((gu && (rc = get(st->F.Z.u, gu, mu, nuu, -1 - ctx->ep, -1))) || (gv && (rc = get(st->F.Z.v, gv, mv, nv, -1, -1))) ||
-            (ge && (rc = get(st->F.Z.e, ge, c.nx + 2, c.ny + 2, 0, 0))) || (gs && (rc = get(st->F.Lsst2, gs, c.nx + 2, c.ny, 0, 1))))
-            return rc;
-        // leave the context at the final state, as after swm_adj_forward
-        if ((rc = copy_state(ctx, st->nsnap + 1, false))) return rc;
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return SWM_OK;
-    }
-
-    // measurement: which 0 forward step, 1 taped step, 2 reverse step (kernels of one step,
-    // tape in place), 3 stage adjoint (middle stage), 4 dissipation adjoint
-    static int time(swm_ctx *ctx, int which, int reps, double *ms) {
-        int rc = init(ctx);
-        if (rc) return rc;
-        AdjState *st = ctx->adj;
-        if (st->snap.empty() && (rc = snap_alloc(ctx, 2))) return rc;
-        st->i0 = 1; st->t0 = 0;
-        if ((rc = copy_state(ctx, 0, true))) return rc;
-        st->F = st->F0;
-        adj::AdjFields Ft = st->F0;
-        if ((rc = tape_step(ctx, 0, Ft))) return rc;
-        K kl{ctx, st};
-        const adj::AdjConsts<R> k = consts(ctx);
-        adj::StageAdjArgs<R> S{};
-        S.s = Ft.s[1]; S.H = Ft.H; S.fq = Ft.fq; S.fq_2d = Ft.fq_2d; S.qend = Ft.qend; S.lqend = Ft.lqend;
-        S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.a_in = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
-        S.a = k.a[1]; S.b = k.b[1]; S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = k.cont;
-        S.nx = k.nx; S.ny = k.ny; S.P = k.P;
-        adj::DissAdjArgs<R> D{};
-        D.s = Ft.pre; D.H = Ft.H; D.lin = Ft.L; D.out = Ft.Z;
-        D.cD = k.cD; D.rD = k.rD; D.cS = k.cS; D.nuB = k.nuB; D.dtd = k.dtd; D.oma = k.oma;
-        D.nx = k.nx; D.ny = k.ny; D.P = k.P;
-        auto launch = [&]() -> int {
-            switch (which) {
-            case 0: { bool nan = false; return LF::step(ctx, 1, 0, &nan); }
-            case 1: { adj::AdjFields G = st->F0; return tape_step(ctx, 0, G); }
-            case 2: { st->F = Ft; return adj::reverse_step<R>(kl, st->F, k, 1); }
-            case 3: return kl.stage(S, k.per, 1, 0, 0);
-            case 4: return kl.diss(D, k.per, k.drag, k.smag);
-            }
-            return ctx->fail(SWM_E_ARG, "unknown adjoint kernel %d", which);
-        };
-        if ((rc = launch())) return rc;
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < reps && rc == SWM_OK; ++r) rc = launch();
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float tm = 0.0f;
-        HIPCHK(hipEventElapsedTime(&tm, e0, e1));
-        HIPCHK(hipEventDestroy(e0));
-        HIPCHK(hipEventDestroy(e1));
-        *ms = (double)tm / reps;
-        // the state the measurement started from
-        if (!rc) rc = copy_state(ctx, 0, false);
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        st->recorded = false;
-        return rc;
-    }
-};
-
-template <class Fn> int adj_with_fmt(swm_ctx *ctx, Fn fn) {
-    if (ctx->fmt == SWM_F64 && ctx->c.Tprog == SWM_F64) return fn(FmtF64{});
-    if (ctx->fmt == SWM_F32 && ctx->c.Tprog == SWM_F32) return fn(FmtF32{});
+// the members' work goes to the ensemble's stream while it runs
+// the adjoint's number formats: T = Tprog = Tcomm in {Float64, Float32}
+int adj_ok(swm_ctx *ctx) {
+    if ((ctx->fmt == SWM_F64 && ctx->c.Tprog == SWM_F64) || (ctx->fmt == SWM_F32 && ctx->c.Tprog == SWM_F32)) return SWM_OK;
     return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
 }
 
-}  // namespace
-
-// ====================================================================== ensembles
-//
-// An ensemble advances several contexts together.  A step of every member is planned on
-// the host (the members' kernels recorded, not launched, with their slots and scalars);
-// all members run the same kernel sequence, so each kernel of the step becomes one batched
-// launch whose waves find their member in a table of tile counts.  The packed plan (every
-// member's arguments and the tables) lives in one of two device buffers: a context's slot
-// roles repeat every two steps, so two buffers hold the steady state, and two steps without
-// host work (tracer, NaN check) replay one captured hipGraph that reads them.
-
-struct swm_ens {
-    std::vector<swm_ctx *> m;
-    std::string err;
-    hipStream_t stream = nullptr;
-    int fmt = SWM_F32;
-    bool per = false;
-    struct Plan {                      // one step of every member, packed for upload
-        std::vector<int> kind, ma, mb; // per kernel of the step
-        std::vector<size_t> args_off, tab_off, qtab_off;
-        std::vector<int> tiles, qblocks;
-        std::vector<char> blob;        // what the device buffer holds
-        std::vector<int> next_pu, next_pe, next_uv;   // member slots after the step
-    };
-    struct Buf {
-        Plan plan;
-        void *dev = nullptr;
-        size_t cap = 0;
-        bool valid = false;
-    };
-    Buf buf[2];
-    int last = 1;                      // buffer of the previous eager step
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    bool graph_ok = false;
-
-    int fail(int code, const char *fmt_, ...) {
-        char b[512];
-        va_list ap;
-        va_start(ap, fmt_);
-        vsnprintf(b, sizeof b, fmt_, ap);
-        va_end(ap);
-        err = b;
-        return code;
-    }
-};
-
-namespace {
-
-template <class F> struct Ens {
-    using Q = typename F::Q;
-    using ST = typename F::ST;
-
-    // record step i of every member (time (i-1)*dtint of the member) into a packed plan
-    static int plan(swm_ens *E, int64_t i, swm_ens::Plan &pl) {
-        const size_t nm = E->m.size();
-        std::vector<Recorder> recs(nm);
-        pl = swm_ens::Plan();
-        for (size_t k = 0; k < nm; ++k) {
-            swm_ctx *ctx = E->m[k];
-            ctx->rec = &recs[k];
-            Slots u0, u1;
-            int rc = Launch<F>::stages(ctx, 1 << 20, (i - 1) * ctx->c.dtint, &u0, &u1);
-            if (!rc && (adv_form(ctx->c) & ADV_KEEP)) {          // time_integration.jl:208-211; the quadratic
-                const bool due = i % ctx->c.nstep_advcor == 0;   // drag's KEu, KEv every step (nstep_diff = 1)
-                if (due || ctx->c.bottom_drag == SWM_DRAG_QUADRATIC) rc = Launch<F>::advcor(ctx, u0, due, false);
-            }
-            const int out = pick({ctx->pu, u0.uv});
-            if (!rc) rc = Launch<F>::dissipation(ctx, u0, out);
-            ctx->rec = nullptr;
-            if (rc) return E->fail(rc, "member %zu: %s", k, ctx->err.c_str());
-            pl.next_pu.push_back(out);
-            pl.next_pe.push_back(u0.e);
-            pl.next_uv.push_back(out);
-        }
-        const size_t nk = recs[0].items.size();
-        for (size_t k = 1; k < nm; ++k) {
-            bool same = recs[k].items.size() == nk;
-            for (size_t j = 0; same && j < nk; ++j)
-                same = recs[k].items[j].kind == recs[0].items[j].kind && recs[k].items[j].ma == recs[0].items[j].ma &&
-                       recs[k].items[j].mb == recs[0].items[j].mb;
-            if (!same) return E->fail(SWM_E_ARG, "member %zu runs another kernel sequence than member 0", k);
-        }
-        auto align = [&]() { pl.blob.resize((pl.blob.size() + 255) & ~(size_t)255, 0); };
-        for (size_t j = 0; j < nk; ++j) {
-            const RecItem &it0 = recs[0].items[j];
-            pl.kind.push_back(it0.kind);
-            pl.ma.push_back(it0.ma);
-            pl.mb.push_back(it0.mb);
-            align();
-            pl.args_off.push_back(pl.blob.size());
-            for (size_t k = 0; k < nm; ++k) pl.blob.insert(pl.blob.end(), recs[k].items[j].args.begin(), recs[k].items[j].args.end());
-            // tile table: waves per member, and (periodic stages) blocks of the q column kernel
-            std::vector<int> start(1, 0), qstart(1, 0);
-            for (size_t k = 0; k < nm; ++k) {
-                int tiles, ny;
-                if (it0.kind == 2) {
-                    const DissArgs<F> *A = (const DissArgs<F> *)recs[k].items[j].args.data();
-                    tiles = A->nstrips * A->nseg;
-                    ny = A->ny;
-                } else {
-                    const StageArgs<F> *A = (const StageArgs<F> *)recs[k].items[j].args.data();
-                    tiles = A->nstrips * A->nseg;
-                    ny = A->ny;
-                }
-                start.push_back(start.back() + tiles);
-                qstart.push_back(qstart.back() + (ny + 1 + 255) / 256);
-            }
-            align();
-            pl.tab_off.push_back(pl.blob.size());
-            pl.blob.insert(pl.blob.end(), (const char *)start.data(), (const char *)(start.data() + start.size()));
-            align();
-            pl.qtab_off.push_back(pl.blob.size());
-            pl.blob.insert(pl.blob.end(), (const char *)qstart.data(), (const char *)(qstart.data() + qstart.size()));
-            pl.tiles.push_back(start.back());
-            pl.qblocks.push_back(qstart.back());
-        }
-        // the kernel sequence itself closes the plan: two steps whose arguments are the same
-        // bytes but whose kernels differ (the kept fields' update with or without q) must not
-        // compare equal, or a buffer or graph of the one would run for the other
-        align();
-        for (size_t j = 0; j < nk; ++j) {
-            const int t[3] = {pl.kind[j], pl.ma[j], pl.mb[j]};
-            pl.blob.insert(pl.blob.end(), (const char *)t, (const char *)(t + 3));
-        }
-        return SWM_OK;
-    }
-
-    // make buffer b hold plan pl (upload only when it differs)
-    static int load(swm_ens *E, int b, const swm_ens::Plan &pl) {
-        swm_ens::Buf &B = E->buf[b];
-        if (B.valid && B.plan.blob == pl.blob) return SWM_OK;
-        if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
-        if (pl.blob.size() > B.cap) {
-            if (B.dev) (void)hipFree(B.dev);
-            B.cap = std::max<size_t>(pl.blob.size() * 2, 1 << 16);
-            if (hipMalloc(&B.dev, B.cap) != hipSuccess) return E->fail(SWM_E_HIP, "ensemble plan buffer");
-        }
-        if (hipMemcpy(B.dev, pl.blob.data(), pl.blob.size(), hipMemcpyHostToDevice) != hipSuccess)
-            return E->fail(SWM_E_HIP, "ensemble plan upload");
-        B.plan = pl;
-        B.valid = true;
-        return SWM_OK;
-    }
-
-    // the batched launches of buffer b's step on the ensemble stream
-    template <bool PER> static int launch(swm_ens *E, int b) {
-        const swm_ens::Buf &B = E->buf[b];
-        const swm_ens::Plan &pl = B.plan;
-        const int nm = (int)E->m.size();
-        const swm_config &c0 = E->m[0]->c;
-        char *dev = (char *)B.dev;
-        const dim3 block(WAVE * WAVES_PER_BLOCK);
-        for (size_t j = 0; j < pl.kind.size(); ++j) {
-            const int *tab = (const int *)(dev + pl.tab_off[j]), *qtab = (const int *)(dev + pl.qtab_off[j]);
-            const dim3 grid(nblocks_waves(pl.tiles[j]));
-            if (pl.kind[j] == 0) {
-                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                const int adv = adv_form(c0);                     // one form for all members
-                if (PER && adv == ADV_AH) {
-                    if (pl.ma[j] == M_RK_FIRST) hipLaunchKernelGGL((qend_batch<F, Q>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
-                    else hipLaunchKernelGGL((qend_batch<F, ST>), dim3(pl.qblocks[j]), dim3(256), 0, E->stream, a, qtab, nm);
-                }
-#define SWM_SB(M_)                                                                                                  \
-    if (adv == ADV_SAD) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm);  \
-    else if (adv == ADV_LIN) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm); \
-    else if (adv == (ADV_AH | ADV_KEEP)) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_AH | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
-    else if (adv == (ADV_SAD | ADV_KEEP)) hipLaunchKernelGGL((stage_batch<F, PER, M_, ADV_SAD | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
-    else hipLaunchKernelGGL((stage_batch<F, PER, M_>), grid, block, 0, E->stream, a, tab, nm);
-                switch (pl.ma[j]) {
-                case M_RK_FIRST: SWM_SB(M_RK_FIRST) break;
-                case M_RK_MID: SWM_SB(M_RK_MID) break;
-                case M_RK_LAST: SWM_SB(M_RK_LAST) break;
-                default: return E->fail(SWM_E_UNSUPPORTED, "stage mode %d in an ensemble", pl.ma[j]);
-                }
-#undef SWM_SB
-            } else if (pl.kind[j] == 1 && adv_form(c0) != ADV_AH) {   // Sadourny / linear members (RK4, T = Tprog)
-                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                if constexpr (!F::mixed) {
-                    const int adv = adv_form(c0);
-                    if (adv != ADV_SAD && adv != ADV_LIN && adv != (ADV_AH | ADV_KEEP) && adv != (ADV_SAD | ADV_KEEP))
-                        return E->fail(SWM_E_UNSUPPORTED, "right-hand side form %d of fused stages in an ensemble", adv);
-#define SWM_FBV(MA_, MB_)                                                                                          \
-    if (adv == ADV_SAD) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_SAD>), grid, block, 0, E->stream, a, tab, nm); \
-    else if (adv == ADV_LIN) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_LIN>), grid, block, 0, E->stream, a, tab, nm); \
-    else if (adv == (ADV_AH | ADV_KEEP)) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_AH | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm); \
-    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER, ADV_SAD | ADV_KEEP>), grid, block, 0, E->stream, a, tab, nm)
-                    if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FBV(M_RK_FIRST, M_RK_MID); }
-                    else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FBV(M_RK_MID, M_RK_LAST); }
-                    else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
-#undef SWM_FBV
-                } else {
-                    return E->fail(SWM_E_UNSUPPORTED, "fused stages of this right-hand side with mixed formats");
-                }
-            } else if (pl.kind[j] == 1) {
-                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                const bool cs = c0.continuity != SWM_CONT_ITSELF;
-#define SWM_FB(MA_, MB_)                                                                                        \
-    if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, PER>), grid, block, 0, E->stream, a, tab, nm);   \
-    else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, PER>), grid, block, 0, E->stream, a, tab, nm)
-                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) SWM_FB(M_RK_FIRST, M_RK_MID);
-                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_MID, M_RK_LAST);
-                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) SWM_FB(M_RK_FIRST, M_RK_LAST);
-                else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
-#undef SWM_FB
-            } else if (pl.kind[j] == 4 || pl.kind[j] == 5) {      // periodic pairs: strips clear of the seam / at it
-                const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
-                const bool cs = c0.continuity != SWM_CONT_ITSELF;
-                if (pl.tiles[j] == 0) continue;
-#define SWM_FB2(MA_, MB_)                                                                                         \
-    if (pl.kind[j] == 4) {                                                                                        \
-        if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, true>), grid, block, 0, E->stream, a, tab, nm); \
-        else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, true>), grid, block, 0, E->stream, a, tab, nm);  \
-    } else if (cs) hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, true>), grid, block, 0, E->stream, a, tab, nm); \
-    else hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, false>), grid, block, 0, E->stream, a, tab, nm)
-                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FB2(M_RK_FIRST, M_RK_MID); }
-                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_MID, M_RK_LAST); }
-                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_FIRST, M_RK_LAST); }
-                else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
-#undef SWM_FB2
-            } else if (pl.kind[j] == 2) {
-                const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
-                const bool smag = c0.diffusion == SWM_DIFF_SMAGORINSKY;
-#define SWM_DB(D_, S_) hipLaunchKernelGGL((dissipation_batch<F, PER, D_, S_>), grid, block, 0, E->stream, a, tab, nm)
-                if (c0.bottom_drag == SWM_DRAG_QUADRATIC) { if (smag) SWM_DB(0, true); else SWM_DB(0, false); }
-                else if (c0.bottom_drag == SWM_DRAG_LINEAR) { if (smag) SWM_DB(1, true); else SWM_DB(1, false); }
-                else { if (smag) SWM_DB(2, true); else SWM_DB(2, false); }
-#undef SWM_DB
-            } else if (pl.kind[j] == 6) {                        // kept q and KEu+KEv: one launch per member
-                for (int k = 0; k < nm; ++k) {
-                    StageArgs<F> A;
-                    memcpy(&A, pl.blob.data() + pl.args_off[j] + (size_t)k * sizeof A, sizeof A);
-                    const swm_config &c = E->m[k]->c;
-                    const dim3 g((c.nx + 2 + 255) / 256, c.ny + 2), b(256);
-                    if (pl.ma[j] & 2) hipLaunchKernelGGL((advcor_kernel<F, Q, true, true>), g, b, 0, E->stream, A);
-                    else if (pl.ma[j] & 1) hipLaunchKernelGGL((advcor_kernel<F, Q, true, false>), g, b, 0, E->stream, A);
-                    else hipLaunchKernelGGL((advcor_kernel<F, Q, false, false>), g, b, 0, E->stream, A);
-                }
-            } else return E->fail(SWM_E_UNSUPPORTED, "kernel kind %d in an ensemble", pl.kind[j]);
-        }
-        if (hipGetLastError() != hipSuccess) return E->fail(SWM_E_HIP, "batched launch");
-        return SWM_OK;
-    }
-    static int launch_buf(swm_ens *E, int b) { return E->per ? launch<true>(E, b) : launch<false>(E, b); }
-
-    // host work a member has at step i: tracer snapshot or advection, NaN check
-    static bool events(const swm_ens *E, int64_t i) {
-        for (const swm_ctx *ctx : E->m) {
-            const swm_config &c = ctx->c;
-            if (c.tracer_advection && ((i + c.nadvstep_half) % c.nadvstep == 0 || i % c.nadvstep == 0)) return true;
-            if (c.nout > 0 && i % c.nout == 0) return true;
-        }
-        return false;
-    }
-
-    static int capture(swm_ens *E) {
-        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
-        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
-        if (hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-            return E->fail(SWM_E_HIP, "graph capture");
-        int rc = launch_buf(E, 0);
-        if (!rc) rc = launch_buf(E, 1);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(E->stream, &g);
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        if (e != hipSuccess) return E->fail(SWM_E_HIP, "graph capture end");
-        E->graph = g;
-        if (hipGraphInstantiate(&E->exec, g, nullptr, nullptr, 0) != hipSuccess) return E->fail(SWM_E_HIP, "graph instantiate");
-        E->graph_ok = true;
-        return SWM_OK;
-    }
-
-    static void advance(swm_ens *E, const swm_ens::Plan &pl) {
-        for (size_t k = 0; k < E->m.size(); ++k) {
-            E->m[k]->pu = pl.next_pu[k];
-            E->m[k]->pe = pl.next_pe[k];
-        }
-    }
-
-    static int step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, int64_t *gsteps) {
-        const size_t nm = E->m.size();
-        *done = 0;
-        *gsteps = 0;
-        for (size_t k = 0; k < nm; ++k) nan_steps[k] = 0;
-        swm_ens::Plan p0, p1;
-        for (int64_t k = 0; k < nsteps;) {
-            const int64_t i = i0 + k;
-            int rc = plan(E, i, p0);
-            if (rc) return rc;
-            // two steps as one graph replay: both plans already in the buffers, no host work
-            if (k + 1 < nsteps && !events(E, i) && !events(E, i + 1) && E->buf[0].valid && E->buf[1].valid &&
-                p0.blob == E->buf[0].plan.blob) {
-                std::vector<int> pu(nm), pe(nm);
-                for (size_t q = 0; q < nm; ++q) { pu[q] = E->m[q]->pu; pe[q] = E->m[q]->pe; }
-                advance(E, p0);
-                rc = plan(E, i + 1, p1);
-                for (size_t q = 0; q < nm; ++q) { E->m[q]->pu = pu[q]; E->m[q]->pe = pe[q]; }
-                if (rc) return rc;
-                if (p1.blob == E->buf[1].plan.blob) {
-                    if (!E->graph_ok && (rc = capture(E))) return rc;
-                    if (hipGraphLaunch(E->exec, E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "graph launch");
-                    advance(E, p0);
-                    advance(E, p1);
-                    k += 2;
-                    *gsteps += 2;
-                    *done = k;
-                    continue;
-                }
-            }
-            // one eager step: the plan into the buffer it matches, else the one not used last
-            int b = E->buf[0].valid && E->buf[0].plan.blob == p0.blob ? 0
-                  : E->buf[1].valid && E->buf[1].plan.blob == p0.blob ? 1 : 1 - E->last;
-            if ((rc = load(E, b, p0))) return rc;
-            if (b != E->last) { /* alternate */ }
-            E->last = b;
-            if ((rc = launch_buf(E, b))) return rc;
-            // per-member host work, in the member's own order (time_integration.jl:226-237)
-            bool any_nan = false;
-            for (size_t q = 0; q < nm; ++q) {
-                swm_ctx *ctx = E->m[q];
-                if ((rc = Launch<F>::tracer(ctx, i, p0.next_uv[q]))) return E->fail(rc, "member %zu: %s", q, ctx->err.c_str());
-                if (ctx->c.nout > 0 && i % ctx->c.nout == 0) {
-                    bool nan = false;
-                    if ((rc = Launch<F>::nan_check(ctx, ctx->pu, ctx->pe, ctx->rsst, &nan)))
-                        return E->fail(rc, "member %zu: %s", q, ctx->err.c_str());
-                    if (nan) { nan_steps[q] = i; any_nan = true; }
-                }
-            }
-            for (size_t q = 0; q < nm; ++q)
-                if (!nan_steps[q]) { E->m[q]->pu = p0.next_pu[q]; E->m[q]->pe = p0.next_pe[q]; }
-            ++k;
-            *done = k;
-            if (any_nan) break;
-        }
-        if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
-        int flags[2] = {0, 0};
-        for (size_t q = 0; q < nm; ++q) {
-            swm_ctx *ctx = E->m[q];
-            if (hipMemcpy(flags, ctx->dflags, sizeof flags, hipMemcpyDeviceToHost) != hipSuccess) return E->fail(SWM_E_HIP, "flags");
-            if (flags[1]) {
-                int z = 0;
-                (void)hipMemcpy(ctx->dflags + 1, &z, sizeof z, hipMemcpyHostToDevice);
-                return E->fail(SWM_E_ARG, "member %zu: tracer departure point is NaN or outside the grid", q);
-            }
-        }
-        return SWM_OK;
-    }
-
-    // time `reps` batched steps (plan of the current state, into buffer 0) with HIP events
-    static int time_step(swm_ens *E, int reps, double *ms) {
-        swm_ens::Plan pl;
-        int rc = plan(E, 1, pl);
-        if (rc) return rc;
-        E->buf[0].valid = false;
-        if ((rc = load(E, 0, pl))) return rc;
-        if ((rc = launch_buf(E, 0))) return rc;                 // warm-up
-        hipEvent_t e0, e1;
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return E->fail(SWM_E_HIP, "event");
-        (void)hipEventRecord(e0, E->stream);
-        for (int r = 0; r < reps && !rc; ++r) rc = launch_buf(E, 0);
-        (void)hipEventRecord(e1, E->stream);
-        (void)hipEventSynchronize(e1);
-        float t = 0;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *ms = t / reps;
-        return rc;
-    }
-
-    // Autotuner: for each kernel class (stage, fused pair, dissipation) try segment lengths
-    // and keep the one with the fastest batched step; the others held at their choice.
-    // The timed steps leave each member's state in its slots, but they also update the kept
-    // q and KEu+KEv (nstep_advcor > 0) and take the pending update from the initial state:
-    // both are saved before timing and put back after it.
-    static int tune(swm_ens *E, int reps, int *Ls, int *Lf, int *Ld) {
-        struct Kept { swm_ctx *ctx; char *buf; size_t n; bool pending; };
-        std::vector<Kept> kept;
-        int rc = hipStreamSynchronize(E->stream) == hipSuccess ? SWM_OK : E->fail(SWM_E_HIP, "stream synchronize");
-        for (swm_ctx *ctx : E->m) {
-            if (rc || !(adv_form(ctx->c) & ADV_KEEP)) continue;
-            Kept k{ctx, nullptr, ctx->elems * (size_t)ctx->qs.es, ctx->advcor_pending};
-            if (hipMalloc((void **)&k.buf, 2 * k.n) != hipSuccess) { rc = E->fail(SWM_E_HIP, "kept fields' copy"); break; }
-            kept.push_back(k);
-            if (hipMemcpy(k.buf, ctx->qs.base, k.n, hipMemcpyDeviceToDevice) != hipSuccess ||
-                hipMemcpy(k.buf + k.n, ctx->ks.base, k.n, hipMemcpyDeviceToDevice) != hipSuccess)
-                rc = E->fail(SWM_E_HIP, "kept fields' copy");
-        }
-        if (!rc) rc = tune_segments(E, reps, Ls, Lf, Ld);
-        if (hipStreamSynchronize(E->stream) != hipSuccess && !rc) rc = E->fail(SWM_E_HIP, "stream synchronize");
-        for (const Kept &k : kept) {
-            if ((hipMemcpy(k.ctx->qs.base, k.buf, k.n, hipMemcpyDeviceToDevice) != hipSuccess ||
-                 hipMemcpy(k.ctx->ks.base, k.buf + k.n, k.n, hipMemcpyDeviceToDevice) != hipSuccess) && !rc)
-                rc = E->fail(SWM_E_HIP, "kept fields' restore");
-            k.ctx->advcor_pending = k.pending;
-            (void)hipFree(k.buf);
-        }
-        return rc;
-    }
-
-    static int tune_segments(swm_ens *E, int reps, int *Ls, int *Lf, int *Ld) {
-        static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48};
-        for (int cls = 0; cls < 3; ++cls) {
-            double best = 1e300;
-            int bestL = 0;
-            for (int L : cand) {
-                for (swm_ctx *ctx : E->m) (cls == 0 ? ctx->L : cls == 1 ? ctx->Lf : ctx->Ld) = (cls == 0 ? std::max(L, 4) : std::max(L, 8));
-                double ms = 0;
-                int rc = time_step(E, reps, &ms);
-                if (rc) return rc;
-                if (ms < best) { best = ms; bestL = cls == 0 ? std::max(L, 4) : std::max(L, 8); }
-            }
-            for (swm_ctx *ctx : E->m) (cls == 0 ? ctx->L : cls == 1 ? ctx->Lf : ctx->Ld) = bestL;
-        }
-        *Ls = E->m[0]->L;
-        *Lf = E->m[0]->Lf;
-        *Ld = E->m[0]->Ld;
-        E->buf[0].valid = E->buf[1].valid = false;     // plans (and the graph's launch sizes) changed
-        E->graph_ok = false;
-        return SWM_OK;
-    }
-};
-
-template <class Fn> int ens_with_fmt(const swm_ens *E, Fn fn) {
-    switch (E->fmt) {
-    case SWM_F64: return fn(FmtF64{});
-    case SWM_F32: return fn(FmtF32{});
-    case SWM_F16: return fn(FmtF16{});
-    default: return fn(FmtBF16{});
-    }
-}
-
-// the members' work goes to the ensemble's stream while it runs
 struct StreamSwap {
     swm_ens *E;
     std::vector<hipStream_t> saved;
@@ -2018,12 +129,7 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         swm_destroy(ctx);
         return SWM_E_HIP;
     }
-    with_fmt(ctx, [&](auto f) {
-        using F = decltype(f);
-        if (c.bc == SWM_PERIODIC) Launch<F>::template segment_lengths<true>(ctx);
-        else Launch<F>::template segment_lengths<false>(ctx);
-        return 0;
-    });
+    fmt_ops(ctx->fmt).segment_lengths(ctx);
     // tuning overrides: rows each wavefront marches in the stage / dissipation kernels
     if (const char *s = getenv("SWM_ROWS_PER_WAVE")) {
         const int l = atoi(s);
@@ -2130,14 +236,7 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
     if ((rc = put_static(ctx, ctx->Fy, &ctx->Fy_sx, &ctx->Fy_sy, Fy, ctx->nvx, ctx->nvy, 1, 1))) return rc;
     if ((rc = put_static(ctx, ctx->fq, &ctx->fq_sx, &ctx->fq_sy, f_q, c.nx + 1, c.ny + 1, 0, 0))) return rc;
     if (c.scale_log2 != 0) {                     // the forcing and f of velocities carried scaled: ×s (exact)
-        rc = with_fmt(ctx, [&](auto f) {
-            using ST = typename decltype(f)::ST;
-            for (Field *g : {&ctx->Fx, &ctx->Fy, &ctx->fq})
-                hipLaunchKernelGGL((scale_kernel<ST>), dim3(1024), dim3(256), 0, ctx->stream, (ST *)g->base, ctx->elems,
-                                   (float)ctx->sc);
-            HIPCHK(hipGetLastError());
-            return SWM_OK;
-        });
+        rc = fmt_ops(ctx->fmt).scale_static(ctx);
         if (rc) return rc;
     }
     if (c.bc == SWM_PERIODIC) {
@@ -2225,19 +324,7 @@ int swm_upload_prog(swm_ctx *ctx, const void *u, const void *v, const void *eta,
     if ((rc = put2d(ctx, ctx->v[pu], v, ctx->nvx + 4, ctx->nvy + 4, -1, -1))) return rc;
     if ((rc = put2d(ctx, ctx->e[pe], eta, c.nx + 2, c.ny + 2, 0, 0))) return rc;
     if ((rc = put2d(ctx, ctx->sst[ctx->rsst], sst, c.nx + 2, c.ny, 0, 1))) return rc;
-    rc = with_fmt(ctx, [&](auto f) {
-        using Q = typename decltype(f)::Q;
-        if (c.scale_log2 != 0)
-            for (Field *g : {&ctx->u[pu], &ctx->v[pu]})
-                hipLaunchKernelGGL((scale_kernel<Q>), dim3(1024), dim3(256), 0, ctx->stream, (Q *)g->base, ctx->elems,
-                                   (float)ctx->sc);
-        if (c.compensated)                       // a new state: no compensation carried over
-            for (Field *g : {&ctx->cu, &ctx->cv, &ctx->ce})
-                hipLaunchKernelGGL(zero_words_kernel, dim3(1024), dim3(256), 0, ctx->stream, (uint32_t *)g->base,
-                                   ctx->elems * ctx->es / 4);
-        HIPCHK(hipGetLastError());
-        return Launch<decltype(f)>::ghosts(ctx, pu, pe, ctx->rsst);
-    });
+    rc = fmt_ops(ctx->fmt).upload_fix(ctx, pu, pe);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_prog = true;
@@ -2273,7 +360,7 @@ int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int64_t *step
     if (!ctx || !steps_done || !nan_step) return SWM_E_ARG;
     if (int rc = step_ready(ctx)) return rc;
     if (nsteps < 0 || i0 < 1) return ctx->fail(SWM_E_ARG, "nsteps >= 0 and i0 >= 1 required");
-    return with_fmt(ctx, [&](auto f) { return run_steps<decltype(f)>(ctx, nsteps, i0, t0, steps_done, nan_step); });
+    return fmt_ops(ctx->fmt).run_steps(ctx, nsteps, i0, t0, steps_done, nan_step);
 }
 
 int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
@@ -2284,7 +371,7 @@ int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
                     : c.time_scheme == SWM_SSPRK3 ? c.ssp_s : 4;
     if (n_stages < 1 || n_stages > total) return ctx->fail(SWM_E_ARG, "n_stages %d out of 1..%d", n_stages, total);
     (void)i0;
-    int rc = with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::stages(ctx, n_stages, t0, &ctx->rk0, &ctx->rk1, true); });
+    int rc = fmt_ops(ctx->fmt).stages(ctx, n_stages, t0);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
@@ -2316,7 +403,7 @@ int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     if (!ctx || !ms_avg) return SWM_E_ARG;
     if (int rc = step_ready(ctx)) return rc;
     if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");
-    return with_fmt(ctx, [&](auto f) { return Launch<decltype(f)>::time_kernel(ctx, which, reps, ms_avg); });
+    return fmt_ops(ctx->fmt).time_kernel(ctx, which, reps, ms_avg);
 }
 
 int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
@@ -2324,16 +411,7 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     const swm_config &c = ctx->c;
     const int pu = ctx->pu, pe = ctx->pe;
-    int rc = with_fmt(ctx, [&](auto f) {
-        using F = decltype(f);
-        using Q = typename F::Q;
-        using ST = typename F::ST;
-        reduce_kernel<Q, ST><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<Q>(ctx->u[pu]), ctx->at<Q>(ctx->v[pu]),
-                                                            ctx->at<Q>(ctx->e[pe]), ctx->at<ST>(ctx->H), c.nx, c.ny,
-                                                            ctx->P, ctx->ep, c.g, ctx->sci, ctx->dpart);
-        HIPCHK(hipGetLastError());
-        return SWM_OK;
-    });
+    int rc = fmt_ops(ctx->fmt).reduce_launch(ctx, pu, pe);
     if (rc) return rc;
     std::vector<double> part(2 * (size_t)c.ny);
     HIPCHK(hipMemcpyAsync(part.data(), ctx->dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -2421,7 +499,7 @@ int swm_ens_tune(swm_ens *E, int32_t reps, int32_t *rows_stage, int32_t *rows_fu
     for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
     StreamSwap sw(E);
     int Ls = 0, Lf = 0, Ld = 0;
-    int rc = ens_with_fmt(E, [&](auto f) { return Ens<decltype(f)>::tune(E, reps, &Ls, &Lf, &Ld); });
+    int rc = fmt_ops(E->fmt).ens_tune(E, reps, &Ls, &Lf, &Ld);
     if (rc) return rc;
     if (rows_stage) *rows_stage = Ls;
     if (rows_fused) *rows_fused = Lf;
@@ -2433,7 +511,7 @@ int swm_ens_step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *steps_done, in
     if (!E || !steps_done || !nan_steps || !graph_steps || nsteps < 0 || i0 < 1) return SWM_E_ARG;
     for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
     StreamSwap sw(E);
-    return ens_with_fmt(E, [&](auto f) { return Ens<decltype(f)>::step(E, nsteps, i0, steps_done, nan_steps, graph_steps); });
+    return fmt_ops(E->fmt).ens_step(E, nsteps, i0, steps_done, nan_steps, graph_steps);
 }
 
 // ---- reverse mode
@@ -2441,7 +519,8 @@ int swm_ens_step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *steps_done, in
 int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots) {
     if (!ctx) return SWM_E_ARG;
     if (nsteps < 1 || i0 < 1 || snapshots < 0) return ctx->fail(SWM_E_ARG, "nsteps >= 1, i0 >= 1 and snapshots >= 0 required");
-    return adj_with_fmt(ctx, [&](auto f) { return Adj<decltype(f)>::forward(ctx, nsteps, i0, t0, snapshots); });
+    if (int rc = adj_ok(ctx)) return rc;
+    return fmt_ops(ctx->fmt).adj_forward(ctx, nsteps, i0, t0, snapshots);
 }
 
 int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
@@ -2449,9 +528,8 @@ int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const
     if (!ctx) return SWM_E_ARG;
     if (!ybar_u || !ybar_v || !ybar_eta || !ybar_sst) return ctx->fail(SWM_E_ARG, "null cotangent array");
     if (!ctx->adj || !ctx->adj->recorded) return ctx->fail(SWM_E_STATE, "call swm_adj_forward first");
-    const int rc = adj_with_fmt(ctx, [&](auto f) {
-        return Adj<decltype(f)>::pullback(ctx, ybar_u, ybar_v, ybar_eta, ybar_sst, xbar_u, xbar_v, xbar_eta, xbar_sst);
-    });
+    int rc = adj_ok(ctx);
+    if (!rc) rc = fmt_ops(ctx->fmt).adj_pullback(ctx, ybar_u, ybar_v, ybar_eta, ybar_sst, xbar_u, xbar_v, xbar_eta, xbar_sst);
     if (forward_steps) *forward_steps = ctx->adj->fwd_steps;
     if (sweep_ms) *sweep_ms = ctx->adj->sweep_ms;
     return rc;
@@ -2460,7 +538,8 @@ int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const
 int swm_adj_time(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     if (!ctx || !ms_avg) return SWM_E_ARG;
     if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");
-    return adj_with_fmt(ctx, [&](auto f) { return Adj<decltype(f)>::time(ctx, which, reps, ms_avg); });
+    if (int rc = adj_ok(ctx)) return rc;
+    return fmt_ops(ctx->fmt).adj_time(ctx, which, reps, ms_avg);
 }
 
 }  // extern "C"
