@@ -87,6 +87,67 @@ def tracer(cfg, i, u, v, um, vm, sst):
     return um, vm, sst
 
 
+# outputs of the dissipation operator (swmo_op op 1), in order
+DISSIPATION_AUX = ("Bu", "Bv", "LLu1", "LLu2", "LLv1", "LLv2", "DT", "DS", "nuSmag")
+
+
+def _aux_shapes(G):
+    nx, ny = G.nx, G.ny
+    return {"Bu": (nx + 1, ny + 2), "Bv": (nx + 2, ny + 1), "LLu1": (G.nux, G.nuy + 2), "LLu2": (nx + 1, ny),
+            "LLv1": (nx, ny + 1), "LLv2": (G.nvx + 2, G.nvy), "DT": (nx + 2, ny + 2), "DS": (nx + 2, ny + 2),
+            "nuSmag": (nx + 2, ny + 2)}
+
+
+def op(S, cfg, which, u, v, eta, sst, i=0, um=None, vm=None):
+    """One operator of the step on copies of a state (Tprog storage, Julia layout):
+    ``"ghost"`` ghost_points! of u, v, η and sst; ``"dissipation"`` drag, diffusion,
+    add_drag_diff_tendencies! and ghost_points_uv! of (u, v, η) as (u0, v0, η0);
+    ``"tracer"`` tracer! at step ``i`` with (u, v) as the post-dissipation velocities and the
+    mid-point snapshot (um, vm).  Returns ``(u, v, η, sst, aux)``: aux is a dict of the
+    dissipation's intermediates (float64) or, for the tracer, the updated um, vm."""
+    F, G = S.forcing, S.grid
+    code = {"ghost": 0, "dissipation": 1, "tracer": 2}[which]
+    u, v, e, s = (np.array(x, order="F", copy=True) for x in (u, v, eta, sst))
+    vp = ctypes.c_void_p
+    aux, names = None, ()
+    if code == 1:
+        names = DISSIPATION_AUX
+        shp = _aux_shapes(G)
+        aux = {k: np.zeros(shp[k], order="F") for k in names}
+    elif code == 2:
+        names = ("um", "vm")
+        aux = {"um": np.array(um, dtype=np.float64, order="F", copy=True),
+               "vm": np.array(vm, dtype=np.float64, order="F", copy=True)}
+    ptrs = (vp * 9)(*[aux[k].ctypes.data for k in names]) if aux else None
+    L = lib()
+    L.swmo_op.argtypes = [vp] * 5 + [ctypes.c_int32] + [vp] * 4 + [ctypes.c_int64, vp]
+    statics = [_fortran(F.H), _fortran(F.Fx), _fortran(F.Fy), _fortran(G.f_q)]
+    rc = L.swmo_op(ctypes.byref(cfg), *[_ptr(a) for a in statics], code, _ptr(u), _ptr(v), _ptr(e), _ptr(s), i,
+                   ctypes.cast(ptrs, vp) if ptrs is not None else None)
+    if rc != 0:
+        raise OracleError(f"oracle operator {which} failed")
+    return u, v, e, s, aux
+
+
+class mutation:
+    """Context manager planting mistake ``k`` in the oracle (the MUT_* list of
+    swm_oracle.cpp) for the pins' own tests: inside it the restatement is deliberately
+    wrong, and the pin that targets the mistake must fail."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        L = lib()
+        L.swmo_set_mutation.argtypes = [ctypes.c_int32]
+        L.swmo_set_mutation(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().swmo_set_mutation(0)
+        return False
+
+
 def pv_advection(cfg, q, U, V):
     """The PV advection terms alone (Float64) of the configured scheme: from q at the corners
     (nx+1, ny+1) and the mass fluxes U (nx+1, ny+2), V (nx+2, ny+1) -> (qhv, qhu) of shapes

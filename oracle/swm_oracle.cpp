@@ -275,6 +275,26 @@ template <class T> static void Ixy(Arr<T> &o, const Arr<T> &u) {
 
 // ------------------------------------------------------------------ the model
 
+// ------------------------------------------------------------------ planted mistakes
+// Test infrastructure of the pins: swmo_set_mutation(k) plants mistake k in this restatement
+// so that a test can show its closed form fails under that mistake (0: none, the default).
+enum {
+    MUT_NONE = 0,
+    MUT_DT_MINUS = 1,        // DT = (dudx - dvdy)^2; diffusion.jl:95 has +
+    MUT_S12_NO_EP = 2,       // S12 from dLudy[i,j] instead of dLudy[i+ep,j]; diffusion.jl:148,197
+    MUT_BU_NO_CD = 3,        // Bu without the drag coefficient cD; bottom_drag.jl:62
+    MUT_SQRTKE_ROW = 4,      // sqrtKE from KEu[i+ep,j] instead of KEu[i+ep,j+1]; bottom_drag.jl:48
+    MUT_ADD_BU_SHIFT = 5,    // u0 += Δt_diff·Bu[i-ep,j+1] instead of Bu[i+1-ep,j+1]; diffusion.jl:237
+    MUT_RK_U1_FROM_U1 = 6,   // u1 = u1 + RKb·du instead of u + RKb·du; time_integration.jl:69
+    MUT_SSPRK2_WEIGHTS = 7,  // u0 = b·u + a·u1 (weights swapped); time_integration.jl:108
+    MUT_SSPRK3_KN = 8,       // kna and knb swapped at stage kn; time_integration.jl:129
+    MUT_4SSPRK3_THIRDS = 9,  // u1 = 1/3·u + 2/3·u1; time_integration.jl:187
+    MUT_RELAX_SIGN = 10,     // γ·(η - η_ref); continuity.jl:21
+    MUT_FTIME_COS = 11,      // Ftime = cos(ω t); forcing.jl:216
+    MUT_S11_ROW = 12,        // constant viscosity: S11 from dLudx[i,j+1]; diffusion.jl:188
+};
+static int g_mutation = MUT_NONE;
+
 template <class T, class Tp, class Tc> struct Model {
     const swm_config &c;
     int nx, ny, nux, nuy, nvx, nvy, ep;
@@ -583,7 +603,9 @@ template <class T, class Tp, class Tc> struct Model {
         for (int j = 1; j <= p.n; ++j)
             for (int i = 1; i <= p.m; ++i) p(i, j) = half * (KEu(i + ep, j + 1) + KEv(i + 1, j)) + g * e(i, j);
     }
-    T Ftime(int64_t t, double w) const { return cvt<T>(std::sin(w * (double)t)); }  // forcing.jl:215-217
+    T Ftime(int64_t t, double w) const {                  // forcing.jl:215-217
+        return cvt<T>(g_mutation == MUT_FTIME_COS ? std::cos(w * (double)t) : std::sin(w * (double)t));
+    }
     void momentum_u(int64_t t) {                               // rhs.jl:218-245
         const int m = du.m - 4, n = du.n - 4;
         CHECK(m == qhv.m && n == qhv.n && m + 2 - ep == dpdx.m && n + 2 == dpdx.n && m == Fx.m && n == Fx.n);
@@ -642,7 +664,8 @@ template <class T, class Tp, class Tc> struct Model {
             for (int j = 1; j <= n; ++j)
                 for (int i = 1; i <= m; ++i)
                     deta(i + 1, j + 1) = -(cvt<Tp>(dUdx(i, j + 1)) + cvt<Tp>(dVdy(i + 1, j))) * Tp(sci) +
-                                         cvt<Tp>(gam * (eta_ref(i, j) - e(i + 1, j + 1)));
+                                         cvt<Tp>(gam * (g_mutation == MUT_RELAX_SIGN ? e(i + 1, j + 1) - eta_ref(i, j)
+                                                                                     : eta_ref(i, j) - e(i + 1, j + 1)));
         } else if (c.continuity == SWM_CONT_FORCING) {         // continuity.jl:27-49
             const T Fet = Ftime(t, c.omegaFeta);
             for (int j = 1; j <= n; ++j)
@@ -666,11 +689,13 @@ template <class T, class Tp, class Tc> struct Model {
             Iy(KEv, v2);
             CHECK(sqrtKE.m + ep == KEu.m && sqrtKE.n + 2 == KEu.n);
             for (int j = 1; j <= sqrtKE.n; ++j)
-                for (int i = 1; i <= sqrtKE.m; ++i) sqrtKE(i, j) = num_sqrt(KEu(i + ep, j + 1) + KEv(i + 1, j)) * T(sci);
+                for (int i = 1; i <= sqrtKE.m; ++i)
+                    sqrtKE(i, j) = num_sqrt(KEu(i + ep, g_mutation == MUT_SQRTKE_ROW ? j : j + 1) + KEv(i + 1, j)) * T(sci);
             Ix(sqrtKE_u, sqrtKE);
             Iy(sqrtKE_v, sqrtKE);
             for (int j = 1; j <= Bu.n; ++j)
-                for (int i = 1; i <= Bu.m; ++i) Bu(i, j) = cD * sqrtKE_u(i, j) * uu(i + 1 + ep, j + 1) / h_u(i, j);
+                for (int i = 1; i <= Bu.m; ++i)
+                    Bu(i, j) = (g_mutation == MUT_BU_NO_CD ? T(1.0) : cD) * sqrtKE_u(i, j) * uu(i + 1 + ep, j + 1) / h_u(i, j);
             for (int j = 1; j <= Bv.n; ++j)
                 for (int i = 1; i <= Bv.m; ++i) Bv(i, j) = cD * sqrtKE_v(i, j) * vv(i + 1, j + 1) / h_v(i, j);
         } else if (c.bottom_drag == SWM_DRAG_LINEAR) {         // bottom_drag.jl:80-107
@@ -706,7 +731,7 @@ template <class T, class Tp, class Tc> struct Model {
         CHECK(DT.m + ep == dudx.m && DT.n + 2 == dudx.n && DT.m + 2 == dvdy.m && DT.n == dvdy.n);
         for (int j = 1; j <= DT.n; ++j)
             for (int i = 1; i <= DT.m; ++i) {
-                T s = dudx(i + ep, j + 1) + dvdy(i + 1, j);
+                T s = g_mutation == MUT_DT_MINUS ? dudx(i + ep, j + 1) - dvdy(i + 1, j) : dudx(i + ep, j + 1) + dvdy(i + 1, j);
                 DT(i, j) = s * s;
             }
         for (int j = 1; j <= nuSmag.n; ++j)
@@ -718,7 +743,7 @@ template <class T, class Tp, class Tc> struct Model {
         for (int j = 1; j <= S11.n; ++j)
             for (int i = 1; i <= S11.m; ++i) S11(i, j) = nuSmag(i + 1 - ep, j) * dLudx(i, j);
         for (int j = 1; j <= S12.n; ++j)
-            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nuSmag_q(i, j) * dLudy(i + ep, j);
+            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nuSmag_q(i, j) * dLudy(i + (g_mutation == MUT_S12_NO_EP ? 0 : ep), j);
         for (int j = 1; j <= S21.n; ++j)
             for (int i = 1; i <= S21.m; ++i) S21(i, j) = nuSmag_q(i, j) * dLvdx(i, j);
         for (int j = 1; j <= S22.n; ++j)
@@ -727,9 +752,9 @@ template <class T, class Tp, class Tc> struct Model {
     void viscous_tensor_constant() {                           // diffusion.jl:175-218
         const T nu = T(c.nuB);
         for (int j = 1; j <= S11.n; ++j)
-            for (int i = 1; i <= S11.m; ++i) S11(i, j) = nu * dLudx(i, j);
+            for (int i = 1; i <= S11.m; ++i) S11(i, j) = nu * dLudx(i, g_mutation == MUT_S11_ROW && j < S11.n ? j + 1 : j);
         for (int j = 1; j <= S12.n; ++j)
-            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nu * dLudy(i + ep, j);
+            for (int i = 1; i <= S12.m; ++i) S12(i, j) = nu * dLudy(i + (g_mutation == MUT_S12_NO_EP ? 0 : ep), j);
         for (int j = 1; j <= S21.n; ++j)
             for (int i = 1; i <= S21.m; ++i) S21(i, j) = nu * dLvdx(i, j);
         for (int j = 1; j <= S22.n; ++j)
@@ -759,7 +784,8 @@ template <class T, class Tp, class Tc> struct Model {
         CHECK(m + 2 - ep == Bu.m && n + 2 == Bu.n && m == LLu1.m && n + 2 == LLu1.n && m + 2 - ep == LLu2.m && n == LLu2.n);
         for (int j = 1; j <= n; ++j)
             for (int i = 1; i <= m; ++i)
-                uu(i + 2, j + 2) += dtd * (cvt<Tp>(Bu(i + 1 - ep, j + 1)) + cvt<Tp>(LLu1(i, j + 1)) + cvt<Tp>(LLu2(i + 1 - ep, j)));
+                uu(i + 2, j + 2) += dtd * (cvt<Tp>(Bu(i + 1 - ep - (g_mutation == MUT_ADD_BU_SHIFT && i > 1 ? 1 : 0), j + 1)) +
+                                           cvt<Tp>(LLu1(i, j + 1)) + cvt<Tp>(LLu2(i + 1 - ep, j)));
         m = vv.m - 4;
         n = vv.n - 4;
         CHECK(m + 2 == Bv.m && n + 2 == Bv.n && m == LLv1.m && n + 2 == LLv1.n && m + 2 == LLv2.m && n == LLv2.n);
@@ -951,9 +977,10 @@ template <class T, class Tp, class Tc> struct Model {
                 rhs(a, b, e, t);
                 continuity(a, b, e, t);
                 if (rki < c.RKo) {
-                    caxb(u1, u, Tp(c.RKbDt[rki - 1]), du);
-                    caxb(v1, v, Tp(c.RKbDt[rki - 1]), dv);
-                    caxb(eta1, eta, Tp(c.RKbDt[rki - 1]), deta);
+                    const bool m6 = g_mutation == MUT_RK_U1_FROM_U1;
+                    caxb(u1, m6 ? u1 : u, Tp(c.RKbDt[rki - 1]), du);
+                    caxb(v1, m6 ? v1 : v, Tp(c.RKbDt[rki - 1]), dv);
+                    caxb(eta1, m6 ? eta1 : eta, Tp(c.RKbDt[rki - 1]), deta);
                 }
                 axb(u0, Tp(c.RKaDt[rki - 1]), du);
                 axb(v0, Tp(c.RKaDt[rki - 1]), dv);
@@ -980,7 +1007,8 @@ template <class T, class Tp, class Tc> struct Model {
                 ++stages_run;
                 if (stop() && rki < c.RKs) return false;
             }
-            const double a = 1.0 / c.RKs, b = (double)(c.RKs - 1) / c.RKs;
+            double a = 1.0 / c.RKs, b = (double)(c.RKs - 1) / c.RKs;
+            if (g_mutation == MUT_SSPRK2_WEIGHTS) std::swap(a, b);
             cxayb(u0, a, u, b, u1);
             cxayb(v0, a, v, b, v1);
             cxayb(eta0, a, eta, b, eta1);
@@ -990,16 +1018,18 @@ template <class T, class Tp, class Tc> struct Model {
                 if (rki > 2) ghost_points_eta(eta1);
                 const Arr<T> &e = rhs_view(eta1, etar);
                 rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
+                const bool m8 = g_mutation == MUT_SSPRK3_KN;
+                const double kna = m8 ? c.ssp_knb : c.ssp_kna, knb = m8 ? c.ssp_kna : c.ssp_knb;
                 if (rki == c.ssp_kn) {
-                    dxaybzc(u1, c.ssp_kna, u1, c.ssp_knb, u0, to_f64(dtnc), du);
-                    dxaybzc(v1, c.ssp_kna, v1, c.ssp_knb, v0, to_f64(dtnc), dv);
+                    dxaybzc(u1, kna, u1, knb, u0, to_f64(dtnc), du);
+                    dxaybzc(v1, kna, v1, knb, v0, to_f64(dtnc), dv);
                 } else {
                     axb(u1, dtn, du);
                     axb(v1, dtn, dv);
                 }
                 ghost_points_uv(u1, v1);
                 continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
-                if (rki == c.ssp_kn) dxaybzc(eta1, c.ssp_kna, eta1, c.ssp_knb, eta0, to_f64(dtnc), deta);
+                if (rki == c.ssp_kn) dxaybzc(eta1, kna, eta1, knb, eta0, to_f64(dtnc), deta);
                 else axb(eta1, dtn, deta);
                 if (rki == c.ssp_mn || rki == c.ssp_s + 1) {
                     copyto(u0, u1);
@@ -1025,9 +1055,11 @@ template <class T, class Tp, class Tc> struct Model {
                 caxb(eta0, eta1, dt, deta);
                 cxab(eta1, 0.5, eta1, eta0);
                 if (rki == 3) {
-                    cxayb(u1, 2.0 / 3.0, u, 1.0 / 3.0, u1);
-                    cxayb(v1, 2.0 / 3.0, v, 1.0 / 3.0, v1);
-                    cxayb(eta1, 2.0 / 3.0, eta, 1.0 / 3.0, eta1);
+                    const bool m9 = g_mutation == MUT_4SSPRK3_THIRDS;
+                    const double x = m9 ? 1.0 / 3.0 : 2.0 / 3.0, y = m9 ? 2.0 / 3.0 : 1.0 / 3.0;
+                    cxayb(u1, x, u, y, u1);
+                    cxayb(v1, x, v, y, v1);
+                    cxayb(eta1, x, eta, y, eta1);
                 } else if (rki == 4) {
                     copyto(u0, u1);
                     copyto(v0, v1);
@@ -1125,6 +1157,97 @@ static int run_typed(const swm_config *c, const OracleStatic *s, void *u, void *
             if (rk_out[k]) store_arr(*arrs[k], rk_out[k]);
     }
     return 0;
+}
+
+// One operator of the step on a state as given (Tprog storage, Julia layout, in place):
+//   op 0  ghost_points! of (u, v, η) and of sst (what uploading a state applies,
+//         time_integration.jl:48 and ghost_points.jl:136-199)
+//   op 1  bottom_drag!, diffusion!, add_drag_diff_tendencies!, ghost_points_uv! of
+//         (u, v, η) as u0, v0, η0 (time_integration.jl:216-221); aux[0..8] (nullable)
+//         receive Bu, Bv, LLu1, LLu2, LLv1, LLv2, DT, DS, νSmag as float64
+//   op 2  tracer! at step i with (u, v) as the post-dissipation velocities
+//         (tracer_advection.jl:1-44); aux[0], aux[1] = um, vm (values of format T in
+//         float64, in place)
+template <class T, class Tp, class Tc>
+static int op_typed(const swm_config *c, const OracleStatic *s, int op, void *u, void *v, void *eta, void *sst, int64_t i,
+                    double **aux) {
+    Model<T, Tp, Tc> M(*c);
+    load_arr(M.H, s->H);
+    load_arr(M.Fx, s->Fx);
+    load_arr(M.Fy, s->Fy);
+    load_arr(M.f_q, s->f_q);
+    load_arr(M.u0, u);
+    load_arr(M.v0, v);
+    load_arr(M.eta0, eta);
+    load_arr(M.sst, sst);
+    const double sc = M.sc, sci = M.sci;
+    for (auto *A : {&M.u0, &M.v0})
+        for (auto &x : A->a) x = x * Tp(sc);
+    auto get_aux = [&](Arr<T> &A, const double *p) {
+        for (size_t k = 0; k < A.size(); ++k) A.a[k] = cvt<T>(p[k]) * T(sc);
+    };
+    auto put_aux = [&](const Arr<T> &A, double *p, double f) {
+        if (p) for (size_t k = 0; k < A.size(); ++k) p[k] = to_f64(A.a[k]) * f;
+    };
+    if (op == 0) {
+        M.ghost_points(M.u0, M.v0, M.eta0);
+        M.ghost_sst_any(M.sst);
+    } else if (op == 1) {
+        const Arr<T> &a0 = M.rhs_view(M.u0, M.ur), &b0 = M.rhs_view(M.v0, M.vr), &e0 = M.rhs_view(M.eta0, M.etar);
+        M.bottom_drag(a0, b0, e0);
+        M.diffusion(a0, b0);
+        M.add_drag_diff_tendencies(M.u0, M.v0);
+        M.ghost_points_uv(M.u0, M.v0);
+        if (aux) {
+            const Arr<T> *A[9] = {&M.Bu, &M.Bv, &M.LLu1, &M.LLu2, &M.LLv1, &M.LLv2, &M.DT, &M.DS, &M.nuSmag};
+            for (int k = 0; k < 9; ++k) put_aux(*A[k], aux[k], k < 6 ? sci : 1.0);
+        }
+    } else if (op == 2) {
+        if (!aux || !aux[0] || !aux[1]) return -1;
+        get_aux(M.um, aux[0]);
+        get_aux(M.vm, aux[1]);
+        M.tracer(i, M.rhs_view(M.u0, M.ur), M.rhs_view(M.v0, M.vr));
+        put_aux(M.um, aux[0], sci);
+        put_aux(M.vm, aux[1], sci);
+    } else {
+        return -1;
+    }
+    for (auto *A : {&M.u0, &M.v0})
+        for (auto &x : A->a) x = x * Tp(sci);
+    store_arr(M.u0, u);
+    store_arr(M.v0, v);
+    store_arr(M.eta0, eta);
+    store_arr(M.sst, sst);
+    return 0;
+}
+
+// calls fn(T(), Tp(), Tc()) with the number types the config names
+template <class Fn> static int with_types(const swm_config *c, Fn &&fn) {
+    auto comm = [&](auto t, auto tp) -> int {
+        switch (c->Tcomm) {
+        case SWM_F64: return fn(t, tp, double());
+        case SWM_F32: return fn(t, tp, float());
+        case SWM_F16: return fn(t, tp, f16());
+        case SWM_BF16: return fn(t, tp, bf16());
+        }
+        return -1;
+    };
+    auto prog = [&](auto t) -> int {
+        switch (c->Tprog) {
+        case SWM_F64: return comm(t, double());
+        case SWM_F32: return comm(t, float());
+        case SWM_F16: return comm(t, f16());
+        case SWM_BF16: return comm(t, bf16());
+        }
+        return -1;
+    };
+    switch (c->T) {
+    case SWM_F64: return prog(double());
+    case SWM_F32: return prog(float());
+    case SWM_F16: return prog(f16());
+    case SWM_BF16: return prog(bf16());
+    }
+    return -1;
 }
 
 template <class T, class Tp>
@@ -1377,6 +1500,27 @@ int swmo_stencil(int op, const double *in, int m_in, int n_in, double *out) {
         return -1;
     }
     return 0;
+}
+
+// One operator of the step (see op_typed) in the config's number formats.  Returns 0, or
+// -1 on a shape/option error or an index the reference would throw on.
+int swmo_op(const swm_config *c, const void *H, const void *Fx, const void *Fy, const void *f_q, int32_t op, void *u,
+            void *v, void *eta, void *sst, int64_t i, double **aux) {
+    OracleStatic s{H, Fx, Fy, f_q, nullptr, nullptr, nullptr, nullptr};
+    try {
+        return with_types(c, [&](auto t, auto tp, auto tc) {
+            return op_typed<decltype(t), decltype(tp), decltype(tc)>(c, &s, op, u, v, eta, sst, i, aux);
+        });
+    } catch (ShapeError &) {
+        return -1;
+    }
+}
+
+// Plant mistake k (the MUT_* list) in the restatement, 0 to remove it; returns the previous.
+int swmo_set_mutation(int32_t k) {
+    const int prev = g_mutation;
+    g_mutation = k;
+    return prev;
 }
 
 // Format conversions of the oracle (Julia's T(x)), for checking the emulated formats.
