@@ -27,9 +27,7 @@ NX = 4096
 
 
 def workload_parameter(swm, T="Float32", nx=NX, Tprog=None, **kw):
-    return swm.Parameter(T=T, Tprog=Tprog or T, Tcomm=Tprog or T, nx=nx, Lx=3840e3, L_ratio=1, bc="nonperiodic", α=0.0, topography="seamount",
-                         wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky", bottom_drag="quadratic",
-                         tracer_advection=True, Ndays=10, **kw)
+    return swm.workloads.double_gyre(nx=nx, T=T, Tprog=Tprog, **kw)
 
 
 def bytes_per_cell_stage(es: int, est: int = None) -> dict:
@@ -89,14 +87,8 @@ def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
     steps of the tuned ensemble and, as the unbatched comparison, `seq_steps` steps of each
     member on its own, one after the other."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([128, 256, 512, 1024], size=nmem)
-    setups = []
-    for k, n in enumerate(sizes):
-        P = swm.Parameter(T="Float32", nx=int(n), L_ratio=1, Fx0=float(rng.uniform(0.08, 0.16)),
-                          H=float(rng.uniform(400, 600)))
-        S = swm.make_setup(P)
-        setups.append((S, swm.seeded_initial_conditions(S, seed=1000 + k, σ=0.05)))
+    setups = swm.workloads.ensemble_members(nmem, seed)
+    sizes = np.array([S.grid.nx for S, _ in setups])
     cells = sum(S.grid.nx * S.grid.ny for S, _ in setups)
     sims = [swm.Simulation(S) for S, _ in setups]
     try:
@@ -258,10 +250,7 @@ def cpu_protocol(swm, orc, config2_cpu_steps: int = 100):
     for T in ("Float32", "Float64"):
         S = swm.make_setup(workload_parameter(swm, T=T))
         row(f"double_gyre_4096x4096 {T}", S, swm.seeded_initial_conditions(S, seed=2024, σ=0.1), 200, 5)
-    P1 = swm.Parameter(T="Float32", nx=100, L_ratio=1, Lx=2000e3, H=500, g=10, bc="nonperiodic", α=0.0,
-                       topography="seamount", wind_forcing_x="double_gyre", Fx0=0.12, diffusion="Smagorinsky",
-                       bottom_drag="quadratic", tracer_advection=False, Ndays=100, cfl=1.0)
-    S1 = swm.make_setup(P1)
+    S1 = swm.make_setup(swm.workloads.readme_double_gyre())
     row("readme_double_gyre_100x100 100 days", S1, swm.initial_conditions("Float32", S1), S1.grid.nt, S1.grid.nt)
     S2 = swm.make_setup(channel_2048(swm))
     row("channel_2048x1024 periodic", S2, swm.seeded_initial_conditions(S2, seed=2024, σ=0.1), 1000,
@@ -270,8 +259,7 @@ def cpu_protocol(swm, orc, config2_cpu_steps: int = 100):
 
 def channel_2048(swm, T="Float32", topography="flat"):
     """The 2048x1024 periodic channel (flat bottom, channel wind, tracer every 300 steps)."""
-    return swm.Parameter(T=T, nx=2048, L_ratio=2, bc="periodic", topography=topography, wind_forcing_x="channel",
-                         Fx0=0.12)
+    return swm.workloads.channel(T, topography)
 
 
 def channel_run(swm, T, steps, warmup, kernel_reps, topography="flat"):

@@ -406,6 +406,7 @@ struct swm_ens {
         std::vector<int> tiles, qblocks;
         std::vector<char> blob;        // what the device buffer holds
         std::vector<int> next_pu, next_pe, next_uv;   // member slots after the step
+        std::vector<int> init;         // members whose kept q, KEu+KEv the step forms from their new state first
     };
     struct Buf {
         Plan plan;

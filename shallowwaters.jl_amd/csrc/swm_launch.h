@@ -1101,13 +1101,17 @@ template <class F> struct Ens {
     using Q = typename F::Q;
     using ST = typename F::ST;
 
-    // record step i of every member (time (i-1)*dtint of the member) into a packed plan
+    // Record step i of every member (time (i-1)*dtint of the member) into a packed plan.
+    // Planning changes no member: a member whose kept q and KEu+KEv are still to be formed
+    // from its state (advcor_pending) gets that update as the plan's first item and is listed
+    // in pl.init; the flag is cleared when the plan is launched (launched()).
     static int plan(swm_ens *E, int64_t i, swm_ens::Plan &pl) {
         const size_t nm = E->m.size();
         std::vector<Recorder> recs(nm);
         pl = swm_ens::Plan();
         for (size_t k = 0; k < nm; ++k) {
             swm_ctx *ctx = E->m[k];
+            const bool pending = ctx->advcor_pending;
             ctx->rec = &recs[k];
             Slots u0, u1;
             int rc = Launch<F>::stages(ctx, 1 << 20, (i - 1) * ctx->c.dtint, &u0, &u1);
@@ -1118,7 +1122,9 @@ template <class F> struct Ens {
             const int out = pick({ctx->pu, u0.uv});
             if (!rc) rc = Launch<F>::dissipation(ctx, u0, out);
             ctx->rec = nullptr;
+            ctx->advcor_pending = pending;                      // stages() cleared it while recording
             if (rc) return E->fail(rc, "member %zu: %s", k, ctx->err.c_str());
+            if (pending) pl.init.push_back((int)k);
             pl.next_pu.push_back(out);
             pl.next_pe.push_back(u0.e);
             pl.next_uv.push_back(out);
@@ -1176,10 +1182,18 @@ template <class F> struct Ens {
         return SWM_OK;
     }
 
-    // make buffer b hold plan pl (upload only when it differs)
+    // the members' kept fields formed by a launched plan are no longer pending
+    static void launched(swm_ens *E, const swm_ens::Plan &pl) {
+        for (int k : pl.init) E->m[k]->advcor_pending = false;
+    }
+
+    // Make buffer b hold plan pl (upload only when it differs).  The captured graph holds the
+    // launch sizes of both buffers' plans and the per-member items' arguments by value, so an
+    // upload invalidates it.
     static int load(swm_ens *E, int b, const swm_ens::Plan &pl) {
         swm_ens::Buf &B = E->buf[b];
         if (B.valid && B.plan.blob == pl.blob) return SWM_OK;
+        E->graph_ok = false;
         if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
         if (pl.blob.size() > B.cap) {
             if (B.dev) (void)hipFree(B.dev);
@@ -1342,15 +1356,19 @@ template <class F> struct Ens {
                 p0.blob == E->buf[0].plan.blob) {
                 std::vector<int> pu(nm), pe(nm);
                 for (size_t q = 0; q < nm; ++q) { pu[q] = E->m[q]->pu; pe[q] = E->m[q]->pe; }
-                advance(E, p0);
+                advance(E, p0);                                  // step i+1 planned as after step i
+                launched(E, p0);
                 rc = plan(E, i + 1, p1);
                 for (size_t q = 0; q < nm; ++q) { E->m[q]->pu = pu[q]; E->m[q]->pe = pe[q]; }
+                for (int q : p0.init) E->m[q]->advcor_pending = true;
                 if (rc) return rc;
                 if (p1.blob == E->buf[1].plan.blob) {
                     if (!E->graph_ok && (rc = capture(E))) return rc;
                     if (hipGraphLaunch(E->exec, E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "graph launch");
                     advance(E, p0);
                     advance(E, p1);
+                    launched(E, p0);
+                    launched(E, p1);
                     k += 2;
                     *gsteps += 2;
                     *done = k;
@@ -1361,9 +1379,9 @@ template <class F> struct Ens {
             int b = E->buf[0].valid && E->buf[0].plan.blob == p0.blob ? 0
                   : E->buf[1].valid && E->buf[1].plan.blob == p0.blob ? 1 : 1 - E->last;
             if ((rc = load(E, b, p0))) return rc;
-            if (b != E->last) { /* alternate */ }
             E->last = b;
             if ((rc = launch_buf(E, b))) return rc;
+            launched(E, p0);
             // per-member host work, in the member's own order (time_integration.jl:226-237)
             bool any_nan = false;
             for (size_t q = 0; q < nm; ++q) {
@@ -1396,10 +1414,18 @@ template <class F> struct Ens {
         return SWM_OK;
     }
 
-    // time `reps` batched steps (plan of the current state, into buffer 0) with HIP events
+    // Time `reps` batched steps (plan of the current state, into buffer 0) with HIP events.
+    // The plan leaves out pending updates of the kept fields from the initial state, so that
+    // every candidate of the autotuner times the same kernel sequence.
     static int time_step(swm_ens *E, int reps, double *ms) {
         swm_ens::Plan pl;
+        std::vector<char> pending;
+        for (swm_ctx *ctx : E->m) {
+            pending.push_back(ctx->advcor_pending);
+            ctx->advcor_pending = false;
+        }
         int rc = plan(E, 1, pl);
+        for (size_t k = 0; k < E->m.size(); ++k) E->m[k]->advcor_pending = pending[k] != 0;
         if (rc) return rc;
         E->buf[0].valid = false;
         if ((rc = load(E, 0, pl))) return rc;

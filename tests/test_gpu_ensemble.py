@@ -1,5 +1,5 @@
 """Ensembles (swm_ens_*): members of different grid sizes advanced by batched launches and
-graph replays equal their own single-context runs bit for bit."""
+graph replays equal their own single-context runs and the oracle bit for bit."""
 import numpy as np
 import pytest
 
@@ -35,6 +35,14 @@ def run_alone(swm, S, prog, n):
         return sim.download()
 
 
+def assert_oracle(swm, orc, S, prog, n, got, tag):
+    """The member's state after n steps from prog equals the oracle's, bitwise."""
+    r = orc.run(S, swm.make_config(S), prog, n)
+    for name, x, y in zip(("u", "v", "η", "sst"), r[:4], (got.u, got.v, got.η, got.sst)):
+        mm = first_mismatch(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
+        assert mm is None, f"{tag} {name} against the oracle: {mm}"
+
+
 def assert_equal(S, a, b, tag):
     T = S.parameters.Tprog
     import shallowwaters_amd as _  # noqa: F401  (package import for to_float64)
@@ -46,7 +54,7 @@ def assert_equal(S, a, b, tag):
 
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat", "periodic-mixed-seams"])
 @pytest.mark.parametrize("events", [False, True])
-def test_ensemble_matches_single_runs(swm, gpu, bc, events):
+def test_ensemble_matches_single_runs(swm, orc, gpu, bc, events):
     """Without tracer / NaN-check steps the ensemble runs step pairs as graph replays; with
     them (tracer every few steps, NaN check every step) it runs them eagerly in between.
     Periodic members on a flat bottom run fused stage pairs wrapping around the seam; with
@@ -66,8 +74,10 @@ def test_ensemble_matches_single_runs(swm, gpu, bc, events):
         with swm.Ensemble(sims) as ens:
             done, nan, graph = ens.step(n, 1)
             assert done == n and not any(nan)
-        for sim, (S, _), r in zip(sims, ms, ref):
-            assert_equal(S, r, sim.download(), f"{bc} nx={S.grid.nx}")
+        for sim, (S, p), r in zip(sims, ms, ref):
+            got = sim.download()
+            assert_equal(S, r, got, f"{bc} nx={S.grid.nx}")
+            assert_oracle(swm, orc, S, p, n, got, f"{bc} nx={S.grid.nx}")
         if not events:      # the first steps settle the slot roles; then step pairs replay the graph
             assert graph >= n - 4, graph
     finally:
@@ -131,7 +141,7 @@ def test_ensemble_rejects_different_nstep_advcor(swm, gpu):
 @pytest.mark.parametrize("kw", [dict(adv_scheme="Sadourny"), dict(dynamics="linear"), dict(nstep_advcor=2),
                                 dict(adv_scheme="Sadourny", nstep_advcor=3)])
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat"])
-def test_ensemble_of_other_right_hand_sides(swm, gpu, kw, bc):
+def test_ensemble_of_other_right_hand_sides(swm, orc, gpu, kw, bc):
     """Members with Sadourny's scheme, the linear dynamics or q and KEu+KEv kept between
     nstep_advcor updates equal their own single-context runs bit for bit: fused batched
     pairs (nonperiodic, and periodic on a flat bottom, whose seams the lanes wrap around),
@@ -159,8 +169,106 @@ def test_ensemble_of_other_right_hand_sides(swm, gpu, kw, bc):
             ens.tune(reps=2)                 # between steps: the timed steps leave every field as it was
             done, nan, _ = ens.step(n - 3, 4)
             assert done == n - 3 and not any(nan)
+        for sim, (S, p), r in zip(sims, ms, ref):
+            got = sim.download()
+            assert_equal(S, r, got, f"{kw} {bc} nx={S.grid.nx}")
+            assert_oracle(swm, orc, S, p, n, got, f"{kw} {bc} nx={S.grid.nx}")
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+def kept_field_members(swm, bc, nstep_advcor, k=4):
+    """Members keeping q and KEu+KEv between updates, with no host work between steps (no
+    tracer, no NaN check within the run), so that step pairs replay the captured graph."""
+    rng = np.random.default_rng(8)
+    ms = []
+    for j, (nx, ratio) in enumerate(SIZES[:k]):
+        P = swm.Parameter(T="Float32", nx=nx, L_ratio=ratio, bc=bc.split("-")[0], Fx0=float(rng.uniform(0.08, 0.16)),
+                          topography="flat" if bc == "periodic-flat" else "seamount", wind_forcing_x="double_gyre",
+                          tracer_advection=False, output_dt=1000.0, nstep_advcor=nstep_advcor)
+        S = swm.make_setup(P)
+        ms.append((S, swm.seeded_initial_conditions(S, seed=300 + j, σ=0.6)))
+    return ms
+
+
+@pytest.mark.parametrize("nstep_advcor", [1, 2])
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic-flat"])
+def test_graph_replays_of_kept_field_updates(swm, orc, gpu, bc, nstep_advcor):
+    """The kept fields' updates (per member, inside the step's plan) replayed in captured
+    graphs: bitwise equal to single runs and the oracle, most steps as graph replays."""
+    n = 12
+    ms = kept_field_members(swm, bc, nstep_advcor)
+    ref = [run_alone(swm, S, p, n) for S, p in ms]
+    sims = [swm.Simulation(S) for S, _ in ms]
+    try:
+        for sim, (S, p) in zip(sims, ms):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            done, nan, graph = ens.step(n, 1)
+        assert done == n and not any(nan)
+        assert graph >= n - 4, graph
+        for sim, (S, p), r in zip(sims, ms, ref):
+            got = sim.download()
+            assert_equal(S, r, got, f"{bc} N={nstep_advcor} nx={S.grid.nx}")
+            assert_oracle(swm, orc, S, p, n, got, f"{bc} N={nstep_advcor} nx={S.grid.nx}")
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+@pytest.mark.parametrize("nstep_advcor", [1, 2])
+def test_member_stepped_alone_between_ensemble_calls(swm, gpu, nstep_advcor):
+    """A member stepped on its own between two ensemble calls changes its slot roles, so the
+    next call plans other kernels' arguments than the graph captured: the graph must not be
+    replayed stale (regression: it was)."""
+    ms = kept_field_members(swm, "nonperiodic", nstep_advcor)
+    dt = [S.grid.dtint for S, _ in ms]
+    ref = []
+    for k, (S, p) in enumerate(ms):                       # the same sequence of steps, one member at a time
+        with swm.Simulation(S) as sim:
+            sim.upload(p)
+            assert sim.step(6, 1, 0) == (6, 0)
+            if k == 0:
+                assert sim.step(1, 7, 6 * dt[k]) == (1, 0)
+            assert sim.step(8, 8, 7 * dt[k]) == (8, 0)
+            ref.append(sim.download())
+    sims = [swm.Simulation(S) for S, _ in ms]
+    try:
+        for sim, (S, p) in zip(sims, ms):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            assert ens.step(6, 1)[0] == 6
+            assert sims[0].step(1, 7, 6 * dt[0]) == (1, 0)
+            done, nan, graph = ens.step(8, 8)
+            assert done == 8 and not any(nan)
         for sim, (S, _), r in zip(sims, ms, ref):
-            assert_equal(S, r, sim.download(), f"{kw} {bc} nx={S.grid.nx}")
+            assert_equal(S, r, sim.download(), f"N={nstep_advcor} nx={S.grid.nx}")
+    finally:
+        for sim in sims:
+            sim.close()
+
+
+def test_refused_ensemble_step_keeps_the_pending_update(swm, gpu):
+    """A member given a new state between ensemble calls still has to form its kept q and
+    KEu+KEv from it; the others do not, so the ensemble refuses the step.  The refusal must
+    leave that update pending: the member run on its own then equals a fresh run from the
+    new state (regression: planning had cleared it)."""
+    ms = kept_field_members(swm, "nonperiodic", 2)
+    S0 = ms[0][0]
+    fresh = swm.seeded_initial_conditions(S0, seed=77, σ=0.6)
+    ref = run_alone(swm, S0, fresh, 5)
+    sims = [swm.Simulation(S) for S, _ in ms]
+    try:
+        for sim, (S, p) in zip(sims, ms):
+            sim.upload(p)
+        with swm.Ensemble(sims) as ens:
+            assert ens.step(2, 1)[0] == 2
+            sims[0].upload(fresh)
+            with pytest.raises(Exception, match="kernel sequence"):
+                ens.step(2, 3)
+        assert sims[0].step(5, 1, 0) == (5, 0)
+        assert_equal(S0, ref, sims[0].download(), "re-uploaded member")
     finally:
         for sim in sims:
             sim.close()
