@@ -148,6 +148,15 @@ int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void
  * at its right face (x = 1..nx) and v at its top face (y = 1..ny-1). */
 int swm_reduce(swm_ctx *ctx, double *mass, double *energy);
 
+/* swm_reduce without allocating: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (serving loops, code under HIP graph capture, harnesses that forbid
+ * hipMalloc).  The same mass and energy, bitwise, as swm_reduce; the row partials and the
+ * compensated sums live in a static device array the library holds for the process (one copy
+ * per GPU, 512 KiB).  At most 32768 rows (ny); larger grids get SWM_E_ARG.  Calls on one GPU
+ * share that array: calls that overlap in time on one GPU (several threads, several
+ * contexts' streams) race and must be serialised by the caller. */
+int swm_reduce_noalloc(swm_ctx *ctx, double *mass, double *energy);
+
 /* Measurement hook (no counterpart in the reference): launch kernel `which` `reps` times
  * back to back on the context's stream between two HIP events and return the average
  * duration in milliseconds.  which: 0 middle RK stage (stage 2), 1 first stage, 2 last

@@ -14,6 +14,8 @@
 // nothing else (time_integration.jl:57-64), so two extra slots of T storage replace the
 // Tprog u1 without changing a bit of the result.
 
+#include <mutex>
+
 #include "swm_host.h"
 
 namespace {
@@ -53,6 +55,50 @@ int check_config(swm_ctx *ctx) {
 }
 
 // the members' work goes to the ensemble's stream while it runs
+// ---- the static scratch of swm_reduce_noalloc
+//
+// The row partials (mass, energy) of up to NOALLOC_MAX_ROWS rows and, after them, the two
+// compensated sums, in device memory the library holds for the whole process (one copy on
+// each GPU it runs on, 512 KiB).  Its device address is looked up once per device, with that
+// device current, when the first context is created there; host code passes that address,
+// never the array's name, to kernels and copies.  Every call on a device shares its copy.
+constexpr int NOALLOC_MAX_ROWS = 32768;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_ROWS + 2];
+double *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// scratch's address on the current device ctx->dev (set up once per device)
+int scratch_setup(swm_ctx *ctx) {
+    if (ctx->dev < 0 || ctx->dev >= MAX_DEVICES) return ctx->fail(SWM_E_HIP, "device %d out of range", ctx->dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[ctx->dev]) return SWM_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return ctx->fail(SWM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->dev, hipGetErrorString(e));
+    g_scratch[ctx->dev] = (double *)p;
+    return SWM_OK;
+}
+
+// The rows' partials summed in row order with Neumaier's compensation, by one lane: the
+// same operations in the same order as swm_reduce's host loop, so the same doubles.
+__global__ void reduce_rows_kernel(const double *part, int ny, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double m = 0.0, e = 0.0, cm = 0.0, ce = 0.0;
+    for (int y = 0; y < ny; ++y) {
+        const double a = part[2 * y], b = part[2 * y + 1];
+        double t = m + a;
+        cm += fabs(m) >= fabs(a) ? (m - t) + a : (a - t) + m;
+        m = t;
+        t = e + b;
+        ce += fabs(e) >= fabs(b) ? (e - t) + b : (b - t) + e;
+        e = t;
+    }
+    out[0] = m + cm;
+    out[1] = e + ce;
+}
+
 // the adjoint's number formats: T = Tprog = Tcomm in {Float64, Float32}
 int adj_ok(swm_ctx *ctx) {
     if ((ctx->fmt == SWM_F64 && ctx->c.Tprog == SWM_F64) || (ctx->fmt == SWM_F32 && ctx->c.Tprog == SWM_F32)) return SWM_OK;
@@ -119,6 +165,11 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
     if (hipGetDevice(&ctx->dev) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         fprintf(stderr, "swm_create: no HIP device/stream\n");
         delete ctx;
+        return SWM_E_HIP;
+    }
+    if (scratch_setup(ctx) != SWM_OK) {
+        fprintf(stderr, "swm_create: %s\n", ctx->err.c_str());
+        swm_destroy(ctx);
         return SWM_E_HIP;
     }
     if (c.bc == SWM_PERIODIC &&
@@ -411,7 +462,7 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
     const swm_config &c = ctx->c;
     const int pu = ctx->pu, pe = ctx->pe;
-    int rc = fmt_ops(ctx->fmt).reduce_launch(ctx, pu, pe);
+    int rc = fmt_ops(ctx->fmt).reduce_launch(ctx, pu, pe, ctx->dpart);
     if (rc) return rc;
     std::vector<double> part(2 * (size_t)c.ny);
     HIPCHK(hipMemcpyAsync(part.data(), ctx->dpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -428,6 +479,28 @@ int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     }
     *mass = m + cm;
     *energy = e + ce;
+    return SWM_OK;
+}
+
+int swm_reduce_noalloc(swm_ctx *ctx, double *mass, double *energy) {
+    if (!ctx || !mass || !energy) return SWM_E_ARG;
+    if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
+    const swm_config &c = ctx->c;
+    if (c.ny > NOALLOC_MAX_ROWS)
+        return ctx->fail(SWM_E_ARG, "ny = %d exceeds the %d rows the static scratch of swm_reduce_noalloc holds; use swm_reduce",
+                         c.ny, NOALLOC_MAX_ROWS);
+    double *s = (ctx->dev >= 0 && ctx->dev < MAX_DEVICES) ? g_scratch[ctx->dev] : nullptr;
+    if (!s) return ctx->fail(SWM_E_STATE, "no scratch address for device %d", ctx->dev);
+    int rc = fmt_ops(ctx->fmt).reduce_launch(ctx, ctx->pu, ctx->pe, s);
+    if (rc) return rc;
+    double *sums = s + 2 * NOALLOC_MAX_ROWS;
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(64), 0, ctx->stream, (const double *)s, c.ny, sums);
+    HIPCHK(hipGetLastError());
+    double out[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(out, sums, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *mass = out[0];
+    *energy = out[1];
     return SWM_OK;
 }
 

@@ -1526,11 +1526,11 @@ template <class F, bool ADJ> struct Entry {
         return Launch<F>::ghosts(ctx, pu, pe, ctx->rsst);
     }
     static int stages(swm_ctx *ctx, int32_t n, int64_t t0) { return Launch<F>::stages(ctx, n, t0, &ctx->rk0, &ctx->rk1, true); }
-    static int reduce_launch(swm_ctx *ctx, int pu, int pe) {
+    static int reduce_launch(swm_ctx *ctx, int pu, int pe, double *part) {
         const swm_config &c = ctx->c;
         reduce_kernel<Q, ST><<<c.ny, 256, 0, ctx->stream>>>(ctx->at<Q>(ctx->u[pu]), ctx->at<Q>(ctx->v[pu]), ctx->at<Q>(ctx->e[pe]),
                                                             ctx->at<ST>(ctx->H), c.nx, c.ny, ctx->P, ctx->ep, c.g, ctx->sci,
-                                                            ctx->dpart);
+                                                            part);
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
