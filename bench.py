@@ -4,11 +4,15 @@ Workload: the nonperiodic double gyre at 4096x4096 (Lx = Ly = 3840 km, seamount,
 wind Fx0 = 0.12 Pa, free slip, Smagorinsky biharmonic diffusion, quadratic drag, tracer on,
 RK4, Float32, dt = 13 s), started from a seeded smoothed η (σ = 0.1 m) with geostrophic
 u, v.  A step is one full pass of the time loop (4 RK stages, ghost points, dissipation,
-tracer when due) over the whole grid; the state stays resident in HBM.
+tracer when due) over the whole grid; the state stays resident in HBM.  The timed steps are
+placed so that they contain a tracer advection (every 288 steps at this size).
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the middle RK stage), timed
-with HIP events on the library's stream; `cpu_baseline` times the CPU restatement (oracle,
-one thread) on a bounded sample of the same workload.
+Prints one JSON line.  `roofline` is for the kernel with the largest share of the step (a
+fused pair of RK stages), timed with HIP events on the library's stream: `achieved` is the
+bytes that kernel must move (each field it reads once, each output written once) over its
+average duration; `step_roofline` gives the whole step, also against the per-stage
+synchronisation bytes of the reference's loop.  `cpu_baseline` times the CPU restatement
+(oracle, one thread, the -O3 vectorised build) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -58,18 +62,26 @@ STEP_KERNELS = {"unfused": ("stage1", "stage_mid", "stage_mid", "stage4", "dissi
                 "fused": ("fused12", "fused34", "dissipation")}
 
 
+def timed_window(G, steps, warmup):
+    """First warm-up step and first timed step: the timed steps contain the tracer's
+    advection step (i = nadvstep) when the run has a tracer."""
+    i1 = max(1 + warmup, G.nadvstep - steps // 2) if G.nadvstep > 0 else 1 + warmup
+    return i1 - warmup, i1
+
+
 def run_form(swm, T, Tprog, nx, steps, warmup, kernel_reps, **kw):
     """Time `steps` steps of the workload with right-hand side format T and state format
     Tprog (state resident in HBM), then the four step kernels with HIP events."""
     S = swm.make_setup(workload_parameter(swm, T=T, nx=nx, Tprog=Tprog, **kw))
     G = S.grid
     prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+    i0, i1 = timed_window(G, steps, warmup)
     with swm.Simulation(S) as sim:
         sim.upload(prog)
-        done, nan = sim.step(warmup, 1, 0)
+        done, nan = sim.step(warmup, i0, (i0 - 1) * G.dtint)
         assert nan == 0
         t0 = time.perf_counter()
-        done, nan = sim.step(steps, 1 + warmup, warmup * G.dtint)   # returns after a stream sync
+        done, nan = sim.step(steps, i1, (i1 - 1) * G.dtint)   # returns after a stream sync
         elapsed = time.perf_counter() - t0
         assert (done, nan) == (steps, 0), (done, nan)
         kms = {name: sim.time_kernel(k, kernel_reps)
@@ -308,18 +320,29 @@ def scale_probe(swm, nx):
 
 
 def cpu_baseline(swm, orc, sample_steps: int, nx: int):
+    """The oracle on one host thread, both builds (oracle/Makefile): the -O3 vectorised one
+    (x86-64-v3) is the reported baseline, the -O2 one a labelled second row.  Same
+    arithmetic (no FMA contraction in either), so the same results."""
     P = workload_parameter(swm, nx=nx)
     S = swm.make_setup(P)
     prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
     cfg = swm.make_config(S)
-    orc.run(S, cfg, prog, 1)                     # warm-up
-    t = time.perf_counter()
-    orc.run(S, cfg, prog, sample_steps)
-    dt = time.perf_counter() - t
-    return {"value": S.grid.nx * S.grid.ny * sample_steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_steps} steps of the same workload at {S.grid.nx}x{S.grid.ny}, oracle/swm_oracle.cpp "
-                      f"(g++ -O2, one thread) on {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
-            "seconds": dt}
+    host = f"{platform.processor() or platform.machine()}, nproc={os.cpu_count()}"
+    rows = {}
+    for variant, flags in (("o3", "g++ -O3 -march=x86-64-v3"), ("o2", "g++ -O2")):
+        orc.use(variant)
+        orc.run(S, cfg, prog, 1)                     # warm-up
+        t = time.perf_counter()
+        orc.run(S, cfg, prog, sample_steps)
+        dt = time.perf_counter() - t
+        rows[variant] = {"value": S.grid.nx * S.grid.ny * sample_steps / dt, "unit": "cell-updates/s", "cores": 1,
+                         "kind": "port", "seconds": dt,
+                         "sample": f"{sample_steps} steps of the same workload at {S.grid.nx}x{S.grid.ny}, "
+                                   f"oracle/swm_oracle.cpp ({flags}, -ffp-contract=off, one thread) on {host}"}
+    orc.use("o2")
+    out = dict(rows["o3"])
+    out["o2_build"] = rows["o2"]
+    return out
 
 
 def measured_traffic(workload: str, dtype: str, kernel: str):
@@ -420,7 +443,7 @@ def main():
         channel_run(swm, args.dtype, args.steps, args.warmup, args.kernel_reps, args.topography)
         return
     es = 4 if args.dtype == "Float32" else 8
-    # dominant kernel: middle RK stage, HIP events on the library stream
+    # kernel times with HIP events on the library stream
     S, elapsed, kms, mass, energy = run_form(swm, args.dtype, args.dtype, args.nx, args.steps, args.warmup,
                                              args.kernel_reps)
     G = S.grid
@@ -433,7 +456,7 @@ def main():
     step = STEP_KERNELS["fused" if "fused12" in kms else "unfused"]
     dom = max(set(step), key=lambda k: kms[k] * step.count(k))          # largest share of the step
     traffic = measured_traffic(f"double_gyre_{G.nx}x{G.ny}", "f32" if es == 4 else "f64", dom)
-    achieved = alg[dom] * cells / (kms[dom] * 1e-3) / 1e9
+    achieved = moved[dom] * cells / (kms[dom] * 1e-3) / 1e9
     step_bytes = sum(moved[k] for k in step) * cells
     ref_bytes = sum(bpc[k] for k in STEP_KERNELS["unfused"]) * cells   # per-stage sync points (232 B at FP32)
     out = {
@@ -448,15 +471,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if es == 4 else "f64",
-        "data": "synthetic: seeded smoothed η (σ=0.1 m, seed 2024) with geostrophic u,v; analytic sst",
+        "data": "synthetic: seeded smoothed η (σ=0.1 m, seed 2024) with geostrophic u,v; seeded sinusoid plus uniform noise sst",
         "config": {"workload": f"double_gyre_{G.nx}x{G.ny}", "nx": G.nx, "ny": G.ny, "Lx_m": G.Lx, "bc": "nonperiodic",
                    "topography": "seamount", "wind": "double_gyre Fx0=0.12", "diffusion": "Smagorinsky",
                    "bottom_drag": "quadratic", "alpha": 0.0, "time_scheme": "RK4", "dt_s": G.dtint,
-                   "tracer_every": G.nadvstep, "parallelism": "single GPU"},
+                   "tracer_every": G.nadvstep, "timed_steps_from": timed_window(G, args.steps, args.warmup)[1],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                     "bytes_per_cell": alg[dom], "min_bytes_moved_per_cell": moved[dom],
-                     "min_moved_GBs": moved[dom] * cells / (kms[dom] * 1e-3) / 1e9, "kernel_ms": kms[dom]},
+                     "bytes_per_cell": moved[dom], "kernel_ms": kms[dom],
+                     "traffic_GBs": traffic / (kms[dom] * 1e-3) / 1e9 if traffic else None,
+                     "stages_sync_bytes_per_cell": alg[dom],
+                     "stages_sync_equivalent_GBs": alg[dom] * cells / (kms[dom] * 1e-3) / 1e9},
         "step_roofline": {"kernels": list(step), "alg_bytes_per_cell_step": step_bytes / cells,
                           "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
                           "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,

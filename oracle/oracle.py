@@ -12,24 +12,36 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libswm_oracle.so")
-_lib = None
+# the two builds of the same source (oracle/Makefile): "o2" checks the tests, "o3" (vectorised)
+# is the CPU baseline; both evaluate every operation in the same order with no FMA contraction
+LIB_PATHS = {"o2": os.path.join(HERE, "libswm_oracle.so"), "o3": os.path.join(HERE, "libswm_oracle_o3.so")}
+LIB_PATH = LIB_PATHS["o2"]
+_libs = {}
+_variant = "o2"
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (strict IEEE flags)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
+    """Compile the oracle with its Makefile (strict IEEE flags), both builds."""
+    if force or not all(os.path.exists(p) for p in LIB_PATHS.values()):
+        subprocess.run(["make", "-C", HERE, "-j2", "-B" if force else "all"], check=True,
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     return LIB_PATH
 
 
+def use(variant: str) -> None:
+    """Select the build later calls go to ("o2", the default, or "o3")."""
+    global _variant
+    if variant not in LIB_PATHS:
+        raise ValueError(variant)
+    _variant = variant
+
+
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library {LIB_PATH} is not built (run `make -C oracle`)")
-        L = ctypes.CDLL(LIB_PATH)
+    if _variant not in _libs:
+        path = LIB_PATHS[_variant]
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library {path} is not built (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
         vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
         L.swmo_run.argtypes = [vp] * 13 + [i64, i64, i64, i32, ctypes.POINTER(vp),
                                            ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -37,8 +49,8 @@ def lib():
         L.swmo_stencil.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
         L.swmo_round.argtypes = [ctypes.c_int, vp, vp, i64]
         L.swmo_arith.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, i64]
-        _lib = L
-    return _lib
+        _libs[_variant] = L
+    return _libs[_variant]
 
 
 def _ptr(a):

@@ -1221,7 +1221,9 @@ static int op_typed(const swm_config *c, const OracleStatic *s, int op, void *u,
     return 0;
 }
 
-// calls fn(T(), Tp(), Tc()) with the number types the config names
+// Calls fn(T(), Tp(), Tc()) with the number types the config names: T = Tprog in {Float64,
+// Float32} with any Tcomm, or T in {Float16, BFloat16} over Tprog = Float32 with Tcomm in
+// {T, Float32, Float64} (the formats the library runs); -1 for any other combination.
 template <class Fn> static int with_types(const swm_config *c, Fn &&fn) {
     auto comm = [&](auto t, auto tp) -> int {
         switch (c->Tcomm) {
@@ -1232,45 +1234,17 @@ template <class Fn> static int with_types(const swm_config *c, Fn &&fn) {
         }
         return -1;
     };
-    auto prog = [&](auto t) -> int {
-        switch (c->Tprog) {
-        case SWM_F64: return comm(t, double());
-        case SWM_F32: return comm(t, float());
-        case SWM_F16: return comm(t, f16());
-        case SWM_BF16: return comm(t, bf16());
-        }
+    auto low = [&](auto t) -> int {                 // T narrower than Tprog = Float32
+        using TT = decltype(t);
+        if (c->Tcomm == c->T) return fn(t, float(), TT());
+        if (c->Tcomm == SWM_F32) return fn(t, float(), float());
+        if (c->Tcomm == SWM_F64) return fn(t, float(), double());
         return -1;
     };
-    switch (c->T) {
-    case SWM_F64: return prog(double());
-    case SWM_F32: return prog(float());
-    case SWM_F16: return prog(f16());
-    case SWM_BF16: return prog(bf16());
-    }
-    return -1;
-}
-
-template <class T, class Tp>
-static int dispatch_comm(const swm_config *c, const OracleStatic *s, void *u, void *v, void *eta, void *sst,
-                         int64_t n, int64_t i0, int64_t t0, int32_t ss, void **rk, int64_t *sd, int64_t *ns) {
-    switch (c->Tcomm) {
-    case SWM_F64: return run_typed<T, Tp, double>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_F32: return run_typed<T, Tp, float>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_F16: return run_typed<T, Tp, f16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_BF16: return run_typed<T, Tp, bf16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    }
-    return -1;
-}
-
-template <class T>
-static int dispatch_prog(const swm_config *c, const OracleStatic *s, void *u, void *v, void *eta, void *sst,
-                         int64_t n, int64_t i0, int64_t t0, int32_t ss, void **rk, int64_t *sd, int64_t *ns) {
-    switch (c->Tprog) {
-    case SWM_F64: return dispatch_comm<T, double>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_F32: return dispatch_comm<T, float>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_F16: return dispatch_comm<T, f16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    case SWM_BF16: return dispatch_comm<T, bf16>(c, s, u, v, eta, sst, n, i0, t0, ss, rk, sd, ns);
-    }
+    if (c->T == c->Tprog && c->T == SWM_F64) return comm(double(), double());
+    if (c->T == c->Tprog && c->T == SWM_F32) return comm(float(), float());
+    if (c->Tprog == SWM_F32 && c->T == SWM_F16) return low(f16());
+    if (c->Tprog == SWM_F32 && c->T == SWM_BF16) return low(bf16());
     return -1;
 }
 
@@ -1312,16 +1286,13 @@ int swmo_run(const swm_config *c, const void *H, const void *Fx, const void *Fy,
              int64_t *nan_step) {
     OracleStatic s{H, Fx, Fy, f_q, eta_ref, Feta, f_u, f_v};
     try {
-        switch (c->T) {
-        case SWM_F64: return dispatch_prog<double>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
-        case SWM_F32: return dispatch_prog<float>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
-        case SWM_F16: return dispatch_prog<f16>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
-        case SWM_BF16: return dispatch_prog<bf16>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage, rk_out, steps_done, nan_step);
-        }
+        return with_types(c, [&](auto t, auto tp, auto tc) {
+            return run_typed<decltype(t), decltype(tp), decltype(tc)>(c, &s, u, v, eta, sst, nsteps, i0, t0, stop_stage,
+                                                                      rk_out, steps_done, nan_step);
+        });
     } catch (ShapeError &) {
         return -1;
     }
-    return -1;
 }
 
 // tracer! alone (tracer_advection.jl:1-44) at step i for T = Tprog = double: u, v are the

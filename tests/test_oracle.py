@@ -476,3 +476,25 @@ def test_nstep_advcor_schedule(swm, orc):
         a = run(2, nstep_advcor=n, bottom_drag="quadratic", cD=0.0)
         b = run(2, nstep_advcor=n, bottom_drag="none")
         assert same(a, b) != differ, n
+
+
+# ------------------------------------------------------------------ the two oracle builds
+
+@pytest.mark.parametrize("T,Tprog", [("Float32", "Float32"), ("Float64", "Float64"), ("BFloat16", "Float32")])
+def test_vectorised_build_gives_the_same_bits(swm, orc, T, Tprog):
+    """The -O3 vectorised build (the CPU baseline beside the GPU) evaluates the same
+    operations in the same order, without FMA contraction: its results equal the -O2
+    build's bit for bit."""
+    P = swm.Parameter(T=T, Tprog=Tprog, nx=36, L_ratio=1.2, bc="nonperiodic", topography="seamount",
+                      wind_forcing_x="double_gyre", diffusion="Smagorinsky", Uadv=20.0)
+    S = swm.make_setup(P)
+    prog = swm.seeded_initial_conditions(S, seed=4, σ=0.5)
+    runs = []
+    try:
+        for variant in ("o2", "o3"):
+            orc.use(variant)
+            runs.append(orc.run(S, swm.make_config(S), prog, 8)[:4])
+    finally:
+        orc.use("o2")
+    for name, a, b in zip(("u", "v", "η", "sst"), *runs):
+        assert np.array_equal(a, b), name
