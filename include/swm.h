@@ -142,6 +142,22 @@ int swm_step(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0,
 int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0);
 int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void *v1, void *eta1);
 
+/* swm_ghost: ghost_points! of (u, v, eta) and ghost_points_sst! of sst, on the current Prog
+ * in place (ghost_points.jl:136-199; what time_integration.jl:48 applies each step). */
+int swm_ghost(swm_ctx *ctx);
+
+/* swm_dissipation: bottom_drag!, diffusion!, add_drag_diff_tendencies! and ghost_points_uv!
+ * (time_integration.jl:216-221; bottom_drag.jl:2-119, diffusion.jl:2-251) with the current
+ * Prog as u0, v0, eta0; the result becomes Prog (eta unchanged). */
+int swm_dissipation(swm_ctx *ctx);
+
+/* swm_tracer: tracer! at step i (tracer_advection.jl:1-44) with the current Prog velocities as
+ * the post-dissipation velocities (time_integration.jl:226-228): the mid-point snapshot
+ * um, vm <- u, v when (i + nadvstep_half) % nadvstep == 0, the semi-Lagrangian advection of
+ * sst when i % nadvstep == 0.  A departure point that is NaN or outside the grid returns
+ * SWM_E_ARG (the reference throws InexactError / BoundsError). */
+int swm_tracer(swm_ctx *ctx, int64_t i);
+
 /* FP64 reductions over the current Prog (no counterpart in the reference, which only sums
  * isnan, feedback.jl:68): mass = sum of eta over the nx*ny T-cells; energy = sum over
  * T-cells (x,y) of 0.5*g*eta^2 + 0.5*h*u^2 + 0.5*h*v^2 with h = eta + H of that cell, u

@@ -16,7 +16,7 @@ EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_
            "swm_set_static", "swm_set_static_linear", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
            "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
-           "swm_adj_time", "swm_reduce_noalloc"]
+           "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer"]
 
 _lib = None
 
@@ -47,6 +47,9 @@ def load():
     L.swm_step.argtypes = [vp, i64, i64, i64, pi64, pi64]
     L.swm_stages.argtypes = [vp, i32, i64, i64]
     L.swm_download_rk.argtypes = [vp] * 7
+    L.swm_ghost.argtypes = [vp]
+    L.swm_dissipation.argtypes = [vp]
+    L.swm_tracer.argtypes = [vp, i64]
     L.swm_reduce.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.swm_reduce_noalloc.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.swm_time_kernel.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]

@@ -428,6 +428,38 @@ int swm_stages(swm_ctx *ctx, int32_t n_stages, int64_t i0, int64_t t0) {
     return SWM_OK;
 }
 
+int swm_ghost(swm_ctx *ctx) {
+    if (!ctx) return SWM_E_ARG;
+    if (int rc = step_ready(ctx)) return rc;
+    if (int rc = fmt_ops(ctx->fmt).ghost(ctx)) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SWM_OK;
+}
+
+int swm_dissipation(swm_ctx *ctx) {
+    if (!ctx) return SWM_E_ARG;
+    if (int rc = step_ready(ctx)) return rc;
+    if (int rc = fmt_ops(ctx->fmt).dissipation(ctx)) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SWM_OK;
+}
+
+int swm_tracer(swm_ctx *ctx, int64_t i) {
+    if (!ctx) return SWM_E_ARG;
+    if (int rc = step_ready(ctx)) return rc;
+    if (i < 1) return ctx->fail(SWM_E_ARG, "step i >= 1 required");
+    if (int rc = fmt_ops(ctx->fmt).tracer(ctx, i)) return rc;
+    int flags[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(flags, ctx->dflags, sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (flags[1]) {
+        int z = 0;
+        HIPCHK(hipMemcpy(ctx->dflags + 1, &z, sizeof z, hipMemcpyHostToDevice));
+        return ctx->fail(SWM_E_ARG, "tracer departure point is NaN or outside the grid (InexactError/BoundsError in the reference)");
+    }
+    return SWM_OK;
+}
+
 int swm_download_rk(swm_ctx *ctx, void *u0, void *v0, void *eta0, void *u1, void *v1, void *eta1) {
     if (!ctx) return SWM_E_ARG;
     if (ctx->rk0.uv < 0 || ctx->rk1.uv < 0) return ctx->fail(SWM_E_STATE, "call swm_stages first");

@@ -444,6 +444,9 @@ struct FmtOps {
     int (*stages)(swm_ctx *ctx, int32_t n_stages, int64_t t0);
     int (*time_kernel)(swm_ctx *ctx, int32_t which, int32_t reps, double *ms);
     int (*reduce_launch)(swm_ctx *ctx, int pu, int pe, double *part);   // row partials (mass, energy) into part
+    int (*ghost)(swm_ctx *ctx);                              // ghost_points! of Prog and sst in place
+    int (*dissipation)(swm_ctx *ctx);                        // Prog as (u0, v0, η0) -> dissipated Prog
+    int (*tracer)(swm_ctx *ctx, int64_t i);                  // tracer! at step i with Prog's velocities
     int (*ens_tune)(swm_ens *E, int32_t reps, int *Ls, int *Lf, int *Ld);
     int (*ens_step)(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, int64_t *gsteps);
     int (*adj_forward)(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots);

@@ -1534,8 +1534,19 @@ template <class F, bool ADJ> struct Entry {
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
+    static int ghost(swm_ctx *ctx) { return Launch<F>::ghosts(ctx, ctx->pu, ctx->pe, ctx->rsst); }
+    static int dissipation(swm_ctx *ctx) {
+        const int out = pick({ctx->pu});
+        const int rc = Launch<F>::dissipation(ctx, Slots{ctx->pu, ctx->pe}, out);
+        if (!rc) ctx->pu = out;
+        return rc;
+    }
+    static int tracer(swm_ctx *ctx, int64_t i) { return Launch<F>::tracer(ctx, i, ctx->pu); }
     static FmtOps make() {
         FmtOps o{};
+        o.ghost = &ghost;
+        o.dissipation = &dissipation;
+        o.tracer = &tracer;
         o.segment_lengths = &segment_lengths;
         o.scale_static = &scale_static;
         o.upload_fix = &upload_fix;

@@ -100,6 +100,21 @@ class Simulation:
     def stages(self, n_stages: int, i0: int = 1, t0: int = 0):
         _abi.check(_abi.load().swm_stages(self.ctx, n_stages, i0, t0), self.ctx, "swm_stages")
 
+    # ---- single operators of the step on the current state (parity tests)
+    def ghost(self):
+        """``ghost_points!`` of u, v, η and sst in place (``src/ghost_points.jl:136-199``)."""
+        _abi.check(_abi.load().swm_ghost(self.ctx), self.ctx, "swm_ghost")
+
+    def dissipation(self):
+        """Drag, diffusion and their tendencies on the state as (u0, v0, η0); the result
+        becomes the state (``src/time_integration.jl:216-221``)."""
+        _abi.check(_abi.load().swm_dissipation(self.ctx), self.ctx, "swm_dissipation")
+
+    def tracer(self, i: int):
+        """``tracer!`` at step ``i`` with the state's velocities as the post-dissipation
+        ones (``src/tracer_advection.jl:1-44``)."""
+        _abi.check(_abi.load().swm_tracer(self.ctx, i), self.ctx, "swm_tracer")
+
     def download_rk(self):
         G, dt = self.S.grid, storage_dtype(self.S.parameters.Tprog)
         z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
