@@ -235,6 +235,14 @@ int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_
 int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const void *ybar_eta, const void *ybar_sst,
                      void *xbar_u, void *xbar_v, void *xbar_eta, void *xbar_sst, int64_t *forward_steps,
                      double *sweep_ms);
+/* swm_adj_track_Fx: with on != 0, the following pullbacks also accumulate the gradient of
+ * <ybar, y> with respect to the zonal wind stress Fx (Forcing.Fx, forcing.jl:15-25), which
+ * enters every stage's du as Tprog(Fxt*Fx) (rhs.jl:242): the sum over the reversed steps and
+ * stages of Fxt times the cotangent of du.  Off by default (one more field read and written
+ * per stage adjoint).  swm_adj_download_Fx: that gradient from the last pullback, format T
+ * (= Tprog), shape (nux, nuy) of Fx. */
+int swm_adj_track_Fx(swm_ctx *ctx, int32_t on);
+int swm_adj_download_Fx(swm_ctx *ctx, void *xbar_Fx);
 /* Measurement hook: average duration in ms of `reps` launches of: 0 a forward step, 1 a taped
  * forward step (unfused stages, intermediates kept), 2 the reverse of one step (all its
  * adjoint kernels), 3 one stage adjoint kernel, 4 the dissipation adjoint kernel.  The state

@@ -219,13 +219,14 @@ def _out_shapes(S, op):
     return {0: [su, sv, se, ss], 1: [su, sv, se], 2: [su, sv], 3: [su, sv, ss]}[op]
 
 
-def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0):
+def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0, dFx=None):
     """Tangent-linear model of the oracle (Float64 values with one directional derivative).
 
     ``op`` 0: ``nsteps`` steps of the loop from x = (u, v, η, sst) after the upload's ghost
     copies; 1: rhs + continuity of a classic RK stage at x = (u, v, η) -> (du, dv, dη);
     2: drag + diffusion + ghost_points_uv! of x = (u0, v0, η0) -> (u0, v0);
     3: tracer! at step i0 of x = (u, v, um, vm, sst) -> (um, vm, sst).
+    ``dFx``: a tangent of the wind stress Fx (shape (nux, nuy)), for any op.
     Returns (y, dy) as lists of float64 Julia-layout arrays."""
     x = [np.asfortranarray(a, dtype=np.float64) for a in x]
     dx = None if dx is None else [np.asfortranarray(a, dtype=np.float64) for a in dx]
@@ -234,12 +235,13 @@ def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0):
     dy = [np.zeros(s, order="F") for s in shapes]
     vp = ctypes.c_void_p
     L = lib()
-    L.swmo_tlm.argtypes = [vp] * 7 + [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [vp] * 4
+    L.swmo_tlm.argtypes = [vp] * 7 + [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [vp] * 5
     arr = lambda lst: (vp * len(lst))(*[a.ctypes.data if a is not None else None for a in lst])
     xs, ys, dys = arr(x), arr(y), arr(dy)
     dxs = arr(dx) if dx is not None else None
     st = _statics64(S)
-    rc = L.swmo_tlm(ctypes.byref(cfg), *[_ptr(a) for a in st], op, nsteps, i0, t0, xs, dxs, ys, dys)
+    dfx = np.asfortranarray(dFx, dtype=np.float64) if dFx is not None else None
+    rc = L.swmo_tlm(ctypes.byref(cfg), *[_ptr(a) for a in st], op, nsteps, i0, t0, xs, dxs, ys, dys, _ptr(dfx))
     if rc != 0:
         raise OracleError("tangent-linear run failed")
     return y, dy

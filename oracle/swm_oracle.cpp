@@ -1350,6 +1350,7 @@ int swmo_pv_advection(const swm_config *c, const double *q, const double *U, con
 //   op 2  bottom_drag!, diffusion!, add_drag_diff_tendencies! and ghost_points_uv!
 //         (time_integration.jl:216-221) of (u0, v0, η0) as given; x = (u0, v0, η0), y = (u0, v0)
 //   op 3  tracer! at step i0 (tracer_advection.jl:1-44); x = (u, v, um, vm, sst), y = (um, vm, sst)
+// dFx (nullable): a tangent of the zonal wind stress Fx (nux, nuy), for any op.
 static void load_dual(Arr<Dual> &A, const double *v, const double *d) {
     for (size_t k = 0; k < A.size(); ++k) A.a[k] = Dual(v[k], d ? d[k] : 0.0);
 }
@@ -1362,14 +1363,14 @@ static void store_dual(const Arr<Dual> &A, double *v, double *d) {
 
 int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
              const double *eta_ref, const double *Feta, int32_t op, int64_t nsteps, int64_t i0, int64_t t0,
-             const double *const *x, const double *const *dx, double *const *y, double *const *dy) {
+             const double *const *x, const double *const *dx, double *const *y, double *const *dy, const double *dFx) {
     if (c->scale_log2 != 0 || c->compensated) return -1;   // the tangent-linear model is of the plain step
     auto dxk = [&](int k) { return dx ? dx[k] : nullptr; };
     auto dyk = [&](int k) { return dy ? dy[k] : nullptr; };
     try {
         Model<Dual, Dual, Dual> M(*c);
         load_dual(M.H, H, nullptr);
-        load_dual(M.Fx, Fx, nullptr);
+        load_dual(M.Fx, Fx, dFx);
         load_dual(M.Fy, Fy, nullptr);
         load_dual(M.f_q, f_q, nullptr);
         if (eta_ref) load_dual(M.eta_ref, eta_ref, nullptr);

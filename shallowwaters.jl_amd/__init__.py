@@ -16,11 +16,11 @@ from .model_setup import ModelSetup, make_setup, make_config, swm_config
 from .initial_conditions import (PrognosticVars, initial_conditions, add_halo, remove_halo,
                                  seeded_initial_conditions, zero_prognostic)
 from .time_integration import (time_integration, run_model, preallocate, DiagnosticVars, Simulation, Ensemble,
-                               rrule, NoTangent)
+                               rrule, NoTangent, Tangent)
 from . import workloads
 
 __all__ = ["Parameter", "run_model", "time_integration", "preallocate", "DiagnosticVars",
            "PrognosticVars", "ModelSetup", "make_setup", "make_config", "initial_conditions",
            "add_halo", "remove_halo", "seeded_initial_conditions", "Simulation", "Ensemble", "Grid", "Constants",
            "Forcing", "canonical_type", "to_storage", "to_float64", "swm_config", "zero_prognostic",
-           "make_grid", "make_constants", "make_forcing", "workloads"]
+           "make_grid", "make_constants", "make_forcing", "workloads", "rrule", "NoTangent", "Tangent"]

@@ -16,7 +16,8 @@ EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_
            "swm_set_static", "swm_set_static_linear", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
            "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
-           "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer"]
+           "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer",
+           "swm_adj_track_Fx", "swm_adj_download_Fx"]
 
 _lib = None
 
@@ -62,6 +63,8 @@ def load():
     L.swm_adj_forward.argtypes = [vp, i64, i64, i64, i32]
     L.swm_adj_pullback.argtypes = [vp] * 9 + [pi64, ctypes.POINTER(ctypes.c_double)]
     L.swm_adj_time.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]
+    L.swm_adj_track_Fx.argtypes = [vp, i32]
+    L.swm_adj_download_Fx.argtypes = [vp, vp]
     for name in EXPORTS:
         if name not in ("swm_last_error", "swm_ens_last_error"):
             getattr(L, name).restype = ctypes.c_int

@@ -95,9 +95,12 @@ template <class R> struct StageAdjArgs {
     UVE a_in;              // A_{k+1} = λ0 + Σ_{j>k} λs_j on interior cells
     UVE mu_out, a_out;     // μ_k, A_k (or, for the first stage, λ of the step's input: A_1 + μ_1)
     uint32_t lqend;        // periodic: λ of q(nx, y), 1-D
+    uint32_t lfx;          // fxg: λ of the zonal wind stress Fx at the u points, accumulated
     R a, b;                // RKaΔt[k], RKbΔt[k] (b of the stage whose input is s_{k+1})
     R g, gamma, k12, oma;
+    R fxt;                 // the step's factor of Fx in du (Ftime, forcing.jl:215-217; 1 unless seasonal)
     int cont, nx, ny, P, L, nstrips, nseg;
+    int fxg;               // accumulate λFx += fxt·λdu (du = ... + Fxt·Fx, rhs.jl:242)
 };
 
 template <class R> struct DissAdjArgs {
@@ -175,6 +178,7 @@ template <class V> struct StageLoads {
     V u, v, e, H, fq, qe;      // stage input at row r; f_q and q(nx) at corner row r-1
     V lu, lv, le, mu, mv, me;  // cells of λ0 and μ_{k+1} at row r-1 (the folds add their halo)
     V au, av, ae;              // running sum A at the row emitted, r-2
+    V fx;                      // λFx at the row emitted, r-2 (fxg)
 };
 
 template <class W, bool PER, bool NEXT, bool A_L0>
@@ -192,10 +196,11 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
     p.mu = p.mv = p.me = zero;
     if (NEXT) { p.mu = w.ld(A.mu_in.u, yu); p.mv = w.ld(A.mu_in.v, yv); p.me = w.ld(A.mu_in.e, yu); }
     p.au = p.av = p.ae = zero;
+    const int ye = amin(amax(rs - 2, ya), yb - 1);
     if (NEXT && !A_L0) {
-        const int ye = amin(amax(rs - 2, ya), yb - 1);
         p.au = w.ld(A.a_in.u, ye); p.av = w.ld(A.a_in.v, ye); p.ae = w.ld(A.a_in.e, ye);
     }
+    p.fx = A.fxg ? w.ld(A.lfx, ye) : zero;
     return p;
 }
 
@@ -359,6 +364,8 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             }
             // periodic: λ of q(nx, y) from qβ(0, y) and qγ(0, y+1)
             if (PER && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
+            // the wind stress enters du as Tprog(Fxt·Fx) (rhs.jl:242): λFx += Fxt·λdu, u cells
+            if (A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
         }
 
         // ---- advance the rows
@@ -885,6 +892,7 @@ struct AdjFields {
     int fq_2d;
     uint32_t qend, lqend;
     uint32_t tr0, tr1;
+    uint32_t lfx;                   // λ of Fx at the u points (accumulated over the reversed steps)
 };
 
 // Scalars of the configuration the reverse step needs (all in the arithmetic type R).
@@ -895,6 +903,8 @@ template <class R> struct AdjConsts {
     R hdu, hdv, du, dv, jSST;
     int RKo, cont, drag, smag, per, consume, tracer, nadvstep, nadvstep_half, nstep_diff;
     int nx, ny, P, ep;
+    R fxt;                          // the step's factor of Fx in du (1 unless the wind is seasonal)
+    int fxg;                        // accumulate the cotangent of Fx
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -943,6 +953,7 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
     S.l0 = l0;
     S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
+    S.lfx = F.lfx; S.fxt = c.fxt; S.fxg = c.fxg;
     UVE mu_prev = F.Ma, mu_next = F.Mb;
     for (int st = c.RKo; st >= 1; --st) {
         const bool next = st < c.RKo, al0 = st == c.RKo - 1, pre = st == 1;

@@ -640,6 +640,19 @@ int swm_adj_pullback(swm_ctx *ctx, const void *ybar_u, const void *ybar_v, const
     return rc;
 }
 
+int swm_adj_track_Fx(swm_ctx *ctx, int32_t on) {
+    if (!ctx) return SWM_E_ARG;
+    if (int rc = adj_ok(ctx)) return rc;
+    ctx->adj_fx = on != 0;
+    return SWM_OK;
+}
+
+int swm_adj_download_Fx(swm_ctx *ctx, void *xbar_Fx) {
+    if (!ctx || !xbar_Fx) return SWM_E_ARG;
+    if (int rc = adj_ok(ctx)) return rc;
+    return fmt_ops(ctx->fmt).adj_download_fx(ctx, xbar_Fx);
+}
+
 int swm_adj_time(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     if (!ctx || !ms_avg) return SWM_E_ARG;
     if (reps < 1) return ctx->fail(SWM_E_ARG, "reps must be >= 1");

@@ -151,6 +151,7 @@ struct swm_ctx {
     int nslots = 4;                // Tprog slot sets in use (6 once the adjoint is set up)
     int ens_refs = 0;              // ensembles holding this context (their plans point into the arena)
     struct AdjState *adj = nullptr;
+    bool adj_fx = false;           // pullbacks accumulate the cotangent of Fx (swm_adj_track_Fx)
     std::vector<void *> allocs;
     void *arena = nullptr;         // every field of the context; kernels address it through one buffer descriptor
     size_t arena_bytes = 0, arena_used = 0;
@@ -316,6 +317,7 @@ struct AdjState {
     int nsnap = 0;
     bool recorded = false;
     int64_t fwd_steps = 0;         // forward steps run by the last pullback (taped steps included)
+    bool fx_valid = false;         // the last pullback accumulated the cotangent of Fx
     double sweep_ms = 0.0;         // device time of the last reverse sweep (HIP events)
     void release() {
         for (char *p : snap) if (p) (void)hipFree(p);
@@ -453,6 +455,7 @@ struct FmtOps {
     int (*adj_pullback)(swm_ctx *ctx, const void *yu, const void *yv, const void *ye, const void *ys, void *gu, void *gv,
                         void *ge, void *gs);
     int (*adj_time)(swm_ctx *ctx, int32_t which, int32_t reps, double *ms);
+    int (*adj_download_fx)(swm_ctx *ctx, void *xbar_Fx);
 };
 const FmtOps &ops_f64();
 const FmtOps &ops_f32();

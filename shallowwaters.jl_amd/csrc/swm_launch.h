@@ -689,7 +689,7 @@ template <class Fm> struct Adj {
         if (ctx->adj) return SWM_OK;
         int rc = check(ctx);
         if (rc) return rc;
-        const int nf2 = 6 + 21;
+        const int nf2 = 6 + 22;
         const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
         if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
         AdjState *st = new AdjState();
@@ -710,6 +710,7 @@ template <class Fm> struct Adj {
         uve(F.L); uve(F.Z); uve(F.Ma); uve(F.Mb); uve(F.A);
         one(F.Lsst, true); one(F.Lsst2, true); one(F.Lum, true); one(F.Lvm, true);
         one(F.tr0, true); one(F.tr1, true);
+        one(F.lfx, true);
         one(F.lqend, false);
         if (rc) {
             delete st;
@@ -732,9 +733,12 @@ template <class Fm> struct Adj {
         return SWM_OK;
     }
 
-    static adj::AdjConsts<R> consts(const swm_ctx *ctx) {
+    // the constants of the reverse of a step at model time t
+    static adj::AdjConsts<R> consts(const swm_ctx *ctx, int64_t t) {
         const swm_config &c = ctx->c;
         adj::AdjConsts<R> k{};
+        k.fxt = (R)(c.seasonal_wind_x ? round_host(std::sin(c.omegaFx * (double)t), c.T) : 1.0);
+        k.fxg = ctx->adj_fx ? 1 : 0;
         for (int j = 0; j < 4; ++j) {
             k.a[j] = (R)c.RKaDt[j];
             k.b[j] = j < 3 ? (R)c.RKbDt[j] : R(0);
@@ -914,7 +918,7 @@ template <class Fm> struct Adj {
         int rc = tape_step(ctx, k, st->F);
         if (rc) return rc;
         K kl{ctx, st};
-        return adj::reverse_step<R>(kl, st->F, consts(ctx), (long)(st->i0 + k));
+        return adj::reverse_step<R>(kl, st->F, consts(ctx, st->t0 + k * ctx->c.dtint), (long)(st->i0 + k));
     }
 
     // reverse steps [a, b) with the state at a in snapshot slot sa and s free snapshots
@@ -993,6 +997,7 @@ template <class Fm> struct Adj {
         int rc;
         st->F = st->F0;
         st->fwd_steps = 0;
+        st->fx_valid = false;
         if ((rc = zero_adjoint(ctx))) return rc;
         Field tmp;
         auto put = [&](uint32_t o, const void *h, int m, int nn, int x0, int y0) {
@@ -1040,6 +1045,22 @@ template <class Fm> struct Adj {
         // leave the context at the final state, as after swm_adj_forward
         if ((rc = copy_state(ctx, st->nsnap + 1, false))) return rc;
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        st->fx_valid = ctx->adj_fx;
+        return SWM_OK;
+    }
+
+    // the cotangent of Fx (nux, nuy) accumulated by the last pullback
+    static int download_fx(swm_ctx *ctx, void *out) {
+        AdjState *st = ctx->adj;
+        if (!st || !st->fx_valid) return ctx->fail(SWM_E_STATE, "no pullback with swm_adj_track_Fx on");
+        Field f;
+        f.origin = st->F0.lfx;
+        f.base = (char *)ctx->arena + f.origin;
+        f.es = ctx->es;
+        f.p0 = (char *)f.base + ((size_t)OX + (size_t)OY * ctx->P) * ctx->es;
+        int rc = get2d(ctx, f, out, ctx->nux, ctx->nuy, 1 - ctx->ep, 1);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         return SWM_OK;
     }
 
@@ -1056,7 +1077,7 @@ template <class Fm> struct Adj {
         adj::AdjFields Ft = st->F0;
         if ((rc = tape_step(ctx, 0, Ft))) return rc;
         K kl{ctx, st};
-        const adj::AdjConsts<R> k = consts(ctx);
+        const adj::AdjConsts<R> k = consts(ctx, 0);
         adj::StageAdjArgs<R> S{};
         S.s = Ft.s[1]; S.H = Ft.H; S.fq = Ft.fq; S.fq_2d = Ft.fq_2d; S.qend = Ft.qend; S.lqend = Ft.lqend;
         S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.a_in = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
@@ -1560,6 +1581,7 @@ template <class F, bool ADJ> struct Entry {
             o.adj_forward = &Adj<F>::forward;
             o.adj_pullback = &Adj<F>::pullback;
             o.adj_time = &Adj<F>::time;
+            o.adj_download_fx = &Adj<F>::download_fx;
         }
         return o;
     }

@@ -160,6 +160,19 @@ class Simulation:
         self.sweep_ms = ms.value
         return out, nf.value
 
+    def adjoint_track_Fx(self, on: bool = True):
+        """Let the following pullbacks also accumulate the gradient with respect to the
+        zonal wind stress Fx (``swm_adj_track_Fx``)."""
+        _abi.check(_abi.load().swm_adj_track_Fx(self.ctx, int(bool(on))), self.ctx, "swm_adj_track_Fx")
+
+    def pullback_Fx(self):
+        """The gradient with respect to Fx (shape (nux, nuy) of ``Forcing.Fx``, format T) from
+        the last pullback."""
+        G = self.S.grid
+        out = np.zeros((G.nux, G.nuy), dtype=storage_dtype(self.S.parameters.T), order="F")
+        _abi.check(_abi.load().swm_adj_download_Fx(self.ctx, _abi.ptr(out)), self.ctx, "swm_adj_download_Fx")
+        return out
+
     def adjoint_time(self, which: int, reps: int) -> float:
         """Average duration (ms): 0 forward step, 1 taped step, 2 reverse step, 3 stage
         adjoint kernel, 4 dissipation adjoint kernel."""
@@ -281,18 +294,32 @@ class NoTangent:
         return "NoTangent()"
 
 
-def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshots: int = 20):
+class Tangent:
+    """ChainRulesCore.Tangent{P}(; fields...): the derivative with respect to a struct, one
+    entry per differentiated field (the others have none)."""
+
+    def __init__(self, **fields):
+        self.__dict__.update(fields)
+
+    def __repr__(self):
+        return "Tangent(" + ", ".join(f"{k}=..." for k in self.__dict__) + ")"
+
+
+def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshots: int = 20, wrt_Fx: bool = False):
     """``ChainRulesCore.rrule(::typeof(time_integration), Prog, Diag, S)``: runs
     ``time_integration`` on the GPU keeping ``snapshots`` device checkpoints (revolve) and
     returns ``(y, pullback)``; ``pullback(ȳ)`` takes the cotangent of the returned interior
     fields (a PrognosticVars without halo) and returns ``(NoTangent(), Proḡ, NoTangent(),
-    NoTangent())`` with Proḡ the gradient with respect to ``Prog`` (halo cells zero: the
-    upload's ghost copies make the halo a function of the interior)."""
+    S̄)`` with Proḡ the gradient with respect to ``Prog`` (halo cells zero: the
+    upload's ghost copies make the halo a function of the interior).  S̄ is ``NoTangent()``,
+    or with ``wrt_Fx`` the ``Tangent(forcing=Tangent(Fx=...))`` of the zonal wind stress."""
     if f is not time_integration:
         raise TypeError("rrule is defined for time_integration only")
     G, T = S.grid, S.parameters.Tprog
     sim = Simulation(S)
     sim.upload(Prog)
+    if wrt_Fx:
+        sim.adjoint_track_Fx(True)
     sim.adjoint_forward(G.nt, 1, 0, snapshots)
     Y = sim.download()
     u, v, η, sst = remove_halo(Y.u, Y.v, Y.η, Y.sst, S)
@@ -308,7 +335,8 @@ def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshot
         yb.η[1:-1, 1:-1] = ybar.η
         yb.sst[1:-1, :] = ybar.sst
         xbar, _ = sim.pullback(yb)
-        return NoTangent(), xbar, NoTangent(), NoTangent()
+        sbar = Tangent(forcing=Tangent(Fx=sim.pullback_Fx())) if wrt_Fx else NoTangent()
+        return NoTangent(), xbar, NoTangent(), sbar
 
     pullback.simulation = sim
     return y, pullback

@@ -52,3 +52,51 @@ def ensemble_members(nmem: int = 64, seed: int = 4, T="Float32"):
         S = make_setup(P)
         out.append((S, seeded_initial_conditions(S, seed=1000 + k, σ=0.05)))
     return out
+
+
+# ------------------------------------------------------------------ the adjoint workload's loss
+
+def observation_twin(S, prog, nsteps: int, seed: int = 17, fraction: float = 0.01, σ: float = 0.02):
+    """Synthetic observations for the adjoint workload: a twin run of ``nsteps`` steps from
+    ``prog`` with η perturbed by seeded noise of σ metres, observed in η at a seeded
+    ``fraction`` of the interior cells.  Returns (η of the twin's final state, mask), both
+    of the halo-padded η shape."""
+    from .numtypes import to_float64, to_storage
+    from .initial_conditions import PrognosticVars
+    from .time_integration import Simulation
+    G, T = S.grid, S.parameters.Tprog
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((G.nx + 2, G.ny + 2), dtype=bool)
+    cells = rng.choice(G.nx * G.ny, size=max(1, int(round(fraction * G.nx * G.ny))), replace=False)
+    mask[1 + cells % G.nx, 1 + cells // G.nx] = True
+    e = to_float64(prog.η, T).copy(order="F")
+    e[1:-1, 1:-1] += σ * rng.standard_normal((G.nx, G.ny))
+    twin = PrognosticVars(u=prog.u, v=prog.v, η=to_storage(e, T), sst=prog.sst, T=T)
+    with Simulation(S) as sim:
+        sim.upload(twin)
+        done, nan = sim.step(nsteps, 1, 0)
+        if (done, nan) != (nsteps, 0):
+            raise RuntimeError(f"twin run stopped at step {done} (NaN at {nan})")
+        obs = to_float64(sim.download().η, T)
+    return obs, mask
+
+
+def final_state_loss(S, y, obs, mask):
+    """The adjoint workload's loss of a final state y (halo-padded PrognosticVars): the
+    kinetic energy ½Σu² + ½Σv² over the interior u and v points plus the η misfit
+    ½Σ(η - η_obs)² over the observed cells.  Returns (loss, ȳ) with ȳ its gradient with
+    respect to y (halo cells zero), in y's storage format."""
+    from .numtypes import to_float64, to_storage
+    from .initial_conditions import PrognosticVars
+    T = S.parameters.Tprog
+    u, v, e = (to_float64(a, T) for a in (y.u, y.v, y.η))
+    ui, vi = u[2:-2, 2:-2], v[2:-2, 2:-2]
+    r = np.where(mask, e - obs, 0.0)
+    L = 0.5 * float(np.sum(ui * ui)) + 0.5 * float(np.sum(vi * vi)) + 0.5 * float(np.sum(r * r))
+    gu = np.zeros_like(u, order="F")
+    gv = np.zeros_like(v, order="F")
+    gu[2:-2, 2:-2] = ui
+    gv[2:-2, 2:-2] = vi
+    ybar = PrognosticVars(u=to_storage(gu, T), v=to_storage(gv, T), η=to_storage(np.asfortranarray(r), T),
+                          sst=to_storage(np.zeros_like(to_float64(y.sst, T), order="F"), T), T=T)
+    return L, ybar

@@ -116,6 +116,36 @@ def test_tangent_linear_model_matches_central_differences(swm, orc, bc):
             assert err < 1e-8, f"direction {d} {name}: {err:.2e}"
 
 
+@pytest.mark.parametrize("bc,seasonal", [("nonperiodic", False), ("periodic", True)])
+def test_wind_stress_tangent_matches_central_differences(swm, orc, bc, seasonal):
+    """The tangent of the state with respect to the wind stress field Fx (rhs.jl:242, a
+    seasonal factor sin(ωt) when seasonal_wind_x) against central differences of the oracle
+    with Fx ± ε·δFx (8 seeded directions, two tracer periods)."""
+    P, S, cfg, prog = setup(swm, bc, seasonal_wind_x=seasonal, ωFx=300.0)
+    G = S.grid
+    n = 2 * G.nadvstep
+    x = list(ghosted_state(swm, S, prog))
+    rng = np.random.default_rng(12)
+    eps = 1e-6
+    Fx0 = S.forcing.Fx.copy()
+    for d in range(8):
+        dF = rng.standard_normal(Fx0.shape) * np.abs(Fx0).max()
+        _, dy = orc.tlm(S, cfg, 0, x, None, nsteps=n, dFx=dF)
+        fd = []
+        for sgn in (1, -1):
+            S2 = copy.copy(S)
+            S2.forcing = copy.copy(S.forcing)
+            S2.forcing.Fx = np.asfortranarray(Fx0 + sgn * eps * dF)
+            fd.append(orc.run(S2, cfg, prog, n)[:4])
+        for k, name in enumerate(("u", "v", "η", "sst")):
+            c = (fd[0][k] - fd[1][k]) / (2 * eps)
+            # the differences' rounding, ~1e-16·|y|/ε per cell, bounds what they can resolve:
+            # the tracer's tangent (Fx acts on it only through the departure points) is small
+            floor = 1e-16 * np.linalg.norm(fd[0][k]) / eps
+            diff, nrm = np.linalg.norm(c - dy[k]), np.linalg.norm(dy[k])
+            assert diff <= 1e-7 * nrm + 10 * floor, f"direction {d} {name}: |fd - tlm| {diff:.2e}, |tlm| {nrm:.2e}"
+
+
 def test_tangent_linear_values_are_the_oracle(swm, orc):
     P, S, cfg, prog = setup(swm, "nonperiodic")
     x = list(ghosted_state(swm, S, prog))

@@ -147,3 +147,81 @@ def test_adjoint_rejects_unsupported_configurations(swm, gpu):
         sim.upload(prog)
         with pytest.raises(RuntimeError, match="swm_adj_forward first"):
             sim.pullback(prog)
+
+
+@pytest.mark.parametrize("bc,seasonal", [("nonperiodic", False), ("periodic", True)])
+def test_wind_stress_gradient_dot_product_fp64(swm, orc, gpu, bc, seasonal):
+    """The gradient with respect to the wind stress field Fx (swm_adj_track_Fx): FP64
+    dot-product identity <J_Fx δFx, ȳ> = <δFx, ∇_Fx> with J_Fx from the oracle's
+    tangent-linear model, with a seasonal factor sin(ωt) on the wind in the periodic case."""
+    P, S, cfg, prog = setup(swm, bc, nx=80, seasonal_wind_x=seasonal, ωFx=300.0)
+    n = 6
+    ybar = cotangent(S, 3, np.float64)
+    print(f"wind stress dot product {bc} seasonal={seasonal} nx={S.grid.nx} n={n}", flush=True)
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        sim.adjoint_track_Fx(True)
+        sim.adjoint_forward(n, 1, 0, 2)
+        sim.pullback(swm.PrognosticVars(u=ybar[0], v=ybar[1], η=ybar[2], sst=ybar[3], T="Float64"))
+        gFx = sim.pullback_Fx()
+    x = list(ghosted_state(swm, S, prog))
+    rng = np.random.default_rng(9)
+    for m in range(3):
+        dF = rng.standard_normal(S.forcing.Fx.shape)
+        _, dy = orc.tlm(S, cfg, 0, x, None, nsteps=n, dFx=dF)
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(dy, ybar))
+        rhs = float(np.sum(dF * gFx))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dFx, ȳ> = {lhs!r}, <dFx, ∇Fx> = {rhs!r}"
+
+
+def test_adjoint_workload_gradients_by_central_differences(swm, gpu):
+    """The adjoint workload at its size: 1024x1024 double gyre, 200 steps with the tracer
+    advecting (Uadv = 20 m/s), revolve with 20 snapshots; the loss is the final kinetic energy
+    plus the η misfit at a seeded 1% of the cells against a perturbed twin.  Gradients with
+    respect to the initial η and the wind stress field Fx, in FP64, against central
+    differences of the GPU forward run on 16 seeded directions in (η0, Fx)."""
+    S = swm.make_setup(swm.workloads.double_gyre(1024, T="Float64", Uadv=20.0))
+    G = S.grid
+    n = 200
+    prog = swm.seeded_initial_conditions(S, seed=5, σ=0.1)
+    obs, mask = swm.workloads.observation_twin(S, prog, n)
+    print(f"adjoint workload {G.nx}x{G.ny} n={n}, {int(mask.sum())} observed cells", flush=True)
+    with swm.Simulation(S) as sim:
+        sim.upload(prog)
+        sim.adjoint_track_Fx(True)
+        sim.adjoint_forward(n, 1, 0, 20)
+        L0, ybar = swm.workloads.final_state_loss(S, sim.download(), obs, mask)
+        g, _ = sim.pullback(ybar)
+        gFx = sim.pullback_Fx()
+    assert np.any(g.η != 0) and np.any(gFx != 0)
+
+    def loss(dη, dF, eps):
+        import copy
+        S2 = copy.copy(S)
+        S2.forcing = copy.copy(S.forcing)
+        S2.forcing.Fx = np.asfortranarray(S.forcing.Fx + eps * dF)
+        p = swm.PrognosticVars(u=prog.u, v=prog.v, η=np.asfortranarray(prog.η + eps * dη), sst=prog.sst, T="Float64")
+        with swm.Simulation(S2) as sim:
+            sim.upload(p)
+            assert sim.step(n, 1, 0) == (n, 0)
+            return swm.workloads.final_state_loss(S2, sim.download(), obs, mask)[0]
+
+    # The differences' own error is about the rounding of the loss (a sum over 10^6 cells)
+    # over ε, the same in every direction; it is measured against the typical size of the
+    # directional derivatives (their median), so that a direction with a small derivative
+    # is held to the same absolute accuracy.
+    rng = np.random.default_rng(16)
+    eps = 3e-4
+    pairs = []
+    for d in range(16):
+        dη = np.zeros_like(prog.η)
+        dη[1:-1, 1:-1] = 0.1 * rng.standard_normal((G.nx, G.ny))
+        dF = np.abs(S.forcing.Fx).max() * rng.standard_normal(S.forcing.Fx.shape)
+        ad = float(np.sum(g.η * dη)) + float(np.sum(gFx * dF))
+        fd = (loss(dη, dF, eps) - loss(dη, dF, -eps)) / (2 * eps)
+        pairs.append((ad, fd))
+        print(f"direction {d}: adjoint {ad:.10e} finite differences {fd:.10e}", flush=True)
+    scale = float(np.median([abs(a) for a, _ in pairs]))
+    worst = max(abs(f - a) / max(abs(a), scale) for a, f in pairs)
+    print(f"worst relative difference {worst:.2e} (median |derivative| {scale:.3e}, loss {L0:.6e})", flush=True)
+    assert worst <= 1e-6
