@@ -110,6 +110,30 @@ const char *swm_last_error(const swm_ctx *ctx);
 int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy,
                    const void *f_q, const void *eta_ref, const void *Feta);
 
+/* The same static fields (and f_u, f_v with the linear dynamics) computed on the device
+ * instead of uploaded: Forcing{T}(P, G) and Grid.f_q / f_u / f_v (forcing.jl:11-217,
+ * grid.jl:91-108), each field evaluated in Float64 from the grid coordinates with the
+ * reference's formulas and operation order and rounded once to T.  The setup below carries
+ * the scalars the host derives from Parameter and Grid. */
+enum { SWM_TOPO_FLAT = 0, SWM_TOPO_RIDGE = 1, SWM_TOPO_RIDGES = 2, SWM_TOPO_SEAMOUNT = 3 };
+enum { SWM_WIND_CHANNEL = 0, SWM_WIND_SHEAR = 1, SWM_WIND_DOUBLE_GYRE = 2, SWM_WIND_CONSTANT = 3, SWM_WIND_NONE = 4 };
+typedef struct swm_setup {
+    int32_t topography;           /* SWM_TOPO_*, P.topography */
+    int32_t wind_x, wind_y;       /* SWM_WIND_*, P.wind_forcing_x / _y (y: CONSTANT or NONE) */
+    int32_t pad0;
+    double Lx, Ly, Delta;         /* grid.jl:21-23 (F64) */
+    double H, topo_height, topo_width;
+    double Fx0, Fy0, rho;
+    double f0, beta;              /* grid.jl:91-92 (F64) */
+    double eta_refh, eta_refw;    /* forcing.jl:127-136 */
+    double A, phik, wk, R, phi;   /* forcing.jl:198-212 */
+} swm_setup;
+int swm_set_static_device(swm_ctx *ctx, const swm_setup *setup);
+/* The static fields as the context holds them, in the shapes swm_set_static takes (NULL
+ * skips one; eta_ref / Feta only where continuity reads them; Fx, Fy, f_q multiplied by
+ * 2^scale_log2 where the velocities are carried scaled). */
+int swm_download_static(swm_ctx *ctx, void *H, void *Fx, void *Fy, void *f_q, void *eta_ref, void *Feta);
+
 /* The Coriolis parameter on the u- and v-grid, read by the linear dynamics only
  * (rhs_linear!, src/rhs.jl:55-86; Grid.f_u, f_v, src/grid.jl:93-94), format T, Julia layout:
  *   f_u   (nux, nuy)

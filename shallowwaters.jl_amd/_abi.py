@@ -17,7 +17,18 @@ EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
            "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
            "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer",
-           "swm_adj_track_Fx", "swm_adj_download_Fx"]
+           "swm_adj_track_Fx", "swm_adj_download_Fx", "swm_set_static_device", "swm_download_static"]
+
+TOPOGRAPHY = {"flat": 0, "ridge": 1, "ridges": 2, "seamount": 3}
+WIND = {"channel": 0, "shear": 1, "double_gyre": 2, "constant": 3, "none": 4}
+
+
+class SwmSetup(ctypes.Structure):
+    """``swm_setup``: the scalars the device setup of the static fields reads."""
+    _fields_ = [("topography", ctypes.c_int32), ("wind_x", ctypes.c_int32), ("wind_y", ctypes.c_int32),
+                ("pad0", ctypes.c_int32)] + [(n, ctypes.c_double) for n in (
+                    "Lx", "Ly", "Delta", "H", "topo_height", "topo_width", "Fx0", "Fy0", "rho", "f0", "beta",
+                    "eta_refh", "eta_refw", "A", "phik", "wk", "R", "phi")]
 
 _lib = None
 
@@ -65,6 +76,8 @@ def load():
     L.swm_adj_time.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]
     L.swm_adj_track_Fx.argtypes = [vp, i32]
     L.swm_adj_download_Fx.argtypes = [vp, vp]
+    L.swm_set_static_device.argtypes = [vp, ctypes.POINTER(SwmSetup)]
+    L.swm_download_static.argtypes = [vp] * 7
     for name in EXPORTS:
         if name not in ("swm_last_error", "swm_ens_last_error"):
             getattr(L, name).restype = ctypes.c_int

@@ -275,6 +275,22 @@ int swm_destroy(swm_ctx *ctx) {
     return SWM_OK;
 }
 
+// The periodic fused stage pairs compute the columns beyond the seam from the columns they
+// copy; that gives the reference's values where the copies are exact (Tcomm at least as wide
+// as the conversion to T needs), H's ghost columns equal the columns they face across the
+// seam, and f_q does not vary along x.  H: the host copy, Julia layout (nx+2, ny+2).
+static void seam_check(swm_ctx *ctx, const char *h) {
+    const swm_config &c = ctx->c;
+    const bool exact = c.Tcomm == c.T || c.Tcomm == c.Tprog || c.Tcomm == SWM_F64 ||
+                       (c.Tcomm == SWM_F32 && c.Tprog != SWM_F64);
+    const size_t es = (size_t)ctx->est, m = (size_t)c.nx + 2;
+    bool same = true;
+    for (size_t j = 0; j < (size_t)c.ny + 2 && same; ++j)
+        same = std::memcmp(h + (j * m + 0) * es, h + (j * m + c.nx) * es, es) == 0 &&
+               std::memcmp(h + (j * m + c.nx + 1) * es, h + (j * m + 1) * es, es) == 0;
+    ctx->seam_ok = exact && same && ctx->fq_sx == 0;
+}
+
 int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, const void *f_q, const void *eta_ref,
                    const void *Feta) {
     if (!ctx) return SWM_E_ARG;
@@ -290,21 +306,7 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
         rc = fmt_ops(ctx->fmt).scale_static(ctx);
         if (rc) return rc;
     }
-    if (c.bc == SWM_PERIODIC) {
-        // The periodic fused stage pairs compute the columns beyond the seam from the columns
-        // they copy; that gives the reference's values where the copies are exact (Tcomm at
-        // least as wide as the conversion to T needs), H's ghost columns equal the columns
-        // they face across the seam, and f_q does not vary along x.
-        const bool exact = c.Tcomm == c.T || c.Tcomm == c.Tprog || c.Tcomm == SWM_F64 ||
-                           (c.Tcomm == SWM_F32 && c.Tprog != SWM_F64);
-        const char *h = (const char *)H;
-        const size_t es = (size_t)ctx->est, m = (size_t)c.nx + 2;
-        bool same = true;
-        for (size_t j = 0; j < (size_t)c.ny + 2 && same; ++j)
-            same = std::memcmp(h + (j * m + 0) * es, h + (j * m + c.nx) * es, es) == 0 &&
-                   std::memcmp(h + (j * m + c.nx + 1) * es, h + (j * m + 1) * es, es) == 0;
-        ctx->seam_ok = exact && same && ctx->fq_sx == 0;
-    }
+    if (c.bc == SWM_PERIODIC) seam_check(ctx, (const char *)H);
     if (c.continuity == SWM_CONT_RELAX) {
         if (!eta_ref) return ctx->fail(SWM_E_ARG, "eta_ref is required for surface relaxation");
         if ((rc = put2d(ctx, ctx->eta_ref, eta_ref, c.nx, c.ny, 1, 1))) return rc;
@@ -315,6 +317,62 @@ int swm_set_static(swm_ctx *ctx, const void *H, const void *Fx, const void *Fy, 
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_static = true;
+    return SWM_OK;
+}
+
+int swm_set_static_device(swm_ctx *ctx, const swm_setup *st) {
+    if (!ctx) return SWM_E_ARG;
+    if (!st) return ctx->fail(SWM_E_ARG, "setup is required");
+    if (ctx->have_static) return ctx->fail(SWM_E_STATE, "static fields already set");
+    if (ctx->c.dynamics == SWM_LINEAR && ctx->have_fuv) return ctx->fail(SWM_E_STATE, "f_u, f_v already set");
+    if (st->topography < SWM_TOPO_FLAT || st->topography > SWM_TOPO_SEAMOUNT)
+        return ctx->fail(SWM_E_ARG, "unknown topography");
+    if (st->wind_x < SWM_WIND_CHANNEL || st->wind_x > SWM_WIND_NONE) return ctx->fail(SWM_E_ARG, "unknown wind_x");
+    // the other meridional winds are functions on the u-grid (forcing.jl:15-37), of another shape than Fy
+    if (st->wind_y != SWM_WIND_CONSTANT && st->wind_y != SWM_WIND_NONE)
+        return ctx->fail(SWM_E_ARG, "wind_y must be constant or none");
+    if (!(st->Delta > 0) || !(st->Lx > 0) || !(st->Ly > 0) || !(st->H > 0) || !(st->rho > 0))
+        return ctx->fail(SWM_E_ARG, "Delta, Lx, Ly, H and rho must be positive");
+    const swm_config &c = ctx->c;
+    int rc = fmt_ops(ctx->fmt).setup_static(ctx, st);
+    if (rc) return rc;
+    if (c.scale_log2 != 0 && (rc = fmt_ops(ctx->fmt).scale_static(ctx))) return rc;
+    if (c.bc == SWM_PERIODIC) {
+        std::vector<char> h((size_t)(c.nx + 2) * (c.ny + 2) * ctx->est);
+        if ((rc = get2d(ctx, ctx->H, h.data(), c.nx + 2, c.ny + 2, 0, 0))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        seam_check(ctx, h.data());
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_static = true;
+    if (c.dynamics == SWM_LINEAR) ctx->have_fuv = true;
+    return SWM_OK;
+}
+
+int swm_download_static(swm_ctx *ctx, void *H, void *Fx, void *Fy, void *f_q, void *eta_ref, void *Feta) {
+    if (!ctx) return SWM_E_ARG;
+    if (!ctx->have_static) return ctx->fail(SWM_E_STATE, "static fields not set");
+    const swm_config &c = ctx->c;
+    int rc;
+    // a field kept one value per row comes back spread along x, as it was given
+    auto get = [&](const Field &f, int sx, void *host, int m, int n, int x0, int y0) -> int {
+        if (!host) return SWM_OK;
+        if (sx != 0) return get2d(ctx, f, host, m, n, x0, y0);
+        const size_t es = (size_t)f.es;
+        std::vector<char> col((size_t)n * es);
+        HIPCHK(hipMemcpyAsync(col.data(), f.p0 + (ptrdiff_t)y0 * es, col.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        char *o = (char *)host;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) std::memcpy(o + ((size_t)j * m + i) * es, &col[(size_t)j * es], es);
+        return SWM_OK;
+    };
+    if ((rc = get(ctx->H, 1, H, c.nx + 2, c.ny + 2, 0, 0)) || (rc = get(ctx->Fx, ctx->Fx_sx, Fx, ctx->nux, ctx->nuy, 1 - ctx->ep, 1)) ||
+        (rc = get(ctx->Fy, ctx->Fy_sx, Fy, ctx->nvx, ctx->nvy, 1, 1)) || (rc = get(ctx->fq, ctx->fq_sx, f_q, c.nx + 1, c.ny + 1, 0, 0)))
+        return rc;
+    if (eta_ref && c.continuity == SWM_CONT_RELAX && (rc = get(ctx->eta_ref, 1, eta_ref, c.nx, c.ny, 1, 1))) return rc;
+    if (Feta && c.continuity == SWM_CONT_FORCING && (rc = get(ctx->Feta, 1, Feta, c.nx, c.ny, 1, 1))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return SWM_OK;
 }
 

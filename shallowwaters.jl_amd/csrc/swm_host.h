@@ -456,6 +456,7 @@ struct FmtOps {
                         void *ge, void *gs);
     int (*adj_time)(swm_ctx *ctx, int32_t which, int32_t reps, double *ms);
     int (*adj_download_fx)(swm_ctx *ctx, void *xbar_Fx);
+    int (*setup_static)(swm_ctx *ctx, const swm_setup *s);   // the static fields computed on the device
 };
 const FmtOps &ops_f64();
 const FmtOps &ops_f32();

@@ -8,6 +8,7 @@
 #include "k_adjoint.h"
 #include "k_dissipation.h"
 #include "k_misc.h"
+#include "k_setup.h"
 #include "k_stage.h"
 
 namespace swmh {
@@ -1563,8 +1564,50 @@ template <class F, bool ADJ> struct Entry {
         return rc;
     }
     static int tracer(swm_ctx *ctx, int64_t i) { return Launch<F>::tracer(ctx, i, ctx->pu); }
+    // Forcing{T} and f on the device: 2-D fields in the padded layout, the x-invariant
+    // ones (winds, f) as one value per row, as swm_set_static stores them
+    static int setup_static(swm_ctx *ctx, const swm_setup *st) {
+        const swm_config &c = ctx->c;
+        SetupArgs a{};
+        a.topo = st->topography; a.wind_x = st->wind_x; a.wind_y = st->wind_y;
+        a.Lx = st->Lx; a.Ly = st->Ly; a.D = st->Delta; a.H = st->H; a.th = st->topo_height; a.w = st->topo_width;
+        a.Fx0 = st->Fx0; a.Fy0 = st->Fy0; a.rho = st->rho; a.f0 = st->f0; a.beta = st->beta;
+        a.eta_refh = st->eta_refh; a.eta_refw = st->eta_refw; a.A = st->A; a.phik = st->phik; a.wk = st->wk;
+        a.R = st->R; a.phi = st->phi;
+        a.nx = c.nx; a.ny = c.ny; a.ep = ctx->ep;
+        auto fill = [&](int field, Field &f, bool rows, int m, int n, int x0, int y0) {
+            a.field = field;
+            if (rows) {
+                as_rows(f);
+                a.p = f.p0 + (ptrdiff_t)y0 * f.es;
+                a.sx = 0; a.sy = 1; a.m = 1;
+            } else {
+                a.p = elem(ctx, f, x0, y0);
+                a.sx = 1; a.sy = ctx->P; a.m = m;
+            }
+            a.n = n;
+            hipLaunchKernelGGL((setup_kernel<ST>), dim3((a.m + 255) / 256, n), dim3(256), 0, ctx->stream, a);
+        };
+        fill(SET_H, ctx->H, false, c.nx + 2, c.ny + 2, 0, 0);
+        fill(SET_FX, ctx->Fx, true, ctx->nux, ctx->nuy, 1 - ctx->ep, 1);
+        fill(SET_FY, ctx->Fy, true, ctx->nvx, ctx->nvy, 1, 1);
+        fill(SET_FQ, ctx->fq, true, c.nx + 1, c.ny + 1, 0, 0);
+        ctx->Fx_sx = ctx->Fy_sx = ctx->fq_sx = 0;
+        ctx->Fx_sy = ctx->Fy_sy = ctx->fq_sy = 1;
+        if (c.continuity == SWM_CONT_RELAX) fill(SET_ETA_REF, ctx->eta_ref, false, c.nx, c.ny, 1, 1);
+        if (c.continuity == SWM_CONT_FORCING) fill(SET_FETA, ctx->Feta, false, c.nx, c.ny, 1, 1);
+        if (c.dynamics == SWM_LINEAR) {
+            fill(SET_FU, ctx->f_u, true, ctx->nux, ctx->nuy, 1 - ctx->ep, 1);
+            fill(SET_FV, ctx->f_v, true, ctx->nvx, ctx->nvy, 1, 1);
+            ctx->fu_sx = ctx->fv_sx = 0;
+            ctx->fu_sy = ctx->fv_sy = 1;
+        }
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
     static FmtOps make() {
         FmtOps o{};
+        o.setup_static = &setup_static;
         o.ghost = &ghost;
         o.dissipation = &dissipation;
         o.tracer = &tracer;

@@ -22,6 +22,17 @@ from .numtypes import canonical_type, storage_dtype
 from .parameter import Parameter
 
 
+def setup_of(S: ModelSetup) -> "_abi.SwmSetup":
+    """The ``swm_setup`` of a model setup: the parameters of its winds and topography and the
+    grid scalars the fields are evaluated from (Δ, Lx, Ly, f₀ and β from ``src/grid.jl:21-23,91-92``)."""
+    P, G = S.parameters, S.grid
+    return _abi.SwmSetup(topography=_abi.TOPOGRAPHY[P.topography], wind_x=_abi.WIND[P.wind_forcing_x],
+                         wind_y=_abi.WIND[P.wind_forcing_y], Lx=G.Lx, Ly=G.Ly, Delta=G.Δ, H=P.H,
+                         topo_height=P.topo_height, topo_width=P.topo_width, Fx0=P.Fx0, Fy0=P.Fy0, rho=P.ρ,
+                         f0=G.f0, beta=G.β, eta_refh=P.η_refh, eta_refw=P.η_refw, A=P.A, phik=P.ϕk, wk=P.wk,
+                         R=G.R, phi=G.ϕ)
+
+
 @dataclass
 class DiagnosticVars:
     """``DiagnosticVars{T,Tprog}`` (``src/preallocate.jl:439-451``).  On the GPU the
@@ -38,15 +49,25 @@ def preallocate(T, Tprog, G) -> DiagnosticVars:
 
 
 class Simulation:
-    """One library context: device buffers, static fields and the prognostic state."""
+    """One library context: device buffers, static fields and the prognostic state.
 
-    def __init__(self, S: ModelSetup):
+    ``statics="host"`` uploads the static fields ``S.forcing`` and ``S.grid`` hold;
+    ``statics="device"`` computes them on the GPU from the parameters (``Forcing{T}(P,G)``,
+    ``src/forcing.jl:11-53``, and ``f_q``, ``f_u``, ``f_v``, ``src/grid.jl:91-108``)."""
+
+    def __init__(self, S: ModelSetup, statics: str = "host"):
         self.S = S
         self.cfg = make_config(S)
         L = _abi.load()
         ctx = ctypes.c_void_p()
         _abi.check(L.swm_create(ctypes.byref(self.cfg), ctypes.byref(ctx)), None, "swm_create")
         self.ctx = ctx
+        if statics == "device":
+            self._static = []
+            _abi.check(L.swm_set_static_device(ctx, ctypes.byref(setup_of(S))), ctx, "swm_set_static_device")
+            return
+        if statics != "host":
+            raise ValueError(f"statics must be 'host' or 'device', got {statics!r}")
         F, G = S.forcing, S.grid
         self._static = [np.asfortranarray(a) for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
         _abi.check(L.swm_set_static(ctx, *[_abi.ptr(a) for a in self._static]), ctx, "swm_set_static")
@@ -54,6 +75,22 @@ class Simulation:
             self._static += [np.asfortranarray(a) for a in (G.f_u, G.f_v)]
             _abi.check(L.swm_set_static_linear(ctx, *[_abi.ptr(a) for a in self._static[-2:]]), ctx,
                        "swm_set_static_linear")
+
+    def download_static(self):
+        """The static fields as the context holds them: dict of H, Fx, Fy, f_q (and η_ref or
+        Fη where continuity reads them) in the shapes ``Forcing`` and ``Grid`` hold."""
+        G, P = self.S.grid, self.S.parameters
+        dt = storage_dtype(P.T)
+        z = lambda m, n: np.zeros((m, n), dtype=dt, order="F")
+        out = {"H": z(G.nx + 2, G.ny + 2), "Fx": z(G.nux, G.nuy), "Fy": z(G.nvx, G.nvy), "f_q": z(G.nx + 1, G.ny + 1)}
+        if P.surface_relax:
+            out["η_ref"] = z(G.nx, G.ny)
+        if P.surface_forcing:
+            out["Fη"] = z(G.nx, G.ny)
+        args = [out[k] if k in out else None for k in ("H", "Fx", "Fy", "f_q", "η_ref", "Fη")]
+        _abi.check(_abi.load().swm_download_static(self.ctx, *[None if a is None else _abi.ptr(a) for a in args]),
+                   self.ctx, "swm_download_static")
+        return out
 
     def close(self):
         if getattr(self, "ctx", None):

@@ -35,10 +35,9 @@ def test_library_loads_and_exports_every_declared_symbol(swm):
     assert L.swm_version(ctypes.byref(v)) == 0 and v.value == 2
 
 
-def test_config_struct_matches_header(swm):
-    """The ctypes mirror of swm_config lists the header's fields in order."""
+def struct_fields(name):
     text = open(HEADER).read()
-    body = text[text.index("typedef struct swm_config {"):text.index("} swm_config;")]
+    body = text[text.index(f"typedef struct {name} {{"):text.index(f"}} {name};")]
     names = []
     for line in body.splitlines()[1:]:
         line = line.split("/*")[0].strip()
@@ -47,7 +46,33 @@ def test_config_struct_matches_header(swm):
         decl = line.split(None, 1)[1].rstrip(";")
         for part in decl.split(","):
             names.append(part.strip().split("[")[0])
-    assert [f[0] for f in swm.swm_config._fields_] == names
+    return names
+
+
+def test_config_struct_matches_header(swm):
+    """The ctypes mirror of swm_config lists the header's fields in order."""
+    assert [f[0] for f in swm.swm_config._fields_] == struct_fields("swm_config")
+
+
+def test_setup_struct_and_enums_match_header(swm):
+    """swm_setup's ctypes mirror, and the topography and wind codes, as the header has them."""
+    from shallowwaters_amd import _abi
+    assert [f[0] for f in _abi.SwmSetup._fields_] == struct_fields("swm_setup")
+    assert ctypes.sizeof(_abi.SwmSetup) == 4 * 4 + 18 * 8
+    text = open(HEADER).read()
+    for prefix, table in (("SWM_TOPO_", _abi.TOPOGRAPHY), ("SWM_WIND_", _abi.WIND)):
+        codes = {k.lower(): int(v) for k, v in re.findall(prefix + r"(\w+)\s*=\s*(\d+)", text)}
+        assert codes == table
+
+
+def test_setup_of_carries_the_grid_scalars(swm):
+    from shallowwaters_amd.time_integration import setup_of
+    S = swm.make_setup(swm.Parameter(nx=64, topography="seamount", wind_forcing_x="double_gyre", ϕ=30.0))
+    st = setup_of(S)
+    G, P = S.grid, S.parameters
+    assert (st.Delta, st.Lx, st.Ly, st.f0, st.beta) == (G.Δ, G.Lx, G.Ly, G.f0, G.β)
+    assert (st.topography, st.wind_x, st.wind_y) == (3, 2, 3)
+    assert (st.H, st.rho, st.phi, st.phik) == (P.H, P.ρ, 30.0, 30.0)
 
 
 def test_product_path_never_uses_the_oracle():
