@@ -84,6 +84,7 @@ template <class F> struct StageArgs {
     // fused pairs: strip s of the launch is strip s + sfirst of the field, plus sjump from
     // strip sjump_at on (a launch over the strips at the periodic seam skips the others)
     int sfirst = 0, sjump_at = 1 << 30, sjump = 0;
+    uint32_t tp_u = 0, tp_v = 0, tp_e = 0;   // fused pairs with ADV_TAPE: where the first stage's output is kept
 };
 
 
@@ -474,6 +475,9 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using SI = StageIn<F, MA>;
     using VI = typename Sto<SI>::V;
     static_assert(MIA::rk && MIB::rk && !MIB::first && MIA::out1, "fused pair of classic RK stages");
+    constexpr bool TAPE = (ADV & ADV_TAPE) != 0;
+    constexpr int AF = ADV & ~ADV_TAPE;                 // the right-hand side form
+    static_assert(!TAPE || !F::mixed, "the tape of a fused pair is kept in the stage input's format");
     const WaveTile t = fused_tile(A, w);
     if (!t.live) return;
     const int x = t.x, P = A.P, nx = A.nx, ny = A.ny, lane = t.lane;
@@ -502,6 +506,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const Fld<Q> o2_u{&ar, A.o2_u}, o2_v{&ar, A.o2_v}, o2_e{&ar, A.o2_e};
     const Fld<ST> fH{&ar, A.st.H};
     const Fld<Q> cmp_u{&ar, A.c_u}, cmp_v{&ar, A.c_v}, cmp_e{&ar, A.c_e};
+    const Fld<ST> tp_u{&ar, A.tp_u}, tp_v{&ar, A.tp_v}, tp_e{&ar, A.tp_e};
     const uint32_t vxQ = colb<Q>(x), vxT = colb<ST>(x);                 // stores (output lanes)
     const uint32_t vIu = colb<SI>(xu), vIt = colb<SI>(xt), vQu = colb<Q>(xu), vQt = colb<Q>(xt), vTt = colb<ST>(xt);
     const uint32_t vfq = scolb<ST>(xu, A.st.fq_2d), vFx = scolb<ST>(xu, A.st.Fx_2d), vFy = scolb<ST>(xt, A.st.Fy_2d);
@@ -509,7 +514,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     const R half = R(0.5), quarter = R(0.25), k12 = A.k12;
     const Q oma = A.oma;
 
-    RhsPipe<R, PER, ADV, PER> ra, rb;
+    RhsPipe<R, PER, AF, PER> ra, rb;
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
     // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
@@ -526,7 +531,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     Q bpu = 0, bpe = 0, bpv = 0;                                 // B's Prog operands (rows r-3, r-4)
     // A's statics a row ahead (f_q, Fx, η static at row r-1, Fy at r-2); B takes those A had
     // two iterations before (rows r-3, r-4)
-    RowStatics<R> nst = row_statics<F, false, CS, ADV>(A, ar, vfq, vFx, vFy, vTt, r - 1, vfu, vfv);
+    RowStatics<R> nst = row_statics<F, false, CS, AF>(A, ar, vfq, vFx, vFy, vTt, r - 1, vfu, vfv);
     RowStatics<R> st_1 = {0, 0, 0, 0, 0}, st_2 = {0, 0, 0, 0, 0};
     const uint32_t dI = (uint32_t)P * sizeof(SI), dQ = (uint32_t)P * sizeof(Q), dT = (uint32_t)P * sizeof(ST);
     uint32_t sI = rowb<SI>(r, P), sQ = rowb<Q>(r, P), sT = rowb<ST>(r, P);   // byte offsets of row r
@@ -541,7 +546,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
         const RowStatics<R> sa = nst, sb = st_2;
-        nst = row_statics<F, false, CS, ADV>(A, ar, vfq, vFx, vFy, vTt, r, vfu, vfv);
+        nst = row_statics<F, false, CS, AF>(A, ar, vfq, vFx, vFy, vTt, r, vfu, vfv);
         {   // the input row r+1, clamped to the last row the march reads
             const int rn = min(r + 1, rlast);
             nu = in_u.ld(vIu, rowb<SI>(rn, P)); nv = in_v.ld(vIt, rowb<SI>(rn, P)); ne = in_e.ld(vIt, rowb<SI>(rn, P));
@@ -565,6 +570,17 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
             combine<MA>((Q)we_1, MIA::first ? (Q)we_1 : pe_1, ae_1, deta, A.k, ce1, ce0, nocc);
             const Q dv = -(toQ<Q>(ra.qhu) + toQ<Q>(ra.py)) + toQ<Q>(A.Fyt * sa.Fy);
             combine<MA>((Q)wv_2, MIA::first ? (Q)wv_2 : pv_2, av_2, dv, A.k, cv1, cv0, nocc);
+        }
+        if constexpr (TAPE) {
+            // A's first outputs as the stage kernel stores them: the rows of this segment,
+            // the columns B stores, with the ghost copies
+            const int yu = r - 1, yv = r - 2;
+            if (yu >= t.ya && yu < t.yb) {
+                if (u_col) put_u<ST, PER>(tp_u, x, vxT, yu, sT - dT, cu1, gl.u, nx, ny, P, oma, A.cm);
+                if (e_col) put_e<ST, PER>(tp_e, x, vxT, yu, sT - dT, ce1, gl.e, nx, ny, P, A.cm);
+            }
+            if (v_col && yv >= t.ya && yv < t.yb && yv <= ny - 1)
+                put_v<ST, PER>(tp_v, x, vxT, yv, sT - 2 * dT, cv1, gl.v, nx, P, oma, A.cm);
         }
         // ghost_points!(u1, v1, η1) on A's first outputs.  Nonperiodic: u's normal halo is
         // 0, its tangential rows mirror with (1-α); v's normal rows are 0, its columns mirror

@@ -317,6 +317,9 @@ struct StaticRefs {
 // ADV_KEEP (with ADV_AH or ADV_SAD): q and the kinetic part of the Bernoulli potential are
 // read from the last update instead of formed (nstep_advcor > 0, rhs.jl:36-38)
 enum { ADV_AH = 0, ADV_SAD = 1, ADV_LIN = 2, ADV_KEEP = 4 };
+// a flag beside the right-hand side form of a fused pair: the first stage's output is also
+// stored (the adjoint's taped step keeps every stage input)
+enum { ADV_TAPE = 8 };
 
 // byte offset of (x, y) in a static field that may be stored 1-D (one value per row)
 template <class S> __device__ __forceinline__ uint32_t scolb(int x, int is2d) { return is2d ? colb<S>(x) : 0u; }

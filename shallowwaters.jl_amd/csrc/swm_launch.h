@@ -272,6 +272,43 @@ template <class F> struct Launch {
         return SWM_OK;
     }
 
+    // A fused pair of the adjoint's taped step: as fused(), and the first stage's output is
+    // also stored, in slots tp (T = Tprog, Arakawa-Hsu, a seam the lanes wrap around).
+    static bool can_tape(const swm_ctx *ctx, int ma, int mb) {
+        return !F::mixed && ctx->fuse && can_fuse(ctx, ma, mb) && adv_form(ctx->c) == ADV_AH &&
+               (ctx->c.bc == SWM_NONPERIODIC || ctx->seam_ok) && !ctx->rec;
+    }
+    static int fused_tape(swm_ctx *ctx, int ma, int mb, Slots in, Slots pr, Slots ac, Slots o1, Slots o2, Slots tp,
+                          const Coef<Q> &ka, const Coef<Q> &kb, const Times &T) {
+        auto ok = [&](Slots s) { return !s.t && s.uv >= 0 && s.uv < ctx->nslots && s.e >= 0 && s.e < ctx->nslots; };
+        const bool afirst = ma == M_RK_FIRST, blast = mb == M_RK_LAST;
+        if (!can_tape(ctx, ma, mb) || !ok(in) || !ok(pr) || !(afirst || ok(ac)) || !(blast || ok(o1)) || !ok(o2) || !ok(tp))
+            return ctx->fail(SWM_E_STATE, "taped fused stages %d+%d: not available here", ma, mb);
+        StageArgs<F> A = args(ctx, in, pr, ac, o1, o2, ka, T);
+        A.k2 = kb;
+        A.nstrips = nstrips2(ctx);
+        A.L = ctx->Lf;
+        A.nseg = nseg(ctx, ctx->Lf);
+        A.tp_u = ou(ctx, tp); A.tp_v = ov(ctx, tp); A.tp_e = oe(ctx, tp);
+        const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+        const bool per = ctx->c.bc == SWM_PERIODIC, cs = ctx->c.continuity != SWM_CONT_ITSELF;
+        if constexpr (!F::mixed) {
+#define SWM_FT(MA_, MB_)                                                                                                \
+    if (per) {                                                                                                          \
+        if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, true, ADV_TAPE>), grid, block, 0, ctx->stream, A);   \
+        else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, true, ADV_TAPE>), grid, block, 0, ctx->stream, A);     \
+    } else if (cs) hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, true, false, ADV_TAPE>), grid, block, 0, ctx->stream, A); \
+    else hipLaunchKernelGGL((fused_kernel<F, MA_, MB_, false, false, ADV_TAPE>), grid, block, 0, ctx->stream, A)
+            if (ma == M_RK_FIRST && mb == M_RK_MID) SWM_FT(M_RK_FIRST, M_RK_MID);
+            else if (ma == M_RK_MID && mb == M_RK_LAST) SWM_FT(M_RK_MID, M_RK_LAST);
+            else if (ma == M_RK_FIRST && mb == M_RK_LAST) SWM_FT(M_RK_FIRST, M_RK_LAST);
+            else return ctx->fail(SWM_E_ARG, "taped fused stages %d+%d", ma, mb);
+#undef SWM_FT
+        }
+        HIPCHK(hipGetLastError());
+        return SWM_OK;
+    }
+
     // A periodic fused pair whose seam the lanes cannot wrap around: the strips clear of
     // the seam on the wrapping kernel (no lane of theirs wraps), and beside them, on the
     // context's second stream, the strips at the seam on fused_seam_kernel.  The two write
@@ -885,12 +922,22 @@ template <class Fm> struct Adj {
         const Slots P{uv[0], ee[0]}, a{uv[1], ee[1]}, none{};
         auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
         int rc = SWM_OK;
-        for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
+        auto coef = [&](int q) {
             Coef<Q> kk{};
             kk.a = (Q)c.RKaDt[q - 1];
             kk.b = q < c.RKo ? (Q)c.RKbDt[q - 1] : Q(0);
-            const int m = q == 1 ? M_RK_FIRST : (q == c.RKo ? M_RK_LAST : M_RK_MID);
-            rc = LF::stage(ctx, m, sk(q), P, a, q < c.RKo ? sk(q + 1) : none, a, kk, T);
+            return kk;
+        };
+        auto mode = [&](int q) { return q == 1 ? M_RK_FIRST : (q == c.RKo ? M_RK_LAST : M_RK_MID); };
+        for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
+            // stages q, q+1 as one fused pair that also keeps the input of stage q+1
+            if (q + 1 <= c.RKo && LF::can_tape(ctx, mode(q), mode(q + 1))) {
+                rc = LF::fused_tape(ctx, mode(q), mode(q + 1), sk(q), P, q == 1 ? none : a, q + 1 < c.RKo ? sk(q + 2) : none,
+                                    a, sk(q + 1), coef(q), coef(q + 1), T);
+                ++q;
+                continue;
+            }
+            rc = LF::stage(ctx, mode(q), sk(q), P, a, q < c.RKo ? sk(q + 1) : none, a, coef(q), T);
         }
         if (rc) return rc;
         int post = a.uv;

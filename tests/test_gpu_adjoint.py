@@ -64,6 +64,28 @@ def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T)
         assert len(bad) == 0, f"{name}: {len(bad)} cells differ, first {bad[0].tolist()}: {r[tuple(bad[0])]!r} vs {a[tuple(bad[0])]!r}"
 
 
+@pytest.mark.parametrize("case", ["nonperiodic-F32-RK4", "nonperiodic-F64-RK3", "periodic-F32-RK4", "periodic-F64-RK2",
+                                  "nonperiodic-F32-RK4-relax"])
+def test_taped_fused_pairs_equal_taped_stages(swm, gpu, monkeypatch, case):
+    """The taped step on fused stage pairs (each keeping its first stage's output) gives the
+    gradient of the taped step on one kernel per stage, bitwise; periodic over a flat bottom,
+    whose seam the pairs' lanes wrap around."""
+    bc, T, rk = case.split("-")[:3]
+    kw = dict(RKo=int(rk[2]), topography="flat" if bc == "periodic" else "seamount")
+    if case.endswith("relax"):
+        kw.update(surface_relax=True, t_relax=2.0)
+    P, S, cfg, prog = setup(swm, bc, T=T, **kw)
+    dt = np.float64 if T == "Float64" else np.float32
+    ybar = cotangent(S, 3, dt)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("SWM_FUSE", fuse)
+        print(f"taped {case} fuse={fuse}", flush=True)
+        out[fuse], _ = gpu_gradient(swm, S, prog, ybar, 4, snapshots=2)
+    for name, a, b in zip(("u", "v", "η", "sst"), out["1"], out["0"]):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), f"{case} {name}: {np.count_nonzero(a != b)} cells differ"
+
+
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
 def test_dot_product_identity_fp64(swm, orc, gpu, bc):
     P, S, cfg, prog = setup(swm, bc, nx=90)
