@@ -86,7 +86,10 @@ typedef struct swm_config {
     /* classic RK: sum the stage increments apart and add them to the state with Kahan's
      * compensated summation (a persistent compensation per variable; not in the reference) */
     int32_t compensated;
-    int32_t pad1;
+    /* T = Float16 or BFloat16 with Tprog = Float32 only: T's values (the stage inputs, the
+     * static fields, the (T) constants) in T, the arithmetic on them in Float32, results not
+     * rounded back to T (a form of this library, not of the reference; nstep_advcor = 0) */
+    int32_t t_arith32;
 } swm_config;
 
 typedef struct swm_ctx swm_ctx;

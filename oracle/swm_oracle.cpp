@@ -99,6 +99,38 @@ struct Low {
 typedef Low<0> f16;
 typedef Low<1> bf16;
 
+// Binary16 / bfloat16 values with float arithmetic (swm_config.t_arith32, a form of the
+// library, not of the reference): a value entering the format (a conversion from Tprog or
+// Float64, a T constant) is rounded to it; + - * / and sqrt are float's, rounded once to
+// float and not back to the format.
+// the planted mistake MUT_WIDE_ROUNDS (see the MUT_* list): Wide's arithmetic rounds to the
+// format, which makes it Low's
+static bool g_wide_rounds = false;
+template <int KIND>
+struct Wide {
+    float v;
+    static float rnd(float x) { return Low<KIND>::rnd(x); }
+    static float op(float x) { return g_wide_rounds ? rnd(x) : x; }
+    Wide() : v(0.0f) {}
+    explicit Wide(double x) : v(rnd(f32_round_odd(x))) {}
+    explicit Wide(float x) : v(rnd(x)) {}
+    explicit Wide(int x) : v(rnd((float)x)) {}
+    static Wide raw(float x) { Wide r; r.v = x; return r; }
+    explicit operator double() const { return (double)v; }
+    explicit operator float() const { return v; }
+    friend Wide operator+(Wide a, Wide b) { return raw(op(a.v + b.v)); }
+    friend Wide operator-(Wide a, Wide b) { return raw(op(a.v - b.v)); }
+    friend Wide operator*(Wide a, Wide b) { return raw(op(a.v * b.v)); }
+    friend Wide operator/(Wide a, Wide b) { return raw(op(a.v / b.v)); }
+    friend Wide operator-(Wide a) { return raw(-a.v); }
+    Wide &operator+=(Wide b) { v = op(v + b.v); return *this; }
+    friend bool operator<(Wide a, Wide b) { return a.v < b.v; }
+    friend bool operator>(Wide a, Wide b) { return a.v > b.v; }
+    friend bool operator==(Wide a, Wide b) { return a.v == b.v; }
+};
+typedef Wide<0> f16w;
+typedef Wide<1> bf16w;
+
 // A float64 value carrying one directional derivative (forward-mode differentiation).  The
 // model instantiated on Dual returns its outputs together with the tangent-linear map applied
 // to the input tangents, operation by operation in the model's own order.  Kinks follow fixed
@@ -129,6 +161,8 @@ struct Dual {
 template <class T> static inline T num_sqrt(T x) { return std::sqrt(x); }
 template <> inline f16 num_sqrt<f16>(f16 x) { return f16::raw(f16::rnd(std::sqrt(x.v))); }
 template <> inline bf16 num_sqrt<bf16>(bf16 x) { return bf16::raw(bf16::rnd(std::sqrt(x.v))); }
+template <> inline f16w num_sqrt<f16w>(f16w x) { return f16w::raw(f16w::op(std::sqrt(x.v))); }
+template <> inline bf16w num_sqrt<bf16w>(bf16w x) { return bf16w::raw(bf16w::op(std::sqrt(x.v))); }
 template <> inline Dual num_sqrt<Dual>(Dual x) {
     const double s = std::sqrt(x.v);
     return Dual(s, s > 0.0 ? x.d / (2.0 * s) : 0.0);
@@ -178,7 +212,8 @@ static inline uint16_t float_to_half_bits(float x) {  // x already binary16-repr
 template <class T> static inline T load_elem(const void *p, size_t k) {
     if constexpr (std::is_same_v<T, double>) return ((const double *)p)[k];
     else if constexpr (std::is_same_v<T, float>) return ((const float *)p)[k];
-    else if constexpr (std::is_same_v<T, f16>) return T::raw(half_bits_to_float(((const uint16_t *)p)[k]));
+    else if constexpr (std::is_same_v<T, f16> || std::is_same_v<T, f16w>)
+        return T::raw(half_bits_to_float(((const uint16_t *)p)[k]));
     else {
         uint32_t b = (uint32_t)((const uint16_t *)p)[k] << 16;
         float f;
@@ -190,9 +225,11 @@ template <class T> static inline void store_elem(void *p, size_t k, T x) {
     if constexpr (std::is_same_v<T, double>) ((double *)p)[k] = x;
     else if constexpr (std::is_same_v<T, float>) ((float *)p)[k] = x;
     else if constexpr (std::is_same_v<T, f16>) ((uint16_t *)p)[k] = float_to_half_bits(x.v);
+    else if constexpr (std::is_same_v<T, f16w>) ((uint16_t *)p)[k] = float_to_half_bits(round_binary16(x.v));
     else {
         uint32_t b;
-        std::memcpy(&b, &x.v, 4);
+        const float r = std::is_same_v<T, bf16w> ? round_bfloat16(x.v) : x.v;
+        std::memcpy(&b, &r, 4);
         ((uint16_t *)p)[k] = (uint16_t)(b >> 16);
     }
 }
@@ -292,6 +329,7 @@ enum {
     MUT_RELAX_SIGN = 10,     // γ·(η - η_ref); continuity.jl:21
     MUT_FTIME_COS = 11,      // Ftime = cos(ω t); forcing.jl:216
     MUT_S11_ROW = 12,        // constant viscosity: S11 from dLudx[i,j+1]; diffusion.jl:188
+    MUT_WIDE_ROUNDS = 13,    // Float16 / BFloat16 values with float arithmetic: every operation rounds to the format
 };
 static int g_mutation = MUT_NONE;
 
@@ -1243,8 +1281,8 @@ template <class Fn> static int with_types(const swm_config *c, Fn &&fn) {
     };
     if (c->T == c->Tprog && c->T == SWM_F64) return comm(double(), double());
     if (c->T == c->Tprog && c->T == SWM_F32) return comm(float(), float());
-    if (c->Tprog == SWM_F32 && c->T == SWM_F16) return low(f16());
-    if (c->Tprog == SWM_F32 && c->T == SWM_BF16) return low(bf16());
+    if (c->Tprog == SWM_F32 && c->T == SWM_F16) return c->t_arith32 ? low(f16w()) : low(f16());
+    if (c->Tprog == SWM_F32 && c->T == SWM_BF16) return c->t_arith32 ? low(bf16w()) : low(bf16());
     return -1;
 }
 
@@ -1492,6 +1530,7 @@ int swmo_op(const swm_config *c, const void *H, const void *Fx, const void *Fy, 
 int swmo_set_mutation(int32_t k) {
     const int prev = g_mutation;
     g_mutation = k;
+    g_wide_rounds = k == MUT_WIDE_ROUNDS;
     return prev;
 }
 

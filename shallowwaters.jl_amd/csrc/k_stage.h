@@ -118,6 +118,16 @@ template <class R> struct RowStatics {
     R fq, Fx, Fy, cs, qend, fu, fv, qs, ks;
 };
 
+// q at the wrap column of row y: a T value, held as it is (float for T's values with float
+// arithmetic, where it is an unrounded result)
+template <class F> __device__ __forceinline__ typename F::R ld_qend(const Arena &ar, uint32_t o, int y) {
+    using R = typename F::R;
+    const Fld<QES<F>> q{&ar, o};
+    const auto v = q.ld(0, (uint32_t)(y + OY) * (uint32_t)sizeof(QES<F>));
+    if constexpr (is_ar<R>::value) return R::raw(v);
+    else return dec<R, QES<F>>(v);
+}
+
 template <class F, bool PER, bool CS, int ADV = ADV_AH>
 __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs<F> &A, const Arena &ar, uint32_t vfq,
                                                                  uint32_t vFx, uint32_t vFy, uint32_t vxT, int y,
@@ -136,7 +146,7 @@ __device__ __forceinline__ RowStatics<typename F::R> row_statics(const StageArgs
                : Fld<ST>{&ar, A.cont == CONT_RELAX ? A.st.eta_ref : A.st.Feta}.template ldr<R>(vxT, rowb<ST>(y, P));
     s.qend = !(PER && SCH == ADV_AH) ? R(0)
              : KEEP ? Fld<ST>{&ar, A.qs}.template ldr<R>(colb<ST>(A.nx), rowb<ST>(y, P))
-                    : Fld<ST>{&ar, A.qend}.template ldr<R>(0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST));
+                    : ld_qend<F>(ar, A.qend, y);
     s.fu = SCH == ADV_LIN ? Fld<ST>{&ar, A.st.fu}.template ldr<R>(vfu, srowb<ST>(y, P, A.st.fu_2d)) : R(0);
     s.fv = SCH == ADV_LIN ? Fld<ST>{&ar, A.st.fv}.template ldr<R>(vfv, srowb<ST>(y - 1, P, A.st.fv_2d)) : R(0);
     s.qs = KEEP ? Fld<ST>{&ar, A.qs}.template ldr<R>(vxT, rowb<ST>(y, P)) : R(0);
@@ -1020,7 +1030,8 @@ __device__ __forceinline__ void qend_body(const StageArgs<F> &A, int y) {
     if (y > A.ny) return;
     const Arena ar(A.ar);
     const Fld<SI> u{&ar, A.in_u}, v{&ar, A.in_v}, e{&ar, A.in_e};
-    const Fld<ST> H{&ar, A.st.H}, fq{&ar, A.st.fq}, qe{&ar, A.qend};
+    const Fld<ST> H{&ar, A.st.H}, fq{&ar, A.st.fq};
+    const Fld<QES<F>> qe{&ar, A.qend};
     const R quarter = R(0.25);
     const int x = A.nx, P = A.P;
     const uint32_t c0 = colb<SI>(x), c1 = colb<SI>(x + 1), r0 = rowb<SI>(y, P), r1 = rowb<SI>(y + 1, P);
@@ -1033,7 +1044,7 @@ __device__ __forceinline__ void qend_body(const StageArgs<F> &A, int y) {
     const R dvdx = v.template ldr<R>(c1, r0) - v.template ldr<R>(c0, r0);
     const R dudy = u.template ldr<R>(c0, r1) - u.template ldr<R>(c0, r0);
     const R f = fq.template ldr<R>(scolb<ST>(x, A.st.fq_2d), srowb<ST>(y, P, A.st.fq_2d));
-    qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(ST));
+    qe.st((f + dvdx - dudy) / hq, 0, (uint32_t)(y + OY) * (uint32_t)sizeof(QES<F>));
 }
 
 template <class F, class SI>

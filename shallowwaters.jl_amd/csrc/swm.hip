@@ -49,8 +49,13 @@ int check_config(swm_ctx *ctx) {
     if (c.nstep_diff < 1 || (c.tracer_advection && c.nadvstep < 1)) return ctx->fail(SWM_E_ARG, "nstep_diff/nadvstep must be >= 1");
     if (c.bottom_drag < 0 || c.bottom_drag > 2 || c.diffusion < 0 || c.diffusion > 1)
         return ctx->fail(SWM_E_ARG, "bottom_drag %d / diffusion %d", c.bottom_drag, c.diffusion);
+    if (c.t_arith32 != 0 && c.t_arith32 != 1) return ctx->fail(SWM_E_ARG, "t_arith32 %d", c.t_arith32);
+    if (c.t_arith32 && !mixed)
+        return ctx->fail(SWM_E_UNSUPPORTED, "t_arith32 needs T in {Float16, BFloat16} with Tprog = Float32");
+    if (c.t_arith32 && c.nstep_advcor != 0)
+        return ctx->fail(SWM_E_UNSUPPORTED, "t_arith32 with nstep_advcor > 0 (q and KE kept in T) is not implemented");
     ctx->mixed = mixed;
-    ctx->fmt = c.T;
+    ctx->fmt = c.t_arith32 ? (c.T == SWM_F16 ? FMT_F16A : FMT_BF16A) : c.T;
     return SWM_OK;
 }
 
@@ -206,7 +211,7 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
                    ((adv_form(c) & ADV_KEEP) ? 2 : 0);
     const size_t n1d = (size_t)c.ny + 2 * OY + 16;
     ctx->arena_bytes = (size_t)nq * (ctx->elems * ctx->es + 256) + (size_t)nt * (ctx->elems * ctx->est + 256) +
-                       n1d * ctx->est + 256;
+                       n1d * (c.t_arith32 ? 4 : ctx->est) + 256;
     if (ctx->arena_bytes > 0xFFFF0000ull) {     // byte offsets into the arena are 32-bit
         ctx->fail(SWM_E_UNSUPPORTED, "grid %dx%d needs %zu bytes of device fields; one context addresses at most 4 GiB",
                   c.nx, c.ny, ctx->arena_bytes);
@@ -231,7 +236,7 @@ int swm_create(const swm_config *cfg, swm_ctx **out) {
         (rc = alloc_field(ctx, ctx->um, 0, true, est)) || (rc = alloc_field(ctx, ctx->vm, 0, true, est)) ||
         (rc = alloc_field(ctx, ctx->H, 0, true, est)) || (rc = alloc_field(ctx, ctx->Fx, 0, true, est)) ||
         (rc = alloc_field(ctx, ctx->Fy, 0, true, est)) || (rc = alloc_field(ctx, ctx->fq, 0, true, est)) ||
-        (rc = alloc_field(ctx, ctx->qend, n1d, false, est)))
+        (rc = alloc_field(ctx, ctx->qend, n1d, false, c.t_arith32 ? 4 : est)))
         return bail(rc);
     if (c.compensated && ((rc = alloc_field(ctx, ctx->cu, 0, true, es)) || (rc = alloc_field(ctx, ctx->cv, 0, true, es)) ||
                           (rc = alloc_field(ctx, ctx->ce, 0, true, es))))

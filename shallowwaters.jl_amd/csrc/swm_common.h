@@ -53,6 +53,8 @@ constexpr int NXCD = 8;      // accelerator complex dies of an MI355X, each with
 // ------------------------------------------------------------------ number formats
 
 enum { FMT_F64 = 0, FMT_F32 = 1, FMT_F16 = 2, FMT_BF16 = 3 };   // the SWM_F64 ... codes
+// the library's bundles of Float16 / BFloat16 values with float arithmetic (swm_config.t_arith32)
+enum { FMT_F16A = 4, FMT_BF16A = 5 };
 
 struct h16 { uint16_t b; };  // storage of an IEEE binary16 value
 struct b16 { uint16_t b; };  // storage of a bfloat16 value
@@ -113,10 +115,38 @@ template <int K> struct lo {
     __device__ __forceinline__ friend bool operator>(lo a, lo b) { return a.v > b.v; }
 };
 
+// Binary16 / bfloat16 values with float arithmetic (K = FMT_F16 / FMT_BF16; a form of this
+// library, not of the reference): every value that enters T (the stage input converted from
+// Tprog, the static fields, the T constants) is rounded to the format, the arithmetic on
+// them is float's, each operation rounded once to float, and its results are not rounded
+// back to the format.
+template <int K> struct ar {
+    float v;
+    __device__ __forceinline__ static float rnd(float x) { return K == FMT_F16 ? round_f16(x) : round_bf16(x); }
+    __device__ __forceinline__ static ar raw(float x) { ar r; r.v = x; return r; }
+    ar() = default;
+    __device__ __forceinline__ ar(int i) : v(rnd((float)i)) {}
+    __device__ __forceinline__ explicit ar(double d) : v(rnd((float)d)) {}
+    __device__ __forceinline__ friend ar operator+(ar a, ar b) { return raw(a.v + b.v); }
+    __device__ __forceinline__ friend ar operator-(ar a, ar b) { return raw(a.v - b.v); }
+    __device__ __forceinline__ friend ar operator*(ar a, ar b) { return raw(a.v * b.v); }
+    __device__ __forceinline__ friend ar operator/(ar a, ar b) { return raw(a.v / b.v); }
+    __device__ __forceinline__ friend ar operator-(ar a) { return raw(-a.v); }
+    __device__ __forceinline__ ar &operator+=(ar b) { v = v + b.v; return *this; }
+    __device__ __forceinline__ friend bool operator<(ar a, ar b) { return a.v < b.v; }
+    __device__ __forceinline__ friend bool operator>(ar a, ar b) { return a.v > b.v; }
+};
+template <class R> struct is_ar : std::false_type {};
+template <int K> struct is_ar<ar<K>> : std::true_type {};
+
 // sqrt, floor and the finiteness test of every arithmetic type (IEEE sqrt, correctly rounded)
 __device__ __forceinline__ float nsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double nsqrt(double x) { return ::sqrt(x); }
 template <int K> __device__ __forceinline__ lo<K> nsqrt(lo<K> a) { return lo<K>::raw(lo<K>::rnd(sqrtf(a.v))); }
+template <int K> __device__ __forceinline__ ar<K> nsqrt(ar<K> a) { return ar<K>::raw(sqrtf(a.v)); }
+template <int K> __device__ __forceinline__ ar<K> nfloor(ar<K> a) { return ar<K>::raw(floorf(a.v)); }
+template <int K> __device__ __forceinline__ bool nfinite(ar<K> a) { return __builtin_isfinite(a.v); }
+template <int K> __device__ __forceinline__ float wide(ar<K> x) { return x.v; }
 __device__ __forceinline__ float nfloor(float x) { return floorf(x); }
 __device__ __forceinline__ double nfloor(double x) { return ::floor(x); }
 template <int K> __device__ __forceinline__ lo<K> nfloor(lo<K> a) { return lo<K>::raw(floorf(a.v)); }
@@ -148,6 +178,11 @@ using FmtF64 = Fmt<double, double, double>;
 using FmtF32 = Fmt<float, float, float>;
 using FmtF16 = Fmt<lo<FMT_F16>, float, h16>;     // T = Float16, Tprog = Float32
 using FmtBF16 = Fmt<lo<FMT_BF16>, float, b16>;   // T = BFloat16, Tprog = Float32
+using FmtF16A = Fmt<ar<FMT_F16>, float, h16>;    // Float16 values, float arithmetic, Tprog = Float32
+using FmtBF16A = Fmt<ar<FMT_BF16>, float, b16>;  // BFloat16 values, float arithmetic, Tprog = Float32
+// storage of q at the periodic wrap column (a T intermediate): T's storage, float where T's
+// arithmetic is float's
+template <class F> using QES = std::conditional_t<is_ar<typename F::R>::value, float, typename F::ST>;
 
 // Tprog(x) of a T value (exact widening) and T(x) of a Tprog value (one rounding)
 template <class Q, class R> __device__ __forceinline__ Q toQ(R r) {
@@ -175,6 +210,8 @@ __device__ __forceinline__ float comm_round(float x, int cm) {
     return x;
 }
 template <int K> __device__ __forceinline__ lo<K> comm_round(lo<K> x, int) { return x; }   // Tcomm is T or Tprog
+// Tcomm is T or wider: T(Tcomm(x)) of a T value is x (as the reference's copy between equal formats)
+template <int K> __device__ __forceinline__ ar<K> comm_round(ar<K> x, int) { return x; }
 
 // ------------------------------------------------------------------ field access
 
@@ -239,8 +276,8 @@ template <class S, class X> __device__ __forceinline__ typename Sto<S>::V enc(X 
     if constexpr (std::is_same_v<S, double>) return (double)wide(x);
     else if constexpr (std::is_same_v<S, float>) return (float)wide(x);
     else if constexpr (std::is_same_v<X, lo<FMT_F16>> || std::is_same_v<X, lo<FMT_BF16>>) return x.v;
-    else if constexpr (std::is_same_v<S, h16>) return round_f16(x);
-    else return round_bf16(x);
+    else if constexpr (std::is_same_v<S, h16>) return round_f16(wide(x));
+    else return round_bf16(wide(x));
 }
 // arithmetic value R of a stored S value (exact for T-format storage, one rounding from Tprog)
 template <class R, class S> __device__ __forceinline__ R dec(typename Sto<S>::V v) {
@@ -301,6 +338,8 @@ template <int K> __device__ __forceinline__ lo<K> sh_xm(lo<K> v) {
     if constexpr (K == FMT_F16) return lo<K>::raw(dpp_h<DPP_WAVE_SHR1>(v.v));
     else return lo<K>::raw(sh_xm(v.v));
 }
+template <int K> __device__ __forceinline__ ar<K> sh_xp(ar<K> v) { return ar<K>::raw(sh_xp(v.v)); }
+template <int K> __device__ __forceinline__ ar<K> sh_xm(ar<K> v) { return ar<K>::raw(sh_xm(v.v)); }
 
 // ------------------------------------------------------------------ static fields
 

@@ -120,7 +120,8 @@ struct swm_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;   // periodic: the seam strips of a fused pair, beside the others
     hipEvent_t fork = nullptr, join = nullptr;
-    int fmt = SWM_F32;     // format bundle: SWM_F64, SWM_F32 (T = Tprog), SWM_F16, SWM_BF16 (T, Tprog = F32)
+    int fmt = SWM_F32;     // format bundle: SWM_F64, SWM_F32 (T = Tprog), SWM_F16, SWM_BF16 (T, Tprog = F32),
+                           // FMT_F16A, FMT_BF16A (T's values, float arithmetic, Tprog = F32)
     int es = 4;            // element size of Tprog fields
     int est = 4;           // element size of T fields
     bool mixed = false;
@@ -227,7 +228,7 @@ inline int get2d_widen(swm_ctx *ctx, const Field &f, void *host, int m, int n, i
     float *out = (float *)host;
     for (size_t k = 0; k < tmp.size(); ++k) {
         uint32_t b;
-        if (ctx->fmt == SWM_BF16) b = (uint32_t)tmp[k] << 16;
+        if (ctx->c.T == SWM_BF16) b = (uint32_t)tmp[k] << 16;
         else {                                     // binary16 -> binary32
             const uint32_t h = tmp[k], s = (h >> 15) << 31, ex = (h >> 10) & 0x1f, mt = h & 0x3ff;
             if (ex == 0) {
@@ -462,11 +463,15 @@ const FmtOps &ops_f64();
 const FmtOps &ops_f32();
 const FmtOps &ops_f16();
 const FmtOps &ops_bf16();
+const FmtOps &ops_f16a();
+const FmtOps &ops_bf16a();
 inline const FmtOps &fmt_ops(int fmt) {
     switch (fmt) {
     case SWM_F64: return ops_f64();
     case SWM_F32: return ops_f32();
     case SWM_F16: return ops_f16();
+    case FMT_F16A: return ops_f16a();
+    case FMT_BF16A: return ops_bf16a();
     default: return ops_bf16();
     }
 }

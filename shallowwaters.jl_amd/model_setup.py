@@ -54,7 +54,7 @@ class swm_config(ctypes.Structure):
         ("half_dtadvu", ctypes.c_double), ("half_dtadvv", ctypes.c_double),
         ("jSST", ctypes.c_double), ("SSTmin", ctypes.c_double),
         ("omegaFx", ctypes.c_double), ("omegaFy", ctypes.c_double), ("omegaFeta", ctypes.c_double),
-        ("compensated", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("compensated", ctypes.c_int32), ("t_arith32", ctypes.c_int32),
     ]
 
 
@@ -105,4 +105,5 @@ def make_config(S: ModelSetup) -> swm_config:
     c.omegaFx, c.omegaFy, c.omegaFeta = C.ωFx, C.ωFy, C.ωFη
     c.scale_log2 = int(round(math.log2(P.scale)))
     c.compensated = int(P.compensated)
+    c.t_arith32 = int(P.arithmetic == "Float32")
     return c

@@ -103,6 +103,10 @@ class Parameter:
     # the RK increments apart and adds them with Kahan summation (classic RK only)
     scale: float = 1.0
     compensated: bool = False
+    # arithmetic="Float32" with T Float16 / BFloat16 and Tprog Float32: T's values (stage
+    # inputs, static fields, T constants) stay in T, the right-hand side is computed in Float32
+    # and its results are not rounded back to T (None: the arithmetic of T itself)
+    arithmetic: object = None
 
     # TRACER ADVECTION
     tracer_advection: bool = True
@@ -192,6 +196,12 @@ class Parameter:
         assert self.scale > 0 and k == int(k) and -20 <= k <= 20, \
             f"scale has to be a power of two in 2^-20 .. 2^20, {self.scale} given."
         assert not self.compensated or self.time_scheme == "RK", "compensated requires the classic RK scheme"
+        if self.arithmetic is not None:
+            self.arithmetic = canonical_type(self.arithmetic)
+            assert self.arithmetic == "Float32" and self.T in ("Float16", "BFloat16") and self.Tprog == "Float32", \
+                "arithmetic='Float32' applies to T in {Float16, BFloat16} with Tprog = Float32"
+            assert self.time_scheme == "RK" and self.nstep_advcor == 0, \
+                "arithmetic='Float32' is implemented for the classic RK scheme with nstep_advcor = 0"
         # rejected here although the reference accepts them (see module docstring)
         assert self.wind_forcing_y in ["constant", "none"], \
             f"wind_forcing_y '{self.wind_forcing_y}' builds Fy on the u-grid; only 'constant' and 'none' are supported"

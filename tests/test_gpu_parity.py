@@ -78,6 +78,21 @@ CASES = {
     "per-bf16-f32-scaled": dict(T="BFloat16", Tprog="Float32", nx=66, L_ratio=1.5, bc="periodic",
                                 diffusion="Smagorinsky", bottom_drag="quadratic", topography="seamount",
                                 wind_forcing_x="channel", scale=64.0, tracer_consumption=True),
+    # Float16 / BFloat16 values with Float32 arithmetic (arithmetic="Float32", a form of this
+    # library): the stage inputs, statics and constants in T, the right-hand side in Float32
+    "np-bf16a-smag-quad": dict(T="BFloat16", Tprog="Float32", arithmetic="Float32", nx=64, L_ratio=1.0,
+                               bc="nonperiodic", α=0.5, diffusion="Smagorinsky", bottom_drag="quadratic",
+                               topography="seamount", wind_forcing_x="double_gyre"),
+    "per-bf16a-comm-bf16": dict(T="BFloat16", Tprog="Float32", Tcomm="BFloat16", arithmetic="Float32", nx=66,
+                                L_ratio=1.5, bc="periodic", diffusion="Smagorinsky", bottom_drag="quadratic",
+                                topography="seamount", wind_forcing_x="channel"),
+    "per-f16a-flat-lin-rk3": dict(T="Float16", Tprog="Float32", Tcomm="Float16", arithmetic="Float32", nx=62,
+                                  L_ratio=1.0, bc="periodic", diffusion="constant", bottom_drag="linear",
+                                  topography="flat", RKo=3, cfl=0.6),
+    "np-bf16a-scaled-comp-relax": dict(T="BFloat16", Tprog="Float32", arithmetic="Float32", nx=64, L_ratio=1.0,
+                                       bc="nonperiodic", surface_relax=True, t_relax=2.0, topography="seamount",
+                                       wind_forcing_x="double_gyre", scale=64.0, compensated=True,
+                                       tracer_consumption=True),
     "np-f32-comp-rk3": dict(T="Float32", nx=61, L_ratio=0.8, bc="nonperiodic", α=0.5, diffusion="Smagorinsky",
                             topography="seamount", wind_forcing_x="double_gyre", compensated=True, RKo=3, cfl=0.6),
     "per-f64-scaled-comp-ssp": dict(T="Float64", nx=60, L_ratio=1.0, bc="periodic", scale=64.0,

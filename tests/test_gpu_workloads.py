@@ -134,3 +134,33 @@ def test_adjoint_1024_dot_product_with_the_tracer(swm, orc, gpu):
     rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, (g.u, g.v, g.η, g.sst)))
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dx, ȳ> = {lhs!r}, <dx, J^T ȳ> = {rhs!r}"
     assert float(np.sum(dy[3] * ybar[3])) != 0.0                 # the tracer's tangent took part
+
+
+def test_float32_arithmetic_forms_against_float64(swm, gpu):
+    """BFloat16 / Float16 values with Float32 arithmetic against the Float64 run of the same
+    seeded double gyre (512², one model day, tracer on).  Bounds: the mean surface height of
+    the closed basin (conserved exactly in exact arithmetic) moves by at most 1e-8 m, and by at
+    most 1e-3 of what the reference's BFloat16 arithmetic moves it; the relative L2 error of η
+    is at most half the BFloat16 arithmetic's and below 0.1."""
+    def run(T, **kw):
+        S = swm.make_setup(swm.workloads.double_gyre(nx=512, T=T, Tprog="Float64" if T == "Float64" else "Float32",
+                                                     **kw))
+        prog = swm.seeded_initial_conditions(S, seed=2024, σ=0.1)
+        n = int(np.ceil(86400 / S.grid.dtint))
+        with swm.Simulation(S) as sim:
+            sim.upload(prog)
+            m0, _ = sim.reduce()
+            assert sim.step(n, 1, 0) == (n, 0)
+            m1, _ = sim.reduce()
+            y = sim.download()
+        e = swm.to_float64(y.η, S.parameters.Tprog)[1:-1, 1:-1]
+        return e, (m1 - m0) / (S.grid.nx * S.grid.ny)
+    e64, _ = run("Float64")
+    rel = lambda e: float(np.linalg.norm(e - e64) / np.linalg.norm(e64))
+    e_b, dm_b = run("BFloat16")
+    for T in ("BFloat16", "Float16"):
+        e, dm = run(T, arithmetic="Float32")
+        print(f"{T} values, Float32 arithmetic: mass drift {dm:.3e} m (BFloat16 arithmetic {dm_b:.3e}), "
+              f"η rel L2 {rel(e):.3e} (BFloat16 arithmetic {rel(e_b):.3e})", flush=True)
+        assert abs(dm) <= 1e-8 and abs(dm) <= 1e-3 * abs(dm_b), (T, dm, dm_b)
+        assert rel(e) <= 0.5 * rel(e_b) and rel(e) < 0.1, (T, rel(e), rel(e_b))
