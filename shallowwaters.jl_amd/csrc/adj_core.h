@@ -639,6 +639,7 @@ template <class R> struct TracerAdjArgs {
     uint32_t lu, lv, lum, lvm;      // accumulated λ of u0, v0, um, vm (all cells)
     uint32_t rs[4], ru[4], rv[4];   // records of the sst, um_T, vm_T interpolations: λ, x0, y0, corner
     uint32_t luT, lvT;              // λ of u_T, v_T
+    uint32_t gu, gv;                // the um_T / vm_T records gathered at each corner (second pass)
     R hdu, hdv, du, dv, jSST;
     int consume, nx, ny, P, ep;
 };
@@ -773,20 +774,27 @@ SWM_HD R tracer_gather(const C &c, const uint32_t rec[4], int X, int Y, int rx, 
     return acc;
 }
 
-// second pass, one thread per cell (X, Y) of the u/v/η/sst layouts (X = -2 .. nx+2,
-// Y = -1 .. ny+2): λ of the old tracer, and accumulation into λu0, λv0 (through u_T, v_T)
-// and λum, λvm (through um_T, vm_T)
+// Second pass, in two parts.  (a) one thread per corner (X, Y), X = -2 .. nx+3,
+// Y = -1 .. ny+3: λ of the old tracer, and the records of the um_T and vm_T interpolations
+// gathered at the corner (kept in gu, gv: each is read at two corners by (b)).
 template <class R, bool PER, class C>
-SWM_HD void tracer_adj_gather(const C &c, const TracerAdjArgs<R> &A, int X, int Y, int rx, int ry) {
+SWM_HD void tracer_adj_gather_uv(const C &c, const TracerAdjArgs<R> &A, int X, int Y, int rx, int ry) {
     const int nx = A.nx, ny = A.ny;
-    const R half = R(0.5);
     if (X >= 0 && X <= nx + 1 && Y >= 1 && Y <= ny)
         c.at(A.lsst_out, X, Y) = tracer_gather<R, PER>(c, A.rs, X, Y, rx, ry, nx, ny);
-    const R lumT0 = tracer_gather<R, PER>(c, A.ru, X, Y, rx + 1, ry + 1, nx, ny);
-    const R lumT1 = tracer_gather<R, PER>(c, A.ru, X + 1, Y, rx + 1, ry + 1, nx, ny);
+    c.at(A.gu, X, Y) = tracer_gather<R, PER>(c, A.ru, X, Y, rx + 1, ry + 1, nx, ny);
+    c.at(A.gv, X, Y) = tracer_gather<R, PER>(c, A.rv, X, Y, rx + 1, ry + 1, nx, ny);
+}
+
+// (b) one thread per cell (X, Y) of the u/v/η/sst layouts (X = -2 .. nx+2, Y = -1 .. ny+2):
+// accumulation into λu0, λv0 (through u_T, v_T) and λum, λvm (through um_T, vm_T)
+template <class R, bool PER, class C>
+SWM_HD void tracer_adj_gather(const C &c, const TracerAdjArgs<R> &A, int X, int Y) {
+    const int nx = A.nx, ny = A.ny;
+    const R half = R(0.5);
+    const R lumT0 = c.at(A.gu, X, Y), lumT1 = c.at(A.gu, X + 1, Y);
     c.at(A.lum, X, Y) = c.at(A.lum, X, Y) + half * (lumT0 + lumT1);
-    const R lvmT0 = tracer_gather<R, PER>(c, A.rv, X, Y, rx + 1, ry + 1, nx, ny);
-    const R lvmT1 = tracer_gather<R, PER>(c, A.rv, X, Y + 1, rx + 1, ry + 1, nx, ny);
+    const R lvmT0 = c.at(A.gv, X, Y), lvmT1 = c.at(A.gv, X, Y + 1);
     c.at(A.lvm, X, Y) = c.at(A.lvm, X, Y) + half * (lvmT0 + lvmT1);
     auto inT = [&](int x, int y) { return x >= 1 && x <= nx && y >= 1 && y <= ny; };
     const R a0 = inT(X, Y) ? c.at(A.luT, X, Y) : R(0), a1 = inT(X + 1, Y) ? c.at(A.luT, X + 1, Y) : R(0);
@@ -892,6 +900,7 @@ struct AdjFields {
     int fq_2d;
     uint32_t qend, lqend;
     uint32_t tr0, tr1;
+    uint32_t tg0, tg1;              // the tracer adjoint's gathered interpolation records
     uint32_t lfx;                   // λ of Fx at the u points (accumulated over the reversed steps)
 };
 
@@ -922,7 +931,7 @@ template <class R, class K> int reverse_tracer(K &k, AdjFields &F, const AdjCons
             T.lu = F.L.u; T.lv = F.L.v; T.lum = F.Lum; T.lvm = F.Lvm;
             const uint32_t rec[12] = {F.Z.u, F.Z.v, F.Z.e, F.Ma.u, F.Ma.v, F.Ma.e, F.Mb.u, F.Mb.v, F.Mb.e, F.A.u, F.A.v, F.A.e};
             for (int j = 0; j < 4; ++j) { T.rs[j] = rec[j]; T.ru[j] = rec[4 + j]; T.rv[j] = rec[8 + j]; }
-            T.luT = F.tr0; T.lvT = F.tr1;
+            T.luT = F.tr0; T.lvT = F.tr1; T.gu = F.tg0; T.gv = F.tg1;
             T.hdu = c.hdu; T.hdv = c.hdv; T.du = c.du; T.dv = c.dv; T.jSST = c.jSST;
             T.consume = c.consume; T.nx = c.nx; T.ny = c.ny; T.P = c.P; T.ep = c.ep;
             if ((rc = k.tracer(T))) return rc;

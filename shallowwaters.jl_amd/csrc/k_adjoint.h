@@ -89,21 +89,48 @@ template <class R> __global__ void qend_adj_kernel(ArenaRef ar, adj::StageAdjArg
     adj::qend_adjoint_row<R>(GpuCells<R>{(char *)ar.base, A.P}, A, mu, yy);
 }
 
-// first tracer pass, one thread per T-cell; dmax[0] (dmax[1]) receives the largest x (y)
-// distance between a cell and the corners it interpolates from
-template <class R, bool PER> __global__ void tracer_src_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, int *dmax) {
-    const int x = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x), y = 1 + (int)blockIdx.y;
-    if (x > A.nx) return;
-    const int d = adj::tracer_adj_source<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, x, y);
-    atomicMax(dmax, d & 0xFFFF);
-    atomicMax(dmax + 1, d >> 16);
+// first tracer pass, one thread per T-column over `rows` rows; dmax[0] (dmax[1]) receives the
+// largest x (y) distance between a cell and the corners it interpolates from.  Atomics on
+// one address are serialised (about 10 ns each), so the block takes the maxima of its
+// cells first and issues one pair: about two thousand blocks whatever the grid.
+template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A,
+                                                                                      int rows, int *dmax) {
+    __shared__ int sm[2][256 / WAVE];
+    const int x = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y0 = 1 + (int)blockIdx.y * rows, y1 = min(A.ny, y0 + rows - 1);
+    int dx = 0, dy = 0;
+    if (x <= A.nx)
+        for (int y = y0; y <= y1; ++y) {
+            const int d = adj::tracer_adj_source<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, x, y);
+            dx = max(dx, d & 0xFFFF);
+            dy = max(dy, d >> 16);
+        }
+    for (int o = WAVE / 2; o > 0; o >>= 1) {          // every lane takes part
+        dx = max(dx, __shfl_xor(dx, o));
+        dy = max(dy, __shfl_xor(dy, o));
+    }
+    const int w = (int)threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) { sm[0][w] = dx; sm[1][w] = dy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)blockDim.x / WAVE; ++k) { dx = max(dx, sm[0][k]); dy = max(dy, sm[1][k]); }
+        atomicMax(dmax, dx);
+        atomicMax(dmax + 1, dy);
+    }
 }
 
-// second pass, one thread per cell X = -2 .. nx+2, Y = -1 .. ny+2
-template <class R, bool PER> __global__ void tracer_gather_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
+// second pass (a), one thread per corner X = -2 .. nx+3, Y = -1 .. ny+3
+template <class R, bool PER> __global__ void tracer_gather_uv_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
+    const int X = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x), Y = -1 + (int)blockIdx.y;
+    if (X > A.nx + 3) return;
+    adj::tracer_adj_gather_uv<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, X, Y, dmax[0], dmax[1]);
+}
+
+// second pass (b), one thread per cell X = -2 .. nx+2, Y = -1 .. ny+2
+template <class R, bool PER> __global__ void tracer_gather_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A) {
     const int X = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x), Y = -1 + (int)blockIdx.y;
     if (X > A.nx + 2) return;
-    adj::tracer_adj_gather<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, X, Y, dmax[0], dmax[1]);
+    adj::tracer_adj_gather<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, X, Y);
 }
 
 template <class R> __global__ void snapshot_adj_kernel(R *lu, R *lum, R *lv, R *lvm, size_t n) {

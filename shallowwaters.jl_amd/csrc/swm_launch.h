@@ -727,7 +727,7 @@ template <class Fm> struct Adj {
         if (ctx->adj) return SWM_OK;
         int rc = check(ctx);
         if (rc) return rc;
-        const int nf2 = 6 + 22;
+        const int nf2 = 6 + 24;
         const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
         if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
         AdjState *st = new AdjState();
@@ -748,6 +748,7 @@ template <class Fm> struct Adj {
         uve(F.L); uve(F.Z); uve(F.Ma); uve(F.Mb); uve(F.A);
         one(F.Lsst, true); one(F.Lsst2, true); one(F.Lum, true); one(F.Lvm, true);
         one(F.tr0, true); one(F.tr1, true);
+        one(F.tg0, true); one(F.tg1, true);
         one(F.lfx, true);
         one(F.lqend, false);
         if (rc) {
@@ -846,13 +847,17 @@ template <class Fm> struct Adj {
         }
         int tracer(const adj::TracerAdjArgs<R> &T) {
             hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
-            const dim3 g1((T.nx + 127) / 128, T.ny), g2((T.nx + 5 + 127) / 128, T.ny + 4);
+            // the first pass: 256 columns per block, rows per thread so that about 2048 blocks run
+            const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 2047) / 2048);
+            const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + 127) / 128, T.ny + 5), g2((T.nx + 5 + 127) / 128, T.ny + 4);
             if (T.ep) {
-                hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_kernel<R, true>), ga, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             } else {
-                hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_kernel<R, false>), ga, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             }
             return fail_launch();
         }

@@ -257,8 +257,10 @@ template <class R> struct EmuK {
             }
         dmax_x = dx;
         dmax_y = dy;
+        for (int Y = -1; Y <= T.ny + 3; ++Y)
+            for (int X = -2; X <= T.nx + 3; ++X) tracer_adj_gather_uv<R, PER>(c, T, X, Y, dx, dy);
         for (int Y = -1; Y <= T.ny + 2; ++Y)
-            for (int X = -2; X <= T.nx + 2; ++X) tracer_adj_gather<R, PER>(c, T, X, Y, dx, dy);
+            for (int X = -2; X <= T.nx + 2; ++X) tracer_adj_gather<R, PER>(c, T, X, Y);
     }
     int tracer(const TracerAdjArgs<R> &T) {
         if (T.ep) tracer_t<true>(T);
@@ -363,6 +365,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
     F.H = A.add(true); F.fq = A.add(true); F.fq_2d = 1;
     F.qend = A.add(false); F.lqend = A.add(false);
     F.tr0 = A.add(true); F.tr1 = A.add(true);
+    F.tg0 = A.add(true); F.tg1 = A.add(true);
     try {
         if (op == 1) {
             put<R>(A, F.s[0].u, in[0], sh.u); put<R>(A, F.s[0].v, in[1], sh.v); put<R>(A, F.s[0].e, in[2], sh.e);
