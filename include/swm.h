@@ -207,6 +207,13 @@ int swm_reduce_noalloc(swm_ctx *ctx, double *mass, double *energy);
  * (nonperiodic).  Operates on the current buffers; the prognostic state is left unchanged
  * except the RK scratch slots. */
 int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg);
+/* Measurement hook (no counterpart in the reference): run steps i0 .. i0+nsteps-1 as
+ * swm_step does, with HIP events recorded around each kernel launch, and return per kind of
+ * kernel the average duration (ms) and the number of launches timed: [0] fused stages 1+2,
+ * [1] fused stages 3+4, [2] dissipation, [3] tracer advection, [4] the whole step, [5] first
+ * stage, [6] middle stage, [7] last stage (one kernel per stage).  ms_avg and counts hold 8
+ * entries; kinds that did not run give 0.  The state advances as by swm_step. */
+int swm_time_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, double *ms_avg, int32_t *counts);
 
 /* ---- Ensembles (no counterpart in the reference, whose "ensemble mode" runs separate
  * processes with run ids, README.md:88, src/output.jl:168-207).  An ensemble advances

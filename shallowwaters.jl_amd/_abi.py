@@ -17,7 +17,7 @@ EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
            "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
            "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer",
-           "swm_adj_track_Fx", "swm_adj_download_Fx", "swm_set_static_device", "swm_download_static"]
+           "swm_adj_track_Fx", "swm_adj_download_Fx", "swm_set_static_device", "swm_download_static", "swm_time_steps"]
 
 TOPOGRAPHY = {"flat": 0, "ridge": 1, "ridges": 2, "seamount": 3}
 WIND = {"channel": 0, "shear": 1, "double_gyre": 2, "constant": 3, "none": 4}
@@ -78,6 +78,7 @@ def load():
     L.swm_adj_download_Fx.argtypes = [vp, vp]
     L.swm_set_static_device.argtypes = [vp, ctypes.POINTER(SwmSetup)]
     L.swm_download_static.argtypes = [vp] * 7
+    L.swm_time_steps.argtypes = [vp, i64, i64, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
     for name in EXPORTS:
         if name not in ("swm_last_error", "swm_ens_last_error"):
             getattr(L, name).restype = ctypes.c_int

@@ -552,6 +552,36 @@ int swm_time_kernel(swm_ctx *ctx, int32_t which, int32_t reps, double *ms_avg) {
     return fmt_ops(ctx->fmt).time_kernel(ctx, which, reps, ms_avg);
 }
 
+int swm_time_steps(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, double *ms_avg, int32_t *counts) {
+    if (!ctx || !ms_avg || !counts) return SWM_E_ARG;
+    if (int rc = step_ready(ctx)) return rc;
+    if (nsteps < 1 || i0 < 1) return ctx->fail(SWM_E_ARG, "nsteps >= 1 and i0 >= 1 required");
+    StepTrace tr;
+    ctx->strace = &tr;
+    int64_t done = 0, nan = 0;
+    int rc = fmt_ops(ctx->fmt).run_steps(ctx, nsteps, i0, t0, &done, &nan);
+    ctx->strace = nullptr;
+    const bool synced = hipStreamSynchronize(ctx->stream) == hipSuccess;
+    for (int k = 0; k < KT_KINDS; ++k) {
+        double sum = 0.0;
+        int n = 0;
+        for (size_t q = 0; q + 1 < tr.ev[k].size(); q += 2) {
+            float ms = 0.0f;
+            if (synced && hipEventElapsedTime(&ms, tr.ev[k][q], tr.ev[k][q + 1]) == hipSuccess) {
+                sum += ms;
+                ++n;
+            }
+        }
+        for (hipEvent_t e : tr.ev[k]) (void)hipEventDestroy(e);
+        ms_avg[k] = n ? sum / n : 0.0;
+        counts[k] = n;
+    }
+    if (rc) return rc;
+    if (!synced) return ctx->fail(SWM_E_HIP, "stream synchronize");
+    if (nan) return ctx->fail(SWM_E_STATE, "NaN at step %lld", (long long)nan);
+    return SWM_OK;
+}
+
 int swm_reduce(swm_ctx *ctx, double *mass, double *energy) {
     if (!ctx || !mass || !energy) return SWM_E_ARG;
     if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");

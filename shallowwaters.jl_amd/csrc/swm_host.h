@@ -112,9 +112,18 @@ struct Recorder {
     }
 };
 
+// In-step kernel timing (swm_time_steps): HIP events recorded around each launch of the
+// step's kernels, per kind of kernel, begin and end alternately.
+enum { KT_FUSED12 = 0, KT_FUSED34, KT_DISSIPATION, KT_TRACER, KT_STEP, KT_STAGE_FIRST, KT_STAGE_MID, KT_STAGE_LAST,
+       KT_KINDS };
+struct StepTrace {
+    std::vector<hipEvent_t> ev[KT_KINDS];
+};
+
 struct swm_ctx {
     swm_config c;
     Recorder *rec = nullptr;   // set while an ensemble plans a step
+    StepTrace *strace = nullptr;   // set while swm_time_steps runs
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -465,6 +474,27 @@ const FmtOps &ops_f16();
 const FmtOps &ops_bf16();
 const FmtOps &ops_f16a();
 const FmtOps &ops_bf16a();
+// records an event on the context's stream when constructed and when destroyed, into kind k
+// of the context's step trace (nothing when no trace runs or an ensemble records a plan)
+struct KTrace {
+    swm_ctx *ctx;
+    int k;
+    static void mark(swm_ctx *c, int kind) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, c->stream);
+        c->strace->ev[kind].push_back(e);
+    }
+    KTrace(swm_ctx *c, int kind) : ctx(c->strace && !c->rec ? c : nullptr), k(kind) {
+        if (ctx) mark(ctx, k);
+    }
+    ~KTrace() {
+        if (ctx) mark(ctx, k);
+    }
+    KTrace(const KTrace &) = delete;
+    KTrace &operator=(const KTrace &) = delete;
+};
+
 inline const FmtOps &fmt_ops(int fmt) {
     switch (fmt) {
     case SWM_F64: return ops_f64();

@@ -147,6 +147,8 @@ template <class F> struct Launch {
             else hipLaunchKernelGGL((qend_kernel<F, ST>), qg, qb, 0, ctx->stream, A);
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
+        const bool first = m == M_RK_FIRST || m == M_RKC_FIRST, last = m == M_RK_LAST || m == M_RKC_LAST;
+        const KTrace kt(ctx, first ? KT_STAGE_FIRST : last ? KT_STAGE_LAST : KT_STAGE_MID);
 #define SWM_KA(M_, ADV_)                                                                                   \
     if (per) hipLaunchKernelGGL((stage_kernel<F, true, M_, ADV_>), grid, block, 0, ctx->stream, A);        \
     else hipLaunchKernelGGL((stage_kernel<F, false, M_, ADV_>), grid, block, 0, ctx->stream, A);
@@ -234,6 +236,7 @@ template <class F> struct Launch {
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool cs = ctx->c.continuity != SWM_CONT_ITSELF;
+        const KTrace kt(ctx, ma == M_RK_FIRST || ma == M_RKC_FIRST ? KT_FUSED12 : KT_FUSED34);
         if (adv != ADV_AH) {
             if constexpr (!F::mixed) {
 #define SWM_FV(MA_, MB_, ADV_)                                                                             \
@@ -518,6 +521,7 @@ template <class F> struct Launch {
         }
         const dim3 grid(nblocks_waves(A.nstrips * A.nseg)), block(WAVE * WAVES_PER_BLOCK);
         const bool per = c.bc == SWM_PERIODIC, smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
+        const KTrace kt(ctx, KT_DISSIPATION);
 #define SWM_DISS(PER_, D_, S_) hipLaunchKernelGGL((dissipation_kernel<F, PER_, D_, S_>), grid, block, 0, ctx->stream, A)
 #define SWM_DISS_D(PER_, S_)                                   \
     switch (c.bottom_drag) {                                   \
@@ -541,6 +545,7 @@ template <class F> struct Launch {
         const dim3 grid((c.nx + 127) / 128, c.ny), block(128);
         const bool per = c.bc == SWM_PERIODIC;
         auto k = per ? tracer_kernel<F, true> : tracer_kernel<F, false>;
+        const KTrace kt(ctx, KT_TRACER);
         hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<Q>(ctx->u[uv_slot]), ctx->at<Q>(ctx->v[uv_slot]),
                            ctx->at<ST>(ctx->um), ctx->at<ST>(ctx->vm), ctx->at<Q>(ctx->sst[src]), ctx->at<Q>(ctx->sst[dst]),
                            mkR<R>(c.half_dtadvu), mkR<R>(c.half_dtadvv), mkR<R>(c.dtadvu), mkR<R>(c.dtadvv), c.nx, c.ny,
@@ -607,6 +612,7 @@ template <class F> struct Launch {
     static int step(swm_ctx *ctx, int64_t i, int64_t t, bool *nan) {
         const swm_config &c = ctx->c;
         *nan = false;
+        const KTrace kt(ctx, KT_STEP);
         Slots u0, u1;
         int rc = stages(ctx, 1 << 20, t, &u0, &u1);
         if (rc) return rc;

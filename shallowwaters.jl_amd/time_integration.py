@@ -134,6 +134,16 @@ class Simulation:
                    self.ctx, "swm_step")
         return sd.value, ns.value
 
+    def time_steps(self, nsteps: int, i0: int = 1, t0: int = 0):
+        """Steps ``i0 .. i0+nsteps-1`` with each kernel launch timed by HIP events inside the
+        step: returns {kind: (average ms, launches)} for the kinds that ran (the state
+        advances as by ``step``)."""
+        ms = (ctypes.c_double * 8)()
+        n = (ctypes.c_int32 * 8)()
+        _abi.check(_abi.load().swm_time_steps(self.ctx, nsteps, i0, t0, ms, n), self.ctx, "swm_time_steps")
+        kinds = ("fused12", "fused34", "dissipation", "tracer", "step", "stage1", "stage_mid", "stage4")
+        return {k: (ms[j], n[j]) for j, k in enumerate(kinds) if n[j] > 0}
+
     def stages(self, n_stages: int, i0: int = 1, t0: int = 0):
         _abi.check(_abi.load().swm_stages(self.ctx, n_stages, i0, t0), self.ctx, "swm_stages")
 

@@ -418,3 +418,22 @@ def test_linear_dynamics_needs_f_on_the_u_and_v_grids(swm, gpu):
                     _abi.check(L.swm_set_static_linear(ctx, *[_abi.ptr(a) for a in fuv]), ctx, "swm_set_static_linear")
         finally:
             L.swm_destroy(ctx)
+
+
+def test_time_steps_advances_like_step(swm, gpu):
+    """swm_time_steps runs the same steps as swm_step (bitwise the same state) and times
+    each kernel launch inside them: one fused pair of each kind, one dissipation and one
+    step per step, a tracer advection every nadvstep steps."""
+    P, S, prog = setup_case(swm, "np-f64-smag-quad")
+    n = 2 * S.grid.nadvstep + 1
+    with swm.Simulation(S) as a, swm.Simulation(S) as b:
+        a.upload(prog)
+        b.upload(prog)
+        t = a.time_steps(n, 1, 0)
+        assert b.step(n, 1, 0) == (n, 0)
+        ya, yb = a.download(), b.download()
+    for x, y in zip((ya.u, ya.v, ya.η, ya.sst), (yb.u, yb.v, yb.η, yb.sst)):
+        assert np.array_equal(x, y)
+    assert {k: c for k, (_, c) in t.items()} == {"fused12": n, "fused34": n, "dissipation": n, "step": n, "tracer": 2}
+    assert all(ms > 0 for ms, _ in t.values())
+    assert t["step"][0] > t["fused12"][0] + t["fused34"][0]
