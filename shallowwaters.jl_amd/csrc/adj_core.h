@@ -229,7 +229,15 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
 //   λd_k = a_k·λ0 + b_k·λs_{k+1},  μ_k = (∂R/∂s_k)ᵀ λd_k,  A_k = A_{k+1} + λs_{k+1}.
 // NEXT: the stage has a successor (k < RKo).  A_L0: A_{k+1} is λ0 itself (k+1 = RKo).
 // PRE: the first stage, whose output is the adjoint of the step's input, A_1 + μ_1.
-template <class W, bool PER, bool NEXT, bool A_L0, bool PRE>
+//
+// SAD: Sadourny's enstrophy-conserving PV fluxes (PV_advection.jl:31-59) in place of
+// Arakawa-Hsu's, k12 = ½ (the weight of Iy!(q_u,q) and Ix!(q_v,q)):
+//   qhv = Qu·Vu, Qu = ½(q + q(y-1)) at the u point, Vu = ¼(V(y-1) + V(x+1,y-1)) + ¼(V + V(x+1))
+//   qhu = Qv·Uv, Qv = ½(q + q(x-1)) at the v point, Uv = ¼(U(x-1) + U) + ¼(U(x-1,y+1) + U(y+1))
+// (no column wraps: q_u and q_v read no q[end,·]).  Reverse, with λqhv = gu, λqhu = -gv:
+//   λQu = gu·Vu, λVu = gu·Qu, λQv = -gv·Uv, λUv = -gv·Qv, and each of them gathered back
+//   onto the q, V, U it read.
+template <class W, bool PER, bool NEXT, bool A_L0, bool PRE, bool SAD = false>
 SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, int yb) {
     using R = typename W::R;
     using V = typename W::V;
@@ -239,8 +247,8 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     const M ucol = PER ? M((x >= 0) & (x <= nx - 1)) : M((x >= 1) & (x <= nx - 1));
     const M vcol = (x >= 1) & (x <= nx);
     const M qcol = (x >= 0) & (x <= nx);
-    const M qfix = PER ? M(x == 0) : M(x == -1000);
-    const M qdrop = PER ? M(x == -1) : M(x == -1000);     // q(-1) is not read: qβ(0), qγ(0) read q(nx)
+    const M qfix = PER && !SAD ? M(x == 0) : M(x == -1000);
+    const M qdrop = PER && !SAD ? M(x == -1) : M(x == -1000);   // q(-1) is not read: qβ(0), qγ(0) read q(nx)
     const M st_u = w.own & (PER ? M((x >= -2) & (x <= nx + 1)) : ucol);
     const M st_v = w.own & (x >= -1) & (x <= nx + 2);
     const M st_e = w.own & (x >= 0) & (x <= nx + 1);
@@ -255,6 +263,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     V lp_2 = zero, lU_2 = zero, la_2 = zero, lb_2 = zero, lg_2 = zero, ld_2 = zero, lnum_3 = zero, lhv_3 = zero, lhq_3 = zero;
     V su_2 = zero, sv_2 = zero, se_2 = zero;           // λs_{k+1} of row r-2
     V zu_2 = zero, zv_2 = zero, ze_2 = zero;           // folded λ0 of row r-2 (A_L0)
+    V Qu_2 = zero, Vu_2 = zero, Uv_2 = zero;           // Sadourny: Qu, Vu of u row r-2, Uv of v row r-2
 
     StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0>(w, A, ya - 2, ya, yb);
     for (int r = ya - 2; r <= yb + 1; ++r) {
@@ -298,31 +307,50 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const M qv_1 = qcol & yq1;
         const V q_1 = W::sel(qv_1, (fq_1 + dvdx_1 - dudy_1) / W::sel(qv_1, hq_1, V(R(1))), zero);
         const V qxp_1 = W::xp(q_1);
-        const V qxm_1 = PER ? W::sel(qfix, qend_1, W::xm(q_1)) : W::xm(q_1);
-        const V qa_1 = k12 * (q_2 + q_1 + qxp_1);
-        const V qb_1 = k12 * (q_2 + q_1 + qxm_1);
-        const V qg_1 = PER ? W::sel(qfix, k12 * (q_2 + q_1 + qend_2), k12 * (q_2 + qxm_2 + q_1)) : k12 * (q_2 + qxm_2 + q_1);
-        const V qd_1 = k12 * (q_2 + q_1 + qxp_2);
-
-        // ---- adjoint of row r-1: Bernoulli potential, U, the Arakawa-Hsu weights
+        const V qxm_1 = PER && !SAD ? W::sel(qfix, qend_1, W::xm(q_1)) : W::xm(q_1);
+        V qa_1 = zero, qb_1 = zero, qg_1 = zero, qd_1 = zero, la_1 = zero, lb_1 = zero, lg_1 = zero, ld_1 = zero;
+        V lU_1, lq_2, lV_2, Qu_1 = zero, Vu_1 = zero, Uv_1 = zero;
         const V lp_1 = (gu_1 - W::xm(gu_1)) + (gv_1 - gv_2);
-        const V gvx_1 = W::xp(gv_1), gvx_2 = W::xp(gv_2);
-        const V lU_1 = (W::xp(ge_1) - ge_1) - (((gvx_1 * qa_1 + gv_1 * qb_1) + gv_2 * qg_1) + gvx_2 * qd_1);
-        const V la_1 = gu_1 * W::xp(V_1) - gvx_1 * U_1;
-        const V lb_1 = gu_1 * V_1 - gv_1 * U_1;
-        const V lg_1 = gu_1 * V_2 - gv_2 * U_1;
-        const V ld_1 = gu_1 * W::xp(V_2) - gvx_2 * U_1;
+        if constexpr (!SAD) {
+            qa_1 = k12 * (q_2 + q_1 + qxp_1);
+            qb_1 = k12 * (q_2 + q_1 + qxm_1);
+            qg_1 = PER ? W::sel(qfix, k12 * (q_2 + q_1 + qend_2), k12 * (q_2 + qxm_2 + q_1)) : k12 * (q_2 + qxm_2 + q_1);
+            qd_1 = k12 * (q_2 + q_1 + qxp_2);
 
-        // ---- adjoint of row r-2: PV, V, thickness, inputs
-        const V xlb = PER ? W::sel(qdrop, zero, W::xp(lb_2)) : W::xp(lb_2);
-        const V xlg = PER ? W::sel(qdrop, zero, W::xp(lg_1)) : W::xp(lg_1);
-        const V lq_2 = k12 * ((((la_1 + la_2) + W::xm(la_2)) + ((lb_1 + lb_2) + xlb)) +
-                              (((lg_1 + xlg) + lg_2) + ((ld_1 + ld_2) + W::xm(ld_1))));
+            // ---- adjoint of row r-1: Bernoulli potential, U, the Arakawa-Hsu weights
+            const V gvx_1 = W::xp(gv_1), gvx_2 = W::xp(gv_2);
+            lU_1 = (W::xp(ge_1) - ge_1) - (((gvx_1 * qa_1 + gv_1 * qb_1) + gv_2 * qg_1) + gvx_2 * qd_1);
+            la_1 = gu_1 * W::xp(V_1) - gvx_1 * U_1;
+            lb_1 = gu_1 * V_1 - gv_1 * U_1;
+            lg_1 = gu_1 * V_2 - gv_2 * U_1;
+            ld_1 = gu_1 * W::xp(V_2) - gvx_2 * U_1;
+
+            // ---- adjoint of row r-2: PV, V
+            const V xlb = PER ? W::sel(qdrop, zero, W::xp(lb_2)) : W::xp(lb_2);
+            const V xlg = PER ? W::sel(qdrop, zero, W::xp(lg_1)) : W::xp(lg_1);
+            lq_2 = k12 * ((((la_1 + la_2) + W::xm(la_2)) + ((lb_1 + lb_2) + xlb)) +
+                          (((lg_1 + xlg) + lg_2) + ((ld_1 + ld_2) + W::xm(ld_1))));
+            lV_2 = (ge_1 - ge_2) + (((W::xm(gu_2 * qa_2) + gu_2 * qb_2) + gu_1 * qg_1) + W::xm(gu_1 * qd_1));
+        } else {
+            // Sadourny: qhv of u row r-1 reads q rows r-1, r-2 and V rows r-2, r-1; qhu of v row
+            // r-1 reads q row r-1 and U rows r-1, r (and v row r-2 likewise, one row down)
+            Qu_1 = k12 * (q_1 + q_2);
+            Vu_1 = quarter * (V_2 + W::xp(V_2)) + quarter * (V_1 + W::xp(V_1));
+            Uv_1 = quarter * (W::xm(U_1) + U_1) + quarter * (W::xm(U_0) + U_0);
+            const V Qv_1 = k12 * (q_1 + qxm_1), Qv_2 = k12 * (q_2 + qxm_2);
+            const V lUv_1 = -(gv_1 * Qv_1), lUv_2 = -(gv_2 * Qv_2);
+            lU_1 = (W::xp(ge_1) - ge_1) + (quarter * (lUv_1 + W::xp(lUv_1)) + quarter * (lUv_2 + W::xp(lUv_2)));
+            const V lVu_1 = gu_1 * Qu_1, lVu_2 = gu_2 * Qu_2;
+            lV_2 = (ge_1 - ge_2) + (quarter * (lVu_1 + W::xm(lVu_1)) + quarter * (lVu_2 + W::xm(lVu_2)));
+            const V lQv_2 = -(gv_2 * Uv_2);
+            lq_2 = k12 * ((gu_1 * Vu_1 + gu_2 * Vu_2) + (lQv_2 + W::xp(lQv_2)));
+        }
+
+        // ---- adjoint of row r-2: thickness, inputs
         const bool yq2 = r - 2 >= 0 && r - 2 <= ny;
         const M qv_2 = qcol & yq2;
         const V lnum_2 = W::sel(qv_2, lq_2 / W::sel(qv_2, hq_2, V(R(1))), zero);
         const V lhq_2 = -(lnum_2 * q_2);
-        const V lV_2 = (ge_1 - ge_2) + (((W::xm(gu_2 * qa_2) + gu_2 * qb_2) + gu_1 * qg_1) + W::xm(gu_1 * qd_1));
         const V lhu_2 = lU_2 * u_2;
         const V lhv_2 = lV_2 * v_2;
         const V lh_2 = half * (lhu_2 + W::xm(lhu_2)) + half * (lhv_2 + lhv_3) +
@@ -363,7 +391,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
                 w.st(A.mu_out.e, y, le_2, st_e & ye_st);
             }
             // periodic: λ of q(nx, y) from qβ(0, y) and qγ(0, y+1)
-            if (PER && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
+            if (PER && !SAD && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
             // the wind stress enters du as Tprog(Fxt·Fx) (rhs.jl:242): λFx += Fxt·λdu, u cells
             if (A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
         }
@@ -376,6 +404,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         lnum_3 = lnum_2; lhv_3 = lhv_2; lhq_3 = lhq_2;
         su_2 = su; sv_2 = sv; se_2 = se;
         zu_2 = zu; zv_2 = zv; ze_2 = ze;
+        if constexpr (SAD) { Qu_2 = Qu_1; Vu_2 = Vu_1; Uv_2 = Uv_1; }
     }
 }
 
@@ -914,6 +943,7 @@ template <class R> struct AdjConsts {
     int nx, ny, P, ep;
     R fxt;                          // the step's factor of Fx in du (1 unless the wind is seasonal)
     int fxg;                        // accumulate the cotangent of Fx
+    int sad;                        // Sadourny's PV fluxes (k12 = ½)
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -973,9 +1003,10 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
         S.a_in = F.A;
         S.mu_out = next ? mu_next : mu_prev;
         S.a_out = pre ? out : F.A;
-        if (c.per && (rc = k.qend(S))) return rc;
-        if ((rc = k.stage(S, c.per, next, al0, pre))) return rc;
-        if (c.per && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
+        const bool wrap = c.per && !c.sad;              // the Arakawa-Hsu first-column wrap to q[end,·]
+        if (wrap && (rc = k.qend(S))) return rc;
+        if ((rc = k.stage(S, c.per, next, al0, pre, c.sad))) return rc;
+        if (wrap && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
         if (next) { const UVE t = mu_prev; mu_prev = mu_next; mu_next = t; }
     }
     F.L = out;

@@ -720,9 +720,9 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (adv_form(c) != ADV_AH)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear Arakawa-Hsu right-hand side only "
-                                                "(nstep_advcor = 0)");
+        if (adv_form(c) != ADV_AH && adv_form(c) != ADV_SAD)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear right-hand side with the Arakawa-Hsu or "
+                                                "Sadourny scheme only (nstep_advcor = 0)");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -788,7 +788,8 @@ template <class Fm> struct Adj {
             k.a[j] = (R)c.RKaDt[j];
             k.b[j] = j < 3 ? (R)c.RKbDt[j] : R(0);
         }
-        k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
+        k.sad = c.adv_scheme == SWM_ADV_SADOURNY;
+        k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(k.sad ? 0.5 : 1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
         k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
         k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
         k.RKo = c.RKo; k.cont = c.continuity; k.drag = c.bottom_drag; k.smag = c.diffusion == SWM_DIFF_SMAGORINSKY;
@@ -806,23 +807,24 @@ template <class Fm> struct Adj {
             hipError_t e = hipGetLastError();
             return e == hipSuccess ? SWM_OK : ctx->fail(SWM_E_HIP, "adjoint kernel launch: %s", hipGetErrorString(e));
         }
-        template <bool PER, bool NEXT, bool AL0, bool PRE> void launch_stage(const adj::StageAdjArgs<R> &S) {
-            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE>), dim3(nblocks_waves(S.nstrips * S.nseg)),
+        template <bool PER, bool NEXT, bool AL0, bool PRE, bool SAD> void launch_stage(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE, SAD>), dim3(nblocks_waves(S.nstrips * S.nseg)),
                                dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), S);
         }
-        template <bool PER> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
-            if (!next) launch_stage<PER, false, false, false>(S);
-            else if (al0 && pre) launch_stage<PER, true, true, true>(S);
-            else if (al0) launch_stage<PER, true, true, false>(S);
-            else if (pre) launch_stage<PER, true, false, true>(S);
-            else launch_stage<PER, true, false, false>(S);
+        template <bool PER, bool SAD> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
+            if (!next) launch_stage<PER, false, false, false, SAD>(S);
+            else if (al0 && pre) launch_stage<PER, true, true, true, SAD>(S);
+            else if (al0) launch_stage<PER, true, true, false, SAD>(S);
+            else if (pre) launch_stage<PER, true, false, true, SAD>(S);
+            else launch_stage<PER, true, false, false, SAD>(S);
         }
-        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre) {
+        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre, int sad) {
             S.L = st->La;
             S.nstrips = adj::nstrips_for<adj::RT_HALO>(S.nx + 2);
             S.nseg = (adj::adj_y1(S.ny) - adj::ADJ_Y0 + S.L - 1) / S.L;
-            if (per) stage_p<true>(S, next, al0, pre);
-            else stage_p<false>(S, next, al0, pre);
+            if (per) { if (sad) stage_p<true, true>(S, next, al0, pre); else stage_p<true, false>(S, next, al0, pre); }
+            else if (sad) stage_p<false, true>(S, next, al0, pre);
+            else stage_p<false, false>(S, next, al0, pre);
             return fail_launch();
         }
         int qend(const adj::StageAdjArgs<R> &S) {
@@ -1151,7 +1153,7 @@ template <class Fm> struct Adj {
             case 0: { bool nan = false; return LF::step(ctx, 1, 0, &nan); }
             case 1: { adj::AdjFields G = st->F0; return tape_step(ctx, 0, G); }
             case 2: { st->F = Ft; return adj::reverse_step<R>(kl, st->F, k, 1); }
-            case 3: return kl.stage(S, k.per, 1, 0, 0);
+            case 3: return kl.stage(S, k.per, 1, 0, 0, k.sad);
             case 4: return kl.diss(D, k.per, k.drag, k.smag);
             }
             return ctx->fail(SWM_E_ARG, "unknown adjoint kernel %d", which);

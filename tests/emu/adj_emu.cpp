@@ -204,21 +204,22 @@ template <class R> struct EmuK {
             fn(W, t);
         }
     }
-    template <bool PER, bool NEXT, bool AL0, bool PRE> void stage_t(const StageAdjArgs<R> &S) {
+    template <bool PER, bool NEXT, bool AL0, bool PRE, bool SAD> void stage_t(const StageAdjArgs<R> &S) {
         march<RT_HALO>(S.nx, S.ny, ADJ_Y0, adj_y1(S.ny), [&](EmuWave<R> &W, const Tile &t) {
-            stage_adjoint<EmuWave<R>, PER, NEXT, AL0, PRE>(W, S, t.ya, t.yb);
+            stage_adjoint<EmuWave<R>, PER, NEXT, AL0, PRE, SAD>(W, S, t.ya, t.yb);
         });
     }
-    template <bool PER> void stage_p(const StageAdjArgs<R> &S, int next, int al0, int pre) {
-        if (!next) stage_t<PER, false, false, false>(S);
-        else if (al0 && pre) stage_t<PER, true, true, true>(S);
-        else if (al0) stage_t<PER, true, true, false>(S);
-        else if (pre) stage_t<PER, true, false, true>(S);
-        else stage_t<PER, true, false, false>(S);
+    template <bool PER, bool SAD> void stage_p(const StageAdjArgs<R> &S, int next, int al0, int pre) {
+        if (!next) stage_t<PER, false, false, false, SAD>(S);
+        else if (al0 && pre) stage_t<PER, true, true, true, SAD>(S);
+        else if (al0) stage_t<PER, true, true, false, SAD>(S);
+        else if (pre) stage_t<PER, true, false, true, SAD>(S);
+        else stage_t<PER, true, false, false, SAD>(S);
     }
-    int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre) {
-        if (per) stage_p<true>(S, next, al0, pre);
-        else stage_p<false>(S, next, al0, pre);
+    int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre, int sad) {
+        if (per) { if (sad) stage_p<true, true>(S, next, al0, pre); else stage_p<true, false>(S, next, al0, pre); }
+        else if (sad) stage_p<false, true>(S, next, al0, pre);
+        else stage_p<false, false>(S, next, al0, pre);
         return 0;
     }
     int qend(const StageAdjArgs<R> &S) {
@@ -287,7 +288,7 @@ extern "C" {
 // Scalars of a configuration, as the GPU driver derives them from swm_config.
 struct EmuCfg {
     int32_t nx, ny, per, RKo, cont, drag, smag, consume, tracer, nadvstep, nadvstep_half, nstep_diff;
-    int32_t L, nan_edges, fmt, pad;
+    int32_t L, nan_edges, fmt, sad;
     double a[4], b[4];
     double g, gamma, k12, oma, cD, rD, cS, nuB, dtd, hdu, hdv, du, dv, jSST;
 };
@@ -336,6 +337,7 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
     k.RKo = c.RKo; k.cont = c.cont; k.drag = c.drag; k.smag = c.smag; k.per = c.per; k.consume = c.consume;
     k.tracer = c.tracer; k.nadvstep = c.nadvstep; k.nadvstep_half = c.nadvstep_half; k.nstep_diff = c.nstep_diff;
     k.nx = c.nx; k.ny = c.ny; k.ep = c.per ? 1 : 0;
+    k.sad = c.sad;
     return k;
 }
 
@@ -377,9 +379,10 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
             S.a = R(1); S.b = R(0);
             S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = c.cont;
             S.nx = c.nx; S.ny = c.ny; S.P = A.P;
-            if (c.per) K.qend(S);
-            K.stage(S, c.per, 0, 0, 0);
-            if (c.per) K.qend_adj(S, S.mu_out);
+            const bool wrap = c.per && !c.sad;
+            if (wrap) K.qend(S);
+            K.stage(S, c.per, 0, 0, 0, c.sad);
+            if (wrap) K.qend_adj(S, S.mu_out);
             get<R>(A, F.Mb.u, out[0], sh.u); get<R>(A, F.Mb.v, out[1], sh.v); get<R>(A, F.Mb.e, out[2], sh.e);
         } else if (op == 2) {
             put<R>(A, F.pre.u, in[0], sh.u); put<R>(A, F.pre.v, in[1], sh.v); put<R>(A, F.pre.e, in[2], sh.e);

@@ -158,7 +158,9 @@ def test_tangent_linear_values_are_the_oracle(swm, orc):
 # ------------------------------------------------------------------ operators
 
 @pytest.mark.parametrize("bc,cont", [("nonperiodic", {}), ("periodic", {}),
-                                      ("nonperiodic", {"surface_relax": True, "t_relax": 2.0})])
+                                      ("nonperiodic", {"surface_relax": True, "t_relax": 2.0}),
+                                      ("nonperiodic", {"adv_scheme": "Sadourny"}),
+                                      ("periodic", {"adv_scheme": "Sadourny"})])
 def test_stage_adjoint_is_the_transposed_jacobian(swm, orc, bc, cont):
     P, S, cfg, prog = setup(swm, bc, **cont)
     rng = np.random.default_rng(3)
@@ -232,7 +234,9 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
 
 
 @pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"RKo": 3}),
-                                   ("periodic", {"nstep_diff": 2, "diffusion": "constant", "bottom_drag": "linear"})])
+                                   ("periodic", {"nstep_diff": 2, "diffusion": "constant", "bottom_drag": "linear"}),
+                                   ("periodic", {"adv_scheme": "Sadourny"}),
+                                   ("nonperiodic", {"adv_scheme": "Sadourny", "RKo": 3})])
 def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
     """Three steps: the tracer's snapshot (step 2) and advection (step 3) included."""
     P, S, cfg, prog = setup(swm, bc, **kw)

@@ -47,14 +47,15 @@ def gpu_gradient(swm, S, prog, ybar, n, snapshots):
 
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
 @pytest.mark.parametrize("T", ["Float64", "Float32"])
-def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T):
+@pytest.mark.parametrize("adv", ["ArakawaHsu", "Sadourny"])
+def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T, adv):
     monkeypatch.setenv("SWM_ROWS_PER_WAVE_ADJ", "8")        # many segments on a small grid
-    P, S, cfg, prog = setup(swm, bc, T=T)
+    P, S, cfg, prog = setup(swm, bc, T=T, adv_scheme=adv)
     assert (S.grid.nadvstep, S.grid.nadvstep_half) == (3, 1)
     n = 3                                                     # snapshot at step 2, advection at step 3
     dt = np.float64 if T == "Float64" else np.float32
     ybar = cotangent(S, 1, dt)
-    print(f"pullback {bc} {T} nx={S.grid.nx} ny={S.grid.ny}", flush=True)
+    print(f"pullback {bc} {T} {adv} nx={S.grid.nx} ny={S.grid.ny}", flush=True)
     g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=1)
     k = adjemu.config(S, cfg, L=8, fmt=8 if T == "Float64" else 4)
     ref, _, _ = reverse_steps(swm, orc, S, cfg, prog, n, [a.astype(np.float64) for a in ybar], k)
@@ -86,9 +87,10 @@ def test_taped_fused_pairs_equal_taped_stages(swm, gpu, monkeypatch, case):
         assert np.array_equal(np.asarray(a), np.asarray(b)), f"{case} {name}: {np.count_nonzero(a != b)} cells differ"
 
 
-@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
-def test_dot_product_identity_fp64(swm, orc, gpu, bc):
-    P, S, cfg, prog = setup(swm, bc, nx=90)
+@pytest.mark.parametrize("bc,adv", [("nonperiodic", "ArakawaHsu"), ("periodic", "ArakawaHsu"),
+                                    ("nonperiodic", "Sadourny"), ("periodic", "Sadourny")])
+def test_dot_product_identity_fp64(swm, orc, gpu, bc, adv):
+    P, S, cfg, prog = setup(swm, bc, nx=90, adv_scheme=adv)
     n = 7
     ybar = cotangent(S, 2, np.float64)
     print(f"dot product {bc} nx={S.grid.nx} ny={S.grid.ny} n={n}", flush=True)
@@ -157,8 +159,8 @@ def test_rrule_of_time_integration(swm, gpu):
 
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
-    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", adv_scheme="Sadourny"),
-               dict(T="Float32", dynamics="linear"), dict(T="Float32", nstep_advcor=2)):
+    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", dynamics="linear"),
+               dict(T="Float32", nstep_advcor=2), dict(T="Float32", adv_scheme="Sadourny", nstep_advcor=2)):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
             sim.upload(prog)
