@@ -101,8 +101,10 @@ def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
     """The batched ensemble: nmem members with grid sizes drawn from {128, 256, 512, 1024}^2
     (L_ratio = 1), Fx0 ~ U(0.08, 0.16) Pa, H ~ U(400, 600) m, seeded smoothed η noise
     (σ = 0.05 m), Float32, RK4, the other parameters at their defaults.  Times `steps`
-    steps of the tuned ensemble and, as the unbatched comparison, `seq_steps` steps of each
-    member on its own, one after the other."""
+    steps of the tuned ensemble from the initial state as batched launches and graph
+    replays (the default); the same steps in the persistent launch (every step, tracer and
+    NaN-check steps included, in one cooperative kernel, its phases timed from inside); and
+    `seq_steps` steps of each member on its own, one after the other."""
     import numpy as np
     setups = swm.workloads.ensemble_members(nmem, seed)
     sizes = np.array([S.grid.nx for S, _ in setups])
@@ -120,18 +122,30 @@ def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
             sim.upload(p)
         with swm.Ensemble(sims) as ens:
             tuned = ens.tune(reps=3)
-            ens.step(10, 1)
             t0 = time.perf_counter()
-            done, nan, graph = ens.step(steps, 11)
+            done, nan, graph = ens.step(steps, 1)
             el = time.perf_counter() - t0
-        assert done == steps and not any(nan), (done, nan)
+            assert done == steps and not any(nan), (done, nan)
+            for sim, (_, p) in zip(sims, setups):
+                sim.upload(p)
+            ens.persistent = "timed"
+            t0 = time.perf_counter()
+            done, nan, _ = ens.step(steps, 1)
+            el_p = time.perf_counter() - t0
+            assert done == steps and not any(nan), (done, nan)
+            persistent, why = ens.persistent_status()
+            phases = ens.phase_ms()
     finally:
         for sim in sims:
             sim.close()
+    value = cells * steps / el
     return {"members": nmem, "sizes": {str(int(n)): int((sizes == n).sum()) for n in (128, 256, 512, 1024)},
-            "cells_per_step": cells, "steps": steps, "value": cells * steps / el, "unit": "cell-updates/s",
+            "cells_per_step": cells, "steps": steps, "value": value, "unit": "cell-updates/s",
             "ms_per_step": el / steps * 1e3, "graph_steps": graph, "rows_per_wave_tuned": list(tuned),
-            "unbatched_value": cells * seq_steps / seq, "speedup_vs_unbatched": (cells * steps / el) / (cells * seq_steps / seq)}
+            "persistent_launch": {"value": cells * steps / el_p, "ms_per_step": el_p / steps * 1e3,
+                                  "steps_in_launch": persistent, "not_used_because": why or None,
+                                  "phase_ms": phases},
+            "unbatched_value": cells * seq_steps / seq, "speedup_vs_unbatched": value / (cells * seq_steps / seq)}
 
 
 def adjoint_bytes_per_cell(es: int) -> dict:
@@ -431,7 +445,7 @@ def main():
     ap.add_argument("--scale-probe", action="store_true",
                     help="only report how far the Float16 / BFloat16 right-hand sides run with velocities scaled "
                          "by s = 1 .. 64, then exit")
-    ap.add_argument("--ensemble-steps", type=int, default=200,
+    ap.add_argument("--ensemble-steps", type=int, default=2000,
                     help="steps timed for the 64-member batched ensemble (0 skips it)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=20,
                     help="steps of each ensemble member run on its own, for the unbatched comparison")

@@ -243,6 +243,33 @@ int swm_ens_tune(swm_ens *ens, int32_t reps, int32_t *rows_stage, int32_t *rows_
  * steps completed, *graph_steps how many of them ran as graph replays. */
 int swm_ens_step(swm_ens *ens, int64_t nsteps, int64_t i0, int64_t *steps_done, int64_t *nan_steps,
                  int64_t *graph_steps);
+/* Persistent launch (off by default; swm_ens_set_persistent(ens, 1) turns it on):
+ * swm_ens_step runs all its steps in one cooperative launch that holds as many blocks as
+ * stay resident on the device.  For every step it runs
+ * the step's batched kernels (fused RK4 stage pairs, dissipation) over the tiles of every
+ * member, then the members' tracer snapshot, advection and NaN check where each member's own
+ * schedule has them, with grid-wide barriers between phases; a NaN found stops every member
+ * after that step, as above.  Used when every member runs the fused RK4 pairs with
+ * T = Tprog (Float64 or Float32), the continuity itself and Arakawa-Hsu advection (any
+ * drag and diffusion), with a plan that repeats every two steps (no seasonal wind; a state
+ * just uploaded takes one batched step first to settle its slot roles); other steps run as
+ * batched launches and graph replays.  Results are the same bit for bit either way.  The batched launches are the
+ * default because they run faster on an MI355X: the persistent kernel's phases share one
+ * register allocation (the largest of the stage pair, seam and dissipation bodies, with
+ * spills) and each grid barrier costs tens of microseconds of cross-XCD cache write-back and
+ * atomics, where a graph replays its kernels back to back.  swm_ens_persistent_steps gives the steps of the last swm_ens_step that ran
+ * in it (0 when that call used the other path) and, where it did not (why_not nullable), a
+ * static string saying why.  A barrier that waits longer than 5 s (a
+ * launch whose blocks are not all resident) makes every block leave and the call fails with
+ * SWM_E_HIP; the members' states are then undefined. */
+int swm_ens_set_persistent(swm_ens *ens, int32_t on);
+/* Measurement hook: with swm_ens_set_persistent(ens, 2) the persistent launch records the
+ * time block 0 leaves each of its first grid barriers; ms[0..3] then receive the average
+ * duration in ms of the first fused stage pair, the second and the dissipation (each up to
+ * the barrier after it), and of whole steps, over the steps of the last call's first launch
+ * that follow a step without tracer or NaN-check work (-1 where none). */
+int swm_ens_phase_ms(const swm_ens *ens, double *ms);
+int swm_ens_persistent_steps(const swm_ens *ens, int64_t *steps, const char **why_not);
 
 /* ---- Reverse mode (no counterpart in the reference, which has no derivatives; it is what a
  * ChainRulesCore.rrule of time_integration, src/time_integration.jl:2-250, would call).

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libswm_hip.so")
 EXPORTS = ["swm_version", "swm_device_count", "swm_create", "swm_destroy", "swm_last_error",
            "swm_set_static", "swm_set_static_linear", "swm_upload_prog", "swm_download_prog", "swm_step", "swm_stages",
            "swm_download_rk", "swm_reduce", "swm_time_kernel", "swm_ens_create", "swm_ens_destroy",
-           "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_adj_forward", "swm_adj_pullback",
+           "swm_ens_last_error", "swm_ens_tune", "swm_ens_step", "swm_ens_set_persistent",
+           "swm_ens_persistent_steps", "swm_ens_phase_ms", "swm_adj_forward", "swm_adj_pullback",
            "swm_adj_time", "swm_reduce_noalloc", "swm_ghost", "swm_dissipation", "swm_tracer",
            "swm_adj_track_Fx", "swm_adj_download_Fx", "swm_set_static_device", "swm_download_static", "swm_time_steps"]
 
@@ -71,6 +72,9 @@ def load():
     L.swm_ens_last_error.restype = ctypes.c_char_p
     L.swm_ens_tune.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.swm_ens_step.argtypes = [vp, i64, i64, pi64, pi64, pi64]
+    L.swm_ens_set_persistent.argtypes = [vp, i32]
+    L.swm_ens_phase_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.swm_ens_persistent_steps.argtypes = [vp, pi64, ctypes.POINTER(ctypes.c_char_p)]
     L.swm_adj_forward.argtypes = [vp, i64, i64, i64, i32]
     L.swm_adj_pullback.argtypes = [vp] * 9 + [pi64, ctypes.POINTER(ctypes.c_double)]
     L.swm_adj_time.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_double)]

@@ -20,7 +20,7 @@ OUT = os.path.join(PKG, "libswm_hip.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 UNITS = ["swm.hip", "swm_f32.hip", "swm_f64.hip", "swm_f16.hip", "swm_bf16.hip", "swm_f16a.hip", "swm_bf16a.hip"]
 SOURCES = [os.path.join(HERE, u) for u in UNITS]
-HEADERS = ["swm_host.h", "swm_launch.h", "swm_common.h", "k_setup.h", "k_stage.h", "k_dissipation.h", "k_misc.h", "k_adjoint.h",
+HEADERS = ["swm_host.h", "swm_launch.h", "swm_common.h", "k_setup.h", "k_stage.h", "k_dissipation.h", "k_misc.h", "k_persist.h", "k_adjoint.h",
            "adj_core.h"]
 DEPS = SOURCES + [os.path.join(HERE, h) for h in HEADERS] + [os.path.join(ROOT, "include", "swm.h")]
 

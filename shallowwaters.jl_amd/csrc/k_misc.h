@@ -28,18 +28,32 @@ template <class S> __device__ __forceinline__ void stp(S *p, long i, typename St
 // indices of u_T/v_T/sst back to (x,y).  Arithmetic in T: u, v (= T(u0) after
 // dissipation, time_integration.jl:226-228) and sst (Tprog) are rounded to T on load; um,
 // vm are T snapshots; the result is written to sst in Tprog (exact).
-template <class F, bool PER>
-__global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, const typename F::ST *um,
-                              const typename F::ST *vm, const typename F::Q *sst, typename F::Q *ssti,
-                              typename F::R hdu, typename F::R hdv, typename F::R du_, typename F::R dv_, int nx,
-                              int ny, int P, int ep, int consume, typename F::R jSST, typename F::R SSTmin, int cm,
-                              typename F::R sinv, int *bad) {
+template <class F> struct TracerArgs {
     using R = typename F::R;
     using Q = typename F::Q;
     using ST = typename F::ST;
-    const int x = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = 1 + (int)blockIdx.y;
-    if (x > nx) return;
+    const Q *u, *v;                   // post-dissipation velocities (Prog of the next step)
+    const ST *um, *vm;                // mid-point snapshots
+    const Q *sst;                     // tracer before the advection
+    Q *ssti;                          // after it (the other sst slot)
+    R hdu, hdv, du_, dv_;             // ½dtadvu, ½dtadvv, dtadvu, dtadvv
+    R jSST, SSTmin, sinv;
+    int nx, ny, P, ep, consume, cm;
+    int *bad;                         // bit 0: departure point NaN, bit 1: index outside the field
+};
+
+// the advection of T-cell (x, y), 1 <= x <= nx, 1 <= y <= ny
+template <class F, bool PER>
+__device__ __forceinline__ void tracer_cell(const TracerArgs<F> &A, int x, int y) {
+    using R = typename F::R;
+    using Q = typename F::Q;
+    using ST = typename F::ST;
+    const Q *u = A.u, *v = A.v, *sst = A.sst;
+    const ST *um = A.um, *vm = A.vm;
+    Q *ssti = A.ssti;
+    const R hdu = A.hdu, hdv = A.hdv, du_ = A.du_, dv_ = A.dv_, jSST = A.jSST, SSTmin = A.SSTmin, sinv = A.sinv;
+    const int nx = A.nx, ny = A.ny, P = A.P, ep = A.ep, consume = A.consume, cm = A.cm;
+    int *bad = A.bad;
     const R half = R(0.5), one = R(1.0), zero = R(0.0);
     auto U = [&](int cx, int cy) { return toR<R>(u[cx + cy * P]); };
     auto V = [&](int cx, int cy) { return toR<R>(v[cx + cy * P]); };
@@ -107,6 +121,14 @@ __global__ void tracer_kernel(const typename F::Q *u, const typename F::Q *v, co
     const R gs = PER ? comm_round(s, cm) : s;
     if (x == (PER ? nx : 1)) ssti[y * P] = toQ<Q>(gs);
     if (x == (PER ? 1 : nx)) ssti[nx + 1 + y * P] = toQ<Q>(gs);
+}
+
+template <class F, bool PER>
+__global__ void tracer_kernel(TracerArgs<F> A) {
+    const int x = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = 1 + (int)blockIdx.y;
+    if (x > A.nx) return;
+    tracer_cell<F, PER>(A, x, y);
 }
 
 // ------------------------------------------------------------------ small kernels

@@ -684,6 +684,7 @@ int swm_ens_destroy(swm_ens *E) {
     if (E->exec) (void)hipGraphExecDestroy(E->exec);
     if (E->graph) (void)hipGraphDestroy(E->graph);
     for (auto &b : E->buf) if (b.dev) (void)hipFree(b.dev);
+    if (E->pdev) (void)hipFree(E->pdev);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     for (swm_ctx *c : E->m) c->ens_refs -= 1;
     delete E;
@@ -710,6 +711,25 @@ int swm_ens_step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *steps_done, in
     for (swm_ctx *c : E->m) (void)hipStreamSynchronize(c->stream);
     StreamSwap sw(E);
     return fmt_ops(E->fmt).ens_step(E, nsteps, i0, steps_done, nan_steps, graph_steps);
+}
+
+int swm_ens_set_persistent(swm_ens *E, int32_t on) {
+    if (!E || on < 0 || on > 2) return SWM_E_ARG;
+    E->persistent = on;
+    return SWM_OK;
+}
+
+int swm_ens_phase_ms(const swm_ens *E, double *ms) {
+    if (!E || !ms) return SWM_E_ARG;
+    for (int k = 0; k < 4; ++k) ms[k] = E->phase_ms[k];
+    return SWM_OK;
+}
+
+int swm_ens_persistent_steps(const swm_ens *E, int64_t *steps, const char **why_not) {
+    if (!E || !steps) return SWM_E_ARG;
+    *steps = E->persist_steps;
+    if (why_not) *why_not = E->persist_why;
+    return SWM_OK;
 }
 
 // ---- reverse mode

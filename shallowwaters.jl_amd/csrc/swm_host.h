@@ -431,6 +431,12 @@ struct swm_ens {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
+    int persistent = 0;                // run the steps in one persistent launch where the form allows (2: and time its phases)
+    double phase_ms[4] = {-1, -1, -1, -1};   // its phases' average durations (first pair, second, dissipation, step)
+    void *pdev = nullptr;              // its member records, control word and NaN flags
+    size_t pcap = 0;
+    int64_t persist_steps = 0;         // steps of the last swm_ens_step that ran in it
+    const char *persist_why = "";      // why the last swm_ens_step did not use it
 
     int fail(int code, const char *fmt_, ...) {
         char b[512];

@@ -8,6 +8,7 @@
 #include "k_adjoint.h"
 #include "k_dissipation.h"
 #include "k_misc.h"
+#include "k_persist.h"
 #include "k_setup.h"
 #include "k_stage.h"
 
@@ -540,17 +541,29 @@ template <class F> struct Launch {
         return SWM_OK;
     }
 
+    static TracerArgs<F> tracer_args(const swm_ctx *ctx, int uv_slot, int src, int dst) {
+        const swm_config &c = ctx->c;
+        TracerArgs<F> a;
+        a.u = ctx->at<Q>(ctx->u[uv_slot]);
+        a.v = ctx->at<Q>(ctx->v[uv_slot]);
+        a.um = ctx->at<ST>(ctx->um);
+        a.vm = ctx->at<ST>(ctx->vm);
+        a.sst = ctx->at<Q>(ctx->sst[src]);
+        a.ssti = ctx->at<Q>(ctx->sst[dst]);
+        a.hdu = mkR<R>(c.half_dtadvu); a.hdv = mkR<R>(c.half_dtadvv); a.du_ = mkR<R>(c.dtadvu); a.dv_ = mkR<R>(c.dtadvv);
+        a.jSST = mkR<R>(c.jSST); a.SSTmin = mkR<R>(c.SSTmin); a.sinv = mkR<R>(ctx->sci);
+        a.nx = c.nx; a.ny = c.ny; a.P = ctx->P; a.ep = ctx->ep; a.consume = c.tracer_consumption; a.cm = c.Tcomm;
+        a.bad = ctx->dflags + 1;
+        return a;
+    }
+
     static int tracer_advect(swm_ctx *ctx, int uv_slot, int src, int dst) {
         const swm_config &c = ctx->c;
         const dim3 grid((c.nx + 127) / 128, c.ny), block(128);
         const bool per = c.bc == SWM_PERIODIC;
         auto k = per ? tracer_kernel<F, true> : tracer_kernel<F, false>;
         const KTrace kt(ctx, KT_TRACER);
-        hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, ctx->at<Q>(ctx->u[uv_slot]), ctx->at<Q>(ctx->v[uv_slot]),
-                           ctx->at<ST>(ctx->um), ctx->at<ST>(ctx->vm), ctx->at<Q>(ctx->sst[src]), ctx->at<Q>(ctx->sst[dst]),
-                           mkR<R>(c.half_dtadvu), mkR<R>(c.half_dtadvv), mkR<R>(c.dtadvu), mkR<R>(c.dtadvv), c.nx, c.ny,
-                           ctx->P, ctx->ep, c.tracer_consumption, mkR<R>(c.jSST), mkR<R>(c.SSTmin), c.Tcomm, mkR<R>(ctx->sci),
-                           ctx->dflags + 1);
+        hipLaunchKernelGGL(k, grid, block, 0, ctx->stream, tracer_args(ctx, uv_slot, src, dst));
         HIPCHK(hipGetLastError());
         return SWM_OK;
     }
@@ -1423,7 +1436,300 @@ template <class F> struct Ens {
         }
     }
 
+    // ------------------------------------------------------------- persistent launch
+    static constexpr const char *WHY_SETTLE = "the plans do not repeat every two steps";
+    static constexpr int PHASE_TRACE = 8192;              // barrier exit times recorded when timing phases
+    //
+    // Steps whose plans repeat every two steps and whose members all run the fused RK4 pairs
+    // with T = Tprog, the continuity itself and Arakawa-Hsu advection run in one cooperative
+    // launch (ens_persist_kernel): the plans of
+    // the two parities in the plan buffers, each member's tracer and NaN-check schedule in a
+    // record.  *ran is false (and nothing changed) when the form does not allow it.
+    static int persist(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, bool *ran) {
+        *ran = false;
+        E->persist_why = "";
+        auto skip = [&](const char *why) { E->persist_why = why; return SWM_OK; };
+        if constexpr (F::mixed) {
+            return skip("mixed number formats (no fused stage pairs)");
+        } else {
+            const size_t nm = E->m.size();
+            if (!E->persistent) return skip("turned off");
+            if (nsteps < 1) return SWM_OK;
+            for (const swm_ctx *ctx : E->m) {
+                const swm_config &c = ctx->c;
+                if (ctx->advcor_pending || adv_form(c) != ADV_AH || c.continuity != SWM_CONT_ITSELF || c.nstep_diff != 1)
+                    return skip("right-hand side or continuity form");
+            }
+            // the plans of steps i0, i0+1 and i0+2: the third must repeat the first
+            swm_ens::Plan pl[3];
+            std::vector<int> pu(nm), pe(nm);
+            for (size_t q = 0; q < nm; ++q) { pu[q] = E->m[q]->pu; pe[q] = E->m[q]->pe; }
+            int rc = SWM_OK;
+            for (int k = 0; k < 3 && !rc; ++k) {
+                rc = plan(E, i0 + k, pl[k]);
+                if (!rc) advance(E, pl[k]);
+            }
+            for (size_t q = 0; q < nm; ++q) { E->m[q]->pu = pu[q]; E->m[q]->pe = pe[q]; }
+            if (rc) return rc;
+            if (pl[2].blob != pl[0].blob) return skip(WHY_SETTLE);
+            for (int k = 0; k < 2; ++k) {
+                const swm_ens::Plan &P = pl[k];
+                if (!P.init.empty() || P.kind.empty() || P.kind.size() > (size_t)PERSIST_ITEMS)
+                    return skip("pending kept-field update, or too many kernels per step");
+                for (size_t j = 0; j < P.kind.size(); ++j) {
+                    const int kd = P.kind[j];
+                    const bool pair = (P.ma[j] == M_RK_FIRST && P.mb[j] == M_RK_MID) || (P.ma[j] == M_RK_MID && P.mb[j] == M_RK_LAST);
+                    if (!(kd == 2 || ((kd == 1 || kd == 4 || (kd == 5 && E->per)) && pair)))
+                        return skip("kernels other than fused RK4 pairs and the dissipation");
+                }
+            }
+            if ((rc = load(E, 0, pl[0])) || (rc = load(E, 1, pl[1]))) return rc;
+            E->last = 1;
+            // kernel arguments: items, barriers between phases (the strips clear of a seam and
+            // the seam strips of one pair write disjoint cells and share a phase)
+            PArgs<F> A;
+            std::memset((void *)&A, 0, sizeof A);
+            PItem items[2][PERSIST_ITEMS];
+            std::memset((void *)items, 0, sizeof items);
+            for (int k = 0; k < 2; ++k) {
+                const swm_ens::Plan &P = pl[k];
+                A.plan[k] = (const char *)E->buf[k].dev;
+                A.nitems[k] = (int)P.kind.size();
+                for (size_t j = 0; j < P.kind.size(); ++j) {
+                    PItem &it = items[k][j];
+                    it.kind = P.kind[j]; it.ma = P.ma[j]; it.mb = P.mb[j];
+                    it.args = (uint32_t)P.args_off[j]; it.tab = (uint32_t)P.tab_off[j]; it.tiles = P.tiles[j];
+                    const bool joined = P.kind[j] == 4 && j + 1 < P.kind.size() && P.kind[j + 1] == 5 &&
+                                        P.ma[j + 1] == P.ma[j] && P.mb[j + 1] == P.mb[j];
+                    it.sync = joined ? 0 : 1;
+                }
+            }
+            // member records
+            std::vector<PMember<F>> mem(nm);
+            const int ntrace = E->persistent == 2 ? PHASE_TRACE : 0;
+            const size_t nan_off = (sizeof(PCtl) + 255) & ~(size_t)255;
+            const size_t mem_off = nan_off + ((nm * sizeof(int) + 255) & ~(size_t)255);
+            const size_t trace_off = mem_off + ((nm * sizeof(PMember<F>) + 255) & ~(size_t)255);
+            const size_t item_off = trace_off + (((size_t)ntrace * sizeof(long long) + 255) & ~(size_t)255);
+            const size_t need = item_off + 2 * PERSIST_ITEMS * sizeof(PItem);
+            if (need > E->pcap) {
+                if (E->pdev) (void)hipFree(E->pdev);
+                E->pdev = nullptr;
+                E->pcap = std::max<size_t>(need, 1 << 16);
+                if (hipMalloc(&E->pdev, E->pcap) != hipSuccess) { E->pcap = 0; return E->fail(SWM_E_HIP, "persistent launch records"); }
+            }
+            char *dv = (char *)E->pdev;
+            PCtl *dctl = (PCtl *)dv;
+            int *dnan = (int *)(dv + nan_off);
+            PMember<F> *dmem = (PMember<F> *)(dv + mem_off);
+            long long *dtrace = (long long *)(dv + trace_off);
+            PItem *ditem = (PItem *)(dv + item_off);
+            for (size_t q = 0; q < nm; ++q) {
+                swm_ctx *ctx = E->m[q];
+                const swm_config &c = ctx->c;
+                PMember<F> &M = mem[q];
+                std::memset((void *)&M, 0, sizeof M);
+                M.tr = Launch<F>::tracer_args(ctx, pl[0].next_uv[q], 0, 1);
+                for (int k = 0; k < 2; ++k) {
+                    const int uv = pl[k].next_uv[q];
+                    M.tu[k] = ctx->at<Q>(ctx->u[uv]);
+                    M.tv[k] = ctx->at<Q>(ctx->v[uv]);
+                    M.su[k] = (const Q *)ctx->u[uv].base;
+                    M.sv[k] = (const Q *)ctx->v[uv].base;
+                    const int su = k == 0 ? pu[q] : pl[0].next_pu[q], se = k == 0 ? pe[q] : pl[0].next_pe[q];
+                    NanArgs &N = M.nan[k];
+                    N.f[0] = ctx->at<Q>(ctx->u[su]); N.x0[0] = -1 - ctx->ep; N.x1[0] = ctx->nux + 2 - ctx->ep; N.y0[0] = -1; N.y1[0] = c.ny + 2;
+                    N.f[1] = ctx->at<Q>(ctx->v[su]); N.x0[1] = -1; N.x1[1] = c.nx + 2; N.y0[1] = -1; N.y1[1] = c.ny + 1;
+                    N.f[2] = ctx->at<Q>(ctx->e[se]); N.x0[2] = 0; N.x1[2] = c.nx + 1; N.y0[2] = 0; N.y1[2] = c.ny + 1;
+                    N.f[3] = nullptr; N.x0[3] = 0; N.x1[3] = c.nx + 1; N.y0[3] = 1; N.y1[3] = c.ny;
+                    N.P = ctx->P;
+                    N.nf = 4;
+                }
+                M.sst[0] = ctx->at<Q>(ctx->sst[0]);
+                M.sst[1] = ctx->at<Q>(ctx->sst[1]);
+                M.um = (ST *)ctx->um.base;
+                M.vm = (ST *)ctx->vm.base;
+                M.elems = (long long)ctx->elems;
+                M.tracer = c.tracer_advection ? 1 : 0;
+                M.nadvstep = c.tracer_advection ? (int)c.nadvstep : 1;
+                M.nout = c.nout > 0 ? (int)c.nout : 0;
+                M.r_adv = (int)(i0 % M.nadvstep);
+                M.r_half = (int)((i0 + (c.tracer_advection ? c.nadvstep_half : 0)) % M.nadvstep);
+                M.r_nan = M.nout > 0 ? (int)(i0 % M.nout) : 0;
+                M.rsst0 = ctx->rsst;
+                M.nanflag = dnan + q;
+            }
+            if (hipMemcpyAsync(dmem, mem.data(), nm * sizeof(PMember<F>), hipMemcpyHostToDevice, E->stream) != hipSuccess)
+                return E->fail(SWM_E_HIP, "persistent launch records");
+            A.item = ditem;
+            A.mem = dmem;
+            A.nm = (int)nm;
+            A.ctl = dctl;
+            A.ntrace = ntrace;
+            A.trace = dtrace;
+            {
+                const swm_config &c0 = E->m[0]->c;      // members share drag and diffusion
+                const int drag = c0.bottom_drag == SWM_DRAG_QUADRATIC ? 0 : c0.bottom_drag == SWM_DRAG_LINEAR ? 1 : 2;
+                A.diss = 2 * drag + (c0.diffusion == SWM_DIFF_SMAGORINSKY ? 0 : 1);
+            }
+            // the launch: as many blocks as stay resident, a multiple of the XCD count
+            auto kern = E->per ? (const void *)&ens_persist_kernel<F, true> : (const void *)&ens_persist_kernel<F, false>;
+            int dev = 0, ncu = 0, per_cu = 0, coop = 0, clk_khz = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+                hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVE * WAVES_PER_BLOCK, 0) != hipSuccess)
+                return E->fail(SWM_E_HIP, "persistent launch: device attributes");
+            const int nb = (per_cu * ncu) / NXCD * NXCD;
+            if (!coop || nb < NXCD) return skip("no cooperative launch on this device");
+            A.timeout = (long long)clk_khz * 1000 * 5;       // 5 s of the wall clock
+            int64_t left = nsteps, at = i0, stopped = -1;
+            std::vector<int> nanf(nm, 0);
+            while (left > 0 && stopped < 0) {
+                const int chunk = (int)std::min<int64_t>(left, 1 << 20);
+                A.nsteps = chunk;
+                // members' residues at the chunk's first step
+                if (at != i0) {
+                    for (size_t q = 0; q < nm; ++q) {
+                        const swm_config &c = E->m[q]->c;
+                        mem[q].r_adv = (int)(at % mem[q].nadvstep);
+                        mem[q].r_half = (int)((at + (c.tracer_advection ? c.nadvstep_half : 0)) % mem[q].nadvstep);
+                        mem[q].r_nan = mem[q].nout > 0 ? (int)(at % mem[q].nout) : 0;
+                        mem[q].rsst0 = E->m[q]->rsst;
+                    }
+                    if (hipMemcpyAsync(dmem, mem.data(), nm * sizeof(PMember<F>), hipMemcpyHostToDevice, E->stream) != hipSuccess)
+                        return E->fail(SWM_E_HIP, "persistent launch records");
+                }
+                PCtl c0{};
+                c0.stop = 0x7fffffff;
+                if (hipMemcpyAsync(dctl, &c0, sizeof c0, hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+                    hipMemcpyAsync(ditem, items, sizeof items, hipMemcpyHostToDevice, E->stream) != hipSuccess ||
+                    hipMemsetAsync(dnan, 0, nm * sizeof(int), E->stream) != hipSuccess)
+                    return E->fail(SWM_E_HIP, "persistent launch control");
+                void *args[] = {&A};
+                if (hipLaunchCooperativeKernel(kern, dim3(nb), dim3(WAVE * WAVES_PER_BLOCK), args, 0, E->stream) != hipSuccess)
+                    return E->fail(SWM_E_HIP, "persistent launch");
+                PCtl c1{};
+                if (hipMemcpyAsync(&c1, dctl, sizeof c1, hipMemcpyDeviceToHost, E->stream) != hipSuccess ||
+                    hipMemcpyAsync(nanf.data(), dnan, nm * sizeof(int), hipMemcpyDeviceToHost, E->stream) != hipSuccess ||
+                    hipStreamSynchronize(E->stream) != hipSuccess)
+                    return E->fail(SWM_E_HIP, "persistent launch");
+                if (c1.abort) return E->fail(SWM_E_HIP, "persistent launch: a grid barrier timed out (members' states undefined)");
+                const int ran_steps = c1.stop < chunk ? c1.stop + 1 : chunk;
+                if (ntrace > 0 && at == i0) {
+                    std::vector<long long> tr(ntrace);
+                    if (hipMemcpy(tr.data(), dtrace, ntrace * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess)
+                        return E->fail(SWM_E_HIP, "persistent launch trace");
+                    phase_times(E, A, items, mem, tr, ran_steps, clk_khz);
+                }
+                // the members' slots and tracer slot after the steps that ran
+                for (size_t q = 0; q < nm; ++q) {
+                    swm_ctx *ctx = E->m[q];
+                    const PMember<F> &M = mem[q];
+                    const bool nan = c1.stop < chunk && nanf[q];
+                    // steps since i0 whose copy-back happened: the slots repeat every two steps
+                    const int64_t acc = (at - i0) + (nan ? ran_steps - 1 : ran_steps);
+                    ctx->pu = (acc & 1) ? pl[0].next_pu[q] : pu[q];
+                    ctx->pe = (acc & 1) ? pl[0].next_pe[q] : pe[q];
+                    if (M.tracer) {
+                        const int a0 = (M.nadvstep - M.r_adv) % M.nadvstep;
+                        const int n = ran_steps > a0 ? (ran_steps - 1 - a0) / M.nadvstep + 1 : 0;
+                        ctx->rsst = M.rsst0 ^ (n & 1);
+                    }
+                    if (nan) nan_steps[q] = at + c1.stop;
+                }
+                if (c1.stop < chunk) stopped = at + c1.stop;
+                at += ran_steps;
+                left -= ran_steps;
+                *done += ran_steps;
+                // the next chunk starts at the parity reached
+                if (left > 0 && stopped < 0 && ((at - i0) & 1)) {
+                    std::swap(A.plan[0], A.plan[1]);
+                    std::swap(items[0], items[1]);
+                    std::swap(A.nitems[0], A.nitems[1]);
+                    for (PMember<F> &M : mem) {
+                        std::swap(M.tu[0], M.tu[1]); std::swap(M.tv[0], M.tv[1]);
+                        std::swap(M.su[0], M.su[1]); std::swap(M.sv[0], M.sv[1]);
+                        std::swap(M.nan[0], M.nan[1]);
+                    }
+                }
+            }
+            E->persist_steps += *done;
+            *ran = true;
+            return SWM_OK;
+        }
+    }
+
+    // Average duration of the phases of the persistent launch's steps from block 0's barrier
+    // exit times: the first fused pair, the second, the dissipation (each with its barrier),
+    // over the steps that follow a step without tracer or NaN-check work; and whole steps.
+    static void phase_times(swm_ens *E, const PArgs<F> &A, const PItem (&items)[2][PERSIST_ITEMS], const std::vector<PMember<F>> &mem,
+                            const std::vector<long long> &tr, int steps, int clk_khz) {
+        double sum[4] = {0, 0, 0, 0};
+        int cnt[4] = {0, 0, 0, 0};
+        int k = 0;                                        // barriers so far
+        bool clean = false;                               // the previous step had no events
+        const int ntr = (int)tr.size();
+        auto ms = [&](int b) { return (double)(tr[b] - tr[b - 1]) / clk_khz; };
+        for (int s = 0; s < steps && k + 8 < ntr; ++s) {
+            const int p = s & 1, k0 = k;
+            for (int j = 0; j < A.nitems[p]; ++j) {
+                const PItem &it = items[p][j];
+                if (!it.sync) continue;
+                ++k;
+                const int ph = it.kind == 2 ? 2 : it.ma == M_RK_FIRST ? 0 : 1;
+                if (clean) { sum[ph] += ms(k); cnt[ph] += 1; }
+            }
+            bool snap = false, adv = false, both = false, nan = false;
+            for (const PMember<F> &M : mem) {
+                const bool sn = M.tracer && (M.r_half + s) % M.nadvstep == 0, ad = M.tracer && (M.r_adv + s) % M.nadvstep == 0;
+                snap |= sn; adv |= ad; both |= sn && ad;
+                nan |= M.nout > 0 && (M.r_nan + s) % M.nout == 0;
+            }
+            k += (snap && both) + (adv && nan) + nan;
+            if (clean && !(snap || adv || nan) && k0 > 0) {
+                sum[3] += (double)(tr[k] - tr[k0]) / clk_khz;
+                cnt[3] += 1;
+            }
+            clean = !(snap || adv || nan);
+        }
+        for (int q = 0; q < 4; ++q) E->phase_ms[q] = cnt[q] ? sum[q] / cnt[q] : -1.0;
+    }
+
     static int step(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, int64_t *gsteps) {
+        const size_t nm = E->m.size();
+        *done = 0;
+        *gsteps = 0;
+        E->persist_steps = 0;
+        for (size_t k = 0; k < nm; ++k) nan_steps[k] = 0;
+        bool ran = false;
+        int rc = persist(E, nsteps, i0, done, nan_steps, &ran);
+        if (rc) return rc;
+        if (ran) return check_tracer_flags(E);
+        if (E->persist_why == WHY_SETTLE && nsteps > 1) {
+            // a fresh state: the slot roles settle into their two-step cycle after one step
+            if ((rc = batched(E, 1, i0, done, nan_steps, gsteps))) return rc;
+            if (*done < 1) return check_tracer_flags(E);
+            for (size_t k = 0; k < nm; ++k)
+                if (nan_steps[k]) return check_tracer_flags(E);
+            int64_t d = 0;
+            if ((rc = persist(E, nsteps - 1, i0 + 1, &d, nan_steps, &ran))) return rc;
+            if (ran) {
+                *done += d;
+                return check_tracer_flags(E);
+            }
+            int64_t g = 0;
+            rc = batched(E, nsteps - 1, i0 + 1, &d, nan_steps, &g);
+            *done += d;
+            *gsteps += g;
+            return rc;
+        }
+        return batched(E, nsteps, i0, done, nan_steps, gsteps);
+    }
+
+    // steps as batched launches, pairs of steps without host work as graph replays
+    static int batched(swm_ens *E, int64_t nsteps, int64_t i0, int64_t *done, int64_t *nan_steps, int64_t *gsteps) {
         const size_t nm = E->m.size();
         *done = 0;
         *gsteps = 0;
@@ -1482,6 +1788,12 @@ template <class F> struct Ens {
             *done = k;
             if (any_nan) break;
         }
+        return check_tracer_flags(E);
+    }
+
+    // a tracer departure point that was NaN or outside the grid in any member
+    static int check_tracer_flags(swm_ens *E) {
+        const size_t nm = E->m.size();
         if (hipStreamSynchronize(E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "stream synchronize");
         int flags[2] = {0, 0};
         for (size_t q = 0; q < nm; ++q) {

@@ -230,8 +230,8 @@ class Simulation:
 
 class Ensemble:
     """Several ``Simulation`` s advanced together (``swm_ens_*``): one batched launch per
-    kernel of a step for all members, graph replays of step pairs, an autotuner for the
-    segment lengths.  The members stay usable on their own (``download`` ...)."""
+    kernel of a step for all members and graph replays of step pairs, or (``persistent``)
+    all steps in one persistent launch; an autotuner for the segment lengths.  The members stay usable on their own (``download`` ...)."""
 
     def __init__(self, members):
         self.members = list(members)
@@ -262,6 +262,38 @@ class Ensemble:
         self._check(_abi.load().swm_ens_step(self.ens, nsteps, i0, ctypes.byref(done), nan, ctypes.byref(graph)),
                     "swm_ens_step")
         return done.value, list(nan), graph.value
+
+    @property
+    def persistent(self):
+        return getattr(self, "_persistent", False)
+
+    @persistent.setter
+    def persistent(self, on):
+        """Run eligible steps in one persistent launch or as batched launches and graph
+        replays (default, faster); the results are the same.  "timed": persistent, with its
+        phases timed (``phase_ms``)."""
+        mode = 2 if on == "timed" else (1 if on else 0)
+        self._check(_abi.load().swm_ens_set_persistent(self.ens, mode), "swm_ens_set_persistent")
+        self._persistent = on
+
+    def phase_ms(self):
+        """Average ms of the persistent launch's phases in the last timed call: first fused
+        stage pair, second, dissipation, whole step (each phase up to the barrier after it)."""
+        out = (ctypes.c_double * 4)()
+        self._check(_abi.load().swm_ens_phase_ms(self.ens, out), "swm_ens_phase_ms")
+        return {"fused12": out[0], "fused34": out[1], "dissipation": out[2], "step": out[3]}
+
+    def persistent_steps(self) -> int:
+        """Steps of the last ``step`` call that ran in the persistent launch."""
+        return self.persistent_status()[0]
+
+    def persistent_status(self):
+        """(steps of the last ``step`` call that ran in the persistent launch, why it was not
+        used when it was not)."""
+        n, why = ctypes.c_int64(0), ctypes.c_char_p()
+        self._check(_abi.load().swm_ens_persistent_steps(self.ens, ctypes.byref(n), ctypes.byref(why)),
+                    "swm_ens_persistent_steps")
+        return n.value, (why.value or b"").decode()
 
     def close(self):
         if getattr(self, "ens", None):

@@ -54,12 +54,15 @@ def assert_equal(S, a, b, tag):
 
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic", "periodic-flat", "periodic-mixed-seams"])
 @pytest.mark.parametrize("events", [False, True])
-def test_ensemble_matches_single_runs(swm, orc, gpu, bc, events):
-    """Without tracer / NaN-check steps the ensemble runs step pairs as graph replays; with
-    them (tracer every few steps, NaN check every step) it runs them eagerly in between.
-    Periodic members on a flat bottom run fused stage pairs wrapping around the seam; with
-    one member whose seam does not allow that, every member takes the form with the seam
-    strips on their own kernel (as do the seamount members of "periodic")."""
+@pytest.mark.parametrize("persistent", [True, False])
+def test_ensemble_matches_single_runs(swm, orc, gpu, bc, events, persistent):
+    """In the persistent launch every step runs inside one cooperative kernel, tracer and
+    NaN-check steps included.  Without it: without tracer / NaN-check steps the ensemble runs
+    step pairs as graph replays; with them (tracer every few steps, NaN check every step) it
+    runs them eagerly in between.  Periodic members on a flat bottom run fused stage pairs
+    wrapping around the seam; with one member whose seam does not allow that, every member
+    takes the form with the seam strips on their own kernel (as do the seamount members of
+    "periodic")."""
     n = 14
     topo = ["flat", "flat", "ridge", "flat", "flat"] if bc == "periodic-mixed-seams" else None
     ms = members(swm, "periodic" if bc == "periodic-mixed-seams" else bc, Uadv=30.0 if events else 0.25,
@@ -72,13 +75,17 @@ def test_ensemble_matches_single_runs(swm, orc, gpu, bc, events):
         for sim, (S, p) in zip(sims, ms):
             sim.upload(p)
         with swm.Ensemble(sims) as ens:
+            ens.persistent = persistent
             done, nan, graph = ens.step(n, 1)
             assert done == n and not any(nan)
+            used, why = ens.persistent_status()
+            # a fresh state takes one batched step to settle its slot roles
+            assert used == (n - 1 if persistent else 0), why
         for sim, (S, p), r in zip(sims, ms, ref):
             got = sim.download()
             assert_equal(S, r, got, f"{bc} nx={S.grid.nx}")
             assert_oracle(swm, orc, S, p, n, got, f"{bc} nx={S.grid.nx}")
-        if not events:      # the first steps settle the slot roles; then step pairs replay the graph
+        if not events and not persistent:   # the first steps settle the slot roles; then step pairs replay the graph
             assert graph >= n - 4, graph
     finally:
         for sim in sims:
@@ -272,3 +279,46 @@ def test_refused_ensemble_step_keeps_the_pending_update(swm, gpu):
     finally:
         for sim in sims:
             sim.close()
+
+
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+def test_persistent_launch_in_calls_with_a_nan_stop(swm, gpu, bc):
+    """The persistent launch against the batched launches and graph replays, call by call:
+    odd and even step counts (the slot roles and tracer slot carried between calls), events
+    on each member's own schedule, then a member whose tracer holds a NaN: every member stops
+    after the first step whose check sees it, the member keeping its state from before that
+    step (feedback.jl:136-148, time_integration.jl:235-237)."""
+    ms = members(swm, bc, Uadv=10.0, output_dt=0.5)
+    nouts = [S.grid.nout for S, _ in ms]
+    assert min(nouts) >= 2 and len(set(nouts)) > 1, nouts
+    assert len(set(S.grid.nadvstep for S, _ in ms)) > 1
+    results = {}
+    for persistent in (True, False):
+        sims = [swm.Simulation(S) for S, _ in ms]
+        try:
+            for sim, (S, p) in zip(sims, ms):
+                sim.upload(p)
+            with swm.Ensemble(sims) as ens:
+                ens.persistent = persistent
+                calls = []
+                for nsteps, i0 in ((3, 1), (4, 4), (1, 8)):
+                    done, nan, _ = ens.step(nsteps, i0)
+                    calls.append((done, tuple(nan), ens.persistent_steps()))
+                mid = [sim.download() for sim in sims]
+                bad = mid[2]
+                bad.sst[5, 7] = np.nan
+                sims[2].upload(bad)
+                done, nan, _ = ens.step(20, 9)
+                calls.append((done, tuple(nan), ens.persistent_steps()))
+            results[persistent] = (calls, [sim.download() for sim in sims])
+        finally:
+            for sim in sims:
+                sim.close()
+    (cp, sp), (cg, sg) = results[True], results[False]
+    assert [c[:2] for c in cp] == [c[:2] for c in cg], (cp, cg)
+    assert all(c[0] - 1 <= c[2] <= c[0] for c in cp) and all(c[2] == 0 for c in cg), (cp, cg)
+    done, nan = cp[3][:2]
+    stop = 9 + done - 1
+    assert stop % nouts[2] == 0 and nan[2] == stop and sum(1 for x in nan if x) == 1, (done, nan, nouts)
+    for (S, _), a, b in zip(ms, sp, sg):
+        assert_equal(S, a, b, f"{bc} nx={S.grid.nx}")

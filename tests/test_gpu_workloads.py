@@ -88,11 +88,14 @@ def test_double_gyre_4096_reduced_precision_and_float64(swm, orc, gpu):
             assert_bitwise(name, f.result()[:4], got)
 
 
-def test_ensemble_of_64_members_against_the_oracle(swm, orc, gpu):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_ensemble_of_64_members_against_the_oracle(swm, orc, gpu, persistent):
+    """Batched launches and graph replays, or the persistent launch (after the one batched
+    step that settles a fresh state's slot roles)."""
     ms = swm.workloads.ensemble_members(64, seed=4)
     assert sorted({S.grid.nx for S, _ in ms}) == [128, 256, 512, 1024]
-    n = 2
-    print(f"ensemble of {len(ms)}: {n} steps", flush=True)
+    n = 3
+    print(f"ensemble of {len(ms)}: {n} steps, persistent={persistent}", flush=True)
     with ThreadPoolExecutor(8) as ex:
         refs = [ex.submit(orc.run, S, swm.make_config(S), p, n) for S, p in ms]
         sims = [swm.Simulation(S) for S, _ in ms]
@@ -100,8 +103,11 @@ def test_ensemble_of_64_members_against_the_oracle(swm, orc, gpu):
             for sim, (_, p) in zip(sims, ms):
                 sim.upload(p)
             with swm.Ensemble(sims) as ens:
+                ens.persistent = persistent
                 done, nan, _ = ens.step(n, 1)
+                used, why = ens.persistent_status()
             assert done == n and not any(nan)
+            assert used == (n - 1 if persistent else 0), why
             for k, (sim, (S, _), r) in enumerate(zip(sims, ms, refs)):
                 assert_bitwise(f"member {k} ({S.grid.nx}²)", r.result()[:4], sim.download())
         finally:
