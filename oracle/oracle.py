@@ -210,7 +210,7 @@ def arith(fmt_code, op, a, b):
 def _statics64(S):
     F, G = S.forcing, S.grid
     return [np.asfortranarray(a, dtype=np.float64) if a is not None else None
-            for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη)]
+            for a in (F.H, F.Fx, F.Fy, G.f_q, F.η_ref, F.Fη, G.f_u, G.f_v)]
 
 
 def _out_shapes(S, op):
@@ -235,7 +235,7 @@ def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0, dFx=None):
     dy = [np.zeros(s, order="F") for s in shapes]
     vp = ctypes.c_void_p
     L = lib()
-    L.swmo_tlm.argtypes = [vp] * 7 + [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [vp] * 5
+    L.swmo_tlm.argtypes = [vp] * 9 + [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [vp] * 5
     arr = lambda lst: (vp * len(lst))(*[a.ctypes.data if a is not None else None for a in lst])
     xs, ys, dys = arr(x), arr(y), arr(dy)
     dxs = arr(dx) if dx is not None else None
@@ -259,7 +259,7 @@ def tape(S, cfg, i, t, u, v, e, sst, um, vm):
     out = [np.zeros(s, order="F") for s in shapes]
     vp = ctypes.c_void_p
     L = lib()
-    L.swmo_tape.argtypes = [vp] * 7 + [ctypes.c_int64, ctypes.c_int64] + [vp] * 7
+    L.swmo_tape.argtypes = [vp] * 9 + [ctypes.c_int64, ctypes.c_int64] + [vp] * 7
     ins = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, e, sst, um, vm)]
     ptrs = (vp * len(out))(*[a.ctypes.data for a in out])
     st = _statics64(S)

@@ -1295,6 +1295,7 @@ static int tape_typed(const swm_config *c, const double *const *st, int64_t i, i
         if (p) for (size_t k = 0; k < A.size(); ++k) A.a[k] = (T)p[k];
     };
     in(M.H, st[0]); in(M.Fx, st[1]); in(M.Fy, st[2]); in(M.f_q, st[3]); in(M.eta_ref, st[4]); in(M.Feta, st[5]);
+    if (c->dynamics == SWM_LINEAR) { in(M.f_u, st[6]); in(M.f_v, st[7]); }
     in(M.u, x[0]); in(M.v, x[1]); in(M.eta, x[2]); in(M.sst, x[3]); in(M.um, x[4]); in(M.vm, x[5]);
     M.init();
     M.tape = &tp;
@@ -1400,7 +1401,8 @@ static void store_dual(const Arr<Dual> &A, double *v, double *d) {
 }
 
 int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
-             const double *eta_ref, const double *Feta, int32_t op, int64_t nsteps, int64_t i0, int64_t t0,
+             const double *eta_ref, const double *Feta, const double *f_u, const double *f_v, int32_t op,
+             int64_t nsteps, int64_t i0, int64_t t0,
              const double *const *x, const double *const *dx, double *const *y, double *const *dy, const double *dFx) {
     if (c->scale_log2 != 0 || c->compensated) return -1;   // the tangent-linear model is of the plain step
     auto dxk = [&](int k) { return dx ? dx[k] : nullptr; };
@@ -1413,6 +1415,11 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
         load_dual(M.f_q, f_q, nullptr);
         if (eta_ref) load_dual(M.eta_ref, eta_ref, nullptr);
         if (Feta) load_dual(M.Feta, Feta, nullptr);
+        if (c->dynamics == SWM_LINEAR) {
+            if (!f_u || !f_v) return -1;
+            load_dual(M.f_u, f_u, nullptr);
+            load_dual(M.f_v, f_v, nullptr);
+        }
         if (op == 0) {
             load_dual(M.u, x[0], dxk(0));
             load_dual(M.v, x[1], dxk(1));
@@ -1474,11 +1481,13 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
 // then u0, v0, η0 after ghost_points! (:198), then u0, v0 after the dissipation, then um, vm
 // after the step, then the new u, v, η, sst.  Classic RK only.
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
-              const double *eta_ref, const double *Feta, int64_t i, int64_t t, const double *u, const double *v,
-              const double *eta, const double *sst, const double *um, const double *vm, double *const *out) {
+              const double *eta_ref, const double *Feta, const double *f_u, const double *f_v, int64_t i, int64_t t,
+              const double *u, const double *v, const double *eta, const double *sst, const double *um,
+              const double *vm, double *const *out) {
     if (c->time_scheme != SWM_RK || c->T != c->Tprog || c->Tcomm != c->Tprog || c->scale_log2 != 0 || c->compensated)
         return -1;
-    const double *st[6] = {H, Fx, Fy, f_q, eta_ref, Feta};
+    if (c->dynamics == SWM_LINEAR && (!f_u || !f_v)) return -1;
+    const double *st[8] = {H, Fx, Fy, f_q, eta_ref, Feta, f_u, f_v};
     const double *x[6] = {u, v, eta, sst, um, vm};
     try {
         if (c->T == SWM_F64) return tape_typed<double>(c, st, i, t, x, out);

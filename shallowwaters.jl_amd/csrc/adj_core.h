@@ -52,6 +52,10 @@ constexpr int RT_HALO = 2;  // stage_adjoint
 constexpr int DT_HALO = 3;  // diss_adjoint
 // output rows of both marches: [-1, ny+3) (u rows -1 .. ny+2 of the nonperiodic halo)
 constexpr int ADJ_Y0 = -1;
+
+// the right-hand side forms of the stage adjoint (template argument SCH): the nonlinear
+// dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics
+enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2 };
 SWM_HD int adj_y1(int ny) { return ny + 3; }
 
 template <class T> SWM_HD T amin(T a, T b) { return a < b ? a : b; }
@@ -89,6 +93,8 @@ template <class R> struct StageAdjArgs {
     UVE s;                 // stage input s_k (u1, v1, η1 after ghost_points!)
     uint32_t H, fq;        // static fields (f_q may be one value per row: fq_2d = 0)
     int fq_2d;
+    uint32_t fu, fv;       // linear dynamics: f on the u- and v-grid (one value per row when fu_2d / fv_2d = 0)
+    int fu_2d, fv_2d;
     uint32_t qend;         // periodic: q at column nx for every corner row of s_k (1-D)
     UVE l0;                // λ of u0 after the stages (halo unfolded)
     UVE mu_in;             // μ_{k+1} = (∂R/∂s_{k+1})ᵀ λd_{k+1} (halo unfolded)
@@ -179,18 +185,26 @@ template <class V> struct StageLoads {
     V lu, lv, le, mu, mv, me;  // cells of λ0 and μ_{k+1} at row r-1 (the folds add their halo)
     V au, av, ae;              // running sum A at the row emitted, r-2
     V fx;                      // λFx at the row emitted, r-2 (fxg)
+    V fu, fv;                  // linear dynamics: f_u at row r-1, f_v at row r-2
 };
 
-template <class W, bool PER, bool NEXT, bool A_L0>
+template <class W, bool PER, bool NEXT, bool A_L0, bool LIN = false>
 SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<typename W::R> &A, int r, int ya, int yb) {
     using V = typename W::V;
     using R = typename W::R;
     const V zero = V(R(0));
     StageLoads<V> p;
     const int rs = amin(r, yb + 1), ny = A.ny;
-    p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs); p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs);
-    p.fq = w.ld_s(A.fq, rs - 1, A.fq_2d);
-    p.qe = PER ? w.ld_row1(A.qend, rs - 1) : zero;
+    if (LIN) {                 // the linear stage's transpose does not depend on the state
+        p.u = p.v = p.e = p.H = p.fq = p.qe = zero;
+        p.fu = w.ld_s(A.fu, amax(rs - 1, -1), A.fu_2d);
+        p.fv = w.ld_s(A.fv, amax(rs - 2, -1), A.fv_2d);
+    } else {
+        p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs); p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs);
+        p.fq = w.ld_s(A.fq, rs - 1, A.fq_2d);
+        p.qe = PER ? w.ld_row1(A.qend, rs - 1) : zero;
+        p.fu = p.fv = zero;
+    }
     const int yu = amin(amax(rs - 1, 1), ny), yv = amin(amax(rs - 1, 1), amax(ny - 1, 1));
     p.lu = w.ld(A.l0.u, yu); p.lv = w.ld(A.l0.v, yv); p.le = w.ld(A.l0.e, yu);
     p.mu = p.mv = p.me = zero;
@@ -237,11 +251,21 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
 // (no column wraps: q_u and q_v read no q[end,·]).  Reverse, with λqhv = gu, λqhu = -gv:
 //   λQu = gu·Vu, λVu = gu·Qu, λQv = -gv·Uv, λUv = -gv·Qv, and each of them gathered back
 //   onto the q, V, U it read.
-template <class W, bool PER, bool NEXT, bool A_L0, bool PRE, bool SAD = false>
+//
+// LIN: the linear dynamics (rhs_linear!, rhs.jl:55-86), with f_u, f_v on the u- and v-grid:
+//   du = f_u·(¼(v(y-1) + v(x+1,y-1)) + ¼(v + v(x+1))) - (gη(x+1) - gη) + F
+//   dv = -(f_v·(¼(u(x-1) + u) + ¼(u(x-1,y+1) + u(y+1))) + (gη(y+1) - gη)) + F
+//   dη = 0 under the classic RK scheme (no mass fluxes are formed, continuity.jl:83-85),
+//   plus the relaxation's -γη.  The transpose is independent of the state:
+//   λu = -¼·((f_v·gv + (f_v·gv)(x+1)) + ((f_v·gv)(y-1) + (f_v·gv)(x+1,y-1)))
+//   λv = ¼·((f_u·gu + (f_u·gu)(x-1)) + ((f_u·gu)(y+1) + (f_u·gu)(x-1,y+1)))
+//   λη = g·λp (- γ·ge), λp as above.
+template <class W, bool PER, bool NEXT, bool A_L0, bool PRE, int SCH = ADJ_AH>
 SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, int yb) {
     using R = typename W::R;
     using V = typename W::V;
     using M = typename W::M;
+    constexpr bool SAD = SCH == ADJ_SAD, LIN = SCH == ADJ_LIN;
     const int nx = A.nx, ny = A.ny;
     const auto x = w.x;
     const M ucol = PER ? M((x >= 0) & (x <= nx - 1)) : M((x >= 1) & (x <= nx - 1));
@@ -264,11 +288,12 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     V su_2 = zero, sv_2 = zero, se_2 = zero;           // λs_{k+1} of row r-2
     V zu_2 = zero, zv_2 = zero, ze_2 = zero;           // folded λ0 of row r-2 (A_L0)
     V Qu_2 = zero, Vu_2 = zero, Uv_2 = zero;           // Sadourny: Qu, Vu of u row r-2, Uv of v row r-2
+    V pu_2 = zero, pv_3 = zero;                        // linear: f_u·λdu of row r-2, f_v·λdv of row r-3
 
-    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0>(w, A, ya - 2, ya, yb);
+    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0, LIN>(w, A, ya - 2, ya, yb);
     for (int r = ya - 2; r <= yb + 1; ++r) {
         const StageLoads<V> c = nl;
-        nl = stage_loads<W, PER, NEXT, A_L0>(w, A, r + 1, ya, yb);
+        nl = stage_loads<W, PER, NEXT, A_L0, LIN>(w, A, r + 1, ya, yb);
         // ---- stage input row r
         const V u_0 = c.u, v_0 = c.v, e_0 = c.e, H_0 = c.H;
         const V fq_1 = c.fq;
@@ -293,6 +318,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const V gv_1 = W::sel(vcol & ydv, NEXT ? A.a * zv + A.b * sv : A.a * zv, zero);
         const V ge_1 = W::sel(vcol & ydu, NEXT ? A.a * ze + A.b * se : A.a * ze, zero);
 
+        V lu_2, lv_2, le_2, lp_1;
         // ---- forward: U of row r; V, hq, q and the weights of row r-1
         const V h_0 = e_0 + H_0;
         const V hx_0 = W::xp(h_0);
@@ -310,7 +336,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const V qxm_1 = PER && !SAD ? W::sel(qfix, qend_1, W::xm(q_1)) : W::xm(q_1);
         V qa_1 = zero, qb_1 = zero, qg_1 = zero, qd_1 = zero, la_1 = zero, lb_1 = zero, lg_1 = zero, ld_1 = zero;
         V lU_1, lq_2, lV_2, Qu_1 = zero, Vu_1 = zero, Uv_1 = zero;
-        const V lp_1 = (gu_1 - W::xm(gu_1)) + (gv_1 - gv_2);
+        lp_1 = (gu_1 - W::xm(gu_1)) + (gv_1 - gv_2);
         if constexpr (!SAD) {
             qa_1 = k12 * (q_2 + q_1 + qxp_1);
             qb_1 = k12 * (q_2 + q_1 + qxm_1);
@@ -355,10 +381,22 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const V lhv_2 = lV_2 * v_2;
         const V lh_2 = half * (lhu_2 + W::xm(lhu_2)) + half * (lhv_2 + lhv_3) +
                        quarter * ((lhq_2 + W::xm(lhq_2)) + (lhq_3 + W::xm(lhq_3)));
-        const V lu_2 = (lU_2 * hu_2 + u_2 * (half * (lp_2 + W::xp(lp_2)))) + (lnum_2 - lnum_3);
-        const V lv_2 = (lV_2 * hv_2 + v_2 * (half * (lp_2 + lp_1))) + (W::xm(lnum_2) - lnum_2);
-        V le_2 = lh_2 + g * lp_2;
+        lu_2 = (lU_2 * hu_2 + u_2 * (half * (lp_2 + W::xp(lp_2)))) + (lnum_2 - lnum_3);
+        lv_2 = (lV_2 * hv_2 + v_2 * (half * (lp_2 + lp_1))) + (W::xm(lnum_2) - lnum_2);
+        le_2 = lh_2 + g * lp_2;
         if (A.cont == 1) le_2 = le_2 - A.gamma * ge_2;          // surface relaxation: dη += γ(η_ref - η)
+        if constexpr (LIN) {
+            // the linear dynamics' transpose (see above): f_u of rows r-1 and r-2, f_v of rows
+            // r-2 and r-3, the tendency adjoint of rows r-1 .. r-3
+            // (each u / v point's own f times its cotangent, then the four points summed)
+            const V pu_1 = c.fu * gu_1, pv_2 = c.fv * gv_2;
+            lv_2 = quarter * (pu_2 + W::xm(pu_2)) + quarter * (pu_1 + W::xm(pu_1));
+            lu_2 = -(quarter * (pv_2 + W::xp(pv_2)) + quarter * (pv_3 + W::xp(pv_3)));
+            le_2 = g * lp_2;
+            if (A.cont == 1) le_2 = le_2 - A.gamma * ge_2;      // surface relaxation: dη += γ(η_ref - η)
+            pu_2 = pu_1;
+            pv_3 = pv_2;
+        }
 
         // ---- emit row y = r-2
         const int y = r - 2;
@@ -391,7 +429,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
                 w.st(A.mu_out.e, y, le_2, st_e & ye_st);
             }
             // periodic: λ of q(nx, y) from qβ(0, y) and qγ(0, y+1)
-            if (PER && !SAD && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
+            if (PER && !SAD && !LIN && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
             // the wind stress enters du as Tprog(Fxt·Fx) (rhs.jl:242): λFx += Fxt·λdu, u cells
             if (A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
         }
@@ -927,6 +965,8 @@ struct AdjFields {
     uint32_t Lsst, Lsst2, Lum, Lvm;
     uint32_t H, fq;
     int fq_2d;
+    uint32_t fu, fv;                // linear dynamics: f on the u- and v-grid
+    int fu_2d, fv_2d;
     uint32_t qend, lqend;
     uint32_t tr0, tr1;
     uint32_t tg0, tg1;              // the tracer adjoint's gathered interpolation records
@@ -944,6 +984,7 @@ template <class R> struct AdjConsts {
     R fxt;                          // the step's factor of Fx in du (1 unless the wind is seasonal)
     int fxg;                        // accumulate the cotangent of Fx
     int sad;                        // Sadourny's PV fluxes (k12 = ½)
+    int lin;                        // the linear dynamics (rhs_linear!)
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -989,6 +1030,8 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
     // the RK stages, last first (time_integration.jl:55-78)
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;
+    S.fu = F.fu; S.fv = F.fv; S.fu_2d = F.fu_2d; S.fv_2d = F.fv_2d;
+    const int sch = c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH;
     S.l0 = l0;
     S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
@@ -1003,9 +1046,9 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
         S.a_in = F.A;
         S.mu_out = next ? mu_next : mu_prev;
         S.a_out = pre ? out : F.A;
-        const bool wrap = c.per && !c.sad;              // the Arakawa-Hsu first-column wrap to q[end,·]
+        const bool wrap = c.per && sch == ADJ_AH;       // the Arakawa-Hsu first-column wrap to q[end,·]
         if (wrap && (rc = k.qend(S))) return rc;
-        if ((rc = k.stage(S, c.per, next, al0, pre, c.sad))) return rc;
+        if ((rc = k.stage(S, c.per, next, al0, pre, sch))) return rc;
         if (wrap && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
         if (next) { const UVE t = mu_prev; mu_prev = mu_next; mu_next = t; }
     }

@@ -733,9 +733,11 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (adv_form(c) != ADV_AH && adv_form(c) != ADV_SAD)
+        if (adv_form(c) != ADV_AH && adv_form(c) != ADV_SAD && adv_form(c) != ADV_LIN)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear right-hand side with the Arakawa-Hsu or "
-                                                "Sadourny scheme only (nstep_advcor = 0)");
+                                                "Sadourny scheme or of the linear dynamics (nstep_advcor = 0)");
+        if (c.dynamics == SWM_LINEAR && !ctx->have_fuv)
+            return ctx->fail(SWM_E_STATE, "the linear dynamics need f_u, f_v (swm_set_static_linear)");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
         if (ctx->ens_refs) return ctx->fail(SWM_E_STATE, "the context belongs to an ensemble; destroy it first");
         return SWM_OK;
@@ -777,6 +779,8 @@ template <class Fm> struct Adj {
         F.H = ctx->H.ref();
         F.fq = ctx->fq.ref();
         F.fq_2d = ctx->fq_sx;
+        F.fu = ctx->f_u.ref(); F.fu_2d = ctx->fu_sx;
+        F.fv = ctx->f_v.ref(); F.fv_2d = ctx->fv_sx;
         F.qend = ctx->qend.ref();
         st->F0 = F;
         // rows per wave: about four resident waves per wave slot
@@ -802,6 +806,7 @@ template <class Fm> struct Adj {
             k.b[j] = j < 3 ? (R)c.RKbDt[j] : R(0);
         }
         k.sad = c.adv_scheme == SWM_ADV_SADOURNY;
+        k.lin = c.dynamics == SWM_LINEAR;
         k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(k.sad ? 0.5 : 1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
         k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
         k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
@@ -820,24 +825,28 @@ template <class Fm> struct Adj {
             hipError_t e = hipGetLastError();
             return e == hipSuccess ? SWM_OK : ctx->fail(SWM_E_HIP, "adjoint kernel launch: %s", hipGetErrorString(e));
         }
-        template <bool PER, bool NEXT, bool AL0, bool PRE, bool SAD> void launch_stage(const adj::StageAdjArgs<R> &S) {
-            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE, SAD>), dim3(nblocks_waves(S.nstrips * S.nseg)),
+        template <bool PER, bool NEXT, bool AL0, bool PRE, int SCH> void launch_stage(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE, SCH>), dim3(nblocks_waves(S.nstrips * S.nseg)),
                                dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), S);
         }
-        template <bool PER, bool SAD> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
-            if (!next) launch_stage<PER, false, false, false, SAD>(S);
-            else if (al0 && pre) launch_stage<PER, true, true, true, SAD>(S);
-            else if (al0) launch_stage<PER, true, true, false, SAD>(S);
-            else if (pre) launch_stage<PER, true, false, true, SAD>(S);
-            else launch_stage<PER, true, false, false, SAD>(S);
+        template <bool PER, int SCH> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
+            if (!next) launch_stage<PER, false, false, false, SCH>(S);
+            else if (al0 && pre) launch_stage<PER, true, true, true, SCH>(S);
+            else if (al0) launch_stage<PER, true, true, false, SCH>(S);
+            else if (pre) launch_stage<PER, true, false, true, SCH>(S);
+            else launch_stage<PER, true, false, false, SCH>(S);
         }
-        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre, int sad) {
+        template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
+            if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
+            else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, al0, pre);
+            else stage_p<PER, adj::ADJ_AH>(S, next, al0, pre);
+        }
+        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre, int sch) {
             S.L = st->La;
             S.nstrips = adj::nstrips_for<adj::RT_HALO>(S.nx + 2);
             S.nseg = (adj::adj_y1(S.ny) - adj::ADJ_Y0 + S.L - 1) / S.L;
-            if (per) { if (sad) stage_p<true, true>(S, next, al0, pre); else stage_p<true, false>(S, next, al0, pre); }
-            else if (sad) stage_p<false, true>(S, next, al0, pre);
-            else stage_p<false, false>(S, next, al0, pre);
+            if (per) stage_s<true>(S, next, al0, pre, sch);
+            else stage_s<false>(S, next, al0, pre, sch);
             return fail_launch();
         }
         int qend(const adj::StageAdjArgs<R> &S) {
@@ -1154,6 +1163,7 @@ template <class Fm> struct Adj {
         const adj::AdjConsts<R> k = consts(ctx, 0);
         adj::StageAdjArgs<R> S{};
         S.s = Ft.s[1]; S.H = Ft.H; S.fq = Ft.fq; S.fq_2d = Ft.fq_2d; S.qend = Ft.qend; S.lqend = Ft.lqend;
+        S.fu = Ft.fu; S.fv = Ft.fv; S.fu_2d = Ft.fu_2d; S.fv_2d = Ft.fv_2d;
         S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.a_in = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
         S.a = k.a[1]; S.b = k.b[1]; S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = k.cont;
         S.nx = k.nx; S.ny = k.ny; S.P = k.P;
@@ -1166,7 +1176,7 @@ template <class Fm> struct Adj {
             case 0: { bool nan = false; return LF::step(ctx, 1, 0, &nan); }
             case 1: { adj::AdjFields G = st->F0; return tape_step(ctx, 0, G); }
             case 2: { st->F = Ft; return adj::reverse_step<R>(kl, st->F, k, 1); }
-            case 3: return kl.stage(S, k.per, 1, 0, 0, k.sad);
+            case 3: return kl.stage(S, k.per, 1, 0, 0, k.lin ? adj::ADJ_LIN : k.sad ? adj::ADJ_SAD : adj::ADJ_AH);
             case 4: return kl.diss(D, k.per, k.drag, k.smag);
             }
             return ctx->fail(SWM_E_ARG, "unknown adjoint kernel %d", which);

@@ -21,7 +21,7 @@ _lib = None
 class EmuCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("nx", "ny", "per", "RKo", "cont", "drag", "smag", "consume", "tracer", "nadvstep",
-                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad")] + \
+                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin")] + \
                [("a", ctypes.c_double * 4), ("b", ctypes.c_double * 4)] + \
                [(n, ctypes.c_double) for n in
                 ("g", "gamma", "k12", "oma", "cD", "rD", "cS", "nuB", "dtd", "hdu", "hdv", "du", "dv", "jSST")]
@@ -64,6 +64,7 @@ def config(S, cfg, L=8, nan_edges=False, fmt=8):
         k.b[j] = cfg.RKbDt[j] if j < 3 else 0.0
     k.g, k.gamma, k.oma = cfg.g, cfg.gamma, cfg.one_minus_alpha
     k.sad = int(cfg.adv_scheme == 1)                       # Sadourny: PV weight ½
+    k.lin = int(cfg.dynamics == 1)                         # the linear dynamics
     k.k12 = round_scalar(0.5 if k.sad else 1.0 / 12.0, P.T)
     k.cD, k.rD, k.cS, k.nuB, k.dtd = cfg.cD, cfg.rD, cfg.cSmag, cfg.nuB, cfg.Dt_diff
     k.hdu, k.hdv, k.du, k.dv, k.jSST = cfg.half_dtadvu, cfg.half_dtadvv, cfg.dtadvu, cfg.dtadvv, cfg.jSST

@@ -204,22 +204,26 @@ template <class R> struct EmuK {
             fn(W, t);
         }
     }
-    template <bool PER, bool NEXT, bool AL0, bool PRE, bool SAD> void stage_t(const StageAdjArgs<R> &S) {
+    template <bool PER, bool NEXT, bool AL0, bool PRE, int SCH> void stage_t(const StageAdjArgs<R> &S) {
         march<RT_HALO>(S.nx, S.ny, ADJ_Y0, adj_y1(S.ny), [&](EmuWave<R> &W, const Tile &t) {
-            stage_adjoint<EmuWave<R>, PER, NEXT, AL0, PRE, SAD>(W, S, t.ya, t.yb);
+            stage_adjoint<EmuWave<R>, PER, NEXT, AL0, PRE, SCH>(W, S, t.ya, t.yb);
         });
     }
-    template <bool PER, bool SAD> void stage_p(const StageAdjArgs<R> &S, int next, int al0, int pre) {
-        if (!next) stage_t<PER, false, false, false, SAD>(S);
-        else if (al0 && pre) stage_t<PER, true, true, true, SAD>(S);
-        else if (al0) stage_t<PER, true, true, false, SAD>(S);
-        else if (pre) stage_t<PER, true, false, true, SAD>(S);
-        else stage_t<PER, true, false, false, SAD>(S);
+    template <bool PER, int SCH> void stage_p(const StageAdjArgs<R> &S, int next, int al0, int pre) {
+        if (!next) stage_t<PER, false, false, false, SCH>(S);
+        else if (al0 && pre) stage_t<PER, true, true, true, SCH>(S);
+        else if (al0) stage_t<PER, true, true, false, SCH>(S);
+        else if (pre) stage_t<PER, true, false, true, SCH>(S);
+        else stage_t<PER, true, false, false, SCH>(S);
     }
-    int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre, int sad) {
-        if (per) { if (sad) stage_p<true, true>(S, next, al0, pre); else stage_p<true, false>(S, next, al0, pre); }
-        else if (sad) stage_p<false, true>(S, next, al0, pre);
-        else stage_p<false, false>(S, next, al0, pre);
+    template <bool PER> void stage_s(const StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
+        if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, al0, pre);
+        else if (sch == ADJ_SAD) stage_p<PER, ADJ_SAD>(S, next, al0, pre);
+        else stage_p<PER, ADJ_AH>(S, next, al0, pre);
+    }
+    int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre, int sch) {
+        if (per) stage_s<true>(S, next, al0, pre, sch);
+        else stage_s<false>(S, next, al0, pre, sch);
         return 0;
     }
     int qend(const StageAdjArgs<R> &S) {
@@ -288,7 +292,7 @@ extern "C" {
 // Scalars of a configuration, as the GPU driver derives them from swm_config.
 struct EmuCfg {
     int32_t nx, ny, per, RKo, cont, drag, smag, consume, tracer, nadvstep, nadvstep_half, nstep_diff;
-    int32_t L, nan_edges, fmt, sad;
+    int32_t L, nan_edges, fmt, sad, lin;
     double a[4], b[4];
     double g, gamma, k12, oma, cD, rD, cS, nuB, dtd, hdu, hdv, du, dv, jSST;
 };
@@ -315,7 +319,7 @@ template <class R> void get(Arena &A, uint32_t o, double *h, Shape s) {
 }
 
 struct Shapes {
-    Shape u, v, e, sst, fq;
+    Shape u, v, e, sst, fq, fu, fv;
 };
 Shapes shapes(const EmuCfg &c) {
     const int ep = c.per ? 1 : 0, nux = c.per ? c.nx : c.nx - 1;
@@ -325,6 +329,8 @@ Shapes shapes(const EmuCfg &c) {
     s.e = {c.nx + 2, c.ny + 2, 0, 0};
     s.sst = {c.nx + 2, c.ny, 0, 1};
     s.fq = {c.nx + 1, c.ny + 1, 0, 0};
+    s.fu = {nux, c.ny, 1 - ep, 1};
+    s.fv = {c.nx, c.ny - 1, 1, 1};
     return s;
 }
 
@@ -338,15 +344,17 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
     k.tracer = c.tracer; k.nadvstep = c.nadvstep; k.nadvstep_half = c.nadvstep_half; k.nstep_diff = c.nstep_diff;
     k.nx = c.nx; k.ny = c.ny; k.ep = c.per ? 1 : 0;
     k.sad = c.sad;
+    k.lin = c.lin;
     return k;
 }
 
-// op 1: μ = (∂R/∂s)ᵀ λd.   in: su sv se H fq gu gv ge;  out: μu μv μe
+// op 1: μ = (∂R/∂s)ᵀ λd.   in: su sv se H fq gu gv ge (+ f_u f_v with the linear dynamics);
+//       out: μu μv μe
 // op 2: λ before the dissipation.  in: u0 v0 e0 H lu lv le;  out: λu λv λe
 // op 3: tracer at step i (advection and snapshot as due).  in: u v um vm sst lum lvm lsst;
 //       out: λu λv λum λvm λsst
 // op 4: reverse of step i.  in: s_1..s_RKo (3 each), pre (3), post (2), um vm sst, H fq,
-//       L (u v e sst um vm);  out: L (u v e sst um vm)
+//       L (u v e sst um vm) (+ f_u f_v with the linear dynamics);  out: L (u v e sst um vm)
 // op 5: fold of the ghost copies.  in: Lu Lv Le Lsst;  out: gu gv ge gsst
 template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *const *in, double *const *out) {
     Arena A;
@@ -365,6 +373,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
     F.L = uve(); F.Z = uve(); F.Ma = uve(); F.Mb = uve(); F.A = uve();
     F.Lsst = A.add(true); F.Lsst2 = A.add(true); F.Lum = A.add(true); F.Lvm = A.add(true);
     F.H = A.add(true); F.fq = A.add(true); F.fq_2d = 1;
+    F.fu = A.add(true); F.fv = A.add(true); F.fu_2d = F.fv_2d = 1;
     F.qend = A.add(false); F.lqend = A.add(false);
     F.tr0 = A.add(true); F.tr1 = A.add(true);
     F.tg0 = A.add(true); F.tg1 = A.add(true);
@@ -373,15 +382,17 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
             put<R>(A, F.s[0].u, in[0], sh.u); put<R>(A, F.s[0].v, in[1], sh.v); put<R>(A, F.s[0].e, in[2], sh.e);
             put<R>(A, F.H, in[3], sh.e); put<R>(A, F.fq, in[4], sh.fq);
             put<R>(A, F.L.u, in[5], sh.u); put<R>(A, F.L.v, in[6], sh.v); put<R>(A, F.L.e, in[7], sh.e);
+            if (c.lin) { put<R>(A, F.fu, in[8], sh.fu); put<R>(A, F.fv, in[9], sh.fv); }
             StageAdjArgs<R> S{};
             S.s = F.s[0]; S.H = F.H; S.fq = F.fq; S.fq_2d = 1; S.qend = F.qend; S.lqend = F.lqend;
+            S.fu = F.fu; S.fv = F.fv; S.fu_2d = S.fv_2d = 1;
             S.l0 = F.L; S.mu_in = F.Ma; S.a_in = F.A; S.mu_out = F.Mb; S.a_out = F.A;
             S.a = R(1); S.b = R(0);
             S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = c.cont;
             S.nx = c.nx; S.ny = c.ny; S.P = A.P;
-            const bool wrap = c.per && !c.sad;
+            const bool wrap = c.per && !c.sad && !c.lin;
             if (wrap) K.qend(S);
-            K.stage(S, c.per, 0, 0, 0, c.sad);
+            K.stage(S, c.per, 0, 0, 0, c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH);
             if (wrap) K.qend_adj(S, S.mu_out);
             get<R>(A, F.Mb.u, out[0], sh.u); get<R>(A, F.Mb.v, out[1], sh.v); get<R>(A, F.Mb.e, out[2], sh.e);
         } else if (op == 2) {
@@ -409,6 +420,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
             if (op == 4) put<R>(A, F.Lsst, in[p++], sh.sst);
             put<R>(A, F.Lum, in[p++], sh.u); put<R>(A, F.Lvm, in[p++], sh.v);
             if (op == 3) put<R>(A, F.Lsst, in[p++], sh.sst);
+            if (op == 4 && c.lin) { put<R>(A, F.fu, in[p++], sh.fu); put<R>(A, F.fv, in[p++], sh.fv); }
             if (op == 3) {
                 reverse_tracer<R>(K, F, k, i);
                 get<R>(A, F.L.u, out[0], sh.u); get<R>(A, F.L.v, out[1], sh.v);

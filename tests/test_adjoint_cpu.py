@@ -157,10 +157,18 @@ def test_tangent_linear_values_are_the_oracle(swm, orc):
 
 # ------------------------------------------------------------------ operators
 
+def linear_statics(S):
+    """f_u, f_v appended to the emulated ops' inputs under the linear dynamics."""
+    return [S.grid.f_u, S.grid.f_v] if S.parameters.dynamics == "linear" else []
+
+
 @pytest.mark.parametrize("bc,cont", [("nonperiodic", {}), ("periodic", {}),
                                       ("nonperiodic", {"surface_relax": True, "t_relax": 2.0}),
                                       ("nonperiodic", {"adv_scheme": "Sadourny"}),
-                                      ("periodic", {"adv_scheme": "Sadourny"})])
+                                      ("periodic", {"adv_scheme": "Sadourny"}),
+                                      ("nonperiodic", {"dynamics": "linear"}),
+                                      ("periodic", {"dynamics": "linear"}),
+                                      ("nonperiodic", {"dynamics": "linear", "surface_relax": True, "t_relax": 2.0})])
 def test_stage_adjoint_is_the_transposed_jacobian(swm, orc, bc, cont):
     P, S, cfg, prog = setup(swm, bc, **cont)
     rng = np.random.default_rng(3)
@@ -172,7 +180,7 @@ def test_stage_adjoint_is_the_transposed_jacobian(swm, orc, bc, cont):
     lam = [np.where(m, rng.standard_normal(a.shape), 0.0) for a, m in zip(x, im[:3])]
     ref = jacobian_T(orc, S, cfg, 1, x, lam, (mu, mv, me))
     k = adjemu.config(S, cfg, L=5, nan_edges=True)
-    got = adjemu.run(S, k, 1, [x[0], x[1], x[2], S.forcing.H, S.grid.f_q] + lam)
+    got = adjemu.run(S, k, 1, [x[0], x[1], x[2], S.forcing.H, S.grid.f_q] + lam + linear_statics(S))
     for name, r, g, m in zip("uvη", ref, got, (mu, mv, me)):
         assert_close(f"{bc} μ{name}", r, g, m)
 
@@ -228,7 +236,7 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
     for i in range(n, 0, -1):
         x0, tp = tapes[i - 1]
         ins = [a for sk in tp["s"] for a in sk] + list(tp["pre"]) + list(tp["post"]) + list(tp["mid"]) + \
-              [x0[3], S.forcing.H, S.grid.f_q] + L
+              [x0[3], S.forcing.H, S.grid.f_q] + L + linear_statics(S)
         L = adjemu.run(S, k, 4, ins, i=i)
     return adjemu.run(S, k, 5, L[:4]), [u, v, e, s], st
 
@@ -236,7 +244,9 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
 @pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"RKo": 3}),
                                    ("periodic", {"nstep_diff": 2, "diffusion": "constant", "bottom_drag": "linear"}),
                                    ("periodic", {"adv_scheme": "Sadourny"}),
-                                   ("nonperiodic", {"adv_scheme": "Sadourny", "RKo": 3})])
+                                   ("nonperiodic", {"adv_scheme": "Sadourny", "RKo": 3}),
+                                   ("nonperiodic", {"dynamics": "linear"}),
+                                   ("periodic", {"dynamics": "linear", "RKo": 3, "bottom_drag": "linear"})])
 def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
     """Three steps: the tracer's snapshot (step 2) and advection (step 3) included."""
     P, S, cfg, prog = setup(swm, bc, **kw)
@@ -253,10 +263,10 @@ def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
         assert np.all(gg[~m] == 0), f"{name}: halo of the gradient not cleared"
 
 
-@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
-def test_dot_product_identity_across_wave_strips(swm, orc, bc):
+@pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"dynamics": "linear"})])
+def test_dot_product_identity_across_wave_strips(swm, orc, bc, kw):
     """A grid of several wave strips and short segments: <J dx, y> = <dx, J^T y>."""
-    P, S, cfg, prog = setup(swm, bc, nx=130, L_ratio=0.5)
+    P, S, cfg, prog = setup(swm, bc, nx=130, L_ratio=0.5, **kw)
     n = 4
     rng = np.random.default_rng(9)
     im = interior_masks(S)

@@ -28,6 +28,11 @@ def setup(swm, bc, T="Float64", nx=70, **kw):
     return P, S, swm.make_config(S), swm.seeded_initial_conditions(S, seed=5, σ=0.8)
 
 
+def form(adv):
+    """The right-hand side of a test case: a PV scheme of the nonlinear dynamics, or "linear"."""
+    return dict(dynamics="linear") if adv == "linear" else dict(adv_scheme=adv)
+
+
 def cotangent(S, seed, dtype):
     rng = np.random.default_rng(seed)
     return [np.asfortranarray(rng.standard_normal(m.shape).astype(dtype)) for m in interior_masks(S)]
@@ -47,10 +52,10 @@ def gpu_gradient(swm, S, prog, ybar, n, snapshots):
 
 @pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
 @pytest.mark.parametrize("T", ["Float64", "Float32"])
-@pytest.mark.parametrize("adv", ["ArakawaHsu", "Sadourny"])
+@pytest.mark.parametrize("adv", ["ArakawaHsu", "Sadourny", "linear"])
 def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T, adv):
     monkeypatch.setenv("SWM_ROWS_PER_WAVE_ADJ", "8")        # many segments on a small grid
-    P, S, cfg, prog = setup(swm, bc, T=T, adv_scheme=adv)
+    P, S, cfg, prog = setup(swm, bc, T=T, **form(adv))
     assert (S.grid.nadvstep, S.grid.nadvstep_half) == (3, 1)
     n = 3                                                     # snapshot at step 2, advection at step 3
     dt = np.float64 if T == "Float64" else np.float32
@@ -88,9 +93,10 @@ def test_taped_fused_pairs_equal_taped_stages(swm, gpu, monkeypatch, case):
 
 
 @pytest.mark.parametrize("bc,adv", [("nonperiodic", "ArakawaHsu"), ("periodic", "ArakawaHsu"),
-                                    ("nonperiodic", "Sadourny"), ("periodic", "Sadourny")])
+                                    ("nonperiodic", "Sadourny"), ("periodic", "Sadourny"),
+                                    ("nonperiodic", "linear"), ("periodic", "linear")])
 def test_dot_product_identity_fp64(swm, orc, gpu, bc, adv):
-    P, S, cfg, prog = setup(swm, bc, nx=90, adv_scheme=adv)
+    P, S, cfg, prog = setup(swm, bc, nx=90, **form(adv))
     n = 7
     ybar = cotangent(S, 2, np.float64)
     print(f"dot product {bc} nx={S.grid.nx} ny={S.grid.ny} n={n}", flush=True)
@@ -159,7 +165,7 @@ def test_rrule_of_time_integration(swm, gpu):
 
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
-    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", dynamics="linear"),
+    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", time_scheme="SSPRK2"),
                dict(T="Float32", nstep_advcor=2), dict(T="Float32", adv_scheme="Sadourny", nstep_advcor=2)):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
