@@ -447,6 +447,8 @@ def main():
                          "by s = 1 .. 64, then exit")
     ap.add_argument("--ensemble-steps", type=int, default=2000,
                     help="steps timed for the 64-member batched ensemble (0 skips it)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only time the 64-member ensemble (--ensemble-steps on both paths), print it and exit")
     ap.add_argument("--ensemble-seq-steps", type=int, default=20,
                     help="steps of each ensemble member run on its own, for the unbatched comparison")
     ap.add_argument("--adjoint-nx", type=str, default="1024,4096",
@@ -480,6 +482,9 @@ def main():
         return
     if args.channel:
         channel_run(swm, args.dtype, args.steps, args.warmup, args.kernel_reps, args.topography)
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": run_ensemble(swm, 64, args.ensemble_steps, args.ensemble_seq_steps)}), flush=True)
         return
     es = 4 if args.dtype == "Float32" else 8
     # kernel times with HIP events on the library stream
