@@ -274,7 +274,10 @@ int swm_ens_persistent_steps(const swm_ens *ens, int64_t *steps, const char **wh
 /* ---- Reverse mode (no counterpart in the reference, which has no derivatives; it is what a
  * ChainRulesCore.rrule of time_integration, src/time_integration.jl:2-250, would call).
  * Classic RK scheme with T = Tprog = Tcomm in {Float64, Float32}; the nonlinear dynamics
- * (Arakawa-Hsu or Sadourny PV fluxes) or the linear dynamics (rhs.jl:55-86), nstep_advcor = 0.
+ * (Arakawa-Hsu or Sadourny PV fluxes, any nstep_advcor: the kept q and KEu+KEv are
+ * differentiated through to the states they are formed from) or the linear dynamics
+ * (rhs.jl:55-86).  Kept fields formed before swm_adj_forward's first step (not by it) are
+ * constants.
  * Derivatives are those of
  * the exact-arithmetic linearisation of each operation; at the kinks: sqrt'(0) = 0
  * (Smagorinsky ν, drag √KE), floor / clip / wrap of the tracer's departure points held

@@ -54,8 +54,9 @@ constexpr int DT_HALO = 3;  // diss_adjoint
 constexpr int ADJ_Y0 = -1;
 
 // the right-hand side forms of the stage adjoint (template argument SCH): the nonlinear
-// dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics
-enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2 };
+// dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics; ADJ_KEEP (with a
+// nonlinear form): q and KEu+KEv kept from an earlier state (nstep_advcor > 0)
+enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2, ADJ_KEEP = 4 };
 SWM_HD int adj_y1(int ny) { return ny + 3; }
 
 template <class T> SWM_HD T amin(T a, T b) { return a < b ? a : b; }
@@ -95,6 +96,8 @@ template <class R> struct StageAdjArgs {
     int fq_2d;
     uint32_t fu, fv;       // linear dynamics: f on the u- and v-grid (one value per row when fu_2d / fv_2d = 0)
     int fu_2d, fv_2d;
+    uint32_t qk;           // kept q the stage read (nstep_advcor > 0), corners
+    uint32_t lqk, lkk;     // λ of the kept q (corners) and KEu+KEv (T points), accumulated
     uint32_t qend;         // periodic: q at column nx for every corner row of s_k (1-D)
     UVE l0;                // λ of u0 after the stages (halo unfolded)
     UVE mu_in;             // μ_{k+1} = (∂R/∂s_{k+1})ᵀ λd_{k+1} (halo unfolded)
@@ -186,12 +189,15 @@ template <class V> struct StageLoads {
     V au, av, ae;              // running sum A at the row emitted, r-2
     V fx;                      // λFx at the row emitted, r-2 (fxg)
     V fu, fv;                  // linear dynamics: f_u at row r-1, f_v at row r-2
+    V qk, qke;                 // kept q at corner row r-1, and at its column nx (periodic wrap)
+    V lq, lk;                  // λ of the kept q and KEu+KEv at the row emitted, r-2
 };
 
-template <class W, bool PER, bool NEXT, bool A_L0, bool LIN = false>
+template <class W, bool PER, bool NEXT, bool A_L0, int SCH = ADJ_AH>
 SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<typename W::R> &A, int r, int ya, int yb) {
     using V = typename W::V;
     using R = typename W::R;
+    constexpr bool LIN = (SCH & 3) == ADJ_LIN, KEEP = (SCH & ADJ_KEEP) != 0;
     const V zero = V(R(0));
     StageLoads<V> p;
     const int rs = amin(r, yb + 1), ny = A.ny;
@@ -201,9 +207,17 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
         p.fv = w.ld_s(A.fv, amax(rs - 2, -1), A.fv_2d);
     } else {
         p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs); p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs);
-        p.fq = w.ld_s(A.fq, rs - 1, A.fq_2d);
-        p.qe = PER ? w.ld_row1(A.qend, rs - 1) : zero;
+        p.fq = KEEP ? zero : w.ld_s(A.fq, rs - 1, A.fq_2d);
+        p.qe = PER && !KEEP ? w.ld_row1(A.qend, rs - 1) : zero;
         p.fu = p.fv = zero;
+    }
+    p.qk = p.qke = p.lq = p.lk = zero;
+    if (KEEP) {
+        const int yq = amin(amax(rs - 1, 0), ny), ye = amin(amax(rs - 2, ya), yb - 1);
+        p.qk = w.ld(A.qk, yq);
+        if (PER) p.qke = w.ld_at(A.qk, A.nx, yq, true);
+        p.lq = w.ld(A.lqk, ye);
+        p.lk = w.ld(A.lkk, ye);
     }
     const int yu = amin(amax(rs - 1, 1), ny), yv = amin(amax(rs - 1, 1), amax(ny - 1, 1));
     p.lu = w.ld(A.l0.u, yu); p.lv = w.ld(A.l0.v, yv); p.le = w.ld(A.l0.e, yu);
@@ -265,7 +279,8 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     using R = typename W::R;
     using V = typename W::V;
     using M = typename W::M;
-    constexpr bool SAD = SCH == ADJ_SAD, LIN = SCH == ADJ_LIN;
+    constexpr bool SAD = (SCH & 3) == ADJ_SAD, LIN = (SCH & 3) == ADJ_LIN, KEEP = (SCH & ADJ_KEEP) != 0;
+    static_assert(!(LIN && KEEP), "the linear dynamics keep nothing");
     const int nx = A.nx, ny = A.ny;
     const auto x = w.x;
     const M ucol = PER ? M((x >= 0) & (x <= nx - 1)) : M((x >= 1) & (x <= nx - 1));
@@ -290,14 +305,14 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     V Qu_2 = zero, Vu_2 = zero, Uv_2 = zero;           // Sadourny: Qu, Vu of u row r-2, Uv of v row r-2
     V pu_2 = zero, pv_3 = zero;                        // linear: f_u·λdu of row r-2, f_v·λdv of row r-3
 
-    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0, LIN>(w, A, ya - 2, ya, yb);
+    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0, SCH>(w, A, ya - 2, ya, yb);
     for (int r = ya - 2; r <= yb + 1; ++r) {
         const StageLoads<V> c = nl;
-        nl = stage_loads<W, PER, NEXT, A_L0, LIN>(w, A, r + 1, ya, yb);
+        nl = stage_loads<W, PER, NEXT, A_L0, SCH>(w, A, r + 1, ya, yb);
         // ---- stage input row r
         const V u_0 = c.u, v_0 = c.v, e_0 = c.e, H_0 = c.H;
         const V fq_1 = c.fq;
-        const V qend_1 = c.qe;
+        const V qend_1 = KEEP ? c.qke : c.qe;
         // ---- tendency adjoint row r-1
         const int yd = r - 1;
         const bool ydu = yd >= 1 && yd <= ny, ydv = yd >= 1 && yd <= ny - 1;
@@ -331,7 +346,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const V dudy_1 = u_0 - u_1;
         const bool yq1 = r - 1 >= 0 && r - 1 <= ny;
         const M qv_1 = qcol & yq1;
-        const V q_1 = W::sel(qv_1, (fq_1 + dvdx_1 - dudy_1) / W::sel(qv_1, hq_1, V(R(1))), zero);
+        const V q_1 = KEEP ? W::sel(qv_1, c.qk, zero) : W::sel(qv_1, (fq_1 + dvdx_1 - dudy_1) / W::sel(qv_1, hq_1, V(R(1))), zero);
         const V qxp_1 = W::xp(q_1);
         const V qxm_1 = PER && !SAD ? W::sel(qfix, qend_1, W::xm(q_1)) : W::xm(q_1);
         V qa_1 = zero, qb_1 = zero, qg_1 = zero, qd_1 = zero, la_1 = zero, lb_1 = zero, lg_1 = zero, ld_1 = zero;
@@ -372,17 +387,24 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             lq_2 = k12 * ((gu_1 * Vu_1 + gu_2 * Vu_2) + (lQv_2 + W::xp(lQv_2)));
         }
 
+        const V lq_2_keep = lq_2;
         // ---- adjoint of row r-2: thickness, inputs
         const bool yq2 = r - 2 >= 0 && r - 2 <= ny;
         const M qv_2 = qcol & yq2;
-        const V lnum_2 = W::sel(qv_2, lq_2 / W::sel(qv_2, hq_2, V(R(1))), zero);
+        // (kept q: λq goes to the kept field, not through PV! to the state)
+        const V lnum_2 = KEEP ? zero : W::sel(qv_2, lq_2 / W::sel(qv_2, hq_2, V(R(1))), zero);
         const V lhq_2 = -(lnum_2 * q_2);
         const V lhu_2 = lU_2 * u_2;
         const V lhv_2 = lV_2 * v_2;
         const V lh_2 = half * (lhu_2 + W::xm(lhu_2)) + half * (lhv_2 + lhv_3) +
                        quarter * ((lhq_2 + W::xm(lhq_2)) + (lhq_3 + W::xm(lhq_3)));
-        lu_2 = (lU_2 * hu_2 + u_2 * (half * (lp_2 + W::xp(lp_2)))) + (lnum_2 - lnum_3);
-        lv_2 = (lV_2 * hv_2 + v_2 * (half * (lp_2 + lp_1))) + (W::xm(lnum_2) - lnum_2);
+        if constexpr (KEEP) {       // KEu+KEv kept: the Bernoulli potential reads no velocity
+            lu_2 = lU_2 * hu_2;
+            lv_2 = lV_2 * hv_2;
+        } else {
+            lu_2 = (lU_2 * hu_2 + u_2 * (half * (lp_2 + W::xp(lp_2)))) + (lnum_2 - lnum_3);
+            lv_2 = (lV_2 * hv_2 + v_2 * (half * (lp_2 + lp_1))) + (W::xm(lnum_2) - lnum_2);
+        }
         le_2 = lh_2 + g * lp_2;
         if (A.cont == 1) le_2 = le_2 - A.gamma * ge_2;          // surface relaxation: dη += γ(η_ref - η)
         if constexpr (LIN) {
@@ -432,6 +454,10 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             if (PER && !SAD && !LIN && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
             // the wind stress enters du as Tprog(Fxt·Fx) (rhs.jl:242): λFx += Fxt·λdu, u cells
             if (A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
+            if constexpr (KEEP) {   // λ of the kept q (corner row y) and KEu+KEv (p = ½(KEu+KEv) + gη)
+                if (y >= 0 && y <= ny) w.st(A.lqk, y, c.lq + W::sel(qcol, lq_2_keep, zero), w.own & qcol);
+                if (y >= 0 && y <= ny + 1) w.st(A.lkk, y, c.lk + half * lp_2, w.own & (x >= 0) & (x <= nx + 1));
+            }
         }
 
         // ---- advance the rows
@@ -476,6 +502,67 @@ SWM_HD void qend_adjoint_row(const C &c, const StageAdjArgs<R> &A, const UVE &mu
     const R le = quarter * (lh[0] + lh[1]);
     c.at(mu.e, nx, yy) = c.at(mu.e, nx, yy) + le;
     c.at(mu.e, nx + 1, yy) = c.at(mu.e, nx + 1, yy) + le;
+}
+
+// Periodic with kept q: the λ of q(nx, y) the wrap gathered (lqend) belongs to the kept
+// field's corner (nx, y).  One thread per corner row.
+template <class R, class C> SWM_HD void lqend_keep_row(const C &c, const StageAdjArgs<R> &A, int y) {
+    c.at(A.lqk, A.nx, y) = c.at(A.lqk, A.nx, y) + c.row1(A.lqend, y);
+}
+
+// ------------------------------------------------------------------ kept q and KEu+KEv
+//
+// nstep_advcor > 0 (time_integration.jl:20-31, 208-211): q and KEu+KEv are formed from a
+// state (u, v, η) -- the initial state, the state after the stages of every nstep_advcor-th
+// step -- and read by the stages of the following steps; the quadratic drag forms KEu+KEv
+// again from the state after the stages of its steps (bottom_drag.jl:25-38).  Forward, at
+// the state (T points (x, y), corners (x, y)):
+//   KE = ½(u² + u(x-1)²) + ½(v² + v(y-1)²)
+//   q = (f + (v(x+1) - v) - (u(y+1) - u)) / hq,  hq = ¼(h + h(x+1)) + ¼(h(y+1) + h(x+1,y+1))
+// Transpose, with λk, λq the accumulated cotangents of the kept fields (zero outside their
+// cells), λn = λq/hq, λhq = -λn·q:
+//   λu += u·(λk + λk(x+1)) + (λn - λn(y-1));  λv += v·(λk + λk(y+1)) + (λn(x-1) - λn)
+//   λη += ¼·(λhq + λhq(x-1) + λhq(y-1) + λhq(x-1,y-1))
+// added to the (halo unfolded) adjoint of the state.  wq: q was formed too (else KE alone).
+struct AdvcorAdjArgs {
+    UVE s;                 // the state the kept fields were formed from
+    uint32_t H, fq;
+    int fq_2d;
+    uint32_t lqk, lkk;     // λ of the kept q and KEu+KEv
+    UVE out;               // the adjoint of the state, added to
+    int wq, nx, ny;
+};
+
+template <class R, class C> SWM_HD void advcor_adjoint_cell(const C &c, const AdvcorAdjArgs &A, int x, int y) {
+    const int nx = A.nx, ny = A.ny;
+    const R quarter = R(0.25), zero = R(0);
+    auto lk = [&](int xx, int yy) { return (xx >= 0 && xx <= nx + 1 && yy >= 0 && yy <= ny + 1) ? c.at(A.lkk, xx, yy) : zero; };
+    auto cq = [&](int xx, int yy) { return xx >= 0 && xx <= nx && yy >= 0 && yy <= ny; };
+    auto hq = [&](int xx, int yy) {
+        const R h00 = c.at(A.s.e, xx, yy) + c.at(A.H, xx, yy), h10 = c.at(A.s.e, xx + 1, yy) + c.at(A.H, xx + 1, yy);
+        const R h01 = c.at(A.s.e, xx, yy + 1) + c.at(A.H, xx, yy + 1);
+        const R h11 = c.at(A.s.e, xx + 1, yy + 1) + c.at(A.H, xx + 1, yy + 1);
+        return quarter * (h10 + h00) + quarter * (h11 + h01);
+    };
+    auto ln = [&](int xx, int yy) { return cq(xx, yy) ? c.at(A.lqk, xx, yy) / hq(xx, yy) : zero; };
+    auto lh = [&](int xx, int yy) {
+        if (!cq(xx, yy)) return zero;
+        const R d = hq(xx, yy);
+        const R f = A.fq_2d ? c.at(A.fq, xx, yy) : c.row1(A.fq, yy);
+        const R q = (f + (c.at(A.s.v, xx + 1, yy) - c.at(A.s.v, xx, yy)) - (c.at(A.s.u, xx, yy + 1) - c.at(A.s.u, xx, yy))) / d;
+        return -((c.at(A.lqk, xx, yy) / d) * q);
+    };
+    R du = c.at(A.s.u, x, y) * (lk(x, y) + lk(x + 1, y));
+    R dv = c.at(A.s.v, x, y) * (lk(x, y) + lk(x, y + 1));
+    R de = zero;
+    if (A.wq) {
+        du = du + (ln(x, y) - ln(x, y - 1));
+        dv = dv + (ln(x - 1, y) - ln(x, y));
+        de = quarter * ((lh(x, y) + lh(x - 1, y)) + (lh(x, y - 1) + lh(x - 1, y - 1)));
+    }
+    c.at(A.out.u, x, y) = c.at(A.out.u, x, y) + du;
+    c.at(A.out.v, x, y) = c.at(A.out.v, x, y) + dv;
+    c.at(A.out.e, x, y) = c.at(A.out.e, x, y) + de;
 }
 
 // q at column nx of every corner row y (PV!, the value the periodic qβ/qγ wrap reads)
@@ -967,6 +1054,7 @@ struct AdjFields {
     int fq_2d;
     uint32_t fu, fv;                // linear dynamics: f on the u- and v-grid
     int fu_2d, fv_2d;
+    uint32_t qk, lqk, lkk;          // kept q the step's stages read (taped), λ of the kept q and KEu+KEv
     uint32_t qend, lqend;
     uint32_t tr0, tr1;
     uint32_t tg0, tg1;              // the tracer adjoint's gathered interpolation records
@@ -985,6 +1073,7 @@ template <class R> struct AdjConsts {
     int fxg;                        // accumulate the cotangent of Fx
     int sad;                        // Sadourny's PV fluxes (k12 = ½)
     int lin;                        // the linear dynamics (rhs_linear!)
+    int nadvcor;                    // nstep_advcor: q and KEu+KEv kept (0: formed by every stage)
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -1018,7 +1107,8 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
     if (rc) return rc;
     // dissipation (time_integration.jl:216-221)
     UVE l0 = F.L, out = F.Z;
-    if (i % c.nstep_diff == 0) {
+    const bool diss = i % c.nstep_diff == 0;
+    if (diss) {
         DissAdjArgs<R> D{};
         D.s = F.pre; D.H = F.H; D.lin = F.L; D.out = F.Z;
         D.cD = c.cD; D.rD = c.rD; D.cS = c.cS; D.nuB = c.nuB; D.dtd = c.dtd; D.oma = c.oma;
@@ -1027,11 +1117,26 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
         l0 = F.Z;
         out = F.L;
     }
+    // the kept q and KEu+KEv formed from the state after the stages (time_integration.jl:208-211;
+    // KEu+KEv again by the quadratic drag): their cotangents, gathered from the steps that read
+    // them, pass to that state, and the stages of this step read the values formed before
+    if (c.nadvcor > 0) {
+        const bool wq = i % c.nadvcor == 0, wk = wq || (diss && c.drag == 0);
+        if (wk) {
+            AdvcorAdjArgs X{};
+            X.s = F.pre; X.H = F.H; X.fq = F.fq; X.fq_2d = F.fq_2d; X.lqk = F.lqk; X.lkk = F.lkk; X.out = l0;
+            X.wq = wq; X.nx = c.nx; X.ny = c.ny;
+            if ((rc = k.advcor_adj(X))) return rc;
+            if ((rc = k.zero(F.lkk))) return rc;
+            if (wq && (rc = k.zero(F.lqk))) return rc;
+        }
+    }
     // the RK stages, last first (time_integration.jl:55-78)
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;
     S.fu = F.fu; S.fv = F.fv; S.fu_2d = F.fu_2d; S.fv_2d = F.fv_2d;
-    const int sch = c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH;
+    S.qk = F.qk; S.lqk = F.lqk; S.lkk = F.lkk;
+    const int sch = (c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH) | (c.nadvcor > 0 && !c.lin ? ADJ_KEEP : 0);
     S.l0 = l0;
     S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
@@ -1046,10 +1151,12 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
         S.a_in = F.A;
         S.mu_out = next ? mu_next : mu_prev;
         S.a_out = pre ? out : F.A;
-        const bool wrap = c.per && sch == ADJ_AH;       // the Arakawa-Hsu first-column wrap to q[end,·]
-        if (wrap && (rc = k.qend(S))) return rc;
+        const bool wrap = c.per && (sch & 3) == ADJ_AH; // the Arakawa-Hsu first-column wrap to q[end,·]
+        const bool keep = (sch & ADJ_KEEP) != 0;        // (kept q: q[end,·] is the kept field's)
+        if (wrap && !keep && (rc = k.qend(S))) return rc;
         if ((rc = k.stage(S, c.per, next, al0, pre, sch))) return rc;
-        if (wrap && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
+        if (wrap && !keep && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
+        if (wrap && keep && (rc = k.lqend_keep(S))) return rc;
         if (next) { const UVE t = mu_prev; mu_prev = mu_next; mu_next = t; }
     }
     F.L = out;

@@ -83,6 +83,20 @@ template <class R> __global__ void qend_fwd_kernel(ArenaRef ar, adj::StageAdjArg
     adj::qend_row<R>(GpuCells<R>{(char *)ar.base, A.P}, A, y);
 }
 
+// the kept q and KEu+KEv (nstep_advcor > 0): their cotangents into the state's, one thread
+// per cell x in -2 .. nx+2, y in -1 .. ny+2
+template <class R> __global__ void advcor_adj_kernel(ArenaRef ar, adj::AdvcorAdjArgs A, int P) {
+    const int x = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x), y = -1 + (int)blockIdx.y;
+    if (x > A.nx + 2) return;
+    adj::advcor_adjoint_cell<R>(GpuCells<R>{(char *)ar.base, P}, A, x, y);
+}
+
+template <class R> __global__ void lqend_keep_kernel(ArenaRef ar, adj::StageAdjArgs<R> A) {
+    const int y = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (y > A.ny) return;
+    adj::lqend_keep_row<R>(GpuCells<R>{(char *)ar.base, A.P}, A, y);
+}
+
 template <class R> __global__ void qend_adj_kernel(ArenaRef ar, adj::StageAdjArgs<R> A, adj::UVE mu) {
     const int yy = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (yy > A.ny + 1) return;

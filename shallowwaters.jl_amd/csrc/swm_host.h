@@ -321,6 +321,8 @@ struct AdjState {
     int La = 64;                   // rows per wave segment of the adjoint marches
     std::vector<char *> snap;      // snapshots: [0] the initial state, [1..s] the schedule's, [s+1] the final state
     std::vector<int64_t> at;       // step held by each schedule snapshot (-1: free)
+    std::vector<char> snap_pending; // each snapshot's kept q and KEu+KEv still to be formed (nstep_advcor > 0)
+    bool keep_init = false;        // the forward run formed them from its initial state
     size_t snap_bytes = 0;
     int *dmax = nullptr;           // the tracer adjoint's gather window (device, 2 ints)
     int64_t i0 = 0, t0 = 0, n = 0;

@@ -733,9 +733,10 @@ template <class Fm> struct Adj {
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the unscaled, uncompensated step");
-        if (adv_form(c) != ADV_AH && adv_form(c) != ADV_SAD && adv_form(c) != ADV_LIN)
+        const int af = adv_form(c);
+        if (af != ADV_AH && af != ADV_SAD && af != ADV_LIN && af != (ADV_AH | ADV_KEEP) && af != (ADV_SAD | ADV_KEEP))
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint is of the nonlinear right-hand side with the Arakawa-Hsu or "
-                                                "Sadourny scheme or of the linear dynamics (nstep_advcor = 0)");
+                                                "Sadourny scheme or of the linear dynamics");
         if (c.dynamics == SWM_LINEAR && !ctx->have_fuv)
             return ctx->fail(SWM_E_STATE, "the linear dynamics need f_u, f_v (swm_set_static_linear)");
         if (!ctx->have_static || !ctx->have_prog) return ctx->fail(SWM_E_STATE, "set_static and upload_prog first");
@@ -748,7 +749,7 @@ template <class Fm> struct Adj {
         if (ctx->adj) return SWM_OK;
         int rc = check(ctx);
         if (rc) return rc;
-        const int nf2 = 6 + 24;
+        const int nf2 = 6 + 27;
         const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
         if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
         AdjState *st = new AdjState();
@@ -772,6 +773,7 @@ template <class Fm> struct Adj {
         one(F.tg0, true); one(F.tg1, true);
         one(F.lfx, true);
         one(F.lqend, false);
+        one(F.qk, true); one(F.lqk, true); one(F.lkk, true);
         if (rc) {
             delete st;
             return rc;
@@ -807,6 +809,7 @@ template <class Fm> struct Adj {
         }
         k.sad = c.adv_scheme == SWM_ADV_SADOURNY;
         k.lin = c.dynamics == SWM_LINEAR;
+        k.nadvcor = (adv_form(c) & ADV_KEEP) ? (int)c.nstep_advcor : 0;
         k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(k.sad ? 0.5 : 1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
         k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
         k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
@@ -839,7 +842,23 @@ template <class Fm> struct Adj {
         template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
             if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
             else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, al0, pre);
+            else if (sch == (adj::ADJ_AH | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_AH | adj::ADJ_KEEP>(S, next, al0, pre);
+            else if (sch == (adj::ADJ_SAD | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_SAD | adj::ADJ_KEEP>(S, next, al0, pre);
             else stage_p<PER, adj::ADJ_AH>(S, next, al0, pre);
+        }
+        int advcor_adj(const adj::AdvcorAdjArgs &X) {
+            hipLaunchKernelGGL((advcor_adj_kernel<R>), dim3((X.nx + 5 + 127) / 128, X.ny + 4), dim3(128), 0, ctx->stream,
+                               ctx->aref(), X, ctx->P);
+            return fail_launch();
+        }
+        int lqend_keep(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((lqend_keep_kernel<R>), dim3((S.ny + 1 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S);
+            return fail_launch();
+        }
+        int zero(uint32_t o) {             // a 2-D field of the arena
+            hipLaunchKernelGGL(zero_words_kernel, dim3(1024), dim3(256), 0, ctx->stream,
+                               (uint32_t *)((char *)ctx->arena + o), ctx->elems * ctx->es / 4);
+            return fail_launch();
         }
         int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre, int sch) {
             S.L = st->La;
@@ -900,10 +919,13 @@ template <class Fm> struct Adj {
     };
 
     // ---- snapshots of full states (u, v, η, sst, um, vm)
+    // with nstep_advcor > 0 also the kept q and KEu+KEv and whether they are still to be formed
+    static int snap_fields(const swm_ctx *ctx) { return (adv_form(ctx->c) & ADV_KEEP) ? 8 : 6; }
     static int snap_alloc(swm_ctx *ctx, int count) {
         AdjState *st = ctx->adj;
         st->release();
-        st->snap_bytes = 6 * ctx->elems * ctx->es;
+        st->snap_bytes = (size_t)snap_fields(ctx) * ctx->elems * ctx->es;
+        st->snap_pending.assign(count, 0);
         for (int k = 0; k < count; ++k) {
             void *p = nullptr;
             if (hipMalloc(&p, st->snap_bytes) != hipSuccess) {
@@ -919,11 +941,15 @@ template <class Fm> struct Adj {
     static int copy_state(swm_ctx *ctx, int k, bool save) {
         AdjState *st = ctx->adj;
         const size_t fb = ctx->elems * ctx->es;
-        const Field *f[6] = {&ctx->u[ctx->pu], &ctx->v[ctx->pu], &ctx->e[ctx->pe], &ctx->sst[ctx->rsst], &ctx->um, &ctx->vm};
-        for (int j = 0; j < 6; ++j) {
+        const Field *f[8] = {&ctx->u[ctx->pu], &ctx->v[ctx->pu], &ctx->e[ctx->pe], &ctx->sst[ctx->rsst], &ctx->um, &ctx->vm,
+                             &ctx->qs, &ctx->ks};
+        const int nf = snap_fields(ctx);
+        for (int j = 0; j < nf; ++j) {
             char *s = st->snap[k] + j * fb;
             HIPCHK(hipMemcpyAsync(save ? s : f[j]->base, save ? f[j]->base : s, fb, hipMemcpyDeviceToDevice, ctx->stream));
         }
+        if (save) st->snap_pending[k] = ctx->advcor_pending ? 1 : 0;
+        else ctx->advcor_pending = st->snap_pending[k] != 0;
         return SWM_OK;
     }
 
@@ -964,6 +990,17 @@ template <class Fm> struct Adj {
             return kk;
         };
         auto mode = [&](int q) { return q == 1 ? M_RK_FIRST : (q == c.RKo ? M_RK_LAST : M_RK_MID); };
+        const bool keep = (adv_form(c) & ADV_KEEP) != 0;
+        if (keep) {
+            // q and KEu+KEv from the step's input when still to be formed (:20-31); the q the
+            // stages read is taped
+            if (ctx->advcor_pending) {
+                if ((rc = LF::advcor(ctx, P, true, true))) return rc;
+                ctx->advcor_pending = false;
+            }
+            HIPCHK(hipMemcpyAsync((char *)ctx->arena + F.qk, ctx->qs.base, ctx->elems * ctx->qs.es, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        }
         for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
             // stages q, q+1 as one fused pair that also keeps the input of stage q+1
             if (q + 1 <= c.RKo && LF::can_tape(ctx, mode(q), mode(q + 1))) {
@@ -975,6 +1012,11 @@ template <class Fm> struct Adj {
             rc = LF::stage(ctx, mode(q), sk(q), P, a, q < c.RKo ? sk(q + 1) : none, a, coef(q), T);
         }
         if (rc) return rc;
+        if (keep) {                                          // :208-211, and the quadratic drag's KEu, KEv
+            const bool due = i % c.nstep_advcor == 0;
+            if (due || (c.bottom_drag == SWM_DRAG_QUADRATIC && i % c.nstep_diff == 0))
+                if ((rc = LF::advcor(ctx, a, due, false))) return rc;
+        }
         int post = a.uv;
         if (i % c.nstep_diff == 0) {
             post = uv[5];
@@ -1042,6 +1084,7 @@ template <class Fm> struct Adj {
         if ((rc = snap_alloc(ctx, s + 2))) return rc;
         st->i0 = i0; st->t0 = t0; st->n = n; st->nsnap = s; st->recorded = false; st->fwd_steps = 0;
         if ((rc = copy_state(ctx, 0, true))) return rc;
+        st->keep_init = ctx->advcor_pending;
         // the snapshots the schedule takes first: the chain of splits toward the last step
         std::vector<int64_t> chain;
         for (int64_t a = 0, sf = s; n - a > 1 && sf > 0; --sf) {
@@ -1111,6 +1154,17 @@ template <class Fm> struct Adj {
         st->sweep_ms = tm;
         HIPCHK(hipEventDestroy(e0));
         HIPCHK(hipEventDestroy(e1));
+        if (st->keep_init) {
+            // q and KEu+KEv were formed from the initial state: their cotangents, gathered from
+            // the steps before any later forming, pass to it (time_integration.jl:20-31)
+            if ((rc = copy_state(ctx, 0, false))) return rc;
+            adj::AdvcorAdjArgs X{};
+            X.s = adj::UVE{ctx->u[ctx->pu].ref(), ctx->v[ctx->pu].ref(), ctx->e[ctx->pe].ref()};
+            X.H = st->F.H; X.fq = st->F.fq; X.fq_2d = st->F.fq_2d; X.lqk = st->F.lqk; X.lkk = st->F.lkk;
+            X.out = st->F.L; X.wq = 1; X.nx = c.nx; X.ny = c.ny;
+            K kl{ctx, st};
+            if ((rc = kl.advcor_adj(X))) return rc;
+        }
         // the gradient: ghost copies of the uploaded state folded in (swm_upload_prog applies them)
         adj::FoldOutArgs O{st->F.L, st->F.Z, st->F.Lsst, st->F.Lsst2, c.nx, c.ny, c.bc == SWM_PERIODIC};
         hipLaunchKernelGGL((fold_out_kernel<R>), dim3((c.nx + 5 + 127) / 128, c.ny + 4), dim3(128), 0, ctx->stream,

@@ -219,6 +219,8 @@ template <class R> struct EmuK {
     template <bool PER> void stage_s(const StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
         if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, al0, pre);
         else if (sch == ADJ_SAD) stage_p<PER, ADJ_SAD>(S, next, al0, pre);
+        else if (sch == (ADJ_AH | ADJ_KEEP)) stage_p<PER, ADJ_AH | ADJ_KEEP>(S, next, al0, pre);
+        else if (sch == (ADJ_SAD | ADJ_KEEP)) stage_p<PER, ADJ_SAD | ADJ_KEEP>(S, next, al0, pre);
         else stage_p<PER, ADJ_AH>(S, next, al0, pre);
     }
     int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre, int sch) {
@@ -229,6 +231,22 @@ template <class R> struct EmuK {
     int qend(const StageAdjArgs<R> &S) {
         Cells<R> c{a};
         for (int y = 0; y <= S.ny; ++y) qend_row<R>(c, S, y);
+        return 0;
+    }
+    int advcor_adj(const AdvcorAdjArgs &X) {
+        Cells<R> c{a};
+        for (int y = -1; y <= X.ny + 2; ++y)
+            for (int x = -2; x <= X.nx + 2; ++x) advcor_adjoint_cell<R>(c, X, x, y);
+        return 0;
+    }
+    int lqend_keep(const StageAdjArgs<R> &S) {
+        Cells<R> c{a};
+        for (int y = 0; y <= S.ny; ++y) lqend_keep_row<R>(c, S, y);
+        return 0;
+    }
+    int zero(uint32_t o) {
+        R *b = (R *)(a->mem.data() + o);
+        for (size_t k = 0; k < (size_t)a->P * a->rows; ++k) b[k] = R(0);
         return 0;
     }
     int qend_adj(const StageAdjArgs<R> &S, const UVE &mu) {

@@ -166,7 +166,7 @@ def test_rrule_of_time_integration(swm, gpu):
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
     for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", time_scheme="SSPRK2"),
-               dict(T="Float32", nstep_advcor=2), dict(T="Float32", adv_scheme="Sadourny", nstep_advcor=2)):
+               dict(T="Float32", time_scheme="SSPRK3")):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
             sim.upload(prog)
@@ -255,3 +255,32 @@ def test_adjoint_workload_gradients_by_central_differences(swm, gpu):
     worst = max(abs(f - a) / max(abs(a), scale) for a, f in pairs)
     print(f"worst relative difference {worst:.2e} (median |derivative| {scale:.3e}, loss {L0:.6e})", flush=True)
     assert worst <= 1e-6
+
+
+@pytest.mark.parametrize("bc,kw", [("nonperiodic", dict(nstep_advcor=2)),
+                                   ("periodic", dict(nstep_advcor=3, bottom_drag="linear")),
+                                   ("nonperiodic", dict(nstep_advcor=2, adv_scheme="Sadourny", bottom_drag="none")),
+                                   ("periodic", dict(nstep_advcor=1, adv_scheme="Sadourny"))])
+def test_kept_fields_dot_product_fp64(swm, orc, gpu, bc, kw):
+    """q and KEu+KEv kept over nstep_advcor steps (time_integration.jl:20-31, 208-211; KEu+KEv
+    formed again by every quadratic-drag step): the FP64 dot-product identity with the
+    oracle's tangent-linear model, whose kept fields are formed from the initial state as the
+    uploaded context's are; revolve with 2 snapshots restores the kept fields it recomputes
+    from."""
+    P, S, cfg, prog = setup(swm, bc, nx=90, **kw)
+    n = 7
+    ybar = cotangent(S, 4, np.float64)
+    print(f"kept fields {bc} {kw} n={n}", flush=True)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=2)
+    g0, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=0)
+    for a, b in zip(g, g0):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), "the gradient depends on the snapshots"
+    x = list(ghosted_state(swm, S, prog))
+    rng = np.random.default_rng(5)
+    for m in range(3):
+        dx = [np.where(mk, rng.standard_normal(mk.shape), 0.0) for mk in interior_masks(S)]
+        dxg = list(ghosted_state(swm, S, type(prog)(*dx)))
+        _, dy = orc.tlm(S, cfg, 0, x, dxg, nsteps=n)
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(dy, ybar))
+        rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, g))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dx, ȳ> = {lhs!r}, <dx, J^T ȳ> = {rhs!r}"
