@@ -248,13 +248,15 @@ def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0, dFx=None):
 
 
 def tape(S, cfg, i, t, u, v, e, sst, um, vm):
-    """Intermediate states of one Float64 step (classic RK) from the given state: returns a
-    dict with 's' (list of the RK stage inputs (u1, v1, η1)), 'pre' (u0, v0, η0 after the
-    stages' ghost copies), 'post' (u0, v0 after the dissipation), 'mid' (um, vm after the
-    step) and 'new' (u, v, η, sst)."""
+    """Intermediate states of one Float64 step (classic RK or SSPRK2) from the given state:
+    returns a dict with 's' (list of the RK stage inputs (u1, v1, η1); under SSPRK2 the RKs
+    stage inputs and (u1, v1, η1) after the last stage), 'pre' (u0, v0, η0 after the stages'
+    ghost copies), 'post' (u0, v0 after the dissipation), 'mid' (um, vm after the step) and
+    'new' (u, v, η, sst)."""
     G = S.grid
     su, sv, se, ss = (G.nux + 4, G.nuy + 4), (G.nvx + 4, G.nvy + 4), (G.nx + 2, G.ny + 2), (G.nx + 2, G.ny)
-    RKo = S.parameters.RKo
+    P = S.parameters
+    RKo = P.RKs + 1 if P.time_scheme == "SSPRK2" else P.RKo
     shapes = [su, sv, se] * RKo + [su, sv, se] + [su, sv] + [su, sv] + [su, sv, se, ss]
     out = [np.zeros(s, order="F") for s in shapes]
     vp = ctypes.c_void_p

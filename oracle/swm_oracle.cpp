@@ -1035,6 +1035,7 @@ template <class T, class Tp, class Tc> struct Model {
             const Tp dts = Tp(c.Dt_Ds);
             for (int rki = 1; rki <= c.RKs; ++rki) {
                 if (rki > 1) ghost_points_eta(eta1);
+                if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
                 const Arr<T> &e = rhs_view(eta1, etar);
                 rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
                 axb(u1, dts, du);
@@ -1045,6 +1046,7 @@ template <class T, class Tp, class Tc> struct Model {
                 ++stages_run;
                 if (stop() && rki < c.RKs) return false;
             }
+            if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
             double a = 1.0 / c.RKs, b = (double)(c.RKs - 1) / c.RKs;
             if (g_mutation == MUT_SSPRK2_WEIGHTS) std::swap(a, b);
             cxayb(u0, a, u, b, u1);
@@ -1477,14 +1479,16 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
 // One step at step i, time t from the state (u, v, η, sst, um, vm) as given (Prog with its
 // ghost copies; um, vm the tracer's mid-point velocities), T = Tprog = Tcomm = Float64 or
 // Float32 (c->T; arrays are float64 holding values of that format), returning its
-// intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo),
+// intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo) --
+// under SSPRK2 of stage k+1 for k < RKs, then u1, v1, η1 after the last stage (k = RKs) --
 // then u0, v0, η0 after ghost_points! (:198), then u0, v0 after the dissipation, then um, vm
-// after the step, then the new u, v, η, sst.  Classic RK only.
+// after the step, then the new u, v, η, sst.  Classic RK and SSPRK2.
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
               const double *eta_ref, const double *Feta, const double *f_u, const double *f_v, int64_t i, int64_t t,
               const double *u, const double *v, const double *eta, const double *sst, const double *um,
               const double *vm, double *const *out) {
-    if (c->time_scheme != SWM_RK || c->T != c->Tprog || c->Tcomm != c->Tprog || c->scale_log2 != 0 || c->compensated)
+    if ((c->time_scheme != SWM_RK && c->time_scheme != SWM_SSPRK2) || c->T != c->Tprog || c->Tcomm != c->Tprog ||
+        c->scale_log2 != 0 || c->compensated)
         return -1;
     if (c->dynamics == SWM_LINEAR && (!f_u || !f_v)) return -1;
     const double *st[8] = {H, Fx, Fy, f_q, eta_ref, Feta, f_u, f_v};

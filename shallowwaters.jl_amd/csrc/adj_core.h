@@ -1,6 +1,7 @@
-// adj_core.h -- reverse-mode (adjoint) kernels of one classic Runge-Kutta time step.
+// adj_core.h -- reverse-mode (adjoint) kernels of one Runge-Kutta time step (classic RK or
+// SSPRK2).
 //
-// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (classic RK, T = Tprog) is
+// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (classic RK or SSPRK2, T = Tprog) is
 // differentiated by hand, operator by operator, in gather form: every adjoint value is a
 // sum over the forward quantities that read the corresponding input, so each output cell
 // is written by one lane and no atomics are needed; results do not depend on the launch
@@ -55,8 +56,10 @@ constexpr int ADJ_Y0 = -1;
 
 // the right-hand side forms of the stage adjoint (template argument SCH): the nonlinear
 // dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics; ADJ_KEEP (with a
-// nonlinear form): q and KEu+KEv kept from an earlier state (nstep_advcor > 0)
-enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2, ADJ_KEEP = 4 };
+// nonlinear form): q and KEu+KEv kept from an earlier state (nstep_advcor > 0); ADJ_CONT
+// (alone): the semi-implicit continuity of the SSPRK2 scheme only, the mass fluxes
+// recomputed from the state (UVfluxes!, continuity.jl:83-85) whatever the dynamics
+enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2, ADJ_KEEP = 4, ADJ_CONT = 8 };
 SWM_HD int adj_y1(int ny) { return ny + 3; }
 
 template <class T> SWM_HD T amin(T a, T b) { return a < b ? a : b; }
@@ -197,7 +200,7 @@ template <class W, bool PER, bool NEXT, bool A_L0, int SCH = ADJ_AH>
 SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<typename W::R> &A, int r, int ya, int yb) {
     using V = typename W::V;
     using R = typename W::R;
-    constexpr bool LIN = (SCH & 3) == ADJ_LIN, KEEP = (SCH & ADJ_KEEP) != 0;
+    constexpr bool LIN = (SCH & 3) == ADJ_LIN, KEEP = (SCH & ADJ_KEEP) != 0, CONT = SCH == ADJ_CONT;
     const V zero = V(R(0));
     StageLoads<V> p;
     const int rs = amin(r, yb + 1), ny = A.ny;
@@ -207,8 +210,8 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
         p.fv = w.ld_s(A.fv, amax(rs - 2, -1), A.fv_2d);
     } else {
         p.u = w.ld(A.s.u, rs); p.v = w.ld(A.s.v, rs); p.e = w.ld(A.s.e, rs); p.H = w.ld(A.H, rs);
-        p.fq = KEEP ? zero : w.ld_s(A.fq, rs - 1, A.fq_2d);
-        p.qe = PER && !KEEP ? w.ld_row1(A.qend, rs - 1) : zero;
+        p.fq = KEEP || CONT ? zero : w.ld_s(A.fq, rs - 1, A.fq_2d);
+        p.qe = PER && !KEEP && !CONT ? w.ld_row1(A.qend, rs - 1) : zero;
         p.fu = p.fv = zero;
     }
     p.qk = p.qke = p.lq = p.lk = zero;
@@ -228,7 +231,7 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
     if (NEXT && !A_L0) {
         p.au = w.ld(A.a_in.u, ye); p.av = w.ld(A.a_in.v, ye); p.ae = w.ld(A.a_in.e, ye);
     }
-    p.fx = A.fxg ? w.ld(A.lfx, ye) : zero;
+    p.fx = A.fxg && !CONT ? w.ld(A.lfx, ye) : zero;
     return p;
 }
 
@@ -274,13 +277,18 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
 //   λu = -¼·((f_v·gv + (f_v·gv)(x+1)) + ((f_v·gv)(y-1) + (f_v·gv)(x+1,y-1)))
 //   λv = ¼·((f_u·gu + (f_u·gu)(x-1)) + ((f_u·gu)(y+1) + (f_u·gu)(x-1,y+1)))
 //   λη = g·λp (- γ·ge), λp as above.
+//
+// CONT: gu = gv = 0 (and no PV): λU = ge(x+1) - ge, λV = ge(y+1) - ge, λu = λU·hu, λv = λV·hv,
+//   λη = λh (- γ·ge): the transpose of dη = -((U-U(x-1)) + (V-V(y-1))) (- γη) alone.
 template <class W, bool PER, bool NEXT, bool A_L0, bool PRE, int SCH = ADJ_AH>
 SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, int yb) {
     using R = typename W::R;
     using V = typename W::V;
     using M = typename W::M;
     constexpr bool SAD = (SCH & 3) == ADJ_SAD, LIN = (SCH & 3) == ADJ_LIN, KEEP = (SCH & ADJ_KEEP) != 0;
+    constexpr bool CONT = SCH == ADJ_CONT;
     static_assert(!(LIN && KEEP), "the linear dynamics keep nothing");
+    static_assert(!(CONT && NEXT), "the continuity form is not part of a classic RK stage");
     const int nx = A.nx, ny = A.ny;
     const auto x = w.x;
     const M ucol = PER ? M((x >= 0) & (x <= nx - 1)) : M((x >= 1) & (x <= nx - 1));
@@ -329,8 +337,8 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             zv = fold_v<W, PER>(w, A.l0.v, yd, c.lv, A.oma, nx);
             if (NEXT) sv = fold_v<W, PER>(w, A.mu_in.v, yd, c.mv, A.oma, nx);
         }
-        const V gu_1 = W::sel(ucol & ydu, NEXT ? A.a * zu + A.b * su : A.a * zu, zero);
-        const V gv_1 = W::sel(vcol & ydv, NEXT ? A.a * zv + A.b * sv : A.a * zv, zero);
+        const V gu_1 = CONT ? zero : W::sel(ucol & ydu, NEXT ? A.a * zu + A.b * su : A.a * zu, zero);
+        const V gv_1 = CONT ? zero : W::sel(vcol & ydv, NEXT ? A.a * zv + A.b * sv : A.a * zv, zero);
         const V ge_1 = W::sel(vcol & ydu, NEXT ? A.a * ze + A.b * se : A.a * ze, zero);
 
         V lu_2, lv_2, le_2, lp_1;
@@ -346,7 +354,9 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const V dudy_1 = u_0 - u_1;
         const bool yq1 = r - 1 >= 0 && r - 1 <= ny;
         const M qv_1 = qcol & yq1;
-        const V q_1 = KEEP ? W::sel(qv_1, c.qk, zero) : W::sel(qv_1, (fq_1 + dvdx_1 - dudy_1) / W::sel(qv_1, hq_1, V(R(1))), zero);
+        const V q_1 = CONT ? zero
+                    : KEEP ? W::sel(qv_1, c.qk, zero)
+                           : W::sel(qv_1, (fq_1 + dvdx_1 - dudy_1) / W::sel(qv_1, hq_1, V(R(1))), zero);
         const V qxp_1 = W::xp(q_1);
         const V qxm_1 = PER && !SAD ? W::sel(qfix, qend_1, W::xm(q_1)) : W::xm(q_1);
         V qa_1 = zero, qb_1 = zero, qg_1 = zero, qd_1 = zero, la_1 = zero, lb_1 = zero, lg_1 = zero, ld_1 = zero;
@@ -451,9 +461,9 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
                 w.st(A.mu_out.e, y, le_2, st_e & ye_st);
             }
             // periodic: λ of q(nx, y) from qβ(0, y) and qγ(0, y+1)
-            if (PER && !SAD && !LIN && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
+            if (PER && !SAD && !LIN && !CONT && y >= 0 && y <= ny) w.st_row1(A.lqend, y, k12 * (lb_2 + lg_1), w.own & (x == 0));
             // the wind stress enters du as Tprog(Fxt·Fx) (rhs.jl:242): λFx += Fxt·λdu, u cells
-            if (A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
+            if (!CONT && A.fxg) w.st(A.lfx, y, c.fx + A.fxt * gu_2, w.own & iu);
             if constexpr (KEEP) {   // λ of the kept q (corner row y) and KEu+KEv (p = ½(KEu+KEv) + gη)
                 if (y >= 0 && y <= ny) w.st(A.lqk, y, c.lq + W::sel(qcol, lq_2_keep, zero), w.own & qcol);
                 if (y >= 0 && y <= ny + 1) w.st(A.lkk, y, c.lk + half * lp_2, w.own & (x >= 0) & (x <= nx + 1));
@@ -964,6 +974,15 @@ template <class R> SWM_HD void snapshot_adjoint_elem(R *lu, R *lum, long k) {
     lum[k] = R(0);
 }
 
+// ---- d = ax·x + ay·y + az·z at element k of the layout (every cell, halo included); y and z
+// are left out when null.  d may be x.
+template <class R> SWM_HD void lincomb_elem(R *d, const R *x, R ax, const R *y, R ay, const R *z, R az, long k) {
+    R s = ax * x[k];
+    if (y) s = s + ay * y[k];
+    if (z) s = s + az * z[k];
+    d[k] = s;
+}
+
 // ---- adjoint of the step's input: the ghost copies of the state folded into the interior
 // and the halo cleared.  One thread per cell (x, y), x = -2 .. nx+2, y = -1 .. ny+2.
 struct FoldOutArgs {
@@ -1040,7 +1059,8 @@ template <class R, class C> SWM_HD void fold_out_cell(const C &c, const FoldOutA
 // ------------------------------------------------------------------ the reverse of one step
 //
 // Buffers of the adjoint.  The tape holds the forward intermediates of the step being
-// reversed (the stage inputs s_1..s_RKo, u0 before and after the dissipation, the tracer
+// reversed (the stage inputs s_1..s_RKo, or under SSPRK2 the stage inputs w_1..w_RKs and the
+// velocities u1, v1 after the last stage; u0 before and after the dissipation, the tracer
 // state it read); L is the carried adjoint (halo unfolded), Z a second set of the same,
 // Ma / Mb / A the stage adjoint's μ ping-pong and running sum.  The tracer adjoint's
 // records reuse Z, Ma, Mb, A and two more fields.
@@ -1059,6 +1079,7 @@ struct AdjFields {
     uint32_t tr0, tr1;
     uint32_t tg0, tg1;              // the tracer adjoint's gathered interpolation records
     uint32_t lfx;                   // λ of Fx at the u points (accumulated over the reversed steps)
+    uint32_t zero;                  // a field of zeros (never written)
 };
 
 // Scalars of the configuration the reverse step needs (all in the arithmetic type R).
@@ -1074,12 +1095,15 @@ template <class R> struct AdjConsts {
     int sad;                        // Sadourny's PV fluxes (k12 = ½)
     int lin;                        // the linear dynamics (rhs_linear!)
     int nadvcor;                    // nstep_advcor: q and KEu+KEv kept (0: formed by every stage)
+    int ssp;                        // SSPRK2: the number of stages RKs (0: classic RK)
+    R cs, sa, sb;                   // SSPRK2: Δt_Δs, and the weights 1/RKs, (RKs-1)/RKs of u0 = a·u + b·u1
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
 // the state before it.  Launcher K issues the kernels: stage<PER, NEXT, A_L0, PRE>(args),
 // qend(args), qend_adj(args, mu), diss<PER, DRAG, SMAG>(args), tracer(args), snapshot(),
-// in order on one queue.
+// advcor_adj(args), lqend_keep(args), zero(field), lincomb(d, x, ax, y, ay, z, az), in order
+// on one queue.
 template <class R, class K> int reverse_tracer(K &k, AdjFields &F, const AdjConsts<R> &c, long i) {
     int rc = 0;
     // tracer! (tracer_advection.jl:1-44): advection (:19-42), then the snapshot (:12-17)
@@ -1100,6 +1124,67 @@ template <class R, class K> int reverse_tracer(K &k, AdjFields &F, const AdjCons
         if ((i + c.nadvstep_half) % c.nadvstep == 0 && (rc = k.snapshot(F.L.u, F.Lum, F.L.v, F.Lvm))) return rc;
     }
     return rc;
+}
+
+constexpr uint32_t NOF = 0xFFFFFFFFu;   // "no field" (lincomb operands left out)
+
+// The stages of an SSPRK2 step, last first (time_integration.jl:80-110).  Forward, stage k
+// from w_k = (u_k, v_k, η_k) (ghost copies applied):
+//   u_{k+1} = u_k + c·du(w_k), v likewise, then ghost_points_uv!;
+//   η_{k+1} = η_k + c·dη(u_{k+1}, v_{k+1}, η_k)       (c = Δt_Δs, η ghost copies at the next stage)
+// and u0 = a·u + b·u_{RKs+1} (a = 1/RKs, b = (RKs-1)/RKs), ghost_points! on u0.  Every
+// adjoint is carried with its halo unfolded (a halo cell holds the adjoint of the ghost copy,
+// folded into the cell it copies by its reader), so the stage bookkeeping is elementwise:
+// with Λ the adjoint of w_{k+1} and λ0 that of u0 (l0),
+//   Λ ← b·λ0;  for k = RKs .. 1:
+//     μ2 = (∂dη/∂(u_{k+1}, v_{k+1}, η_k))ᵀ c·Λη           (stage march, continuity form)
+//     Λu += μ2u, Λv += μ2v                                 (Λ of u_{k+1}, v_{k+1} complete)
+//     μ1 = (∂(du, dv)/∂w_k)ᵀ c·(Λu, Λv)                    (stage march with λη = 0)
+//     Λu += μ1u, Λv += μ1v, Λη += μ2η + μ1η                (Λ of w_k)
+//   the step's input: Λ + a·λ0.
+// The result goes to `out`; F.L and F.Z are swapped as by the classic RK stages.
+template <class R, class K> int reverse_ssp2(K &k, AdjFields &F, const AdjConsts<R> &c, UVE l0, UVE out) {
+    int rc = 0;
+    const UVE Lm = out, M2 = F.Ma, M1 = F.Mb;
+    const R one = R(1), nil = R(0);
+    if ((rc = k.lincomb(Lm.u, l0.u, c.sb, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.v, l0.v, c.sb, NOF, nil, NOF, nil)) ||
+        (rc = k.lincomb(Lm.e, l0.e, c.sb, NOF, nil, NOF, nil)))
+        return rc;
+    StageAdjArgs<R> S{};
+    S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;
+    S.fu = F.fu; S.fv = F.fv; S.fu_2d = F.fu_2d; S.fv_2d = F.fv_2d;
+    S.qk = F.qk; S.lqk = F.lqk; S.lkk = F.lkk;
+    S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
+    S.nx = c.nx; S.ny = c.ny; S.P = c.P;
+    S.lfx = F.lfx; S.fxt = c.fxt;
+    S.a = c.cs; S.b = nil;
+    S.mu_in = S.a_in = S.a_out = F.A;               // (read and written by NEXT stages only)
+    const int sch = (c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH) | (c.nadvcor > 0 && !c.lin ? ADJ_KEEP : 0);
+    const bool wrap = c.per && (sch & 3) == ADJ_AH, keep = (sch & ADJ_KEEP) != 0;
+    for (int st = c.ssp; st >= 1; --st) {
+        // continuity at (u_{k+1}, v_{k+1}, η_k)
+        S.s = UVE{F.s[st].u, F.s[st].v, F.s[st - 1].e};
+        S.l0 = Lm; S.mu_out = M2; S.fxg = 0;
+        if ((rc = k.stage(S, c.per, 0, 0, 0, ADJ_CONT))) return rc;
+        if ((rc = k.lincomb(Lm.u, Lm.u, one, M2.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, M2.v, one, NOF, nil)))
+            return rc;
+        // momentum at w_k
+        S.s = F.s[st - 1];
+        S.l0 = UVE{Lm.u, Lm.v, F.zero}; S.mu_out = M1; S.fxg = c.fxg;
+        if (wrap && !keep && (rc = k.qend(S))) return rc;
+        if ((rc = k.stage(S, c.per, 0, 0, 0, sch))) return rc;
+        if (wrap && !keep && (rc = k.qend_adj(S, M1))) return rc;
+        if (wrap && keep && (rc = k.lqend_keep(S))) return rc;
+        if ((rc = k.lincomb(Lm.u, Lm.u, one, M1.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, M1.v, one, NOF, nil)) ||
+            (rc = k.lincomb(Lm.e, Lm.e, one, M2.e, one, M1.e, one)))
+            return rc;
+    }
+    if ((rc = k.lincomb(Lm.u, Lm.u, one, l0.u, c.sa, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, l0.v, c.sa, NOF, nil)) ||
+        (rc = k.lincomb(Lm.e, Lm.e, one, l0.e, c.sa, NOF, nil)))
+        return rc;
+    F.L = out;
+    F.Z = l0;
+    return 0;
 }
 
 template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts<R> &c, long i) {
@@ -1131,6 +1216,7 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
             if (wq && (rc = k.zero(F.lqk))) return rc;
         }
     }
+    if (c.ssp > 0) return reverse_ssp2(k, F, c, l0, out);
     // the RK stages, last first (time_integration.jl:55-78)
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;

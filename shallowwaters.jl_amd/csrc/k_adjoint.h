@@ -160,6 +160,13 @@ template <class R> __global__ void fold_out_kernel(ArenaRef ar, adj::FoldOutArgs
     adj::fold_out_cell<R>(GpuCells<R>{(char *)ar.base, P}, A, oma, x, y);
 }
 
+// d = ax·x + ay·y + az·z over n elements (the SSPRK2 stage bookkeeping); y, z nullable
+template <class R>
+__global__ void lincomb_kernel(R *d, const R *x, R ax, const R *y, R ay, const R *z, R az, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        adj::lincomb_elem<R>(d, x, ax, y, ay, z, az, (long)k);
+}
+
 // zero n elements of size 4 (a fill kernel: usable where a memset node is not)
 static __global__ void zero_words_kernel(uint32_t *p, size_t n) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0u;

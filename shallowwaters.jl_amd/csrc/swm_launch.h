@@ -728,7 +728,10 @@ template <class Fm> struct Adj {
 
     static int check(swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        if (c.time_scheme != SWM_RK) return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK scheme only");
+        if (c.time_scheme != SWM_RK && c.time_scheme != SWM_SSPRK2)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK and the SSPRK2 schemes only");
+        if (c.time_scheme == SWM_SSPRK2 && (c.RKs < 2 || c.RKs > 3))   // the taped step keeps RKs+3 velocity slots of 6
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint of the SSPRK2 scheme supports RKs = 2 or 3 stages");
         if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint needs T = Tprog = Tcomm in {Float64, Float32}");
         if (c.scale_log2 != 0 || c.compensated)
@@ -744,12 +747,12 @@ template <class Fm> struct Adj {
         return SWM_OK;
     }
 
-    // carve the adjoint's fields: two more slot sets and the adjoint fields (21), zeroed
+    // carve the adjoint's fields: two more slot sets and the adjoint fields (28), zeroed
     static int init(swm_ctx *ctx) {
         if (ctx->adj) return SWM_OK;
         int rc = check(ctx);
         if (rc) return rc;
-        const int nf2 = 6 + 27;
+        const int nf2 = 6 + 28;
         const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
         if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
         AdjState *st = new AdjState();
@@ -774,6 +777,7 @@ template <class Fm> struct Adj {
         one(F.lfx, true);
         one(F.lqend, false);
         one(F.qk, true); one(F.lqk, true); one(F.lkk, true);
+        one(F.zero, true);
         if (rc) {
             delete st;
             return rc;
@@ -810,6 +814,12 @@ template <class Fm> struct Adj {
         k.sad = c.adv_scheme == SWM_ADV_SADOURNY;
         k.lin = c.dynamics == SWM_LINEAR;
         k.nadvcor = (adv_form(c) & ADV_KEEP) ? (int)c.nstep_advcor : 0;
+        if (c.time_scheme == SWM_SSPRK2) {               // the forward's coefficients, in the same format
+            k.ssp = c.RKs;
+            k.cs = (R)c.Dt_Ds;
+            k.sa = (R)(1.0 / c.RKs);
+            k.sb = (R)((double)(c.RKs - 1) / c.RKs);
+        }
         k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(k.sad ? 0.5 : 1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
         k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
         k.hdu = (R)c.half_dtadvu; k.hdv = (R)c.half_dtadvv; k.du = (R)c.dtadvu; k.dv = (R)c.dtadvv; k.jSST = (R)c.jSST;
@@ -840,7 +850,8 @@ template <class Fm> struct Adj {
             else launch_stage<PER, true, false, false, SCH>(S);
         }
         template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
-            if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
+            if (sch == adj::ADJ_CONT) launch_stage<PER, false, false, false, adj::ADJ_CONT>(S);   // (SSPRK2: no successor)
+            else if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
             else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, al0, pre);
             else if (sch == (adj::ADJ_AH | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_AH | adj::ADJ_KEEP>(S, next, al0, pre);
             else if (sch == (adj::ADJ_SAD | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_SAD | adj::ADJ_KEEP>(S, next, al0, pre);
@@ -853,6 +864,13 @@ template <class Fm> struct Adj {
         }
         int lqend_keep(const adj::StageAdjArgs<R> &S) {
             hipLaunchKernelGGL((lqend_keep_kernel<R>), dim3((S.ny + 1 + 255) / 256), dim3(256), 0, ctx->stream, ctx->aref(), S);
+            return fail_launch();
+        }
+        int lincomb(uint32_t d, uint32_t x, R ax, uint32_t y, R ay, uint32_t z, R az) {   // 2-D fields of the arena
+            char *b = (char *)ctx->arena;
+            hipLaunchKernelGGL((lincomb_kernel<R>), dim3(1024), dim3(256), 0, ctx->stream, (R *)(b + d), (const R *)(b + x), ax,
+                               y == adj::NOF ? nullptr : (const R *)(b + y), ay, z == adj::NOF ? nullptr : (const R *)(b + z), az,
+                               ctx->elems);
             return fail_launch();
         }
         int zero(uint32_t o) {             // a 2-D field of the arena
@@ -966,7 +984,8 @@ template <class Fm> struct Adj {
     }
 
     // The taped step k: step i = i0 + k from Prog without copy-back, every intermediate kept
-    // in its own slot (the stage inputs s_1..s_RKo, u0 before / after the dissipation).
+    // in its own slot (the stage inputs s_1..s_RKo, or w_1..w_RKs+1 under SSPRK2; u0 before /
+    // after the dissipation).
     static int tape_step(swm_ctx *ctx, int64_t k, adj::AdjFields &F) {
         AdjState *st = ctx->adj;
         const swm_config &c = ctx->c;
@@ -979,7 +998,8 @@ template <class Fm> struct Adj {
             if (s != ctx->pu) uv[nu++] = s;
             if (s != ctx->pe) ee[ne++] = s;
         }
-        // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo, [5] the dissipation output
+        // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo (SSPRK2: [2..RKs+1] w_2..w_RKs+1),
+        // [5] the dissipation output
         const Slots P{uv[0], ee[0]}, a{uv[1], ee[1]}, none{};
         auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
         int rc = SWM_OK;
@@ -1001,7 +1021,21 @@ template <class Fm> struct Adj {
             HIPCHK(hipMemcpyAsync((char *)ctx->arena + F.qk, ctx->qs.base, ctx->elems * ctx->qs.es, hipMemcpyDeviceToDevice,
                                   ctx->stream));
         }
-        for (int q = 1; q <= c.RKo && rc == SWM_OK; ++q) {
+        const bool ssp = c.time_scheme == SWM_SSPRK2;
+        if (ssp) {                                           // :82-110; w_q in sk(q), q = 1 .. RKs+1
+            Coef<Q> kk{};
+            kk.c = (Q)c.Dt_Ds;
+            kk.x = (Q)(1.0 / c.RKs);
+            kk.y = (Q)((double)(c.RKs - 1) / c.RKs);
+            for (int q = 1; q <= c.RKs && rc == SWM_OK; ++q) {
+                const bool last = q == c.RKs;
+                const int m = last ? M_SSP2_LAST : M_SSP2;
+                const Slots n = sk(q + 1);
+                rc = LF::stage(ctx, m, sk(q), P, none, n, last ? a : none, kk, T);
+                if (!rc) rc = LF::continuity(ctx, m, Slots{n.uv, sk(q).e}, P, none, n, last ? a : none, kk, T);
+            }
+        }
+        for (int q = 1; !ssp && q <= c.RKo && rc == SWM_OK; ++q) {
             // stages q, q+1 as one fused pair that also keeps the input of stage q+1
             if (q + 1 <= c.RKo && LF::can_tape(ctx, mode(q), mode(q + 1))) {
                 rc = LF::fused_tape(ctx, mode(q), mode(q + 1), sk(q), P, q == 1 ? none : a, q + 1 < c.RKo ? sk(q + 2) : none,
@@ -1024,7 +1058,7 @@ template <class Fm> struct Adj {
         }
         const int sst_old = ctx->rsst;
         if ((rc = LF::tracer(ctx, i, post))) return rc;
-        for (int q = 1; q <= c.RKo; ++q) {
+        for (int q = 1; q <= (ssp ? c.RKs + 1 : c.RKo); ++q) {
             const Slots s = sk(q);
             F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
         }

@@ -21,16 +21,17 @@ _lib = None
 class EmuCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("nx", "ny", "per", "RKo", "cont", "drag", "smag", "consume", "tracer", "nadvstep",
-                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin")] + \
+                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin", "ssp")] + \
                [("a", ctypes.c_double * 4), ("b", ctypes.c_double * 4)] + \
+               [(n, ctypes.c_double) for n in ("cs", "sa", "sb")] + \
                [(n, ctypes.c_double) for n in
                 ("g", "gamma", "k12", "oma", "cD", "rD", "cS", "nuB", "dtd", "hdu", "hdv", "du", "dv", "jSST")]
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-C", EMU_DIR, "-B" if force else "all"], check=True,
-                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    """Build the emulation library (make rebuilds it when a source is newer)."""
+    subprocess.run(["make", "-C", EMU_DIR, "-B" if force else "all"], check=True,
+                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     return LIB_PATH
 
 
@@ -65,6 +66,11 @@ def config(S, cfg, L=8, nan_edges=False, fmt=8):
     k.g, k.gamma, k.oma = cfg.g, cfg.gamma, cfg.one_minus_alpha
     k.sad = int(cfg.adv_scheme == 1)                       # Sadourny: PV weight ½
     k.lin = int(cfg.dynamics == 1)                         # the linear dynamics
+    if cfg.time_scheme == 1:                               # SSPRK2: stages, Δt_Δs, weights of u0
+        k.ssp = cfg.RKs
+        k.cs = round_scalar(cfg.Dt_Ds, P.Tprog)
+        k.sa = round_scalar(1.0 / cfg.RKs, P.Tprog)
+        k.sb = round_scalar((cfg.RKs - 1) / cfg.RKs, P.Tprog)
     k.k12 = round_scalar(0.5 if k.sad else 1.0 / 12.0, P.T)
     k.cD, k.rD, k.cS, k.nuB, k.dtd = cfg.cD, cfg.rD, cfg.cSmag, cfg.nuB, cfg.Dt_diff
     k.hdu, k.hdv, k.du, k.dv, k.jSST = cfg.half_dtadvu, cfg.half_dtadvv, cfg.dtadvu, cfg.dtadvv, cfg.jSST

@@ -217,7 +217,8 @@ template <class R> struct EmuK {
         else stage_t<PER, true, false, false, SCH>(S);
     }
     template <bool PER> void stage_s(const StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
-        if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, al0, pre);
+        if (sch == ADJ_CONT) stage_t<PER, false, false, false, ADJ_CONT>(S);
+        else if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, al0, pre);
         else if (sch == ADJ_SAD) stage_p<PER, ADJ_SAD>(S, next, al0, pre);
         else if (sch == (ADJ_AH | ADJ_KEEP)) stage_p<PER, ADJ_AH | ADJ_KEEP>(S, next, al0, pre);
         else if (sch == (ADJ_SAD | ADJ_KEEP)) stage_p<PER, ADJ_SAD | ADJ_KEEP>(S, next, al0, pre);
@@ -242,6 +243,14 @@ template <class R> struct EmuK {
     int lqend_keep(const StageAdjArgs<R> &S) {
         Cells<R> c{a};
         for (int y = 0; y <= S.ny; ++y) lqend_keep_row<R>(c, S, y);
+        return 0;
+    }
+    int lincomb(uint32_t d, uint32_t x, R ax, uint32_t y, R ay, uint32_t z, R az) {
+        R *b = (R *)a->mem.data();
+        const size_t n = (size_t)a->P * a->rows;
+        for (size_t k = 0; k < n; ++k)
+            lincomb_elem<R>(b + d / sizeof(R), b + x / sizeof(R), ax, y == NOF ? nullptr : b + y / sizeof(R), ay,
+                            z == NOF ? nullptr : b + z / sizeof(R), az, (long)k);
         return 0;
     }
     int zero(uint32_t o) {
@@ -310,8 +319,9 @@ extern "C" {
 // Scalars of a configuration, as the GPU driver derives them from swm_config.
 struct EmuCfg {
     int32_t nx, ny, per, RKo, cont, drag, smag, consume, tracer, nadvstep, nadvstep_half, nstep_diff;
-    int32_t L, nan_edges, fmt, sad, lin;
+    int32_t L, nan_edges, fmt, sad, lin, ssp;
     double a[4], b[4];
+    double cs, sa, sb;
     double g, gamma, k12, oma, cD, rD, cS, nuB, dtd, hdu, hdv, du, dv, jSST;
 };
 
@@ -363,6 +373,8 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
     k.nx = c.nx; k.ny = c.ny; k.ep = c.per ? 1 : 0;
     k.sad = c.sad;
     k.lin = c.lin;
+    k.ssp = c.ssp;
+    k.cs = (R)c.cs; k.sa = (R)c.sa; k.sb = (R)c.sb;
     return k;
 }
 
@@ -371,7 +383,7 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
 // op 2: λ before the dissipation.  in: u0 v0 e0 H lu lv le;  out: λu λv λe
 // op 3: tracer at step i (advection and snapshot as due).  in: u v um vm sst lum lvm lsst;
 //       out: λu λv λum λvm λsst
-// op 4: reverse of step i.  in: s_1..s_RKo (3 each), pre (3), post (2), um vm sst, H fq,
+// op 4: reverse of step i.  in: s_1..s_RKo (3 each; SSPRK2: w_1..w_RKs+1), pre (3), post (2), um vm sst, H fq,
 //       L (u v e sst um vm) (+ f_u f_v with the linear dynamics);  out: L (u v e sst um vm)
 // op 5: fold of the ghost copies.  in: Lu Lv Le Lsst;  out: gu gv ge gsst
 template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *const *in, double *const *out) {
@@ -395,6 +407,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
     F.qend = A.add(false); F.lqend = A.add(false);
     F.tr0 = A.add(true); F.tr1 = A.add(true);
     F.tg0 = A.add(true); F.tg1 = A.add(true);
+    F.zero = A.add(true);
     try {
         if (op == 1) {
             put<R>(A, F.s[0].u, in[0], sh.u); put<R>(A, F.s[0].v, in[1], sh.v); put<R>(A, F.s[0].e, in[2], sh.e);
@@ -426,7 +439,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
         } else if (op == 3 || op == 4) {
             int p = 0;
             if (op == 4) {
-                for (int j = 0; j < c.RKo; ++j) {
+                for (int j = 0; j < (c.ssp ? c.ssp + 1 : c.RKo); ++j) {
                     put<R>(A, F.s[j].u, in[p++], sh.u); put<R>(A, F.s[j].v, in[p++], sh.v); put<R>(A, F.s[j].e, in[p++], sh.e);
                 }
                 put<R>(A, F.pre.u, in[p++], sh.u); put<R>(A, F.pre.v, in[p++], sh.v); put<R>(A, F.pre.e, in[p++], sh.e);

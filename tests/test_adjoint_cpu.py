@@ -246,7 +246,13 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
                                    ("periodic", {"adv_scheme": "Sadourny"}),
                                    ("nonperiodic", {"adv_scheme": "Sadourny", "RKo": 3}),
                                    ("nonperiodic", {"dynamics": "linear"}),
-                                   ("periodic", {"dynamics": "linear", "RKo": 3, "bottom_drag": "linear"})])
+                                   ("periodic", {"dynamics": "linear", "RKo": 3, "bottom_drag": "linear"}),
+                                   ("nonperiodic", {"time_scheme": "SSPRK2"}),
+                                   ("periodic", {"time_scheme": "SSPRK2", "RKs": 2, "cfl": 0.5, "Uadv": 40.0}),
+                                   ("periodic", {"time_scheme": "SSPRK2", "adv_scheme": "Sadourny",
+                                                 "surface_relax": True}),
+                                   ("nonperiodic", {"time_scheme": "SSPRK2", "dynamics": "linear", "RKs": 2,
+                                                    "cfl": 0.5, "Uadv": 40.0})])
 def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
     """Three steps: the tracer's snapshot (step 2) and advection (step 3) included."""
     P, S, cfg, prog = setup(swm, bc, **kw)
@@ -263,7 +269,8 @@ def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
         assert np.all(gg[~m] == 0), f"{name}: halo of the gradient not cleared"
 
 
-@pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"dynamics": "linear"})])
+@pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"dynamics": "linear"}),
+                                   ("periodic", {"time_scheme": "SSPRK2"})])
 def test_dot_product_identity_across_wave_strips(swm, orc, bc, kw):
     """A grid of several wave strips and short segments: <J dx, y> = <dx, J^T y>."""
     P, S, cfg, prog = setup(swm, bc, nx=130, L_ratio=0.5, **kw)

@@ -165,8 +165,8 @@ def test_rrule_of_time_integration(swm, gpu):
 
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
-    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", time_scheme="SSPRK2"),
-               dict(T="Float32", time_scheme="SSPRK3")):
+    for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", time_scheme="SSPRK2", RKs=4),
+               dict(T="Float32", time_scheme="SSPRK3"), dict(T="Float32", time_scheme="4SSPRK3")):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
             sim.upload(prog)
@@ -277,6 +277,36 @@ def test_kept_fields_dot_product_fp64(swm, orc, gpu, bc, kw):
         assert np.array_equal(np.asarray(a), np.asarray(b)), "the gradient depends on the snapshots"
     x = list(ghosted_state(swm, S, prog))
     rng = np.random.default_rng(5)
+    for m in range(3):
+        dx = [np.where(mk, rng.standard_normal(mk.shape), 0.0) for mk in interior_masks(S)]
+        dxg = list(ghosted_state(swm, S, type(prog)(*dx)))
+        _, dy = orc.tlm(S, cfg, 0, x, dxg, nsteps=n)
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(dy, ybar))
+        rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, g))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dx, ȳ> = {lhs!r}, <dx, J^T ȳ> = {rhs!r}"
+
+
+@pytest.mark.parametrize("bc,kw", [("nonperiodic", dict()),
+                                   ("periodic", dict(RKs=2, cfl=0.5, surface_relax=True)),
+                                   ("periodic", dict(adv_scheme="Sadourny", bottom_drag="linear")),
+                                   ("nonperiodic", dict(dynamics="linear", RKs=2, cfl=0.5)),
+                                   ("periodic", dict(nstep_advcor=2))])
+def test_ssprk2_dot_product_fp64(swm, orc, gpu, bc, kw):
+    """The SSPRK2 scheme (time_integration.jl:80-110: momentum, then the continuity from the
+    updated velocities, RKs stages, u0 = u/RKs + (RKs-1)/RKs·u1; two stages need a CFL number
+    below one): the FP64 dot-product identity
+    with the oracle's tangent-linear model of the same steps; the gradient does not depend on
+    the number of snapshots."""
+    P, S, cfg, prog = setup(swm, bc, nx=90, time_scheme="SSPRK2", **kw)
+    n = 7
+    ybar = cotangent(S, 6, np.float64)
+    print(f"SSPRK2 {bc} {kw} n={n}", flush=True)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=2)
+    g0, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=0)
+    for a, b in zip(g, g0):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), "the gradient depends on the snapshots"
+    x = list(ghosted_state(swm, S, prog))
+    rng = np.random.default_rng(6)
     for m in range(3):
         dx = [np.where(mk, rng.standard_normal(mk.shape), 0.0) for mk in interior_masks(S)]
         dxg = list(ghosted_state(swm, S, type(prog)(*dx)))
