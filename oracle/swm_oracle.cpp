@@ -370,6 +370,7 @@ template <class T, class Tp, class Tc> struct Model {
     // after the dissipation, and the tracer's mid-point velocities after the step
     struct Tape {
         std::vector<Arr<Tp>> s;
+        std::vector<Arr<Tp>> p;          // 4SSPRK3: u1, v1 of every stage after ghost_points_uv!
         Arr<Tp> au, av, ae, du, dv;
         Arr<T> um, vm;
     } *tape = nullptr;
@@ -1084,6 +1085,7 @@ template <class T, class Tp, class Tc> struct Model {
             const Tp dt = Tp(c.Dt_D);
             for (int rki = 1; rki <= 4; ++rki) {
                 if (rki > 1) ghost_points(u1, v1, eta1);
+                if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
                 const Arr<T> &e = rhs_view(eta1, etar);
                 rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
                 caxb(u0, u1, dt, du);
@@ -1091,6 +1093,7 @@ template <class T, class Tp, class Tc> struct Model {
                 cxab(u1, 0.5, u1, u0);
                 cxab(v1, 0.5, v1, v0);
                 ghost_points_uv(u1, v1);
+                if (tape) { tape->p.push_back(u1); tape->p.push_back(v1); }
                 continuity(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
                 caxb(eta0, eta1, dt, deta);
                 cxab(eta1, 0.5, eta1, eta0);
@@ -1308,6 +1311,7 @@ static int tape_typed(const swm_config *c, const double *const *st, int64_t i, i
         ++k;
     };
     for (auto &A : tp.s) put(A);
+    for (auto &A : tp.p) put(A);
     put(tp.au); put(tp.av); put(tp.ae);
     put(tp.du); put(tp.dv);
     put(tp.um); put(tp.vm);
@@ -1480,14 +1484,16 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
 // ghost copies; um, vm the tracer's mid-point velocities), T = Tprog = Tcomm = Float64 or
 // Float32 (c->T; arrays are float64 holding values of that format), returning its
 // intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo) --
-// under SSPRK2 of stage k+1 for k < RKs, then u1, v1, η1 after the last stage (k = RKs) --
-// then u0, v0, η0 after ghost_points! (:198), then u0, v0 after the dissipation, then um, vm
-// after the step, then the new u, v, η, sst.  Classic RK and SSPRK2.
+// under SSPRK2 of stage k+1 for k < RKs, then u1, v1, η1 after the last stage (k = RKs);
+// under 4SSPRK3 of stage k+1 for k < 4, followed by u1, v1 of each of the 4 stages after its
+// ghost_points_uv! (:178) -- then u0, v0, η0 after ghost_points! (:198), then u0, v0 after
+// the dissipation, then um, vm after the step, then the new u, v, η, sst.  Classic RK, SSPRK2
+// and 4SSPRK3.
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
               const double *eta_ref, const double *Feta, const double *f_u, const double *f_v, int64_t i, int64_t t,
               const double *u, const double *v, const double *eta, const double *sst, const double *um,
               const double *vm, double *const *out) {
-    if ((c->time_scheme != SWM_RK && c->time_scheme != SWM_SSPRK2) || c->T != c->Tprog || c->Tcomm != c->Tprog ||
+    if (c->time_scheme == SWM_SSPRK3 || c->T != c->Tprog || c->Tcomm != c->Tprog ||
         c->scale_log2 != 0 || c->compensated)
         return -1;
     if (c->dynamics == SWM_LINEAR && (!f_u || !f_v)) return -1;

@@ -1,7 +1,8 @@
-// adj_core.h -- reverse-mode (adjoint) kernels of one Runge-Kutta time step (classic RK or
-// SSPRK2).
+// adj_core.h -- reverse-mode (adjoint) kernels of one Runge-Kutta time step (classic RK,
+// SSPRK2 or 4SSPRK3).
 //
-// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (classic RK or SSPRK2, T = Tprog) is
+// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (classic RK, SSPRK2 or 4SSPRK3,
+// T = Tprog) is
 // differentiated by hand, operator by operator, in gather form: every adjoint value is a
 // sum over the forward quantities that read the corresponding input, so each output cell
 // is written by one lane and no atomics are needed; results do not depend on the launch
@@ -57,7 +58,7 @@ constexpr int ADJ_Y0 = -1;
 // the right-hand side forms of the stage adjoint (template argument SCH): the nonlinear
 // dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics; ADJ_KEEP (with a
 // nonlinear form): q and KEu+KEv kept from an earlier state (nstep_advcor > 0); ADJ_CONT
-// (alone): the semi-implicit continuity of the SSPRK2 scheme only, the mass fluxes
+// (alone): the semi-implicit continuity of the SSPRK2 / 4SSPRK3 schemes only, the mass fluxes
 // recomputed from the state (UVfluxes!, continuity.jl:83-85) whatever the dynamics
 enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2, ADJ_KEEP = 4, ADJ_CONT = 8 };
 SWM_HD int adj_y1(int ny) { return ny + 3; }
@@ -1059,13 +1060,14 @@ template <class R, class C> SWM_HD void fold_out_cell(const C &c, const FoldOutA
 // ------------------------------------------------------------------ the reverse of one step
 //
 // Buffers of the adjoint.  The tape holds the forward intermediates of the step being
-// reversed (the stage inputs s_1..s_RKo, or under SSPRK2 the stage inputs w_1..w_RKs and the
-// velocities u1, v1 after the last stage; u0 before and after the dissipation, the tracer
-// state it read); L is the carried adjoint (halo unfolded), Z a second set of the same,
+// reversed (the stage inputs s_1..s_RKo, or under SSPRK2 / 4SSPRK3 the stage inputs w_k and
+// the velocities after each stage's momentum update; u0 before and after the dissipation, the
+// tracer state it read); L is the carried adjoint (halo unfolded), Z a second set of the same,
 // Ma / Mb / A the stage adjoint's μ ping-pong and running sum.  The tracer adjoint's
 // records reuse Z, Ma, Mb, A and two more fields.
 struct AdjFields {
     UVE s[4];
+    UVE p[4];                       // SSPRK2 / 4SSPRK3: the velocities of stage k after its momentum update (u, v)
     UVE pre;
     uint32_t post_u, post_v, um, vm, sst;
     UVE L, Z, Ma, Mb, A;
@@ -1096,7 +1098,9 @@ template <class R> struct AdjConsts {
     int lin;                        // the linear dynamics (rhs_linear!)
     int nadvcor;                    // nstep_advcor: q and KEu+KEv kept (0: formed by every stage)
     int ssp;                        // SSPRK2: the number of stages RKs (0: classic RK)
-    R cs, sa, sb;                   // SSPRK2: Δt_Δs, and the weights 1/RKs, (RKs-1)/RKs of u0 = a·u + b·u1
+    int s4;                         // 4SSPRK3
+    R cs, sa, sb;                   // SSPRK2: Δt_Δs, and the weights 1/RKs, (RKs-1)/RKs of u0 = a·u + b·u1;
+                                    // 4SSPRK3: ½·Δt_Δ, and the weights 2/3, 1/3 of the third stage's combination
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -1128,27 +1132,36 @@ template <class R, class K> int reverse_tracer(K &k, AdjFields &F, const AdjCons
 
 constexpr uint32_t NOF = 0xFFFFFFFFu;   // "no field" (lincomb operands left out)
 
-// The stages of an SSPRK2 step, last first (time_integration.jl:80-110).  Forward, stage k
-// from w_k = (u_k, v_k, η_k) (ghost copies applied):
-//   u_{k+1} = u_k + c·du(w_k), v likewise, then ghost_points_uv!;
-//   η_{k+1} = η_k + c·dη(u_{k+1}, v_{k+1}, η_k)       (c = Δt_Δs, η ghost copies at the next stage)
-// and u0 = a·u + b·u_{RKs+1} (a = 1/RKs, b = (RKs-1)/RKs), ghost_points! on u0.  Every
-// adjoint is carried with its halo unfolded (a halo cell holds the adjoint of the ghost copy,
-// folded into the cell it copies by its reader), so the stage bookkeeping is elementwise:
-// with Λ the adjoint of w_{k+1} and λ0 that of u0 (l0),
-//   Λ ← b·λ0;  for k = RKs .. 1:
-//     μ2 = (∂dη/∂(u_{k+1}, v_{k+1}, η_k))ᵀ c·Λη           (stage march, continuity form)
-//     Λu += μ2u, Λv += μ2v                                 (Λ of u_{k+1}, v_{k+1} complete)
+// The stages of an SSPRK2 or 4SSPRK3 step, last first.  Both move the velocities first and
+// then η with the mass fluxes of the updated velocities (the semi-implicit continuity):
+//   SSPRK2 (time_integration.jl:80-110), stage k from w_k = (u_k, v_k, η_k):
+//     u'_k = u_k + c·du(w_k), v likewise, ghost_points_uv!;  η'_k = η_k + c·dη(u'_k, v'_k, η_k)
+//     w_{k+1} = (u'_k, v'_k, η'_k) (η ghost copies at the next stage); u0 = a·u + b·w_{RKs+1}
+//     (c = Δt_Δs, a = 1/RKs, b = (RKs-1)/RKs).
+//   4SSPRK3 (:158-195): u'_k = ½(u_k + (u_k + Δt_Δ·du(w_k))), η'_k likewise, i.e. weight 1 on the
+//     input and c = ½·Δt_Δ on the tendency; w_{k+1} = (u'_k, v'_k, η'_k) except
+//     w_4 = ⅔·w_1 + ⅓·(u'_3, v'_3, η'_3); u0 = (u'_4, v'_4, η'_4).
+// ghost_points! on u0 follows.  Every adjoint is carried with its halo unfolded (a halo cell
+// holds the adjoint of the ghost copy, folded into the cell it copies by its reader), so the
+// stage bookkeeping is elementwise.  With Λ the adjoint of the stage variables and λ0 that of u0
+// (l0):
+//   Λ ← b·λ0 (SSPRK2) or λ0 (4SSPRK3);  for k = last .. 1:
+//     (4SSPRK3, k = 3: A ← ⅔·Λ, Λ ← ⅓·Λ)
+//     μ2 = (∂dη/∂(u'_k, v'_k, η_k))ᵀ c·Λη                  (stage march, continuity form)
+//     Λu += μ2u, Λv += μ2v                                 (Λ of u'_k, v'_k complete)
 //     μ1 = (∂(du, dv)/∂w_k)ᵀ c·(Λu, Λv)                    (stage march with λη = 0)
 //     Λu += μ1u, Λv += μ1v, Λη += μ2η + μ1η                (Λ of w_k)
-//   the step's input: Λ + a·λ0.
-// The result goes to `out`; F.L and F.Z are swapped as by the classic RK stages.
-template <class R, class K> int reverse_ssp2(K &k, AdjFields &F, const AdjConsts<R> &c, UVE l0, UVE out) {
+//   the step's input: Λ + a·λ0 (SSPRK2) or Λ + A (4SSPRK3).
+// The taped fields: F.s[k-1] = w_k, F.p[k-1] = (u'_k, v'_k).  The result goes to `out`; F.L
+// and F.Z are swapped as by the classic RK stages.
+template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<R> &c, UVE l0, UVE out) {
     int rc = 0;
+    const bool s4 = c.s4 != 0;
+    const int ns = s4 ? 4 : c.ssp;
     const UVE Lm = out, M2 = F.Ma, M1 = F.Mb;
-    const R one = R(1), nil = R(0);
-    if ((rc = k.lincomb(Lm.u, l0.u, c.sb, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.v, l0.v, c.sb, NOF, nil, NOF, nil)) ||
-        (rc = k.lincomb(Lm.e, l0.e, c.sb, NOF, nil, NOF, nil)))
+    const R one = R(1), nil = R(0), b = s4 ? one : c.sb;
+    if ((rc = k.lincomb(Lm.u, l0.u, b, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.v, l0.v, b, NOF, nil, NOF, nil)) ||
+        (rc = k.lincomb(Lm.e, l0.e, b, NOF, nil, NOF, nil)))
         return rc;
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;
@@ -1161,9 +1174,15 @@ template <class R, class K> int reverse_ssp2(K &k, AdjFields &F, const AdjConsts
     S.mu_in = S.a_in = S.a_out = F.A;               // (read and written by NEXT stages only)
     const int sch = (c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH) | (c.nadvcor > 0 && !c.lin ? ADJ_KEEP : 0);
     const bool wrap = c.per && (sch & 3) == ADJ_AH, keep = (sch & ADJ_KEEP) != 0;
-    for (int st = c.ssp; st >= 1; --st) {
-        // continuity at (u_{k+1}, v_{k+1}, η_k)
-        S.s = UVE{F.s[st].u, F.s[st].v, F.s[st - 1].e};
+    for (int st = ns; st >= 1; --st) {
+        if (s4 && st == 3) {                        // w_4 = ⅔·w_1 + ⅓·(u'_3, v'_3, η'_3)
+            if ((rc = k.lincomb(F.A.u, Lm.u, c.sa, NOF, nil, NOF, nil)) || (rc = k.lincomb(F.A.v, Lm.v, c.sa, NOF, nil, NOF, nil)) ||
+                (rc = k.lincomb(F.A.e, Lm.e, c.sa, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.u, Lm.u, c.sb, NOF, nil, NOF, nil)) ||
+                (rc = k.lincomb(Lm.v, Lm.v, c.sb, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.e, Lm.e, c.sb, NOF, nil, NOF, nil)))
+                return rc;
+        }
+        // continuity at (u'_k, v'_k, η_k)
+        S.s = UVE{F.p[st - 1].u, F.p[st - 1].v, F.s[st - 1].e};
         S.l0 = Lm; S.mu_out = M2; S.fxg = 0;
         if ((rc = k.stage(S, c.per, 0, 0, 0, ADJ_CONT))) return rc;
         if ((rc = k.lincomb(Lm.u, Lm.u, one, M2.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, M2.v, one, NOF, nil)))
@@ -1179,8 +1198,10 @@ template <class R, class K> int reverse_ssp2(K &k, AdjFields &F, const AdjConsts
             (rc = k.lincomb(Lm.e, Lm.e, one, M2.e, one, M1.e, one)))
             return rc;
     }
-    if ((rc = k.lincomb(Lm.u, Lm.u, one, l0.u, c.sa, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, l0.v, c.sa, NOF, nil)) ||
-        (rc = k.lincomb(Lm.e, Lm.e, one, l0.e, c.sa, NOF, nil)))
+    const UVE in = s4 ? F.A : l0;
+    const R w = s4 ? one : c.sa;
+    if ((rc = k.lincomb(Lm.u, Lm.u, one, in.u, w, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, in.v, w, NOF, nil)) ||
+        (rc = k.lincomb(Lm.e, Lm.e, one, in.e, w, NOF, nil)))
         return rc;
     F.L = out;
     F.Z = l0;
@@ -1216,7 +1237,7 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
             if (wq && (rc = k.zero(F.lqk))) return rc;
         }
     }
-    if (c.ssp > 0) return reverse_ssp2(k, F, c, l0, out);
+    if (c.ssp > 0 || c.s4) return reverse_ssp(k, F, c, l0, out);
     // the RK stages, last first (time_integration.jl:55-78)
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;

@@ -44,6 +44,10 @@ template <class F> struct Launch {
         ctx->Ld = rows_per_wave(ctx, reinterpret_cast<const void *>(&dissipation_kernel<F, PER, 0, true>), nstrips(ctx), 2, 8);
         ctx->Lf = rows_per_wave(ctx, reinterpret_cast<const void *>(&fused_kernel<F, M_RK_MID, M_RK_LAST, false>),
                                 nstrips2(ctx), 5, 8);
+        if (const char *e = getenv("SWM_ROWS_PER_WAVE_FUSED")) {     // rows per wave of the fused pairs, fixed
+            const int l = atoi(e);
+            if (l >= 8 && l <= 4096) ctx->Lf = l;
+        }
     }
 
     static StaticRefs statics(const swm_ctx *ctx) {
@@ -728,8 +732,8 @@ template <class Fm> struct Adj {
 
     static int check(swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        if (c.time_scheme != SWM_RK && c.time_scheme != SWM_SSPRK2)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK and the SSPRK2 schemes only");
+        if (c.time_scheme == SWM_SSPRK3)
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK, SSPRK2 and 4SSPRK3 schemes only");
         if (c.time_scheme == SWM_SSPRK2 && (c.RKs < 2 || c.RKs > 3))   // the taped step keeps RKs+3 velocity slots of 6
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint of the SSPRK2 scheme supports RKs = 2 or 3 stages");
         if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
@@ -747,17 +751,23 @@ template <class Fm> struct Adj {
         return SWM_OK;
     }
 
-    // carve the adjoint's fields: two more slot sets and the adjoint fields (28), zeroed
+    // slot sets the taped step keeps: Prog, u0, the dissipation output and the stage inputs
+    // (classic RK: s_2..s_RKo; SSPRK2: w_2..w_RKs+1; 4SSPRK3: w_2, w_3, (u'_3, v'_3), w_4, so 7)
+    static int adj_slots(const swm_config &c) { return c.time_scheme == SWM_4SSPRK3 ? 7 : 6; }
+
+    // carve the adjoint's fields: the taped step's slot sets beyond the forward's four and the
+    // adjoint fields (28), zeroed
     static int init(swm_ctx *ctx) {
         if (ctx->adj) return SWM_OK;
         int rc = check(ctx);
         if (rc) return rc;
-        const int nf2 = 6 + 28;
+        const int nsl = adj_slots(ctx->c);                // slot sets of the taped step
+        const int nf2 = 3 * (nsl - 4) + 28;
         const size_t n1d = (size_t)ctx->c.ny + 2 * OY + 16;
         if ((rc = grow_arena(ctx, (size_t)nf2 * (ctx->elems * ctx->es + 256) + n1d * ctx->es + 256))) return rc;
         AdjState *st = new AdjState();
         const int es = ctx->es;
-        for (int s = 4; s < 6; ++s)
+        for (int s = 4; s < nsl; ++s)
             if ((rc = alloc_field(ctx, ctx->u[s], 0, true, es)) || (rc = alloc_field(ctx, ctx->v[s], 0, true, es)) ||
                 (rc = alloc_field(ctx, ctx->e[s], 0, true, es))) {
                 delete st;
@@ -797,7 +807,7 @@ template <class Fm> struct Adj {
             if (l >= 4 && l <= 4096) st->La = l;
         }
         ctx->adj = st;
-        ctx->nslots = 6;
+        ctx->nslots = nsl;
         return SWM_OK;
     }
 
@@ -819,6 +829,11 @@ template <class Fm> struct Adj {
             k.cs = (R)c.Dt_Ds;
             k.sa = (R)(1.0 / c.RKs);
             k.sb = (R)((double)(c.RKs - 1) / c.RKs);
+        } else if (c.time_scheme == SWM_4SSPRK3) {       // u' = ½(u + (u + Δt_Δ·du)): ½·Δt_Δ on du (exact)
+            k.s4 = 1;
+            k.cs = (R)(0.5 * (double)(R)c.Dt_D);
+            k.sa = (R)(2.0 / 3.0);
+            k.sb = (R)(1.0 / 3.0);
         }
         k.g = (R)c.g; k.gamma = (R)c.gamma; k.k12 = (R)round_host(k.sad ? 0.5 : 1.0 / 12.0, c.T); k.oma = (R)c.one_minus_alpha;
         k.cD = (R)c.cD; k.rD = (R)c.rD; k.cS = (R)c.cSmag; k.nuB = (R)c.nuB; k.dtd = (R)c.Dt_diff;
@@ -850,7 +865,7 @@ template <class Fm> struct Adj {
             else launch_stage<PER, true, false, false, SCH>(S);
         }
         template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
-            if (sch == adj::ADJ_CONT) launch_stage<PER, false, false, false, adj::ADJ_CONT>(S);   // (SSPRK2: no successor)
+            if (sch == adj::ADJ_CONT) launch_stage<PER, false, false, false, adj::ADJ_CONT>(S);   // (no successor)
             else if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
             else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, al0, pre);
             else if (sch == (adj::ADJ_AH | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_AH | adj::ADJ_KEEP>(S, next, al0, pre);
@@ -984,23 +999,25 @@ template <class Fm> struct Adj {
     }
 
     // The taped step k: step i = i0 + k from Prog without copy-back, every intermediate kept
-    // in its own slot (the stage inputs s_1..s_RKo, or w_1..w_RKs+1 under SSPRK2; u0 before /
+    // in its own slot (the stage inputs s_1..s_RKo, or w_1..w_RKs+1 under SSPRK2, or under
+    // 4SSPRK3 w_1..w_4 and each stage's velocities after its momentum update; u0 before /
     // after the dissipation).
     static int tape_step(swm_ctx *ctx, int64_t k, adj::AdjFields &F) {
         AdjState *st = ctx->adj;
         const swm_config &c = ctx->c;
         const int64_t i = st->i0 + k, t = st->t0 + k * c.dtint;
         const typename LF::Times T = LF::forcing_times(ctx, t);
-        int uv[6], ee[6], nu = 0, ne = 0;
+        int uv[8], ee[8], nu = 0, ne = 0;
         uv[nu++] = ctx->pu;
         ee[ne++] = ctx->pe;
-        for (int s = 0; s < 6; ++s) {
+        for (int s = 0; s < ctx->nslots; ++s) {
             if (s != ctx->pu) uv[nu++] = s;
             if (s != ctx->pe) ee[ne++] = s;
         }
         // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo (SSPRK2: [2..RKs+1] w_2..w_RKs+1),
-        // [5] the dissipation output
-        const Slots P{uv[0], ee[0]}, a{uv[1], ee[1]}, none{};
+        // [5] the dissipation output; 4SSPRK3: see below
+        const bool s4 = c.time_scheme == SWM_4SSPRK3;
+        const Slots P{uv[0], ee[0]}, a = s4 ? Slots{uv[5], ee[4]} : Slots{uv[1], ee[1]}, none{};
         auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
         int rc = SWM_OK;
         auto coef = [&](int q) {
@@ -1035,7 +1052,25 @@ template <class Fm> struct Adj {
                 if (!rc) rc = LF::continuity(ctx, m, Slots{n.uv, sk(q).e}, P, none, n, last ? a : none, kk, T);
             }
         }
-        for (int q = 1; !ssp && q <= c.RKo && rc == SWM_OK; ++q) {
+        if (s4) {                                            // :160-195
+            // w_1 = P; u'_1 = w_2 in uv[1], η_2 in ee[1]; u'_2 = w_3 in uv[2], η_3 in ee[2];
+            // u'_3 in uv[3], w_4 in (uv[4], ee[3]); u0 = (u'_4, η'_4) in (uv[5], ee[4])
+            Coef<Q> kk{};
+            kk.c = (Q)c.Dt_D;
+            kk.h = Q(0.5);
+            kk.x = (Q)(2.0 / 3.0);
+            kk.y = (Q)(1.0 / 3.0);
+            const Slots w[5] = {P, Slots{uv[1], ee[1]}, Slots{uv[2], ee[2]}, Slots{uv[4], ee[3]}, a};
+            const int up[4] = {uv[1], uv[2], uv[3], uv[5]};
+            for (int q = 1; q <= 4 && rc == SWM_OK; ++q) {
+                const int m = q == 3 ? M_S4_3 : M_S4;
+                rc = LF::stage(ctx, m, w[q - 1], P, none, Slots{up[q - 1], -1}, Slots{q == 3 ? uv[4] : -1, -1}, kk, T);
+                if (!rc) rc = LF::continuity(ctx, m, Slots{up[q - 1], w[q - 1].e}, P, none, Slots{-1, w[q].e}, none, kk, T);
+                F.s[q - 1] = adj::UVE{ctx->u[w[q - 1].uv].ref(), ctx->v[w[q - 1].uv].ref(), ctx->e[w[q - 1].e].ref()};
+                F.p[q - 1] = adj::UVE{ctx->u[up[q - 1]].ref(), ctx->v[up[q - 1]].ref(), 0u};
+            }
+        }
+        for (int q = 1; !ssp && !s4 && q <= c.RKo && rc == SWM_OK; ++q) {
             // stages q, q+1 as one fused pair that also keeps the input of stage q+1
             if (q + 1 <= c.RKo && LF::can_tape(ctx, mode(q), mode(q + 1))) {
                 rc = LF::fused_tape(ctx, mode(q), mode(q + 1), sk(q), P, q == 1 ? none : a, q + 1 < c.RKo ? sk(q + 2) : none,
@@ -1053,14 +1088,15 @@ template <class Fm> struct Adj {
         }
         int post = a.uv;
         if (i % c.nstep_diff == 0) {
-            post = uv[5];
+            post = s4 ? uv[6] : uv[5];
             if ((rc = LF::dissipation(ctx, a, post))) return rc;
         }
         const int sst_old = ctx->rsst;
         if ((rc = LF::tracer(ctx, i, post))) return rc;
-        for (int q = 1; q <= (ssp ? c.RKs + 1 : c.RKo); ++q) {
+        for (int q = 1; !s4 && q <= (ssp ? c.RKs + 1 : c.RKo); ++q) {
             const Slots s = sk(q);
             F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
+            if (ssp && q >= 2) F.p[q - 2] = adj::UVE{F.s[q - 1].u, F.s[q - 1].v, 0u};    // u'_k = u_{k+1}
         }
         F.pre = adj::UVE{ctx->u[a.uv].ref(), ctx->v[a.uv].ref(), ctx->e[a.e].ref()};
         F.post_u = ctx->u[post].ref();

@@ -21,7 +21,7 @@ _lib = None
 class EmuCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("nx", "ny", "per", "RKo", "cont", "drag", "smag", "consume", "tracer", "nadvstep",
-                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin", "ssp")] + \
+                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin", "ssp", "s4")] + \
                [("a", ctypes.c_double * 4), ("b", ctypes.c_double * 4)] + \
                [(n, ctypes.c_double) for n in ("cs", "sa", "sb")] + \
                [(n, ctypes.c_double) for n in
@@ -71,6 +71,11 @@ def config(S, cfg, L=8, nan_edges=False, fmt=8):
         k.cs = round_scalar(cfg.Dt_Ds, P.Tprog)
         k.sa = round_scalar(1.0 / cfg.RKs, P.Tprog)
         k.sb = round_scalar((cfg.RKs - 1) / cfg.RKs, P.Tprog)
+    elif cfg.time_scheme == 3:                             # 4SSPRK3: ½·Δt_Δ on the tendency, thirds of stage 3
+        k.s4 = 1
+        k.cs = 0.5 * round_scalar(cfg.Dt_D, P.Tprog)
+        k.sa = round_scalar(2.0 / 3.0, P.Tprog)
+        k.sb = round_scalar(1.0 / 3.0, P.Tprog)
     k.k12 = round_scalar(0.5 if k.sad else 1.0 / 12.0, P.T)
     k.cD, k.rD, k.cS, k.nuB, k.dtd = cfg.cD, cfg.rD, cfg.cSmag, cfg.nuB, cfg.Dt_diff
     k.hdu, k.hdv, k.du, k.dv, k.jSST = cfg.half_dtadvu, cfg.half_dtadvv, cfg.dtadvu, cfg.dtadvv, cfg.jSST

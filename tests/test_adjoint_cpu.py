@@ -235,7 +235,8 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
     L = list(lam) + [np.zeros_like(u), np.zeros_like(v)]
     for i in range(n, 0, -1):
         x0, tp = tapes[i - 1]
-        ins = [a for sk in tp["s"] for a in sk] + list(tp["pre"]) + list(tp["post"]) + list(tp["mid"]) + \
+        ins = [a for sk in tp["s"] for a in sk] + [a for pk in tp["p"] for a in pk] + list(tp["pre"]) + \
+              list(tp["post"]) + list(tp["mid"]) + \
               [x0[3], S.forcing.H, S.grid.f_q] + L + linear_statics(S)
         L = adjemu.run(S, k, 4, ins, i=i)
     return adjemu.run(S, k, 5, L[:4]), [u, v, e, s], st
@@ -252,7 +253,11 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
                                    ("periodic", {"time_scheme": "SSPRK2", "adv_scheme": "Sadourny",
                                                  "surface_relax": True}),
                                    ("nonperiodic", {"time_scheme": "SSPRK2", "dynamics": "linear", "RKs": 2,
-                                                    "cfl": 0.5, "Uadv": 40.0})])
+                                                    "cfl": 0.5, "Uadv": 40.0}),
+                                   ("nonperiodic", {"time_scheme": "4SSPRK3"}),
+                                   ("periodic", {"time_scheme": "4SSPRK3", "adv_scheme": "Sadourny"}),
+                                   ("periodic", {"time_scheme": "4SSPRK3", "dynamics": "linear",
+                                                 "surface_relax": True})])
 def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
     """Three steps: the tracer's snapshot (step 2) and advection (step 3) included."""
     P, S, cfg, prog = setup(swm, bc, **kw)
@@ -270,7 +275,8 @@ def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
 
 
 @pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"dynamics": "linear"}),
-                                   ("periodic", {"time_scheme": "SSPRK2"})])
+                                   ("periodic", {"time_scheme": "SSPRK2"}),
+                                   ("nonperiodic", {"time_scheme": "4SSPRK3"})])
 def test_dot_product_identity_across_wave_strips(swm, orc, bc, kw):
     """A grid of several wave strips and short segments: <J dx, y> = <dx, J^T y>."""
     P, S, cfg, prog = setup(swm, bc, nx=130, L_ratio=0.5, **kw)
