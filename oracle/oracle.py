@@ -248,16 +248,16 @@ def tlm(S, cfg, op, x, dx=None, nsteps=1, i0=1, t0=0, dFx=None):
 
 
 def tape(S, cfg, i, t, u, v, e, sst, um, vm):
-    """Intermediate states of one Float64 step (classic RK, SSPRK2 or 4SSPRK3) from the given
-    state: returns a dict with 's' (list of the RK stage inputs (u1, v1, η1); under SSPRK2 the
-    RKs stage inputs and (u1, v1, η1) after the last stage), 'p' (4SSPRK3: list of every
+    """Intermediate states of one Float64 step (any scheme) from the given state: returns a
+    dict with 's' (list of the RK stage inputs (u1, v1, η1); under SSPRK2 / SSPRK3 the stage
+    inputs and (u1, v1, η1) after the last stage), 'p' (4SSPRK3: list of every
     stage's (u1, v1) after its momentum update and ghost copies; else empty), 'pre' (u0, v0,
     η0 after the stages' ghost copies), 'post' (u0, v0 after the dissipation), 'mid' (um, vm
     after the step) and 'new' (u, v, η, sst)."""
     G = S.grid
     su, sv, se, ss = (G.nux + 4, G.nuy + 4), (G.nvx + 4, G.nvy + 4), (G.nx + 2, G.ny + 2), (G.nx + 2, G.ny)
     P = S.parameters
-    RKo = {"SSPRK2": P.RKs + 1, "4SSPRK3": 4}.get(P.time_scheme, P.RKo)
+    RKo = {"SSPRK2": P.RKs + 1, "SSPRK3": P.RKn ** 2 + 1, "4SSPRK3": 4}.get(P.time_scheme, P.RKo)
     npair = 4 if P.time_scheme == "4SSPRK3" else 0
     shapes = [su, sv, se] * RKo + [su, sv] * npair + [su, sv, se] + [su, sv] + [su, sv] + [su, sv, se, ss]
     out = [np.zeros(s, order="F") for s in shapes]

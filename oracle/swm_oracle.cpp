@@ -1057,6 +1057,7 @@ template <class T, class Tp, class Tc> struct Model {
             const Tp dtn = Tp(c.ssp_Dt_Dn), dtnc = Tp(c.ssp_Dt_Dnc);
             for (int rki = 2; rki <= c.ssp_s + 1; ++rki) {
                 if (rki > 2) ghost_points_eta(eta1);
+                if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
                 const Arr<T> &e = rhs_view(eta1, etar);
                 rhs(rhs_view(u1, ur), rhs_view(v1, vr), e, t);
                 const bool m8 = g_mutation == MUT_SSPRK3_KN;
@@ -1081,6 +1082,7 @@ template <class T, class Tp, class Tc> struct Model {
                 ++stages_run;
                 if (stop()) return false;
             }
+            if (tape) { tape->s.push_back(u1); tape->s.push_back(v1); tape->s.push_back(eta1); }
         } else {                                               // 4SSPRK3 :158-195
             const Tp dt = Tp(c.Dt_D);
             for (int rki = 1; rki <= 4; ++rki) {
@@ -1484,16 +1486,15 @@ int swmo_tlm(const swm_config *c, const double *H, const double *Fx, const doubl
 // ghost copies; um, vm the tracer's mid-point velocities), T = Tprog = Tcomm = Float64 or
 // Float32 (c->T; arrays are float64 holding values of that format), returning its
 // intermediate states: out[3k..3k+2] = the input (u1, v1, η1) of RK stage k+1 (k < RKo) --
-// under SSPRK2 of stage k+1 for k < RKs, then u1, v1, η1 after the last stage (k = RKs);
+// under SSPRK2 (SSPRK3) of stage k+1 for k < RKs (s), then u1, v1, η1 after the last stage;
 // under 4SSPRK3 of stage k+1 for k < 4, followed by u1, v1 of each of the 4 stages after its
 // ghost_points_uv! (:178) -- then u0, v0, η0 after ghost_points! (:198), then u0, v0 after
-// the dissipation, then um, vm after the step, then the new u, v, η, sst.  Classic RK, SSPRK2
-// and 4SSPRK3.
+// the dissipation, then um, vm after the step, then the new u, v, η, sst.  Every scheme.
 int swmo_tape(const swm_config *c, const double *H, const double *Fx, const double *Fy, const double *f_q,
               const double *eta_ref, const double *Feta, const double *f_u, const double *f_v, int64_t i, int64_t t,
               const double *u, const double *v, const double *eta, const double *sst, const double *um,
               const double *vm, double *const *out) {
-    if (c->time_scheme == SWM_SSPRK3 || c->T != c->Tprog || c->Tcomm != c->Tprog ||
+    if (c->T != c->Tprog || c->Tcomm != c->Tprog ||
         c->scale_log2 != 0 || c->compensated)
         return -1;
     if (c->dynamics == SWM_LINEAR && (!f_u || !f_v)) return -1;

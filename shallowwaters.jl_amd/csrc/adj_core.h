@@ -1,8 +1,7 @@
-// adj_core.h -- reverse-mode (adjoint) kernels of one Runge-Kutta time step (classic RK,
-// SSPRK2 or 4SSPRK3).
+// adj_core.h -- reverse-mode (adjoint) kernels of one Runge-Kutta time step (any of the
+// reference's schemes: classic RK, SSPRK2, SSPRK3, 4SSPRK3).
 //
-// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (classic RK, SSPRK2 or 4SSPRK3,
-// T = Tprog) is
+// The time step x_{n+1} = F(x_n) of time_integration.jl:45-243 (T = Tprog) is
 // differentiated by hand, operator by operator, in gather form: every adjoint value is a
 // sum over the forward quantities that read the corresponding input, so each output cell
 // is written by one lane and no atomics are needed; results do not depend on the launch
@@ -58,7 +57,7 @@ constexpr int ADJ_Y0 = -1;
 // the right-hand side forms of the stage adjoint (template argument SCH): the nonlinear
 // dynamics with Arakawa-Hsu or Sadourny PV fluxes, the linear dynamics; ADJ_KEEP (with a
 // nonlinear form): q and KEu+KEv kept from an earlier state (nstep_advcor > 0); ADJ_CONT
-// (alone): the semi-implicit continuity of the SSPRK2 / 4SSPRK3 schemes only, the mass fluxes
+// (alone): the semi-implicit continuity of the SSP schemes only, the mass fluxes
 // recomputed from the state (UVfluxes!, continuity.jl:83-85) whatever the dynamics
 enum { ADJ_AH = 0, ADJ_SAD = 1, ADJ_LIN = 2, ADJ_KEEP = 4, ADJ_CONT = 8 };
 SWM_HD int adj_y1(int ny) { return ny + 3; }
@@ -1066,8 +1065,8 @@ template <class R, class C> SWM_HD void fold_out_cell(const C &c, const FoldOutA
 // Ma / Mb / A the stage adjoint's μ ping-pong and running sum.  The tracer adjoint's
 // records reuse Z, Ma, Mb, A and two more fields.
 struct AdjFields {
-    UVE s[4];
-    UVE p[4];                       // SSPRK2 / 4SSPRK3: the velocities of stage k after its momentum update (u, v)
+    UVE s[12];                      // stage inputs (classic RK: 4 at most; SSPRK3 with n = 3: 10)
+    UVE p[12];                      // SSP schemes: the velocities of stage k after its momentum update (u, v)
     UVE pre;
     uint32_t post_u, post_v, um, vm, sst;
     UVE L, Z, Ma, Mb, A;
@@ -1099,8 +1098,12 @@ template <class R> struct AdjConsts {
     int nadvcor;                    // nstep_advcor: q and KEu+KEv kept (0: formed by every stage)
     int ssp;                        // SSPRK2: the number of stages RKs (0: classic RK)
     int s4;                         // 4SSPRK3
+    int s3, kn, mn;                 // SSPRK3: its s stages; the kn-th (counted from 1) combines with the
+                                    // state after the mn-th (mn = 0: the step's input)
     R cs, sa, sb;                   // SSPRK2: Δt_Δs, and the weights 1/RKs, (RKs-1)/RKs of u0 = a·u + b·u1;
-                                    // 4SSPRK3: ½·Δt_Δ, and the weights 2/3, 1/3 of the third stage's combination
+                                    // 4SSPRK3: ½·Δt_Δ, and the weights 2/3, 1/3 of the third stage's combination;
+                                    // SSPRK3: Δt_Δn, and kna, knb of the kn-th stage
+    R cs2;                          // SSPRK3: Δt_Δnc, the kn-th stage's factor of its tendency
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
@@ -1141,25 +1144,30 @@ constexpr uint32_t NOF = 0xFFFFFFFFu;   // "no field" (lincomb operands left out
 //   4SSPRK3 (:158-195): u'_k = ½(u_k + (u_k + Δt_Δ·du(w_k))), η'_k likewise, i.e. weight 1 on the
 //     input and c = ½·Δt_Δ on the tendency; w_{k+1} = (u'_k, v'_k, η'_k) except
 //     w_4 = ⅔·w_1 + ⅓·(u'_3, v'_3, η'_3); u0 = (u'_4, v'_4, η'_4).
+//   SSPRK3 (:112-156), s stages: weight 1 and c = Δt_Δn, except the kn-th stage, which has
+//     weight kna on its input, c = Δt_Δnc, and adds knb times the state X kept after the mn-th
+//     stage (the step's input when mn = 0); w_{k+1} = (u'_k, v'_k, η'_k); u0 = w_{s+1}.
 // ghost_points! on u0 follows.  Every adjoint is carried with its halo unfolded (a halo cell
 // holds the adjoint of the ghost copy, folded into the cell it copies by its reader), so the
 // stage bookkeeping is elementwise.  With Λ the adjoint of the stage variables and λ0 that of u0
-// (l0):
-//   Λ ← b·λ0 (SSPRK2) or λ0 (4SSPRK3);  for k = last .. 1:
+// (l0), and (id_k, c_k) a stage's weights:
+//   Λ ← b·λ0 (SSPRK2) or λ0;  for k = last .. 1:
 //     (4SSPRK3, k = 3: A ← ⅔·Λ, Λ ← ⅓·Λ)
-//     μ2 = (∂dη/∂(u'_k, v'_k, η_k))ᵀ c·Λη                  (stage march, continuity form)
+//     μ2 = (∂dη/∂(u'_k, v'_k, η_k))ᵀ c_k·Λη                (stage march, continuity form)
 //     Λu += μ2u, Λv += μ2v                                 (Λ of u'_k, v'_k complete)
-//     μ1 = (∂(du, dv)/∂w_k)ᵀ c·(Λu, Λv)                    (stage march with λη = 0)
-//     Λu += μ1u, Λv += μ1v, Λη += μ2η + μ1η                (Λ of w_k)
-//   the step's input: Λ + a·λ0 (SSPRK2) or Λ + A (4SSPRK3).
+//     (SSPRK3, k = kn: B ← knb·Λ, the adjoint of X)
+//     μ1 = (∂(du, dv)/∂w_k)ᵀ c_k·(Λu, Λv)                  (stage march with λη = 0)
+//     Λu ← id_k·Λu + μ1u, Λv likewise, Λη ← id_k·Λη + μ2η + μ1η   (Λ of w_k)
+//     (SSPRK3, k = mn + 1: Λ += B, as w_k is X)
+//   the step's input: Λ + a·λ0 (SSPRK2), Λ + A (4SSPRK3) or Λ (SSPRK3).
 // The taped fields: F.s[k-1] = w_k, F.p[k-1] = (u'_k, v'_k).  The result goes to `out`; F.L
 // and F.Z are swapped as by the classic RK stages.
 template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<R> &c, UVE l0, UVE out) {
     int rc = 0;
-    const bool s4 = c.s4 != 0;
-    const int ns = s4 ? 4 : c.ssp;
+    const bool s4 = c.s4 != 0, s3 = c.s3 > 0;
+    const int ns = s4 ? 4 : s3 ? c.s3 : c.ssp;
     const UVE Lm = out, M2 = F.Ma, M1 = F.Mb;
-    const R one = R(1), nil = R(0), b = s4 ? one : c.sb;
+    const R one = R(1), nil = R(0), b = c.ssp > 0 ? c.sb : one;
     if ((rc = k.lincomb(Lm.u, l0.u, b, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.v, l0.v, b, NOF, nil, NOF, nil)) ||
         (rc = k.lincomb(Lm.e, l0.e, b, NOF, nil, NOF, nil)))
         return rc;
@@ -1170,11 +1178,14 @@ template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<
     S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
     S.lfx = F.lfx; S.fxt = c.fxt;
-    S.a = c.cs; S.b = nil;
+    S.b = nil;
     S.mu_in = S.a_in = S.a_out = F.A;               // (read and written by NEXT stages only)
     const int sch = (c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH) | (c.nadvcor > 0 && !c.lin ? ADJ_KEEP : 0);
     const bool wrap = c.per && (sch & 3) == ADJ_AH, keep = (sch & ADJ_KEEP) != 0;
     for (int st = ns; st >= 1; --st) {
+        const bool kn = s3 && st == c.kn;
+        const R id = kn ? c.sa : one;
+        S.a = kn ? c.cs2 : c.cs;
         if (s4 && st == 3) {                        // w_4 = ⅔·w_1 + ⅓·(u'_3, v'_3, η'_3)
             if ((rc = k.lincomb(F.A.u, Lm.u, c.sa, NOF, nil, NOF, nil)) || (rc = k.lincomb(F.A.v, Lm.v, c.sa, NOF, nil, NOF, nil)) ||
                 (rc = k.lincomb(F.A.e, Lm.e, c.sa, NOF, nil, NOF, nil)) || (rc = k.lincomb(Lm.u, Lm.u, c.sb, NOF, nil, NOF, nil)) ||
@@ -1187,6 +1198,9 @@ template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<
         if ((rc = k.stage(S, c.per, 0, 0, 0, ADJ_CONT))) return rc;
         if ((rc = k.lincomb(Lm.u, Lm.u, one, M2.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, M2.v, one, NOF, nil)))
             return rc;
+        if (kn && ((rc = k.lincomb(F.A.u, Lm.u, c.sb, NOF, nil, NOF, nil)) ||
+                   (rc = k.lincomb(F.A.v, Lm.v, c.sb, NOF, nil, NOF, nil)) || (rc = k.lincomb(F.A.e, Lm.e, c.sb, NOF, nil, NOF, nil))))
+            return rc;
         // momentum at w_k
         S.s = F.s[st - 1];
         S.l0 = UVE{Lm.u, Lm.v, F.zero}; S.mu_out = M1; S.fxg = c.fxg;
@@ -1194,15 +1208,21 @@ template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<
         if ((rc = k.stage(S, c.per, 0, 0, 0, sch))) return rc;
         if (wrap && !keep && (rc = k.qend_adj(S, M1))) return rc;
         if (wrap && keep && (rc = k.lqend_keep(S))) return rc;
-        if ((rc = k.lincomb(Lm.u, Lm.u, one, M1.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, M1.v, one, NOF, nil)) ||
-            (rc = k.lincomb(Lm.e, Lm.e, one, M2.e, one, M1.e, one)))
+        if ((rc = k.lincomb(Lm.u, Lm.u, id, M1.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, id, M1.v, one, NOF, nil)) ||
+            (rc = k.lincomb(Lm.e, Lm.e, id, M2.e, one, M1.e, one)))
+            return rc;
+        if (s3 && st == c.mn + 1 &&
+            ((rc = k.lincomb(Lm.u, Lm.u, one, F.A.u, one, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, F.A.v, one, NOF, nil)) ||
+             (rc = k.lincomb(Lm.e, Lm.e, one, F.A.e, one, NOF, nil))))
             return rc;
     }
-    const UVE in = s4 ? F.A : l0;
-    const R w = s4 ? one : c.sa;
-    if ((rc = k.lincomb(Lm.u, Lm.u, one, in.u, w, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, in.v, w, NOF, nil)) ||
-        (rc = k.lincomb(Lm.e, Lm.e, one, in.e, w, NOF, nil)))
-        return rc;
+    if (!s3) {
+        const UVE in = s4 ? F.A : l0;
+        const R w = s4 ? one : c.sa;
+        if ((rc = k.lincomb(Lm.u, Lm.u, one, in.u, w, NOF, nil)) || (rc = k.lincomb(Lm.v, Lm.v, one, in.v, w, NOF, nil)) ||
+            (rc = k.lincomb(Lm.e, Lm.e, one, in.e, w, NOF, nil)))
+            return rc;
+    }
     F.L = out;
     F.Z = l0;
     return 0;
@@ -1237,7 +1257,7 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
             if (wq && (rc = k.zero(F.lqk))) return rc;
         }
     }
-    if (c.ssp > 0 || c.s4) return reverse_ssp(k, F, c, l0, out);
+    if (c.ssp > 0 || c.s4 || c.s3 > 0) return reverse_ssp(k, F, c, l0, out);
     // the RK stages, last first (time_integration.jl:55-78)
     StageAdjArgs<R> S{};
     S.H = F.H; S.fq = F.fq; S.fq_2d = F.fq_2d; S.qend = F.qend; S.lqend = F.lqend;

@@ -141,7 +141,7 @@ struct swm_ctx {
     int L = 64;            // rows per wave segment of the stage kernels
     int Ld = 64;           // rows per wave segment of the dissipation kernel
     int Lf = 64;           // rows per wave segment of the fused stage pairs
-    Field u[8], v[8], e[8], sst[2];                          // Tprog (slots 4..7: the adjoint's taped step)
+    Field u[12], v[12], e[12], sst[2];                       // Tprog (slots 4..11: the adjoint's taped step)
     Field tu[2], tv[2], te[2];                               // T: u1 between RK stages (mixed formats)
     Field um, vm, H, Fx, Fy, fq, eta_ref, Feta, qend;         // T
     Field f_u, f_v;                                          // T: f on the u-, v-grid (linear dynamics)
@@ -158,7 +158,7 @@ struct swm_ctx {
     int *dflags = nullptr; // [0] NaN flag, [1] tracer index error
     double *dpart = nullptr;
     bool have_static = false, have_prog = false;
-    int nslots = 4;                // Tprog slot sets in use (6, or 7 under 4SSPRK3, once the adjoint is set up)
+    int nslots = 4;                // Tprog slot sets in use (6 to 11 once the adjoint is set up)
     int ens_refs = 0;              // ensembles holding this context (their plans point into the arena)
     struct AdjState *adj = nullptr;
     bool adj_fx = false;           // pullbacks accumulate the cotangent of Fx (swm_adj_track_Fx)
@@ -345,7 +345,7 @@ namespace swmh {
 // the fields of a context, for re-basing after the arena moves
 inline std::vector<Field *> all_fields(swm_ctx *ctx) {
     std::vector<Field *> f;
-    for (int s = 0; s < 8; ++s) { f.push_back(&ctx->u[s]); f.push_back(&ctx->v[s]); f.push_back(&ctx->e[s]); }
+    for (int s = 0; s < 12; ++s) { f.push_back(&ctx->u[s]); f.push_back(&ctx->v[s]); f.push_back(&ctx->e[s]); }
     for (int s = 0; s < 2; ++s) {
         f.push_back(&ctx->sst[s]); f.push_back(&ctx->tu[s]); f.push_back(&ctx->tv[s]); f.push_back(&ctx->te[s]);
     }

@@ -732,8 +732,8 @@ template <class Fm> struct Adj {
 
     static int check(swm_ctx *ctx) {
         const swm_config &c = ctx->c;
-        if (c.time_scheme == SWM_SSPRK3)
-            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint supports the classic RK, SSPRK2 and 4SSPRK3 schemes only");
+        if (c.time_scheme == SWM_SSPRK3 && c.ssp_s + 2 > 12)            // the taped step keeps s+2 slot sets of 12
+            return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint of the SSPRK3 scheme supports RKn = 2 or 3");
         if (c.time_scheme == SWM_SSPRK2 && (c.RKs < 2 || c.RKs > 3))   // the taped step keeps RKs+3 velocity slots of 6
             return ctx->fail(SWM_E_UNSUPPORTED, "the adjoint of the SSPRK2 scheme supports RKs = 2 or 3 stages");
         if (c.T != c.Tprog || c.Tcomm != c.Tprog || (c.T != SWM_F64 && c.T != SWM_F32))
@@ -752,8 +752,11 @@ template <class Fm> struct Adj {
     }
 
     // slot sets the taped step keeps: Prog, u0, the dissipation output and the stage inputs
-    // (classic RK: s_2..s_RKo; SSPRK2: w_2..w_RKs+1; 4SSPRK3: w_2, w_3, (u'_3, v'_3), w_4, so 7)
-    static int adj_slots(const swm_config &c) { return c.time_scheme == SWM_4SSPRK3 ? 7 : 6; }
+    // (classic RK: s_2..s_RKo; SSPRK2: w_2..w_RKs+1; 4SSPRK3: w_2, w_3, (u'_3, v'_3), w_4, so 7;
+    // SSPRK3: w_2..w_s+1 with u0 = w_s+1, so s+2)
+    static int adj_slots(const swm_config &c) {
+        return c.time_scheme == SWM_4SSPRK3 ? 7 : c.time_scheme == SWM_SSPRK3 ? std::max(6, c.ssp_s + 2) : 6;
+    }
 
     // carve the adjoint's fields: the taped step's slot sets beyond the forward's four and the
     // adjoint fields (28), zeroed
@@ -829,6 +832,14 @@ template <class Fm> struct Adj {
             k.cs = (R)c.Dt_Ds;
             k.sa = (R)(1.0 / c.RKs);
             k.sb = (R)((double)(c.RKs - 1) / c.RKs);
+        } else if (c.time_scheme == SWM_SSPRK3) {        // stages counted from 1 (the reference's rki = 2 .. s+1)
+            k.s3 = c.ssp_s;
+            k.kn = c.ssp_kn - 1;
+            k.mn = c.ssp_mn - 1;
+            k.cs = (R)c.ssp_Dt_Dn;
+            k.cs2 = (R)c.ssp_Dt_Dnc;
+            k.sa = (R)c.ssp_kna;
+            k.sb = (R)c.ssp_knb;
         } else if (c.time_scheme == SWM_4SSPRK3) {       // u' = ½(u + (u + Δt_Δ·du)): ½·Δt_Δ on du (exact)
             k.s4 = 1;
             k.cs = (R)(0.5 * (double)(R)c.Dt_D);
@@ -1007,7 +1018,7 @@ template <class Fm> struct Adj {
         const swm_config &c = ctx->c;
         const int64_t i = st->i0 + k, t = st->t0 + k * c.dtint;
         const typename LF::Times T = LF::forcing_times(ctx, t);
-        int uv[8], ee[8], nu = 0, ne = 0;
+        int uv[12], ee[12], nu = 0, ne = 0;
         uv[nu++] = ctx->pu;
         ee[ne++] = ctx->pe;
         for (int s = 0; s < ctx->nslots; ++s) {
@@ -1016,9 +1027,10 @@ template <class Fm> struct Adj {
         }
         // slots: [0] Prog = s_1, [1] u0, [2..RKo] s_2..s_RKo (SSPRK2: [2..RKs+1] w_2..w_RKs+1),
         // [5] the dissipation output; 4SSPRK3: see below
-        const bool s4 = c.time_scheme == SWM_4SSPRK3;
-        const Slots P{uv[0], ee[0]}, a = s4 ? Slots{uv[5], ee[4]} : Slots{uv[1], ee[1]}, none{};
-        auto sk = [&](int q) { return q == 1 ? P : Slots{uv[q], ee[q]}; };     // input of stage q (1-based)
+        const bool s4 = c.time_scheme == SWM_4SSPRK3, s3 = c.time_scheme == SWM_SSPRK3;
+        auto sk = [&](int q) { return q == 1 ? Slots{uv[0], ee[0]} : Slots{uv[q], ee[q]}; };   // input of stage q (1-based)
+        // SSPRK3: w_q in sk(q), q = 1 .. s+1 (slots 0, 2 .. s+1), u0 = w_s+1, the dissipation output in uv[1]
+        const Slots P{uv[0], ee[0]}, a = s4 ? Slots{uv[5], ee[4]} : s3 ? sk(c.ssp_s + 1) : Slots{uv[1], ee[1]}, none{};
         int rc = SWM_OK;
         auto coef = [&](int q) {
             Coef<Q> kk{};
@@ -1070,7 +1082,20 @@ template <class Fm> struct Adj {
                 F.p[q - 1] = adj::UVE{ctx->u[up[q - 1]].ref(), ctx->v[up[q - 1]].ref(), 0u};
             }
         }
-        for (int q = 1; !ssp && !s4 && q <= c.RKo && rc == SWM_OK; ++q) {
+        if (s3) {                                            // :114-156, stage q = rki - 1
+            Coef<Q> kk{};
+            kk.c = (Q)c.ssp_Dt_Dn;
+            kk.x = (Q)c.ssp_kna;
+            kk.y = (Q)c.ssp_knb;
+            kk.z = (Q)c.ssp_Dt_Dnc;
+            const Slots X = sk(std::max(1, c.ssp_mn));      // the state kept after stage mn (the input when mn = 1)
+            for (int q = 1; q <= c.ssp_s && rc == SWM_OK; ++q) {
+                const int m = q + 1 == c.ssp_kn ? M_SSP3_KN : M_SSP3;
+                rc = LF::stage(ctx, m, sk(q), P, X, sk(q + 1), none, kk, T);
+                if (!rc) rc = LF::continuity(ctx, m, Slots{sk(q + 1).uv, sk(q).e}, P, X, sk(q + 1), none, kk, T);
+            }
+        }
+        for (int q = 1; !ssp && !s4 && !s3 && q <= c.RKo && rc == SWM_OK; ++q) {
             // stages q, q+1 as one fused pair that also keeps the input of stage q+1
             if (q + 1 <= c.RKo && LF::can_tape(ctx, mode(q), mode(q + 1))) {
                 rc = LF::fused_tape(ctx, mode(q), mode(q + 1), sk(q), P, q == 1 ? none : a, q + 1 < c.RKo ? sk(q + 2) : none,
@@ -1088,15 +1113,15 @@ template <class Fm> struct Adj {
         }
         int post = a.uv;
         if (i % c.nstep_diff == 0) {
-            post = s4 ? uv[6] : uv[5];
+            post = s4 ? uv[6] : s3 ? uv[1] : uv[5];
             if ((rc = LF::dissipation(ctx, a, post))) return rc;
         }
         const int sst_old = ctx->rsst;
         if ((rc = LF::tracer(ctx, i, post))) return rc;
-        for (int q = 1; !s4 && q <= (ssp ? c.RKs + 1 : c.RKo); ++q) {
+        for (int q = 1; !s4 && q <= (ssp ? c.RKs + 1 : s3 ? c.ssp_s + 1 : c.RKo); ++q) {
             const Slots s = sk(q);
             F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
-            if (ssp && q >= 2) F.p[q - 2] = adj::UVE{F.s[q - 1].u, F.s[q - 1].v, 0u};    // u'_k = u_{k+1}
+            if ((ssp || s3) && q >= 2) F.p[q - 2] = adj::UVE{F.s[q - 1].u, F.s[q - 1].v, 0u};    // u'_k = u_{k+1}
         }
         F.pre = adj::UVE{ctx->u[a.uv].ref(), ctx->v[a.uv].ref(), ctx->e[a.e].ref()};
         F.post_u = ctx->u[post].ref();

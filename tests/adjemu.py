@@ -21,9 +21,9 @@ _lib = None
 class EmuCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("nx", "ny", "per", "RKo", "cont", "drag", "smag", "consume", "tracer", "nadvstep",
-                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin", "ssp", "s4")] + \
+                 "nadvstep_half", "nstep_diff", "L", "nan_edges", "fmt", "sad", "lin", "ssp", "s4", "s3", "kn", "mn")] + \
                [("a", ctypes.c_double * 4), ("b", ctypes.c_double * 4)] + \
-               [(n, ctypes.c_double) for n in ("cs", "sa", "sb")] + \
+               [(n, ctypes.c_double) for n in ("cs", "sa", "sb", "cs2")] + \
                [(n, ctypes.c_double) for n in
                 ("g", "gamma", "k12", "oma", "cD", "rD", "cS", "nuB", "dtd", "hdu", "hdv", "du", "dv", "jSST")]
 
@@ -71,6 +71,10 @@ def config(S, cfg, L=8, nan_edges=False, fmt=8):
         k.cs = round_scalar(cfg.Dt_Ds, P.Tprog)
         k.sa = round_scalar(1.0 / cfg.RKs, P.Tprog)
         k.sb = round_scalar((cfg.RKs - 1) / cfg.RKs, P.Tprog)
+    elif cfg.time_scheme == 2:                             # SSPRK3: stages from 1, kn-th combines with the mn-th's state
+        k.s3, k.kn, k.mn = cfg.ssp_s, cfg.ssp_kn - 1, cfg.ssp_mn - 1
+        k.cs, k.cs2 = round_scalar(cfg.ssp_Dt_Dn, P.Tprog), round_scalar(cfg.ssp_Dt_Dnc, P.Tprog)
+        k.sa, k.sb = round_scalar(cfg.ssp_kna, P.Tprog), round_scalar(cfg.ssp_knb, P.Tprog)
     elif cfg.time_scheme == 3:                             # 4SSPRK3: ½·Δt_Δ on the tendency, thirds of stage 3
         k.s4 = 1
         k.cs = 0.5 * round_scalar(cfg.Dt_D, P.Tprog)

@@ -319,9 +319,9 @@ extern "C" {
 // Scalars of a configuration, as the GPU driver derives them from swm_config.
 struct EmuCfg {
     int32_t nx, ny, per, RKo, cont, drag, smag, consume, tracer, nadvstep, nadvstep_half, nstep_diff;
-    int32_t L, nan_edges, fmt, sad, lin, ssp, s4;
+    int32_t L, nan_edges, fmt, sad, lin, ssp, s4, s3, kn, mn;
     double a[4], b[4];
-    double cs, sa, sb;
+    double cs, sa, sb, cs2;
     double g, gamma, k12, oma, cD, rD, cS, nuB, dtd, hdu, hdv, du, dv, jSST;
 };
 
@@ -375,6 +375,7 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
     k.lin = c.lin;
     k.ssp = c.ssp;
     k.s4 = c.s4;
+    k.s3 = c.s3; k.kn = c.kn; k.mn = c.mn; k.cs2 = (R)c.cs2;
     k.cs = (R)c.cs; k.sa = (R)c.sa; k.sb = (R)c.sb;
     return k;
 }
@@ -384,7 +385,7 @@ template <class R> AdjConsts<R> consts(const EmuCfg &c) {
 // op 2: λ before the dissipation.  in: u0 v0 e0 H lu lv le;  out: λu λv λe
 // op 3: tracer at step i (advection and snapshot as due).  in: u v um vm sst lum lvm lsst;
 //       out: λu λv λum λvm λsst
-// op 4: reverse of step i.  in: s_1..s_RKo (3 each; SSPRK2: w_1..w_RKs+1; 4SSPRK3: w_1..w_4 and
+// op 4: reverse of step i.  in: s_1..s_RKo (3 each; SSPRK2: w_1..w_RKs+1; SSPRK3: w_1..w_s+1; 4SSPRK3: w_1..w_4 and
 //       the 4 stages' (u', v')), pre (3), post (2), um vm sst, H fq,
 //       L (u v e sst um vm) (+ f_u f_v with the linear dynamics);  out: L (u v e sst um vm)
 // op 5: fold of the ghost copies.  in: Lu Lv Le Lsst;  out: gu gv ge gsst
@@ -399,10 +400,11 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
     EmuK<R> K{&A, c.L};
     auto uve = [&]() { UVE f; f.u = A.add(true); f.v = A.add(true); f.e = A.add(true); return f; };
     AdjFields F{};
-    for (int j = 0; j < 4; ++j) F.s[j] = uve();
-    for (int j = 0; j < 4; ++j) {                        // SSPRK2: u'_k = u_{k+1}; 4SSPRK3: fields of their own
+    const int ns = c.ssp ? c.ssp + 1 : c.s4 ? 4 : c.s3 ? c.s3 + 1 : c.RKo;    // taped stage inputs
+    for (int j = 0; j < ns; ++j) F.s[j] = uve();
+    for (int j = 0; j < ns; ++j) {                       // SSPRK2, SSPRK3: u'_k = u_{k+1}; 4SSPRK3: fields of their own
         if (c.s4) F.p[j] = UVE{A.add(true), A.add(true), 0};
-        else if (j < 3) F.p[j] = UVE{F.s[j + 1].u, F.s[j + 1].v, 0};
+        else if (j + 1 < ns) F.p[j] = UVE{F.s[j + 1].u, F.s[j + 1].v, 0};
     }
     F.pre = uve();
     F.post_u = A.add(true); F.post_v = A.add(true); F.um = A.add(true); F.vm = A.add(true); F.sst = A.add(true);
@@ -445,7 +447,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
         } else if (op == 3 || op == 4) {
             int p = 0;
             if (op == 4) {
-                for (int j = 0; j < (c.ssp ? c.ssp + 1 : c.s4 ? 4 : c.RKo); ++j) {
+                for (int j = 0; j < ns; ++j) {
                     put<R>(A, F.s[j].u, in[p++], sh.u); put<R>(A, F.s[j].v, in[p++], sh.v); put<R>(A, F.s[j].e, in[p++], sh.e);
                 }
                 for (int j = 0; c.s4 && j < 4; ++j) { put<R>(A, F.p[j].u, in[p++], sh.u); put<R>(A, F.p[j].v, in[p++], sh.v); }

@@ -257,6 +257,10 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
                                    ("nonperiodic", {"time_scheme": "4SSPRK3"}),
                                    ("periodic", {"time_scheme": "4SSPRK3", "adv_scheme": "Sadourny"}),
                                    ("periodic", {"time_scheme": "4SSPRK3", "dynamics": "linear",
+                                                 "surface_relax": True}),
+                                   ("nonperiodic", {"time_scheme": "SSPRK3"}),
+                                   ("periodic", {"time_scheme": "SSPRK3", "RKn": 3, "adv_scheme": "Sadourny"}),
+                                   ("periodic", {"time_scheme": "SSPRK3", "dynamics": "linear",
                                                  "surface_relax": True})])
 def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
     """Three steps: the tracer's snapshot (step 2) and advection (step 3) included."""
@@ -276,7 +280,8 @@ def test_reverse_steps_are_the_transposed_jacobian(swm, orc, bc, kw):
 
 @pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"dynamics": "linear"}),
                                    ("periodic", {"time_scheme": "SSPRK2"}),
-                                   ("nonperiodic", {"time_scheme": "4SSPRK3"})])
+                                   ("nonperiodic", {"time_scheme": "4SSPRK3"}),
+                                   ("periodic", {"time_scheme": "SSPRK3", "RKn": 3})])
 def test_dot_product_identity_across_wave_strips(swm, orc, bc, kw):
     """A grid of several wave strips and short segments: <J dx, y> = <dx, J^T y>."""
     P, S, cfg, prog = setup(swm, bc, nx=130, L_ratio=0.5, **kw)

@@ -166,7 +166,7 @@ def test_rrule_of_time_integration(swm, gpu):
 
 def test_adjoint_rejects_unsupported_configurations(swm, gpu):
     for kw in (dict(T="BFloat16", Tprog="Float32"), dict(T="Float32", time_scheme="SSPRK2", RKs=4),
-               dict(T="Float32", time_scheme="SSPRK3")):
+               dict(T="Float32", time_scheme="SSPRK3", RKn=4)):
         P, S, cfg, prog = setup(swm, "nonperiodic", **kw)
         with swm.Simulation(S) as sim:
             sim.upload(prog)
@@ -293,17 +293,21 @@ SSP_CASES = [("nonperiodic", dict()),
              ("periodic", dict(nstep_advcor=2))]
 
 
-@pytest.mark.parametrize("scheme", ["SSPRK2", "4SSPRK3"])
+@pytest.mark.parametrize("scheme", ["SSPRK2", "4SSPRK3", "SSPRK3"])
 @pytest.mark.parametrize("bc,kw", SSP_CASES)
 def test_ssp_schemes_dot_product_fp64(swm, orc, gpu, scheme, bc, kw):
     """The SSPRK2 scheme (time_integration.jl:80-110: momentum, then the continuity from the
     updated velocities, RKs stages, u0 = u/RKs + (RKs-1)/RKs·u1; two stages need a CFL number
-    below one, so the two-stage cases run at cfl = 0.5) and the 4SSPRK3 scheme (:158-195: four
+    below one, so the two-stage cases run at cfl = 0.5), the 4SSPRK3 scheme (:158-195: four
     such stages with ½(u + (u + Δt_Δ·du)) updates, the third stage combined with the step's
-    input by thirds): the FP64 dot-product identity with the oracle's tangent-linear model of
-    the same steps; the gradient does not depend on the number of snapshots."""
+    input by thirds) and the SSPRK3 scheme (:112-156: n² stages, the kn-th combined with the
+    state kept after the mn-th; n = 2, whose kept state is the step's input, and n = 3): the
+    FP64 dot-product identity with the oracle's tangent-linear model of the same steps; the
+    gradient does not depend on the number of snapshots."""
     if scheme == "SSPRK2" and kw in (SSP_CASES[1][1], SSP_CASES[3][1]):
         kw = dict(kw, RKs=2, cfl=0.5)
+    if scheme == "SSPRK3" and kw in (SSP_CASES[2][1], SSP_CASES[4][1]):
+        kw = dict(kw, RKn=3)
     P, S, cfg, prog = setup(swm, bc, nx=90, time_scheme=scheme, **kw)
     n = 7
     ybar = cotangent(S, 6, np.float64)
