@@ -273,8 +273,10 @@ int swm_ens_persistent_steps(const swm_ens *ens, int64_t *steps, const char **wh
 
 /* ---- Reverse mode (no counterpart in the reference, which has no derivatives; it is what a
  * ChainRulesCore.rrule of time_integration, src/time_integration.jl:2-250, would call).
- * Classic RK scheme, or SSPRK2 with RKs = 2 or 3 stages (time_integration.jl:80-110), with
- * T = Tprog = Tcomm in {Float64, Float32}; the nonlinear dynamics
+ * Every time scheme of the reference: classic RK, SSPRK2 with RKs = 2 or 3 stages, SSPRK3
+ * with RKn = 2 or 3, 4SSPRK3 (time_integration.jl:53-195; the taped step keeps every stage
+ * input, so more stages are refused), with T = Tprog = Tcomm in {Float64, Float32}; the
+ * nonlinear dynamics
  * (Arakawa-Hsu or Sadourny PV fluxes, any nstep_advcor: the kept q and KEu+KEv are
  * differentiated through to the states they are formed from) or the linear dynamics
  * (rhs.jl:55-86).  Kept fields formed before swm_adj_forward's first step (not by it) are

@@ -183,7 +183,8 @@ class Simulation:
         _abi.check(_abi.load().swm_reduce(self.ctx, ctypes.byref(m), ctypes.byref(e)), self.ctx, "swm_reduce")
         return m.value, e.value
 
-    # ---- reverse mode (swm_adj_*): classic RK or SSPRK2 (RKs <= 3), T = Tprog = Tcomm in {Float64, Float32}
+    # ---- reverse mode (swm_adj_*): every time scheme (SSPRK2 RKs <= 3, SSPRK3 RKn <= 3), T = Tprog = Tcomm in
+    # {Float64, Float32}
     def adjoint_forward(self, nsteps: int, i0: int = 1, t0: int = 0, snapshots: int = 20):
         """Steps ``i0 .. i0+nsteps-1`` recorded for a pullback, with ``snapshots`` device
         checkpoints placed by the binomial (revolve) schedule."""
