@@ -291,7 +291,11 @@ int swm_ens_persistent_steps(const swm_ens *ens, int64_t *steps, const char **wh
  * (u, v, η, sst and the tracer's mid-point velocities) where a binomial checkpointing
  * schedule with `snapshots` snapshots places them (revolve: the steps are recomputed at most
  * r times for the least r with nsteps <= C(snapshots + r, snapshots); 0 snapshots recompute
- * from the start for every step).  Leaves the final state, as swm_step would. */
+ * from the start for every step; nsteps - 1 or more keep every state, so the reverse sweep
+ * recomputes nothing beyond its taped steps).  snapshots < 0: as many as half of the device's
+ * free memory holds, at most nsteps - 1 (on an MI355X every state of most runs).  The
+ * snapshots live in one device block, reused by later calls of the same layout.  Leaves the
+ * final state, as swm_step would. */
 int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots);
 /* swm_adj_pullback: the vector-Jacobian product of the recorded steps.  ybar_* are the
  * cotangents of the final Prog (Julia layout and format Tprog, halo included, as

@@ -736,7 +736,7 @@ int swm_ens_persistent_steps(const swm_ens *E, int64_t *steps, const char **why_
 
 int swm_adj_forward(swm_ctx *ctx, int64_t nsteps, int64_t i0, int64_t t0, int32_t snapshots) {
     if (!ctx) return SWM_E_ARG;
-    if (nsteps < 1 || i0 < 1 || snapshots < 0) return ctx->fail(SWM_E_ARG, "nsteps >= 1, i0 >= 1 and snapshots >= 0 required");
+    if (nsteps < 1 || i0 < 1) return ctx->fail(SWM_E_ARG, "nsteps >= 1 and i0 >= 1 required");
     if (int rc = adj_ok(ctx)) return rc;
     return fmt_ops(ctx->fmt).adj_forward(ctx, nsteps, i0, t0, snapshots);
 }

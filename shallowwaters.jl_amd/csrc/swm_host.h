@@ -320,6 +320,7 @@ struct AdjState {
     swm::adj::AdjFields F0{};
     int La = 64;                   // rows per wave segment of the adjoint marches
     std::vector<char *> snap;      // snapshots: [0] the initial state, [1..s] the schedule's, [s+1] the final state
+    char *snap_block = nullptr;    // one device allocation holding them all (kept while the layout repeats)
     std::vector<int64_t> at;       // step held by each schedule snapshot (-1: free)
     std::vector<char> snap_pending; // each snapshot's kept q and KEu+KEv still to be formed (nstep_advcor > 0)
     bool keep_init = false;        // the forward run formed them from its initial state
@@ -332,7 +333,8 @@ struct AdjState {
     bool fx_valid = false;         // the last pullback accumulated the cotangent of Fx
     double sweep_ms = 0.0;         // device time of the last reverse sweep (HIP events)
     void release() {
-        for (char *p : snap) if (p) (void)hipFree(p);
+        if (snap_block) (void)hipFree(snap_block);
+        snap_block = nullptr;
         snap.clear();
         at.clear();
         if (dmax) (void)hipFree(dmax);

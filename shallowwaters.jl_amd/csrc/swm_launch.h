@@ -965,22 +965,39 @@ template <class Fm> struct Adj {
     // ---- snapshots of full states (u, v, η, sst, um, vm)
     // with nstep_advcor > 0 also the kept q and KEu+KEv and whether they are still to be formed
     static int snap_fields(const swm_ctx *ctx) { return (adv_form(ctx->c) & ADV_KEEP) ? 8 : 6; }
+    // `count` snapshots in one device block; a block of the same layout is reused
     static int snap_alloc(swm_ctx *ctx, int count) {
         AdjState *st = ctx->adj;
-        st->release();
-        st->snap_bytes = (size_t)snap_fields(ctx) * ctx->elems * ctx->es;
+        const size_t bytes = (size_t)snap_fields(ctx) * ctx->elems * ctx->es;
         st->snap_pending.assign(count, 0);
-        for (int k = 0; k < count; ++k) {
+        if (!(st->snap_block && (int)st->snap.size() == count && st->snap_bytes == bytes && st->dmax)) {
+            st->release();
+            st->snap_bytes = bytes;
             void *p = nullptr;
-            if (hipMalloc(&p, st->snap_bytes) != hipSuccess) {
-                st->release();
-                return ctx->fail(SWM_E_HIP, "hipMalloc of snapshot %d (%zu bytes) failed", k, st->snap_bytes);
+            if (hipMalloc(&p, (size_t)count * bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                return ctx->fail(SWM_E_HIP, "hipMalloc of %d snapshots (%zu bytes) failed", count, (size_t)count * bytes);
             }
-            st->snap.push_back((char *)p);
+            st->snap_block = (char *)p;
+            for (int k = 0; k < count; ++k) st->snap.push_back(st->snap_block + (size_t)k * bytes);
+            if (hipMalloc(&st->dmax, 2 * sizeof(int)) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc");
         }
-        if (hipMalloc(&st->dmax, 2 * sizeof(int)) != hipSuccess) return ctx->fail(SWM_E_HIP, "hipMalloc");
         st->at.assign(count, -1);
         return SWM_OK;
+    }
+
+    // snapshots < 0: as many as half of the device's free memory holds (beside the ones a
+    // schedule always has), at most nsteps - 1: every state kept, nothing recomputed
+    static int auto_snapshots(swm_ctx *ctx, int64_t n) {
+        size_t freeb = 0, total = 0;
+        AdjState *st = ctx->adj;
+        if (st->snap_block) freeb += st->snap.size() * st->snap_bytes;     // (reused or freed by snap_alloc)
+        size_t f = 0;
+        if (hipMemGetInfo(&f, &total) != hipSuccess) f = 0;
+        freeb += f;
+        const size_t bytes = (size_t)snap_fields(ctx) * ctx->elems * ctx->es;
+        const int64_t fit = (int64_t)(freeb / 2 / bytes) - 2;
+        return (int)std::max<int64_t>(0, std::min<int64_t>({fit, n - 1, (int64_t)INT32_MAX - 2}));
     }
     static int copy_state(swm_ctx *ctx, int k, bool save) {
         AdjState *st = ctx->adj;
@@ -1176,6 +1193,7 @@ template <class Fm> struct Adj {
         int rc = init(ctx);
         if (rc) return rc;
         AdjState *st = ctx->adj;
+        if (s < 0) s = auto_snapshots(ctx, n);
         if ((rc = snap_alloc(ctx, s + 2))) return rc;
         st->i0 = i0; st->t0 = t0; st->n = n; st->nsnap = s; st->recorded = false; st->fwd_steps = 0;
         if ((rc = copy_state(ctx, 0, true))) return rc;

@@ -185,9 +185,10 @@ class Simulation:
 
     # ---- reverse mode (swm_adj_*): every time scheme (SSPRK2 RKs <= 3, SSPRK3 RKn <= 3), T = Tprog = Tcomm in
     # {Float64, Float32}
-    def adjoint_forward(self, nsteps: int, i0: int = 1, t0: int = 0, snapshots: int = 20):
+    def adjoint_forward(self, nsteps: int, i0: int = 1, t0: int = 0, snapshots: int = -1):
         """Steps ``i0 .. i0+nsteps-1`` recorded for a pullback, with ``snapshots`` device
-        checkpoints placed by the binomial (revolve) schedule."""
+        checkpoints placed by the binomial (revolve) schedule; -1: as many as half of the
+        device's free memory holds, at most one per step (nothing recomputed)."""
         _abi.check(_abi.load().swm_adj_forward(self.ctx, nsteps, i0, t0, snapshots), self.ctx, "swm_adj_forward")
 
     def pullback(self, ybar: PrognosticVars):
@@ -385,9 +386,10 @@ class Tangent:
         return "Tangent(" + ", ".join(f"{k}=..." for k in self.__dict__) + ")"
 
 
-def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshots: int = 20, wrt_Fx: bool = False):
+def rrule(f, Prog: PrognosticVars, Diag: DiagnosticVars, S: ModelSetup, snapshots: int = -1, wrt_Fx: bool = False):
     """``ChainRulesCore.rrule(::typeof(time_integration), Prog, Diag, S)``: runs
-    ``time_integration`` on the GPU keeping ``snapshots`` device checkpoints (revolve) and
+    ``time_integration`` on the GPU keeping ``snapshots`` device checkpoints (revolve; -1: as
+    many as half of the free device memory holds, at most one per step) and
     returns ``(y, pullback)``; ``pullback(ȳ)`` takes the cotangent of the returned interior
     fields (a PrognosticVars without halo) and returns ``(NoTangent(), Proḡ, NoTangent(),
     S̄)`` with Proḡ the gradient with respect to ``Prog`` (halo cells zero: the

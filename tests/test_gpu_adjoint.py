@@ -325,3 +325,28 @@ def test_ssp_schemes_dot_product_fp64(swm, orc, gpu, scheme, bc, kw):
         lhs = sum(float(np.sum(a * b)) for a, b in zip(dy, ybar))
         rhs = sum(float(np.sum(a * b)) for a, b in zip(dx, g))
         assert abs(lhs - rhs) <= 1e-11 * abs(lhs), f"<J dx, ȳ> = {lhs!r}, <dx, J^T ȳ> = {rhs!r}"
+
+
+def test_snapshots_from_free_memory_keep_every_state(swm, gpu):
+    """snapshots < 0 (the default of adjoint_forward and rrule): as many snapshots as half of
+    the free device memory holds, at most one per step.  The gradient equals that of a
+    two-snapshot schedule bit for bit, the reverse sweep recomputes only its taped steps,
+    and a second recording in the same context (the snapshot block reused) gives the same
+    gradient."""
+    P, S, cfg, prog = setup(swm, "nonperiodic", T="Float32", nx=70)
+    n = 9
+    ybar = cotangent(S, 7, np.float32)
+    g2, nf2 = gpu_gradient(swm, S, prog, ybar, n, snapshots=2)
+    yb = swm.PrognosticVars(u=ybar[0], v=ybar[1], η=ybar[2], sst=ybar[3], T=S.parameters.Tprog)
+    with swm.Simulation(S) as sim:
+        runs = []
+        for _ in range(2):
+            sim.upload(prog)
+            sim.adjoint_forward(n, 1, 0)
+            g, nf = sim.pullback(yb)
+            runs.append(([g.u, g.v, g.η, g.sst], nf))
+    assert nf2 > n
+    for g, nf in runs:
+        assert nf == n, nf
+        for a, b in zip(g, g2):
+            assert np.array_equal(np.asarray(a), np.asarray(b))
