@@ -526,6 +526,12 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
 
     RhsPipe<R, PER, AF, PER> ra, rb;
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
+    // A first stage reads Prog as its input, which B then needs three rows later (its Prog
+    // operand, u and η at row r-3, v at r-4): kept in registers rather than loaded again, as
+    // the load would find it gone from L2 and read HBM a second time.  Own lanes only are
+    // stored by B, and there A's input column is B's column (xu = xt = x).
+    constexpr bool PRHIST = MIA::first && MIB::need_pr;
+    VI wu_2 = 0, wu_3 = 0, we_2 = 0, we_3 = 0, wv_3 = 0, wv_4 = 0;
     // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
     Q U0_0 = 0, U0_1 = 0, U0_2 = 0, E0_0 = 0, E0_1 = 0, E0_2 = 0, V0_0 = 0, V0_1 = 0, V0_2 = 0;
@@ -554,7 +560,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         const VI wu_0 = nu, wv_0 = nv, we_0 = ne;
         const R u_0 = dec<R, SI>(wu_0), v_0 = dec<R, SI>(wv_0), e_0 = dec<R, SI>(we_0), H_0 = nH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
-        const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
+        const Q qpu_3 = PRHIST ? (Q)wu_3 : bpu, qpe_3 = PRHIST ? (Q)we_3 : bpe, qpv_4 = PRHIST ? (Q)wv_4 : bpv;
         const RowStatics<R> sa = nst, sb = st_2;
         nst = row_statics<F, false, CS, AF>(A, ar, vfq, vFx, vFy, vTt, r, vfu, vfv);
         {   // the input row r+1, clamped to the last row the march reads
@@ -564,9 +570,16 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         }
         {   // A's pointwise operands for the next iteration: rows r (u, η) and r-1 (v)
             if (MIA::need_pr) { npu = pr_u.ld(vQu, sQ); npe = pr_e.ld(vQt, sQ); npv = pr_v.ld(vQt, sQ - dQ); }
-            if (MIA::need_ac) { nau = ac_u.ld(vQu, sQ); nae = ac_e.ld(vQt, sQ); nav = ac_v.ld(vQt, sQ - dQ); }
+            if (MIA::need_ac) {
+                // A's u0 reaches B's combination on the segment's rows only: the rows of the
+                // march beyond it load a row of the segment instead (a cache hit, never used)
+                const uint32_t su = rowb<Q>(min(max(r, t.ya), t.yb - 1), P), sv = rowb<Q>(min(max(r - 1, t.ya), t.yb - 1), P);
+                nau = ac_u.ld(vQu, su); nae = ac_e.ld(vQt, su); nav = ac_v.ld(vQt, sv);
+            }
             // B's Prog operands for the next iteration: rows r-2 (u, η) and r-3 (v)
-            if (MIB::need_pr) { bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ); }
+            if (MIB::need_pr && !PRHIST) {
+                bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ);
+            }
         }
 
         // ---- stage A at input row r: u1, η1, u0, η0 at row r-1 and v1, v0 at row r-2
@@ -661,6 +674,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
                 else o2_v.st(o2, vxQ, sQ - 4 * dQ);
             }
         }
+        if constexpr (PRHIST) { wu_3 = wu_2; wu_2 = wu_1; we_3 = we_2; we_2 = we_1; wv_4 = wv_3; wv_3 = wv_2; }
         wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
         H_2 = H_1; H_1 = H_0;
         st_2 = st_1; st_1 = sa;
