@@ -211,12 +211,14 @@ ens_persist_kernel(PArgs<F> A) {
                 } else {
                     const StageArgs<F> a = ((const StageArgs<F> *)(pl + it.args))[m];
                     if (PER && it.kind == 5) {
-                        if (it.ma == M_RK_FIRST) fused_seam_body<F, M_RK_FIRST, M_RK_MID, false>(a, wl);
-                        else fused_seam_body<F, M_RK_MID, M_RK_LAST, false>(a, wl);
+                        // (the bodies reload Prog for stage B: one register allocation serves
+                        // every body here, and the extra rows would spill)
+                        if (it.ma == M_RK_FIRST) fused_seam_body<F, M_RK_FIRST, M_RK_MID, false, false>(a, wl);
+                        else fused_seam_body<F, M_RK_MID, M_RK_LAST, false, false>(a, wl);
                     } else if (it.ma == M_RK_FIRST) {
-                        fused_body<F, M_RK_FIRST, M_RK_MID, false, PER, ADV_AH>(a, wl);
+                        fused_body<F, M_RK_FIRST, M_RK_MID, false, PER, ADV_AH, false>(a, wl);
                     } else {
-                        fused_body<F, M_RK_MID, M_RK_LAST, false, PER, ADV_AH>(a, wl);
+                        fused_body<F, M_RK_MID, M_RK_LAST, false, PER, ADV_AH, false>(a, wl);
                     }
                 }
             }

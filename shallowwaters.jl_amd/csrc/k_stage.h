@@ -475,7 +475,7 @@ template <class F> __device__ __forceinline__ WaveTile fused_tile(const StageArg
     return t;
 }
 
-template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH>
+template <class F, int MA, int MB, bool CS, bool PER = false, int ADV = ADV_AH, bool LEAN = true>
 __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
@@ -526,11 +526,13 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
 
     RhsPipe<R, PER, AF, PER> ra, rb;
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0;        // stage A's input at rows r-1 (u, η), r-2 (v)
-    // A first stage reads Prog as its input, which B then needs three rows later (its Prog
-    // operand, u and η at row r-3, v at r-4): kept in registers rather than loaded again, as
-    // the load would find it gone from L2 and read HBM a second time.  Own lanes only are
-    // stored by B, and there A's input column is B's column (xu = xt = x).
-    constexpr bool PRHIST = MIA::first && MIB::need_pr;
+    // LEAN (the default): the loads that would read HBM a second time are avoided at the cost
+    // of a few registers; callers short of registers pass false.  A first stage reads Prog as
+    // its input, which B then needs three rows later (its Prog operand, u and η at row r-3,
+    // v at r-4): kept in registers rather than loaded again, as the load would find it gone
+    // from L2.  Own lanes only are stored by B, and there A's input column is B's column
+    // (xu = xt = x).
+    constexpr bool PRHIST = LEAN && MIA::first && MIB::need_pr;
     VI wu_2 = 0, wu_3 = 0, we_2 = 0, we_3 = 0, wv_3 = 0, wv_4 = 0;
     // stage A's outputs, k iterations old: u1, η1 of row r-1-k, v1 of row r-2-k, u0 / η0 / v0 alike
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;
@@ -571,9 +573,10 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
         {   // A's pointwise operands for the next iteration: rows r (u, η) and r-1 (v)
             if (MIA::need_pr) { npu = pr_u.ld(vQu, sQ); npe = pr_e.ld(vQt, sQ); npv = pr_v.ld(vQt, sQ - dQ); }
             if (MIA::need_ac) {
-                // A's u0 reaches B's combination on the segment's rows only: the rows of the
-                // march beyond it load a row of the segment instead (a cache hit, never used)
-                const uint32_t su = rowb<Q>(min(max(r, t.ya), t.yb - 1), P), sv = rowb<Q>(min(max(r - 1, t.ya), t.yb - 1), P);
+                // A's u0 reaches B's combination on the segment's rows only (LEAN: the rows of
+                // the march beyond it load a row of the segment instead, a cache hit never used)
+                const uint32_t su = LEAN ? rowb<Q>(min(max(r, t.ya), t.yb - 1), P) : sQ;
+                const uint32_t sv = LEAN ? rowb<Q>(min(max(r - 1, t.ya), t.yb - 1), P) : sQ - dQ;
                 nau = ac_u.ld(vQu, su); nae = ac_e.ld(vQt, su); nav = ac_v.ld(vQt, sv);
             }
             // B's Prog operands for the next iteration: rows r-2 (u, η) and r-3 (v)
@@ -695,7 +698,7 @@ __device__ __forceinline__ void fused_body(const StageArgs<F> &A, int w) {
 // copies; B runs on those in the first view, and the q[end,·] of its AHβ/AHγ wrap (q at
 // the corner nx of B's input) is formed from B's inputs as the second view sees them.  A's
 // own q[end,·] is the q the other view forms at the same lane (the corner nx).
-template <class F, int MA, int MB, bool CS>
+template <class F, int MA, int MB, bool CS, bool LEAN = true>
 __device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
     using MIA = Mode<MA>;
     using MIB = Mode<MB>;
@@ -736,6 +739,10 @@ __device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
 
     RhsPipe<R, true> ra, ras, rb;                      // A as column x, A as column xs, B
     VI wu_1 = 0, wv_1 = 0, wv_2 = 0, we_1 = 0, su_1 = 0, sv_1 = 0, sv_2 = 0, se_1 = 0;
+    // LEAN: a first stage's input as column x is B's Prog operand three rows later (u, η at
+    // row r-3, v at r-4): kept in registers rather than read from HBM a second time
+    constexpr bool PRHIST = LEAN && MIA::first && MIB::need_pr;
+    VI wu_2 = 0, wu_3 = 0, we_2 = 0, we_3 = 0, wv_3 = 0, wv_4 = 0;
     Q U1_0 = 0, U1_1 = 0, E1_0 = 0, E1_1 = 0, V1_0 = 0;       // B's input as column x sees it
     Q W1_0 = 0, W1_1 = 0, F1_0 = 0, F1_1 = 0, X1_0 = 0;       // ... and as column xs sees it (u, η, v)
     Q U0_0 = 0, U0_1 = 0, U0_2 = 0, E0_0 = 0, E0_1 = 0, E0_2 = 0, V0_0 = 0, V0_1 = 0, V0_2 = 0;
@@ -761,7 +768,7 @@ __device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
         const R H_0 = nH, Hs_0 = mH;
         const Q pu_1 = npu, au_1 = nau, pe_1 = npe, ae_1 = nae, pv_2 = npv, av_2 = nav;
         const Q qpu_1 = mpu, qau_1 = mau, qpe_1 = mpe, qae_1 = mae, qpv_2 = mpv, qav_2 = mav;
-        const Q qpu_3 = bpu, qpe_3 = bpe, qpv_4 = bpv;
+        const Q qpu_3 = PRHIST ? (Q)wu_3 : bpu, qpe_3 = PRHIST ? (Q)we_3 : bpe, qpv_4 = PRHIST ? (Q)wv_4 : bpv;
         const RowStatics<R> sa = nst, sas = mst, sb = st_2, sbs = ss_2;
         nst = row_statics<F, false, CS>(A, ar, vfq, vFx, vFy, vxT, r);
         mst = row_statics<F, false, CS>(A, ar, sfq, sFx, sFy, vsT, r);
@@ -775,11 +782,15 @@ __device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
             npu = pr_u.ld(vxQ, sQ); npe = pr_e.ld(vxQ, sQ); npv = pr_v.ld(vxQ, sQ - dQ);
             mpu = pr_u.ld(vsQ, sQ); mpe = pr_e.ld(vsQ, sQ); mpv = pr_v.ld(vsQ, sQ - dQ);
         }
-        if (MIA::need_ac) {
-            nau = ac_u.ld(vxQ, sQ); nae = ac_e.ld(vxQ, sQ); nav = ac_v.ld(vxQ, sQ - dQ);
-            mau = ac_u.ld(vsQ, sQ); mae = ac_e.ld(vsQ, sQ); mav = ac_v.ld(vsQ, sQ - dQ);
+        if (MIA::need_ac) {       // A's u0 reaches B's combination on the segment's rows only (LEAN)
+            const uint32_t su = LEAN ? rowb<Q>(min(max(r, t.ya), t.yb - 1), P) : sQ;
+            const uint32_t sv = LEAN ? rowb<Q>(min(max(r - 1, t.ya), t.yb - 1), P) : sQ - dQ;
+            nau = ac_u.ld(vxQ, su); nae = ac_e.ld(vxQ, su); nav = ac_v.ld(vxQ, sv);
+            mau = ac_u.ld(vsQ, su); mae = ac_e.ld(vsQ, su); mav = ac_v.ld(vsQ, sv);
         }
-        if (MIB::need_pr) { bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ); }
+        if (MIB::need_pr && !PRHIST) {
+            bpu = pr_u.ld(vxQ, sQ - 2 * dQ); bpe = pr_e.ld(vxQ, sQ - 2 * dQ); bpv = pr_v.ld(vxQ, sQ - 3 * dQ);
+        }
 
         // ---- stage A at input row r in both views.  The wrap's q[end,·] of one view is q at
         // the corner nx, which the other view forms at the same lane.
@@ -876,6 +887,7 @@ __device__ __forceinline__ void fused_seam_body(const StageArgs<F> &A, int w) {
                 else o2_v.st(o2, vxQ, sQ - 4 * dQ);
             }
         }
+        if constexpr (PRHIST) { wu_3 = wu_2; wu_2 = wu_1; we_3 = we_2; we_2 = we_1; wv_4 = wv_3; wv_3 = wv_2; }
         wu_1 = wu_0; wv_2 = wv_1; wv_1 = wv_0; we_1 = we_0;
         su_1 = su_0; sv_2 = sv_1; sv_1 = sv_0; se_1 = se_0;
         H_2 = H_1; H_1 = H_0; Hs_2 = Hs_1; Hs_1 = Hs_0;
