@@ -1129,13 +1129,18 @@ template <class Fm> struct Adj {
             if (due || (c.bottom_drag == SWM_DRAG_QUADRATIC && i % c.nstep_diff == 0))
                 if ((rc = LF::advcor(ctx, a, due, false))) return rc;
         }
+        // The reverse step reads the state after the dissipation, and what the tracer leaves,
+        // only through the adjoint of the tracer's advection: a step without it ends here
+        // (the mid-point snapshot of um, vm serves later forward steps, which the sweep
+        // recomputes from stored states)
+        const bool adv = c.tracer_advection && i % c.nadvstep == 0;
         int post = a.uv;
-        if (i % c.nstep_diff == 0) {
+        if (adv && i % c.nstep_diff == 0) {
             post = s4 ? uv[6] : s3 ? uv[1] : uv[5];
             if ((rc = LF::dissipation(ctx, a, post))) return rc;
         }
         const int sst_old = ctx->rsst;
-        if ((rc = LF::tracer(ctx, i, post))) return rc;
+        if (adv && (rc = LF::tracer(ctx, i, post))) return rc;
         for (int q = 1; !s4 && q <= (ssp ? c.RKs + 1 : s3 ? c.ssp_s + 1 : c.RKo); ++q) {
             const Slots s = sk(q);
             F.s[q - 1] = adj::UVE{ctx->u[s.uv].ref(), ctx->v[s.uv].ref(), ctx->e[s.e].ref()};
