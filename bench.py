@@ -149,12 +149,12 @@ def run_ensemble(swm, nmem: int, steps: int, seq_steps: int, seed: int = 4):
 
 
 def adjoint_bytes_per_cell(es: int) -> dict:
-    """Algorithmic HBM bytes per cell of the reverse step's kernels (each field read or
+    """Algorithmic HBM bytes per cell of the RK4 reverse step's kernels (each field read or
     written once): the last RK stage's adjoint reads the stage input, H and λ0 and writes
-    μ; the middle ones also read μ of the later stage and the running sum A and write A;
-    the first writes the step input's adjoint; the dissipation adjoint reads the state
-    before the dissipation, H and λ after it, and writes λ before it."""
-    return {"stage_adj_last": 10 * es, "stage_adj_next": 16 * es, "stage_adj_mid": 19 * es, "stage_adj_first": 16 * es,
+    μ; the middle ones also read μ of the later stage; the first reads the μ of all three
+    later stages and writes the step input's adjoint; the dissipation adjoint reads the
+    state before the dissipation, H and λ after it, and writes λ before it."""
+    return {"stage_adj_last": 10 * es, "stage_adj_next": 13 * es, "stage_adj_mid": 13 * es, "stage_adj_first": 19 * es,
             "diss_adj": 10 * es}
 
 

@@ -104,8 +104,8 @@ template <class R> struct StageAdjArgs {
     uint32_t qend;         // periodic: q at column nx for every corner row of s_k (1-D)
     UVE l0;                // λ of u0 after the stages (halo unfolded)
     UVE mu_in;             // μ_{k+1} = (∂R/∂s_{k+1})ᵀ λd_{k+1} (halo unfolded)
-    UVE a_in;              // A_{k+1} = λ0 + Σ_{j>k} λs_j on interior cells
-    UVE mu_out, a_out;     // μ_k, A_k (or, for the first stage, λ of the step's input: A_1 + μ_1)
+    UVE mu_x[2];           // the first stage: μ of the stages after its successor, oldest first (NX of them)
+    UVE mu_out, a_out;     // μ_k; the first stage: λ of the step's input, A_1 + μ_1
     uint32_t lqend;        // periodic: λ of q(nx, y), 1-D
     uint32_t lfx;          // fxg: λ of the zonal wind stress Fx at the u points, accumulated
     R a, b;                // RKaΔt[k], RKbΔt[k] (b of the stage whose input is s_{k+1})
@@ -189,14 +189,14 @@ SWM_HD typename W::V fold_e(const W &w, uint32_t o, int y, typename W::V l, int 
 template <class V> struct StageLoads {
     V u, v, e, H, fq, qe;      // stage input at row r; f_q and q(nx) at corner row r-1
     V lu, lv, le, mu, mv, me;  // cells of λ0 and μ_{k+1} at row r-1 (the folds add their halo)
-    V au, av, ae;              // running sum A at the row emitted, r-2
+    V xau, xav, xae, xbu, xbv, xbe;   // the first stage: cells of the extra μ's at row r-1
     V fx;                      // λFx at the row emitted, r-2 (fxg)
     V fu, fv;                  // linear dynamics: f_u at row r-1, f_v at row r-2
     V qk, qke;                 // kept q at corner row r-1, and at its column nx (periodic wrap)
     V lq, lk;                  // λ of the kept q and KEu+KEv at the row emitted, r-2
 };
 
-template <class W, bool PER, bool NEXT, bool A_L0, int SCH = ADJ_AH>
+template <class W, bool PER, bool NEXT, int NX, int SCH = ADJ_AH>
 SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<typename W::R> &A, int r, int ya, int yb) {
     using V = typename W::V;
     using R = typename W::R;
@@ -226,11 +226,10 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
     p.lu = w.ld(A.l0.u, yu); p.lv = w.ld(A.l0.v, yv); p.le = w.ld(A.l0.e, yu);
     p.mu = p.mv = p.me = zero;
     if (NEXT) { p.mu = w.ld(A.mu_in.u, yu); p.mv = w.ld(A.mu_in.v, yv); p.me = w.ld(A.mu_in.e, yu); }
-    p.au = p.av = p.ae = zero;
+    p.xau = p.xav = p.xae = p.xbu = p.xbv = p.xbe = zero;
+    if (NX >= 1) { p.xau = w.ld(A.mu_x[0].u, yu); p.xav = w.ld(A.mu_x[0].v, yv); p.xae = w.ld(A.mu_x[0].e, yu); }
+    if (NX >= 2) { p.xbu = w.ld(A.mu_x[1].u, yu); p.xbv = w.ld(A.mu_x[1].v, yv); p.xbe = w.ld(A.mu_x[1].e, yu); }
     const int ye = amin(amax(rs - 2, ya), yb - 1);
-    if (NEXT && !A_L0) {
-        p.au = w.ld(A.a_in.u, ye); p.av = w.ld(A.a_in.v, ye); p.ae = w.ld(A.a_in.e, ye);
-    }
     p.fx = A.fxg && !CONT ? w.ld(A.lfx, ye) : zero;
     return p;
 }
@@ -256,10 +255,12 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
 // The march reads stage-input row r and tendency-adjoint row r-1 and emits row r-2.
 //
 // Stage bookkeeping (classic RK, time_integration.jl:55-78): with λ0 the adjoint of u0
-// after the stages, λs_{k+1} the folded μ_{k+1} and A the running sum,
-//   λd_k = a_k·λ0 + b_k·λs_{k+1},  μ_k = (∂R/∂s_k)ᵀ λd_k,  A_k = A_{k+1} + λs_{k+1}.
-// NEXT: the stage has a successor (k < RKo).  A_L0: A_{k+1} is λ0 itself (k+1 = RKo).
-// PRE: the first stage, whose output is the adjoint of the step's input, A_1 + μ_1.
+// after the stages and λs_{k+1} the folded μ_{k+1},
+//   λd_k = a_k·λ0 + b_k·λs_{k+1},  μ_k = (∂R/∂s_k)ᵀ λd_k,
+// and the adjoint of the step's input A_1 + μ_1, A_1 = λ0 + Σ_{k>1} λs_k summed as the
+// recursion A_k = A_{k+1} + λs_{k+1} from A_RKo = λ0 would (((λ0 + λs_4) + λs_3) + λs_2).
+// NEXT: the stage has a successor (k < RKo).  PRE: the first stage, which forms the step
+// input's adjoint from λ0, its own μ_2 and NX = RKo - 2 more μ's (mu_x, oldest first).
 //
 // SAD: Sadourny's enstrophy-conserving PV fluxes (PV_advection.jl:31-59) in place of
 // Arakawa-Hsu's, k12 = ½ (the weight of Iy!(q_u,q) and Ix!(q_v,q)):
@@ -280,7 +281,7 @@ SWM_HD StageLoads<typename W::V> stage_loads(const W &w, const StageAdjArgs<type
 //
 // CONT: gu = gv = 0 (and no PV): λU = ge(x+1) - ge, λV = ge(y+1) - ge, λu = λU·hu, λv = λV·hv,
 //   λη = λh (- γ·ge): the transpose of dη = -((U-U(x-1)) + (V-V(y-1))) (- γη) alone.
-template <class W, bool PER, bool NEXT, bool A_L0, bool PRE, int SCH = ADJ_AH>
+template <class W, bool PER, bool NEXT, int NX, bool PRE, int SCH = ADJ_AH>
 SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, int yb) {
     using R = typename W::R;
     using V = typename W::V;
@@ -289,6 +290,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     constexpr bool CONT = SCH == ADJ_CONT;
     static_assert(!(LIN && KEEP), "the linear dynamics keep nothing");
     static_assert(!(CONT && NEXT), "the continuity form is not part of a classic RK stage");
+    static_assert(NX == 0 || (NEXT && PRE), "extra μ's are summed by the first stage only");
     const int nx = A.nx, ny = A.ny;
     const auto x = w.x;
     const M ucol = PER ? M((x >= 0) & (x <= nx - 1)) : M((x >= 1) & (x <= nx - 1));
@@ -309,14 +311,15 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
     V gu_2 = zero, gv_2 = zero, ge_2 = zero;
     V lp_2 = zero, lU_2 = zero, la_2 = zero, lb_2 = zero, lg_2 = zero, ld_2 = zero, lnum_3 = zero, lhv_3 = zero, lhq_3 = zero;
     V su_2 = zero, sv_2 = zero, se_2 = zero;           // λs_{k+1} of row r-2
-    V zu_2 = zero, zv_2 = zero, ze_2 = zero;           // folded λ0 of row r-2 (A_L0)
+    V zu_2 = zero, zv_2 = zero, ze_2 = zero;           // folded λ0 of row r-2
+    V xau_2 = zero, xav_2 = zero, xae_2 = zero, xbu_2 = zero, xbv_2 = zero, xbe_2 = zero;   // folded extra μ's, row r-2
     V Qu_2 = zero, Vu_2 = zero, Uv_2 = zero;           // Sadourny: Qu, Vu of u row r-2, Uv of v row r-2
     V pu_2 = zero, pv_3 = zero;                        // linear: f_u·λdu of row r-2, f_v·λdv of row r-3
 
-    StageLoads<V> nl = stage_loads<W, PER, NEXT, A_L0, SCH>(w, A, ya - 2, ya, yb);
+    StageLoads<V> nl = stage_loads<W, PER, NEXT, NX, SCH>(w, A, ya - 2, ya, yb);
     for (int r = ya - 2; r <= yb + 1; ++r) {
         const StageLoads<V> c = nl;
-        nl = stage_loads<W, PER, NEXT, A_L0, SCH>(w, A, r + 1, ya, yb);
+        nl = stage_loads<W, PER, NEXT, NX, SCH>(w, A, r + 1, ya, yb);
         // ---- stage input row r
         const V u_0 = c.u, v_0 = c.v, e_0 = c.e, H_0 = c.H;
         const V fq_1 = c.fq;
@@ -325,6 +328,7 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         const int yd = r - 1;
         const bool ydu = yd >= 1 && yd <= ny, ydv = yd >= 1 && yd <= ny - 1;
         V zu = zero, zv = zero, ze = zero, su = zero, sv = zero, se = zero;
+        V xau = zero, xav = zero, xae = zero, xbu = zero, xbv = zero, xbe = zero;
         if (ydu) {
             zu = fold_u<W, PER>(w, A.l0.u, yd, c.lu, A.oma, nx, ny);
             ze = fold_e<W, PER>(w, A.l0.e, yd, c.le, nx, ny);
@@ -332,10 +336,20 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
                 su = fold_u<W, PER>(w, A.mu_in.u, yd, c.mu, A.oma, nx, ny);
                 se = fold_e<W, PER>(w, A.mu_in.e, yd, c.me, nx, ny);
             }
+            if (NX >= 1) {
+                xau = fold_u<W, PER>(w, A.mu_x[0].u, yd, c.xau, A.oma, nx, ny);
+                xae = fold_e<W, PER>(w, A.mu_x[0].e, yd, c.xae, nx, ny);
+            }
+            if (NX >= 2) {
+                xbu = fold_u<W, PER>(w, A.mu_x[1].u, yd, c.xbu, A.oma, nx, ny);
+                xbe = fold_e<W, PER>(w, A.mu_x[1].e, yd, c.xbe, nx, ny);
+            }
         }
         if (ydv) {
             zv = fold_v<W, PER>(w, A.l0.v, yd, c.lv, A.oma, nx);
             if (NEXT) sv = fold_v<W, PER>(w, A.mu_in.v, yd, c.mv, A.oma, nx);
+            if (NX >= 1) xav = fold_v<W, PER>(w, A.mu_x[0].v, yd, c.xav, A.oma, nx);
+            if (NX >= 2) xbv = fold_v<W, PER>(w, A.mu_x[1].v, yd, c.xbv, A.oma, nx);
         }
         const V gu_1 = CONT ? zero : W::sel(ucol & ydu, NEXT ? A.a * zu + A.b * su : A.a * zu, zero);
         const V gv_1 = CONT ? zero : W::sel(vcol & ydv, NEXT ? A.a * zv + A.b * sv : A.a * zv, zero);
@@ -438,23 +452,14 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
             const bool yu_st = PER ? yu_in : (y >= -1 && y <= ny + 2);
             const bool yv_st = yv_in, ye_st = y >= 0 && y <= ny + 1;
             const M iu = ucol & yu_in, iv = vcol & yv_in, ie = vcol & yu_in;
-            if (NEXT) {
-                const V au = A_L0 ? zu_2 : c.au;
-                const V av = A_L0 ? zv_2 : c.av;
-                const V ae = A_L0 ? ze_2 : c.ae;
-                const V Au = au + su_2, Av = av + sv_2, Ae = ae + se_2;     // A_k on interior cells
-                if (PRE) {
-                    w.st(A.a_out.u, y, W::sel(iu, Au + lu_2, lu_2), st_u & yu_st);
-                    w.st(A.a_out.v, y, W::sel(iv, Av + lv_2, lv_2), st_v & yv_st);
-                    w.st(A.a_out.e, y, W::sel(ie, Ae + le_2, le_2), st_e & ye_st);
-                } else {
-                    w.st(A.mu_out.u, y, lu_2, st_u & yu_st);
-                    w.st(A.mu_out.v, y, lv_2, st_v & yv_st);
-                    w.st(A.mu_out.e, y, le_2, st_e & ye_st);
-                    w.st(A.a_out.u, y, Au, w.own & iu);
-                    w.st(A.a_out.v, y, Av, w.own & iv);
-                    w.st(A.a_out.e, y, Ae, w.own & ie);
-                }
+            if (NEXT && PRE) {
+                V au = zu_2, av = zv_2, ae = ze_2;              // A_RKo = λ0, then the recursion
+                if (NX >= 1) { au = au + xau_2; av = av + xav_2; ae = ae + xae_2; }
+                if (NX >= 2) { au = au + xbu_2; av = av + xbv_2; ae = ae + xbe_2; }
+                const V Au = au + su_2, Av = av + sv_2, Ae = ae + se_2;     // A_1 on interior cells
+                w.st(A.a_out.u, y, W::sel(iu, Au + lu_2, lu_2), st_u & yu_st);
+                w.st(A.a_out.v, y, W::sel(iv, Av + lv_2, lv_2), st_v & yv_st);
+                w.st(A.a_out.e, y, W::sel(ie, Ae + le_2, le_2), st_e & ye_st);
             } else {
                 w.st(A.mu_out.u, y, lu_2, st_u & yu_st);
                 w.st(A.mu_out.v, y, lv_2, st_v & yv_st);
@@ -478,6 +483,8 @@ SWM_HD void stage_adjoint(W &w, const StageAdjArgs<typename W::R> &A, int ya, in
         lnum_3 = lnum_2; lhv_3 = lhv_2; lhq_3 = lhq_2;
         su_2 = su; sv_2 = sv; se_2 = se;
         zu_2 = zu; zv_2 = zv; ze_2 = ze;
+        if constexpr (NX >= 1) { xau_2 = xau; xav_2 = xav; xae_2 = xae; }
+        if constexpr (NX >= 2) { xbu_2 = xbu; xbv_2 = xbv; xbe_2 = xbe; }
         if constexpr (SAD) { Qu_2 = Qu_1; Vu_2 = Vu_1; Uv_2 = Uv_1; }
     }
 }
@@ -1062,7 +1069,8 @@ template <class R, class C> SWM_HD void fold_out_cell(const C &c, const FoldOutA
 // reversed (the stage inputs s_1..s_RKo, or under SSPRK2 / 4SSPRK3 the stage inputs w_k and
 // the velocities after each stage's momentum update; u0 before and after the dissipation, the
 // tracer state it read); L is the carried adjoint (halo unfolded), Z a second set of the same,
-// Ma / Mb / A the stage adjoint's μ ping-pong and running sum.  The tracer adjoint's
+// Ma / Mb / A the μ of the classic RK stages RKo, RKo-1, RKo-2 (the SSP stages use Ma, Mb
+// for the two marches of a stage and A for the kept state's adjoint).  The tracer adjoint's
 // records reuse Z, Ma, Mb, A and two more fields.
 struct AdjFields {
     UVE s[12];                      // stage inputs (classic RK: 4 at most; SSPRK3 with n = 3: 10)
@@ -1107,7 +1115,7 @@ template <class R> struct AdjConsts {
 };
 
 // The reverse of step i: L (the adjoint of the state after the step) becomes the adjoint of
-// the state before it.  Launcher K issues the kernels: stage<PER, NEXT, A_L0, PRE>(args),
+// the state before it.  Launcher K issues the kernels: stage(args, per, next, nx, pre, form),
 // qend(args), qend_adj(args, mu), diss<PER, DRAG, SMAG>(args), tracer(args), snapshot(),
 // advcor_adj(args), lqend_keep(args), zero(field), lincomb(d, x, ax, y, ay, z, az), in order
 // on one queue.
@@ -1179,7 +1187,7 @@ template <class R, class K> int reverse_ssp(K &k, AdjFields &F, const AdjConsts<
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
     S.lfx = F.lfx; S.fxt = c.fxt;
     S.b = nil;
-    S.mu_in = S.a_in = S.a_out = F.A;               // (read and written by NEXT stages only)
+    S.mu_in = S.a_out = S.mu_x[0] = S.mu_x[1] = F.A;   // (read and written by NEXT stages only)
     const int sch = (c.lin ? ADJ_LIN : c.sad ? ADJ_SAD : ADJ_AH) | (c.nadvcor > 0 && !c.lin ? ADJ_KEEP : 0);
     const bool wrap = c.per && (sch & 3) == ADJ_AH, keep = (sch & ADJ_KEEP) != 0;
     for (int st = ns; st >= 1; --st) {
@@ -1268,23 +1276,25 @@ template <class R, class K> int reverse_step(K &k, AdjFields &F, const AdjConsts
     S.g = c.g; S.gamma = c.gamma; S.k12 = c.k12; S.oma = c.oma; S.cont = c.cont;
     S.nx = c.nx; S.ny = c.ny; S.P = c.P;
     S.lfx = F.lfx; S.fxt = c.fxt; S.fxg = c.fxg;
-    UVE mu_prev = F.Ma, mu_next = F.Mb;
+    // μ of stages RKo, RKo-1, RKo-2 in three buffers: the first stage reads them all
+    const UVE mu[3] = {F.Ma, F.Mb, F.A};
+    S.mu_x[0] = mu[0];
+    S.mu_x[1] = mu[1];
+    S.a_out = out;
     for (int st = c.RKo; st >= 1; --st) {
-        const bool next = st < c.RKo, al0 = st == c.RKo - 1, pre = st == 1;
+        const bool next = st < c.RKo, pre = st == 1;
+        const int nx = pre ? c.RKo - 2 : 0;
         S.s = F.s[st - 1];
         S.a = c.a[st - 1];
         S.b = next ? c.b[st - 1] : R(0);
-        S.mu_in = mu_prev;
-        S.a_in = F.A;
-        S.mu_out = next ? mu_next : mu_prev;
-        S.a_out = pre ? out : F.A;
+        S.mu_in = next ? mu[c.RKo - st - 1] : F.A;
+        S.mu_out = pre ? out : mu[c.RKo - st];
         const bool wrap = c.per && (sch & 3) == ADJ_AH; // the Arakawa-Hsu first-column wrap to q[end,·]
         const bool keep = (sch & ADJ_KEEP) != 0;        // (kept q: q[end,·] is the kept field's)
         if (wrap && !keep && (rc = k.qend(S))) return rc;
-        if ((rc = k.stage(S, c.per, next, al0, pre, sch))) return rc;
-        if (wrap && !keep && (rc = k.qend_adj(S, pre ? out : S.mu_out))) return rc;
+        if ((rc = k.stage(S, c.per, next, nx, pre, sch))) return rc;
+        if (wrap && !keep && (rc = k.qend_adj(S, S.mu_out))) return rc;
         if (wrap && keep && (rc = k.lqend_keep(S))) return rc;
-        if (next) { const UVE t = mu_prev; mu_prev = mu_next; mu_next = t; }
     }
     F.L = out;
     F.Z = l0;

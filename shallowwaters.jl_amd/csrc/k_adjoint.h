@@ -61,12 +61,12 @@ template <class R> struct GpuCells {
     __device__ __forceinline__ R &row1(uint32_t o, int y) const { return *(R *)(base + o + (size_t)(y + OY) * sizeof(R)); }
 };
 
-template <class R, bool PER, bool NEXT, bool AL0, bool PRE, int SCH = adj::ADJ_AH>
+template <class R, bool PER, bool NEXT, int NX, bool PRE, int SCH = adj::ADJ_AH>
 __global__ void __launch_bounds__(256) stage_adj_kernel(ArenaRef ar, adj::StageAdjArgs<R> A) {
     const adj::Tile t = adj::tile_of<adj::RT_HALO>(wave_index(), A.nstrips, A.nseg, A.L, adj::ADJ_Y0, adj::adj_y1(A.ny));
     if (!t.live) return;
     GpuWave<R> w(ar, A.P, t.x_lane0, adj::RT_HALO);
-    adj::stage_adjoint<GpuWave<R>, PER, NEXT, AL0, PRE, SCH>(w, A, t.ya, t.yb);
+    adj::stage_adjoint<GpuWave<R>, PER, NEXT, NX, PRE, SCH>(w, A, t.ya, t.yb);
 }
 
 template <class R, bool PER, int DRAG, bool SMAG>

@@ -804,7 +804,7 @@ template <class Fm> struct Adj {
         st->F0 = F;
         // rows per wave: about four resident waves per wave slot, at least 6 (a sweep of 4 .. 64 at
         // 1024² and 4096² finds the result within 1 % of the best at 4096² and 6 the best at 1024²)
-        st->La = LF::rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_adj_kernel<R, false, true, false, false>),
+        st->La = LF::rows_per_wave(ctx, reinterpret_cast<const void *>(&stage_adj_kernel<R, false, true, 0, false>),
                                    adj::nstrips_for<adj::RT_HALO>(ctx->c.nx + 2), 4, 6);
         if (const char *e = getenv("SWM_ROWS_PER_WAVE_ADJ")) {
             const int l = atoi(e);
@@ -865,24 +865,24 @@ template <class Fm> struct Adj {
             hipError_t e = hipGetLastError();
             return e == hipSuccess ? SWM_OK : ctx->fail(SWM_E_HIP, "adjoint kernel launch: %s", hipGetErrorString(e));
         }
-        template <bool PER, bool NEXT, bool AL0, bool PRE, int SCH> void launch_stage(const adj::StageAdjArgs<R> &S) {
-            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, AL0, PRE, SCH>), dim3(nblocks_waves(S.nstrips * S.nseg)),
+        template <bool PER, bool NEXT, int NX, bool PRE, int SCH> void launch_stage(const adj::StageAdjArgs<R> &S) {
+            hipLaunchKernelGGL((stage_adj_kernel<R, PER, NEXT, NX, PRE, SCH>), dim3(nblocks_waves(S.nstrips * S.nseg)),
                                dim3(WAVE * WAVES_PER_BLOCK), 0, ctx->stream, ctx->aref(), S);
         }
-        template <bool PER, int SCH> void stage_p(const adj::StageAdjArgs<R> &S, int next, int al0, int pre) {
-            if (!next) launch_stage<PER, false, false, false, SCH>(S);
-            else if (al0 && pre) launch_stage<PER, true, true, true, SCH>(S);
-            else if (al0) launch_stage<PER, true, true, false, SCH>(S);
-            else if (pre) launch_stage<PER, true, false, true, SCH>(S);
-            else launch_stage<PER, true, false, false, SCH>(S);
+        template <bool PER, int SCH> void stage_p(const adj::StageAdjArgs<R> &S, int next, int nx, int pre) {
+            if (!next) launch_stage<PER, false, 0, false, SCH>(S);
+            else if (!pre) launch_stage<PER, true, 0, false, SCH>(S);
+            else if (nx == 0) launch_stage<PER, true, 0, true, SCH>(S);
+            else if (nx == 1) launch_stage<PER, true, 1, true, SCH>(S);
+            else launch_stage<PER, true, 2, true, SCH>(S);
         }
-        template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
-            if (sch == adj::ADJ_CONT) launch_stage<PER, false, false, false, adj::ADJ_CONT>(S);   // (no successor)
-            else if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, al0, pre);
-            else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, al0, pre);
-            else if (sch == (adj::ADJ_AH | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_AH | adj::ADJ_KEEP>(S, next, al0, pre);
-            else if (sch == (adj::ADJ_SAD | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_SAD | adj::ADJ_KEEP>(S, next, al0, pre);
-            else stage_p<PER, adj::ADJ_AH>(S, next, al0, pre);
+        template <bool PER> void stage_s(const adj::StageAdjArgs<R> &S, int next, int nx, int pre, int sch) {
+            if (sch == adj::ADJ_CONT) launch_stage<PER, false, 0, false, adj::ADJ_CONT>(S);   // (no successor)
+            else if (sch == adj::ADJ_LIN) stage_p<PER, adj::ADJ_LIN>(S, next, nx, pre);
+            else if (sch == adj::ADJ_SAD) stage_p<PER, adj::ADJ_SAD>(S, next, nx, pre);
+            else if (sch == (adj::ADJ_AH | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_AH | adj::ADJ_KEEP>(S, next, nx, pre);
+            else if (sch == (adj::ADJ_SAD | adj::ADJ_KEEP)) stage_p<PER, adj::ADJ_SAD | adj::ADJ_KEEP>(S, next, nx, pre);
+            else stage_p<PER, adj::ADJ_AH>(S, next, nx, pre);
         }
         int advcor_adj(const adj::AdvcorAdjArgs &X) {
             hipLaunchKernelGGL((advcor_adj_kernel<R>), dim3((X.nx + 5 + 127) / 128, X.ny + 4), dim3(128), 0, ctx->stream,
@@ -905,12 +905,12 @@ template <class Fm> struct Adj {
                                (uint32_t *)((char *)ctx->arena + o), ctx->elems * ctx->es / 4);
             return fail_launch();
         }
-        int stage(adj::StageAdjArgs<R> S, int per, int next, int al0, int pre, int sch) {
+        int stage(adj::StageAdjArgs<R> S, int per, int next, int nx, int pre, int sch) {
             S.L = st->La;
             S.nstrips = adj::nstrips_for<adj::RT_HALO>(S.nx + 2);
             S.nseg = (adj::adj_y1(S.ny) - adj::ADJ_Y0 + S.L - 1) / S.L;
-            if (per) stage_s<true>(S, next, al0, pre, sch);
-            else stage_s<false>(S, next, al0, pre, sch);
+            if (per) stage_s<true>(S, next, nx, pre, sch);
+            else stage_s<false>(S, next, nx, pre, sch);
             return fail_launch();
         }
         int qend(const adj::StageAdjArgs<R> &S) {
@@ -1337,7 +1337,7 @@ template <class Fm> struct Adj {
         adj::StageAdjArgs<R> S{};
         S.s = Ft.s[1]; S.H = Ft.H; S.fq = Ft.fq; S.fq_2d = Ft.fq_2d; S.qend = Ft.qend; S.lqend = Ft.lqend;
         S.fu = Ft.fu; S.fv = Ft.fv; S.fu_2d = Ft.fu_2d; S.fv_2d = Ft.fv_2d;
-        S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.a_in = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
+        S.l0 = Ft.Z; S.mu_in = Ft.Ma; S.mu_x[0] = S.mu_x[1] = Ft.A; S.mu_out = Ft.Mb; S.a_out = Ft.A;
         S.a = k.a[1]; S.b = k.b[1]; S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = k.cont;
         S.nx = k.nx; S.ny = k.ny; S.P = k.P;
         adj::DissAdjArgs<R> D{};

@@ -204,29 +204,29 @@ template <class R> struct EmuK {
             fn(W, t);
         }
     }
-    template <bool PER, bool NEXT, bool AL0, bool PRE, int SCH> void stage_t(const StageAdjArgs<R> &S) {
+    template <bool PER, bool NEXT, int NX, bool PRE, int SCH> void stage_t(const StageAdjArgs<R> &S) {
         march<RT_HALO>(S.nx, S.ny, ADJ_Y0, adj_y1(S.ny), [&](EmuWave<R> &W, const Tile &t) {
-            stage_adjoint<EmuWave<R>, PER, NEXT, AL0, PRE, SCH>(W, S, t.ya, t.yb);
+            stage_adjoint<EmuWave<R>, PER, NEXT, NX, PRE, SCH>(W, S, t.ya, t.yb);
         });
     }
-    template <bool PER, int SCH> void stage_p(const StageAdjArgs<R> &S, int next, int al0, int pre) {
-        if (!next) stage_t<PER, false, false, false, SCH>(S);
-        else if (al0 && pre) stage_t<PER, true, true, true, SCH>(S);
-        else if (al0) stage_t<PER, true, true, false, SCH>(S);
-        else if (pre) stage_t<PER, true, false, true, SCH>(S);
-        else stage_t<PER, true, false, false, SCH>(S);
+    template <bool PER, int SCH> void stage_p(const StageAdjArgs<R> &S, int next, int nx, int pre) {
+        if (!next) stage_t<PER, false, 0, false, SCH>(S);
+        else if (!pre) stage_t<PER, true, 0, false, SCH>(S);
+        else if (nx == 0) stage_t<PER, true, 0, true, SCH>(S);
+        else if (nx == 1) stage_t<PER, true, 1, true, SCH>(S);
+        else stage_t<PER, true, 2, true, SCH>(S);
     }
-    template <bool PER> void stage_s(const StageAdjArgs<R> &S, int next, int al0, int pre, int sch) {
-        if (sch == ADJ_CONT) stage_t<PER, false, false, false, ADJ_CONT>(S);
-        else if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, al0, pre);
-        else if (sch == ADJ_SAD) stage_p<PER, ADJ_SAD>(S, next, al0, pre);
-        else if (sch == (ADJ_AH | ADJ_KEEP)) stage_p<PER, ADJ_AH | ADJ_KEEP>(S, next, al0, pre);
-        else if (sch == (ADJ_SAD | ADJ_KEEP)) stage_p<PER, ADJ_SAD | ADJ_KEEP>(S, next, al0, pre);
-        else stage_p<PER, ADJ_AH>(S, next, al0, pre);
+    template <bool PER> void stage_s(const StageAdjArgs<R> &S, int next, int nx, int pre, int sch) {
+        if (sch == ADJ_CONT) stage_t<PER, false, 0, false, ADJ_CONT>(S);
+        else if (sch == ADJ_LIN) stage_p<PER, ADJ_LIN>(S, next, nx, pre);
+        else if (sch == ADJ_SAD) stage_p<PER, ADJ_SAD>(S, next, nx, pre);
+        else if (sch == (ADJ_AH | ADJ_KEEP)) stage_p<PER, ADJ_AH | ADJ_KEEP>(S, next, nx, pre);
+        else if (sch == (ADJ_SAD | ADJ_KEEP)) stage_p<PER, ADJ_SAD | ADJ_KEEP>(S, next, nx, pre);
+        else stage_p<PER, ADJ_AH>(S, next, nx, pre);
     }
-    int stage(const StageAdjArgs<R> &S, int per, int next, int al0, int pre, int sch) {
-        if (per) stage_s<true>(S, next, al0, pre, sch);
-        else stage_s<false>(S, next, al0, pre, sch);
+    int stage(const StageAdjArgs<R> &S, int per, int next, int nx, int pre, int sch) {
+        if (per) stage_s<true>(S, next, nx, pre, sch);
+        else stage_s<false>(S, next, nx, pre, sch);
         return 0;
     }
     int qend(const StageAdjArgs<R> &S) {
@@ -425,7 +425,7 @@ template <class R> int run_op(const EmuCfg &c, int op, int64_t i, const double *
             StageAdjArgs<R> S{};
             S.s = F.s[0]; S.H = F.H; S.fq = F.fq; S.fq_2d = 1; S.qend = F.qend; S.lqend = F.lqend;
             S.fu = F.fu; S.fv = F.fv; S.fu_2d = S.fv_2d = 1;
-            S.l0 = F.L; S.mu_in = F.Ma; S.a_in = F.A; S.mu_out = F.Mb; S.a_out = F.A;
+            S.l0 = F.L; S.mu_in = F.Ma; S.mu_x[0] = S.mu_x[1] = F.A; S.mu_out = F.Mb; S.a_out = F.A;
             S.a = R(1); S.b = R(0);
             S.g = k.g; S.gamma = k.gamma; S.k12 = k.k12; S.oma = k.oma; S.cont = c.cont;
             S.nx = c.nx; S.ny = c.ny; S.P = A.P;

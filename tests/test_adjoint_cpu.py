@@ -243,6 +243,7 @@ def reverse_steps(swm, orc, S, cfg, prog, n, lam, k):
 
 
 @pytest.mark.parametrize("bc,kw", [("nonperiodic", {}), ("periodic", {}), ("nonperiodic", {"RKo": 3}),
+                                   ("periodic", {"RKo": 2, "cfl": 0.5, "Uadv": 40.0}),
                                    ("periodic", {"nstep_diff": 2, "diffusion": "constant", "bottom_drag": "linear"}),
                                    ("periodic", {"adv_scheme": "Sadourny"}),
                                    ("nonperiodic", {"adv_scheme": "Sadourny", "RKo": 3}),
