@@ -133,11 +133,104 @@ template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_k
     }
 }
 
-// second pass (a), one thread per corner X = -2 .. nx+3, Y = -1 .. ny+3
-template <class R, bool PER> __global__ void tracer_gather_uv_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
-    const int X = -2 + (int)(blockIdx.x * blockDim.x + threadIdx.x), Y = -1 + (int)blockIdx.y;
-    if (X > A.nx + 3) return;
-    adj::tracer_adj_gather_uv<R, PER>(GpuCells<R>{(char *)ar.base, A.P}, A, X, Y, dmax[0], dmax[1]);
+// second pass (a) on tiles of GT_X x GT_Y corners, 256 threads, four rows per thread: the
+// records of the sources within reach of the tile are staged in LDS, one record kind at a
+// time, and each corner scans its window there in tracer_gather's order (rows y = Y+ry down
+// to Y-ry, columns x = X+rx down to X-rx), so each sum is the same to the bit.  A source in
+// a row outside 1..ny, or (not periodic) a column outside 1..nx, is staged as a corner that
+// no (X, Y) matches; periodic columns wrap into 1..nx as there.
+// The window shrinks to the tile's own reach: a source at staged column xs, row y whose
+// first corner is (cx, cy) can only match the corners (X, Y) = (xs + dx, y + dy) with
+// dx in {cx - xs, cx - xs + 1} and dy in {cy - y, cy - y + 1}, so the steps (dx, dy) outside
+// the span of those offsets over the staged sources add nothing, and the steps kept are
+// summed in the same order.  A corner matches when K - p is 0, 1, 0x10000 or 0x10001 for the
+// packed K of (X, Y) and p of the source's first corner (fields of 16 bits, differences
+// below 2^15: grids up to 16000 cells a side).  Reaches of more than GT_RM - 1 cells, and
+// larger grids, fall back to the scan in global memory.
+constexpr int GT_X = 64, GT_Y = 16, GT_RM = 6, GT_W = GT_X + 2 * GT_RM, GT_H = GT_Y + 2 * GT_RM;
+template <class R> struct GatherTile {
+    R lam[GT_H][GT_W], x0[GT_H][GT_W], y0[GT_H][GT_W];
+    int cor[GT_H][GT_W];
+    int span[4][4];                                // per wave: min / max of the x and y offsets
+};
+constexpr int GT_NONE = (int)0x80000000;          // K - GT_NONE has bit 31 set: never a match
+
+template <class R, bool PER>
+__global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
+    static_assert(GT_X == WAVE && GT_Y % 4 == 0, "one wave per tile row, four waves");
+    const int rx = dmax[0], ry = dmax[1];
+    const GpuCells<R> c{(char *)ar.base, A.P};
+    const int X0 = -2 + (int)blockIdx.x * GT_X, Y0 = -1 + (int)blockIdx.y * GT_Y;
+    const int lx = (int)threadIdx.x % WAVE, wv = (int)threadIdx.x / WAVE;
+    const int X = X0 + lx, nx = A.nx, ny = A.ny;
+    if (rx + 1 > GT_RM || ry + 1 > GT_RM || nx > 16000 || ny > 16000) {     // uniform over the launch
+        if (X <= nx + 3)
+            for (int k = 0; k < GT_Y / 4; ++k) {
+                const int Y = Y0 + wv + 4 * k;
+                if (Y <= ny + 3) adj::tracer_adj_gather_uv<R, PER>(c, A, X, Y, rx, ry);
+            }
+        return;
+    }
+    __shared__ GatherTile<R> t;
+    const R one = R(1), zero = R(0);
+    for (int kind = 0; kind < 3; ++kind) {
+        const uint32_t *rec = kind == 0 ? A.rs : kind == 1 ? A.ru : A.rv;
+        const int Rx = rx + (kind ? 1 : 0), Ry = ry + (kind ? 1 : 0);
+        const int wa = GT_X + 2 * Rx, ha = GT_Y + 2 * Ry;
+        if (kind) __syncthreads();                           // the scans of the previous kind are done
+        int lox = 1 << 20, hix = -(1 << 20), loy = 1 << 20, hiy = -(1 << 20);
+        for (int i = wv; i < ha; i += 4) {
+            const int y = Y0 - Ry + i;
+            for (int j = lx; j < wa; j += WAVE) {
+                const int xs = X0 - Rx + j;
+                int x = xs;
+                bool ok = y >= 1 && y <= ny;
+                if (PER) x = ((x - 1) % nx + nx) % nx + 1;
+                else ok = ok && x >= 1 && x <= nx;
+                int p = GT_NONE;
+                if (ok) {
+                    p = c.ati(rec[3], x, y);
+                    t.lam[i][j] = c.at(rec[0], x, y);
+                    t.x0[i][j] = c.at(rec[1], x, y);
+                    t.y0[i][j] = c.at(rec[2], x, y);
+                    const int ox = adj::corner_x(p) - xs, oy = adj::corner_y(p) - y;
+                    lox = min(lox, ox); hix = max(hix, ox);
+                    loy = min(loy, oy); hiy = max(hiy, oy);
+                }
+                t.cor[i][j] = p;
+            }
+        }
+        for (int o = WAVE / 2; o > 0; o >>= 1) {             // every lane takes part
+            lox = min(lox, __shfl_xor(lox, o)); hix = max(hix, __shfl_xor(hix, o));
+            loy = min(loy, __shfl_xor(loy, o)); hiy = max(hiy, __shfl_xor(hiy, o));
+        }
+        if (lx == 0) { t.span[wv][0] = lox; t.span[wv][1] = hix; t.span[wv][2] = loy; t.span[wv][3] = hiy; }
+        __syncthreads();
+        for (int w = 0; w < 4; ++w) {
+            lox = min(lox, t.span[w][0]); hix = max(hix, t.span[w][1]);
+            loy = min(loy, t.span[w][2]); hiy = max(hiy, t.span[w][3]);
+        }
+        const int dx0 = max(-Rx, lox), dx1 = min(Rx, hix + 1), dy0 = max(-Ry, loy), dy1 = min(Ry, hiy + 1);
+        for (int k = 0; k < GT_Y / 4; ++k) {
+            const int ly = wv + 4 * k, Y = Y0 + ly;
+            if (X > nx + 3 || Y > ny + 3) continue;
+            if (kind == 0 && !(X >= 0 && X <= nx + 1 && Y >= 1 && Y <= ny)) continue;
+            const uint32_t K = (uint32_t)adj::pack_corner(X, Y);
+            R acc = zero;
+            for (int dy = dy0; dy <= dy1; ++dy) {
+                const int i = ly - dy + Ry;
+                for (int dx = dx0; dx <= dx1; ++dx) {
+                    const int j = lx - dx + Rx;
+                    const uint32_t d = K - (uint32_t)t.cor[i][j];    // (X - cx) + 0x10000 (Y - cy) mod 2^32
+                    const R x0 = t.x0[i][j], y0 = t.y0[i][j];
+                    const R wx = (d & 1) ? x0 : one - x0, wy = (d & 0x10000) ? y0 : one - y0;
+                    const R term = t.lam[i][j] * (wx * wy);
+                    acc = acc + ((d & 0xFFFEFFFEu) == 0u ? term : zero);   // acc is never -0: + 0 leaves it
+                }
+            }
+            c.at(kind == 0 ? A.lsst_out : kind == 1 ? A.gu : A.gv, X, Y) = acc;
+        }
+    }
 }
 
 // second pass (b), one thread per cell X = -2 .. nx+2, Y = -1 .. ny+2

@@ -943,14 +943,16 @@ template <class Fm> struct Adj {
             hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
             // the first pass: 256 columns per block, rows per thread so that about 2048 blocks run
             const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 2047) / 2048);
-            const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + 127) / 128, T.ny + 5), g2((T.nx + 5 + 127) / 128, T.ny + 4);
+            // the gather of (a): corners X = -2 .. nx+3, Y = -1 .. ny+3 on tiles of GT_X x GT_Y
+            const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + GT_X - 1) / GT_X, (T.ny + 5 + GT_Y - 1) / GT_Y),
+                g2((T.nx + 5 + 127) / 128, T.ny + 4);
             if (T.ep) {
                 hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_uv_kernel<R, true>), ga, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, true>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax);
                 hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             } else {
                 hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_uv_kernel<R, false>), ga, dim3(128), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, false>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax);
                 hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             }
             return fail_launch();
