@@ -145,19 +145,25 @@ template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_k
 // the span of those offsets over the staged sources add nothing, and the steps kept are
 // summed in the same order.  A corner matches when K - p is 0, 1, 0x10000 or 0x10001 for the
 // packed K of (X, Y) and p of the source's first corner (fields of 16 bits, differences
-// below 2^15: grids up to 16000 cells a side).  Reaches of more than GT_RM - 1 cells, and
-// larger grids, fall back to the scan in global memory.
-constexpr int GT_X = 64, GT_Y = 16, GT_RM = 6, GT_W = GT_X + 2 * GT_RM, GT_H = GT_Y + 2 * GT_RM;
+// below 2^15: grids up to 16000 cells a side); bits 0 and 16 of K - p then pick which of the
+// source's four terms λ·(wx·wy), formed once when the source is staged, the corner adds.
+// Reaches of more than GT_RM - 1 cells, and larger grids, fall back to the scan in global
+// memory.  Tiles are GT_X x gt_y<R> corners (64 KiB of LDS at most).
+constexpr int GT_X = 64, GT_RM = 5, GT_W = GT_X + 2 * GT_RM;
+template <class R> constexpr int gt_y = sizeof(R) == 8 ? 8 : 16;
 template <class R> struct GatherTile {
-    R lam[GT_H][GT_W], x0[GT_H][GT_W], y0[GT_H][GT_W];
-    int cor[GT_H][GT_W];
+    static constexpr int H = gt_y<R> + 2 * GT_RM;
+    R w[4][H][GT_W];                               // the terms of corners (cx, cy), (cx+1, cy), (cx, cy+1), (cx+1, cy+1)
+    int cor[H][GT_W];
     int span[4][4];                                // per wave: min / max of the x and y offsets
 };
 constexpr int GT_NONE = (int)0x80000000;          // K - GT_NONE has bit 31 set: never a match
 
 template <class R, bool PER>
 __global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
+    constexpr int GT_Y = gt_y<R>;
     static_assert(GT_X == WAVE && GT_Y % 4 == 0, "one wave per tile row, four waves");
+    static_assert(sizeof(GatherTile<R>) <= 65536, "LDS of one workgroup");
     const int rx = dmax[0], ry = dmax[1];
     const GpuCells<R> c{(char *)ar.base, A.P};
     const int X0 = -2 + (int)blockIdx.x * GT_X, Y0 = -1 + (int)blockIdx.y * GT_Y;
@@ -173,33 +179,59 @@ __global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar
     }
     __shared__ GatherTile<R> t;
     const R one = R(1), zero = R(0);
-    for (int kind = 0; kind < 3; ++kind) {
+    // the staged block of a kind: rows wv, wv+4, ... and columns lx, lx + 64 of each thread,
+    // every load issued before the first is used (loads of the next kind issued before the
+    // scan of the current one were slower at 4096^2)
+    constexpr int NI = (GatherTile<R>::H + 3) / 4;
+    int pv[NI][2];
+    R lv[NI][2], xv[NI][2], yv[NI][2];
+    auto load = [&](int kind) {
         const uint32_t *rec = kind == 0 ? A.rs : kind == 1 ? A.ru : A.rv;
         const int Rx = rx + (kind ? 1 : 0), Ry = ry + (kind ? 1 : 0);
         const int wa = GT_X + 2 * Rx, ha = GT_Y + 2 * Ry;
-        if (kind) __syncthreads();                           // the scans of the previous kind are done
-        int lox = 1 << 20, hix = -(1 << 20), loy = 1 << 20, hiy = -(1 << 20);
-        for (int i = wv; i < ha; i += 4) {
-            const int y = Y0 - Ry + i;
-            for (int j = lx; j < wa; j += WAVE) {
-                const int xs = X0 - Rx + j;
-                int x = xs;
-                bool ok = y >= 1 && y <= ny;
+#pragma unroll
+        for (int a = 0; a < NI; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int i = wv + 4 * a, j = lx + WAVE * b, y = Y0 - Ry + i;
+                int x = X0 - Rx + j;
+                bool ok = i < ha && j < wa && y >= 1 && y <= ny;
                 if (PER) x = ((x - 1) % nx + nx) % nx + 1;
                 else ok = ok && x >= 1 && x <= nx;
-                int p = GT_NONE;
+                pv[a][b] = GT_NONE;
+                lv[a][b] = xv[a][b] = yv[a][b] = zero;
                 if (ok) {
-                    p = c.ati(rec[3], x, y);
-                    t.lam[i][j] = c.at(rec[0], x, y);
-                    t.x0[i][j] = c.at(rec[1], x, y);
-                    t.y0[i][j] = c.at(rec[2], x, y);
-                    const int ox = adj::corner_x(p) - xs, oy = adj::corner_y(p) - y;
-                    lox = min(lox, ox); hix = max(hix, ox);
-                    loy = min(loy, oy); hiy = max(hiy, oy);
+                    pv[a][b] = c.ati(rec[3], x, y);
+                    lv[a][b] = c.at(rec[0], x, y);
+                    xv[a][b] = c.at(rec[1], x, y);
+                    yv[a][b] = c.at(rec[2], x, y);
                 }
-                t.cor[i][j] = p;
             }
-        }
+    };
+    for (int kind = 0; kind < 3; ++kind) {
+        const int Rx = rx + (kind ? 1 : 0), Ry = ry + (kind ? 1 : 0);
+        const int wa = GT_X + 2 * Rx, ha = GT_Y + 2 * Ry;
+        if (kind) __syncthreads();                           // the scans of the previous kind are done
+        load(kind);
+        int lox = 1 << 20, hix = -(1 << 20), loy = 1 << 20, hiy = -(1 << 20);
+#pragma unroll
+        for (int a = 0; a < NI; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int i = wv + 4 * a, j = lx + WAVE * b;
+                if (i >= ha || j >= wa) continue;
+                const int p = pv[a][b];                      // a record's corner packs to a value >= 0
+                t.cor[i][j] = p;
+                if (p == GT_NONE) continue;
+                const R lam = lv[a][b], x0 = xv[a][b], y0 = yv[a][b];
+                t.w[0][i][j] = lam * ((one - x0) * (one - y0));
+                t.w[1][i][j] = lam * (x0 * (one - y0));
+                t.w[2][i][j] = lam * ((one - x0) * y0);
+                t.w[3][i][j] = lam * (x0 * y0);
+                const int ox = adj::corner_x(p) - (X0 - Rx + j), oy = adj::corner_y(p) - (Y0 - Ry + i);
+                lox = min(lox, ox); hix = max(hix, ox);
+                loy = min(loy, oy); hiy = max(hiy, oy);
+            }
         for (int o = WAVE / 2; o > 0; o >>= 1) {             // every lane takes part
             lox = min(lox, __shfl_xor(lox, o)); hix = max(hix, __shfl_xor(hix, o));
             loy = min(loy, __shfl_xor(loy, o)); hiy = max(hiy, __shfl_xor(hiy, o));
@@ -222,9 +254,7 @@ __global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar
                 for (int dx = dx0; dx <= dx1; ++dx) {
                     const int j = lx - dx + Rx;
                     const uint32_t d = K - (uint32_t)t.cor[i][j];    // (X - cx) + 0x10000 (Y - cy) mod 2^32
-                    const R x0 = t.x0[i][j], y0 = t.y0[i][j];
-                    const R wx = (d & 1) ? x0 : one - x0, wy = (d & 0x10000) ? y0 : one - y0;
-                    const R term = t.lam[i][j] * (wx * wy);
+                    const R term = t.w[(d & 1u) | ((d >> 15) & 2u)][i][j];
                     acc = acc + ((d & 0xFFFEFFFEu) == 0u ? term : zero);   // acc is never -0: + 0 leaves it
                 }
             }

@@ -943,8 +943,8 @@ template <class Fm> struct Adj {
             hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
             // the first pass: 256 columns per block, rows per thread so that about 2048 blocks run
             const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 2047) / 2048);
-            // the gather of (a): corners X = -2 .. nx+3, Y = -1 .. ny+3 on tiles of GT_X x GT_Y
-            const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + GT_X - 1) / GT_X, (T.ny + 5 + GT_Y - 1) / GT_Y),
+            // the gather of (a): corners X = -2 .. nx+3, Y = -1 .. ny+3 on tiles of GT_X x gt_y<R>
+            const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + GT_X - 1) / GT_X, (T.ny + 5 + gt_y<R> - 1) / gt_y<R>),
                 g2((T.nx + 5 + 127) / 128, T.ny + 4);
             if (T.ep) {
                 hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
