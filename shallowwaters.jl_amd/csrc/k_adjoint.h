@@ -290,6 +290,20 @@ __global__ void lincomb_kernel(R *d, const R *x, R ax, const R *y, R ay, const R
         adj::lincomb_elem<R>(d, x, ax, y, ay, z, az, (long)k);
 }
 
+// the fields of a state snapshot copied in one launch: field f is n16 16-byte words from
+// src[f] to dst[f] (16-byte aligned; the host checks)
+struct CopySet {
+    const uint4 *src[8];
+    uint4 *dst[8];
+    int n;
+    size_t n16;
+};
+static __global__ void __launch_bounds__(256) copy_fields_kernel(CopySet S) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int f = 0; f < S.n; ++f)
+        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < S.n16; k += stride) S.dst[f][k] = S.src[f][k];
+}
+
 // zero n elements of size 4 (a fill kernel: usable where a memset node is not)
 static __global__ void zero_words_kernel(uint32_t *p, size_t n) {
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0u;

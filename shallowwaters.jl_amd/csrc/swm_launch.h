@@ -1008,10 +1008,25 @@ template <class Fm> struct Adj {
         const Field *f[8] = {&ctx->u[ctx->pu], &ctx->v[ctx->pu], &ctx->e[ctx->pe], &ctx->sst[ctx->rsst], &ctx->um, &ctx->vm,
                              &ctx->qs, &ctx->ks};
         const int nf = snap_fields(ctx);
+        // one launch for every field when all are 16-byte aligned, else one copy each
+        CopySet S{};
+        S.n = nf;
+        S.n16 = fb / 16;
+        bool vec = fb % 16 == 0;
         for (int j = 0; j < nf; ++j) {
             char *s = st->snap[k] + j * fb;
-            HIPCHK(hipMemcpyAsync(save ? s : f[j]->base, save ? f[j]->base : s, fb, hipMemcpyDeviceToDevice, ctx->stream));
+            char *src = save ? (char *)f[j]->base : s, *dst = save ? s : (char *)f[j]->base;
+            vec = vec && ((uintptr_t)src | (uintptr_t)dst) % 16 == 0;
+            S.src[j] = (const uint4 *)src;
+            S.dst[j] = (uint4 *)dst;
         }
+        if (vec) {
+            const int blocks = (int)std::min<size_t>(2048, (S.n16 + 255) / 256);
+            hipLaunchKernelGGL(copy_fields_kernel, dim3(std::max(1, blocks)), dim3(256), 0, ctx->stream, S);
+            HIPCHK(hipGetLastError());
+        } else
+            for (int j = 0; j < nf; ++j)
+                HIPCHK(hipMemcpyAsync(S.dst[j], S.src[j], fb, hipMemcpyDeviceToDevice, ctx->stream));
         if (save) st->snap_pending[k] = ctx->advcor_pending ? 1 : 0;
         else ctx->advcor_pending = st->snap_pending[k] != 0;
         return SWM_OK;
