@@ -148,7 +148,7 @@ template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_k
 // below 2^15: grids up to 16000 cells a side); bits 0 and 16 of K - p then pick which of the
 // source's four terms λ·(wx·wy), formed once when the source is staged, the corner adds.
 // Reaches of more than GT_RM - 1 cells, and larger grids, fall back to the scan in global
-// memory.  Tiles are GT_X x gt_y<R> corners (64 KiB of LDS at most).
+// memory, as does tiled = 0.  Tiles are GT_X x gt_y<R> corners (64 KiB of LDS at most).
 constexpr int GT_X = 64, GT_RM = 5, GT_W = GT_X + 2 * GT_RM;
 template <class R> constexpr int gt_y = sizeof(R) == 8 ? 8 : 16;
 template <class R> struct GatherTile {
@@ -160,7 +160,8 @@ template <class R> struct GatherTile {
 constexpr int GT_NONE = (int)0x80000000;          // K - GT_NONE has bit 31 set: never a match
 
 template <class R, bool PER>
-__global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax) {
+__global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A, const int *dmax,
+                                                                     int tiled) {
     constexpr int GT_Y = gt_y<R>;
     static_assert(GT_X == WAVE && GT_Y % 4 == 0, "one wave per tile row, four waves");
     static_assert(sizeof(GatherTile<R>) <= 65536, "LDS of one workgroup");
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(256) tracer_gather_uv_tiled_kernel(ArenaRef ar
     const int X0 = -2 + (int)blockIdx.x * GT_X, Y0 = -1 + (int)blockIdx.y * GT_Y;
     const int lx = (int)threadIdx.x % WAVE, wv = (int)threadIdx.x / WAVE;
     const int X = X0 + lx, nx = A.nx, ny = A.ny;
-    if (rx + 1 > GT_RM || ry + 1 > GT_RM || nx > 16000 || ny > 16000) {     // uniform over the launch
+    if (!tiled || rx + 1 > GT_RM || ry + 1 > GT_RM || nx > 16000 || ny > 16000) {     // uniform over the launch
         if (X <= nx + 3)
             for (int k = 0; k < GT_Y / 4; ++k) {
                 const int Y = Y0 + wv + 4 * k;

@@ -944,15 +944,19 @@ template <class Fm> struct Adj {
             // the first pass: 256 columns per block, rows per thread so that about 2048 blocks run
             const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 2047) / 2048);
             // the gather of (a): corners X = -2 .. nx+3, Y = -1 .. ny+3 on tiles of GT_X x gt_y<R>
+            // (SWM_TRACER_GATHER_GLOBAL=1: every corner scans in global memory, as a check)
+            const int tiled = getenv("SWM_TRACER_GATHER_GLOBAL") ? 0 : 1;
             const dim3 g1(bx, (T.ny + rows - 1) / rows), ga((T.nx + 6 + GT_X - 1) / GT_X, (T.ny + 5 + gt_y<R> - 1) / gt_y<R>),
                 g2((T.nx + 5 + 127) / 128, T.ny + 4);
             if (T.ep) {
                 hipLaunchKernelGGL((tracer_src_kernel<R, true>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, true>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, true>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax,
+                                   tiled);
                 hipLaunchKernelGGL((tracer_gather_kernel<R, true>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             } else {
                 hipLaunchKernelGGL((tracer_src_kernel<R, false>), g1, dim3(256), 0, ctx->stream, ctx->aref(), T, rows, st->dmax);
-                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, false>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax);
+                hipLaunchKernelGGL((tracer_gather_uv_tiled_kernel<R, false>), ga, dim3(256), 0, ctx->stream, ctx->aref(), T, st->dmax,
+                                   tiled);
                 hipLaunchKernelGGL((tracer_gather_kernel<R, false>), g2, dim3(128), 0, ctx->stream, ctx->aref(), T);
             }
             return fail_launch();

@@ -70,6 +70,28 @@ def test_pullback_bitwise_equals_host_kernels(swm, orc, gpu, monkeypatch, bc, T,
         assert len(bad) == 0, f"{name}: {len(bad)} cells differ, first {bad[0].tolist()}: {r[tuple(bad[0])]!r} vs {a[tuple(bad[0])]!r}"
 
 
+@pytest.mark.parametrize("bc", ["nonperiodic", "periodic"])
+@pytest.mark.parametrize("T", ["Float64", "Float32"])
+def test_tracer_gather_global_scan_bitwise(swm, orc, gpu, monkeypatch, bc, T):
+    """The tracer adjoint's gather with every corner scanning in global memory (the path taken
+    for departure distances beyond the LDS tiles' reach) equals the host build bitwise, as the
+    tiled gather does in the test above."""
+    monkeypatch.setenv("SWM_ROWS_PER_WAVE_ADJ", "8")
+    monkeypatch.setenv("SWM_TRACER_GATHER_GLOBAL", "1")
+    P, S, cfg, prog = setup(swm, bc, T=T)
+    assert (S.grid.nadvstep, S.grid.nadvstep_half) == (3, 1)
+    n = 3
+    dt = np.float64 if T == "Float64" else np.float32
+    ybar = cotangent(S, 2, dt)
+    print(f"global-scan gather {bc} {T}", flush=True)
+    g, _ = gpu_gradient(swm, S, prog, ybar, n, snapshots=1)
+    k = adjemu.config(S, cfg, L=8, fmt=8 if T == "Float64" else 4)
+    ref, _, _ = reverse_steps(swm, orc, S, cfg, prog, n, [a.astype(np.float64) for a in ybar], k)
+    for name, r, a in zip(("u", "v", "η", "sst"), ref, g):
+        a = np.asarray(a, dtype=np.float64)
+        assert np.array_equal(r, a), f"{name}: {np.count_nonzero(r != a)} cells differ"
+
+
 @pytest.mark.parametrize("case", ["nonperiodic-F32-RK4", "nonperiodic-F64-RK3", "periodic-F32-RK4", "periodic-F64-RK2",
                                   "nonperiodic-F32-RK4-relax"])
 def test_taped_fused_pairs_equal_taped_stages(swm, gpu, monkeypatch, case):
