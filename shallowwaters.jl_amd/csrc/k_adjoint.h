@@ -106,7 +106,7 @@ template <class R> __global__ void qend_adj_kernel(ArenaRef ar, adj::StageAdjArg
 // first tracer pass, one thread per T-column over `rows` rows; dmax[0] (dmax[1]) receives the
 // largest x (y) distance between a cell and the corners it interpolates from.  Atomics on
 // one address are serialised (about 10 ns each), so the block takes the maxima of its
-// cells first and issues one pair: about two thousand blocks whatever the grid.
+// cells first and issues at most one pair.
 template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_kernel(ArenaRef ar, adj::TracerAdjArgs<R> A,
                                                                                       int rows, int *dmax) {
     __shared__ int sm[2][256 / WAVE];
@@ -128,8 +128,9 @@ template <class R, bool PER> __global__ void __launch_bounds__(256) tracer_src_k
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 1; k < (int)blockDim.x / WAVE; ++k) { dx = max(dx, sm[0][k]); dy = max(dy, sm[1][k]); }
-        atomicMax(dmax, dx);
-        atomicMax(dmax + 1, dy);
+        // the maxima only grow: a block whose value does not exceed what it reads skips the atomic
+        if (dx > __atomic_load_n(dmax, __ATOMIC_RELAXED)) atomicMax(dmax, dx);
+        if (dy > __atomic_load_n(dmax + 1, __ATOMIC_RELAXED)) atomicMax(dmax + 1, dy);
     }
 }
 

@@ -941,8 +941,8 @@ template <class Fm> struct Adj {
         }
         int tracer(const adj::TracerAdjArgs<R> &T) {
             hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)st->dmax, (size_t)2);
-            // the first pass: 256 columns per block, rows per thread so that about 2048 blocks run
-            const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 2047) / 2048);
+            // the first pass: 256 columns per block, rows per thread so that about 8192 blocks run
+            const int bx = (T.nx + 255) / 256, rows = std::max(1, (bx * T.ny + 8191) / 8192);
             // the gather of (a): corners X = -2 .. nx+3, Y = -1 .. ny+3 on tiles of GT_X x gt_y<R>
             // (SWM_TRACER_GATHER_GLOBAL=1: every corner scans in global memory, as a check)
             const int tiled = getenv("SWM_TRACER_GATHER_GLOBAL") ? 0 : 1;
