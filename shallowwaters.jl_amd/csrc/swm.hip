@@ -662,7 +662,15 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
     E->m.assign(members, members + n);
     E->fmt = m0->fmt;
     E->per = c0.bc == SWM_PERIODIC;
-    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&E->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&E->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&E->join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (E->stream) (void)hipStreamDestroy(E->stream);
+        if (E->stream2) (void)hipStreamDestroy(E->stream2);
+        if (E->fork) (void)hipEventDestroy(E->fork);
+        if (E->join) (void)hipEventDestroy(E->join);
         delete E;
         return SWM_E_HIP;
     }
@@ -681,11 +689,15 @@ int swm_ens_create(int32_t n, swm_ctx *const *members, swm_ens **out) {
 int swm_ens_destroy(swm_ens *E) {
     if (!E) return SWM_OK;
     if (E->stream) (void)hipStreamSynchronize(E->stream);
+    if (E->stream2) (void)hipStreamSynchronize(E->stream2);
     if (E->exec) (void)hipGraphExecDestroy(E->exec);
     if (E->graph) (void)hipGraphDestroy(E->graph);
     for (auto &b : E->buf) if (b.dev) (void)hipFree(b.dev);
     if (E->pdev) (void)hipFree(E->pdev);
     if (E->stream) (void)hipStreamDestroy(E->stream);
+    if (E->stream2) (void)hipStreamDestroy(E->stream2);
+    if (E->fork) (void)hipEventDestroy(E->fork);
+    if (E->join) (void)hipEventDestroy(E->join);
     for (swm_ctx *c : E->m) c->ens_refs -= 1;
     delete E;
     return SWM_OK;

@@ -414,6 +414,8 @@ struct swm_ens {
     std::vector<swm_ctx *> m;
     std::string err;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;     // the seam strips of periodic pairs, beside the other strips
+    hipEvent_t fork = nullptr, join = nullptr;
     int fmt = SWM_F32;
     bool per = false;
     struct Plan {                      // one step of every member, packed for upload

@@ -1569,18 +1569,32 @@ template <class F> struct Ens {
             } else if (pl.kind[j] == 4 || pl.kind[j] == 5) {      // periodic pairs: strips clear of the seam / at it
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 const bool cs = c0.continuity != SWM_CONT_ITSELF;
-                if (pl.tiles[j] == 0) continue;
-#define SWM_FB2(MA_, MB_)                                                                                         \
-    if (pl.kind[j] == 4) {                                                                                        \
-        if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, true>), grid, block, 0, E->stream, a, tab, nm); \
-        else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, true>), grid, block, 0, E->stream, a, tab, nm);  \
-    } else if (cs) hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, true>), grid, block, 0, E->stream, a, tab, nm); \
-    else hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, false>), grid, block, 0, E->stream, a, tab, nm)
-                if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FB2(M_RK_FIRST, M_RK_MID); }
-                else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_MID, M_RK_LAST); }
-                else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_FIRST, M_RK_LAST); }
-                else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
+                // the seam strips (kind 5, recorded right after their pair's kind 4) read what the
+                // other strips read and write other cells: they run on the second stream beside
+                // them, forked before kind 4 and joined after kind 5 (graph capture keeps both
+                // branches)
+                hipStream_t st = E->stream;
+                if (pl.kind[j] == 4 && hipEventRecord(E->fork, E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "fork");
+                if (pl.kind[j] == 5) {
+                    if (hipStreamWaitEvent(E->stream2, E->fork, 0) != hipSuccess) return E->fail(SWM_E_HIP, "fork");
+                    st = E->stream2;
+                }
+                if (pl.tiles[j] > 0) {
+#define SWM_FB2(MA_, MB_)                                                                                   \
+    if (pl.kind[j] == 4) {                                                                                  \
+        if (cs) hipLaunchKernelGGL((fused_batch<F, MA_, MB_, true, true>), grid, block, 0, st, a, tab, nm); \
+        else hipLaunchKernelGGL((fused_batch<F, MA_, MB_, false, true>), grid, block, 0, st, a, tab, nm);  \
+    } else if (cs) hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, true>), grid, block, 0, st, a, tab, nm); \
+    else hipLaunchKernelGGL((fused_seam_batch<F, MA_, MB_, false>), grid, block, 0, st, a, tab, nm)
+                    if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_MID) { SWM_FB2(M_RK_FIRST, M_RK_MID); }
+                    else if (pl.ma[j] == M_RK_MID && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_MID, M_RK_LAST); }
+                    else if (pl.ma[j] == M_RK_FIRST && pl.mb[j] == M_RK_LAST) { SWM_FB2(M_RK_FIRST, M_RK_LAST); }
+                    else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
 #undef SWM_FB2
+                }
+                if (pl.kind[j] == 5 && (hipEventRecord(E->join, E->stream2) != hipSuccess ||
+                                        hipStreamWaitEvent(E->stream, E->join, 0) != hipSuccess))
+                    return E->fail(SWM_E_HIP, "join");
             } else if (pl.kind[j] == 2) {
                 const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
                 const bool smag = c0.diffusion == SWM_DIFF_SMAGORINSKY;
