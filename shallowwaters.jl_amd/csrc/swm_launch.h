@@ -1514,7 +1514,22 @@ template <class F> struct Ens {
         const swm_config &c0 = E->m[0]->c;
         char *dev = (char *)B.dev;
         const dim3 block(WAVE * WAVES_PER_BLOCK);
-        for (size_t j = 0; j < pl.kind.size(); ++j) {
+        // a periodic pair's seam strips (kind 5, recorded right after the pair's other strips,
+        // kind 4) are issued first, on the second stream, so that their few long marches start
+        // before the other strips fill the device
+        const size_t nk = pl.kind.size();
+        std::vector<size_t> order;
+        order.reserve(nk);
+        for (size_t j = 0; j < nk; ++j) {
+            if (pl.kind[j] == 4 && j + 1 < nk && pl.kind[j + 1] == 5) {
+                order.push_back(j + 1);
+                order.push_back(j);
+                ++j;
+            } else
+                order.push_back(j);
+        }
+        for (size_t q = 0; q < nk; ++q) {
+            const size_t j = order[q];
             const int *tab = (const int *)(dev + pl.tab_off[j]), *qtab = (const int *)(dev + pl.qtab_off[j]);
             const dim3 grid(nblocks_waves(pl.tiles[j]));
             if (pl.kind[j] == 0) {
@@ -1569,14 +1584,14 @@ template <class F> struct Ens {
             } else if (pl.kind[j] == 4 || pl.kind[j] == 5) {      // periodic pairs: strips clear of the seam / at it
                 const StageArgs<F> *a = (const StageArgs<F> *)(dev + pl.args_off[j]);
                 const bool cs = c0.continuity != SWM_CONT_ITSELF;
-                // the seam strips (kind 5, recorded right after their pair's kind 4) read what the
-                // other strips read and write other cells: they run on the second stream beside
-                // them, forked before kind 4 and joined after kind 5 (graph capture keeps both
-                // branches)
+                // the seam strips read what the other strips read and write other cells: they run
+                // on the second stream beside them, forked before the seam strips and joined after
+                // the other strips (graph capture keeps both branches)
                 hipStream_t st = E->stream;
-                if (pl.kind[j] == 4 && hipEventRecord(E->fork, E->stream) != hipSuccess) return E->fail(SWM_E_HIP, "fork");
+                const bool joins = pl.kind[j] == 4 ? j + 1 < nk && pl.kind[j + 1] == 5 : !(j > 0 && pl.kind[j - 1] == 4);
                 if (pl.kind[j] == 5) {
-                    if (hipStreamWaitEvent(E->stream2, E->fork, 0) != hipSuccess) return E->fail(SWM_E_HIP, "fork");
+                    if (hipEventRecord(E->fork, E->stream) != hipSuccess || hipStreamWaitEvent(E->stream2, E->fork, 0) != hipSuccess)
+                        return E->fail(SWM_E_HIP, "fork");
                     st = E->stream2;
                 }
                 if (pl.tiles[j] > 0) {
@@ -1592,8 +1607,8 @@ template <class F> struct Ens {
                     else return E->fail(SWM_E_UNSUPPORTED, "fused stages %d+%d in an ensemble", pl.ma[j], pl.mb[j]);
 #undef SWM_FB2
                 }
-                if (pl.kind[j] == 5 && (hipEventRecord(E->join, E->stream2) != hipSuccess ||
-                                        hipStreamWaitEvent(E->stream, E->join, 0) != hipSuccess))
+                if (joins && (hipEventRecord(E->join, E->stream2) != hipSuccess ||
+                              hipStreamWaitEvent(E->stream, E->join, 0) != hipSuccess))
                     return E->fail(SWM_E_HIP, "join");
             } else if (pl.kind[j] == 2) {
                 const DissArgs<F> *a = (const DissArgs<F> *)(dev + pl.args_off[j]);
